@@ -1,0 +1,856 @@
+// femdr_oracle.cpp -- CPU FP64 restatement of FEMTools.jl's matrix-free DR
+// Stokes hot path. TEST INFRASTRUCTURE ONLY: it is the checker the HIP
+// library is compared against; the product path never links or calls it.
+//
+// Reference citations are file:line into the FEMTools.jl repository.
+// Element contributions are computed independently per element (OpenMP over
+// elements) and summed into nodes by a gather over node->element lists in
+// ascending element order, which reproduces a sequential scatter in element
+// order bit-for-bit regardless of the thread count.
+#include "femdr_oracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+// Julia eps(x): spacing from |x| to the next larger float.
+double julia_eps(double x) {
+    double a = std::fabs(x);
+    if (!std::isfinite(a)) return std::numeric_limits<double>::quiet_NaN();
+    return std::nextafter(a, kInf) - a;
+}
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers (stand-in for ForwardDiff.Dual used by
+// src/stokes/assemblers/momentum_residuals.jl:781-787).
+// ---------------------------------------------------------------------------
+template <int N>
+struct Dual {
+    double v;
+    double d[N];
+    Dual() : v(0) { for (int i = 0; i < N; ++i) d[i] = 0; }
+    Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0; }
+};
+template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a) { Dual<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r; r.v = a.v / b.v; double ib = 1.0 / (b.v * b.v);
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) * ib;
+    return r;
+}
+template <int N> Dual<N> operator+(const Dual<N>& a, double b) { Dual<N> r = a; r.v += b; return r; }
+template <int N> Dual<N> operator+(double a, const Dual<N>& b) { Dual<N> r = b; r.v += a; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a, double b) { Dual<N> r = a; r.v -= b; return r; }
+template <int N> Dual<N> operator-(double a, const Dual<N>& b) { Dual<N> r = -b; r.v += a; return r; }
+template <int N> Dual<N> operator*(const Dual<N>& a, double b) { Dual<N> r; r.v = a.v * b; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b; return r; }
+template <int N> Dual<N> operator*(double a, const Dual<N>& b) { return b * a; }
+template <int N> Dual<N> operator/(const Dual<N>& a, double b) { Dual<N> r; r.v = a.v / b; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / b; return r; }
+template <int N> Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+template <int N> Dual<N> dsqrt(const Dual<N>& a) { Dual<N> r; r.v = std::sqrt(a.v); double h = 0.5 / r.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h; return r; }
+inline double dsqrt(double a) { return std::sqrt(a); }
+template <int N> double val(const Dual<N>& a) { return a.v; }
+inline double val(double a) { return a; }
+
+// ---------------------------------------------------------------------------
+// Reference elements
+// ---------------------------------------------------------------------------
+// T7 = T6 + cubic bubble, src/elements/shape_functions.jl:115-171.
+void t7_basis(double xi, double eta, double N[7], double dN[7][2]) {
+    const double L1 = 1 - xi - eta;
+    const double b = 27.0 * (1 - xi - eta) * xi * eta;
+    const double dbx = 27.0 * eta * (1 - 2 * xi - eta);
+    const double dby = 27.0 * xi * (1 - xi - 2 * eta);
+    N[0] = L1 * (2 * L1 - 1) + b / 9;
+    N[1] = xi * (2 * xi - 1) + b / 9;
+    N[2] = eta * (2 * eta - 1) + b / 9;
+    N[3] = 4 * (1 - xi - eta) * xi - 4 * b / 9;
+    N[4] = 4 * xi * eta - 4 * b / 9;
+    N[5] = 4 * eta * (1 - xi - eta) - 4 * b / 9;
+    N[6] = b;
+    dN[0][0] = 1 - 4 * L1 + dbx / 9;            dN[0][1] = 1 - 4 * L1 + dby / 9;
+    dN[1][0] = 4 * xi - 1 + dbx / 9;            dN[1][1] = dby / 9;
+    dN[2][0] = dbx / 9;                         dN[2][1] = 4 * eta - 1 + dby / 9;
+    dN[3][0] = 4 * (1 - 2 * xi - eta) - 4 * dbx / 9; dN[3][1] = -4 * xi - 4 * dby / 9;
+    dN[4][0] = 4 * eta - 4 * dbx / 9;           dN[4][1] = 4 * xi - 4 * dby / 9;
+    dN[5][0] = -4 * eta - 4 * dbx / 9;          dN[5][1] = 4 * (1 - xi - 2 * eta) - 4 * dby / 9;
+    dN[6][0] = dbx;                             dN[6][1] = dby;
+}
+
+// 7-point degree-5 Dunavant rule, src/elements/integration_points.jl:188-202.
+void t7_rule(double xi[7], double eta[7], double w[7]) {
+    const double a1 = 0.101286507323456, a2 = 0.470142064105115;
+    const double b1 = 1 - 2 * a1, b2 = 1 - 2 * a2;
+    const double w1 = 0.062969590272414, w2 = 0.066197076394253, wc = 0.1125;
+    const double X[7] = {a1, b1, a1, a2, b2, a2, 1.0 / 3};
+    const double Y[7] = {a1, a1, b1, a2, a2, b2, 1.0 / 3};
+    const double W[7] = {w1, w1, w1, w2, w2, w2, wc};
+    for (int q = 0; q < 7; ++q) { xi[q] = X[q]; eta[q] = Y[q]; w[q] = W[q]; }
+}
+
+// P1 on the reference triangle, src/elements/shape_functions.jl:231-239.
+void p1_basis(double xi, double eta, double N[3]) { N[0] = 1 - xi - eta; N[1] = xi; N[2] = eta; }
+
+struct Tables {
+    double Nq[7][7], NqP[7][3], dNdxi[7][7][2], w[7];
+    Tables() {
+        double xi[7], eta[7];
+        t7_rule(xi, eta, w);
+        for (int q = 0; q < 7; ++q) {
+            t7_basis(xi[q], eta[q], Nq[q], dNdxi[q]);
+            p1_basis(xi[q], eta[q], NqP[q]);
+        }
+    }
+};
+const Tables& tables() { static Tables t; return t; }
+
+// ---------------------------------------------------------------------------
+// Phase helpers, src/utils.jl:37-44 (interp2ip_phase), :70-74 (_gather_phase)
+// ---------------------------------------------------------------------------
+inline int phase_of(const int32_t* ph, int mode, int64_t e, int a, int nloc, int64_t global) {
+    if (mode == ORC_PHASE_CELL) return ph[e] - 1;
+    if (mode == ORC_PHASE_ELEMENT) return ph[e * nloc + a] - 1;
+    return ph[global] - 1;
+}
+inline double interp_phase(const double* N, int n, const double* var, const int* ph) {
+    double out = 0;
+    for (int i = 0; i < n; ++i) out += N[i] * var[ph[i]];
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Constitutive update at one quadrature point.
+//   viscoelastic_coefficients_phase  src/stokes/assemblers/rheology.jl:35-42
+//   deviatoric_stress (2-D)          rheology.jl:226-245
+//   _deviatoric_stress_with_multiplier (DP) rheology.jl:272-334
+// ---------------------------------------------------------------------------
+struct Rheo {
+    const orc_material* mat;
+    const double* eta;   // per-phase viscosity actually used (mat->eta or element max)
+    const orc_dp* dp;
+    double dt;
+};
+
+template <class S>
+double stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, const int* ph,
+                 const Rheo& r, const double tau_o[3], double Pq, S tau[3]) {
+    S exx = gvx[0];
+    S eyy = gvy[1];
+    S exy = (gvx[1] + gvy[0]) / 2.0;
+    S tr = (exx + eyy) / 3.0;
+    double invG[ORC_MAX_PHASES];
+    for (int p = 0; p < r.mat->nphases; ++p) invG[p] = 1.0 / r.mat->G[p];
+    const double etaq = interp_phase(Nv, NV, r.eta, ph);
+    const double invGq = interp_phase(Nv, NV, invG, ph);
+    const double etave = 1.0 / (1.0 / etaq + invGq / r.dt);
+    const double inv2Gdt = invGq / (2 * r.dt);
+    S txx = 2 * etave * ((exx - tr) + tau_o[0] * inv2Gdt);
+    S tyy = 2 * etave * ((eyy - tr) + tau_o[1] * inv2Gdt);
+    S txy = 2 * etave * (exy + tau_o[2] * inv2Gdt);
+    double lambda = 0;
+    if (r.dp) {
+        const orc_dp* dp = r.dp;
+        const double cphi = interp_phase(Nv, NV, dp->cos_phi, ph);
+        const double sphi = interp_phase(Nv, NV, dp->sin_phi, ph);
+        const double spsi = interp_phase(Nv, NV, dp->sin_psi, ph);
+        const double C = interp_phase(Nv, NV, dp->C, ph);
+        const double ereg = interp_phase(Nv, NV, dp->eta_reg, ph);
+        const double Kb = interp_phase(Nv, NV, dp->Kb, ph);
+        // second_invariant, rheology.jl:204-212
+        S tzz = -txx - tyy;
+        S tII = dsqrt((txx * txx + tyy * tyy + tzz * tzz) / 2.0 + txy * txy + kEps * kEps);
+        S tIIs = tII + kEps * kEps;
+        S F = tII - cphi * C - sphi * Pq;
+        const double dFdP = -sphi, dQdP = -spsi;
+        S dQxx = (2.0 * txx + tyy) / (2.0 * tIIs);
+        S dQyy = (txx + 2.0 * tyy) / (2.0 * tIIs);
+        S dQxy = txy / tIIs;
+        if (val(F) > 0) {
+            S lam = F / (etave + ereg + Kb * r.dt * dQdP * dFdP);
+            lambda = val(lam);
+            if (lambda > 0) {
+                txx = txx - 2 * etave * lam * dQxx;
+                tyy = tyy - 2 * etave * lam * dQyy;
+                txy = txy - 2 * etave * lam * dQxy;
+            }
+        }
+    }
+    tau[0] = txx; tau[1] = tyy; tau[2] = txy;
+    return lambda;
+}
+
+// Element momentum residual, src/stokes/assemblers/momentum_residuals.jl:136-189
+// (2-D with Pnum, T and gravity); accumulate_momentum :28-33; eos_density :11-16.
+// Generic in NV/NP/nq; dNdx [nq][NV][2], Nq [nq][NV], NqP [nq][NP].
+template <class S>
+void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* dOm,
+                      const double* Nq, const double* NqP, const S* vx, const S* vy,
+                      const double* P, const S* Pnum /*nullable*/, const double* T,
+                      const double* Pf /*nullable*/, const int* ph, const Rheo& r,
+                      const double* tau_old /*nullable [3][nq]*/, S* Rx, S* Ry,
+                      double* tau_out /*nullable [3][nq]*/, double* Pq_out /*nullable [nq]*/) {
+    const orc_material* m = r.mat;
+    double beta[ORC_MAX_PHASES];
+    for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];
+    for (int a = 0; a < NV; ++a) { Rx[a] = S(0.0); Ry[a] = S(0.0); }
+    for (int q = 0; q < nq; ++q) {
+        const double* G = dNdx + (size_t)q * NV * 2;
+        const double* Nv = Nq + (size_t)q * NV;
+        const double* Np = NqP + (size_t)q * NP;
+        // strain-rate gradients: grad v = dNdx' * vloc
+        S gvx[2] = {S(0.0), S(0.0)}, gvy[2] = {S(0.0), S(0.0)};
+        for (int a = 0; a < NV; ++a) {
+            gvx[0] = gvx[0] + G[a * 2 + 0] * vx[a];
+            gvx[1] = gvx[1] + G[a * 2 + 1] * vx[a];
+            gvy[0] = gvy[0] + G[a * 2 + 0] * vy[a];
+            gvy[1] = gvy[1] + G[a * 2 + 1] * vy[a];
+        }
+        double to[3] = {0, 0, 0};
+        if (tau_old) { to[0] = tau_old[0 * nq + q]; to[1] = tau_old[1 * nq + q]; to[2] = tau_old[2 * nq + q]; }
+        double Pq = 0;
+        for (int i = 0; i < NP; ++i) Pq += Np[i] * P[i];
+        double Pfq = 0;
+        if (Pf) for (int i = 0; i < NP; ++i) Pfq += Np[i] * Pf[i];
+        const double Peq = Pq - Pfq;
+        S tau[3];
+        stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, tau);
+        const double Pq_local = Peq + Pfq;
+        if (tau_out) { tau_out[0 * nq + q] = val(tau[0]); tau_out[1 * nq + q] = val(tau[1]); tau_out[2 * nq + q] = val(tau[2]); }
+        if (Pq_out) Pq_out[q] = Pq_local;
+        S Ptot = S(Pq_local);
+        if (Pnum) { S pn = S(0.0); for (int i = 0; i < NP; ++i) pn = pn + Np[i] * Pnum[i]; Ptot = Ptot + pn; }
+        double Tq = 0;
+        for (int i = 0; i < NP; ++i) Tq += Np[i] * T[i];
+        const double rho = interp_phase(Nv, NV, m->rho0, ph) *
+                           (1 - interp_phase(Nv, NV, m->alpha, ph) * (Tq - m->Tref) +
+                            interp_phase(Nv, NV, beta, ph) * Pq);
+        for (int a = 0; a < NV; ++a) {
+            Rx[a] = Rx[a] + ((G[a * 2 + 0] * (tau[0] - Ptot) + G[a * 2 + 1] * tau[2]) - Nv[a] * (rho * m->g[0])) * dOm[q];
+            Ry[a] = Ry[a] + ((G[a * 2 + 0] * tau[2] + G[a * 2 + 1] * (tau[1] - Ptot)) - Nv[a] * (rho * m->g[1])) * dOm[q];
+        }
+    }
+}
+
+// Element continuity residual, src/stokes/assemblers/pressure_residual.jl:132-158;
+// divergence :18-30.  bulk = etab or K per phase (pressure_residual.jl:1-5).
+template <class S>
+void pressure_element(int NV, int NP, int nq, const double* dNdx, const double* dOmP,
+                      const double* NqP, const S* vx, const S* vy, const double* P,
+                      const double* P0, const double* T, const double* T0, const double* Q,
+                      const int* phP, const double* alpha, const double* bulk, int nphases,
+                      double dt, S* RP) {
+    bool all_inf = true;
+    for (int p = 0; p < nphases; ++p) all_inf = all_inf && std::isinf(bulk[p]);
+    for (int i = 0; i < NP; ++i) RP[i] = S(0.0);
+    for (int q = 0; q < nq; ++q) {
+        const double* G = dNdx + (size_t)q * NV * 2;
+        const double* Nv = NqP + (size_t)q * NP;
+        double Qq = 0;
+        if (Q) for (int i = 0; i < NP; ++i) Qq += Nv[i] * Q[i];
+        const double etabq = all_inf ? kInf : interp_phase(Nv, NP, bulk, phP);
+        const double aq = interp_phase(Nv, NP, alpha, phP);
+        double dP = 0, dT = 0;
+        for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
+        for (int i = 0; i < NP; ++i) dT += Nv[i] * (T[i] - T0[i]);
+        const double dPdt = dP / (etabq * dt);
+        const double dTdt = aq * dT / dt;
+        S div = S(0.0);
+        for (int a = 0; a < NV; ++a) div = div + G[a * 2 + 0] * vx[a];
+        for (int a = 0; a < NV; ++a) div = div + G[a * 2 + 1] * vy[a];
+        for (int i = 0; i < NP; ++i) RP[i] = RP[i] + Nv[i] * (((-div) - dPdt) + dTdt + Qq) * dOmP[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mesh-level helpers
+// ---------------------------------------------------------------------------
+struct Csr {
+    const int32_t* key = nullptr;
+    int64_t nels = 0, nnodes = 0;
+    std::vector<int64_t> off;
+    std::vector<int32_t> ent;  // e*7 + a, ascending e per node (connectivity.jl:341-349)
+};
+Csr& node2elem(const orc_stokes* s) {
+    static thread_local Csr c;
+    if (c.key == s->el2n && c.nels == s->nels && c.nnodes == s->nnodes) return c;
+    c.key = s->el2n; c.nels = s->nels; c.nnodes = s->nnodes;
+    c.off.assign(s->nnodes + 1, 0);
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 7; ++a) c.off[s->el2n[e * 7 + a] + 1]++;
+    for (int64_t n = 0; n < s->nnodes; ++n) c.off[n + 1] += c.off[n];
+    c.ent.assign(c.off[s->nnodes], 0);
+    std::vector<int64_t> fill(c.off.begin(), c.off.end() - 1);
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 7; ++a) c.ent[fill[s->el2n[e * 7 + a]]++] = (int32_t)(e * 7 + a);
+    return c;
+}
+
+void gather(const orc_stokes* s, const std::vector<double>& Re, double* out) {
+    const Csr& c = node2elem(s);
+#pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < s->nnodes; ++n) {
+        double acc = 0;
+        for (int64_t k = c.off[n]; k < c.off[n + 1]; ++k) acc += Re[c.ent[k]];
+        out[n] = acc;
+    }
+}
+
+void elem_dNdx(const orc_stokes* s, int64_t e, double dNdx[7][7][2]) {
+    const Tables& t = tables();
+    for (int q = 0; q < 7; ++q) {
+        const double* iJ = s->invJ + (e * 7 + q) * 4;
+        for (int a = 0; a < 7; ++a) {
+            dNdx[q][a][0] = t.dNdxi[q][a][0] * iJ[0] + t.dNdxi[q][a][1] * iJ[2];
+            dNdx[q][a][1] = t.dNdxi[q][a][0] * iJ[1] + t.dNdxi[q][a][1] * iJ[3];
+        }
+    }
+}
+
+void elem_phases_v(const orc_stokes* s, int64_t e, int ph[7]) {
+    for (int a = 0; a < 7; ++a) ph[a] = phase_of(s->phases_v, s->phase_mode_v, e, a, 7, s->el2n[e * 7 + a]);
+}
+void elem_phases_P(const orc_stokes* s, int64_t e, int ph[3]) {
+    for (int a = 0; a < 3; ++a) ph[a] = phase_of(s->phases_P, s->phase_mode_P, e, a, 3, e * 3 + a);
+}
+
+const double* pressure_bulk(const orc_stokes* s, int finite_K) { return finite_K ? s->mat.K : s->mat.etab; }
+
+void dirichlet(double* v, int64_t n, const int32_t* dofs, const double* vals) {
+    for (int64_t i = 0; i < n; ++i) v[dofs[i]] = vals ? vals[i] : 0.0;
+}
+
+double nrm2(const double* x, int64_t n) {
+    double s = 0;
+    for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
+    return std::sqrt(s);
+}
+
+// _checked_λmax, src/utils.jl:152-158
+bool checked_lmax(const double* J, const double* PC, int64_t n, const char* label, double* out) {
+    double m = -kInf;
+    bool nan = false;
+    for (int64_t i = 0; i < n; ++i) {
+        double r = J[i] / PC[i];
+        if (std::isnan(r)) nan = true;
+        m = std::max(m, r);
+    }
+    if (nan) m = std::numeric_limits<double>::quiet_NaN();
+    *out = m;
+    if (!(std::isfinite(m) && m > 0)) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s preconditioner produced invalid λmax = %g; check for zero or non-finite preconditioner entries", label, m);
+        g_err = buf;
+        return false;
+    }
+    return true;
+}
+
+// _stokes_λmin, src/stokes/solvers/DR.jl:32-37
+double stokes_lmin(double step, const double* rate, const double* R, const double* R0, const double* PC, int64_t n) {
+    double den = 0;
+    for (int64_t i = 0; i < n; ++i) { double t = step * rate[i]; den += t * t; }
+    if (den == 0) return 0;
+    double num = 0;
+    for (int64_t i = 0; i < n; ++i) num += (step * rate[i]) * ((R[i] - R0[i]) / PC[i]);
+    return std::fabs(num) / den;
+}
+
+// _stokes_cheb, DR.jl:39-42
+void stokes_cheb(double dtau, double lmin, double c_fact, double* a, double* b) {
+    double c = 2 * std::sqrt(lmin) * c_fact;
+    *a = 2 * dtau * dtau / (2 + c * dtau);
+    *b = (2 - c * dtau) / (2 + c * dtau);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+void orc_t7_basis(double xi, double eta, double* N, double* dN) {
+    double n[7], d[7][2];
+    t7_basis(xi, eta, n, d);
+    for (int a = 0; a < 7; ++a) { N[a] = n[a]; dN[2 * a] = d[a][0]; dN[2 * a + 1] = d[a][1]; }
+}
+void orc_t7_rule(double* xi, double* eta, double* w) { t7_rule(xi, eta, w); }
+void orc_t7_tables(double* Nq, double* NqP, double* dNdxi, double* w) {
+    const Tables& t = tables();
+    std::memcpy(Nq, t.Nq, sizeof t.Nq);
+    std::memcpy(NqP, t.NqP, sizeof t.NqP);
+    std::memcpy(dNdxi, t.dNdxi, sizeof t.dNdxi);
+    std::memcpy(w, t.w, sizeof t.w);
+}
+
+void orc_t7_mesh_sizes(int32_t nx, int32_t ny, int64_t* nnodes, int64_t* nels) {
+    *nnodes = (int64_t)(2 * nx + 1) * (2 * ny + 1) + 2LL * nx * ny;
+    *nels = 2LL * nx * ny;
+}
+
+// generate_coordinates(T7): src/mesh/mesh.jl:412-440 (T6 grid from
+// _tensor_grid_coordinates :352-361, LinRange interpolation, then centroids).
+// generate_element2node(T7): src/mesh/connectivity.jl:111-140.
+void orc_t7_mesh(int32_t nx, int32_t ny, double x0, double x1, double y0, double y1,
+                 double* coords, int32_t* el2n) {
+    const int64_t sx = 2 * nx + 1, sy = 2 * ny + 1;
+    auto lerp = [](int64_t j, int64_t d, double a, double b) {
+        double t = (double)j / (double)d;
+        return (1 - t) * a + t * b;
+    };
+    for (int64_t iy = 0; iy < sy; ++iy)
+        for (int64_t ix = 0; ix < sx; ++ix) {
+            coords[(iy * sx + ix) * 2 + 0] = lerp(ix, sx - 1, x0, x1);
+            coords[(iy * sx + ix) * 2 + 1] = lerp(iy, sy - 1, y0, y1);
+        }
+    const double dx = (x1 - x0) / nx, dy = (y1 - y0) / ny;
+    auto cent = [&](int64_t ex, int64_t ey, int a, int b, double* c) {
+        c[0] = x0 + (3 * ex + a) * dx / 3;
+        c[1] = y0 + (3 * ey + b) * dy / 3;
+    };
+    const int64_t nT6 = sx * sy;
+    int64_t k = nT6, iel = 0;
+    auto node = [&](int64_t ix, int64_t iy) { return (int32_t)(iy * sx + ix); };
+    for (int64_t ey = 0; ey < ny; ++ey)
+        for (int64_t ex = 0; ex < nx; ++ex) {
+            const int64_t ix = 2 * ex, iy = 2 * ey;
+            const int32_t BL = node(ix, iy), BR = node(ix + 2, iy), TR = node(ix + 2, iy + 2), TL = node(ix, iy + 2);
+            const int32_t bot = node(ix + 1, iy), rgt = node(ix + 2, iy + 1), top = node(ix + 1, iy + 2), lft = node(ix, iy + 1);
+            const int32_t ctr = node(ix + 1, iy + 1);
+            const int32_t cA = (int32_t)(nT6 + iel), cB = (int32_t)(nT6 + iel + 1);
+            int32_t* A = el2n + iel * 7;
+            int32_t* B = el2n + (iel + 1) * 7;
+            if ((ex + ey) % 2 == 0) {
+                const int32_t a[7] = {BL, BR, TR, bot, rgt, ctr, cA};
+                const int32_t b[7] = {BL, TR, TL, ctr, top, lft, cB};
+                std::memcpy(A, a, sizeof a); std::memcpy(B, b, sizeof b);
+                cent(ex, ey, 2, 1, coords + (k++) * 2);
+                cent(ex, ey, 1, 2, coords + (k++) * 2);
+            } else {
+                const int32_t a[7] = {BL, BR, TL, bot, ctr, lft, cA};
+                const int32_t b[7] = {BR, TR, TL, rgt, top, ctr, cB};
+                std::memcpy(A, a, sizeof a); std::memcpy(B, b, sizeof b);
+                cent(ex, ey, 1, 1, coords + (k++) * 2);
+                cent(ex, ey, 2, 2, coords + (k++) * 2);
+            }
+            iel += 2;
+        }
+}
+
+int64_t orc_box_boundary_nodes(int64_t nnodes, const double* coords, double x0, double x1,
+                               double y0, double y1, int32_t* nodes) {
+    int64_t k = 0;
+    const double tol = 32 * kEps * std::max({std::fabs(x0), std::fabs(x1), std::fabs(y0), std::fabs(y1), 1.0});
+    for (int64_t n = 0; n < nnodes; ++n) {
+        double x = coords[2 * n], y = coords[2 * n + 1];
+        if (std::fabs(x - x0) <= tol || std::fabs(x - x1) <= tol || std::fabs(y - y0) <= tol || std::fabs(y - y1) <= tol) {
+            if (nodes) nodes[k] = (int32_t)n;
+            ++k;
+        }
+    }
+    return k;
+}
+
+// precompute_geometry_kernel!: src/mesh/mesh.jl:674-682 with _geometry_entry
+// :550-553. Velocity geometry uses all 7 nodes and T7 gradients; pressure
+// geometry uses the 3 corner nodes and P1 gradients (mixed_mesh.jl:297-328).
+void orc_t7_geometry(int64_t nels, const double* coords, const int32_t* el2n,
+                     double* invJ, double* dOmega, double* dOmegaP) {
+    const Tables& t = tables();
+    const double dP1[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < nels; ++e) {
+        for (int q = 0; q < 7; ++q) {
+            double J[2][2] = {{0, 0}, {0, 0}};
+            for (int a = 0; a < 7; ++a) {
+                const double* c = coords + 2 * (int64_t)el2n[e * 7 + a];
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) J[i][j] += c[i] * t.dNdxi[q][a][j];
+            }
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            double* iJ = invJ + (e * 7 + q) * 4;
+            iJ[0] = J[1][1] / det; iJ[1] = -J[0][1] / det;
+            iJ[2] = -J[1][0] / det; iJ[3] = J[0][0] / det;
+            dOmega[e * 7 + q] = std::fabs(det) * t.w[q];
+            double JP[2][2] = {{0, 0}, {0, 0}};
+            for (int a = 0; a < 3; ++a) {
+                const double* c = coords + 2 * (int64_t)el2n[e * 7 + a];
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) JP[i][j] += c[i] * dP1[a][j];
+            }
+            const double detP = JP[0][0] * JP[1][1] - JP[0][1] * JP[1][0];
+            dOmegaP[e * 7 + q] = std::fabs(detP) * t.w[q];
+        }
+    }
+}
+
+void orc_pressure_element(int32_t NV, int32_t NP, int32_t nq, const double* dNdx, const double* dOm,
+                          const double* dOmP, const double* NqP, const double* vx, const double* vy,
+                          const double* P, const double* P0, const double* T, const double* T0,
+                          const double* Q, const int32_t* phaseP, const double* alpha,
+                          const double* bulk, double dt, double* RP_out) {
+    (void)dOm;
+    int ph[64];
+    int np = 0;
+    for (int i = 0; i < NP; ++i) { ph[i] = phaseP[i] - 1; np = std::max(np, ph[i] + 1); }
+    pressure_element<double>(NV, NP, nq, dNdx, dOmP, NqP, vx, vy, P, P0, T, T0, Q, ph, alpha, bulk, np, dt, RP_out);
+}
+
+void orc_momentum_element(int32_t NV, int32_t NP, int32_t nq, const double* dNdx, const double* dOm,
+                          const double* Nq, const double* NqP, const double* vx, const double* vy,
+                          const double* P, const double* Pnum, const double* T, const int32_t* phase,
+                          const orc_material* mat, const double* eta_override, const orc_dp* dp,
+                          double dt, const double* tau_old, double* Rx, double* Ry, double* tau_out) {
+    int ph[64];
+    for (int i = 0; i < NV; ++i) ph[i] = phase[i] - 1;
+    Rheo r{mat, eta_override ? eta_override : mat->eta, dp, dt};
+    momentum_element<double>(NV, NP, nq, dNdx, dOm, Nq, NqP, vx, vy, P, Pnum, T, nullptr, ph, r,
+                             tau_old, Rx, Ry, tau_out, nullptr);
+}
+
+double orc_stress_at_ip(const double* gradvx, const double* gradvy, const double* Nv, int32_t NV,
+                        const int32_t* phase, const orc_material* mat, const orc_dp* dp, double dt,
+                        const double* tau_old, double Pq, double* tau) {
+    int ph[64];
+    for (int i = 0; i < NV; ++i) ph[i] = phase[i] - 1;
+    Rheo r{mat, mat->eta, dp, dt};
+    return stress_ip<double>(gradvx, gradvy, Nv, NV, ph, r, tau_old, Pq, tau);
+}
+
+// assemble_viscosity_weighted_pressure_scaling!: pressure_scaling.jl:118-149,
+// kernel :166-195, pressure_scale_at_ip :205-211, finalize :213-216.
+void orc_pressure_scaling(orc_stokes* s, double gamma_fact, double dt, int32_t finite_K) {
+    const Tables& t = tables();
+    const double* bulk = pressure_bulk(s, finite_K);
+    double invK[ORC_MAX_PHASES];
+    for (int p = 0; p < s->mat.nphases; ++p) invK[p] = 1.0 / bulk[p];
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        int ph[7];
+        elem_phases_v(s, e, ph);
+        double MP[3] = {0, 0, 0}, gP[3] = {0, 0, 0};
+        for (int q = 0; q < 7; ++q) {
+            const double dO = s->dOmegaP[e * 7 + q];
+            const double etaq = interp_phase(t.Nq[q], 7, s->mat.eta, ph);
+            const double gnum = gamma_fact * etaq;
+            const double bq = interp_phase(t.Nq[q], 7, invK, ph);
+            const double gphy = bq == 0 ? gnum : dt / bq;
+            const double gq = gphy * gnum / (gphy + gnum);
+            for (int a = 0; a < 3; ++a) {
+                const double w = t.NqP[q][a] * dO;
+                MP[a] += w;
+                gP[a] += w * gq;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            s->MP[e * 3 + a] = MP[a];
+            s->gammaP[e * 3 + a] = MP[a] == 0 ? 0.0 : gP[a] / MP[a];
+        }
+    }
+}
+
+// assemble_pressure_residual_kernel!: pressure_residual.jl:256-326.
+void orc_pressure_residual(orc_stokes* s, double dt, int32_t finite_K, const double* P_old) {
+    const Tables& t = tables();
+    const double* bulk = pressure_bulk(s, finite_K);
+    if (!P_old) P_old = s->P0;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        double vx[7], vy[7];
+        for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
+        int ph[3];
+        elem_phases_P(s, e, ph);
+        double RPe[3];
+        pressure_element<double>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], vx, vy,
+                                 s->P + e * 3, P_old + e * 3, s->T + e * 3, s->T0 + e * 3,
+                                 s->Q ? s->Q + e * 3 : nullptr, ph, s->mat.alpha, bulk, s->mat.nphases, dt, RPe);
+        for (int a = 0; a < 3; ++a) s->RP[e * 3 + a] = RPe[a];
+    }
+}
+
+// assemble_momentum_residual_kernel!: momentum_residuals.jl:417-470, 497-528.
+static void momentum_residual_impl(orc_stokes* s, double dt, const double* Pnum, double* tau_store[3]) {
+    const Tables& t = tables();
+    std::vector<double> Rex((size_t)s->nels * 7), Rey((size_t)s->nels * 7);
+    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        double vx[7], vy[7];
+        for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
+        int ph[7];
+        elem_phases_v(s, e, ph);
+        double to[21];
+        bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
+        double tout[21];
+        momentum_element<double>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], vx, vy,
+                                 s->P + e * 3, Pnum ? Pnum + e * 3 : nullptr, s->T + e * 3,
+                                 s->Pf ? s->Pf + e * 3 : nullptr, ph, r, has_old ? to : nullptr,
+                                 &Rex[e * 7], &Rey[e * 7], tau_store ? tout : nullptr, nullptr);
+        if (tau_store)
+            for (int c = 0; c < 3; ++c)
+                if (tau_store[c])
+                    for (int q = 0; q < 7; ++q) tau_store[c][e * 7 + q] = tout[c * 7 + q];
+    }
+    gather(s, Rex, s->Rvx);
+    gather(s, Rey, s->Rvy);
+}
+
+void orc_momentum_residual(orc_stokes* s, double dt, const double* Pnum) {
+    momentum_residual_impl(s, dt, Pnum, nullptr);
+}
+
+// update_stokes_current_stress! (DR.jl:820-897): momentum assembly with
+// stress storage, Pnum = nothing; Rv is overwritten as in the reference.
+void orc_update_current_stress(orc_stokes* s, double dt) {
+    momentum_residual_impl(s, dt, nullptr, s->tau_store);
+}
+
+// assemble_augmented_momentum_jacobian_matrices_atomix!: momentum_residuals.jl
+// :998-1033, kernel :1044-1066, problem :929-970, folding :1073-1117; the
+// element viscosity is the element maximum (:36-44). Each (i, j) block is the
+// forward-mode derivative of R_i with respect to v_j with Pnum(v) inline.
+void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K) {
+    const Tables& t = tables();
+    const double* bulk = pressure_bulk(s, finite_K);
+    std::vector<double> Jx((size_t)s->nels * 7), Jy((size_t)s->nels * 7), Dx((size_t)s->nels * 7), Dy((size_t)s->nels * 7);
+    typedef Dual<7> D7;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        double vx[7], vy[7];
+        for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
+        int ph[7], phP[3];
+        elem_phases_v(s, e, ph);
+        elem_phases_P(s, e, phP);
+        // _element_max_phase_property: every phase gets the element max.
+        double emax = s->mat.eta[ph[0]];
+        for (int a = 1; a < 7; ++a) emax = std::max(emax, s->mat.eta[ph[a]]);
+        double eta_pc[ORC_MAX_PHASES];
+        for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = emax;
+        Rheo r{&s->mat, eta_pc, s->dp, dt};
+        double to[21];
+        bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
+        double block[2][2][7][7];  // [i][j][row][col] = dR_i[row]/dv_j[col]
+        for (int j = 0; j < 2; ++j) {
+            D7 ux[7], uy[7];
+            for (int a = 0; a < 7; ++a) {
+                ux[a] = D7(vx[a]); uy[a] = D7(vy[a]);
+                if (j == 0) ux[a].d[a] = 1.0; else uy[a].d[a] = 1.0;
+            }
+            D7 RPe[3];
+            pressure_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], ux, uy,
+                                 s->P + e * 3, s->P0 + e * 3, s->T + e * 3, s->T0 + e * 3,
+                                 s->Q ? s->Q + e * 3 : nullptr, phP, s->mat.alpha, bulk, s->mat.nphases, dt, RPe);
+            D7 Pn[3];
+            for (int a = 0; a < 3; ++a) Pn[a] = s->gammaP[e * 3 + a] * RPe[a] / s->MP[e * 3 + a];
+            D7 Rx[7], Ry[7];
+            momentum_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], ux, uy,
+                                 s->P + e * 3, Pn, s->T + e * 3, s->Pf ? s->Pf + e * 3 : nullptr, ph, r,
+                                 has_old ? to : nullptr, Rx, Ry, nullptr, nullptr);
+            for (int row = 0; row < 7; ++row)
+                for (int col = 0; col < 7; ++col) {
+                    block[0][j][row][col] = Rx[row].d[col];
+                    block[1][j][row][col] = Ry[row].d[col];
+                }
+        }
+        for (int row = 0; row < 7; ++row) {
+            double sx = 0, sy = 0;
+            for (int j = 0; j < 2; ++j) {
+                double bx = 0, by = 0;
+                for (int col = 0; col < 7; ++col) { bx += std::fabs(block[0][j][row][col]); by += std::fabs(block[1][j][row][col]); }
+                sx += bx; sy += by;
+            }
+            Jx[e * 7 + row] = sx; Jy[e * 7 + row] = sy;
+            Dx[e * 7 + row] = std::fabs(block[0][0][row][row]);
+            Dy[e * 7 + row] = std::fabs(block[1][1][row][row]);
+        }
+    }
+    gather(s, Jx, s->dRdvx);
+    gather(s, Jy, s->dRdvy);
+    gather(s, Dx, s->PCx);
+    gather(s, Dy, s->PCy);
+}
+
+// update_stokes_velocity_kernel!: src/stokes/helpers.jl:31-41 with the
+// negated step of DR.jl:690-695 (alpha here is the positive Chebyshev step).
+void orc_dr_update(orc_stokes* s, const double* alpha, const double* beta) {
+    double* rate[2] = {s->rate_x, s->rate_y};
+    double* v[2] = {s->vx, s->vy};
+    const double* R[2] = {s->Rvx, s->Rvy};
+    const double* PC[2] = {s->PCx, s->PCy};
+    for (int c = 0; c < 2; ++c) {
+        const double a = -alpha[c], b = beta[c];
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < s->nnodes; ++i) {
+            double nr = b * rate[c][i] + R[c][i] / PC[c][i];
+            rate[c][i] = nr;
+            v[c][i] += a * nr;
+        }
+    }
+}
+
+// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802).
+int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
+    std::memset(out, 0, sizeof *out);
+    g_err.clear();
+    const double dt = p->dt;
+    const int finite_K = p->finite_K;
+    const int64_t ncheck = p->check_interval;
+    const int64_t nv = s->nnodes, np = s->nnodesP;
+    const bool freeze = p->freeze_jacobian < 0 ? (s->dp == nullptr) : (p->freeze_jacobian != 0);
+    if (!p->skip_scaling) orc_pressure_scaling(s, p->pressure_factor, dt, finite_K);
+
+    double* rate[2] = {s->rate_x, s->rate_y};
+    double* Rv[2] = {s->Rvx, s->Rvy};
+    double* Rv0[2] = {s->Rv0x, s->Rv0y};
+    double* dRdv[2] = {s->dRdvx, s->dRdvy};
+    double* PC[2] = {s->PCx, s->PCy};
+    double* v[2] = {s->vx, s->vy};
+    for (int c = 0; c < 2; ++c) { std::fill(rate[c], rate[c] + nv, 0.0); std::fill(Rv0[c], Rv0[c] + nv, 0.0); }
+    const char* labels[2] = {"stokes vx", "stokes vy"};
+
+    orc_augmented_jacobian(s, dt, finite_K);
+    double lmax[2];
+    for (int c = 0; c < 2; ++c)
+        if (!checked_lmax(dRdv[c], PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
+    double lmax_g = std::max(lmax[0], lmax[1]);
+    int64_t jac_assemblies = 1;
+    double dtau[2], al[2], be[2];
+    for (int c = 0; c < 2; ++c) {
+        dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
+        stokes_cheb(dtau[c], 0.0, p->c_fact, &al[c], &be[c]);
+    }
+
+    double err_min = kInf, eps_tol = p->tolerance, err = 2 * eps_tol;
+    double err_abs = kInf, err_rel = kInf, err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
+    double err_v = kInf, err_P = kInf;
+    int64_t iter = 0, itPH_done = 0;
+    bool converged = false;
+    double rel_drop = p->rel_drop0;
+    std::vector<double> tmp(np);
+
+    auto err_pressure = [&]() {
+        for (int64_t i = 0; i < np; ++i) tmp[i] = s->RP[i] / s->MP[i];
+        return nrm2(tmp.data(), np) / std::sqrt((double)np);
+    };
+    auto err_velocity = [&]() {
+        return std::max(nrm2(Rv[0], nv), nrm2(Rv[1], nv)) / (2 * std::sqrt((double)nv));
+    };
+
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        orc_pressure_residual(s, dt, finite_K, s->P0);
+        orc_momentum_residual(s, dt, nullptr);
+        for (int c = 0; c < 2; ++c) dirichlet(Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+        err_P = err_pressure();
+        err_v = err_velocity();
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        const double err_v_rel = err_v / err_v0, err_P_rel = err_P / err_P0;
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v_rel, err_P_rel);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            g_err = "NaN detected in outer loop at PH=" + std::to_string(itPH); out->status = -4; return -4;
+        }
+        if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in outer loop at PH=" + std::to_string(itPH); out->status = -5; return -5; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
+                   (long long)itPH, (long long)iter, err, err_abs, err_rel, err_v, err_v_rel, err_P, err_P_rel);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT; ++iter;
+            const bool do_check = iter % ncheck == 0;
+            if (do_check) for (int c = 0; c < 2; ++c) std::copy(Rv[c], Rv[c] + nv, Rv0[c]);
+            orc_pressure_residual(s, dt, finite_K, s->P0);
+            for (int64_t i = 0; i < np; ++i) s->Pnum[i] = s->gammaP[i] * s->RP[i] / s->MP[i];
+            orc_momentum_residual(s, dt, s->Pnum);
+            for (int c = 0; c < 2; ++c) dirichlet(Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+            for (int c = 0; c < 2; ++c) dirichlet(rate[c], s->nbc[c], s->bc_dofs[c], nullptr);
+            orc_dr_update(s, al, be);
+            for (int c = 0; c < 2; ++c) dirichlet(v[c], s->nbc[c], s->bc_dofs[c], s->bc_vals[c]);
+            if (do_check) {
+                const double evi = err_velocity();
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { g_err = "NaN detected in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -4; return -4; }
+                if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -5; return -5; }
+                double lmin[2];
+                for (int c = 0; c < 2; ++c) lmin[c] = stokes_lmin(al[c], rate[c], Rv[c], Rv0[c], PC[c], nv);
+                if (!freeze) {
+                    ++jac_assemblies;
+                    orc_augmented_jacobian(s, dt, finite_K);
+                    for (int c = 0; c < 2; ++c)
+                        if (!checked_lmax(dRdv[c], PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
+                    lmax_g = std::max(lmax[0], lmax[1]);
+                }
+                for (int c = 0; c < 2; ++c) {
+                    dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
+                    stokes_cheb(dtau[c], lmin[c], p->c_fact, &al[c], &be[c]);
+                }
+            }
+            if (iter > p->max_iterations) break;
+        }
+        for (int64_t i = 0; i < np; ++i) s->P[i] += s->gammaP[i] * s->RP[i] / s->MP[i];
+        if (iter > p->max_iterations) break;
+    }
+    out->itPH = itPH_done;
+    out->iter = iter;
+    out->err = err;
+    out->err_abs = err_abs;
+    out->err_rel = err_rel;
+    out->err_v = err_v;
+    out->err_P = err_P;
+    out->converged = converged;
+    out->reached_total_iter = iter > p->max_iterations;
+    out->lambda_max = std::max(lmax[0], lmax[1]);
+    out->lambda_max_gershgorin = lmax_g;
+    out->jacobian_assemblies = jac_assemblies;
+    out->status = 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+/*
+ * femdr_oracle.h -- CPU FP64 restatement of FEMTools.jl's matrix-free
+ * dynamic-relaxation (DR) Stokes hot path, used ONLY as a test checker.
+ *
+ * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library. The product path (libfemdr.so)
+ * never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates
+ * (paths relative to the FEMTools.jl repository root).
+ *
+ * Conventions: 0-based indices; element-node connectivity el2n is [nels][NV]
+ * (element-contiguous, the C view of Julia's NV x nels column-major matrix);
+ * integration-point arrays are [nels][nq]; pressure DoF a of element e is
+ * 3*e + a (src/mesh/mixed_mesh.jl:447-454).
+ */
+#ifndef FEMDR_ORACLE_H
+#define FEMDR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_PHASES 64
+
+/* StokesMaterial (src/stokes/types/stokes_types.jl:19-60). */
+typedef struct {
+    int32_t nphases;
+    double eta[ORC_MAX_PHASES];
+    double etab[ORC_MAX_PHASES];
+    double G[ORC_MAX_PHASES];
+    double alpha[ORC_MAX_PHASES];
+    double rho0[ORC_MAX_PHASES];
+    double K[ORC_MAX_PHASES];
+    double g[2];
+    double Tref;
+} orc_material;
+
+/* DruckerPrager (src/stokes/types/stokes_types.jl:388-423): stored as
+ * cos(phi), sin(phi), sin(psi), C, eta_reg, Kb per phase. */
+typedef struct {
+    int32_t nphases;
+    double cos_phi[ORC_MAX_PHASES];
+    double sin_phi[ORC_MAX_PHASES];
+    double sin_psi[ORC_MAX_PHASES];
+    double C[ORC_MAX_PHASES];
+    double eta_reg[ORC_MAX_PHASES];
+    double Kb[ORC_MAX_PHASES];
+} orc_dp;
+
+/* Phase layouts (src/utils.jl:70-74): */
+#define ORC_PHASE_CELL 0      /* one phase per element (1 x nels matrix)        */
+#define ORC_PHASE_ELEMENT 1   /* one phase per element-local node (NV x nels)  */
+#define ORC_PHASE_NODAL 2     /* one phase per global node / pressure DoF       */
+
+/* Whole Stokes problem on a 2-D T7/P1-disc mixed mesh. All arrays are host
+ * memory owned by the caller. Nullable pointers are marked. */
+typedef struct {
+    int64_t nnodes, nels, nnodesP;
+    const double* coords;   /* [nnodes][2] */
+    const int32_t* el2n;    /* [nels][7], 0-based */
+    const double* invJ;     /* [nels][7][4] row-major J^-1 per quadrature point */
+    const double* dOmega;   /* [nels][7] |det J| w (velocity geometry)         */
+    const double* dOmegaP;  /* [nels][7] |det J| w (pressure geometry)         */
+    const int32_t* phases_v; int32_t phase_mode_v;  /* 1-based phase ids */
+    const int32_t* phases_P; int32_t phase_mode_P;
+    orc_material mat;
+    const orc_dp* dp;       /* nullable: purely visco-elastic                  */
+    /* state (StokesDR fields, src/stokes/types/stokes_types.jl:197-298) */
+    double *vx, *vy, *rate_x, *rate_y, *Rvx, *Rvy, *Rv0x, *Rv0y;
+    double *dRdvx, *dRdvy, *PCx, *PCy;
+    double *P, *P0, *gammaP, *T, *T0, *Q, *RP, *MP, *Pnum;
+    const double* Pf;       /* nullable (zero fluid pressure)                  */
+    const double* tau_old[3]; /* nullable each: [nels][7] xx, yy, xy           */
+    double* tau_store[3];     /* nullable: receives [nels][7] current stress   */
+    /* Dirichlet conditions per component */
+    int64_t nbc[2];
+    const int32_t* bc_dofs[2];
+    const double* bc_vals[2];
+} orc_stokes;
+
+/* solve!/_solve_stokes_dyrel! keyword parameters (src/stokes/solvers/DR.jl:302-386) */
+typedef struct {
+    double dt;
+    double pressure_factor;    /* gamma fact, default 50            */
+    double tolerance;          /* eps_tol                           */
+    int64_t max_iterations;    /* total_iterMax                     */
+    int64_t iter_max;          /* iterMax (inner per PH)            */
+    int64_t check_interval;    /* ncheck                            */
+    int64_t max_ph_iterations; /* default 1000                      */
+    double rel_drop0;          /* default 1e-2                      */
+    double CFL_v, c_fact;      /* StokesDR defaults 0.98, 0.9       */
+    int32_t finite_K;          /* pressure bulk = K if 1 else etab  */
+    int32_t freeze_jacobian;   /* -1: default (plastic == nothing)  */
+    int32_t skip_scaling;      /* 1: gammaP/MP already assembled    */
+    int32_t verbose;
+} orc_params;
+
+typedef struct {
+    int64_t itPH, iter;
+    double err, err_abs, err_rel, err_v, err_P;
+    int32_t converged, reached_total_iter;
+    double lambda_max, lambda_max_gershgorin;
+    int64_t jacobian_assemblies;
+    int32_t status;            /* 0 ok, <0 error (see orc_last_error) */
+} orc_stats;
+
+const char* orc_last_error(void);
+int orc_num_threads(void);
+void orc_set_threads(int n);
+
+/* --- reference elements (src/elements/shape_functions.jl:115-171, 231-239;
+ *     src/elements/integration_points.jl:188-202) --- */
+void orc_t7_basis(double xi, double eta, double* N /*7*/, double* dN /*7x2*/);
+void orc_t7_rule(double* xi /*7*/, double* eta /*7*/, double* w /*7*/);
+void orc_t7_tables(double* Nq /*7x7*/, double* NqP /*7x3*/, double* dNdxi /*7x7x2*/, double* w /*7*/);
+
+/* --- structured T7 mesh (src/mesh/mesh.jl:412-440, src/mesh/connectivity.jl:111-140) --- */
+void orc_t7_mesh_sizes(int32_t nx, int32_t ny, int64_t* nnodes, int64_t* nels);
+void orc_t7_mesh(int32_t nx, int32_t ny, double x0, double x1, double y0, double y1,
+                 double* coords, int32_t* el2n);
+/* boundary nodes of the box (mesh.jl:79-80); returns count, fills nodes if non-null */
+int64_t orc_box_boundary_nodes(int64_t nnodes, const double* coords, double x0, double x1,
+                               double y0, double y1, int32_t* nodes);
+
+/* --- geometry (src/mesh/mesh.jl:550-553, 674-682; mixed_mesh.jl:297-328) --- */
+void orc_t7_geometry(int64_t nels, const double* coords, const int32_t* el2n,
+                     double* invJ, double* dOmega, double* dOmegaP);
+
+/* --- element-level kernels on caller tables (for closed-form checks) ---
+ * dNdx: [nq][NV][2], dOm: [nq], Nq: [nq][NV], NqP: [nq][NP] */
+void orc_pressure_element(int32_t NV, int32_t NP, int32_t nq,
+                          const double* dNdx, const double* dOm, const double* dOmP,
+                          const double* NqP, const double* vx, const double* vy,
+                          const double* P, const double* P0, const double* T, const double* T0,
+                          const double* Q /*nullable*/, const int32_t* phaseP /*NP*/,
+                          const double* alpha, const double* bulk, double dt, double* RP_out);
+void orc_momentum_element(int32_t NV, int32_t NP, int32_t nq,
+                          const double* dNdx, const double* dOm, const double* Nq, const double* NqP,
+                          const double* vx, const double* vy, const double* P,
+                          const double* Pnum /*nullable*/, const double* T,
+                          const int32_t* phase /*NV*/, const orc_material* mat,
+                          const double* eta_override /*nullable: per-phase eta*/,
+                          const orc_dp* dp /*nullable*/, double dt,
+                          const double* tau_old /*nullable [3][nq]*/,
+                          double* Rx, double* Ry, double* tau_out /*nullable [3][nq]*/);
+/* one quadrature-point stress (rheology.jl:226-334): returns lambda */
+double orc_stress_at_ip(const double* gradvx /*2*/, const double* gradvy /*2*/,
+                        const double* Nv, int32_t NV, const int32_t* phase,
+                        const orc_material* mat, const orc_dp* dp, double dt,
+                        const double* tau_old /*3*/, double Pq, double* tau /*3*/);
+
+/* --- assemblers over the whole mesh --- */
+void orc_pressure_scaling(orc_stokes* s, double gamma_fact, double dt, int32_t finite_K);
+void orc_pressure_residual(orc_stokes* s, double dt, int32_t finite_K, const double* P_old);
+void orc_momentum_residual(orc_stokes* s, double dt, const double* Pnum /*nullable*/);
+void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K);
+void orc_dr_update(orc_stokes* s, const double* alpha /*2*/, const double* beta /*2*/);
+
+/* --- full solve (src/stokes/solvers/DR.jl:302-337, 348-802) --- */
+int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out);
+/* update_stokes_current_stress! (DR.jl:820-897): tau_store must be set */
+void orc_update_current_stress(orc_stokes* s, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

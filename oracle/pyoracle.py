@@ -1,0 +1,316 @@
+"""ctypes binding of the CPU oracle (liboracle_femdr.so).
+
+TEST INFRASTRUCTURE ONLY. Imported by tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg, as the checker; the product (libfemdr.so and
+the ``femtools_amd`` package) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_femdr.so")
+MAXPH = 64
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run `make -C oracle`)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+D = C.c_double
+I32 = C.c_int32
+I64 = C.c_int64
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int32)
+
+
+class Material(C.Structure):
+    _fields_ = [("nphases", I32), ("eta", D * MAXPH), ("etab", D * MAXPH), ("G", D * MAXPH),
+                ("alpha", D * MAXPH), ("rho0", D * MAXPH), ("K", D * MAXPH), ("g", D * 2), ("Tref", D)]
+
+
+class DP(C.Structure):
+    _fields_ = [("nphases", I32), ("cos_phi", D * MAXPH), ("sin_phi", D * MAXPH), ("sin_psi", D * MAXPH),
+                ("C", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH)]
+
+
+class Stokes(C.Structure):
+    _fields_ = [("nnodes", I64), ("nels", I64), ("nnodesP", I64),
+                ("coords", PD), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("dOmegaP", PD),
+                ("phases_v", PI), ("phase_mode_v", I32), ("phases_P", PI), ("phase_mode_P", I32),
+                ("mat", Material), ("dp", C.POINTER(DP)),
+                ("vx", PD), ("vy", PD), ("rate_x", PD), ("rate_y", PD), ("Rvx", PD), ("Rvy", PD),
+                ("Rv0x", PD), ("Rv0y", PD), ("dRdvx", PD), ("dRdvy", PD), ("PCx", PD), ("PCy", PD),
+                ("P", PD), ("P0", PD), ("gammaP", PD), ("T", PD), ("T0", PD), ("Q", PD), ("RP", PD),
+                ("MP", PD), ("Pnum", PD), ("Pf", PD), ("tau_old", PD * 3), ("tau_store", PD * 3),
+                ("nbc", I64 * 2), ("bc_dofs", PI * 2), ("bc_vals", PD * 2)]
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", D), ("pressure_factor", D), ("tolerance", D), ("max_iterations", I64),
+                ("iter_max", I64), ("check_interval", I64), ("max_ph_iterations", I64), ("rel_drop0", D),
+                ("CFL_v", D), ("c_fact", D), ("finite_K", I32), ("freeze_jacobian", I32),
+                ("skip_scaling", I32), ("verbose", I32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("itPH", I64), ("iter", I64), ("err", D), ("err_abs", D), ("err_rel", D), ("err_v", D),
+                ("err_P", D), ("converged", I32), ("reached_total_iter", I32), ("lambda_max", D),
+                ("lambda_max_gershgorin", D), ("jacobian_assemblies", I64), ("status", I32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _declare(L):
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_t7_basis.argtypes = [D, D, PD, PD]
+    L.orc_t7_rule.argtypes = [PD, PD, PD]
+    L.orc_t7_tables.argtypes = [PD, PD, PD, PD]
+    L.orc_t7_mesh_sizes.argtypes = [I32, I32, C.POINTER(I64), C.POINTER(I64)]
+    L.orc_t7_mesh.argtypes = [I32, I32, D, D, D, D, PD, PI]
+    L.orc_box_boundary_nodes.argtypes = [I64, PD, D, D, D, D, PI]
+    L.orc_box_boundary_nodes.restype = I64
+    L.orc_t7_geometry.argtypes = [I64, PD, PI, PD, PD, PD]
+    L.orc_pressure_element.argtypes = [I32, I32, I32, PD, PD, PD, PD, PD, PD, PD, PD, PD, PD, PD, PI, PD, PD, D, PD]
+    L.orc_momentum_element.argtypes = [I32, I32, I32, PD, PD, PD, PD, PD, PD, PD, PD, PD, PI,
+                                       C.POINTER(Material), PD, C.POINTER(DP), D, PD, PD, PD, PD]
+    L.orc_stress_at_ip.argtypes = [PD, PD, PD, I32, PI, C.POINTER(Material), C.POINTER(DP), D, PD, D, PD]
+    L.orc_stress_at_ip.restype = D
+    L.orc_pressure_scaling.argtypes = [C.POINTER(Stokes), D, D, I32]
+    L.orc_pressure_residual.argtypes = [C.POINTER(Stokes), D, I32, PD]
+    L.orc_momentum_residual.argtypes = [C.POINTER(Stokes), D, PD]
+    L.orc_augmented_jacobian.argtypes = [C.POINTER(Stokes), D, I32]
+    L.orc_dr_update.argtypes = [C.POINTER(Stokes), PD, PD]
+    L.orc_stokes_solve.argtypes = [C.POINTER(Stokes), C.POINTER(Params), C.POINTER(Stats)]
+    L.orc_stokes_solve.restype = C.c_int
+    L.orc_update_current_stress.argtypes = [C.POINTER(Stokes), D]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_num_threads.restype = C.c_int
+
+
+def ptr(a, kind=PD):
+    if a is None:
+        return C.cast(None, kind)
+    return a.ctypes.data_as(kind)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
+
+
+# ---------------------------------------------------------------------------
+def t7_basis(xi, eta):
+    N = np.zeros(7)
+    dN = np.zeros((7, 2))
+    lib().orc_t7_basis(xi, eta, ptr(N), ptr(dN))
+    return N, dN
+
+
+def t7_rule():
+    xi, eta, w = np.zeros(7), np.zeros(7), np.zeros(7)
+    lib().orc_t7_rule(ptr(xi), ptr(eta), ptr(w))
+    return xi, eta, w
+
+
+def t7_tables():
+    Nq, NqP, dNdxi, w = np.zeros((7, 7)), np.zeros((7, 3)), np.zeros((7, 7, 2)), np.zeros(7)
+    lib().orc_t7_tables(ptr(Nq), ptr(NqP), ptr(dNdxi), ptr(w))
+    return Nq, NqP, dNdxi, w
+
+
+def t7_mesh(nx, ny, box=(0.0, 1.0, 0.0, 1.0)):
+    nn, ne = I64(), I64()
+    lib().orc_t7_mesh_sizes(nx, ny, C.byref(nn), C.byref(ne))
+    coords = np.zeros((nn.value, 2))
+    el2n = np.zeros((ne.value, 7), dtype=np.int32)
+    lib().orc_t7_mesh(nx, ny, *box, ptr(coords), ptr(el2n, PI))
+    return coords, el2n
+
+
+def box_boundary_nodes(coords, box=(0.0, 1.0, 0.0, 1.0)):
+    n = lib().orc_box_boundary_nodes(len(coords), ptr(coords), *box, ptr(None, PI))
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_box_boundary_nodes(len(coords), ptr(coords), *box, ptr(out, PI))
+    return out
+
+
+def t7_geometry(coords, el2n):
+    ne = len(el2n)
+    invJ = np.zeros((ne, 7, 4))
+    dO = np.zeros((ne, 7))
+    dOP = np.zeros((ne, 7))
+    lib().orc_t7_geometry(ne, ptr(coords), ptr(el2n, PI), ptr(invJ), ptr(dO), ptr(dOP))
+    return invJ, dO, dOP
+
+
+def material(eta=(1.0,), etab=(1.0,), G=(math.inf,), alpha=(0.0,), rho0=(1.0,), K=(math.inf,),
+             g=(0.0, 0.0), Tref=0.0):
+    """StokesMaterial defaults (src/stokes/types/stokes_types.jl:45-60)."""
+    props = [eta, etab, G, alpha, rho0, K]
+    n = max(len(p) if isinstance(p, (tuple, list, np.ndarray)) else 1 for p in props)
+    m = Material()
+    m.nphases = n
+    for name, val in zip(("eta", "etab", "G", "alpha", "rho0", "K"), props):
+        vals = list(val) if isinstance(val, (tuple, list, np.ndarray)) else [val] * n
+        if len(vals) == 1:
+            vals = vals * n
+        arr = getattr(m, name)
+        for i, x in enumerate(vals):
+            arr[i] = float(x)
+    m.g[0], m.g[1] = float(g[0]), float(g[1])
+    m.Tref = float(Tref)
+    return m
+
+
+def drucker_prager(phi, psi, C_, eta_reg, Kb):
+    """DruckerPrager(ϕ, Ψ, C, η_reg, Kb) (stokes_types.jl:411-423); angles in radians."""
+    n = len(phi)
+    dp = DP()
+    dp.nphases = n
+    for i in range(n):
+        dp.cos_phi[i] = math.cos(phi[i])
+        dp.sin_phi[i] = math.sin(phi[i])
+        dp.sin_psi[i] = math.sin(psi[i])
+        dp.C[i] = C_[i]
+        dp.eta_reg[i] = eta_reg[i]
+        dp.Kb[i] = Kb[i]
+    return dp
+
+
+@dataclass
+class Problem:
+    """Host arrays of one T7/P1-disc Stokes problem, mirroring StokesDR fields."""
+    coords: np.ndarray
+    el2n: np.ndarray
+    mat: Material
+    dp: DP | None = None
+    phases_v: np.ndarray | None = None
+    phase_mode_v: int = 0
+    phases_P: np.ndarray | None = None
+    phase_mode_P: int = 0
+    tau_old: tuple | None = None
+    bc: tuple = ((np.zeros(0, np.int32), np.zeros(0)), (np.zeros(0, np.int32), np.zeros(0)))
+    arrays: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.coords = f64(self.coords)
+        self.el2n = i32(self.el2n)
+        nn, ne = len(self.coords), len(self.el2n)
+        npp = 3 * ne
+        self.nnodes, self.nels, self.nnodesP = nn, ne, npp
+        self.invJ, self.dOmega, self.dOmegaP = t7_geometry(self.coords, self.el2n)
+        if self.phases_v is None:
+            self.phases_v, self.phase_mode_v = np.ones(ne, np.int32), 0
+        if self.phases_P is None:
+            self.phases_P, self.phase_mode_P = np.ones(ne, np.int32), 0
+        self.phases_v = i32(self.phases_v)
+        self.phases_P = i32(self.phases_P)
+        for k in ("vx", "vy", "rate_x", "rate_y", "Rvx", "Rvy", "Rv0x", "Rv0y", "dRdvx", "dRdvy", "PCx", "PCy"):
+            self.arrays[k] = np.zeros(nn)
+        for k in ("P", "P0", "gammaP", "T", "T0", "Q", "RP", "MP", "Pnum"):
+            self.arrays[k] = np.zeros(npp)
+        self.tau_store = [np.zeros((ne, 7)) for _ in range(3)]
+        if self.tau_old is not None:
+            self.tau_old = tuple(f64(t) for t in self.tau_old)
+        self.bc = tuple((i32(d), f64(v)) for d, v in self.bc)
+        self._dp_ref = self.dp
+
+    def __getattr__(self, k):
+        arrays = self.__dict__.get("arrays", {})
+        if k in arrays:
+            return arrays[k]
+        raise AttributeError(k)
+
+    def struct(self):
+        s = Stokes()
+        s.nnodes, s.nels, s.nnodesP = self.nnodes, self.nels, self.nnodesP
+        s.coords, s.el2n = ptr(self.coords), ptr(self.el2n, PI)
+        s.invJ, s.dOmega, s.dOmegaP = ptr(self.invJ), ptr(self.dOmega), ptr(self.dOmegaP)
+        s.phases_v, s.phase_mode_v = ptr(self.phases_v, PI), self.phase_mode_v
+        s.phases_P, s.phase_mode_P = ptr(self.phases_P, PI), self.phase_mode_P
+        s.mat = self.mat
+        s.dp = C.pointer(self.dp) if self.dp is not None else C.cast(None, C.POINTER(DP))
+        for k, a in self.arrays.items():
+            setattr(s, k, ptr(a))
+        s.Pf = ptr(None)
+        for c in range(3):
+            s.tau_old[c] = ptr(self.tau_old[c]) if self.tau_old is not None else ptr(None)
+            s.tau_store[c] = ptr(self.tau_store[c])
+        for c in range(2):
+            d, v = self.bc[c]
+            s.nbc[c] = len(d)
+            s.bc_dofs[c] = ptr(d, PI)
+            s.bc_vals[c] = ptr(v)
+        self._struct = s
+        return s
+
+    # assemblers -------------------------------------------------------------
+    def pressure_scaling(self, gamma_fact, dt, finite_K=False):
+        lib().orc_pressure_scaling(C.byref(self.struct()), gamma_fact, dt, int(finite_K))
+
+    def pressure_residual(self, dt, finite_K=False):
+        lib().orc_pressure_residual(C.byref(self.struct()), dt, int(finite_K), ptr(None))
+
+    def momentum_residual(self, dt, Pnum=None):
+        lib().orc_momentum_residual(C.byref(self.struct()), dt, ptr(f64(Pnum)) if Pnum is not None else ptr(None))
+
+    def augmented_jacobian(self, dt, finite_K=False):
+        lib().orc_augmented_jacobian(C.byref(self.struct()), dt, int(finite_K))
+
+    def update_current_stress(self, dt):
+        lib().orc_update_current_stress(C.byref(self.struct()), dt)
+        return self.tau_store
+
+    def solve(self, dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
+              check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
+              finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False):
+        p = Params(dt=dt, pressure_factor=pressure_factor, tolerance=tolerance, max_iterations=max_iterations,
+                   iter_max=iter_max, check_interval=check_interval, max_ph_iterations=max_ph_iterations,
+                   rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
+                   freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
+        st = Stats()
+        rc = lib().orc_stokes_solve(C.byref(self.struct()), C.byref(p), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return st.as_dict()
+
+
+def pure_shear_bc(coords, box=(0.0, 1.0, 0.0, 1.0), edot=1.0):
+    """Pure-shear velocity BCs of the pure-shear triangle example
+    (examples/miniapps/stokes/stokes_2D_pure_shear_triangle/...jl:145-154)."""
+    x0, x1, y0, y1 = box
+    gam = box_boundary_nodes(coords, box)
+    tol = max(abs(x1 - x0), abs(y1 - y0)) * np.finfo(float).eps * 32
+    x, y = coords[gam, 0], coords[gam, 1]
+    nx = gam[(np.abs(x - x0) <= tol) | (np.abs(x - x1) <= tol)]
+    ny = gam[(np.abs(y - y0) <= tol) | (np.abs(y - y1) <= tol)]
+    xm, ym = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    vx = edot * (coords[nx, 0] - xm)
+    vy = -edot * (coords[ny, 1] - ym)
+    return (nx, vx), (ny, vy)
