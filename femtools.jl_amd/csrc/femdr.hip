@@ -1,0 +1,1225 @@
+// femdr.hip -- host side of libfemdr.so: the C ABI of include/femdr.h, mesh
+// construction, state, and the Powell-Hestenes/DYREL control loop of
+// src/stokes/solvers/DR.jl:348-802 driving the kernels of femdr_kernels.hpp.
+//
+// The device never waits on the host between convergence checks: the
+// ncheck-1 pseudo-iterations between two checks are launched back to back on
+// the context stream; the host synchronises only to read the few reduced
+// scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
+#include "../../include/femdr.h"
+#include "femdr_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <set>
+#include <string>
+#include <vector>
+
+namespace fem {
+__constant__ Tables c_tab;
+}
+
+using namespace fem;
+
+struct fem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::set<void*> bufs;
+    double* red_partial = nullptr;   // [RED_GRID * RED_MAXK]
+    double* red_out = nullptr;       // [RED_MAXK]
+    double* h_red = nullptr;         // pinned [RED_MAXK]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> tev;     // kernel-timing events (3 per timed iteration)
+    size_t tev_used = 0;
+};
+
+// Kernel timing: events e0 | element | e1 | gather | e2 per iteration, read
+// back at the next host synchronisation.
+struct KernelTimer {
+    fem_ctx* c;
+    bool on;
+    double ms_elem = 0, ms_gather = 0;
+    int64_t n = 0;
+    KernelTimer(fem_ctx* c_, bool on_) : c(c_), on(on_) { c->tev_used = 0; }
+    hipEvent_t ev() {
+        if (c->tev_used == c->tev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            c->tev.push_back(e);
+        }
+        return c->tev[c->tev_used++];
+    }
+    void mark(hipEvent_t e) {
+        if (e) (void)hipEventRecord(e, c->stream);
+    }
+    // call after the stream has been synchronised
+    void collect() {
+        if (!on) return;
+        for (size_t i = 0; i + 2 < c->tev_used; i += 3) {
+            float a = 0, b = 0;
+            if (hipEventElapsedTime(&a, c->tev[i], c->tev[i + 1]) == hipSuccess &&
+                hipEventElapsedTime(&b, c->tev[i + 1], c->tev[i + 2]) == hipSuccess) {
+                ms_elem += a;
+                ms_gather += b;
+                ++n;
+            }
+        }
+        c->tev_used = 0;
+    }
+};
+
+struct fem_mesh {
+    fem_ctx* ctx = nullptr;
+    int64_t nnodes = 0, nels = 0, nnodesP = 0, ns = 0;
+    bool structured = false;
+    double box[4] = {0, 0, 0, 0};
+    std::vector<double> h_coords;      // [nnodes][2]
+    std::vector<int32_t> h_el2n;       // [nels][7] 0-based
+    std::vector<int32_t> h_boundary;   // 0-based
+    int32_t* d_el2n = nullptr;         // [7][nels]
+    double* d_geo = nullptr;           // [5][nels]
+    int32_t* d_off = nullptr;          // [ns+1]
+    int32_t* d_ent = nullptr;
+    int32_t* d_node_of = nullptr;      // nullable
+};
+
+struct fem_stokes {
+    fem_ctx* ctx = nullptr;
+    fem_mesh* mesh = nullptr;
+    fem_material mat;
+    fem_state_opts opts;
+    double* f[FEM_FIELD_COUNT] = {};
+    int64_t len[FEM_FIELD_COUNT] = {};
+    int32_t* d_phase_v = nullptr;   // cell [nels] or element-node [7][nels], 0-based
+    int32_t* d_phase_P = nullptr;   // cell [nels] or element-node [3][nels]
+    int32_t layout_v = FEM_PHASE_CELL, layout_P = FEM_PHASE_CELL;
+    double* d_pscale = nullptr;
+    double2* d_Re = nullptr;
+    double2* d_Jrow = nullptr;
+    double2* d_Jdiag = nullptr;
+    uint8_t* d_bcflag = nullptr;
+    double* d_bcval_x = nullptr;
+    double* d_bcval_y = nullptr;
+    int32_t* d_bc_stage = nullptr;
+    double* d_bc_stage_v = nullptr;
+    int64_t bc_stage_cap = 0;
+    double* d_ptab = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static int set_err(fem_ctx* c, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+#define HIPCHK(ctx, x)                                                                           \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess)                                                                    \
+            return set_err(ctx, FEM_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),   \
+                           __FILE__, __LINE__);                                                  \
+    } while (0)
+#define CHK(x)                 \
+    do {                       \
+        int r_ = (x);          \
+        if (r_ != FEM_OK) return r_; \
+    } while (0)
+
+static int dalloc(fem_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return set_err(c, FEM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return FEM_OK;
+}
+template <class T>
+static int dalloc_t(fem_ctx* c, T** p, size_t n) { return dalloc(c, (void**)p, n * sizeof(T)); }
+
+static int check_launch(fem_ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, FEM_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return FEM_OK;
+}
+
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+static inline unsigned grid_xcd(int64_t n, int block) {
+    unsigned g = grid_for(n, block);
+    return (g + 7u) & ~7u;
+}
+
+// ---------------------------------------------------------------------------
+// reference-element tables (shape_functions.jl:115-171, 231-239;
+// integration_points.jl:188-202)
+// ---------------------------------------------------------------------------
+static void t7_basis(double xi, double eta, double N[7], double dN[7][2]) {
+    const double L1 = 1 - xi - eta;
+    const double b = 27.0 * (1 - xi - eta) * xi * eta;
+    const double bx = 27.0 * eta * (1 - 2 * xi - eta);
+    const double by = 27.0 * xi * (1 - xi - 2 * eta);
+    N[0] = L1 * (2 * L1 - 1) + b / 9;
+    N[1] = xi * (2 * xi - 1) + b / 9;
+    N[2] = eta * (2 * eta - 1) + b / 9;
+    N[3] = 4 * (1 - xi - eta) * xi - 4 * b / 9;
+    N[4] = 4 * xi * eta - 4 * b / 9;
+    N[5] = 4 * eta * (1 - xi - eta) - 4 * b / 9;
+    N[6] = b;
+    dN[0][0] = 1 - 4 * L1 + bx / 9;  dN[0][1] = 1 - 4 * L1 + by / 9;
+    dN[1][0] = 4 * xi - 1 + bx / 9;  dN[1][1] = by / 9;
+    dN[2][0] = bx / 9;               dN[2][1] = 4 * eta - 1 + by / 9;
+    dN[3][0] = 4 * (1 - 2 * xi - eta) - 4 * bx / 9; dN[3][1] = -4 * xi - 4 * by / 9;
+    dN[4][0] = 4 * eta - 4 * bx / 9; dN[4][1] = 4 * xi - 4 * by / 9;
+    dN[5][0] = -4 * eta - 4 * bx / 9; dN[5][1] = 4 * (1 - xi - 2 * eta) - 4 * by / 9;
+    dN[6][0] = bx;                   dN[6][1] = by;
+}
+
+static Tables host_tables() {
+    Tables t;
+    const double a1 = 0.101286507323456, a2 = 0.470142064105115;
+    const double b1 = 1 - 2 * a1, b2 = 1 - 2 * a2;
+    const double w1 = 0.062969590272414, w2 = 0.066197076394253, wc = 0.1125;
+    const double X[7] = {a1, b1, a1, a2, b2, a2, 1.0 / 3};
+    const double Y[7] = {a1, a1, b1, a2, a2, b2, 1.0 / 3};
+    const double W[7] = {w1, w1, w1, w2, w2, w2, wc};
+    for (int q = 0; q < NQ; ++q) {
+        t7_basis(X[q], Y[q], t.N[q], t.dN[q]);
+        t.NP1[q][0] = 1 - X[q] - Y[q];
+        t.NP1[q][1] = X[q];
+        t.NP1[q][2] = Y[q];
+        t.w[q] = W[q];
+    }
+    for (int j = 0; j < NP; ++j)
+        for (int b = 0; b < NV; ++b)
+            for (int k = 0; k < 2; ++k) {
+                double s = 0;
+                for (int q = 0; q < NQ; ++q) s += t.w[q] * t.NP1[q][j] * t.dN[q][b][k];
+                t.TB[j][b][k] = s;
+            }
+    return t;
+}
+
+template <class T>
+static int fill_t(fem_ctx* c, T* p, T v, int64_t n) {
+    if (!c || n < 0) return FEM_EARG;
+    if (n == 0) return FEM_OK;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, p, v);
+    return check_launch(c, "k_fill");
+}
+
+// ---------------------------------------------------------------------------
+// context and buffers
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fem_ctx_create(int device, fem_ctx** out) {
+    if (!out) return FEM_EARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FEM_EHIP;
+    if (device < 0 || device >= n) return FEM_EARG;
+    fem_ctx* c = new fem_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return FEM_EHIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return FEM_EHIP; }
+    Tables t = host_tables();
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof t) != hipSuccess) { delete c; return FEM_EHIP; }
+    if (dalloc_t(c, &c->red_partial, RED_GRID * RED_MAXK) || dalloc_t(c, &c->red_out, RED_MAXK) ||
+        hipHostMalloc((void**)&c->h_red, RED_MAXK * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return FEM_EHIP;
+    }
+    *out = c;
+    return FEM_OK;
+}
+
+int fem_ctx_destroy(fem_ctx* c) {
+    if (!c) return FEM_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->tev) hipEventDestroy(e);
+    for (void* p : c->bufs) hipFree(p);
+    hipFree(c->red_partial);
+    hipFree(c->red_out);
+    hipHostFree(c->h_red);
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return FEM_OK;
+}
+
+int fem_last_error(const fem_ctx* c, char* buf, size_t len) {
+    if (!c || !buf || len == 0) return FEM_EARG;
+    snprintf(buf, len, "%s", c->err.c_str());
+    return FEM_OK;
+}
+
+int fem_sync(fem_ctx* c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+int fem_buf_alloc(fem_ctx* c, size_t bytes, void** dptr) {
+    if (!c || !dptr) return FEM_EARG;
+    CHK(dalloc(c, dptr, bytes));
+    HIPCHK(c, hipMemsetAsync(*dptr, 0, bytes ? bytes : 16, c->stream));
+    c->bufs.insert(*dptr);
+    return FEM_OK;
+}
+int fem_buf_free(fem_ctx* c, void* dptr) {
+    if (!c) return FEM_EARG;
+    if (!dptr) return FEM_OK;
+    if (!c->bufs.count(dptr)) return set_err(c, FEM_EARG, "fem_buf_free: pointer not owned by this context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->bufs.erase(dptr);
+    HIPCHK(c, hipFree(dptr));
+    return FEM_OK;
+}
+int fem_copy_h2d(fem_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return FEM_EARG;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+int fem_copy_d2h(fem_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return FEM_EARG;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+int fem_copy_d2d(fem_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return FEM_EARG;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return FEM_OK;
+}
+int fem_fill_f64(fem_ctx* c, double* p, double v, int64_t n) { return fill_t(c, p, v, n); }
+int fem_fill_f32(fem_ctx* c, float* p, float v, int64_t n) { return fill_t(c, p, v, n); }
+int fem_fill_i32(fem_ctx* c, int32_t* p, int32_t v, int64_t n) { return fill_t(c, p, v, n); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// mesh
+// ---------------------------------------------------------------------------
+static void gen_t7(int32_t nx, int32_t ny, double x0, double x1, double y0, double y1, std::vector<double>& coords,
+                   std::vector<int32_t>& el2n) {
+    // generate_coordinates(T7) mesh.jl:412-440; generate_element2node(T7)
+    // connectivity.jl:111-140 (alternating diagonals, centroids appended).
+    const int64_t sx = 2 * (int64_t)nx + 1, sy = 2 * (int64_t)ny + 1;
+    const int64_t nT6 = sx * sy, nels = 2LL * nx * ny;
+    coords.assign((size_t)(nT6 + nels) * 2, 0.0);
+    el2n.assign((size_t)nels * 7, 0);
+    auto lerp = [](int64_t j, int64_t d, double a, double b) {
+        const double t = (double)j / (double)d;
+        return (1 - t) * a + t * b;
+    };
+    for (int64_t iy = 0; iy < sy; ++iy)
+        for (int64_t ix = 0; ix < sx; ++ix) {
+            coords[(iy * sx + ix) * 2] = lerp(ix, sx - 1, x0, x1);
+            coords[(iy * sx + ix) * 2 + 1] = lerp(iy, sy - 1, y0, y1);
+        }
+    const double dx = (x1 - x0) / nx, dy = (y1 - y0) / ny;
+    int64_t k = nT6, e = 0;
+    auto node = [&](int64_t ix, int64_t iy) { return (int32_t)(iy * sx + ix); };
+    auto cent = [&](int64_t ex, int64_t ey, int a, int b) {
+        coords[k * 2] = x0 + (3 * ex + a) * dx / 3;
+        coords[k * 2 + 1] = y0 + (3 * ey + b) * dy / 3;
+        ++k;
+    };
+    for (int64_t ey = 0; ey < ny; ++ey)
+        for (int64_t ex = 0; ex < nx; ++ex) {
+            const int64_t ix = 2 * ex, iy = 2 * ey;
+            const int32_t BL = node(ix, iy), BR = node(ix + 2, iy), TR = node(ix + 2, iy + 2), TL = node(ix, iy + 2);
+            const int32_t bot = node(ix + 1, iy), rgt = node(ix + 2, iy + 1), top = node(ix + 1, iy + 2),
+                          lft = node(ix, iy + 1), ctr = node(ix + 1, iy + 1);
+            const int32_t cA = (int32_t)(nT6 + e), cB = (int32_t)(nT6 + e + 1);
+            int32_t* A = &el2n[e * 7];
+            int32_t* B = &el2n[(e + 1) * 7];
+            if ((ex + ey) % 2 == 0) {
+                const int32_t a[7] = {BL, BR, TR, bot, rgt, ctr, cA}, b[7] = {BL, TR, TL, ctr, top, lft, cB};
+                std::memcpy(A, a, sizeof a); std::memcpy(B, b, sizeof b);
+                cent(ex, ey, 2, 1); cent(ex, ey, 1, 2);
+            } else {
+                const int32_t a[7] = {BL, BR, TL, bot, ctr, lft, cA}, b[7] = {BR, TR, TL, rgt, top, ctr, cB};
+                std::memcpy(A, a, sizeof a); std::memcpy(B, b, sizeof b);
+                cent(ex, ey, 1, 1); cent(ex, ey, 2, 2);
+            }
+            e += 2;
+        }
+}
+
+static int mesh_finish(fem_ctx* c, fem_mesh* m) {
+    const int64_t ne = m->nels, nn = m->nnodes;
+    m->nnodesP = 3 * ne;
+    // bubble (local node 7) must be private to its element
+    std::vector<int32_t> val(nn, 0);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < 7; ++a) {
+            int32_t n = m->h_el2n[e * 7 + a];
+            if (n < 0 || n >= nn) return set_err(c, FEM_EARG, "el2n entry %d out of range at element %lld", n + 1, (long long)e + 1);
+            val[n]++;
+        }
+    std::vector<uint8_t> is_bubble(nn, 0);
+    for (int64_t e = 0; e < ne; ++e) {
+        int32_t b = m->h_el2n[e * 7 + 6];
+        if (val[b] != 1) return set_err(c, FEM_EARG, "T7 centroid node %d of element %lld is shared", b + 1, (long long)e + 1);
+        is_bubble[b] = 1;
+    }
+    // shared-node list + node->element-slot lists in ascending element order
+    // (generate_node2element, connectivity.jl:341-349)
+    std::vector<int32_t> node_of;
+    std::vector<int64_t> sid(nn, -1);
+    bool identity = true;
+    for (int64_t n = 0; n < nn; ++n)
+        if (!is_bubble[n] && val[n] > 0) {
+            sid[n] = (int64_t)node_of.size();
+            if (sid[n] != n) identity = false;
+            node_of.push_back((int32_t)n);
+        }
+    m->ns = (int64_t)node_of.size();
+    std::vector<int32_t> off(m->ns + 1, 0);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < NSH; ++a) off[sid[m->h_el2n[e * 7 + a]] + 1]++;
+    for (int64_t i = 0; i < m->ns; ++i) off[i + 1] += off[i];
+    std::vector<int32_t> ent(off[m->ns]);
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < NSH; ++a) ent[fill[sid[m->h_el2n[e * 7 + a]]]++] = (int32_t)(e * NSH + a);
+    if ((int64_t)ne * NSH > std::numeric_limits<int32_t>::max())
+        return set_err(c, FEM_EDIM, "mesh too large for 32-bit element-slot indices");
+    // SoA connectivity and per-element geometry (affine T7: J from the corners)
+    std::vector<int32_t> soa((size_t)7 * ne);
+    std::vector<double> geo((size_t)5 * ne);
+    for (int64_t e = 0; e < ne; ++e) {
+        for (int a = 0; a < 7; ++a) soa[a * ne + e] = m->h_el2n[e * 7 + a];
+        const double* p0 = &m->h_coords[2 * (int64_t)m->h_el2n[e * 7 + 0]];
+        const double* p1 = &m->h_coords[2 * (int64_t)m->h_el2n[e * 7 + 1]];
+        const double* p2 = &m->h_coords[2 * (int64_t)m->h_el2n[e * 7 + 2]];
+        const double J00 = p1[0] - p0[0], J01 = p2[0] - p0[0], J10 = p1[1] - p0[1], J11 = p2[1] - p0[1];
+        const double det = J00 * J11 - J01 * J10;
+        if (!(std::fabs(det) > 0)) return set_err(c, FEM_EARG, "degenerate element %lld", (long long)e + 1);
+        geo[0 * ne + e] = J11 / det;
+        geo[1 * ne + e] = -J01 / det;
+        geo[2 * ne + e] = -J10 / det;
+        geo[3 * ne + e] = J00 / det;
+        geo[4 * ne + e] = std::fabs(det);
+    }
+    CHK(dalloc_t(c, &m->d_el2n, soa.size()));
+    CHK(dalloc_t(c, &m->d_geo, geo.size()));
+    CHK(dalloc_t(c, &m->d_off, off.size()));
+    CHK(dalloc_t(c, &m->d_ent, ent.size()));
+    HIPCHK(c, hipMemcpy(m->d_el2n, soa.data(), soa.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_geo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    if (!identity) {
+        CHK(dalloc_t(c, &m->d_node_of, node_of.size()));
+        HIPCHK(c, hipMemcpy(m->d_node_of, node_of.data(), node_of.size() * 4, hipMemcpyHostToDevice));
+    }
+    return FEM_OK;
+}
+
+extern "C" {
+
+int fem_mesh_t7_create(fem_ctx* c, int32_t nx, int32_t ny, double x0, double x1, double y0, double y1,
+                       fem_mesh** out) {
+    if (!c || !out) return FEM_EARG;
+    *out = nullptr;
+    if (nx < 1 || ny < 1) return set_err(c, FEM_EARG, "element counts must be positive, got (%d, %d)", nx, ny);
+    if (!(x1 > x0) || !(y1 > y0)) return set_err(c, FEM_EARG, "empty domain");
+    HIPCHK(c, hipSetDevice(c->device));
+    fem_mesh* m = new fem_mesh();
+    m->ctx = c;
+    m->structured = true;
+    m->box[0] = x0; m->box[1] = x1; m->box[2] = y0; m->box[3] = y1;
+    gen_t7(nx, ny, x0, x1, y0, y1, m->h_coords, m->h_el2n);
+    m->nnodes = (int64_t)m->h_coords.size() / 2;
+    m->nels = (int64_t)m->h_el2n.size() / 7;
+    // Γnodes: nodes on the box boundary (mesh.jl:79-80); only T6 grid nodes
+    const int64_t sx = 2 * (int64_t)nx + 1, sy = 2 * (int64_t)ny + 1;
+    for (int64_t iy = 0; iy < sy; ++iy)
+        for (int64_t ix = 0; ix < sx; ++ix)
+            if (ix == 0 || iy == 0 || ix == sx - 1 || iy == sy - 1) m->h_boundary.push_back((int32_t)(iy * sx + ix));
+    int r = mesh_finish(c, m);
+    if (r != FEM_OK) { fem_mesh_destroy(m); return r; }
+    *out = m;
+    return FEM_OK;
+}
+
+int fem_mesh_t7_upload(fem_ctx* c, const double* host_coords, int64_t nnodes, const int32_t* host_el2n_1based,
+                       int64_t nels, fem_mesh** out) {
+    if (!c || !out || !host_coords || !host_el2n_1based) return FEM_EARG;
+    *out = nullptr;
+    if (nnodes <= 0 || nels <= 0) return set_err(c, FEM_EARG, "empty mesh");
+    HIPCHK(c, hipSetDevice(c->device));
+    fem_mesh* m = new fem_mesh();
+    m->ctx = c;
+    m->nnodes = nnodes;
+    m->nels = nels;
+    m->h_coords.assign(host_coords, host_coords + 2 * nnodes);
+    m->h_el2n.resize((size_t)nels * 7);
+    for (int64_t i = 0; i < nels * 7; ++i) m->h_el2n[i] = host_el2n_1based[i] - 1;
+    int r = mesh_finish(c, m);
+    if (r != FEM_OK) { fem_mesh_destroy(m); return r; }
+    *out = m;
+    return FEM_OK;
+}
+
+int fem_mesh_get_info(const fem_mesh* m, fem_mesh_info* o) {
+    if (!m || !o) return FEM_EARG;
+    o->dim = 2; o->nv_local = 7; o->np_local = 3; o->nq = 7;
+    o->nnodes = m->nnodes; o->nels = m->nels; o->nnodesP = m->nnodesP; o->n_shared = m->ns;
+    o->n_boundary = (int64_t)m->h_boundary.size();
+    return FEM_OK;
+}
+int fem_mesh_get_coords(const fem_mesh* m, double* out) {
+    if (!m || !out) return FEM_EARG;
+    std::memcpy(out, m->h_coords.data(), m->h_coords.size() * sizeof(double));
+    return FEM_OK;
+}
+int fem_mesh_get_el2n(const fem_mesh* m, int32_t* out) {
+    if (!m || !out) return FEM_EARG;
+    for (size_t i = 0; i < m->h_el2n.size(); ++i) out[i] = m->h_el2n[i] + 1;
+    return FEM_OK;
+}
+int fem_mesh_get_boundary_nodes(const fem_mesh* m, int32_t* out) {
+    if (!m || !out) return FEM_EARG;
+    for (size_t i = 0; i < m->h_boundary.size(); ++i) out[i] = m->h_boundary[i] + 1;
+    return FEM_OK;
+}
+int fem_mesh_destroy(fem_mesh* m) {
+    if (!m) return FEM_OK;
+    hipFree(m->d_el2n); hipFree(m->d_geo); hipFree(m->d_off); hipFree(m->d_ent); hipFree(m->d_node_of);
+    delete m;
+    return FEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fem_state_opts* opts,
+                      fem_stokes** out) {
+    if (!c || !m || !mat || !out) return FEM_EARG;
+    *out = nullptr;
+    if (mat->nphases < 1 || mat->nphases > FEM_MAX_PHASES)
+        return set_err(c, FEM_EARG, "nphases must be in 1..%d", FEM_MAX_PHASES);
+    if (mat->ndim != 2) return set_err(c, FEM_EDIM, "material gravity must have 2 components");
+    HIPCHK(c, hipSetDevice(c->device));
+    fem_stokes* s = new fem_stokes();
+    s->ctx = c;
+    s->mesh = m;
+    s->mat = *mat;
+    s->opts = opts ? *opts : fem_state_opts{0.98, 0.98, 0.9, 1e-6};
+    const int64_t nv = m->nnodes, np = m->nnodesP, nip = m->nels * NQ;
+    for (int k = 0; k < FEM_FIELD_COUNT; ++k) {
+        int64_t n = k <= FEM_FIELD_PC_V_Y ? nv : (k <= FEM_FIELD_TAU_OLD_XY ? nip : np);
+        s->len[k] = n;
+        int r = dalloc_t(c, &s->f[k], n);
+        if (r != FEM_OK) { fem_stokes_destroy(s); return r; }
+        HIPCHK(c, hipMemset(s->f[k], 0, n * sizeof(double)));
+    }
+    int r = FEM_OK;
+    if ((r = dalloc_t(c, &s->d_phase_v, 7 * m->nels)) || (r = dalloc_t(c, &s->d_phase_P, 3 * m->nels)) ||
+        (r = dalloc_t(c, &s->d_pscale, np)) || (r = dalloc_t(c, &s->d_Re, NSH * m->nels)) ||
+        (r = dalloc_t(c, &s->d_Jrow, NSH * m->nels)) || (r = dalloc_t(c, &s->d_Jdiag, NSH * m->nels)) ||
+        (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
+        (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH))) {
+        fem_stokes_destroy(s);
+        return r;
+    }
+    HIPCHK(c, hipMemset(s->d_phase_v, 0, 7 * m->nels * 4));
+    HIPCHK(c, hipMemset(s->d_phase_P, 0, 3 * m->nels * 4));
+    HIPCHK(c, hipMemset(s->d_bcflag, 0, nv));
+    HIPCHK(c, hipMemset(s->d_pscale, 0, np * 8));
+    *out = s;
+    return FEM_OK;
+}
+
+int fem_stokes_destroy(fem_stokes* s) {
+    if (!s) return FEM_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    for (int k = 0; k < FEM_FIELD_COUNT; ++k) hipFree(s->f[k]);
+    hipFree(s->d_phase_v); hipFree(s->d_phase_P); hipFree(s->d_pscale); hipFree(s->d_Re);
+    hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
+    hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
+    delete s;
+    return FEM_OK;
+}
+
+int fem_stokes_field(fem_stokes* s, int32_t field, void** dptr, int64_t* len, int32_t* dtype) {
+    if (!s || field < 0 || field >= FEM_FIELD_COUNT) return FEM_EARG;
+    if (dptr) *dptr = s->f[field];
+    if (len) *len = s->len[field];
+    if (dtype) *dtype = 0;
+    return FEM_OK;
+}
+
+int fem_stokes_set_phases(fem_stokes* s, int32_t which, int32_t layout, const int32_t* h, int64_t n) {
+    if (!s || !h || which < 0 || which > 1) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    const fem_mesh* m = s->mesh;
+    const int64_t ne = m->nels;
+    const int nloc = which == 0 ? 7 : 3;
+    std::vector<int32_t> loc((size_t)nloc * ne);   // [nloc][ne] 0-based
+    if (layout == FEM_PHASE_CELL) {
+        if (n != ne) return set_err(c, FEM_EDIM, "cell phases must have nels = %lld entries", (long long)ne);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int a = 0; a < nloc; ++a) loc[a * ne + e] = h[e] - 1;
+    } else if (layout == FEM_PHASE_ELEMENT) {
+        if (n != (int64_t)nloc * ne)
+            return set_err(c, FEM_EDIM, "element phases must be %d x nels", nloc);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int a = 0; a < nloc; ++a) loc[a * ne + e] = h[e * nloc + a] - 1;
+    } else if (layout == FEM_PHASE_NODAL) {
+        const int64_t need = which == 0 ? m->nnodes : m->nnodesP;
+        if (n != need) return set_err(c, FEM_EDIM, "nodal phases must have %lld entries", (long long)need);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int a = 0; a < nloc; ++a)
+                loc[a * ne + e] = (which == 0 ? h[m->h_el2n[e * 7 + a]] : h[e * 3 + a]) - 1;
+    } else {
+        return set_err(c, FEM_EARG, "unknown phase layout %d", layout);
+    }
+    for (int32_t p : loc)
+        if (p < 0 || p >= s->mat.nphases) return set_err(c, FEM_EARG, "phase id %d out of range 1..%d", p + 1, s->mat.nphases);
+    // element-constant phases use the cell fast path
+    bool cell = true;
+    for (int64_t e = 0; e < ne && cell; ++e)
+        for (int a = 1; a < nloc; ++a)
+            if (loc[a * ne + e] != loc[e]) { cell = false; break; }
+    int32_t* d = which == 0 ? s->d_phase_v : s->d_phase_P;
+    HIPCHK(c, hipMemcpy(d, loc.data(), (size_t)nloc * ne * 4, hipMemcpyHostToDevice));
+    (which == 0 ? s->layout_v : s->layout_P) = cell ? FEM_PHASE_CELL : FEM_PHASE_ELEMENT;
+    return FEM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// per-phase table, boundary conditions, kernel launches
+// ---------------------------------------------------------------------------
+static int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
+                       bool* thermal, bool* eos_p, bool* elastic) {
+    fem_ctx* c = s->ctx;
+    const fem_material& m = s->mat;
+    if (dp && dp->nphases != m.nphases)
+        return set_err(c, FEM_EDIM, "DruckerPrager has %d phases, material has %d", dp->nphases, m.nphases);
+    std::vector<double> t((size_t)PC_NCOL * MAXPH, 0.0);
+    *bulk_inf = true; *thermal = false; *eos_p = false; *elastic = false;
+    for (int p = 0; p < m.nphases; ++p) {
+        const double invG = 1.0 / m.G[p];
+        const double bulk = finite_K ? m.K[p] : m.etab[p];
+        t[PC_ETA * MAXPH + p] = m.eta[p];
+        t[PC_INVG * MAXPH + p] = invG;
+        t[PC_RHO0 * MAXPH + p] = m.rho0[p];
+        t[PC_ALPHA * MAXPH + p] = m.alpha[p];
+        t[PC_BETA * MAXPH + p] = 1.0 / m.K[p];
+        t[PC_BULK * MAXPH + p] = bulk;
+        t[PC_ETAVE * MAXPH + p] = 1.0 / (1.0 / m.eta[p] + invG / dt);
+        t[PC_INV2GDT * MAXPH + p] = invG / (2 * dt);
+        if (dp) {
+            t[PC_COSPHI * MAXPH + p] = dp->cos_phi[p];
+            t[PC_SINPHI * MAXPH + p] = dp->sin_phi[p];
+            t[PC_SINPSI * MAXPH + p] = dp->sin_psi[p];
+            t[PC_C * MAXPH + p] = dp->C[p];
+            t[PC_EREG * MAXPH + p] = dp->eta_reg[p];
+            t[PC_KB * MAXPH + p] = dp->Kb[p];
+        }
+        if (!std::isinf(bulk)) *bulk_inf = false;
+        if (m.alpha[p] != 0) *thermal = true;
+        if (!std::isinf(m.K[p])) *eos_p = true;
+        if (invG != 0) *elastic = true;
+    }
+    HIPCHK(c, hipMemcpyAsync(s->d_ptab, t.data(), t.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+__global__ void k_set_bc(int64_t n, const int32_t* dofs, const double* vals, uint8_t bit, uint8_t* flag,
+                         double* bcval) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        flag[dofs[i]] |= bit;
+        bcval[dofs[i]] = vals[i];
+    }
+}
+
+static int upload_bcs(fem_stokes* s, const fem_bc bcs[2]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, nv, c->stream));
+    for (int comp = 0; comp < 2; ++comp) {
+        const fem_bc& b = bcs[comp];
+        if (b.n == 0) continue;
+        if (b.n < 0 || !b.host_dofs_1based || !b.host_vals) return set_err(c, FEM_EARG, "invalid boundary condition");
+        // a serial v[dofs[i]] = vals[i] lets a later duplicate win; keep that
+        std::vector<int64_t> last(nv, -1);
+        for (int64_t i = 0; i < b.n; ++i) {
+            const int32_t dof = b.host_dofs_1based[i] - 1;
+            if (dof < 0 || dof >= nv) return set_err(c, FEM_EARG, "Dirichlet DoF %d out of range", dof + 1);
+            last[dof] = i;
+        }
+        std::vector<int32_t> d;
+        std::vector<double> v;
+        for (int64_t i = 0; i < b.n; ++i) {
+            const int32_t dof = b.host_dofs_1based[i] - 1;
+            if (last[dof] == i) { d.push_back(dof); v.push_back(b.host_vals[i]); }
+        }
+        const int64_t nb = (int64_t)d.size();
+        if (nb > s->bc_stage_cap) {
+            hipFree(s->d_bc_stage);
+            hipFree(s->d_bc_stage_v);
+            s->d_bc_stage = nullptr;
+            s->d_bc_stage_v = nullptr;
+            CHK(dalloc_t(c, &s->d_bc_stage, nb));
+            CHK(dalloc_t(c, &s->d_bc_stage_v, nb));
+            s->bc_stage_cap = nb;
+        }
+        HIPCHK(c, hipMemcpyAsync(s->d_bc_stage, d.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s->d_bc_stage_v, v.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_set_bc, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, nb, s->d_bc_stage,
+                           s->d_bc_stage_v, (uint8_t)(1 << comp), s->d_bcflag, comp == 0 ? s->d_bcval_x : s->d_bcval_y);
+        CHK(check_launch(c, "k_set_bc"));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return FEM_OK;
+}
+
+struct LaunchOpts {
+    bool plastic = false, elastic = false, bulk_inf = true, thermal = false, eos_p = false;
+    bool use_pnum = false, do_update = false;
+    bool store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false, store_tau = false;
+    double dt = 1.0;
+    double alpha[2] = {0, 0}, beta[2] = {0, 0};
+};
+
+constexpr int EBLOCK = 128;
+constexpr int GBLOCK = 256;
+
+template <bool EL, bool PL, bool NP_>
+static void launch_elem_t(fem_stokes* s, const ElemArgs<double>& a) {
+    hipLaunchKernelGGL((k_t7_element<double, EL, PL, NP_, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)),
+                       dim3(EBLOCK), 0, s->ctx->stream, a);
+}
+
+static int launch_element(fem_stokes* s, const LaunchOpts& o) {
+    fem_mesh* m = s->mesh;
+    ElemArgs<double> a;
+    std::memset(&a, 0, sizeof a);
+    a.nels = m->nels;
+    a.el2n = m->d_el2n;
+    a.geo = m->d_geo;
+    a.vx = s->f[FEM_FIELD_V_X];
+    a.vy = s->f[FEM_FIELD_V_Y];
+    a.P = s->f[FEM_FIELD_P];
+    a.Pold = o.bulk_inf ? nullptr : s->f[FEM_FIELD_P0];
+    a.pscale = s->d_pscale;
+    a.T = s->f[FEM_FIELD_T];
+    a.T0 = s->f[FEM_FIELD_T0];
+    a.Q = s->f[FEM_FIELD_Q];
+    a.Pf = s->f[FEM_FIELD_PF];
+    a.tau_old0 = s->f[FEM_FIELD_TAU_OLD_XX];
+    a.tau_old1 = s->f[FEM_FIELD_TAU_OLD_YY];
+    a.tau_old2 = s->f[FEM_FIELD_TAU_OLD_XY];
+    if (o.store_tau) {
+        a.tau_out0 = s->f[FEM_FIELD_TAU_XX];
+        a.tau_out1 = s->f[FEM_FIELD_TAU_YY];
+        a.tau_out2 = s->f[FEM_FIELD_TAU_XY];
+    }
+    a.phase_v = s->d_phase_v;
+    a.phase_P = s->d_phase_P;
+    a.ptab = s->d_ptab;
+    a.nphases = s->mat.nphases;
+    a.bulk_all_inf = o.bulk_inf;
+    a.thermal = o.thermal;
+    a.eos_p = o.eos_p;
+    a.gx = s->mat.g[0];
+    a.gy = s->mat.g[1];
+    a.Tref = s->mat.Tref;
+    a.dt = o.dt;
+    a.RP_out = o.store_RP ? s->f[FEM_FIELD_RP] : nullptr;
+    a.Pnum_out = o.store_Pnum ? s->f[FEM_FIELD_PNUM] : nullptr;
+    a.Re = s->d_Re;
+    a.rate_x = s->f[FEM_FIELD_DVDTAU_X];
+    a.rate_y = s->f[FEM_FIELD_DVDTAU_Y];
+    a.PCx = s->f[FEM_FIELD_PC_V_X];
+    a.PCy = s->f[FEM_FIELD_PC_V_Y];
+    a.bcflag = s->d_bcflag;
+    a.bcval_x = s->d_bcval_x;
+    a.bcval_y = s->d_bcval_y;
+    if (o.store_Rv) { a.Rv_out_x = s->f[FEM_FIELD_RV_X]; a.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
+    if (o.store_Rv0) { a.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; a.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
+    a.alpha_x = o.alpha[0]; a.alpha_y = o.alpha[1];
+    a.beta_x = o.beta[0]; a.beta_y = o.beta[1];
+    a.use_pnum = o.use_pnum;
+    a.do_update = o.do_update;
+    // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
+    const bool np_ = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
+    const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
+    switch (key) {
+        case 0: launch_elem_t<false, false, false>(s, a); break;
+        case 1: launch_elem_t<false, false, true>(s, a); break;
+        case 2: launch_elem_t<false, true, false>(s, a); break;
+        case 3: launch_elem_t<false, true, true>(s, a); break;
+        case 4: launch_elem_t<true, false, false>(s, a); break;
+        case 5: launch_elem_t<true, false, true>(s, a); break;
+        case 6: launch_elem_t<true, true, false>(s, a); break;
+        case 7: launch_elem_t<true, true, true>(s, a); break;
+    }
+    return check_launch(s->ctx, "k_t7_element");
+}
+
+static int launch_gather(fem_stokes* s, const LaunchOpts& o) {
+    fem_mesh* m = s->mesh;
+    GatherArgs<double> g;
+    std::memset(&g, 0, sizeof g);
+    g.ns = m->ns;
+    g.node_of = m->d_node_of;
+    g.off = m->d_off;
+    g.ent = m->d_ent;
+    g.Re = s->d_Re;
+    g.vx = s->f[FEM_FIELD_V_X];
+    g.vy = s->f[FEM_FIELD_V_Y];
+    g.rate_x = s->f[FEM_FIELD_DVDTAU_X];
+    g.rate_y = s->f[FEM_FIELD_DVDTAU_Y];
+    g.PCx = s->f[FEM_FIELD_PC_V_X];
+    g.PCy = s->f[FEM_FIELD_PC_V_Y];
+    g.bcflag = s->d_bcflag;
+    g.bcval_x = s->d_bcval_x;
+    g.bcval_y = s->d_bcval_y;
+    if (o.store_Rv) { g.Rv_out_x = s->f[FEM_FIELD_RV_X]; g.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
+    if (o.store_Rv0) { g.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; g.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
+    g.alpha_x = o.alpha[0]; g.alpha_y = o.alpha[1];
+    g.beta_x = o.beta[0]; g.beta_y = o.beta[1];
+    g.do_update = o.do_update;
+    hipLaunchKernelGGL((k_gather_update<double, GBLOCK>), dim3(grid_xcd(m->ns, GBLOCK)), dim3(GBLOCK), 0,
+                       s->ctx->stream, g);
+    return check_launch(s->ctx, "k_gather_update");
+}
+
+template <bool EL, bool PL, bool NP_>
+static void launch_jac_t(fem_stokes* s, const JacArgs<double>& a) {
+    hipLaunchKernelGGL((k_t7_jacobian<double, EL, PL, NP_, 64>), dim3(grid_for(s->mesh->nels, 64)), dim3(64), 0,
+                       s->ctx->stream, a);
+}
+
+static int launch_jacobian(fem_stokes* s, const LaunchOpts& o) {
+    fem_mesh* m = s->mesh;
+    JacArgs<double> a;
+    std::memset(&a, 0, sizeof a);
+    a.nels = m->nels;
+    a.el2n = m->d_el2n;
+    a.geo = m->d_geo;
+    a.vx = s->f[FEM_FIELD_V_X];
+    a.vy = s->f[FEM_FIELD_V_Y];
+    a.P = s->f[FEM_FIELD_P];
+    a.pscale = s->d_pscale;
+    a.tau_old0 = s->f[FEM_FIELD_TAU_OLD_XX];
+    a.tau_old1 = s->f[FEM_FIELD_TAU_OLD_YY];
+    a.tau_old2 = s->f[FEM_FIELD_TAU_OLD_XY];
+    a.Pf = s->f[FEM_FIELD_PF];
+    a.phase_v = s->d_phase_v;
+    a.ptab = s->d_ptab;
+    a.nphases = s->mat.nphases;
+    a.dt = o.dt;
+    a.Jrow = s->d_Jrow;
+    a.Jdiag = s->d_Jdiag;
+    a.dRdv_x = s->f[FEM_FIELD_DRVDV_X];
+    a.dRdv_y = s->f[FEM_FIELD_DRVDV_Y];
+    a.PC_x = s->f[FEM_FIELD_PC_V_X];
+    a.PC_y = s->f[FEM_FIELD_PC_V_Y];
+    const bool np_ = s->layout_v == FEM_PHASE_ELEMENT;
+    const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
+    switch (key) {
+        case 0: launch_jac_t<false, false, false>(s, a); break;
+        case 1: launch_jac_t<false, false, true>(s, a); break;
+        case 2: launch_jac_t<false, true, false>(s, a); break;
+        case 3: launch_jac_t<false, true, true>(s, a); break;
+        case 4: launch_jac_t<true, false, false>(s, a); break;
+        case 5: launch_jac_t<true, false, true>(s, a); break;
+        case 6: launch_jac_t<true, true, false>(s, a); break;
+        case 7: launch_jac_t<true, true, true>(s, a); break;
+    }
+    CHK(check_launch(s->ctx, "k_t7_jacobian"));
+    GatherJacArgs<double> g;
+    g.ns = m->ns;
+    g.node_of = m->d_node_of;
+    g.off = m->d_off;
+    g.ent = m->d_ent;
+    g.Jrow = s->d_Jrow;
+    g.Jdiag = s->d_Jdiag;
+    g.dRdv_x = s->f[FEM_FIELD_DRVDV_X];
+    g.dRdv_y = s->f[FEM_FIELD_DRVDV_Y];
+    g.PC_x = s->f[FEM_FIELD_PC_V_X];
+    g.PC_y = s->f[FEM_FIELD_PC_V_Y];
+    hipLaunchKernelGGL((k_gather_jac<double, GBLOCK>), dim3(grid_for(m->ns, GBLOCK)), dim3(GBLOCK), 0,
+                       s->ctx->stream, g);
+    return check_launch(s->ctx, "k_gather_jac");
+}
+
+static int scaling_impl(fem_stokes* s, double gamma_fact, double dt) {
+    fem_mesh* m = s->mesh;
+    fem_ctx* c = s->ctx;
+    if (s->layout_v == FEM_PHASE_ELEMENT)
+        hipLaunchKernelGGL((k_pressure_scaling<double, true, 256>), dim3(grid_for(m->nels, 256)), dim3(256), 0,
+                           c->stream, m->nels, (const double*)m->d_geo, (const int32_t*)s->d_phase_v,
+                           (const double*)s->d_ptab, s->mat.nphases, gamma_fact, dt, s->f[FEM_FIELD_M_P],
+                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale);
+    else
+        hipLaunchKernelGGL((k_pressure_scaling<double, false, 256>), dim3(grid_for(m->nels, 256)), dim3(256), 0,
+                           c->stream, m->nels, (const double*)m->d_geo, (const int32_t*)s->d_phase_v,
+                           (const double*)s->d_ptab, s->mat.nphases, gamma_fact, dt, s->f[FEM_FIELD_M_P],
+                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale);
+    return check_launch(c, "k_pressure_scaling");
+}
+
+__global__ void k_pscale(int64_t n, const double* g, const double* M, double* ps) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ps[i] = M[i] == 0.0 ? 0.0 : g[i] / M[i];
+}
+
+// reductions -> host scalars (synchronises)
+static int reduce_check(fem_stokes* s, const double al[2], double out[6]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(nv, RED_BLOCK));
+    hipLaunchKernelGGL(k_reduce_check<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv,
+                       (const double*)s->f[FEM_FIELD_RV_X], (const double*)s->f[FEM_FIELD_RV_Y],
+                       (const double*)s->f[FEM_FIELD_RV0_X], (const double*)s->f[FEM_FIELD_RV0_Y],
+                       (const double*)s->f[FEM_FIELD_DVDTAU_X], (const double*)s->f[FEM_FIELD_DVDTAU_Y],
+                       (const double*)s->f[FEM_FIELD_PC_V_X], (const double*)s->f[FEM_FIELD_PC_V_Y], al[0], al[1],
+                       c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 6, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_reduce_check"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_red, 6 * 8);
+    return FEM_OK;
+}
+
+static int reduce_outer(fem_stokes* s, double out[3]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, np), RED_BLOCK));
+    hipLaunchKernelGGL(k_reduce_outer<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np,
+                       (const double*)s->f[FEM_FIELD_RV_X], (const double*)s->f[FEM_FIELD_RV_Y],
+                       (const double*)s->f[FEM_FIELD_RP], (const double*)s->f[FEM_FIELD_M_P], c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_reduce_outer"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_red, 3 * 8);
+    return FEM_OK;
+}
+
+static int reduce_lmax(fem_stokes* s, double out[2]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(nv, RED_BLOCK));
+    hipLaunchKernelGGL(k_reduce_lmax<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv,
+                       (const double*)s->f[FEM_FIELD_DRVDV_X], (const double*)s->f[FEM_FIELD_DRVDV_Y],
+                       (const double*)s->f[FEM_FIELD_PC_V_X], (const double*)s->f[FEM_FIELD_PC_V_Y], c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 1,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_reduce_lmax"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = c->h_red[0];
+    out[1] = c->h_red[1];
+    if (c->h_red[2] > 0) out[0] = out[1] = std::numeric_limits<double>::quiet_NaN();
+    return FEM_OK;
+}
+
+static int checked_lmax(fem_stokes* s, double lmax[2]) {
+    CHK(reduce_lmax(s, lmax));
+    const char* labels[2] = {"stokes vx", "stokes vy"};
+    for (int k = 0; k < 2; ++k)
+        if (!(std::isfinite(lmax[k]) && lmax[k] > 0))
+            return set_err(s->ctx, FEM_EPRECOND,
+                           "%s preconditioner produced invalid λmax = %g; check for zero or non-finite preconditioner entries",
+                           labels[k], lmax[k]);
+    return FEM_OK;
+}
+
+static double julia_eps(double x) {
+    const double a = std::fabs(x);
+    if (!std::isfinite(a)) return std::numeric_limits<double>::quiet_NaN();
+    return std::nextafter(a, std::numeric_limits<double>::infinity()) - a;
+}
+
+static void cheb(double dtau, double lmin, double c_fact, double* a, double* b) {
+    // _stokes_cheb, DR.jl:39-42
+    const double c = 2 * std::sqrt(lmin) * c_fact;
+    *a = 2 * dtau * dtau / (2 + c * dtau);
+    *b = (2 - c * dtau) / (2 + c * dtau);
+}
+
+static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, LaunchOpts& o,
+                   bool use_tau_old) {
+    bool bulk_inf, thermal, eos_p, elastic;
+    CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
+    o.plastic = dp != nullptr;
+    o.elastic = elastic && use_tau_old;
+    o.bulk_inf = bulk_inf;
+    o.thermal = thermal;
+    o.eos_p = eos_p;
+    o.dt = dt;
+    if (!(dt > 0) || !std::isfinite(dt)) return set_err(s->ctx, FEM_EARG, "dt must be positive and finite");
+    return FEM_OK;
+}
+
+extern "C" {
+
+int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt, int32_t finite_K) {
+    if (!s) return FEM_EARG;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, finite_K, o, true));
+    CHK(scaling_impl(s, pressure_factor, dt));
+    return fem_sync(s->ctx);
+}
+
+int fem_stokes_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t with_pnum, int32_t finite_K) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, finite_K, o, true));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    o.use_pnum = with_pnum != 0;
+    o.store_RP = true;
+    o.store_Pnum = with_pnum != 0;
+    o.store_Rv = true;
+    CHK(launch_element(s, o));
+    CHK(launch_gather(s, o));
+    return fem_sync(c);
+}
+
+int fem_stokes_jacobian_diagnostics(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t finite_K) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, finite_K, o, true));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    CHK(launch_jacobian(s, o));
+    return fem_sync(c);
+}
+
+int fem_stokes_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, 0, o, true));
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    o.store_tau = true;
+    o.store_Rv = true;
+    CHK(launch_element(s, o));
+    CHK(launch_gather(s, o));
+    return fem_sync(c);
+}
+
+int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms) {
+    if (!s || n <= 0 || !avg_ms) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, 0, o, true));
+    o.use_pnum = true;
+    o.do_update = true;
+    o.alpha[0] = o.alpha[1] = 0.0;
+    o.beta[0] = o.beta[1] = 1.0;
+    CHK(launch_element(s, o));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < n; ++i) CHK(launch_element(s, o));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / n;
+    return FEM_OK;
+}
+
+// solve! + _solve_stokes_dyrel! (DR.jl:302-337, 348-802)
+int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
+                     fem_solve_stats* st) {
+    if (!s || !bcs || !p || !st) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    std::memset(st, 0, sizeof *st);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
+    if (p->precision != 0) return set_err(c, FEM_EARG, "precision %d not available in this build", p->precision);
+    const auto t0 = std::chrono::steady_clock::now();
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
+    const double dt = p->dt;
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
+    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
+    CHK(upload_bcs(s, bcs));
+    if (!p->skip_scaling) CHK(scaling_impl(s, p->pressure_factor, dt));
+    else
+        hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                           (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_X], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_Y], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_X], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_Y], 0, nv * 8, c->stream));
+
+    CHK(launch_jacobian(s, o));
+    double lmax[2];
+    CHK(checked_lmax(s, lmax));
+    double lmax_g = std::max(lmax[0], lmax[1]);
+    int64_t jac = 1;
+    double al[2], be[2];
+    for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
+
+    const double eps_tol = p->tolerance;
+    double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
+    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
+    int64_t iter = 0, itPH_done = 0, launches = 0;
+    bool converged = false;
+    double rel_drop = p->rel_drop0;
+    int rc = FEM_OK;
+    KernelTimer timer(c, p->time_kernels != 0);
+
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        // outer residuals without Pnum (DR.jl:535-554)
+        LaunchOpts oo = o;
+        oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
+        CHK(launch_element(s, oo));
+        CHK(launch_gather(s, oo));
+        double r3[3];
+        CHK(reduce_outer(s, r3));
+        timer.collect();
+        err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
+        err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v / err_v0, err_P / err_P0);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
+            break;
+        }
+        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
+                   (long long)itPH, (long long)iter, err, err_abs, err_rel, err_v, err_v / err_v0, err_P, err_P / err_P0);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        bool stop_total = false;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT;
+            ++iter;
+            const bool do_check = iter % ncheck == 0;
+            // an iteration after which the loop may end stores what the end needs
+            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
+            if (do_check && ncheck == 1) {
+                HIPCHK(c, hipMemcpyAsync(s->f[FEM_FIELD_RV0_X], s->f[FEM_FIELD_RV_X], nv * 8, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(s->f[FEM_FIELD_RV0_Y], s->f[FEM_FIELD_RV_Y], nv * 8, hipMemcpyDeviceToDevice, c->stream));
+            }
+            LaunchOpts oi = o;
+            oi.use_pnum = true;
+            oi.do_update = true;
+            oi.store_RP = last_possible;
+            oi.store_Pnum = last_possible;
+            oi.store_Rv = last_possible;
+            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
+            oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
+            oi.beta[0] = be[0]; oi.beta[1] = be[1];
+            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
+                       e2 = timer.on ? timer.ev() : nullptr;
+            timer.mark(e0);
+            CHK(launch_element(s, oi));
+            timer.mark(e1);
+            CHK(launch_gather(s, oi));
+            timer.mark(e2);
+            ++launches;
+            if (do_check) {
+                double r6[6];
+                CHK(reduce_check(s, al, r6));
+                timer.collect();
+                const double evi = std::max(std::sqrt(r6[0]), std::sqrt(r6[1])) / (2 * std::sqrt((double)nv));
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (p->verbose > 1) printf("  it = %lld, iter = %lld, err = %.3e\n", (long long)itPT, (long long)iter, err);
+                double lmin[2];
+                for (int k = 0; k < 2; ++k) {
+                    const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
+                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+                }
+                if (!freeze) {
+                    ++jac;
+                    CHK(launch_jacobian(s, o));
+                    rc = checked_lmax(s, lmax);
+                    if (rc != FEM_OK) break;
+                    lmax_g = std::max(lmax[0], lmax[1]);
+                }
+                for (int k = 0; k < 2; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+            }
+            if (p->verbose && itPT == p->iter_max) printf("  inner: max iters (%lld) reached at PH=%lld\n", (long long)p->iter_max, (long long)itPH);
+            if (iter > p->max_iterations) { stop_total = true; break; }
+        }
+        if (rc != FEM_OK) break;
+        hipLaunchKernelGGL(k_pressure_update<double>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                           s->f[FEM_FIELD_P], (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_RP],
+                           (const double*)s->f[FEM_FIELD_M_P]);
+        CHK(check_launch(c, "k_pressure_update"));
+        if (stop_total || iter > p->max_iterations) break;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timer.collect();
+    st->kernel_ms_element = timer.ms_elem;
+    st->kernel_ms_gather = timer.ms_gather;
+    st->timed_launches = timer.n;
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_abs = err_abs;
+    st->err_rel = err_rel;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged;
+    st->reached_total_iter = iter > p->max_iterations;
+    st->lambda_max = std::max(lmax[0], lmax[1]);
+    st->lambda_max_gershgorin = lmax_g;
+    st->lambda_max_iterations = 0;
+    st->jacobian_assemblies = jac;
+    st->element_launches = launches;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != FEM_OK) return rc;
+    if (!converged && p->throw_on_failure)
+        return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                       (long long)iter, err_abs, p->tolerance);
+    return FEM_OK;
+}
+
+}  // extern "C"

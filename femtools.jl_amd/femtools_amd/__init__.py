@@ -1,0 +1,558 @@
+"""femtools_amd -- host-side mirror of FEMTools.jl's DR Stokes API over the
+C ABI of libfemdr.so (include/femdr.h), for MI355X (gfx950).
+
+Names follow the reference: ``StokesMaterial``, ``DruckerPrager``,
+``StokesDR`` with struct-of-arrays fields ``dr.v.x``, ``dr.tau.xy`` ...,
+``DirichletBoundaryCondition``, ``solve`` (the reference's ``solve!``, which
+mutates ``dr`` in place) and ``update_stokes_current_stress``.
+
+Indices are 0-based on the Python side (node ids, DoFs); the ABI takes
+Julia's 1-based ids and the conversion happens here. Integration-point
+arrays (Julia ``nq x nels``) appear as numpy ``(nels, nq)``.
+
+There is no CPU fallback: every operation runs in libfemdr.so on the GPU,
+and importing the package fails loudly if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = [
+    "FemError", "DimensionMismatch", "Context", "DeviceArray", "VectorField2D", "SymmetricTensor2D",
+    "StokesMaterial", "DruckerPrager", "MixedMesh", "t7_box_mesh", "t7_mesh_from_arrays",
+    "DirichletBoundaryCondition", "StokesDR", "solve", "update_stokes_current_stress",
+    "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfemdr.so")
+MAXPH = 64
+
+FEM_OK, FEM_EARG, FEM_EDIM, FEM_ENOCONV, FEM_ENAN, FEM_EKABOOM, FEM_EPRECOND, FEM_EHIP, FEM_ENOMEM = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8)
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(f"libfemdr.so is not built ({_LIB_PATH}); run __graft_entry__.build() "
+                      "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+_lib = C.CDLL(_LIB_PATH)
+
+D, I32, I64 = C.c_double, C.c_int32, C.c_int64
+VP = C.c_void_p
+
+
+class FemError(RuntimeError):
+    """Error raised by the solver (Julia ``ErrorException`` counterpart)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class _Material(C.Structure):
+    _fields_ = [("nphases", I32), ("ndim", I32), ("eta", D * MAXPH), ("etab", D * MAXPH), ("G", D * MAXPH),
+                ("alpha", D * MAXPH), ("rho0", D * MAXPH), ("K", D * MAXPH), ("g", D * 3), ("Tref", D)]
+
+
+class _DP(C.Structure):
+    _fields_ = [("nphases", I32), ("cos_phi", D * MAXPH), ("sin_phi", D * MAXPH), ("sin_psi", D * MAXPH),
+                ("C", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH)]
+
+
+class _MeshInfo(C.Structure):
+    _fields_ = [("dim", I32), ("nv_local", I32), ("np_local", I32), ("nq", I32), ("nnodes", I64), ("nels", I64),
+                ("nnodesP", I64), ("n_shared", I64), ("n_boundary", I64)]
+
+
+class _StateOpts(C.Structure):
+    _fields_ = [("CFL_v", D), ("CFL_P", D), ("c_fact", D), ("eps", D)]
+
+
+class _BC(C.Structure):
+    _fields_ = [("n", I64), ("host_dofs_1based", C.POINTER(I32)), ("host_vals", C.POINTER(D))]
+
+
+class _SolveParams(C.Structure):
+    _fields_ = [("dt", D), ("pressure_factor", D), ("tolerance", D), ("max_iterations", I64), ("iter_max", I64),
+                ("check_interval", I64), ("max_ph_iterations", I64), ("rel_drop0", D), ("finite_K", I32),
+                ("freeze_jacobian", I32), ("throw_on_failure", I32), ("use_tau_old", I32), ("skip_scaling", I32),
+                ("verbose", I32), ("precision", I32), ("time_kernels", I32)]
+
+
+class _SolveStats(C.Structure):
+    _fields_ = [("itPH", I64), ("iter", I64), ("err", D), ("err_abs", D), ("err_rel", D), ("err_v", D),
+                ("err_P", D), ("converged", I32), ("reached_total_iter", I32), ("lambda_max", D),
+                ("lambda_max_gershgorin", D), ("lambda_max_iterations", I64), ("jacobian_assemblies", I64),
+                ("seconds", D), ("kernel_ms_element", D), ("kernel_ms_gather", D), ("element_launches", I64),
+                ("timed_launches", I64)]
+
+
+def _sig(name, restype, *argtypes):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_P = C.POINTER
+_sig("fem_ctx_create", C.c_int, C.c_int, _P(VP))
+_sig("fem_ctx_destroy", C.c_int, VP)
+_sig("fem_last_error", C.c_int, VP, C.c_char_p, C.c_size_t)
+_sig("fem_sync", C.c_int, VP)
+_sig("fem_buf_alloc", C.c_int, VP, C.c_size_t, _P(VP))
+_sig("fem_buf_free", C.c_int, VP, VP)
+_sig("fem_copy_h2d", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("fem_copy_d2h", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("fem_copy_d2d", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("fem_fill_f64", C.c_int, VP, VP, D, I64)
+_sig("fem_fill_f32", C.c_int, VP, VP, C.c_float, I64)
+_sig("fem_fill_i32", C.c_int, VP, VP, I32, I64)
+_sig("fem_mesh_t7_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
+_sig("fem_mesh_t7_upload", C.c_int, VP, _P(D), I64, _P(I32), I64, _P(VP))
+_sig("fem_mesh_get_info", C.c_int, VP, _P(_MeshInfo))
+_sig("fem_mesh_get_coords", C.c_int, VP, _P(D))
+_sig("fem_mesh_get_el2n", C.c_int, VP, _P(I32))
+_sig("fem_mesh_get_boundary_nodes", C.c_int, VP, _P(I32))
+_sig("fem_mesh_destroy", C.c_int, VP)
+_sig("fem_stokes_create", C.c_int, VP, VP, _P(_Material), _P(_StateOpts), _P(VP))
+_sig("fem_stokes_destroy", C.c_int, VP)
+_sig("fem_stokes_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
+_sig("fem_stokes_set_phases", C.c_int, VP, I32, I32, _P(I32), I64)
+_sig("fem_stokes_solve", C.c_int, VP, _P(_BC), _P(_DP), _P(_SolveParams), _P(_SolveStats))
+_sig("fem_stokes_pressure_scaling", C.c_int, VP, D, D, I32)
+_sig("fem_stokes_update_current_stress", C.c_int, VP, _P(_DP), D)
+_sig("fem_stokes_residual", C.c_int, VP, _P(_DP), D, I32, I32)
+_sig("fem_stokes_jacobian_diagnostics", C.c_int, VP, _P(_DP), D, I32)
+_sig("fem_stokes_time_element_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
+
+EXPORTED = [
+    "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
+    "fem_copy_h2d", "fem_copy_d2h", "fem_copy_d2d", "fem_fill_f64", "fem_fill_f32", "fem_fill_i32",
+    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
+    "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
+    "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_pressure_scaling",
+    "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
+    "fem_stokes_time_element_kernel",
+]
+
+
+class Context:
+    """One device + one HIP stream (fem_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = VP()
+        rc = _lib.fem_ctx_create(device, C.byref(h))
+        if rc != FEM_OK:
+            raise FemError(rc, f"fem_ctx_create(device={device}) failed with code {rc} (no HIP device?)")
+        self.handle = h
+        self.device = device
+
+    def error(self) -> str:
+        buf = C.create_string_buffer(2048)
+        _lib.fem_last_error(self.handle, buf, len(buf))
+        return buf.value.decode(errors="replace")
+
+    def check(self, rc: int):
+        if rc == FEM_OK:
+            return
+        msg = self.error()
+        if rc == FEM_EARG:
+            raise ValueError(msg)
+        if rc == FEM_EDIM:
+            raise DimensionMismatch(msg)
+        raise FemError(rc, msg)
+
+    def sync(self):
+        self.check(_lib.fem_sync(self.handle))
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("FEMDR_DEVICE", "0")))
+    return _default_ctx
+
+
+class DeviceArray:
+    """A device array owned by a StokesDR field (or by fem_buf_alloc)."""
+
+    def __init__(self, ctx: Context, ptr: int, shape, dtype=np.float64, owner=None):
+        self.ctx = ctx
+        self.ptr = int(ptr)
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self._owner = owner
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def __len__(self):
+        return self.shape[0]
+
+    @property
+    def nbytes(self):
+        return self.size * self.dtype.itemsize
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.size:
+            self.ctx.check(_lib.fem_copy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.numpy()
+        return a.astype(dtype) if dtype is not None else a
+
+    def copyto(self, src):
+        """copyto!(dest, src)"""
+        a = np.ascontiguousarray(np.asarray(src, dtype=self.dtype).reshape(self.shape))
+        if self.size:
+            self.ctx.check(_lib.fem_copy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, self.nbytes))
+
+    def fill(self, v):
+        """fill!(A, v)"""
+        if self.dtype == np.float64:
+            self.ctx.check(_lib.fem_fill_f64(self.ctx.handle, self.ptr, float(v), self.size))
+        elif self.dtype == np.float32:
+            self.ctx.check(_lib.fem_fill_f32(self.ctx.handle, self.ptr, float(v), self.size))
+        else:
+            self.ctx.check(_lib.fem_fill_i32(self.ctx.handle, self.ptr, int(v), self.size))
+
+    def __repr__(self):
+        return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
+
+
+@dataclass
+class VectorField2D:
+    """VectorField2D (src/vector_fields/vector_fields.jl:24-32)."""
+    x: DeviceArray
+    y: DeviceArray
+
+    def __iter__(self):
+        return iter((self.x, self.y))
+
+
+@dataclass
+class SymmetricTensor2D:
+    """SymmetricTensor2D (src/tensor_fields/symmetric_tensor.jl:44-51); the
+    invariant slot is not stored (stokes_types.jl:71-79)."""
+    xx: DeviceArray
+    yy: DeviceArray
+    xy: DeviceArray
+
+    def __iter__(self):
+        return iter((self.xx, self.yy, self.xy))
+
+
+def _tup(v, n=None):
+    if isinstance(v, (tuple, list, np.ndarray)):
+        return tuple(float(x) for x in v)
+    return (float(v),) * (n or 1)
+
+
+class StokesMaterial:
+    """StokesMaterial(; η=(1.0,), ηb=(1.0,), G=(Inf,), α=(0.0,), ρ0=(1.0,),
+    K=(Inf,), g=(0.0, 0.0), Tref=0.0) (src/stokes/types/stokes_types.jl:19-60).
+    The first tuple given fixes the phase count; scalars apply to every phase."""
+
+    def __init__(self, eta=None, etab=None, G=None, alpha=None, rho0=None, K=None, g=None, Tref=None):
+        given = [v for v in (eta, etab, G, alpha, rho0, K) if isinstance(v, (tuple, list, np.ndarray))]
+        n = len(given[0]) if given else 1
+        if n < 1:
+            raise ValueError("material properties cannot be empty")
+        if n > MAXPH:
+            raise ValueError(f"at most {MAXPH} phases are supported")
+
+        def prop(v, default):
+            t = _tup(default if v is None else v, n)
+            if len(t) != n:
+                raise DimensionMismatch("Stokes material property tuples must have the same length")
+            return t
+
+        self.eta = prop(eta, 1.0)
+        self.etab = prop(etab, 1.0)
+        self.G = prop(G, math.inf)
+        self.alpha = prop(alpha, 0.0)
+        self.rho0 = prop(rho0, 1.0)
+        self.K = prop(K, math.inf)
+        self.g = tuple(float(x) for x in (g if g is not None else (0.0, 0.0)))
+        if len(self.g) not in (2, 3):
+            raise ValueError(f"gravity must have 2 or 3 components, got {len(self.g)}")
+        self.Tref = float(Tref) if Tref is not None else 0.0
+        self.nphases = n
+
+    def _c(self):
+        m = _Material()
+        m.nphases = self.nphases
+        m.ndim = len(self.g)
+        for name in ("eta", "etab", "G", "alpha", "rho0", "K"):
+            arr = getattr(m, name)
+            for i, x in enumerate(getattr(self, name)):
+                arr[i] = x
+        for i, x in enumerate(self.g):
+            m.g[i] = x
+        m.Tref = self.Tref
+        return m
+
+
+class DruckerPrager:
+    """DruckerPrager(ϕ, Ψ, C, η_reg, Kb) (stokes_types.jl:388-423): friction and
+    dilation angles in radians, stored as cos ϕ, sin ϕ, sin Ψ."""
+
+    def __init__(self, phi, psi, C_, eta_reg, Kb):
+        phi, psi, C_, eta_reg, Kb = (_tup(x) for x in (phi, psi, C_, eta_reg, Kb))
+        n = len(phi)
+        if not all(len(t) == n for t in (psi, C_, eta_reg, Kb)):
+            raise DimensionMismatch("DruckerPrager parameter tuples must have the same length")
+        self.cos_phi = tuple(math.cos(x) for x in phi)
+        self.sin_phi = tuple(math.sin(x) for x in phi)
+        self.sin_psi = tuple(math.sin(x) for x in psi)
+        self.C, self.eta_reg, self.Kb = C_, eta_reg, Kb
+        self.nphases = n
+
+    def _c(self):
+        d = _DP()
+        d.nphases = self.nphases
+        for name in ("cos_phi", "sin_phi", "sin_psi", "C", "eta_reg", "Kb"):
+            arr = getattr(d, name)
+            for i, x in enumerate(getattr(self, name)):
+                arr[i] = x
+        return d
+
+
+class MixedMesh:
+    """MixedMesh of T7 velocity / P1-disc pressure elements
+    (src/mesh/mixed_mesh.jl:173-211) resident on the device."""
+
+    def __init__(self, ctx: Context, handle, box=None):
+        self.ctx = ctx
+        self.handle = handle
+        info = _MeshInfo()
+        ctx.check(_lib.fem_mesh_get_info(handle, C.byref(info)))
+        self.nnodes, self.nels, self.nnodesP = info.nnodes, info.nels, info.nnodesP
+        self.n_shared = info.n_shared
+        self.nq = info.nq
+        self.box = box
+        coords = np.empty((self.nnodes, 2))
+        ctx.check(_lib.fem_mesh_get_coords(handle, coords.ctypes.data_as(_P(D))))
+        el2n = np.empty((self.nels, 7), np.int32)
+        ctx.check(_lib.fem_mesh_get_el2n(handle, el2n.ctypes.data_as(_P(I32))))
+        self.coords = coords
+        self.el2n = el2n - 1
+        self.el2nP = self.el2n[:, :3]
+        self.DoFsP = np.arange(self.nnodesP, dtype=np.int32).reshape(self.nels, 3)
+        nb = info.n_boundary
+        b = np.empty(nb, np.int32)
+        if nb:
+            ctx.check(_lib.fem_mesh_get_boundary_nodes(handle, b.ctypes.data_as(_P(I32))))
+        self.boundary_nodes = b - 1
+
+    def __del__(self):
+        try:
+            _lib.fem_mesh_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def t7_box_mesh(nx: int, ny: int, domain=((0.0, 1.0), (0.0, 1.0)), ctx: Context | None = None) -> MixedMesh:
+    """MixedMesh(Mesh(backend, (x0..x1)×(y0..y1), T7, (nx, ny)), P1)."""
+    ctx = ctx or default_context()
+    (x0, x1), (y0, y1) = domain
+    h = VP()
+    ctx.check(_lib.fem_mesh_t7_create(ctx.handle, int(nx), int(ny), x0, x1, y0, y1, C.byref(h)))
+    return MixedMesh(ctx, h, box=(x0, x1, y0, y1))
+
+
+def t7_mesh_from_arrays(coords, el2n, ctx: Context | None = None) -> MixedMesh:
+    """MixedMesh from unstructured T7 arrays (0-based el2n, local node 7 = centroid)."""
+    ctx = ctx or default_context()
+    coords = np.ascontiguousarray(coords, np.float64)
+    e1 = np.ascontiguousarray(np.asarray(el2n, np.int32) + 1)
+    h = VP()
+    ctx.check(_lib.fem_mesh_t7_upload(ctx.handle, coords.ctypes.data_as(_P(D)), len(coords),
+                                      e1.ctypes.data_as(_P(I32)), len(e1), C.byref(h)))
+    return MixedMesh(ctx, h)
+
+
+class DirichletBoundaryCondition:
+    """DirichletBoundaryCondition(DoFs, vals) (boundary_conditions.jl:30-42), 0-based DoFs."""
+
+    def __init__(self, dofs, vals):
+        self.DoFs = np.ascontiguousarray(dofs, np.int32)
+        self.vals = np.ascontiguousarray(vals, np.float64)
+        if self.DoFs.shape != self.vals.shape:
+            raise DimensionMismatch("DoFs and vals must have equal lengths")
+        self.zero_vals = np.zeros_like(self.vals)
+
+    def _c(self, keep):
+        d1 = np.ascontiguousarray(self.DoFs + 1, np.int32)
+        keep.append(d1)
+        b = _BC()
+        b.n = len(d1)
+        b.host_dofs_1based = d1.ctypes.data_as(_P(I32))
+        b.host_vals = self.vals.ctypes.data_as(_P(D))
+        return b
+
+
+_FIELDS = ["v.x", "v.y", "dvdtau.x", "dvdtau.y", "Rv.x", "Rv.y", "Rv0.x", "Rv0.y", "dRvdv.x", "dRvdv.y",
+           "PC_v.x", "PC_v.y", "tau.xx", "tau.yy", "tau.xy", "tau_old.xx", "tau_old.yy", "tau_old.xy",
+           "P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum"]
+
+
+class StokesDR:
+    """StokesDR(mesh, material; CFL_v=0.98, CFL_P=0.98, c_fact=0.9, ϵ=1e-6)
+    (stokes_types.jl:197-298, 359-369). Fields are device arrays:
+    ``v``, ``dvdtau`` (∂v∂τ), ``Rv``, ``Rv0``, ``dRvdv`` (∂Rv∂v), ``PC_v``
+    (VectorField2D); ``tau``, ``tau_old`` (SymmetricTensor2D at integration
+    points, shape (nels, nq)); ``P``, ``P0``, ``gammaP``, ``T``, ``T0``, ``Q``,
+    ``Pf``, ``RP``, ``RP0``, ``M_P``, ``Pnum``."""
+
+    def __init__(self, mesh: MixedMesh, material: StokesMaterial, CFL_v=0.98, CFL_P=0.98, c_fact=0.9, eps=1e-6):
+        self.ctx = mesh.ctx
+        self.mesh = mesh
+        self.material = material
+        if len(material.g) != 2:
+            raise DimensionMismatch("material gravity must have 2 components")
+        self._mat = material._c()
+        opts = _StateOpts(CFL_v, CFL_P, c_fact, eps)
+        h = VP()
+        self.ctx.check(_lib.fem_stokes_create(self.ctx.handle, mesh.handle, C.byref(self._mat), C.byref(opts),
+                                              C.byref(h)))
+        self.handle = h
+        self.CFL_v, self.CFL_P, self.c_fact, self.eps = CFL_v, CFL_P, c_fact, eps
+        arrs = []
+        for k, name in enumerate(_FIELDS):
+            p, n, dt = VP(), I64(), I32()
+            self.ctx.check(_lib.fem_stokes_field(h, k, C.byref(p), C.byref(n), C.byref(dt)))
+            shape = (mesh.nels, mesh.nq) if name.startswith("tau") else (n.value,)
+            arrs.append(DeviceArray(self.ctx, p.value, shape, np.float64, owner=self))
+        a = dict(zip(_FIELDS, arrs))
+        self.v = VectorField2D(a["v.x"], a["v.y"])
+        self.dvdtau = VectorField2D(a["dvdtau.x"], a["dvdtau.y"])
+        self.Rv = VectorField2D(a["Rv.x"], a["Rv.y"])
+        self.Rv0 = VectorField2D(a["Rv0.x"], a["Rv0.y"])
+        self.dRvdv = VectorField2D(a["dRvdv.x"], a["dRvdv.y"])
+        self.PC_v = VectorField2D(a["PC_v.x"], a["PC_v.y"])
+        self.tau = SymmetricTensor2D(a["tau.xx"], a["tau.yy"], a["tau.xy"])
+        self.tau_old = SymmetricTensor2D(a["tau_old.xx"], a["tau_old.yy"], a["tau_old.xy"])
+        for name in ("P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum"):
+            setattr(self, name, a[name])
+
+    def __del__(self):
+        try:
+            _lib.fem_stokes_destroy(self.handle)
+        except Exception:
+            pass
+
+    def set_phases(self, which: str, phases):
+        """phases_v / phases_P: (nels,) cell phases, (nels, NV|NP) element-local
+        phases, or a per-node (per-DoF) vector. 1-based ids as in the reference."""
+        ph = np.asarray(phases)
+        w = 0 if which == "v" else 1
+        nloc = 7 if w == 0 else 3
+        nnod = self.mesh.nnodes if w == 0 else self.mesh.nnodesP
+        if ph.ndim == 2 and ph.shape == (self.mesh.nels, nloc):
+            layout = 1
+        elif ph.ndim == 2 and ph.shape == (1, self.mesh.nels):
+            layout, ph = 0, ph[0]
+        elif ph.ndim == 1 and len(ph) == self.mesh.nels and len(ph) != nnod:
+            layout = 0
+        elif ph.ndim == 1 and len(ph) == nnod:
+            layout = 2
+        else:
+            raise DimensionMismatch(f"phase array of shape {ph.shape} matches no layout")
+        a = np.ascontiguousarray(ph, np.int32).ravel()
+        self.ctx.check(_lib.fem_stokes_set_phases(self.handle, w, layout, a.ctypes.data_as(_P(I32)), len(a)))
+
+
+def _dp_ptr(plastic):
+    if plastic is None:
+        return None, C.cast(None, _P(_DP))
+    d = plastic._c()
+    return d, C.pointer(d)
+
+
+def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, dt, finite_K=False):
+    """assemble_viscosity_weighted_pressure_scaling! (pressure_scaling.jl:8-149):
+    fills dr.M_P and dr.gammaP."""
+    dr.ctx.check(_lib.fem_stokes_pressure_scaling(dr.handle, float(pressure_factor), float(dt), int(finite_K)))
+
+
+def solve(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition, *,
+          dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, check_interval=50, iterMax=50_000,
+          max_ph_iterations=1000, rel_drop0=1e-2, plastic=None, phases_v=None, phases_P=None, finite_K=False,
+          freeze_jacobian=None, throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False,
+          time_kernels=False):
+    """solve!(dr, mesh, bc_vx, bc_vy; dt, ...) (src/stokes/solvers/DR.jl:302-337).
+    Mutates dr in place and returns the stats dict of DR.jl:783-801."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    if phases_v is not None:
+        dr.set_phases("v", phases_v)
+    if phases_P is not None:
+        dr.set_phases("P", phases_P)
+    keep = []
+    bcs = (_BC * 2)(bc_vx._c(keep), bc_vy._c(keep))
+    d, dp = _dp_ptr(plastic)
+    p = _SolveParams(dt=float(dt), pressure_factor=float(pressure_factor), tolerance=float(tolerance),
+                     max_iterations=int(max_iterations), iter_max=int(iterMax), check_interval=int(check_interval),
+                     max_ph_iterations=int(max_ph_iterations), rel_drop0=float(rel_drop0), finite_K=int(finite_K),
+                     freeze_jacobian=-1 if freeze_jacobian is None else int(bool(freeze_jacobian)),
+                     throw_on_failure=int(bool(throw_on_failure)), use_tau_old=int(bool(tau_old)),
+                     skip_scaling=int(bool(skip_scaling)), verbose=int(verbose), precision=0,
+                     time_kernels=int(bool(time_kernels)))
+    st = _SolveStats()
+    rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
+    stats = {k: getattr(st, k) for k, _ in _SolveStats._fields_}
+    stats["converged"] = bool(stats["converged"])
+    stats["reached_total_iter"] = bool(stats["reached_total_iter"])
+    stats["iterations"] = stats["iter"]
+    stats["residual"] = stats["err_abs"]
+    dr.ctx.check(rc)
+    return stats
+
+
+def update_stokes_current_stress(dr: StokesDR, mesh: MixedMesh, dt, plastic=None):
+    """update_stokes_current_stress!(dr, mesh, τ, Δt; plastic) (DR.jl:820-897):
+    writes dr.tau at the integration points."""
+    d, dp = _dp_ptr(plastic)
+    dr.ctx.check(_lib.fem_stokes_update_current_stress(dr.handle, dp, float(dt)))
+    return dr.tau
+
+
+def residual(dr: StokesDR, dt, plastic=None, with_pnum=False, finite_K=False):
+    """Element residual sweep: dr.RP (and dr.Pnum) and dr.Rv at the current state."""
+    d, dp = _dp_ptr(plastic)
+    dr.ctx.check(_lib.fem_stokes_residual(dr.handle, dp, float(dt), int(with_pnum), int(finite_K)))
+
+
+def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False):
+    """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal)."""
+    d, dp = _dp_ptr(plastic)
+    dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
+
+
+def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
+    """Average device time (ms) of the fused element kernel over n launches
+    (HIP events on the context stream). Advances bubble-node velocities."""
+    d, dp = _dp_ptr(plastic)
+    out = D()
+    dr.ctx.check(_lib.fem_stokes_time_element_kernel(dr.handle, dp, float(dt), int(n), C.byref(out)))
+    return out.value

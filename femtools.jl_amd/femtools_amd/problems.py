@@ -1,0 +1,117 @@
+"""Synthetic inputs of the DR Stokes workloads, as plain numpy arrays so the
+same problem can be handed to the GPU library and to a CPU checker.
+
+* viscous inclusion: 2-D incompressible Stokes on the unit box, T7/P1-disc,
+  matrix η=1 with a centred circular inclusion (r=0.1, η=1e3), pure-shear
+  velocity boundary conditions (ε̇ = 1), no gravity, G=K=ηb=∞ -- the setting
+  of the pure-shear triangle example
+  (examples/miniapps/stokes/stokes_2D_pure_shear_triangle/...jl:40-154).
+* visco-elasto-plastic inclusions: seeded random circular inclusions
+  (r ∈ U[0.01, 0.05], log10 η ∈ U[-2, 4], later inclusion wins), shear
+  modulus G=1 and Δt = 0.1 η_matrix/G, Drucker-Prager C=1, ϕ=30°, Ψ=0 with a
+  finite Kb, densities 1 (matrix) / 1.1 (inclusions), g=(0,-1), pure shear.
+
+Phases are per element (1 x nels layout), taken at the element centroid.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def element_centroids(coords, el2n):
+    return coords[el2n[:, :3]].mean(axis=1)
+
+
+def inclusion_phases(coords, el2n, circles):
+    """1-based cell phases: 1 = matrix, k+1 = k-th circle (later circles win)."""
+    c = element_centroids(coords, el2n)
+    ph = np.ones(len(el2n), np.int32)
+    for k, (cx, cy, r) in enumerate(circles):
+        inside = (c[:, 0] - cx) ** 2 + (c[:, 1] - cy) ** 2 <= r * r
+        ph[inside] = k + 2
+    return ph
+
+
+def box_boundary_nodes(coords, box):
+    x0, x1, y0, y1 = box
+    tol = 32 * np.finfo(float).eps * max(1.0, *(abs(v) for v in box))
+    x, y = coords[:, 0], coords[:, 1]
+    return np.nonzero((np.abs(x - x0) <= tol) | (np.abs(x - x1) <= tol) |
+                      (np.abs(y - y0) <= tol) | (np.abs(y - y1) <= tol))[0].astype(np.int32)
+
+
+def pure_shear_bc(coords, box=(0.0, 1.0, 0.0, 1.0), edot=1.0):
+    """vx = ε̇(x - xc) on the x-walls, vy = -ε̇(y - yc) on the y-walls
+    (stokes_2D_pure_shear_triangle.jl:145-154). Returns ((dofs_x, vals_x), (dofs_y, vals_y))."""
+    x0, x1, y0, y1 = box
+    gam = box_boundary_nodes(coords, box)
+    tol = max(abs(x1 - x0), abs(y1 - y0)) * np.finfo(float).eps * 32
+    x, y = coords[gam, 0], coords[gam, 1]
+    nx = gam[(np.abs(x - x0) <= tol) | (np.abs(x - x1) <= tol)]
+    ny = gam[(np.abs(y - y0) <= tol) | (np.abs(y - y1) <= tol)]
+    xm, ym = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    return (nx, edot * (coords[nx, 0] - xm)), (ny, -edot * (coords[ny, 1] - ym))
+
+
+@dataclass
+class Workload:
+    name: str
+    nx: int
+    ny: int
+    eta: tuple
+    etab: tuple
+    G: tuple
+    rho0: tuple
+    K: tuple
+    g: tuple
+    dt: float
+    circles: list
+    plastic: dict | None = None
+    tolerance: float = 1e-8
+    check_interval: int = 50
+    pressure_factor: float = 50.0
+    nsteps: int = 1
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def nphases(self):
+        return len(self.eta)
+
+    def sizes(self):
+        nn = (2 * self.nx + 1) * (2 * self.ny + 1) + 2 * self.nx * self.ny
+        ne = 2 * self.nx * self.ny
+        return nn, ne, 3 * ne
+
+    def dofs(self):
+        nn, ne, npp = self.sizes()
+        return 2 * nn + npp
+
+
+def viscous_inclusion(n=64) -> Workload:
+    """Plain FP64 viscous inclusion workload on an n x n cell grid."""
+    inf = math.inf
+    return Workload(name=f"T7/P1-disc {n}x{n} viscous inclusion", nx=n, ny=n, eta=(1.0, 1e3), etab=(inf, inf),
+                    G=(inf, inf), rho0=(1.0, 1.0), K=(inf, inf), g=(0.0, 0.0), dt=1.0,
+                    circles=[(0.5, 0.5, 0.1)])
+
+
+def vep_inclusions(n=512, ninc=20, seed=0, nsteps=10, eta_reg=8e-3, Kb=100.0) -> Workload:
+    """Visco-elasto-plastic multi-inclusion workload on an n x n cell grid."""
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(0, 1, ninc)
+    cy = rng.uniform(0, 1, ninc)
+    r = rng.uniform(0.01, 0.05, ninc)
+    leta = rng.uniform(-2, 4, ninc)
+    inf = math.inf
+    nph = ninc + 1
+    eta = (1.0,) + tuple(10.0 ** leta)
+    G = 1.0
+    return Workload(name=f"T7/P1-disc {n}x{n} VEP {ninc} inclusions", nx=n, ny=n, eta=eta, etab=(inf,) * nph,
+                    G=(G,) * nph, rho0=(1.0,) + (1.1,) * ninc, K=(inf,) * nph, g=(0.0, -1.0), dt=0.1 * 1.0 / G,
+                    circles=list(zip(cx, cy, r)),
+                    plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(1.0,) * nph,
+                                 eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph),
+                    nsteps=nsteps, extra=dict(seed=seed))

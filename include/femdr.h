@@ -1,0 +1,241 @@
+/*
+ * femdr.h -- C ABI of libfemdr.so, the MI355X (gfx950) implementation of
+ * FEMTools.jl's matrix-free dynamic-relaxation (DR) Stokes hot path.
+ *
+ * Plain pointers and sizes only. "host_*" arguments are host memory; every
+ * other pointer is device memory owned by the library. Connectivity and
+ * node/DoF lists cross the boundary 1-based, as Julia holds them; the
+ * library converts them once. Element-indexed matrices (Julia NV x nels or
+ * nq x nels, column-major) are element-contiguous: C [nels][NV].
+ *
+ * Every function returns FEM_OK (0) or a negative FEM_E* code; the message
+ * of the last failure on a context is fem_last_error(). One context owns one
+ * HIP stream on one device; a context is not thread-safe.
+ *
+ * Reference citations are file:line in FEMTools.jl.
+ */
+#ifndef FEMDR_H
+#define FEMDR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the Julia wrapper maps them to the reference's exceptions */
+#define FEM_OK 0
+#define FEM_EARG -1       /* ArgumentError                                      */
+#define FEM_EDIM -2       /* DimensionMismatch                                  */
+#define FEM_ENOCONV -3    /* "Stokes solve did not converge ..." DR.jl:332-335  */
+#define FEM_ENAN -4       /* "NaN detected in outer/inner loop ..." DR.jl:574,711 */
+#define FEM_EKABOOM -5    /* "Kaboom! Error > 1e10 ..." DR.jl:575,712           */
+#define FEM_EPRECOND -6   /* "... preconditioner produced invalid λmax" utils.jl:155 */
+#define FEM_EHIP -7       /* HIP runtime failure                                */
+#define FEM_ENOMEM -8     /* device allocation failed                           */
+
+#define FEM_MAX_PHASES 64
+
+typedef struct fem_ctx fem_ctx;
+typedef struct fem_mesh fem_mesh;
+typedef struct fem_stokes fem_stokes;
+
+/* ---------------------------------------------------------------- context
+ * Replaces the KernelAbstractions backend object and TA(backend)
+ * (src/FEMTools.jl:30-46, ext/FEMToolsCUDAExt.jl:5). */
+int fem_ctx_create(int device, fem_ctx** out);
+int fem_ctx_destroy(fem_ctx* ctx);
+int fem_last_error(const fem_ctx* ctx, char* buf, size_t len);
+int fem_sync(fem_ctx* ctx);
+
+/* ---------------------------------------------------------------- buffers
+ * Back the wrapper's device arrays (KernelAbstractions.zeros, copyto!,
+ * Array(.), fill!; stokes_types.jl:268-279). */
+int fem_buf_alloc(fem_ctx* ctx, size_t bytes, void** dptr);
+int fem_buf_free(fem_ctx* ctx, void* dptr);
+int fem_copy_h2d(fem_ctx* ctx, void* dst, const void* host_src, size_t bytes);
+int fem_copy_d2h(fem_ctx* ctx, void* host_dst, const void* src, size_t bytes);
+int fem_copy_d2d(fem_ctx* ctx, void* dst, const void* src, size_t bytes);
+int fem_fill_f64(fem_ctx* ctx, double* dptr, double value, int64_t n);
+int fem_fill_f32(fem_ctx* ctx, float* dptr, float value, int64_t n);
+int fem_fill_i32(fem_ctx* ctx, int32_t* dptr, int32_t value, int64_t n);
+
+/* ------------------------------------------------------------------- mesh */
+typedef struct {
+    int32_t dim;        /* 2                                         */
+    int32_t nv_local;   /* 7 (T7)                                    */
+    int32_t np_local;   /* 3 (P1-disc)                               */
+    int32_t nq;         /* 7 (Dunavant degree 5)                     */
+    int64_t nnodes;     /* velocity nodes                            */
+    int64_t nels;       /* elements                                  */
+    int64_t nnodesP;    /* pressure DoFs = 3 nels                    */
+    int64_t n_shared;   /* velocity nodes shared by >1 element slot  */
+    int64_t n_boundary; /* nodes on the box boundary (structured)    */
+} fem_mesh_info;
+
+/* Mesh(backend, (x0..x1)x(y0..y1), ReferenceElement(QuadraticElement{2,7}), (nx,ny))
+ * followed by MixedMesh(mesh, ReferenceElement(LinearElement{2,3}))
+ * (src/mesh/mesh.jl:67-101, 412-440; connectivity.jl:111-140; mixed_mesh.jl:173-211,
+ * 436-457). Builds coordinates, connectivity, node->element lists, geometry
+ * and pressure DoFs on the device. */
+int fem_mesh_t7_create(fem_ctx* ctx, int32_t nx, int32_t ny, double x0, double x1,
+                       double y0, double y1, fem_mesh** out);
+/* Unstructured T7 mesh: Mesh(backend, coords, el2n, element) (mesh.jl:106-115)
+ * + MixedMesh. host_el2n_1based is [nels][7]; local node 7 must be the
+ * element's private centroid (bubble) node. */
+int fem_mesh_t7_upload(fem_ctx* ctx, const double* host_coords, int64_t nnodes,
+                       const int32_t* host_el2n_1based, int64_t nels, fem_mesh** out);
+int fem_mesh_get_info(const fem_mesh* mesh, fem_mesh_info* out);
+int fem_mesh_get_coords(const fem_mesh* mesh, double* host_out /* [nnodes][2] */);
+int fem_mesh_get_el2n(const fem_mesh* mesh, int32_t* host_out_1based /* [nels][7] */);
+/* Γnodes of a structured box mesh (mesh.jl:79-80), 1-based */
+int fem_mesh_get_boundary_nodes(const fem_mesh* mesh, int32_t* host_out_1based);
+int fem_mesh_destroy(fem_mesh* mesh);
+
+/* --------------------------------------------------------------- material */
+/* StokesMaterial (src/stokes/types/stokes_types.jl:19-60) */
+typedef struct {
+    int32_t nphases;
+    int32_t ndim;                      /* length of g: 2              */
+    double eta[FEM_MAX_PHASES];
+    double etab[FEM_MAX_PHASES];
+    double G[FEM_MAX_PHASES];
+    double alpha[FEM_MAX_PHASES];
+    double rho0[FEM_MAX_PHASES];
+    double K[FEM_MAX_PHASES];
+    double g[3];
+    double Tref;
+} fem_material;
+
+/* DruckerPrager (stokes_types.jl:388-423): cos(phi), sin(phi), sin(psi) stored */
+typedef struct {
+    int32_t nphases;
+    double cos_phi[FEM_MAX_PHASES];
+    double sin_phi[FEM_MAX_PHASES];
+    double sin_psi[FEM_MAX_PHASES];
+    double C[FEM_MAX_PHASES];
+    double eta_reg[FEM_MAX_PHASES];
+    double Kb[FEM_MAX_PHASES];
+} fem_drucker_prager;
+
+/* ------------------------------------------------------------------ state */
+/* StokesDR(mesh, material; CFL_v, CFL_P, c_fact, ϵ) (stokes_types.jl:244-298,
+ * 359-369): all fields zero, phases 1, stress at integration points (nq x nels). */
+typedef struct {
+    double CFL_v;   /* 0.98 */
+    double CFL_P;   /* 0.98 */
+    double c_fact;  /* 0.9  */
+    double eps;     /* 1e-6 */
+} fem_state_opts;
+
+int fem_stokes_create(fem_ctx* ctx, fem_mesh* mesh, const fem_material* mat,
+                      const fem_state_opts* opts, fem_stokes** out);
+int fem_stokes_destroy(fem_stokes* st);
+
+/* StokesDR struct-of-arrays fields (stokes_types.jl:197-241); dr.v.x is
+ * FEM_FIELD_V_X, dr.τ.xy is FEM_FIELD_TAU_XY, and so on. */
+enum {
+    FEM_FIELD_V_X = 0, FEM_FIELD_V_Y,
+    FEM_FIELD_DVDTAU_X, FEM_FIELD_DVDTAU_Y,
+    FEM_FIELD_RV_X, FEM_FIELD_RV_Y,
+    FEM_FIELD_RV0_X, FEM_FIELD_RV0_Y,
+    FEM_FIELD_DRVDV_X, FEM_FIELD_DRVDV_Y,
+    FEM_FIELD_PC_V_X, FEM_FIELD_PC_V_Y,
+    FEM_FIELD_TAU_XX, FEM_FIELD_TAU_YY, FEM_FIELD_TAU_XY,
+    FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_XY,
+    FEM_FIELD_P, FEM_FIELD_P0, FEM_FIELD_GAMMA_P, FEM_FIELD_T, FEM_FIELD_T0, FEM_FIELD_Q,
+    FEM_FIELD_PF, FEM_FIELD_RP, FEM_FIELD_RP0, FEM_FIELD_M_P, FEM_FIELD_PNUM,
+    FEM_FIELD_COUNT
+};
+/* device pointer and length of one field; dtype: 0 = f64 */
+int fem_stokes_field(fem_stokes* st, int32_t field, void** dptr, int64_t* len, int32_t* dtype);
+
+/* Phase layouts accepted by the reference (src/utils.jl:70-74):
+ * FEM_PHASE_CELL  1 x nels matrix, FEM_PHASE_ELEMENT  NV x nels matrix,
+ * FEM_PHASE_NODAL  per-node (velocity) / per-DoF (pressure) vector. 1-based ids. */
+#define FEM_PHASE_CELL 0
+#define FEM_PHASE_ELEMENT 1
+#define FEM_PHASE_NODAL 2
+/* which: 0 = phases_v, 1 = phases_P (dr.phases_v / dr.phases_P, or the
+ * phases_v= / phases_P= keywords of solve!, DR.jl:307-310) */
+int fem_stokes_set_phases(fem_stokes* st, int32_t which, int32_t layout, const int32_t* host_phases,
+                          int64_t n);
+
+/* DirichletBoundaryCondition(DoFs, vals) (boundary_conditions.jl:30-42), 1-based */
+typedef struct {
+    int64_t n;
+    const int32_t* host_dofs_1based;
+    const double* host_vals;
+} fem_bc;
+
+/* solve!(dr, mesh, bc_vx, bc_vy; ...) keywords (DR.jl:302-319) and the
+ * _solve_stokes_dyrel! keywords they forward (DR.jl:366-386). */
+typedef struct {
+    double dt;
+    double pressure_factor;     /* 50                                   */
+    double tolerance;           /* ϵ_tol, 1e-6                          */
+    int64_t max_iterations;     /* total_iterMax, 50_000                */
+    int64_t iter_max;           /* iterMax (inner per PH), 50_000       */
+    int64_t check_interval;     /* ncheck, 50                           */
+    int64_t max_ph_iterations;  /* 1000                                 */
+    double rel_drop0;           /* 1e-2                                 */
+    int32_t finite_K;           /* pressure storage modulus K (1) or ηb (0) */
+    int32_t freeze_jacobian;    /* -1 = reference default (plastic === nothing) */
+    int32_t throw_on_failure;   /* 1: non-convergence returns FEM_ENOCONV */
+    int32_t use_tau_old;        /* 1: read dr.τ_old (default), 0: τ_old = nothing */
+    int32_t skip_scaling;       /* 1: γP, M_P already assembled (solve_stokes_dyrel!) */
+    int32_t verbose;
+    int32_t precision;          /* 0 = FP64 throughout                  */
+    int32_t time_kernels;       /* 1: HIP events around every element / gather launch */
+} fem_solve_params;
+
+/* stats NamedTuple (DR.jl:783-801) + device timing */
+typedef struct {
+    int64_t itPH, iter;
+    double err, err_abs, err_rel, err_v, err_P;
+    int32_t converged, reached_total_iter;
+    double lambda_max, lambda_max_gershgorin;
+    int64_t lambda_max_iterations, jacobian_assemblies;
+    double seconds;             /* wall time of the solve (host clock, after a device sync) */
+    double kernel_ms_element;   /* summed device time of the element kernel (time_kernels) */
+    double kernel_ms_gather;    /* summed device time of the gather/update kernel          */
+    int64_t element_launches;   /* inner-iteration element launches                        */
+    int64_t timed_launches;     /* launches covered by kernel_ms_*                          */
+} fem_solve_stats;
+
+/* solve!(::StokesDR, ::MixedMesh, bc_v; dt, ...) DR.jl:302-337: assembles
+ * M_P and γP (pressure_scaling.jl:8-149) and runs the Powell-Hestenes/DYREL
+ * loop of DR.jl:348-802 on the device. plastic may be NULL. */
+int fem_stokes_solve(fem_stokes* st, const fem_bc bcs[2], const fem_drucker_prager* plastic,
+                     const fem_solve_params* params, fem_solve_stats* stats);
+
+/* assemble_viscosity_weighted_pressure_scaling! (pressure_scaling.jl:118-216):
+ * fills M_P and γP. */
+int fem_stokes_pressure_scaling(fem_stokes* st, double pressure_factor, double dt, int32_t finite_K);
+
+/* update_stokes_current_stress! (DR.jl:820-897): τ at integration points
+ * from the current v, P and τ_old; overwrites Rv like the reference. */
+int fem_stokes_update_current_stress(fem_stokes* st, const fem_drucker_prager* plastic, double dt);
+
+/* Element-level entry points used by parity tests.
+ * fem_stokes_residual: RP (pressure_residual.jl:256-294) and Rv
+ * (momentum_residuals.jl:417-470) at the current state; with_pnum=1 forms
+ * Pnum = γP RP / M_P first (DR.jl:674) and uses it in the momentum residual. */
+int fem_stokes_residual(fem_stokes* st, const fem_drucker_prager* plastic, double dt,
+                        int32_t with_pnum, int32_t finite_K);
+/* assemble_augmented_momentum_jacobian_matrices_atomix! (momentum_residuals.jl
+ * :998-1117): ∂Rv∂v row sums and PC_v diagonal. */
+int fem_stokes_jacobian_diagnostics(fem_stokes* st, const fem_drucker_prager* plastic, double dt,
+                                    int32_t finite_K);
+
+/* Sweeps the element kernel alone (no control flow) n times; used by the
+ * bench to time the dominant kernel with device events. Returns the average
+ * kernel time in milliseconds through avg_ms. */
+int fem_stokes_time_element_kernel(fem_stokes* st, const fem_drucker_prager* plastic, double dt,
+                                   int32_t n, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

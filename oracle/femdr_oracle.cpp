@@ -728,6 +728,25 @@ void orc_dr_update(orc_stokes* s, const double* alpha, const double* beta) {
     }
 }
 
+// k bare inner DR iterations with fixed Chebyshev coefficients (DR.jl:665-697):
+// pressure residual, Pnum, momentum residual, Dirichlet, update. Used to time
+// the CPU side of the hot path.
+void orc_dr_iterations(orc_stokes* s, double dt, int32_t finite_K, const double* alpha, const double* beta,
+                       int64_t k) {
+    double* rate[2] = {s->rate_x, s->rate_y};
+    double* Rv[2] = {s->Rvx, s->Rvy};
+    double* v[2] = {s->vx, s->vy};
+    for (int64_t it = 0; it < k; ++it) {
+        orc_pressure_residual(s, dt, finite_K, s->P0);
+        for (int64_t i = 0; i < s->nnodesP; ++i) s->Pnum[i] = s->gammaP[i] * s->RP[i] / s->MP[i];
+        orc_momentum_residual(s, dt, s->Pnum);
+        for (int c = 0; c < 2; ++c) dirichlet(Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+        for (int c = 0; c < 2; ++c) dirichlet(rate[c], s->nbc[c], s->bc_dofs[c], nullptr);
+        orc_dr_update(s, alpha, beta);
+        for (int c = 0; c < 2; ++c) dirichlet(v[c], s->nbc[c], s->bc_dofs[c], s->bc_vals[c]);
+    }
+}
+
 // solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802).
 int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
     std::memset(out, 0, sizeof *out);

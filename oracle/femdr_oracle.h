@@ -159,6 +159,9 @@ void orc_momentum_residual(orc_stokes* s, double dt, const double* Pnum /*nullab
 void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K);
 void orc_dr_update(orc_stokes* s, const double* alpha /*2*/, const double* beta /*2*/);
 
+void orc_dr_iterations(orc_stokes* s, double dt, int32_t finite_K, const double* alpha, const double* beta,
+                       int64_t k);
+
 /* --- full solve (src/stokes/solvers/DR.jl:302-337, 348-802) --- */
 int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out);
 /* update_stokes_current_stress! (DR.jl:820-897): tau_store must be set */

@@ -98,6 +98,7 @@ def _declare(L):
     L.orc_stokes_solve.argtypes = [C.POINTER(Stokes), C.POINTER(Params), C.POINTER(Stats)]
     L.orc_stokes_solve.restype = C.c_int
     L.orc_update_current_stress.argtypes = [C.POINTER(Stokes), D]
+    L.orc_dr_iterations.argtypes = [C.POINTER(Stokes), D, I32, PD, PD, I64]
     L.orc_set_threads.argtypes = [C.c_int]
     L.orc_num_threads.restype = C.c_int
 
@@ -282,6 +283,11 @@ class Problem:
 
     def augmented_jacobian(self, dt, finite_K=False):
         lib().orc_augmented_jacobian(C.byref(self.struct()), dt, int(finite_K))
+
+    def dr_iterations(self, dt, alpha, beta, k, finite_K=False):
+        a = f64(alpha)
+        b = f64(beta)
+        lib().orc_dr_iterations(C.byref(self.struct()), dt, int(finite_K), ptr(a), ptr(b), int(k))
 
     def update_current_stress(self, dt):
         lib().orc_update_current_stress(C.byref(self.struct()), dt)

@@ -1,0 +1,281 @@
+"""GPU parity: libfemdr.so (through its C ABI, via femtools_amd) against the
+CPU oracle on identical seeded inputs. FP64 tolerances: element-level
+quantities 1e-12 relative to the field's scale, converged solve fields
+1e-10 relative (the bar stated for the FP64 GPU path)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+fa = pytest.importorskip("femtools_amd")
+from femtools_amd import problems as PB  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, float).ravel()
+    b = np.asarray(b, float).ravel()
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def make_pair(nx, ny, eta, G=None, rho0=None, g=(0.0, 0.0), etab=None, K=None, dp=None, phases=None,
+              nodal_phases=None, tau_old=None, box=(0.0, 1.0, 0.0, 1.0), bc=None, seed=0, random_state=True):
+    nph = len(eta)
+    inf = math.inf
+    G = G or (inf,) * nph
+    rho0 = rho0 or (1.0,) * nph
+    etab = etab or (inf,) * nph
+    K = K or (inf,) * nph
+    coords, el2n = O.t7_mesh(nx, ny, box)
+    mesh = fa.t7_box_mesh(nx, ny, ((box[0], box[1]), (box[2], box[3])))
+    assert np.array_equal(mesh.coords, coords)
+    assert np.array_equal(mesh.el2n, el2n)
+    mat_o = O.material(eta=eta, etab=etab, G=G, rho0=rho0, K=K, g=g)
+    mat_g = fa.StokesMaterial(eta=eta, etab=etab, G=G, rho0=rho0, K=K, g=g)
+    dp_o = O.drucker_prager(**dp) if dp else None
+    dp_g = fa.DruckerPrager(dp["phi"], dp["psi"], dp["C_"], dp["eta_reg"], dp["Kb"]) if dp else None
+    ne = len(el2n)
+    if nodal_phases is not None:
+        pr = O.Problem(coords, el2n, mat_o, dp=dp_o, phases_v=nodal_phases, phase_mode_v=2,
+                       phases_P=np.repeat(phases, 3), phase_mode_P=2, tau_old=tau_old, bc=bc or O.Problem.bc)
+    else:
+        ph = phases if phases is not None else np.ones(ne, np.int32)
+        pr = O.Problem(coords, el2n, mat_o, dp=dp_o, phases_v=ph, phases_P=ph, tau_old=tau_old,
+                       bc=bc or O.Problem.bc)
+    dr = fa.StokesDR(mesh, mat_g)
+    if nodal_phases is not None:
+        dr.set_phases("v", nodal_phases)
+        dr.set_phases("P", np.repeat(phases, 3))
+    elif phases is not None:
+        dr.set_phases("v", phases)
+        dr.set_phases("P", phases)
+    if tau_old is not None:
+        dr.tau_old.xx.copyto(tau_old[0])
+        dr.tau_old.yy.copyto(tau_old[1])
+        dr.tau_old.xy.copyto(tau_old[2])
+    if random_state:
+        rng = np.random.default_rng(seed)
+        pr.vx[:] = np.sin(3 * coords[:, 0]) * coords[:, 1] + 0.1 * rng.normal(size=len(coords))
+        pr.vy[:] = coords[:, 0] * np.cos(2 * coords[:, 1]) + 0.1 * rng.normal(size=len(coords))
+        pr.P[:] = rng.normal(size=3 * ne)
+        dr.v.x.copyto(pr.vx)
+        dr.v.y.copyto(pr.vy)
+        dr.P.copyto(pr.P)
+    return pr, dr, mesh, dp_o, dp_g
+
+
+def rand_multiphase(nph, seed):
+    rng = np.random.default_rng(seed)
+    eta = tuple(10 ** rng.uniform(-1, 2, nph))
+    return eta, rng
+
+
+@pytest.mark.parametrize("case", ["viscous", "elastic", "plastic", "nodal_phases"])
+def test_residual_parity(case):
+    say(f"[gpu] residual parity case={case}")
+    nph = 3
+    eta, rng = rand_multiphase(nph, 11)
+    nx, ny = 6, 5
+    ne = 2 * nx * ny
+    phases = rng.integers(1, nph + 1, ne).astype(np.int32)
+    kw = dict(g=(0.3, -1.0), rho0=(1.0, 1.2, 0.9))
+    if case in ("elastic", "plastic", "nodal_phases"):
+        kw["G"] = (1.0, 2.0, 0.5)
+        kw["tau_old"] = tuple(rng.normal(size=(ne, 7)) * 0.5 for _ in range(3))
+    if case in ("plastic", "nodal_phases"):
+        kw["dp"] = dict(phi=(0.5, 0.4, 0.6), psi=(0.0, 0.1, 0.0), C_=(0.05, 0.1, 0.02), eta_reg=(1e-2,) * 3,
+                        Kb=(10.0,) * 3)
+    nodal = None
+    if case == "nodal_phases":
+        coords, _ = O.t7_mesh(nx, ny)
+        nodal = rng.integers(1, nph + 1, len(coords)).astype(np.int32)
+    pr, dr, mesh, dp_o, dp_g = make_pair(nx, ny, eta, phases=phases, nodal_phases=nodal, **kw)
+    dt = 0.3
+    pr.pressure_scaling(5.0, dt)
+    fa.assemble_viscosity_weighted_pressure_scaling(dr, 5.0, dt)
+    assert rel_err(dr.M_P.numpy(), pr.MP) < 1e-13
+    assert rel_err(dr.gammaP.numpy(), pr.gammaP) < 1e-12
+    for with_pnum in (False, True):
+        fa.residual(dr, dt, plastic=dp_g, with_pnum=with_pnum)
+        pr.pressure_residual(dt)
+        Pn = pr.gammaP * pr.RP / pr.MP if with_pnum else None
+        pr.momentum_residual(dt, Pn)
+        e_rp = rel_err(dr.RP.numpy(), pr.RP)
+        e_x = rel_err(dr.Rv.x.numpy(), pr.Rvx)
+        e_y = rel_err(dr.Rv.y.numpy(), pr.Rvy)
+        say(f"  with_pnum={with_pnum}: RP {e_rp:.2e} Rvx {e_x:.2e} Rvy {e_y:.2e}")
+        assert e_rp < 1e-12 and e_x < 1e-12 and e_y < 1e-12
+
+
+@pytest.mark.parametrize("case", ["viscous", "plastic"])
+def test_jacobian_parity(case):
+    say(f"[gpu] jacobian parity case={case}")
+    nph = 2
+    eta, rng = rand_multiphase(nph, 5)
+    nx, ny = 5, 4
+    ne = 2 * nx * ny
+    phases = rng.integers(1, nph + 1, ne).astype(np.int32)
+    kw = {}
+    if case == "plastic":
+        kw = dict(G=(1.0, 0.7), tau_old=tuple(rng.normal(size=(ne, 7)) * 0.3 for _ in range(3)),
+                  dp=dict(phi=(0.5, 0.4), psi=(0.0, 0.1), C_=(0.05, 0.02), eta_reg=(1e-2,) * 2, Kb=(10.0,) * 2))
+    pr, dr, mesh, dp_o, dp_g = make_pair(nx, ny, eta, phases=phases, **kw)
+    dt = 0.5
+    pr.pressure_scaling(4.0, dt)
+    fa.assemble_viscosity_weighted_pressure_scaling(dr, 4.0, dt)
+    pr.augmented_jacobian(dt)
+    fa.jacobian_diagnostics(dr, dt, plastic=dp_g)
+    errs = [rel_err(dr.dRvdv.x.numpy(), pr.dRdvx), rel_err(dr.dRvdv.y.numpy(), pr.dRdvy),
+            rel_err(dr.PC_v.x.numpy(), pr.PCx), rel_err(dr.PC_v.y.numpy(), pr.PCy)]
+    say("  errs", ["%.2e" % e for e in errs])
+    assert max(errs) < 1e-12
+
+
+def _viscous_pair(n):
+    w = PB.viscous_inclusion(n)
+    coords, el2n = O.t7_mesh(n, n)
+    ph = PB.inclusion_phases(coords, el2n, w.circles)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(coords)
+    pr, dr, mesh, _, _ = make_pair(n, n, w.eta, G=w.G, rho0=w.rho0, etab=w.etab, K=w.K, g=w.g, phases=ph,
+                                   bc=((bx, vx), (by, vy)), random_state=False)
+    return w, pr, dr, mesh, (bx, vx, by, vy)
+
+
+def test_solve_parity_viscous_inclusion():
+    # reduced viscous-inclusion workload (16x16), FP64, tolerance 1e-8
+    n = 16
+    say(f"[gpu] solve parity viscous inclusion n={n}")
+    w, pr, dr, mesh, (bx, vx, by, vy) = _viscous_pair(n)
+    st_o = pr.solve(w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8, check_interval=w.check_interval)
+    st_g = fa.solve(dr, mesh, fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy),
+                    dt=w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8, check_interval=w.check_interval)
+    say("  oracle", {k: st_o[k] for k in ("iter", "itPH", "err_abs")}, "gpu", {k: st_g[k] for k in ("iter", "itPH", "err_abs")})
+    assert st_g["converged"] and st_o["converged"]
+    assert st_g["iter"] == st_o["iter"] and st_g["itPH"] == st_o["itPH"]
+    for a, b in ((dr.v.x, pr.vx), (dr.v.y, pr.vy), (dr.P, pr.P)):
+        assert rel_err(a.numpy(), b) < 1e-10
+
+
+def test_solve_parity_vep_two_steps():
+    # reduced VEP workload: 12x12, 4 inclusions, two time steps with the
+    # stress-history hand-off of the pure-shear example (DR.jl:820-897), low
+    # cohesion so the Drucker-Prager return is active.
+    n = 12
+    say(f"[gpu] solve parity VEP n={n}")
+    w = PB.vep_inclusions(n, ninc=4, seed=3, nsteps=2)
+    w.plastic["C"] = (0.15,) * w.nphases
+    coords, el2n = O.t7_mesh(n, n)
+    ph = PB.inclusion_phases(coords, el2n, w.circles)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(coords)
+    ne = len(el2n)
+    zero = tuple(np.zeros((ne, 7)) for _ in range(3))
+    dp = dict(phi=w.plastic["phi"], psi=w.plastic["psi"], C_=w.plastic["C"], eta_reg=w.plastic["eta_reg"],
+              Kb=w.plastic["Kb"])
+    pr, dr, mesh, dp_o, dp_g = make_pair(n, n, w.eta, G=w.G, rho0=w.rho0, etab=w.etab, K=w.K, g=w.g, dp=dp,
+                                         phases=ph, tau_old=zero, bc=((bx, vx), (by, vy)), random_state=False)
+    bcx, bcy = fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)
+    tau_o = [np.zeros((ne, 7)) for _ in range(3)]
+    for step in range(w.nsteps):
+        pr.arrays["P0"][:] = pr.P
+        pr.tau_old = tuple(t.copy() for t in tau_o)
+        dr.P0.copyto(dr.P.numpy())
+        st_o = pr.solve(w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8, check_interval=w.check_interval)
+        st_g = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8,
+                        check_interval=w.check_interval, plastic=dp_g)
+        say(f"  step {step}: oracle iter {st_o['iter']} jac {st_o['jacobian_assemblies']} | gpu iter {st_g['iter']} "
+            f"jac {st_g['jacobian_assemblies']}")
+        assert st_g["converged"] and st_o["converged"]
+        assert st_g["iter"] == st_o["iter"]
+        for a, b in ((dr.v.x, pr.vx), (dr.v.y, pr.vy), (dr.P, pr.P)):
+            assert rel_err(a.numpy(), b) < 1e-10
+        tau_o = pr.update_current_stress(w.dt)
+        tau_o = [t.copy() for t in tau_o]
+        fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp_g)
+        for c, comp in enumerate(("xx", "yy", "xy")):
+            assert rel_err(getattr(dr.tau, comp).numpy(), tau_o[c]) < 1e-10
+            getattr(dr.tau_old, comp).copyto(getattr(dr.tau, comp).numpy())
+
+
+def test_solve_run_to_run_bitwise_deterministic():
+    n = 10
+    say(f"[gpu] determinism n={n}")
+    outs = []
+    for _ in range(2):
+        w, pr, dr, mesh, (bx, vx, by, vy) = _viscous_pair(n)
+        fa.solve(dr, mesh, fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy), dt=1.0,
+                 tolerance=1e-8)
+        outs.append((dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_gpu_uniform_extension_closed_form():
+    # test/test_solver_convergence_api.jl:330-409 on the GPU
+    Lx, Ly, dt, edot, etab = 1.0, 0.7, 1.0, 1.0e-5, 6.4e3
+    say("[gpu] uniform extension closed form")
+    mesh = fa.t7_box_mesh(6, 4, ((0.0, Lx), (0.0, Ly)))
+    coords = mesh.coords
+    gam = mesh.boundary_nodes
+    tol = 32 * np.finfo(float).eps
+    left = gam[np.abs(coords[gam, 0]) <= tol]
+    right = gam[np.abs(coords[gam, 0] - Lx) <= tol]
+    bcx = fa.DirichletBoundaryCondition(np.concatenate([left, right]),
+                                        np.concatenate([np.full(len(left), -0.5 * edot * Lx),
+                                                        np.full(len(right), 0.5 * edot * Lx)]))
+    bcy = fa.DirichletBoundaryCondition(gam, np.zeros(len(gam)))
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=(1e30,), etab=(etab,), G=(4e3,), K=(6.4e3,)))
+    v = edot * (coords[:, 0] - Lx / 2)
+    v[bcx.DoFs] = bcx.vals
+    dr.v.x.copyto(v)
+    fa.assemble_viscosity_weighted_pressure_scaling(dr, 20.0, dt)
+    st = fa.solve(dr, mesh, bcx, bcy, dt=dt, tolerance=1e-6, check_interval=25, iterMax=500, skip_scaling=True)
+    assert st["converged"] and st["iter"] <= 100
+    np.testing.assert_allclose(dr.P.numpy(), -etab * dt * edot, rtol=1e-6)
+    assert np.abs(dr.v.y.numpy()).max() < 1e-12
+
+
+def test_gpu_hydrostatic_closed_form():
+    # test/test_stokes_quadrature_viscosity.jl:68-87 on the GPU
+    say("[gpu] hydrostatic closed form")
+    mesh = fa.t7_box_mesh(2, 2)
+    gam = mesh.boundary_nodes
+    bc = fa.DirichletBoundaryCondition(gam, np.zeros(len(gam)))
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=(2.0,), etab=(math.inf,), rho0=(1.0,), g=(0.0, 1.0)))
+    st = fa.solve(dr, mesh, bc, bc, dt=1.0, pressure_factor=50.0, tolerance=1e-11, check_interval=10)
+    assert st["converged"]
+    P, MP = dr.P.numpy(), dr.M_P.numpy()
+    P = P - np.sum(P * MP) / np.sum(MP)
+    pc = mesh.coords[mesh.el2n[:, :3].ravel()]
+    np.testing.assert_allclose(P, pc[:, 1] - 0.5, atol=1e-7)
+    assert np.abs(dr.v.x.numpy()).max() < 1e-7
+
+
+def test_gpu_error_paths():
+    say("[gpu] error paths")
+    # orphan node -> zero preconditioner -> invalid λmax (utils.jl:152-158)
+    coords, el2n = O.t7_mesh(1, 1)
+    coords = np.vstack([coords, [[2.0, 2.0]]])
+    mesh = fa.t7_mesh_from_arrays(coords, el2n)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=(1.0,), etab=(1.0,)))
+    bc = fa.DirichletBoundaryCondition([], [])
+    with pytest.raises(fa.FemError, match="stokes vx preconditioner produced invalid"):
+        fa.solve(dr, mesh, bc, bc, dt=1.0, max_iterations=2)
+    # non-convergence: throw_on_failure semantics (DR.jl:332-335)
+    w, pr, dr, mesh, (bx, vx, by, vy) = _viscous_pair(6)
+    bcx, bcy = fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)
+    with pytest.raises(fa.FemError, match="did not converge"):
+        fa.solve(dr, mesh, bcx, bcy, dt=1.0, check_interval=1, max_iterations=2)
+    st = fa.solve(dr, mesh, bcx, bcy, dt=1.0, check_interval=1, max_iterations=2, throw_on_failure=False)
+    assert not st["converged"] and st["reached_total_iter"]
+    with pytest.raises(ValueError):
+        fa.solve(dr, mesh, bcx, bcy, dt=-1.0)
