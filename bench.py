@@ -4,8 +4,10 @@
 Workload: 2-D visco-elasto-plastic Stokes, 512x512 cells -> 524,288 T7/P1-disc
 elements, 20 seeded random circular inclusions (r in U[0.01,0.05],
 log10 eta in U[-2,4]), G=1, dt=0.1, Drucker-Prager C=1, phi=30 deg, psi=0,
-densities 1 / 1.1, g=(0,-1), pure shear; each physical time step solved to
-residual 1e-8 (FP64), stress history handed over between steps.
+K = etab = Kb = 100, densities 1 / 1.1, g=(0,-1), pure shear; solver keywords
+of the reference's VEP examples (pressure_factor 20, rel_drop0 0.75,
+check_interval 100, CFL_v 0.99); each physical time step solved to residual
+1e-8 (FP64), stress history handed over between steps.
 
 A "step" is one physical time step (one solve! to 1e-8 plus the stress
 refresh). The warmup steps are the first steps of the same time series.
@@ -78,6 +80,7 @@ def cpu_baseline(w, budget_s=15.0):
     pr = O.Problem(coords, el2n, mat, dp=dp, phases_v=ph, phases_P=ph,
                    tau_old=tuple(np.zeros((ne, 7)) for _ in range(3)), bc=((bx, vx), (by, vy)))
     pr.pressure_scaling(w.pressure_factor, w.dt)
+    pr.vx[:], pr.vy[:] = PB.pure_shear_field(coords)
     pr.PCx[:] = 1.0
     pr.PCy[:] = 1.0
     al, be = [1e-3, 1e-3], [0.9, 0.9]
@@ -121,15 +124,21 @@ def main():
     ph = PB.inclusion_phases(mesh.coords, mesh.el2n, w.circles)
     (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
     bcx, bcy = fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)
-    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g),
+                     CFL_v=w.CFL_v)
+    v0x, v0y = PB.pure_shear_field(mesh.coords)
+    dr.v.x.copyto(v0x)
+    dr.v.y.copyto(v0y)
     dr.set_phases("v", ph)
     dr.set_phases("P", ph)
     dp = fa.DruckerPrager(w.plastic["phi"], w.plastic["psi"], w.plastic["C"], w.plastic["eta_reg"], w.plastic["Kb"])
 
     def time_step(timed):
         dr.P0.copyto(dr.P.numpy())
-        st = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, tolerance=args.tolerance, plastic=dp,
-                      max_iterations=args.max_iterations, throw_on_failure=False, time_kernels=timed)
+        kw = w.solve_kwargs()
+        kw["tolerance"] = args.tolerance
+        st = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, plastic=dp, max_iterations=args.max_iterations,
+                      iterMax=args.max_iterations, throw_on_failure=False, time_kernels=timed, **kw)
         fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp)
         for c in ("xx", "yy", "xy"):
             getattr(dr.tau_old, c).copyto(getattr(dr.tau, c).numpy())
@@ -172,7 +181,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded random inclusions, generated in-process)",
-        "config": {"workload": w.name + f", dt={w.dt}, DP C=1 phi=30deg psi=0 eta_reg=8e-3 Kb=100, "
+        "config": {"workload": w.name + f", dt={w.dt}, DP C=1 phi=30deg psi=0 eta_reg=8e-3, K=etab=Kb=100, "
                                         f"pure shear, tol {args.tolerance:g}, FP64",
                    "cells": [w.nx, w.ny], "elements": ne, "velocity_nodes": nv, "pressure_dofs": npp,
                    "dofs": w.dofs(), "time_steps_timed": args.steps, "parallelism": "single GPU"},

@@ -8,8 +8,13 @@ same problem can be handed to the GPU library and to a CPU checker.
   (examples/miniapps/stokes/stokes_2D_pure_shear_triangle/...jl:40-154).
 * visco-elasto-plastic inclusions: seeded random circular inclusions
   (r ∈ U[0.01, 0.05], log10 η ∈ U[-2, 4], later inclusion wins), shear
-  modulus G=1 and Δt = 0.1 η_matrix/G, Drucker-Prager C=1, ϕ=30°, Ψ=0 with a
-  finite Kb, densities 1 (matrix) / 1.1 (inclusions), g=(0,-1), pure shear.
+  modulus G=1 and Δt = 0.1 η_matrix/G, Drucker-Prager C=1, ϕ=30°, Ψ=0,
+  densities 1 (matrix) / 1.1 (inclusions), g=(0,-1), pure shear. The bulk
+  modulus, pressure storage modulus and Kb are 100 and the solver keywords
+  are those of the reference's VEP examples (γfact=20, rel_drop0=0.75,
+  check_interval=100, CFL_v=0.99; examples/miniapps/stokes/vevp/
+  stokes_2D_shear_bands_triangle.jl:48-79, 199-214); the initial velocity is
+  the background pure-shear field, as there (:189-195).
 
 Phases are per element (1 x nels layout), taken at the element centroid.
 """
@@ -73,8 +78,15 @@ class Workload:
     tolerance: float = 1e-8
     check_interval: int = 50
     pressure_factor: float = 50.0
+    rel_drop0: float = 1e-2
+    CFL_v: float = 0.98
     nsteps: int = 1
+    seed_pure_shear: bool = False
     extra: dict = field(default_factory=dict)
+
+    def solve_kwargs(self):
+        return dict(tolerance=self.tolerance, check_interval=self.check_interval,
+                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0)
 
     @property
     def nphases(self):
@@ -98,7 +110,13 @@ def viscous_inclusion(n=64) -> Workload:
                     circles=[(0.5, 0.5, 0.1)])
 
 
-def vep_inclusions(n=512, ninc=20, seed=0, nsteps=10, eta_reg=8e-3, Kb=100.0) -> Workload:
+def pure_shear_field(coords, box=(0.0, 1.0, 0.0, 1.0), edot=1.0):
+    """Background pure-shear velocity (vx, vy) = ε̇ (x - xc, -(y - yc))."""
+    xm, ym = 0.5 * (box[0] + box[1]), 0.5 * (box[2] + box[3])
+    return edot * (coords[:, 0] - xm), -edot * (coords[:, 1] - ym)
+
+
+def vep_inclusions(n=512, ninc=20, seed=0, nsteps=10, eta_reg=8e-3, Kb=100.0, K=100.0, C=1.0) -> Workload:
     """Visco-elasto-plastic multi-inclusion workload on an n x n cell grid."""
     rng = np.random.default_rng(seed)
     cx = rng.uniform(0, 1, ninc)
@@ -109,9 +127,10 @@ def vep_inclusions(n=512, ninc=20, seed=0, nsteps=10, eta_reg=8e-3, Kb=100.0) ->
     nph = ninc + 1
     eta = (1.0,) + tuple(10.0 ** leta)
     G = 1.0
-    return Workload(name=f"T7/P1-disc {n}x{n} VEP {ninc} inclusions", nx=n, ny=n, eta=eta, etab=(inf,) * nph,
-                    G=(G,) * nph, rho0=(1.0,) + (1.1,) * ninc, K=(inf,) * nph, g=(0.0, -1.0), dt=0.1 * 1.0 / G,
+    return Workload(name=f"T7/P1-disc {n}x{n} VEP {ninc} inclusions", nx=n, ny=n, eta=eta, etab=(K,) * nph,
+                    G=(G,) * nph, rho0=(1.0,) + (1.1,) * ninc, K=(K,) * nph, g=(0.0, -1.0), dt=0.1 * 1.0 / G,
                     circles=list(zip(cx, cy, r)),
-                    plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(1.0,) * nph,
+                    plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(C,) * nph,
                                  eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph),
-                    nsteps=nsteps, extra=dict(seed=seed))
+                    pressure_factor=20.0, rel_drop0=0.75, check_interval=100, CFL_v=0.99,
+                    nsteps=nsteps, seed_pure_shear=True, extra=dict(seed=seed))

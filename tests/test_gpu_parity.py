@@ -29,7 +29,8 @@ def rel_err(a, b):
 
 
 def make_pair(nx, ny, eta, G=None, rho0=None, g=(0.0, 0.0), etab=None, K=None, dp=None, phases=None,
-              nodal_phases=None, tau_old=None, box=(0.0, 1.0, 0.0, 1.0), bc=None, seed=0, random_state=True):
+              nodal_phases=None, tau_old=None, box=(0.0, 1.0, 0.0, 1.0), bc=None, seed=0, random_state=True,
+              CFL_v=0.98):
     nph = len(eta)
     inf = math.inf
     G = G or (inf,) * nph
@@ -52,7 +53,7 @@ def make_pair(nx, ny, eta, G=None, rho0=None, g=(0.0, 0.0), etab=None, K=None, d
         ph = phases if phases is not None else np.ones(ne, np.int32)
         pr = O.Problem(coords, el2n, mat_o, dp=dp_o, phases_v=ph, phases_P=ph, tau_old=tau_old,
                        bc=bc or O.Problem.bc)
-    dr = fa.StokesDR(mesh, mat_g)
+    dr = fa.StokesDR(mesh, mat_g, CFL_v=CFL_v)
     if nodal_phases is not None:
         dr.set_phases("v", nodal_phases)
         dr.set_phases("P", np.repeat(phases, 3))
@@ -166,14 +167,13 @@ def test_solve_parity_viscous_inclusion():
         assert rel_err(a.numpy(), b) < 1e-10
 
 
-def test_solve_parity_vep_two_steps():
-    # reduced VEP workload: 12x12, 4 inclusions, two time steps with the
-    # stress-history hand-off of the pure-shear example (DR.jl:820-897), low
-    # cohesion so the Drucker-Prager return is active.
+def test_solve_parity_vep_time_steps():
+    # reduced VEP workload: 12x12, 4 inclusions, three time steps with the
+    # stress-history hand-off of the reference's VEP examples (DR.jl:820-897);
+    # cohesion 0.5 so the Drucker-Prager return is active in the third step.
     n = 12
     say(f"[gpu] solve parity VEP n={n}")
-    w = PB.vep_inclusions(n, ninc=4, seed=3, nsteps=2)
-    w.plastic["C"] = (0.15,) * w.nphases
+    w = PB.vep_inclusions(n, ninc=4, seed=3, nsteps=3, C=0.5)
     coords, el2n = O.t7_mesh(n, n)
     ph = PB.inclusion_phases(coords, el2n, w.circles)
     (bx, vx), (by, vy) = PB.pure_shear_bc(coords)
@@ -182,28 +182,38 @@ def test_solve_parity_vep_two_steps():
     dp = dict(phi=w.plastic["phi"], psi=w.plastic["psi"], C_=w.plastic["C"], eta_reg=w.plastic["eta_reg"],
               Kb=w.plastic["Kb"])
     pr, dr, mesh, dp_o, dp_g = make_pair(n, n, w.eta, G=w.G, rho0=w.rho0, etab=w.etab, K=w.K, g=w.g, dp=dp,
-                                         phases=ph, tau_old=zero, bc=((bx, vx), (by, vy)), random_state=False)
+                                         phases=ph, tau_old=zero, bc=((bx, vx), (by, vy)), random_state=False,
+                                         CFL_v=w.CFL_v)
+    v0x, v0y = PB.pure_shear_field(coords)
+    pr.vx[:], pr.vy[:] = v0x, v0y
+    dr.v.x.copyto(v0x)
+    dr.v.y.copyto(v0y)
     bcx, bcy = fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)
     tau_o = [np.zeros((ne, 7)) for _ in range(3)]
+    yielded = False
     for step in range(w.nsteps):
         pr.arrays["P0"][:] = pr.P
         pr.tau_old = tuple(t.copy() for t in tau_o)
         dr.P0.copyto(dr.P.numpy())
-        st_o = pr.solve(w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8, check_interval=w.check_interval)
-        st_g = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, pressure_factor=w.pressure_factor, tolerance=1e-8,
-                        check_interval=w.check_interval, plastic=dp_g)
+        st_o = pr.solve(w.dt, CFL_v=w.CFL_v, **w.solve_kwargs())
+        st_g = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, plastic=dp_g, **w.solve_kwargs())
         say(f"  step {step}: oracle iter {st_o['iter']} jac {st_o['jacobian_assemblies']} | gpu iter {st_g['iter']} "
             f"jac {st_g['jacobian_assemblies']}")
         assert st_g["converged"] and st_o["converged"]
         assert st_g["iter"] == st_o["iter"]
         for a, b in ((dr.v.x, pr.vx), (dr.v.y, pr.vy), (dr.P, pr.P)):
             assert rel_err(a.numpy(), b) < 1e-10
-        tau_o = pr.update_current_stress(w.dt)
-        tau_o = [t.copy() for t in tau_o]
+        tau_o = [t.copy() for t in pr.update_current_stress(w.dt)]
         fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp_g)
+        # τ is a derived field: strain rates difference the velocity over an
+        # element, so the solved fields' 1e-10 agreement is held to 1e-9 here
         for c, comp in enumerate(("xx", "yy", "xy")):
-            assert rel_err(getattr(dr.tau, comp).numpy(), tau_o[c]) < 1e-10
+            assert rel_err(getattr(dr.tau, comp).numpy(), tau_o[c]) < 1e-9
             getattr(dr.tau_old, comp).copyto(getattr(dr.tau, comp).numpy())
+        t = tau_o
+        tII = np.sqrt((t[0] ** 2 + t[1] ** 2 + (t[0] + t[1]) ** 2) / 2 + t[2] ** 2)
+        yielded = yielded or tII.max() > 0.5 * math.cos(math.radians(30))
+    assert yielded
 
 
 def test_solve_run_to_run_bitwise_deterministic():
@@ -271,8 +281,13 @@ def test_gpu_error_paths():
     with pytest.raises(fa.FemError, match="stokes vx preconditioner produced invalid"):
         fa.solve(dr, mesh, bc, bc, dt=1.0, max_iterations=2)
     # non-convergence: throw_on_failure semantics (DR.jl:332-335)
+    # (v carries the boundary values, as apply_bc! leaves it, so the residual is nonzero)
     w, pr, dr, mesh, (bx, vx, by, vy) = _viscous_pair(6)
     bcx, bcy = fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)
+    v0x, v0y = np.zeros(mesh.nnodes), np.zeros(mesh.nnodes)
+    v0x[bx], v0y[by] = vx, vy
+    dr.v.x.copyto(v0x)
+    dr.v.y.copyto(v0y)
     with pytest.raises(fa.FemError, match="did not converge"):
         fa.solve(dr, mesh, bcx, bcy, dt=1.0, check_interval=1, max_iterations=2)
     st = fa.solve(dr, mesh, bcx, bcy, dt=1.0, check_interval=1, max_iterations=2, throw_on_failure=False)
