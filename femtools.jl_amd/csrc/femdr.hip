@@ -22,9 +22,6 @@
 #include <string>
 #include <vector>
 
-namespace fem {
-__constant__ Tables c_tab;
-}
 
 using namespace fem;
 
@@ -89,6 +86,7 @@ struct fem_mesh {
     int32_t* d_off = nullptr;          // [ns+1]
     int32_t* d_ent = nullptr;
     int32_t* d_node_of = nullptr;      // nullable
+    int32_t nx = 0, ny = 0;            // structured box mesh cell counts
 };
 
 struct fem_stokes {
@@ -101,6 +99,7 @@ struct fem_stokes {
     int32_t* d_phase_v = nullptr;   // cell [nels] or element-node [7][nels], 0-based
     int32_t* d_phase_P = nullptr;   // cell [nels] or element-node [3][nels]
     int32_t layout_v = FEM_PHASE_CELL, layout_P = FEM_PHASE_CELL;
+    bool phase_same = true;         // cell phases of v and P identical
     double* d_pscale = nullptr;
     double2* d_Re = nullptr;
     double2* d_Jrow = nullptr;
@@ -112,6 +111,9 @@ struct fem_stokes {
     double* d_bc_stage_v = nullptr;
     int64_t bc_stage_cap = 0;
     double* d_ptab = nullptr;
+    int32_t* d_flag = nullptr;      // scratch yes/no flag
+    bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
+    bool q_nonzero = false;         // dr.Q holds a nonzero entry
 };
 
 // ---------------------------------------------------------------------------
@@ -185,31 +187,6 @@ static void t7_basis(double xi, double eta, double N[7], double dN[7][2]) {
     dN[6][0] = bx;                   dN[6][1] = by;
 }
 
-static Tables host_tables() {
-    Tables t;
-    const double a1 = 0.101286507323456, a2 = 0.470142064105115;
-    const double b1 = 1 - 2 * a1, b2 = 1 - 2 * a2;
-    const double w1 = 0.062969590272414, w2 = 0.066197076394253, wc = 0.1125;
-    const double X[7] = {a1, b1, a1, a2, b2, a2, 1.0 / 3};
-    const double Y[7] = {a1, a1, b1, a2, a2, b2, 1.0 / 3};
-    const double W[7] = {w1, w1, w1, w2, w2, w2, wc};
-    for (int q = 0; q < NQ; ++q) {
-        t7_basis(X[q], Y[q], t.N[q], t.dN[q]);
-        t.NP1[q][0] = 1 - X[q] - Y[q];
-        t.NP1[q][1] = X[q];
-        t.NP1[q][2] = Y[q];
-        t.w[q] = W[q];
-    }
-    for (int j = 0; j < NP; ++j)
-        for (int b = 0; b < NV; ++b)
-            for (int k = 0; k < 2; ++k) {
-                double s = 0;
-                for (int q = 0; q < NQ; ++q) s += t.w[q] * t.NP1[q][j] * t.dN[q][b][k];
-                t.TB[j][b][k] = s;
-            }
-    return t;
-}
-
 template <class T>
 static int fill_t(fem_ctx* c, T* p, T v, int64_t n) {
     if (!c || n < 0) return FEM_EARG;
@@ -233,8 +210,6 @@ int fem_ctx_create(int device, fem_ctx** out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete c; return FEM_EHIP; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return FEM_EHIP; }
-    Tables t = host_tables();
-    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof t) != hipSuccess) { delete c; return FEM_EHIP; }
     if (dalloc_t(c, &c->red_partial, RED_GRID * RED_MAXK) || dalloc_t(c, &c->red_out, RED_MAXK) ||
         hipHostMalloc((void**)&c->h_red, RED_MAXK * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -445,6 +420,7 @@ int fem_mesh_t7_create(fem_ctx* c, int32_t nx, int32_t ny, double x0, double x1,
     m->ctx = c;
     m->structured = true;
     m->box[0] = x0; m->box[1] = x1; m->box[2] = y0; m->box[3] = y1;
+    m->nx = nx; m->ny = ny;
     gen_t7(nx, ny, x0, x1, y0, y1, m->h_coords, m->h_el2n);
     m->nnodes = (int64_t)m->h_coords.size() / 2;
     m->nels = (int64_t)m->h_el2n.size() / 7;
@@ -527,16 +503,19 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
     for (int k = 0; k < FEM_FIELD_COUNT; ++k) {
         int64_t n = k <= FEM_FIELD_PC_V_Y ? nv : (k <= FEM_FIELD_TAU_OLD_XY ? nip : np);
         s->len[k] = n;
-        int r = dalloc_t(c, &s->f[k], n);
+        // +16 entries: 16-byte global->LDS loads of the last block may read
+        // 8 bytes past the final element of an integration-point array
+        int r = dalloc_t(c, &s->f[k], n + 16);
         if (r != FEM_OK) { fem_stokes_destroy(s); return r; }
-        HIPCHK(c, hipMemset(s->f[k], 0, n * sizeof(double)));
+        HIPCHK(c, hipMemset(s->f[k], 0, (n + 16) * sizeof(double)));
     }
     int r = FEM_OK;
     if ((r = dalloc_t(c, &s->d_phase_v, 7 * m->nels)) || (r = dalloc_t(c, &s->d_phase_P, 3 * m->nels)) ||
         (r = dalloc_t(c, &s->d_pscale, np)) || (r = dalloc_t(c, &s->d_Re, NSH * m->nels)) ||
         (r = dalloc_t(c, &s->d_Jrow, NSH * m->nels)) || (r = dalloc_t(c, &s->d_Jdiag, NSH * m->nels)) ||
         (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
-        (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH))) {
+        (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH)) ||
+        (r = dalloc_t(c, &s->d_flag, 1))) {
         fem_stokes_destroy(s);
         return r;
     }
@@ -555,6 +534,7 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipFree(s->d_phase_v); hipFree(s->d_phase_P); hipFree(s->d_pscale); hipFree(s->d_Re);
     hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
+    hipFree(s->d_flag);
     delete s;
     return FEM_OK;
 }
@@ -602,6 +582,11 @@ int fem_stokes_set_phases(fem_stokes* s, int32_t which, int32_t layout, const in
     int32_t* d = which == 0 ? s->d_phase_v : s->d_phase_P;
     HIPCHK(c, hipMemcpy(d, loc.data(), (size_t)nloc * ne * 4, hipMemcpyHostToDevice));
     (which == 0 ? s->layout_v : s->layout_P) = cell ? FEM_PHASE_CELL : FEM_PHASE_ELEMENT;
+    // compare the cell rows of both phase arrays (device copies)
+    std::vector<int32_t> pv(ne), pp(ne);
+    HIPCHK(c, hipMemcpy(pv.data(), s->d_phase_v, ne * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pp.data(), s->d_phase_P, ne * 4, hipMemcpyDeviceToHost));
+    s->phase_same = pv == pp;
     return FEM_OK;
 }
 
@@ -630,6 +615,9 @@ static int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, i
         t[PC_ETAVE * MAXPH + p] = 1.0 / (1.0 / m.eta[p] + invG / dt);
         t[PC_INV2GDT * MAXPH + p] = invG / (2 * dt);
         if (dp) {
+            const double etave = t[PC_ETAVE * MAXPH + p];
+            t[PC_YIELD0 * MAXPH + p] = dp->cos_phi[p] * dp->C[p];
+            t[PC_INVDEN * MAXPH + p] = 1.0 / ((etave + dp->eta_reg[p]) + ((dp->Kb[p] * dt) * (-dp->sin_psi[p])) * (-dp->sin_phi[p]));
             t[PC_COSPHI * MAXPH + p] = dp->cos_phi[p];
             t[PC_SINPHI * MAXPH + p] = dp->sin_phi[p];
             t[PC_SINPSI * MAXPH + p] = dp->sin_psi[p];
@@ -708,10 +696,16 @@ struct LaunchOpts {
 constexpr int EBLOCK = 128;
 constexpr int GBLOCK = 256;
 
-template <bool EL, bool PL, bool NP_>
+template <int FL>
 static void launch_elem_t(fem_stokes* s, const ElemArgs<double>& a) {
-    hipLaunchKernelGGL((k_t7_element<double, EL, PL, NP_, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)),
-                       dim3(EBLOCK), 0, s->ctx->stream, a);
+    hipLaunchKernelGGL((k_t7_element<double, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
+                       s->ctx->stream, a);
+}
+
+template <int FL>
+static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
+    if (key == FL) launch_elem_t<FL>(s, a);
+    else if constexpr (FL > 0) dispatch_elem<FL - 1>(key, s, a);
 }
 
 static int launch_element(fem_stokes* s, const LaunchOpts& o) {
@@ -724,33 +718,19 @@ static int launch_element(fem_stokes* s, const LaunchOpts& o) {
     a.vx = s->f[FEM_FIELD_V_X];
     a.vy = s->f[FEM_FIELD_V_Y];
     a.P = s->f[FEM_FIELD_P];
-    a.Pold = o.bulk_inf ? nullptr : s->f[FEM_FIELD_P0];
+    a.Pold = s->f[FEM_FIELD_P0];
     a.pscale = s->d_pscale;
-    a.T = s->f[FEM_FIELD_T];
-    a.T0 = s->f[FEM_FIELD_T0];
-    a.Q = s->f[FEM_FIELD_Q];
-    a.Pf = s->f[FEM_FIELD_PF];
     a.tau_old0 = s->f[FEM_FIELD_TAU_OLD_XX];
     a.tau_old1 = s->f[FEM_FIELD_TAU_OLD_YY];
     a.tau_old2 = s->f[FEM_FIELD_TAU_OLD_XY];
-    if (o.store_tau) {
-        a.tau_out0 = s->f[FEM_FIELD_TAU_XX];
-        a.tau_out1 = s->f[FEM_FIELD_TAU_YY];
-        a.tau_out2 = s->f[FEM_FIELD_TAU_XY];
-    }
     a.phase_v = s->d_phase_v;
     a.phase_P = s->d_phase_P;
     a.ptab = s->d_ptab;
     a.nphases = s->mat.nphases;
-    a.bulk_all_inf = o.bulk_inf;
-    a.thermal = o.thermal;
-    a.eos_p = o.eos_p;
     a.gx = s->mat.g[0];
     a.gy = s->mat.g[1];
     a.Tref = s->mat.Tref;
     a.dt = o.dt;
-    a.RP_out = o.store_RP ? s->f[FEM_FIELD_RP] : nullptr;
-    a.Pnum_out = o.store_Pnum ? s->f[FEM_FIELD_PNUM] : nullptr;
     a.Re = s->d_Re;
     a.rate_x = s->f[FEM_FIELD_DVDTAU_X];
     a.rate_y = s->f[FEM_FIELD_DVDTAU_Y];
@@ -759,25 +739,42 @@ static int launch_element(fem_stokes* s, const LaunchOpts& o) {
     a.bcflag = s->d_bcflag;
     a.bcval_x = s->d_bcval_x;
     a.bcval_y = s->d_bcval_y;
-    if (o.store_Rv) { a.Rv_out_x = s->f[FEM_FIELD_RV_X]; a.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
-    if (o.store_Rv0) { a.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; a.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
     a.alpha_x = o.alpha[0]; a.alpha_y = o.alpha[1];
     a.beta_x = o.beta[0]; a.beta_y = o.beta[1];
-    a.use_pnum = o.use_pnum;
     a.do_update = o.do_update;
-    // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
-    const bool np_ = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
-    const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
-    switch (key) {
-        case 0: launch_elem_t<false, false, false>(s, a); break;
-        case 1: launch_elem_t<false, false, true>(s, a); break;
-        case 2: launch_elem_t<false, true, false>(s, a); break;
-        case 3: launch_elem_t<false, true, true>(s, a); break;
-        case 4: launch_elem_t<true, false, false>(s, a); break;
-        case 5: launch_elem_t<true, false, true>(s, a); break;
-        case 6: launch_elem_t<true, true, false>(s, a); break;
-        case 7: launch_elem_t<true, true, true>(s, a); break;
+    a.RP_out = o.store_RP ? s->f[FEM_FIELD_RP] : nullptr;
+    a.Pnum_out = o.store_Pnum ? s->f[FEM_FIELD_PNUM] : nullptr;
+    if (o.store_Rv) { a.Rv_out_x = s->f[FEM_FIELD_RV_X]; a.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
+    if (o.store_Rv0) { a.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; a.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
+    if (o.store_tau) {
+        a.tau_out0 = s->f[FEM_FIELD_TAU_XX];
+        a.tau_out1 = s->f[FEM_FIELD_TAU_YY];
+        a.tau_out2 = s->f[FEM_FIELD_TAU_XY];
     }
+    // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
+    const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
+    const bool gen = nodeph || o.thermal || s->pf_nonzero || s->q_nonzero;
+    a.nodeph = nodeph;
+    a.bulk_all_inf = o.bulk_inf;
+    a.thermal = o.thermal;
+    a.T = s->f[FEM_FIELD_T];
+    a.T0 = s->f[FEM_FIELD_T0];
+    a.Q = s->q_nonzero ? s->f[FEM_FIELD_Q] : nullptr;
+    a.Pf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
+    const bool strct = m->nx > 0;
+    if (strct) {
+        const double dx = (m->box[1] - m->box[0]) / m->nx, dy = (m->box[3] - m->box[2]) / m->ny;
+        a.nx = m->nx;
+        a.sx = 2 * m->nx + 1;
+        a.nT6 = (int64_t)(2 * m->nx + 1) * (2 * m->ny + 1);
+        a.idx = 1.0 / dx;
+        a.idy = 1.0 / dy;
+        a.detc = dx * dy;
+    }
+    a.phase_same = s->phase_same && !nodeph;
+    const int key = (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0) | (!o.bulk_inf ? EF_BULK : 0) |
+                    (o.use_pnum ? EF_ITER : 0) | (gen ? EF_GEN : 0) | (strct ? EF_STRUCT : 0);
+    dispatch_elem<63>(key, s, a);
     return check_launch(s->ctx, "k_t7_element");
 }
 
@@ -804,7 +801,10 @@ static int launch_gather(fem_stokes* s, const LaunchOpts& o) {
     g.alpha_x = o.alpha[0]; g.alpha_y = o.alpha[1];
     g.beta_x = o.beta[0]; g.beta_y = o.beta[1];
     g.do_update = o.do_update;
-    hipLaunchKernelGGL((k_gather_update<double, GBLOCK>), dim3(grid_xcd(m->ns, GBLOCK)), dim3(GBLOCK), 0,
+    // node -> element-slot lists are read from the CSR arrays for every mesh:
+    // deriving them from indices on structured meshes lengthens the path to
+    // the loads more than the list reads cost
+    hipLaunchKernelGGL((k_gather_update<double, false, GBLOCK>), dim3(grid_xcd(m->ns, GBLOCK)), dim3(GBLOCK), 0,
                        s->ctx->stream, g);
     return check_launch(s->ctx, "k_gather_update");
 }
@@ -968,9 +968,23 @@ static void cheb(double dtau, double lmin, double c_fact, double* a, double* b) 
     *b = (2 - c * dtau) / (2 + c * dtau);
 }
 
+static int any_nonzero(fem_stokes* s, const double* x, int64_t n, bool* out) {
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipMemsetAsync(s->d_flag, 0, 4, c->stream));
+    hipLaunchKernelGGL(k_any_nonzero<double>, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, s->d_flag);
+    CHK(check_launch(c, "k_any_nonzero"));
+    int32_t h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, s->d_flag, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = h != 0;
+    return FEM_OK;
+}
+
 static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, LaunchOpts& o,
                    bool use_tau_old) {
     bool bulk_inf, thermal, eos_p, elastic;
+    CHK(any_nonzero(s, s->f[FEM_FIELD_PF], s->mesh->nnodesP, &s->pf_nonzero));
+    CHK(any_nonzero(s, s->f[FEM_FIELD_Q], s->mesh->nnodesP, &s->q_nonzero));
     CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
     o.plastic = dp != nullptr;
     o.elastic = elastic && use_tau_old;

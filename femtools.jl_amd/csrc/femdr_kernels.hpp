@@ -21,24 +21,70 @@ constexpr int NP = 3;      // P1-disc pressure DoFs
 constexpr int NQ = 7;      // Dunavant degree-5 points
 constexpr int MAXPH = 64;
 
-// Per-phase table columns (device global, copied to LDS per block).
-enum PhaseCol {
-    PC_ETA = 0, PC_INVG, PC_RHO0, PC_ALPHA, PC_BETA, PC_BULK,
-    PC_COSPHI, PC_SINPHI, PC_SINPSI, PC_C, PC_EREG, PC_KB,
-    PC_ETAVE, PC_INV2GDT, PC_YIELD0, PC_LAMDEN,   // dt-dependent, cell-phase fast path
-    PC_NCOL
-};
-
 // Reference-element tables (shape_functions.jl:115-171, 231-239;
-// integration_points.jl:188-202), filled from the host at load.
+// integration_points.jl:188-202), evaluated at compile time so that, with the
+// quadrature loops unrolled, every entry becomes an immediate operand.
 struct Tables {
     double N[NQ][NV];
     double NP1[NQ][NP];
     double dN[NQ][NV][2];
     double w[NQ];
-    double TB[NP][NV][2];   // Σ_q w_q NP1_q[j] dN_q[b][k] (pressure coupling of the Jacobian)
+    double TB[NP][NV][2];   // Σ_q w_q NP1_q[j] dN_q[b][k]: the element divergence operator
+    double MP[NP][NP];      // Σ_q w_q NP1_q[i] NP1_q[j]: the P1 mass matrix on the reference triangle
 };
-extern __constant__ Tables c_tab;
+
+constexpr void t7_basis_ct(double xi, double eta, double* N, double (*dN)[2]) {
+    const double L1 = 1 - xi - eta;
+    const double b = 27.0 * (1 - xi - eta) * xi * eta;
+    const double bx = 27.0 * eta * (1 - 2 * xi - eta);
+    const double by = 27.0 * xi * (1 - xi - 2 * eta);
+    N[0] = L1 * (2 * L1 - 1) + b / 9;
+    N[1] = xi * (2 * xi - 1) + b / 9;
+    N[2] = eta * (2 * eta - 1) + b / 9;
+    N[3] = 4 * (1 - xi - eta) * xi - 4 * b / 9;
+    N[4] = 4 * xi * eta - 4 * b / 9;
+    N[5] = 4 * eta * (1 - xi - eta) - 4 * b / 9;
+    N[6] = b;
+    dN[0][0] = 1 - 4 * L1 + bx / 9;  dN[0][1] = 1 - 4 * L1 + by / 9;
+    dN[1][0] = 4 * xi - 1 + bx / 9;  dN[1][1] = by / 9;
+    dN[2][0] = bx / 9;               dN[2][1] = 4 * eta - 1 + by / 9;
+    dN[3][0] = 4 * (1 - 2 * xi - eta) - 4 * bx / 9; dN[3][1] = -4 * xi - 4 * by / 9;
+    dN[4][0] = 4 * eta - 4 * bx / 9; dN[4][1] = 4 * xi - 4 * by / 9;
+    dN[5][0] = -4 * eta - 4 * bx / 9; dN[5][1] = 4 * (1 - xi - 2 * eta) - 4 * by / 9;
+    dN[6][0] = bx;                   dN[6][1] = by;
+}
+
+constexpr Tables make_tables() {
+    Tables t{};
+    const double a1 = 0.101286507323456, a2 = 0.470142064105115;
+    const double b1 = 1 - 2 * a1, b2 = 1 - 2 * a2;
+    const double w1 = 0.062969590272414, w2 = 0.066197076394253, wc = 0.1125;
+    const double X[NQ] = {a1, b1, a1, a2, b2, a2, 1.0 / 3};
+    const double Y[NQ] = {a1, a1, b1, a2, a2, b2, 1.0 / 3};
+    const double W[NQ] = {w1, w1, w1, w2, w2, w2, wc};
+    for (int q = 0; q < NQ; ++q) {
+        t7_basis_ct(X[q], Y[q], t.N[q], t.dN[q]);
+        t.NP1[q][0] = 1 - X[q] - Y[q];
+        t.NP1[q][1] = X[q];
+        t.NP1[q][2] = Y[q];
+        t.w[q] = W[q];
+    }
+    for (int j = 0; j < NP; ++j)
+        for (int b = 0; b < NV; ++b)
+            for (int k = 0; k < 2; ++k) {
+                double s = 0;
+                for (int q = 0; q < NQ; ++q) s += t.w[q] * t.NP1[q][j] * t.dN[q][b][k];
+                t.TB[j][b][k] = s;
+            }
+    for (int i = 0; i < NP; ++i)
+        for (int j = 0; j < NP; ++j) {
+            double s = 0;
+            for (int q = 0; q < NQ; ++q) s += t.w[q] * t.NP1[q][i] * t.NP1[q][j];
+            t.MP[i][j] = s;
+        }
+    return t;
+}
+static constexpr Tables kT = make_tables();
 
 template <class F> struct Vec2;
 template <> struct Vec2<double> { typedef double2 T; };
@@ -58,18 +104,16 @@ template <class F> __device__ inline D3<F> operator+(D3<F> a, D3<F> b) { return 
 template <class F> __device__ inline D3<F> operator-(D3<F> a, D3<F> b) { return D3<F>(a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2); }
 template <class F> __device__ inline D3<F> operator-(D3<F> a) { return D3<F>(-a.v, -a.d0, -a.d1, -a.d2); }
 template <class F> __device__ inline D3<F> operator*(D3<F> a, D3<F> b) { return D3<F>(a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1, a.d2 * b.v + a.v * b.d2); }
-template <class F> __device__ inline D3<F> operator/(D3<F> a, D3<F> b) {
-    F iv = F(1) / b.v; F q = a.v * iv;
-    return D3<F>(q, (a.d0 - q * b.d0) * iv, (a.d1 - q * b.d1) * iv, (a.d2 - q * b.d2) * iv);
-}
 template <class F> __device__ inline D3<F> operator+(D3<F> a, F b) { a.v += b; return a; }
 template <class F> __device__ inline D3<F> operator-(D3<F> a, F b) { a.v -= b; return a; }
 template <class F> __device__ inline D3<F> operator*(D3<F> a, F b) { return D3<F>(a.v * b, a.d0 * b, a.d1 * b, a.d2 * b); }
 template <class F> __device__ inline D3<F> operator*(F b, D3<F> a) { return a * b; }
-template <class F> __device__ inline D3<F> operator/(D3<F> a, F b) { return D3<F>(a.v / b, a.d0 / b, a.d1 / b, a.d2 / b); }
 template <class F> __device__ inline D3<F> dsqrt(D3<F> a) { F s = sqrt(a.v); F h = F(0.5) / s; return D3<F>(s, a.d0 * h, a.d1 * h, a.d2 * h); }
+template <class F> __device__ inline D3<F> drcp(D3<F> a) { F r = F(1) / a.v; F h = -r * r; return D3<F>(r, a.d0 * h, a.d1 * h, a.d2 * h); }
 __device__ inline double dsqrt(double a) { return sqrt(a); }
 __device__ inline float dsqrt(float a) { return sqrtf(a); }
+__device__ inline double drcp(double a) { return 1.0 / a; }
+__device__ inline float drcp(float a) { return 1.0f / a; }
 template <class F> __device__ inline F val(D3<F> a) { return a.v; }
 __device__ inline double val(double a) { return a; }
 __device__ inline float val(float a) { return a; }
@@ -78,77 +122,121 @@ template <class F> __device__ inline F eps2();
 template <> __device__ inline double eps2<double>() { return 4.930380657631324e-32; }   // eps(Float64)^2
 template <> __device__ inline float eps2<float>() { return 1.4210855e-14f; }           // eps(Float32)^2
 
-// Per-quadrature-point material coefficients (rheology.jl:35-42) and
-// Drucker-Prager parameters interpolated at the point (rheology.jl:296-302).
+// Per-phase table columns (device global, copied to LDS per block).
+enum PhaseCol {
+    PC_ETA = 0, PC_INVG, PC_RHO0, PC_ALPHA, PC_BETA, PC_BULK,
+    PC_COSPHI, PC_SINPHI, PC_SINPSI, PC_C, PC_EREG, PC_KB,
+    PC_ETAVE, PC_INV2GDT, PC_YIELD0, PC_INVDEN,   // dt-dependent, element-constant phases
+    PC_NCOL
+};
+
+// Per-quadrature-point material coefficients (rheology.jl:35-42) and the
+// Drucker-Prager parameters at the point (rheology.jl:296-302): yield0 =
+// cos(ϕ)·C, invden = 1 / (ηve + η_reg + Kb Δt sin(Ψ) sin(ϕ)).
 template <class F>
 struct IpMat {
     F etave, inv2Gdt;
-    F cphi, C, sphi, spsi, ereg, Kb;
+    F yield0, sphi, invden;
 };
 
-// Deviatoric stress with the plane-strain DP return (rheology.jl:226-245, 272-334).
+// Deviatoric stress with the plane-strain Drucker-Prager return
+// (rheology.jl:226-245, 272-334). S is F, or D3<F> for the tangent.
 template <class S, class F, bool PLASTIC>
-__device__ inline void stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F dt, F txo, F tyo, F txyo, F Pq,
+__device__ inline void stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
                                 S& txx, S& tyy, S& txy) {
-    S tr = (exx + eyy) / F(3);
-    F two_eve = F(2) * m.etave;
+    S tr = (exx + eyy) * F(1.0 / 3.0);
+    const F two_eve = F(2) * m.etave;
     txx = two_eve * ((exx - tr) + txo * m.inv2Gdt);
     tyy = two_eve * ((eyy - tr) + tyo * m.inv2Gdt);
     txy = two_eve * (exy + txyo * m.inv2Gdt);
     if (PLASTIC) {
         S tzz = -txx - tyy;
-        S tII = dsqrt((txx * txx + tyy * tyy + tzz * tzz) / F(2) + txy * txy + eps2<F>());
-        S F_ = (tII - m.cphi * m.C) - m.sphi * Pq;
-        if (val(F_) > F(0)) {
-            S tIIs = tII + eps2<F>();
-            F den = (m.etave + m.ereg) + ((m.Kb * dt) * (-m.spsi)) * (-m.sphi);
-            S lam = F_ / den;
+        S tII = dsqrt((txx * txx + tyy * tyy + tzz * tzz) * F(0.5) + txy * txy + eps2<F>());
+        S F_ = (tII - m.yield0) - m.sphi * Pq;
+        // whole waves without a yielding point skip the return (no if-conversion)
+        if (__any(val(F_) > F(0)) && val(F_) > F(0)) {
+            S lam = F_ * m.invden;
             if (val(lam) > F(0)) {
-                S fac = two_eve * lam;
-                S dqxx = (F(2) * txx + tyy) / (F(2) * tIIs);
-                S dqyy = (txx + F(2) * tyy) / (F(2) * tIIs);
-                S dqxy = txy / tIIs;
-                txx = txx - fac * dqxx;
-                tyy = tyy - fac * dqyy;
-                txy = txy - fac * dqxy;
+                // ∂Q/∂τ = ((2τxx+τyy), (τxx+2τyy), 2τxy) / (2 τII_s)
+                S h = two_eve * lam * drcp((tII + eps2<F>()) * F(2));
+                S nxx = (F(2) * txx + tyy) * h;
+                S nyy = (txx + F(2) * tyy) * h;
+                S nxy = (F(2) * txy) * h;
+                txx = txx - nxx;
+                tyy = tyy - nyy;
+                txy = txy - nxy;
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Element kernel arguments
+// Element kernel
 // ---------------------------------------------------------------------------
+// compile-time feature flags of the element kernel
+enum ElemFlags {
+    EF_ELASTIC = 1,   // finite shear modulus somewhere: read τ_old
+    EF_PLASTIC = 2,   // Drucker-Prager return
+    EF_BULK = 4,      // finite pressure storage modulus: read P_old
+    EF_ITER = 8,      // inner pseudo-iteration: Pnum in the momentum, bubble update
+    EF_GEN = 16,      // element-node phases, temperature, sources, fluid pressure
+    EF_STRUCT = 32,   // structured box mesh: connectivity and geometry from the element index
+};
+
+// Structured T7 box mesh (connectivity.jl:111-140): element e = 2c + k of cell
+// c = ey nx + ex; the cell's two triangles alternate diagonals with the cell
+// parity. Returns the 7 local nodes (T6 grid nodes, then the centroid
+// nT6 + e) and the affine map J^-1 of the triangle (|det J| = dx dy).
+template <class F>
+__device__ inline void t7_struct_element(int64_t e, int32_t nx, int32_t sx, int64_t nT6, F idx, F idy,
+                                         int32_t n[7], F& i00, F& i01, F& i10, F& i11) {
+    const int32_t c = (int32_t)(e >> 1), k = (int32_t)(e & 1);
+    const int32_t ey = c / nx, ex = c - ey * nx;
+    const int32_t base = 2 * ey * sx + 2 * ex;
+    const int32_t BL = base, BR = base + 2, TL = base + 2 * sx, TR = base + 2 * sx + 2;
+    const int32_t bot = base + 1, lft = base + sx, ctr = base + sx + 1, rgt = base + sx + 2, top = base + 2 * sx + 1;
+    const bool even = ((ex + ey) & 1) == 0;
+    if (even) {
+        if (k == 0) { n[0] = BL; n[1] = BR; n[2] = TR; n[3] = bot; n[4] = rgt; n[5] = ctr; i00 = idx; i01 = -idy; i10 = 0; i11 = idy; }
+        else        { n[0] = BL; n[1] = TR; n[2] = TL; n[3] = ctr; n[4] = top; n[5] = lft; i00 = idx; i01 = 0; i10 = -idx; i11 = idy; }
+    } else {
+        if (k == 0) { n[0] = BL; n[1] = BR; n[2] = TL; n[3] = bot; n[4] = ctr; n[5] = lft; i00 = idx; i01 = 0; i10 = 0; i11 = idy; }
+        else        { n[0] = BR; n[1] = TR; n[2] = TL; n[3] = rgt; n[4] = top; n[5] = ctr; i00 = idx; i01 = idy; i10 = -idx; i11 = 0; }
+    }
+    n[6] = (int32_t)(nT6 + e);
+}
+
 template <class F>
 struct ElemArgs {
     int64_t nels;
     const int32_t* el2n;       // [7][nels] (SoA, 0-based)
     const F* geo;              // [5][nels]: J^-1 (00,01,10,11), |det J|
     F* vx; F* vy;              // velocity (bubble entries updated in place)
-    const F* P; const F* Pold; // Pold nullable (finite bulk only)
+    const F* P; const F* Pold; // Pold read with EF_BULK
     const F* pscale;           // γP / M_P per pressure DoF
-    const F* T; const F* T0; const F* Q; const F* Pf; // nullable
-    const F* tau_old0; const F* tau_old1; const F* tau_old2; // [nels][7] nullable
-    F* tau_out0; F* tau_out1; F* tau_out2;                   // nullable
-    const int32_t* phase_v;    // cell [nels] or element-node [7][nels] (0-based)
-    const int32_t* phase_P;    // cell [nels] or element-node [3][nels] (0-based)
+    const F* tau_old0; const F* tau_old1; const F* tau_old2; // [nels][7], EF_ELASTIC
+    const int32_t* phase_v;    // cell [nels] or, with EF_GEN, element-node [7][nels] (0-based)
+    const int32_t* phase_P;    // cell [nels] or, with EF_GEN, element-node [3][nels]
     const double* ptab;        // [PC_NCOL][MAXPH]
     int32_t nphases;
-    int32_t bulk_all_inf;      // every phase has infinite pressure storage modulus
-    int32_t thermal;           // α != 0 somewhere: read T, T0
-    int32_t eos_p;             // finite K somewhere: density depends on P
     F gx, gy, Tref, dt;
-    F* RP_out;                 // nullable
-    F* Pnum_out;               // nullable
     typename Vec2<F>::T* Re;   // [nels][6] element-slot residuals of shared nodes
-    // bubble-node DR update (mode 1) / residual store
     F* rate_x; F* rate_y; const F* PCx; const F* PCy;
     const uint8_t* bcflag; const F* bcval_x; const F* bcval_y;
-    F* Rv_out_x; F* Rv_out_y;      // nullable: store residual (after Dirichlet)
-    F* Rv0_out_x; F* Rv0_out_y;    // nullable
     F alpha_x, alpha_y, beta_x, beta_y;  // positive Chebyshev step and momentum
-    int32_t use_pnum;
-    int32_t do_update;
+    int32_t do_update;         // EF_ITER launches inside the DR loop update the bubble node
+    // optional outputs (null unless stored this launch)
+    F* RP_out; F* Pnum_out;
+    F* Rv_out_x; F* Rv_out_y; F* Rv0_out_x; F* Rv0_out_y;
+    F* tau_out0; F* tau_out1; F* tau_out2;
+    // EF_STRUCT: nx cells per row, sx = 2 nx + 1 grid nodes per row, nT6 grid nodes
+    int32_t nx, sx; int64_t nT6; F idx, idy, detc;
+    int32_t phase_same;        // phase_P holds the same cell phases as phase_v
+    // EF_GEN extras
+    int32_t nodeph;            // phases stored per element node
+    int32_t bulk_all_inf;
+    int32_t thermal;
+    const F* T; const F* T0; const F* Q; const F* Pf;   // nullable
 };
 
 // Workgroups are dealt round-robin over the 8 XCDs (one L2 each). The grid
@@ -160,110 +248,150 @@ __device__ inline int64_t xcd_tile() {
     return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-template <class F, bool ELASTIC, bool PLASTIC, bool NODEPH, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
-    __shared__ double s_tab[PC_NCOL * MAXPH];
-    const int nph = A.nphases;
-    for (int i = threadIdx.x; i < PC_NCOL * MAXPH; i += BLOCK)
-        if ((i % MAXPH) < nph) s_tab[i] = A.ptab[i];
-    __syncthreads();
-    const int64_t e = xcd_tile() * BLOCK + threadIdx.x;
-    if (e >= A.nels) return;
+// Occupancy target of the element kernel (waves per SIMD); 0 leaves the
+// register allocation to the compiler.
+#ifndef FEM_ELEM_WAVES
+#define FEM_ELEM_WAVES 3
+#endif
+// τ_old staged through LDS by direct global->LDS loads (1) or read per
+// quadrature point (0)
+#ifndef FEM_TAU_LDS
+#define FEM_TAU_LDS 1
+#endif
+#if FEM_ELEM_WAVES > 0
+#define FEM_ELEM_ATTR __attribute__((amdgpu_waves_per_eu(FEM_ELEM_WAVES)))
+#else
+#define FEM_ELEM_ATTR
+#endif
+
+template <class F, int FL, int BLOCK>
+__global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
+    constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
+    constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT;
+    constexpr bool TAU_LDS = ELASTIC && FEM_TAU_LDS;
     const int64_t ne = A.nels;
-#define PT(col, p) ((F)s_tab[(col) * MAXPH + (p)])
+    const int64_t e0 = xcd_tile() * BLOCK;          // first element of this block
+    const bool valid = e0 + threadIdx.x < ne;
+    const int64_t e = valid ? e0 + threadIdx.x : ne - 1;   // idle lanes recompute the last element, store nothing
+    // per-phase coefficients: a few KB read through the caches
+#define PT(col, p) ((F)A.ptab[(col) * MAXPH + (p)])
+    const bool nodeph = GEN && A.nodeph;
+    // τ_old of the block's elements is one contiguous chunk per component:
+    // stream it straight into LDS (no registers held) while the velocity
+    // gathers and the continuity residual proceed.
+    __shared__ F s_tau[TAU_LDS ? 3 * BLOCK * NQ : 1];
+    if (TAU_LDS && e0 < ne) {
+        const int64_t nvalid = ne - e0 < BLOCK ? ne - e0 : BLOCK;
+        const int bytes = (int)(nvalid * NQ * sizeof(F));
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const F* src[3] = {A.tau_old0, A.tau_old1, A.tau_old2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const char* g = (const char*)(src[c] + e0 * NQ);
+            char* l = (char*)(s_tau + c * BLOCK * NQ);
+            for (int off0 = wave * 64 * 16; off0 < bytes; off0 += BLOCK * 16) {
+                if (off0 + lane * 16 < bytes)
+                    __builtin_amdgcn_global_load_lds((const void*)(g + off0 + lane * 16),
+                                                     (__attribute__((address_space(3))) void*)(l + off0), 16, 0, 0);
+            }
+        }
+    }
 
     int32_t n[NV];
+    F i00, i01, i10, i11, det;
+    if (STRUCT) {
+        t7_struct_element<F>(e, A.nx, A.sx, A.nT6, A.idx, A.idy, n, i00, i01, i10, i11);
+        det = A.detc;
+    } else {
 #pragma unroll
-    for (int a = 0; a < NV; ++a) n[a] = A.el2n[a * ne + e];
-    const F i00 = A.geo[0 * ne + e], i01 = A.geo[1 * ne + e];
-    const F i10 = A.geo[2 * ne + e], i11 = A.geo[3 * ne + e];
-    const F det = A.geo[4 * ne + e];
+        for (int a = 0; a < NV; ++a) n[a] = A.el2n[a * ne + e];
+        i00 = A.geo[0 * ne + e]; i01 = A.geo[1 * ne + e];
+        i10 = A.geo[2 * ne + e]; i11 = A.geo[3 * ne + e];
+        det = A.geo[4 * ne + e];
+    }
     F vx[NV], vy[NV];
 #pragma unroll
     for (int a = 0; a < NV; ++a) { vx[a] = A.vx[n[a]]; vy[a] = A.vy[n[a]]; }
     F P[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) P[i] = A.P[e * NP + i];
+    const int pc = A.phase_v[e];
 
-    int pv[NV];
-    int pc = 0;
-    if (NODEPH) {
-#pragma unroll
-        for (int a = 0; a < NV; ++a) pv[a] = A.phase_v[a * ne + e];
-    } else {
-        pc = A.phase_v[e];
-    }
-
-    // ---- velocity gradients at the quadrature points (affine element)
-    F gxx[NQ], gxy[NQ], gyx[NQ], gyy[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        F ax = 0, ay = 0, bx = 0, by = 0;
-#pragma unroll
-        for (int a = 0; a < NV; ++a) {
-            const F d0 = (F)c_tab.dN[q][a][0], d1 = (F)c_tab.dN[q][a][1];
-            ax += d0 * vx[a]; ay += d1 * vx[a];
-            bx += d0 * vy[a]; by += d1 * vy[a];
-        }
-        gxx[q] = ax * i00 + ay * i10;  // ∂vx/∂x
-        gxy[q] = ax * i01 + ay * i11;  // ∂vx/∂y
-        gyx[q] = bx * i00 + by * i10;  // ∂vy/∂x
-        gyy[q] = bx * i01 + by * i11;  // ∂vy/∂y
-    }
-
-    // ---- continuity residual (pressure_residual.jl:132-158)
+    // ---- continuity residual (pressure_residual.jl:132-158). The element is
+    // affine, so -Σ_q N^P_i (∇·v)_q dΩ_q = -det Σ_a Σ_k TB[i][a][k] u_k(a)
+    // with u(a) the nodal velocity rotated by J^-T; storage, thermal and
+    // source terms are P1 functions that the rule integrates exactly (det M ·).
     F RP[NP] = {0, 0, 0};
-    {
-        int pp[NP];
-        if (NODEPH) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) pp[i] = A.phase_P[i * ne + e];
-        } else {
-            const int p = A.phase_P[e];
+    for (int a = 0; a < NV; ++a) {
+        const F u0 = i00 * vx[a] + i01 * vy[a];
+        const F u1 = i10 * vx[a] + i11 * vy[a];
 #pragma unroll
-            for (int i = 0; i < NP; ++i) pp[i] = p;
-        }
+        for (int i = 0; i < NP; ++i) RP[i] -= (F)kT.TB[i][a][0] * u0 + (F)kT.TB[i][a][1] * u1;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) RP[i] *= det;
+    if (BULK || GEN) {
         F dP[NP] = {0, 0, 0}, dT[NP] = {0, 0, 0}, Qv[NP] = {0, 0, 0};
-        if (A.Pold) {
+        if (BULK) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) dP[i] = P[i] - A.Pold[e * NP + i];
         }
-        if (A.thermal) {
+        if (GEN && A.thermal) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) dT[i] = A.T[e * NP + i] - A.T0[e * NP + i];
         }
-        if (A.Q) {
+        if (GEN && A.Q) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) Qv[i] = A.Q[e * NP + i];
         }
+        if (!nodeph) {
+            const int p = A.phase_same ? pc : A.phase_P[e];
+            const F bulk = (GEN && A.bulk_all_inf) || !BULK ? (F)INFINITY : PT(PC_BULK, p);
+            const F al = PT(PC_ALPHA, p);
+            F src[NP];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const F dOm = (F)c_tab.w[q] * det;
-            F src = 0;
-            if (A.Pold || A.thermal || A.Q) {
+            for (int j = 0; j < NP; ++j) src[j] = -(dP[j] / (bulk * A.dt)) + al * dT[j] / A.dt + Qv[j];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                F acc = 0;
+#pragma unroll
+                for (int j = 0; j < NP; ++j) acc += (F)kT.MP[i][j] * src[j];
+                RP[i] += det * acc;
+            }
+        } else {
+            int pp[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) pp[i] = A.phase_P[i * ne + e];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
                 F bulk = 0, alq = 0, dPq = 0, dTq = 0, Qq = 0;
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    const F Ni = (F)c_tab.NP1[q][i];
+                    const F Ni = (F)kT.NP1[q][i];
                     bulk += Ni * PT(PC_BULK, pp[i]);
                     alq += Ni * PT(PC_ALPHA, pp[i]);
                     dPq += Ni * dP[i];
                     dTq += Ni * dT[i];
                     Qq += Ni * Qv[i];
                 }
-                if (A.bulk_all_inf) bulk = (F)INFINITY;
-                src = -(dPq / (bulk * A.dt)) + alq * dTq / A.dt + Qq;
-            }
-            const F r = ((-(gxx[q] + gyy[q])) + src) * dOm;
+                if (A.bulk_all_inf || !BULK) bulk = (F)INFINITY;
+                const F src = (-(dPq / (bulk * A.dt)) + alq * dTq / A.dt + Qq) * ((F)kT.w[q] * det);
 #pragma unroll
-            for (int i = 0; i < NP; ++i) RP[i] += (F)c_tab.NP1[q][i] * r;
+                for (int i = 0; i < NP; ++i) RP[i] += (F)kT.NP1[q][i] * src;
+            }
         }
     }
     F Pn[NP] = {0, 0, 0};
-    if (A.use_pnum) {
+    if (ITER) {
 #pragma unroll
         for (int i = 0; i < NP; ++i) Pn[i] = A.pscale[e * NP + i] * RP[i];
     }
+    if (TAU_LDS) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if (!valid) return;
     if (A.RP_out) {
 #pragma unroll
         for (int i = 0; i < NP; ++i) A.RP_out[e * NP + i] = RP[i];
@@ -275,15 +403,18 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
 
     // ---- momentum residual (momentum_residuals.jl:136-189)
     IpMat<F> m;
-    F rho_c = 0, alpha_c = 0, beta_c = 0;
-    if (!NODEPH) {
-        m.etave = PT(PC_ETAVE, pc);
-        m.inv2Gdt = PT(PC_INV2GDT, pc);
-        if (PLASTIC) {
-            m.cphi = PT(PC_COSPHI, pc); m.C = PT(PC_C, pc); m.sphi = PT(PC_SINPHI, pc);
-            m.spsi = PT(PC_SINPSI, pc); m.ereg = PT(PC_EREG, pc); m.Kb = PT(PC_KB, pc);
-        }
-        rho_c = PT(PC_RHO0, pc); alpha_c = PT(PC_ALPHA, pc); beta_c = PT(PC_BETA, pc);
+    F rho_c = PT(PC_RHO0, pc), alpha_c = PT(PC_ALPHA, pc), beta_c = PT(PC_BETA, pc);
+    m.etave = PT(PC_ETAVE, pc);
+    m.inv2Gdt = PT(PC_INV2GDT, pc);
+    if (PLASTIC) {
+        m.yield0 = PT(PC_YIELD0, pc);
+        m.sphi = PT(PC_SINPHI, pc);
+        m.invden = PT(PC_INVDEN, pc);
+    }
+    int pv[NV];
+    if (nodeph) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) pv[a] = A.phase_v[a * ne + e];
     }
     F Rx[NV], Ry[NV];
 #pragma unroll
@@ -291,11 +422,11 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
     const bool grav = (A.gx != F(0)) || (A.gy != F(0));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        if (NODEPH) {
+        if (nodeph) {
             F eta = 0, invG = 0;
 #pragma unroll
             for (int a = 0; a < NV; ++a) {
-                const F Na = (F)c_tab.N[q][a];
+                const F Na = (F)kT.N[q][a];
                 eta += Na * PT(PC_ETA, pv[a]);
                 invG += Na * PT(PC_INVG, pv[a]);
             }
@@ -305,16 +436,22 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
                 F cphi = 0, C = 0, sphi = 0, spsi = 0, ereg = 0, Kb = 0;
 #pragma unroll
                 for (int a = 0; a < NV; ++a) {
-                    const F Na = (F)c_tab.N[q][a];
+                    const F Na = (F)kT.N[q][a];
                     cphi += Na * PT(PC_COSPHI, pv[a]); C += Na * PT(PC_C, pv[a]);
                     sphi += Na * PT(PC_SINPHI, pv[a]); spsi += Na * PT(PC_SINPSI, pv[a]);
                     ereg += Na * PT(PC_EREG, pv[a]); Kb += Na * PT(PC_KB, pv[a]);
                 }
-                m.cphi = cphi; m.C = C; m.sphi = sphi; m.spsi = spsi; m.ereg = ereg; m.Kb = Kb;
+                m.yield0 = cphi * C;
+                m.sphi = sphi;
+                m.invden = F(1) / ((m.etave + ereg) + ((Kb * A.dt) * (-spsi)) * (-sphi));
             }
         }
         F txo = 0, tyo = 0, txyo = 0;
-        if (ELASTIC) {
+        if (TAU_LDS) {
+            txo = s_tau[0 * BLOCK * NQ + threadIdx.x * NQ + q];
+            tyo = s_tau[1 * BLOCK * NQ + threadIdx.x * NQ + q];
+            txyo = s_tau[2 * BLOCK * NQ + threadIdx.x * NQ + q];
+        } else if (ELASTIC) {
             txo = A.tau_old0[e * NQ + q];
             tyo = A.tau_old1[e * NQ + q];
             txyo = A.tau_old2[e * NQ + q];
@@ -322,23 +459,33 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
         F Pq = 0, Pnq = 0;
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
-            Pq += (F)c_tab.NP1[q][i] * P[i];
-            Pnq += (F)c_tab.NP1[q][i] * Pn[i];
+            Pq += (F)kT.NP1[q][i] * P[i];
+            if (ITER) Pnq += (F)kT.NP1[q][i] * Pn[i];
         }
         F Pfq = 0;
-        if (A.Pf) {
+        if (GEN && A.Pf) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) Pfq += (F)c_tab.NP1[q][i] * A.Pf[e * NP + i];
+            for (int i = 0; i < NP; ++i) Pfq += (F)kT.NP1[q][i] * A.Pf[e * NP + i];
         }
+        F ax = 0, ay = 0, bx_ = 0, by_ = 0;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
+            ax += d0 * vx[a]; ay += d1 * vx[a];
+            bx_ += d0 * vy[a]; by_ += d1 * vy[a];
+        }
+        const F gxx = ax * i00 + ay * i10;   // ∂vx/∂x
+        const F gxy = ax * i01 + ay * i11;   // ∂vx/∂y
+        const F gyx = bx_ * i00 + by_ * i10; // ∂vy/∂x
+        const F gyy = bx_ * i01 + by_ * i11; // ∂vy/∂y
         F txx, tyy, txy;
-        stress2d<F, F, PLASTIC>(gxx[q], gyy[q], (gxy[q] + gyx[q]) / F(2), m, A.dt, txo, tyo, txyo, Pq - Pfq,
-                                txx, tyy, txy);
+        stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy, txy);
         if (A.tau_out0) {
             A.tau_out0[e * NQ + q] = txx;
             A.tau_out1[e * NQ + q] = tyy;
             A.tau_out2[e * NQ + q] = txy;
         }
-        const F dOm = (F)c_tab.w[q] * det;
+        const F dOm = (F)kT.w[q] * det;
         const F Ptot = Pq + Pnq;
         const F sxx = txx - Ptot, syy = tyy - Ptot;
         // reference-space stress: R_i[a] = Σ_j ∂N_a/∂ξ_j s_ji
@@ -348,32 +495,31 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
         const F s1y = (i10 * txy + i11 * syy) * dOm;
         F bx = 0, by = 0;
         if (grav) {
-            F rho;
-            if (NODEPH) {
+            if (nodeph) {
                 F r0 = 0, al = 0, be = 0;
 #pragma unroll
                 for (int a = 0; a < NV; ++a) {
-                    const F Na = (F)c_tab.N[q][a];
+                    const F Na = (F)kT.N[q][a];
                     r0 += Na * PT(PC_RHO0, pv[a]); al += Na * PT(PC_ALPHA, pv[a]); be += Na * PT(PC_BETA, pv[a]);
                 }
                 rho_c = r0; alpha_c = al; beta_c = be;
             }
             F Tq = 0;
-            if (A.thermal) {
+            if (GEN && A.thermal) {
 #pragma unroll
-                for (int i = 0; i < NP; ++i) Tq += (F)c_tab.NP1[q][i] * A.T[e * NP + i];
+                for (int i = 0; i < NP; ++i) Tq += (F)kT.NP1[q][i] * A.T[e * NP + i];
             }
-            rho = rho_c * ((F(1) - alpha_c * (Tq - A.Tref)) + beta_c * Pq);
+            const F rho = rho_c * ((F(1) - alpha_c * (Tq - A.Tref)) + beta_c * Pq);
             bx = rho * A.gx * dOm;
             by = rho * A.gy * dOm;
         }
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
-            const F d0 = (F)c_tab.dN[q][a][0], d1 = (F)c_tab.dN[q][a][1];
+            const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
             Rx[a] += d0 * s0x + d1 * s1x;
             Ry[a] += d0 * s0y + d1 * s1y;
             if (grav) {
-                const F Na = (F)c_tab.N[q][a];
+                const F Na = (F)kT.N[q][a];
                 Rx[a] -= Na * bx;
                 Ry[a] -= Na * by;
             }
@@ -391,11 +537,11 @@ __global__ __launch_bounds__(BLOCK) void k_t7_element(ElemArgs<F> A) {
     {
         const int32_t b = n[NSH];
         const uint8_t fl = A.bcflag[b];
-        F rx = (fl & 1) ? F(0) : Rx[NSH];
-        F ry = (fl & 2) ? F(0) : Ry[NSH];
+        const F rx = (fl & 1) ? F(0) : Rx[NSH];
+        const F ry = (fl & 2) ? F(0) : Ry[NSH];
         if (A.Rv_out_x) { A.Rv_out_x[b] = rx; A.Rv_out_y[b] = ry; }
         if (A.Rv0_out_x) { A.Rv0_out_x[b] = rx; A.Rv0_out_y[b] = ry; }
-        if (A.do_update) {
+        if (ITER && A.do_update) {
             F nrx = A.beta_x * A.rate_x[b] + rx / A.PCx[b];
             F nry = A.beta_y * A.rate_y[b] + ry / A.PCy[b];
             F nvx = vx[NSH] + (-A.alpha_x) * nrx;
@@ -427,13 +573,14 @@ struct GatherArgs {
     int32_t do_update;
 };
 
-template <class F, int BLOCK>
+template <class F, bool STRUCT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_gather_update(GatherArgs<F> A) {
+    (void)STRUCT;
     const int64_t i = xcd_tile() * BLOCK + threadIdx.x;
     if (i >= A.ns) return;
     const int64_t node = A.node_of ? A.node_of[i] : i;
-    const int32_t k0 = A.off[i], k1 = A.off[i + 1];
     F rx = 0, ry = 0;
+    const int32_t k0 = A.off[i], k1 = A.off[i + 1];
     for (int32_t k = k0; k < k1; ++k) {
         const typename Vec2<F>::T r = A.Re[A.ent[k]];
         rx += r.x;
@@ -522,7 +669,7 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
         F ax = 0, ay = 0, bx = 0, by = 0;
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
-            const F d0 = (F)c_tab.dN[q][a][0], d1 = (F)c_tab.dN[q][a][1];
+            const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
             ax += d0 * vx[a]; ay += d1 * vx[a]; bx += d0 * vy[a]; by += d1 * vy[a];
         }
         const F gxx = ax * iJ[0][0] + ay * iJ[1][0], gxy = ax * iJ[0][1] + ay * iJ[1][1];
@@ -530,32 +677,34 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
         IpMat<F> m;
         F invG = 0;
 #pragma unroll
-        for (int a = 0; a < NV; ++a) invG += (F)c_tab.N[q][a] * PT(PC_INVG, pv[a]);
+        for (int a = 0; a < NV; ++a) invG += (F)kT.N[q][a] * PT(PC_INVG, pv[a]);
         m.etave = F(1) / (F(1) / etamax + invG / A.dt);
         m.inv2Gdt = invG / (F(2) * A.dt);
         if (PLASTIC) {
             F cphi = 0, C = 0, sphi = 0, spsi = 0, ereg = 0, Kb = 0;
 #pragma unroll
             for (int a = 0; a < NV; ++a) {
-                const F Na = (F)c_tab.N[q][a];
+                const F Na = (F)kT.N[q][a];
                 cphi += Na * PT(PC_COSPHI, pv[a]); C += Na * PT(PC_C, pv[a]);
                 sphi += Na * PT(PC_SINPHI, pv[a]); spsi += Na * PT(PC_SINPSI, pv[a]);
                 ereg += Na * PT(PC_EREG, pv[a]); Kb += Na * PT(PC_KB, pv[a]);
             }
-            m.cphi = cphi; m.C = C; m.sphi = sphi; m.spsi = spsi; m.ereg = ereg; m.Kb = Kb;
+            m.yield0 = cphi * C;
+            m.sphi = sphi;
+            m.invden = F(1) / ((m.etave + ereg) + ((Kb * A.dt) * (-spsi)) * (-sphi));
         }
         F txo = 0, tyo = 0, txyo = 0;
         if (ELASTIC) { txo = A.tau_old0[e * NQ + q]; tyo = A.tau_old1[e * NQ + q]; txyo = A.tau_old2[e * NQ + q]; }
         F Pq = 0;
 #pragma unroll
-        for (int i = 0; i < NP; ++i) Pq += (F)c_tab.NP1[q][i] * P[i];
+        for (int i = 0; i < NP; ++i) Pq += (F)kT.NP1[q][i] * P[i];
         if (A.Pf) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) Pq -= (F)c_tab.NP1[q][i] * A.Pf[e * NP + i];
+            for (int i = 0; i < NP; ++i) Pq -= (F)kT.NP1[q][i] * A.Pf[e * NP + i];
         }
-        D3<F> exx(gxx, F(1), F(0), F(0)), eyy(gyy, F(0), F(1), F(0)), exy((gxy + gyx) / F(2), F(0), F(0), F(1));
+        D3<F> exx(gxx, F(1), F(0), F(0)), eyy(gyy, F(0), F(1), F(0)), exy((gxy + gyx) * F(0.5), F(0), F(0), F(1));
         D3<F> txx, tyy, txy;
-        stress2d<D3<F>, F, PLASTIC>(exx, eyy, exy, m, A.dt, txo, tyo, txyo, Pq, txx, tyy, txy);
+        stress2d<D3<F>, F, PLASTIC>(exx, eyy, exy, m, txo, tyo, txyo, Pq, txx, tyy, txy);
         Cq[q][0][0] = txx.d0; Cq[q][0][1] = txx.d1; Cq[q][0][2] = txx.d2;
         Cq[q][1][0] = tyy.d0; Cq[q][1][1] = tyy.d1; Cq[q][1][2] = tyy.d2;
         Cq[q][2][0] = txy.d0; Cq[q][2][1] = txy.d1; Cq[q][2][2] = txy.d2;
@@ -569,10 +718,10 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
             for (int b = 0; b < NV; ++b) { Jx[c][b] = 0; Jy[c][b] = 0; }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const F da0 = (F)c_tab.dN[q][a][0], da1 = (F)c_tab.dN[q][a][1];
+            const F da0 = (F)kT.dN[q][a][0], da1 = (F)kT.dN[q][a][1];
             const F nax = da0 * iJ[0][0] + da1 * iJ[1][0];
             const F nay = da0 * iJ[0][1] + da1 * iJ[1][1];
-            const F dOm = (F)c_tab.w[q] * det;
+            const F dOm = (F)kT.w[q] * det;
             // R_x[a] = nax τxx + nay τxy ; R_y[a] = nax τxy + nay τyy
             F wx[3], wy[3];
 #pragma unroll
@@ -592,7 +741,7 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
             const F zyy1 = wy[1] * iJ[1][1] + F(0.5) * wy[2] * iJ[1][0];
 #pragma unroll
             for (int b = 0; b < NV; ++b) {
-                const F d0 = (F)c_tab.dN[q][b][0], d1 = (F)c_tab.dN[q][b][1];
+                const F d0 = (F)kT.dN[q][b][0], d1 = (F)kT.dN[q][b][1];
                 Jx[0][b] += d0 * zxx0 + d1 * zxx1;
                 Jx[1][b] += d0 * zxy0 + d1 * zxy1;
                 Jy[0][b] += d0 * zyx0 + d1 * zyx1;
@@ -603,12 +752,12 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
         // Bm[j][c][b] = Σ_q NP1_q[j] ∂N_b/∂x_c(q) dΩ_q = det Σ_k TB[j][b][k] J^-1[k][c]
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-            const F ta0 = (F)c_tab.TB[j][a][0], ta1 = (F)c_tab.TB[j][a][1];
+            const F ta0 = (F)kT.TB[j][a][0], ta1 = (F)kT.TB[j][a][1];
             const F fx = (ta0 * iJ[0][0] + ta1 * iJ[1][0]) * det * ps[j];
             const F fy = (ta0 * iJ[0][1] + ta1 * iJ[1][1]) * det * ps[j];
 #pragma unroll
             for (int b = 0; b < NV; ++b) {
-                const F t0 = (F)c_tab.TB[j][b][0], t1 = (F)c_tab.TB[j][b][1];
+                const F t0 = (F)kT.TB[j][b][0], t1 = (F)kT.TB[j][b][1];
                 const F bx = (t0 * iJ[0][0] + t1 * iJ[1][0]) * det;
                 const F by = (t0 * iJ[0][1] + t1 * iJ[1][1]) * det;
                 Jx[0][b] += fx * bx; Jx[1][b] += fx * by;
@@ -684,15 +833,15 @@ __global__ __launch_bounds__(BLOCK) void k_pressure_scaling(int64_t nels, const 
     for (int q = 0; q < NQ; ++q) {
         double eta = 0, bq = 0;
         for (int a = 0; a < NV; ++a) {
-            eta += c_tab.N[q][a] * s_tab[PC_ETA * MAXPH + pv[a]];
-            bq += c_tab.N[q][a] * (1.0 / s_tab[PC_BULK * MAXPH + pv[a]]);
+            eta += kT.N[q][a] * s_tab[PC_ETA * MAXPH + pv[a]];
+            bq += kT.N[q][a] * (1.0 / s_tab[PC_BULK * MAXPH + pv[a]]);
         }
         const double gnum = gamma_fact * eta;
         const double gphy = bq == 0.0 ? gnum : dt / bq;
         const double gq = gphy * gnum / (gphy + gnum);
-        const double dO = c_tab.w[q] * det;
+        const double dO = kT.w[q] * det;
         for (int i = 0; i < NP; ++i) {
-            const double w = c_tab.NP1[q][i] * dO;
+            const double w = kT.NP1[q][i] * dO;
             M[i] += w;
             g[i] += w * gq;
         }
@@ -828,6 +977,13 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce_final(int nparts, int K, i
         if (threadIdx.x == 0) out[k] = s[0];
         __syncthreads();
     }
+}
+
+// sets *flag when any entry is nonzero (a yes/no answer: atomic order is irrelevant)
+template <class F>
+__global__ void k_any_nonzero(int64_t n, const F* x, int32_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && x[i] != F(0)) atomicOr(flag, 1);
 }
 
 template <class T>

@@ -30,7 +30,9 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libfemdr.so")
+# FEMDR_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different launch parameters); default: the in-tree build.
+_LIB_PATH = os.environ.get("FEMDR_LIBRARY") or os.path.join(_HERE, "libfemdr.so")
 MAXPH = 64
 
 FEM_OK, FEM_EARG, FEM_EDIM, FEM_ENOCONV, FEM_ENAN, FEM_EKABOOM, FEM_EPRECOND, FEM_EHIP, FEM_ENOMEM = (

@@ -22,16 +22,17 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(REPO, "include"), *SOURCES, "-o", LIB + ".tmp"]
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    cmd = [HIPCC, *FLAGS, *(f"-D{d}" for d in defines), "-I", os.path.join(REPO, "include"), *SOURCES,
+           "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libfemdr.so")
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

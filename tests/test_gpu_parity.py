@@ -206,10 +206,13 @@ def test_solve_parity_vep_time_steps():
         tau_o = [t.copy() for t in pr.update_current_stress(w.dt)]
         fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp_g)
         # τ is a derived field: strain rates difference the velocity over an
-        # element, so the solved fields' 1e-10 agreement is held to 1e-9 here
+        # element, so the solved fields' 1e-10 agreement is held to 1e-9 here,
+        # relative to the scale of the whole stress tensor
+        tscale = max(np.abs(t).max() for t in tau_o)
         for c, comp in enumerate(("xx", "yy", "xy")):
-            assert rel_err(getattr(dr.tau, comp).numpy(), tau_o[c]) < 1e-9
-            getattr(dr.tau_old, comp).copyto(getattr(dr.tau, comp).numpy())
+            got = getattr(dr.tau, comp).numpy()
+            assert np.abs(got - tau_o[c]).max() / tscale < 1e-9
+            getattr(dr.tau_old, comp).copyto(got)
         t = tau_o
         tII = np.sqrt((t[0] ** 2 + t[1] ** 2 + (t[0] + t[1]) ** 2) / 2 + t[2] ** 2)
         yielded = yielded or tII.max() > 0.5 * math.cos(math.radians(30))
@@ -294,3 +297,34 @@ def test_gpu_error_paths():
     assert not st["converged"] and st["reached_total_iter"]
     with pytest.raises(ValueError):
         fa.solve(dr, mesh, bcx, bcy, dt=-1.0)
+
+
+def test_structured_and_unstructured_paths_agree():
+    # The structured box mesh derives connectivity, geometry and node->element
+    # slots from indices; an uploaded mesh reads them from arrays. The affine
+    # maps differ in the last bit (1/dx vs the inverse of J from coordinates),
+    # so the converged fields are held to the solve parity bar of 1e-10.
+    say("[gpu] structured vs uploaded mesh paths")
+    n = 9
+    w = PB.vep_inclusions(n, ninc=3, seed=4, C=0.5)
+    ms = fa.t7_box_mesh(n, n)
+    mu = fa.t7_mesh_from_arrays(ms.coords, ms.el2n)
+    ph = PB.inclusion_phases(ms.coords, ms.el2n, w.circles)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(ms.coords)
+    dp = fa.DruckerPrager(w.plastic["phi"], w.plastic["psi"], w.plastic["C"], w.plastic["eta_reg"], w.plastic["Kb"])
+    outs = []
+    for mesh in (ms, mu):
+        dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g),
+                         CFL_v=w.CFL_v)
+        dr.set_phases("v", ph)
+        dr.set_phases("P", ph)
+        v0x, v0y = PB.pure_shear_field(ms.coords)
+        dr.v.x.copyto(v0x)
+        dr.v.y.copyto(v0y)
+        st = fa.solve(dr, mesh, fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy),
+                      dt=w.dt, plastic=dp, **w.solve_kwargs())
+        assert st["converged"]
+        outs.append((st["iter"], dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy()))
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        assert rel_err(a, b) < 1e-10
