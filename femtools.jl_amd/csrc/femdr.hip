@@ -114,6 +114,15 @@ struct fem_stokes {
     int32_t* d_flag = nullptr;      // scratch yes/no flag
     bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
     bool q_nonzero = false;         // dr.Q holds a nonzero entry
+    // FP32 work arrays of the mixed-precision solve (allocated by its first call):
+    // the correction (δv, δP), its DR rates/residuals, the right-hand sides R(x)
+    // and FP32 copies of the scaling, preconditioner and geometry
+    struct W32 {
+        float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
+        float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
+        float2* Re;
+    };
+    W32* w32 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -535,6 +544,11 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
     hipFree(s->d_flag);
+    if (s->w32) {
+        float** p = (float**)s->w32;
+        for (size_t k = 0; k < sizeof(fem_stokes::W32) / sizeof(float*); ++k) hipFree(p[k]);
+        delete s->w32;
+    }
     delete s;
     return FEM_OK;
 }
@@ -696,115 +710,192 @@ struct LaunchOpts {
 constexpr int EBLOCK = 128;
 constexpr int GBLOCK = 256;
 
-template <int FL>
-static void launch_elem_t(fem_stokes* s, const ElemArgs<double>& a) {
-    hipLaunchKernelGGL((k_t7_element<double, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
+// One set of DR arrays in precision F. The FP64 solve runs on the state's own
+// fields; a correction solve of the mixed-precision path runs on FP32 work
+// arrays holding the correction (δv, δP) of the FP64 solution, with the FP64
+// residual R(x) as right-hand side.
+template <class F>
+struct DRSet {
+    F *vx = nullptr, *vy = nullptr, *rate_x = nullptr, *rate_y = nullptr;
+    F *Rv_x = nullptr, *Rv_y = nullptr, *Rv0_x = nullptr, *Rv0_y = nullptr, *PC_x = nullptr, *PC_y = nullptr;
+    F *P = nullptr, *Pold = nullptr, *RP = nullptr, *Pnum = nullptr, *gammaP = nullptr, *MP = nullptr,
+      *pscale = nullptr;
+    F *tau_old[3] = {nullptr, nullptr, nullptr}, *tau[3] = {nullptr, nullptr, nullptr};
+    F *T = nullptr, *T0 = nullptr, *Q = nullptr, *Pf = nullptr;
+    F *bcval_x = nullptr, *bcval_y = nullptr;
+    const double* ptab = nullptr;   // per-phase table (read in FP64, converted at use)
+    F* geo = nullptr;
+    typename Vec2<F>::T* Re = nullptr;
+    const F *rhs_P = nullptr, *rhs_vx = nullptr, *rhs_vy = nullptr;
+    bool lin = false;   // linearised visco-elastic correction operator (EF_LIN)
+};
+
+static DRSet<double> set64(fem_stokes* s) {
+    DRSet<double> d;
+    double** f = s->f;
+    d.vx = f[FEM_FIELD_V_X]; d.vy = f[FEM_FIELD_V_Y];
+    d.rate_x = f[FEM_FIELD_DVDTAU_X]; d.rate_y = f[FEM_FIELD_DVDTAU_Y];
+    d.Rv_x = f[FEM_FIELD_RV_X]; d.Rv_y = f[FEM_FIELD_RV_Y];
+    d.Rv0_x = f[FEM_FIELD_RV0_X]; d.Rv0_y = f[FEM_FIELD_RV0_Y];
+    d.PC_x = f[FEM_FIELD_PC_V_X]; d.PC_y = f[FEM_FIELD_PC_V_Y];
+    d.P = f[FEM_FIELD_P]; d.Pold = f[FEM_FIELD_P0]; d.RP = f[FEM_FIELD_RP]; d.Pnum = f[FEM_FIELD_PNUM];
+    d.gammaP = f[FEM_FIELD_GAMMA_P]; d.MP = f[FEM_FIELD_M_P]; d.pscale = s->d_pscale;
+    d.tau_old[0] = f[FEM_FIELD_TAU_OLD_XX]; d.tau_old[1] = f[FEM_FIELD_TAU_OLD_YY];
+    d.tau_old[2] = f[FEM_FIELD_TAU_OLD_XY];
+    d.tau[0] = f[FEM_FIELD_TAU_XX]; d.tau[1] = f[FEM_FIELD_TAU_YY]; d.tau[2] = f[FEM_FIELD_TAU_XY];
+    d.T = f[FEM_FIELD_T]; d.T0 = f[FEM_FIELD_T0]; d.Q = f[FEM_FIELD_Q]; d.Pf = f[FEM_FIELD_PF];
+    d.bcval_x = s->d_bcval_x; d.bcval_y = s->d_bcval_y;
+    d.ptab = s->d_ptab; d.geo = s->mesh->d_geo;
+    d.Re = s->d_Re;
+    return d;
+}
+
+static DRSet<float> set32(fem_stokes* s) {
+    DRSet<float> d;
+    fem_stokes::W32& w = *s->w32;
+    d.vx = w.vx; d.vy = w.vy; d.rate_x = w.rate_x; d.rate_y = w.rate_y;
+    d.Rv_x = w.Rv_x; d.Rv_y = w.Rv_y; d.Rv0_x = w.Rv0_x; d.Rv0_y = w.Rv0_y; d.PC_x = w.PC_x; d.PC_y = w.PC_y;
+    d.P = w.P; d.RP = w.RP; d.Pnum = w.Pnum; d.gammaP = w.gammaP; d.MP = w.MP; d.pscale = w.pscale;
+    d.bcval_x = w.zero_v; d.bcval_y = w.zero_v;   // the correction vanishes on Dirichlet nodes
+    d.ptab = s->d_ptab; d.geo = w.geo;
+    d.Re = w.Re;
+    d.rhs_P = w.rhs_P; d.rhs_vx = w.rhs_vx; d.rhs_vy = w.rhs_vy;
+    d.lin = true;
+    return d;
+}
+
+template <class F, int FL>
+static void launch_elem_t(fem_stokes* s, const ElemArgs<F>& a) {
+    hipLaunchKernelGGL((k_t7_element<F, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
                        s->ctx->stream, a);
 }
 
 template <int FL>
 static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
-    if (key == FL) launch_elem_t<FL>(s, a);
+    if (key == FL) launch_elem_t<double, FL>(s, a);
     else if constexpr (FL > 0) dispatch_elem<FL - 1>(key, s, a);
 }
 
-static int launch_element(fem_stokes* s, const LaunchOpts& o) {
+// the correction operator: EF_LIN with any of EF_BULK, EF_ITER, EF_GEN, EF_STRUCT
+template <int I>
+static void dispatch_lin(int key, fem_stokes* s, const ElemArgs<float>& a) {
+    constexpr int FL = EF_LIN | ((I & 1) ? EF_BULK : 0) | ((I & 2) ? EF_ITER : 0) | ((I & 4) ? EF_GEN : 0) |
+                       ((I & 8) ? EF_STRUCT : 0);
+    if (key == FL) launch_elem_t<float, FL>(s, a);
+    else if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
+}
+
+template <class F>
+static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
     fem_mesh* m = s->mesh;
-    ElemArgs<double> a;
+    ElemArgs<F> a;
     std::memset(&a, 0, sizeof a);
     a.nels = m->nels;
     a.el2n = m->d_el2n;
-    a.geo = m->d_geo;
-    a.vx = s->f[FEM_FIELD_V_X];
-    a.vy = s->f[FEM_FIELD_V_Y];
-    a.P = s->f[FEM_FIELD_P];
-    a.Pold = s->f[FEM_FIELD_P0];
-    a.pscale = s->d_pscale;
-    a.tau_old0 = s->f[FEM_FIELD_TAU_OLD_XX];
-    a.tau_old1 = s->f[FEM_FIELD_TAU_OLD_YY];
-    a.tau_old2 = s->f[FEM_FIELD_TAU_OLD_XY];
+    a.geo = d.geo;
+    a.vx = d.vx;
+    a.vy = d.vy;
+    a.P = d.P;
+    a.Pold = d.Pold;
+    a.pscale = d.pscale;
+    a.tau_old0 = d.tau_old[0];
+    a.tau_old1 = d.tau_old[1];
+    a.tau_old2 = d.tau_old[2];
     a.phase_v = s->d_phase_v;
     a.phase_P = s->d_phase_P;
-    a.ptab = s->d_ptab;
+    a.ptab = d.ptab;
     a.nphases = s->mat.nphases;
-    a.gx = s->mat.g[0];
-    a.gy = s->mat.g[1];
-    a.Tref = s->mat.Tref;
-    a.dt = o.dt;
-    a.Re = s->d_Re;
-    a.rate_x = s->f[FEM_FIELD_DVDTAU_X];
-    a.rate_y = s->f[FEM_FIELD_DVDTAU_Y];
-    a.PCx = s->f[FEM_FIELD_PC_V_X];
-    a.PCy = s->f[FEM_FIELD_PC_V_Y];
+    a.gx = (F)s->mat.g[0];
+    a.gy = (F)s->mat.g[1];
+    a.Tref = (F)s->mat.Tref;
+    a.dt = (F)o.dt;
+    a.Re = d.Re;
+    a.rate_x = d.rate_x;
+    a.rate_y = d.rate_y;
+    a.PCx = d.PC_x;
+    a.PCy = d.PC_y;
     a.bcflag = s->d_bcflag;
-    a.bcval_x = s->d_bcval_x;
-    a.bcval_y = s->d_bcval_y;
-    a.alpha_x = o.alpha[0]; a.alpha_y = o.alpha[1];
-    a.beta_x = o.beta[0]; a.beta_y = o.beta[1];
+    a.bcval_x = d.bcval_x;
+    a.bcval_y = d.bcval_y;
+    a.alpha_x = (F)o.alpha[0]; a.alpha_y = (F)o.alpha[1];
+    a.beta_x = (F)o.beta[0]; a.beta_y = (F)o.beta[1];
     a.do_update = o.do_update;
-    a.RP_out = o.store_RP ? s->f[FEM_FIELD_RP] : nullptr;
-    a.Pnum_out = o.store_Pnum ? s->f[FEM_FIELD_PNUM] : nullptr;
-    if (o.store_Rv) { a.Rv_out_x = s->f[FEM_FIELD_RV_X]; a.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
-    if (o.store_Rv0) { a.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; a.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
+    a.RP_out = o.store_RP ? d.RP : nullptr;
+    a.Pnum_out = o.store_Pnum ? d.Pnum : nullptr;
+    if (o.store_Rv) { a.Rv_out_x = d.Rv_x; a.Rv_out_y = d.Rv_y; }
+    if (o.store_Rv0) { a.Rv0_out_x = d.Rv0_x; a.Rv0_out_y = d.Rv0_y; }
     if (o.store_tau) {
-        a.tau_out0 = s->f[FEM_FIELD_TAU_XX];
-        a.tau_out1 = s->f[FEM_FIELD_TAU_YY];
-        a.tau_out2 = s->f[FEM_FIELD_TAU_XY];
+        a.tau_out0 = d.tau[0];
+        a.tau_out1 = d.tau[1];
+        a.tau_out2 = d.tau[2];
     }
+    a.rhs_P = d.rhs_P;
+    a.rhs_vx = d.rhs_vx;
+    a.rhs_vy = d.rhs_vy;
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
-    const bool gen = nodeph || o.thermal || s->pf_nonzero || s->q_nonzero;
+    // the correction operator has no temperature, source or fluid-pressure terms
+    const bool gen = nodeph || (!d.lin && (o.thermal || s->pf_nonzero || s->q_nonzero));
     a.nodeph = nodeph;
     a.bulk_all_inf = o.bulk_inf;
     a.thermal = o.thermal;
-    a.T = s->f[FEM_FIELD_T];
-    a.T0 = s->f[FEM_FIELD_T0];
-    a.Q = s->q_nonzero ? s->f[FEM_FIELD_Q] : nullptr;
-    a.Pf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
+    a.T = d.T;
+    a.T0 = d.T0;
+    a.Q = s->q_nonzero ? d.Q : nullptr;
+    a.Pf = s->pf_nonzero ? d.Pf : nullptr;
     const bool strct = m->nx > 0;
     if (strct) {
         const double dx = (m->box[1] - m->box[0]) / m->nx, dy = (m->box[3] - m->box[2]) / m->ny;
         a.nx = m->nx;
         a.sx = 2 * m->nx + 1;
         a.nT6 = (int64_t)(2 * m->nx + 1) * (2 * m->ny + 1);
-        a.idx = 1.0 / dx;
-        a.idy = 1.0 / dy;
-        a.detc = dx * dy;
+        a.idx = (F)(1.0 / dx);
+        a.idy = (F)(1.0 / dy);
+        a.detc = (F)(dx * dy);
     }
     a.phase_same = s->phase_same && !nodeph;
-    const int key = (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0) | (!o.bulk_inf ? EF_BULK : 0) |
-                    (o.use_pnum ? EF_ITER : 0) | (gen ? EF_GEN : 0) | (strct ? EF_STRUCT : 0);
-    dispatch_elem<63>(key, s, a);
+    const int key = (!o.bulk_inf ? EF_BULK : 0) | (o.use_pnum ? EF_ITER : 0) | (gen ? EF_GEN : 0) |
+                    (strct ? EF_STRUCT : 0);
+    if constexpr (std::is_same<F, double>::value) {
+        if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
+        dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
+    } else {
+        if (!d.lin) return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for the correction operator only");
+        dispatch_lin<15>(key | EF_LIN, s, a);
+    }
     return check_launch(s->ctx, "k_t7_element");
 }
 
-static int launch_gather(fem_stokes* s, const LaunchOpts& o) {
+template <class F>
+static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
     fem_mesh* m = s->mesh;
-    GatherArgs<double> g;
+    GatherArgs<F> g;
     std::memset(&g, 0, sizeof g);
     g.ns = m->ns;
     g.node_of = m->d_node_of;
     g.off = m->d_off;
     g.ent = m->d_ent;
-    g.Re = s->d_Re;
-    g.vx = s->f[FEM_FIELD_V_X];
-    g.vy = s->f[FEM_FIELD_V_Y];
-    g.rate_x = s->f[FEM_FIELD_DVDTAU_X];
-    g.rate_y = s->f[FEM_FIELD_DVDTAU_Y];
-    g.PCx = s->f[FEM_FIELD_PC_V_X];
-    g.PCy = s->f[FEM_FIELD_PC_V_Y];
+    g.Re = d.Re;
+    g.vx = d.vx;
+    g.vy = d.vy;
+    g.rate_x = d.rate_x;
+    g.rate_y = d.rate_y;
+    g.PCx = d.PC_x;
+    g.PCy = d.PC_y;
     g.bcflag = s->d_bcflag;
-    g.bcval_x = s->d_bcval_x;
-    g.bcval_y = s->d_bcval_y;
-    if (o.store_Rv) { g.Rv_out_x = s->f[FEM_FIELD_RV_X]; g.Rv_out_y = s->f[FEM_FIELD_RV_Y]; }
-    if (o.store_Rv0) { g.Rv0_out_x = s->f[FEM_FIELD_RV0_X]; g.Rv0_out_y = s->f[FEM_FIELD_RV0_Y]; }
-    g.alpha_x = o.alpha[0]; g.alpha_y = o.alpha[1];
-    g.beta_x = o.beta[0]; g.beta_y = o.beta[1];
+    g.bcval_x = d.bcval_x;
+    g.bcval_y = d.bcval_y;
+    if (o.store_Rv) { g.Rv_out_x = d.Rv_x; g.Rv_out_y = d.Rv_y; }
+    if (o.store_Rv0) { g.Rv0_out_x = d.Rv0_x; g.Rv0_out_y = d.Rv0_y; }
+    g.alpha_x = (F)o.alpha[0]; g.alpha_y = (F)o.alpha[1];
+    g.beta_x = (F)o.beta[0]; g.beta_y = (F)o.beta[1];
     g.do_update = o.do_update;
+    g.rhs_vx = d.rhs_vx;
+    g.rhs_vy = d.rhs_vy;
     // node -> element-slot lists are read from the CSR arrays for every mesh:
     // deriving them from indices on structured meshes lengthens the path to
     // the loads more than the list reads cost
-    hipLaunchKernelGGL((k_gather_update<double, false, GBLOCK>), dim3(grid_xcd(m->ns, GBLOCK)), dim3(GBLOCK), 0,
+    hipLaunchKernelGGL((k_gather_update<F, false, GBLOCK>), dim3(grid_xcd(m->ns, GBLOCK)), dim3(GBLOCK), 0,
                        s->ctx->stream, g);
     return check_launch(s->ctx, "k_gather_update");
 }
@@ -891,16 +982,14 @@ __global__ void k_pscale(int64_t n, const double* g, const double* M, double* ps
 }
 
 // reductions -> host scalars (synchronises)
-static int reduce_check(fem_stokes* s, const double al[2], double out[6]) {
+template <class F>
+static int reduce_check(fem_stokes* s, const DRSet<F>& d, const double al[2], double out[6]) {
     fem_ctx* c = s->ctx;
     const int64_t nv = s->mesh->nnodes;
     const unsigned g = std::min<unsigned>(RED_GRID, grid_for(nv, RED_BLOCK));
-    hipLaunchKernelGGL(k_reduce_check<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv,
-                       (const double*)s->f[FEM_FIELD_RV_X], (const double*)s->f[FEM_FIELD_RV_Y],
-                       (const double*)s->f[FEM_FIELD_RV0_X], (const double*)s->f[FEM_FIELD_RV0_Y],
-                       (const double*)s->f[FEM_FIELD_DVDTAU_X], (const double*)s->f[FEM_FIELD_DVDTAU_Y],
-                       (const double*)s->f[FEM_FIELD_PC_V_X], (const double*)s->f[FEM_FIELD_PC_V_Y], al[0], al[1],
-                       c->red_partial);
+    hipLaunchKernelGGL(k_reduce_check<F>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, (const F*)d.Rv_x,
+                       (const F*)d.Rv_y, (const F*)d.Rv0_x, (const F*)d.Rv0_y, (const F*)d.rate_x, (const F*)d.rate_y,
+                       (const F*)d.PC_x, (const F*)d.PC_y, al[0], al[1], c->red_partial);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 6, 0,
                        (const double*)c->red_partial, c->red_out);
     CHK(check_launch(c, "k_reduce_check"));
@@ -910,13 +999,13 @@ static int reduce_check(fem_stokes* s, const double al[2], double out[6]) {
     return FEM_OK;
 }
 
-static int reduce_outer(fem_stokes* s, double out[3]) {
+template <class F>
+static int reduce_outer(fem_stokes* s, const DRSet<F>& d, double out[3]) {
     fem_ctx* c = s->ctx;
     const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP;
     const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, np), RED_BLOCK));
-    hipLaunchKernelGGL(k_reduce_outer<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np,
-                       (const double*)s->f[FEM_FIELD_RV_X], (const double*)s->f[FEM_FIELD_RV_Y],
-                       (const double*)s->f[FEM_FIELD_RP], (const double*)s->f[FEM_FIELD_M_P], c->red_partial);
+    hipLaunchKernelGGL(k_reduce_outer<F>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np, (const F*)d.Rv_x,
+                       (const F*)d.Rv_y, (const F*)d.RP, (const F*)d.MP, c->red_partial);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
                        (const double*)c->red_partial, c->red_out);
     CHK(check_launch(c, "k_reduce_outer"));
@@ -996,6 +1085,366 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
     return FEM_OK;
 }
 
+// Control and result of one PH/DYREL run (_solve_stokes_dyrel!, DR.jl:348-802).
+struct DRControl {
+    double tol = 0;            // converged when min(err_abs, err_rel) < tol
+    int64_t max_iterations = 0, iter_max = 0, max_ph = 0, ncheck = 1;
+    double rel_drop0 = 1e-2;
+    bool freeze = true;
+    int verbose = 0;
+    const char* tag = "";
+};
+struct DRResult {
+    int64_t itPH = 0, iter = 0, launches = 0, jac = 0;
+    double err = INFINITY, err_abs = INFINITY, err_rel = INFINITY, err_v = INFINITY, err_P = INFINITY;
+    double lmax_g = 0;
+    bool converged = false;
+};
+
+// The outer Powell-Hestenes loop (outer residuals without Pnum, errors,
+// rel_drop, P += γP RP / M_P) around the inner DYREL loop (Chebyshev-damped
+// pseudo-transient steps, λmin from the residual change at each check, λmax
+// from re-assembled Jacobian diagnostics unless frozen). lmax holds the
+// current λmax on entry and on return.
+template <class F>
+static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const DRControl& ctl, double lmax[2],
+                    KernelTimer& timer, DRResult& r) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP, ncheck = ctl.ncheck;
+    double al[2], be[2];
+    for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
+    r.lmax_g = std::max(lmax[0], lmax[1]);
+    const double eps_tol = ctl.tol;
+    double err_min = INFINITY, err = 2 * eps_tol;
+    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
+    double rel_drop = ctl.rel_drop0;
+    int64_t iter = 0;
+    int rc = FEM_OK;
+
+    for (int64_t itPH = 1; itPH <= ctl.max_ph; ++itPH) {
+        r.itPH = itPH;
+        // outer residuals without Pnum (DR.jl:535-554)
+        LaunchOpts oo = o;
+        oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
+        CHK(launch_element(s, d, oo));
+        CHK(launch_gather(s, d, oo));
+        double r3[3];
+        CHK(reduce_outer(s, d, r3));
+        timer.collect();
+        r.err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
+        r.err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
+        if (itPH == 1) { err_P0 = r.err_P + julia_eps(r.err_P); err_v0 = r.err_v + julia_eps(r.err_v); }
+        else if (itPH == 2) { err_P0 = r.err_P + julia_eps(r.err_P); }
+        r.err_abs = std::max(r.err_v, r.err_P);
+        r.err_rel = std::max(r.err_v / err_v0, r.err_P / err_P0);
+        err = std::min(r.err_abs, r.err_rel);
+        if (std::isnan(r.err_v) || std::isnan(r.err_P) || std::isnan(err)) {
+            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
+            break;
+        }
+        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
+        if (ctl.verbose)
+            printf("%sitPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
+                   ctl.tag, (long long)itPH, (long long)iter, err, r.err_abs, r.err_rel, r.err_v, r.err_v / err_v0,
+                   r.err_P, r.err_P / err_P0);
+        if (err < ctl.tol) { r.converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        bool stop_total = false;
+        while (err > eps_vel && itPT <= ctl.iter_max) {
+            ++itPT;
+            ++iter;
+            const bool do_check = iter % ncheck == 0;
+            // an iteration after which the loop may end stores what the end needs
+            const bool last_possible = do_check || itPT > ctl.iter_max || iter > ctl.max_iterations;
+            if (do_check && ncheck == 1) {
+                HIPCHK(c, hipMemcpyAsync(d.Rv0_x, d.Rv_x, nv * sizeof(F), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(d.Rv0_y, d.Rv_y, nv * sizeof(F), hipMemcpyDeviceToDevice, c->stream));
+            }
+            LaunchOpts oi = o;
+            oi.use_pnum = true;
+            oi.do_update = true;
+            oi.store_RP = last_possible;
+            oi.store_Pnum = last_possible;
+            oi.store_Rv = last_possible;
+            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
+            oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
+            oi.beta[0] = be[0]; oi.beta[1] = be[1];
+            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
+                       e2 = timer.on ? timer.ev() : nullptr;
+            timer.mark(e0);
+            CHK(launch_element(s, d, oi));
+            timer.mark(e1);
+            CHK(launch_gather(s, d, oi));
+            timer.mark(e2);
+            ++r.launches;
+            if (do_check) {
+                double r6[6];
+                CHK(reduce_check(s, d, al, r6));
+                timer.collect();
+                const double evi = std::max(std::sqrt(r6[0]), std::sqrt(r6[1])) / (2 * std::sqrt((double)nv));
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (ctl.verbose > 1) printf("%s  it = %lld, iter = %lld, err = %.3e\n", ctl.tag, (long long)itPT, (long long)iter, err);
+                double lmin[2];
+                for (int k = 0; k < 2; ++k) {
+                    const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
+                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+                }
+                if constexpr (std::is_same<F, double>::value) {
+                    if (!ctl.freeze) {
+                        ++r.jac;
+                        CHK(launch_jacobian(s, o));
+                        rc = checked_lmax(s, lmax);
+                        if (rc != FEM_OK) break;
+                        r.lmax_g = std::max(lmax[0], lmax[1]);
+                    }
+                }
+                for (int k = 0; k < 2; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+            }
+            if (ctl.verbose && itPT == ctl.iter_max) printf("%s  inner: max iters (%lld) reached at PH=%lld\n", ctl.tag, (long long)ctl.iter_max, (long long)itPH);
+            if (iter > ctl.max_iterations) { stop_total = true; break; }
+        }
+        if (rc != FEM_OK) break;
+        hipLaunchKernelGGL(k_pressure_update<F>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, d.P,
+                           (const F*)d.gammaP, (const F*)d.RP, (const F*)d.MP);
+        CHK(check_launch(c, "k_pressure_update"));
+        if (stop_total || iter > ctl.max_iterations) break;
+    }
+    r.iter = iter;
+    r.err = err;
+    return rc;
+}
+
+__global__ void k_pin_bc(int64_t n, const uint8_t* flag, const double* bx, const double* by, double* vx, double* vy) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        if (flag[i] & 1) vx[i] = bx[i];
+        if (flag[i] & 2) vy[i] = by[i];
+    }
+}
+
+static int ensure_w32(fem_stokes* s) {
+    if (s->w32) return FEM_OK;
+    fem_mesh* m = s->mesh;
+    fem_ctx* c = s->ctx;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ne = m->nels;
+    s->w32 = new fem_stokes::W32();
+    std::memset(s->w32, 0, sizeof(fem_stokes::W32));
+    fem_stokes::W32& w = *s->w32;
+    float** vl[] = {&w.vx, &w.vy, &w.rate_x, &w.rate_y, &w.Rv_x, &w.Rv_y, &w.Rv0_x,
+                    &w.Rv0_y, &w.PC_x, &w.PC_y, &w.rhs_vx, &w.rhs_vy, &w.zero_v};
+    for (float** q : vl) CHK(dalloc_t(c, q, nv));
+    float** pl[] = {&w.P, &w.RP, &w.Pnum, &w.gammaP, &w.MP, &w.pscale, &w.rhs_P};
+    for (float** q : pl) CHK(dalloc_t(c, q, np));
+    CHK(dalloc_t(c, &w.geo, 5 * ne));
+    CHK(dalloc_t(c, &w.Re, NSH * ne));
+    HIPCHK(c, hipMemsetAsync(w.zero_v, 0, nv * 4, c->stream));
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(5 * ne, 256)), dim3(256), 0, c->stream, 5 * ne,
+                       (const double*)m->d_geo, w.geo, 1.0);
+    return check_launch(c, "k_cast");
+}
+
+static int cast_to32(fem_ctx* c, const double* x, float* y, int64_t n, double scale = 1.0) {
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, y, scale);
+    return check_launch(c, "k_cast");
+}
+static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, double scale) {
+    hipLaunchKernelGGL(k_add_correction, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, dx, scale);
+    return check_launch(c, "k_add_correction");
+}
+
+// Mixed precision (precision = 1): the reference's Powell-Hestenes/DYREL
+// iteration with its state and outer residuals in FP64 and its inner
+// pseudo-transient iterations in FP32 on a correction.
+//
+// Each outer iteration evaluates the true residual R(x) (no Pnum, DR.jl:535-554)
+// in FP64 and applies the reference's errors, criterion and rel_drop rule to it
+// unchanged. The inner loop then iterates on a correction δ (zero at the start
+// of every outer iteration) of the linear problem R(x) + L δ = 0, L the
+// visco-elastic operator linearised at x, in FP32 with the reference's inner
+// criterion, Chebyshev parameters, λmin estimates and Pnum augmentation; the
+// pseudo-velocities (rates) carry over between outer iterations as in the
+// reference. At every convergence check δv is folded into the FP64 state and
+// the correction restarts from the FP64 residual there (one FP64 residual
+// evaluation per check interval); the pressure update acts on δP, folded in
+// before the next outer residual. L is exact for viscous and
+// visco-elastic rheology; under Drucker-Prager yielding it is the elastic
+// predictor, whose error the next outer residual measures and corrects. The
+// preconditioner and λmax come from the Jacobian diagnostics of L, assembled
+// once. The solution x meets the Dirichlet values before the first iteration.
+static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpts& o, KernelTimer& timer,
+                       fem_solve_stats* st) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
+    CHK(ensure_w32(s));
+    fem_stokes::W32& w = *s->w32;
+    const DRSet<double> d64 = set64(s);
+    const DRSet<float> d32 = set32(s);
+    CHK(cast_to32(c, d64.gammaP, w.gammaP, np));
+    CHK(cast_to32(c, d64.MP, w.MP, np));
+    CHK(cast_to32(c, s->d_pscale, w.pscale, np));
+    hipLaunchKernelGGL(k_pin_bc, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
+                       (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, d64.vx, d64.vy);
+    CHK(check_launch(c, "k_pin_bc"));
+    LaunchOpts oj = o;
+    oj.plastic = false;
+    CHK(launch_jacobian(s, oj));
+    double lmax[2];
+    CHK(checked_lmax(s, lmax));
+    CHK(cast_to32(c, d64.PC_x, w.PC_x, nv));
+    CHK(cast_to32(c, d64.PC_y, w.PC_y, nv));
+    for (float* z : {w.vx, w.vy, w.rate_x, w.rate_y, w.Rv0_x, w.Rv0_y}) HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g) (visco-elastic operator)\n", lmax[0], lmax[1]);
+
+    double al[2], be[2];
+    for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
+    const double eps_tol = p->tolerance;
+    double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
+    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
+    double rel_drop = p->rel_drop0;
+    int64_t iter = 0, itPH_done = 0, launches = 0;
+    bool converged = false;
+    int rc = FEM_OK;
+    LaunchOpts o32 = o;
+    o32.plastic = false;
+    o32.elastic = false;
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        LaunchOpts oo = o;
+        oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
+        CHK(launch_element(s, d64, oo));
+        CHK(launch_gather(s, d64, oo));
+        double r3[3];
+        CHK(reduce_outer(s, d64, r3));
+        timer.collect();
+        err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
+        err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v / err_v0, err_P / err_P0);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
+            break;
+        }
+        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
+                   (long long)itPH, (long long)iter, err, err_abs, err_rel, err_v, err_v / err_v0, err_P, err_P / err_P0);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        // right-hand side of the correction problem: the FP64 residual
+        CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
+        CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
+        CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+        int64_t itPT = 0;
+        bool stop_total = false;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT;
+            ++iter;
+            const bool do_check = iter % ncheck == 0;
+            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
+            if (do_check && ncheck == 1) {
+                HIPCHK(c, hipMemcpyAsync(w.Rv0_x, w.Rv_x, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(w.Rv0_y, w.Rv_y, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+            }
+            LaunchOpts oi = o32;
+            oi.use_pnum = true;
+            oi.do_update = true;
+            oi.store_RP = last_possible;
+            oi.store_Pnum = last_possible;
+            oi.store_Rv = last_possible;
+            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
+            oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
+            oi.beta[0] = be[0]; oi.beta[1] = be[1];
+            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
+                       e2 = timer.on ? timer.ev() : nullptr;
+            timer.mark(e0);
+            CHK(launch_element(s, d32, oi));
+            timer.mark(e1);
+            CHK(launch_gather(s, d32, oi));
+            timer.mark(e2);
+            ++launches;
+            if (do_check) {
+                double r6[6];
+                CHK(reduce_check(s, d32, al, r6));
+                timer.collect();
+                const double evi = std::max(std::sqrt(r6[0]), std::sqrt(r6[1])) / (2 * std::sqrt((double)nv));
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (p->verbose > 1) printf("  it = %lld, iter = %lld, err = %.3e\n", (long long)itPT, (long long)iter, err);
+                double lmin[2];
+                for (int k = 0; k < 2; ++k) {
+                    const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
+                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+                }
+                for (int k = 0; k < 2; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+                // fold δv into the FP64 state and restart the correction from the
+                // FP64 residual there: FP32 rounding of L δ is relative to the
+                // size of δ, so the FP32 iterations only ever reduce the residual
+                // of the last ncheck steps, far above their rounding floor, and
+                // the linearisation is renewed at the current state
+                CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
+                CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
+                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
+                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                CHK(launch_element(s, d64, oo));
+                CHK(launch_gather(s, d64, oo));
+                CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
+                CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
+                CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+            }
+            if (iter > p->max_iterations) { stop_total = true; break; }
+        }
+        if (rc != FEM_OK) break;
+        // P += γP RP / M_P on the correction, then fold δ into the FP64 state
+        hipLaunchKernelGGL(k_pressure_update<float>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, w.P,
+                           (const float*)w.gammaP, (const float*)w.RP, (const float*)w.MP);
+        CHK(check_launch(c, "k_pressure_update"));
+        CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
+        CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
+        CHK(add_correction(c, d64.P, w.P, np, 1.0));
+        HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+        if (stop_total || iter > p->max_iterations) break;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timer.collect();
+    st->kernel_ms_element = timer.ms_elem;
+    st->kernel_ms_gather = timer.ms_gather;
+    st->timed_launches = timer.n;
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_abs = err_abs;
+    st->err_rel = err_rel;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged;
+    st->reached_total_iter = iter > p->max_iterations;
+    st->lambda_max = std::max(lmax[0], lmax[1]);
+    st->lambda_max_gershgorin = st->lambda_max;
+    st->lambda_max_iterations = 0;
+    st->jacobian_assemblies = 1;
+    st->element_launches = launches;
+    return rc;
+}
+
 extern "C" {
 
 int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt, int32_t finite_K) {
@@ -1020,8 +1469,8 @@ int fem_stokes_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, 
     o.store_RP = true;
     o.store_Pnum = with_pnum != 0;
     o.store_Rv = true;
-    CHK(launch_element(s, o));
-    CHK(launch_gather(s, o));
+    CHK(launch_element(s, set64(s), o));
+    CHK(launch_gather(s, set64(s), o));
     return fem_sync(c);
 }
 
@@ -1046,8 +1495,8 @@ int fem_stokes_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp
     HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
     o.store_tau = true;
     o.store_Rv = true;
-    CHK(launch_element(s, o));
-    CHK(launch_gather(s, o));
+    CHK(launch_element(s, set64(s), o));
+    CHK(launch_gather(s, set64(s), o));
     return fem_sync(c);
 }
 
@@ -1061,9 +1510,9 @@ int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, 
     o.do_update = true;
     o.alpha[0] = o.alpha[1] = 0.0;
     o.beta[0] = o.beta[1] = 1.0;
-    CHK(launch_element(s, o));
+    CHK(launch_element(s, set64(s), o));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < n; ++i) CHK(launch_element(s, o));
+    for (int i = 0; i < n; ++i) CHK(launch_element(s, set64(s), o));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -1080,10 +1529,12 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
     std::memset(st, 0, sizeof *st);
     HIPCHK(c, hipSetDevice(c->device));
     if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
-    if (p->precision != 0) return set_err(c, FEM_EARG, "precision %d not available in this build", p->precision);
+    if (p->precision != 0 && p->precision != 1)
+        return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
+                       p->precision);
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
-    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
+    const int64_t nv = m->nnodes, np = m->nnodesP;
     const double dt = p->dt;
     LaunchOpts o;
     CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
@@ -1093,6 +1544,16 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
     else
         hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
                            (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    KernelTimer timer(c, p->time_kernels != 0);
+    if (p->precision == 1) {
+        const int rc = solve_mixed(s, p, o, timer, st);
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != FEM_OK) return rc;
+        if (!st->converged && p->throw_on_failure)
+            return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                           (long long)st->iter, st->err_abs, p->tolerance);
+        return FEM_OK;
+    }
     HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_X], 0, nv * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_Y], 0, nv * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_X], 0, nv * 8, c->stream));
@@ -1101,138 +1562,43 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
     CHK(launch_jacobian(s, o));
     double lmax[2];
     CHK(checked_lmax(s, lmax));
-    double lmax_g = std::max(lmax[0], lmax[1]);
-    int64_t jac = 1;
-    double al[2], be[2];
-    for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
     if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
 
-    const double eps_tol = p->tolerance;
-    double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
-    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
-    int64_t iter = 0, itPH_done = 0, launches = 0;
-    bool converged = false;
-    double rel_drop = p->rel_drop0;
-    int rc = FEM_OK;
-    KernelTimer timer(c, p->time_kernels != 0);
-
-    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
-        itPH_done = itPH;
-        // outer residuals without Pnum (DR.jl:535-554)
-        LaunchOpts oo = o;
-        oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
-        CHK(launch_element(s, oo));
-        CHK(launch_gather(s, oo));
-        double r3[3];
-        CHK(reduce_outer(s, r3));
-        timer.collect();
-        err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
-        err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
-        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
-        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
-        err_abs = std::max(err_v, err_P);
-        err_rel = std::max(err_v / err_v0, err_P / err_P0);
-        err = std::min(err_abs, err_rel);
-        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
-            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
-            break;
-        }
-        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
-        if (p->verbose)
-            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
-                   (long long)itPH, (long long)iter, err, err_abs, err_rel, err_v, err_v / err_v0, err_P, err_P / err_P0);
-        if (err < eps_tol) { converged = true; break; }
-        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
-        err_min = std::min(err_min, err);
-        const double eps_vel = err * rel_drop;
-        int64_t itPT = 0;
-        bool stop_total = false;
-        while (err > eps_vel && itPT <= p->iter_max) {
-            ++itPT;
-            ++iter;
-            const bool do_check = iter % ncheck == 0;
-            // an iteration after which the loop may end stores what the end needs
-            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
-            if (do_check && ncheck == 1) {
-                HIPCHK(c, hipMemcpyAsync(s->f[FEM_FIELD_RV0_X], s->f[FEM_FIELD_RV_X], nv * 8, hipMemcpyDeviceToDevice, c->stream));
-                HIPCHK(c, hipMemcpyAsync(s->f[FEM_FIELD_RV0_Y], s->f[FEM_FIELD_RV_Y], nv * 8, hipMemcpyDeviceToDevice, c->stream));
-            }
-            LaunchOpts oi = o;
-            oi.use_pnum = true;
-            oi.do_update = true;
-            oi.store_RP = last_possible;
-            oi.store_Pnum = last_possible;
-            oi.store_Rv = last_possible;
-            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
-            oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
-            oi.beta[0] = be[0]; oi.beta[1] = be[1];
-            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
-                       e2 = timer.on ? timer.ev() : nullptr;
-            timer.mark(e0);
-            CHK(launch_element(s, oi));
-            timer.mark(e1);
-            CHK(launch_gather(s, oi));
-            timer.mark(e2);
-            ++launches;
-            if (do_check) {
-                double r6[6];
-                CHK(reduce_check(s, al, r6));
-                timer.collect();
-                const double evi = std::max(std::sqrt(r6[0]), std::sqrt(r6[1])) / (2 * std::sqrt((double)nv));
-                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
-                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
-                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
-                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
-                if (p->verbose > 1) printf("  it = %lld, iter = %lld, err = %.3e\n", (long long)itPT, (long long)iter, err);
-                double lmin[2];
-                for (int k = 0; k < 2; ++k) {
-                    const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
-                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
-                }
-                if (!freeze) {
-                    ++jac;
-                    CHK(launch_jacobian(s, o));
-                    rc = checked_lmax(s, lmax);
-                    if (rc != FEM_OK) break;
-                    lmax_g = std::max(lmax[0], lmax[1]);
-                }
-                for (int k = 0; k < 2; ++k)
-                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
-            }
-            if (p->verbose && itPT == p->iter_max) printf("  inner: max iters (%lld) reached at PH=%lld\n", (long long)p->iter_max, (long long)itPH);
-            if (iter > p->max_iterations) { stop_total = true; break; }
-        }
-        if (rc != FEM_OK) break;
-        hipLaunchKernelGGL(k_pressure_update<double>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
-                           s->f[FEM_FIELD_P], (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_RP],
-                           (const double*)s->f[FEM_FIELD_M_P]);
-        CHK(check_launch(c, "k_pressure_update"));
-        if (stop_total || iter > p->max_iterations) break;
-    }
+    DRControl ctl;
+    ctl.tol = p->tolerance;
+    ctl.max_iterations = p->max_iterations;
+    ctl.iter_max = p->iter_max;
+    ctl.max_ph = p->max_ph_iterations;
+    ctl.ncheck = p->check_interval;
+    ctl.rel_drop0 = p->rel_drop0;
+    ctl.freeze = freeze;
+    ctl.verbose = p->verbose;
+    DRResult r;
+    const int rc = ph_dyrel<double>(s, set64(s), o, ctl, lmax, timer, r);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     timer.collect();
     st->kernel_ms_element = timer.ms_elem;
     st->kernel_ms_gather = timer.ms_gather;
     st->timed_launches = timer.n;
-    st->itPH = itPH_done;
-    st->iter = iter;
-    st->err = err;
-    st->err_abs = err_abs;
-    st->err_rel = err_rel;
-    st->err_v = err_v;
-    st->err_P = err_P;
-    st->converged = converged;
-    st->reached_total_iter = iter > p->max_iterations;
+    st->itPH = r.itPH;
+    st->iter = r.iter;
+    st->err = r.err;
+    st->err_abs = r.err_abs;
+    st->err_rel = r.err_rel;
+    st->err_v = r.err_v;
+    st->err_P = r.err_P;
+    st->converged = r.converged;
+    st->reached_total_iter = r.iter > p->max_iterations;
     st->lambda_max = std::max(lmax[0], lmax[1]);
-    st->lambda_max_gershgorin = lmax_g;
+    st->lambda_max_gershgorin = r.lmax_g;
     st->lambda_max_iterations = 0;
-    st->jacobian_assemblies = jac;
-    st->element_launches = launches;
+    st->jacobian_assemblies = 1 + r.jac;
+    st->element_launches = r.launches;
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc != FEM_OK) return rc;
-    if (!converged && p->throw_on_failure)
+    if (!r.converged && p->throw_on_failure)
         return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
-                       (long long)iter, err_abs, p->tolerance);
+                       (long long)r.iter, r.err_abs, p->tolerance);
     return FEM_OK;
 }
 

@@ -181,6 +181,8 @@ enum ElemFlags {
     EF_ITER = 8,      // inner pseudo-iteration: Pnum in the momentum, bubble update
     EF_GEN = 16,      // element-node phases, temperature, sources, fluid pressure
     EF_STRUCT = 32,   // structured box mesh: connectivity and geometry from the element index
+    EF_LIN = 64,      // linearised visco-elastic operator of a defect correction: no τ_old,
+                      // storage on δP alone, density linear in δP, plus right-hand sides
 };
 
 // Structured T7 box mesh (connectivity.jl:111-140): element e = 2c + k of cell
@@ -232,6 +234,8 @@ struct ElemArgs {
     // EF_STRUCT: nx cells per row, sx = 2 nx + 1 grid nodes per row, nT6 grid nodes
     int32_t nx, sx; int64_t nT6; F idx, idy, detc;
     int32_t phase_same;        // phase_P holds the same cell phases as phase_v
+    // EF_LIN right-hand sides R(x) of the correction problem (element DoFs / nodes)
+    const F* rhs_P; const F* rhs_vx; const F* rhs_vy;
     // EF_GEN extras
     int32_t nodeph;            // phases stored per element node
     int32_t bulk_all_inf;
@@ -267,7 +271,8 @@ __device__ inline int64_t xcd_tile() {
 template <class F, int FL, int BLOCK>
 __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
     constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
-    constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT;
+    constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT, LIN = FL & EF_LIN;
+    static_assert(!(LIN && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
     constexpr bool TAU_LDS = ELASTIC && FEM_TAU_LDS;
     const int64_t ne = A.nels;
     const int64_t e0 = xcd_tile() * BLOCK;          // first element of this block
@@ -335,13 +340,13 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         F dP[NP] = {0, 0, 0}, dT[NP] = {0, 0, 0}, Qv[NP] = {0, 0, 0};
         if (BULK) {
 #pragma unroll
-            for (int i = 0; i < NP; ++i) dP[i] = P[i] - A.Pold[e * NP + i];
+            for (int i = 0; i < NP; ++i) dP[i] = LIN ? P[i] : P[i] - A.Pold[e * NP + i];
         }
-        if (GEN && A.thermal) {
+        if (GEN && !LIN && A.thermal) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) dT[i] = A.T[e * NP + i] - A.T0[e * NP + i];
         }
-        if (GEN && A.Q) {
+        if (GEN && !LIN && A.Q) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) Qv[i] = A.Q[e * NP + i];
         }
@@ -381,6 +386,10 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                 for (int i = 0; i < NP; ++i) RP[i] += (F)kT.NP1[q][i] * src;
             }
         }
+    }
+    if (LIN) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) RP[i] += A.rhs_P[e * NP + i];
     }
     F Pn[NP] = {0, 0, 0};
     if (ITER) {
@@ -463,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             if (ITER) Pnq += (F)kT.NP1[q][i] * Pn[i];
         }
         F Pfq = 0;
-        if (GEN && A.Pf) {
+        if (GEN && !LIN && A.Pf) {
 #pragma unroll
             for (int i = 0; i < NP; ++i) Pfq += (F)kT.NP1[q][i] * A.Pf[e * NP + i];
         }
@@ -505,11 +514,12 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                 rho_c = r0; alpha_c = al; beta_c = be;
             }
             F Tq = 0;
-            if (GEN && A.thermal) {
+            if (GEN && !LIN && A.thermal) {
 #pragma unroll
                 for (int i = 0; i < NP; ++i) Tq += (F)kT.NP1[q][i] * A.T[e * NP + i];
             }
-            const F rho = rho_c * ((F(1) - alpha_c * (Tq - A.Tref)) + beta_c * Pq);
+            // the linearised density is ρ0 β δP (eos_density, momentum_residuals.jl:11-16)
+            const F rho = LIN ? rho_c * (beta_c * Pq) : rho_c * ((F(1) - alpha_c * (Tq - A.Tref)) + beta_c * Pq);
             bx = rho * A.gx * dOm;
             by = rho * A.gy * dOm;
         }
@@ -537,6 +547,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     {
         const int32_t b = n[NSH];
         const uint8_t fl = A.bcflag[b];
+        if (LIN) { Rx[NSH] += A.rhs_vx[b]; Ry[NSH] += A.rhs_vy[b]; }
         const F rx = (fl & 1) ? F(0) : Rx[NSH];
         const F ry = (fl & 2) ? F(0) : Ry[NSH];
         if (A.Rv_out_x) { A.Rv_out_x[b] = rx; A.Rv_out_y[b] = ry; }
@@ -571,6 +582,7 @@ struct GatherArgs {
     F* Rv_out_x; F* Rv_out_y; F* Rv0_out_x; F* Rv0_out_y;
     F alpha_x, alpha_y, beta_x, beta_y;
     int32_t do_update;
+    const F* rhs_vx; const F* rhs_vy;   // nullable: nodal right-hand side added to the gathered sum
 };
 
 template <class F, bool STRUCT, int BLOCK>
@@ -586,6 +598,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_update(GatherArgs<F> A) {
         rx += r.x;
         ry += r.y;
     }
+    if (A.rhs_vx) { rx += A.rhs_vx[node]; ry += A.rhs_vy[node]; }
     const uint8_t fl = A.bcflag[node];
     if (fl & 1) rx = 0;
     if (fl & 2) ry = 0;
@@ -984,6 +997,18 @@ template <class F>
 __global__ void k_any_nonzero(int64_t n, const F* x, int32_t* flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && x[i] != F(0)) atomicOr(flag, 1);
+}
+
+// y = (To) (x * scale)
+template <class To, class Ti>
+__global__ void k_cast(int64_t n, const Ti* x, To* y, double scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (To)(x[i] * scale);
+}
+// x64 += scale * δ32 (defect-correction update of the FP64 solution)
+__global__ void k_add_correction(int64_t n, double* x, const float* d, double scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += scale * (double)d[i];
 }
 
 template <class T>

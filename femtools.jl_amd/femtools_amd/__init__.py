@@ -501,9 +501,17 @@ def solve(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_v
           dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, check_interval=50, iterMax=50_000,
           max_ph_iterations=1000, rel_drop0=1e-2, plastic=None, phases_v=None, phases_P=None, finite_K=False,
           freeze_jacobian=None, throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False,
-          time_kernels=False):
+          time_kernels=False, precision="fp64"):
     """solve!(dr, mesh, bc_vx, bc_vy; dt, ...) (src/stokes/solvers/DR.jl:302-337).
-    Mutates dr in place and returns the stats dict of DR.jl:783-801."""
+    Mutates dr in place and returns the stats dict of DR.jl:783-801.
+
+    precision="fp64" runs the reference's iteration in FP64. precision="fp32"
+    keeps the state, the outer residuals, the norms and the convergence test
+    in FP64 (same criterion and tolerance) and runs the inner pseudo-transient
+    iterations in FP32 on a correction of the linearised visco-elastic
+    problem, folded into the FP64 state at every outer iteration."""
+    if precision not in ("fp64", "fp32"):
+        raise ValueError(f"precision must be 'fp64' or 'fp32', got {precision!r}")
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
     if phases_v is not None:
@@ -518,8 +526,8 @@ def solve(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_v
                      max_ph_iterations=int(max_ph_iterations), rel_drop0=float(rel_drop0), finite_K=int(finite_K),
                      freeze_jacobian=-1 if freeze_jacobian is None else int(bool(freeze_jacobian)),
                      throw_on_failure=int(bool(throw_on_failure)), use_tau_old=int(bool(tau_old)),
-                     skip_scaling=int(bool(skip_scaling)), verbose=int(verbose), precision=0,
-                     time_kernels=int(bool(time_kernels)))
+                     skip_scaling=int(bool(skip_scaling)), verbose=int(verbose),
+                     precision=0 if precision == "fp64" else 1, time_kernels=int(bool(time_kernels)))
     st = _SolveStats()
     rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
     stats = {k: getattr(st, k) for k, _ in _SolveStats._fields_}

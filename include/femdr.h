@@ -186,7 +186,11 @@ typedef struct {
     int32_t use_tau_old;        /* 1: read dr.τ_old (default), 0: τ_old = nothing */
     int32_t skip_scaling;       /* 1: γP, M_P already assembled (solve_stokes_dyrel!) */
     int32_t verbose;
-    int32_t precision;          /* 0 = FP64 throughout                  */
+    int32_t precision;          /* 0 = FP64 throughout; 1 = FP64 state, outer
+                                   residuals, norms and convergence test, inner
+                                   DR iterations in FP32 on a correction δ of the
+                                   linearised visco-elastic problem R(x) + L δ = 0,
+                                   folded into x at every outer iteration */
     int32_t time_kernels;       /* 1: HIP events around every element / gather launch */
 } fem_solve_params;
 

@@ -37,11 +37,31 @@ def test_python_mirror_binds_every_declared_symbol():
     assert sorted(fa.EXPORTED) == declared()
 
 
+STRUCTS = {"fem_mesh_info": "_MeshInfo", "fem_material": "_Material", "fem_drucker_prager": "_DP",
+           "fem_state_opts": "_StateOpts", "fem_bc": "_BC", "fem_solve_params": "_SolveParams",
+           "fem_solve_stats": "_SolveStats"}
+
+
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libfemdr.so not built")
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and every field offset of the header's structs, as gcc lays
+    them out, against the ctypes mirror."""
+    import subprocess
     import femtools_amd as fa
-    # sizes derived from the header's field lists (LP64)
-    assert ctypes.sizeof(fa._Material) == 8 + 8 * 64 * 6 + 8 * 3 + 8
-    assert ctypes.sizeof(fa._DP) == 8 + 8 * 64 * 6
-    assert ctypes.sizeof(fa._SolveParams) == 8 * 3 + 8 * 4 + 8 + 4 * 8
-    assert ctypes.sizeof(fa._BC) == 24
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in getattr(fa, pyname)._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(fa, pyname)
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
