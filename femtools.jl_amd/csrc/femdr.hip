@@ -121,6 +121,8 @@ struct fem_stokes {
         float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
         float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
         float2* Re;
+        // state copies for the correction operator with the return map
+        float *xvx, *xvy, *xP, *tau_old0, *tau_old1, *tau_old2;
     };
     W32* w32 = nullptr;
 };
@@ -727,7 +729,9 @@ struct DRSet {
     F* geo = nullptr;
     typename Vec2<F>::T* Re = nullptr;
     const F *rhs_P = nullptr, *rhs_vx = nullptr, *rhs_vy = nullptr;
-    bool lin = false;   // linearised visco-elastic correction operator (EF_LIN)
+    const F *xvx = nullptr, *xvy = nullptr, *xP = nullptr;
+    bool lin = false;     // correction operator (EF_LIN)
+    bool delta = false;   // ... with the return map about the state x (EF_DELTA)
 };
 
 static DRSet<double> set64(fem_stokes* s) {
@@ -760,6 +764,8 @@ static DRSet<float> set32(fem_stokes* s) {
     d.ptab = s->d_ptab; d.geo = w.geo;
     d.Re = w.Re;
     d.rhs_P = w.rhs_P; d.rhs_vx = w.rhs_vx; d.rhs_vy = w.rhs_vy;
+    d.xvx = w.xvx; d.xvy = w.xvy; d.xP = w.xP;
+    d.tau_old[0] = w.tau_old0; d.tau_old[1] = w.tau_old1; d.tau_old[2] = w.tau_old2;
     d.lin = true;
     return d;
 }
@@ -776,13 +782,18 @@ static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
     else if constexpr (FL > 0) dispatch_elem<FL - 1>(key, s, a);
 }
 
-// the correction operator: EF_LIN with any of EF_BULK, EF_ITER, EF_GEN, EF_STRUCT
+// inner iterations on a correction: EF_LIN|EF_ITER with any of EF_BULK,
+// EF_GEN, EF_STRUCT, and the return-map operator EF_DELTA|EF_PLASTIC with or
+// without EF_ELASTIC
 template <int I>
 static void dispatch_lin(int key, fem_stokes* s, const ElemArgs<float>& a) {
-    constexpr int FL = EF_LIN | ((I & 1) ? EF_BULK : 0) | ((I & 2) ? EF_ITER : 0) | ((I & 4) ? EF_GEN : 0) |
-                       ((I & 8) ? EF_STRUCT : 0);
-    if (key == FL) launch_elem_t<float, FL>(s, a);
-    else if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
+    constexpr bool el = I & 8, delta = I & 16;
+    constexpr int FL = EF_LIN | EF_ITER | ((I & 1) ? EF_BULK : 0) | ((I & 2) ? EF_GEN : 0) |
+                       ((I & 4) ? EF_STRUCT : 0) | (el ? EF_ELASTIC : 0) | (delta ? (EF_DELTA | EF_PLASTIC) : 0);
+    if constexpr (!el || delta) {
+        if (key == FL) { launch_elem_t<float, FL>(s, a); return; }
+    }
+    if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
 }
 
 template <class F>
@@ -832,6 +843,9 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     a.rhs_P = d.rhs_P;
     a.rhs_vx = d.rhs_vx;
     a.rhs_vy = d.rhs_vy;
+    a.xvx = d.xvx;
+    a.xvy = d.xvy;
+    a.xP = d.xP;
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
@@ -860,8 +874,9 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
         if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
         dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
     } else {
-        if (!d.lin) return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for the correction operator only");
-        dispatch_lin<15>(key | EF_LIN, s, a);
+        if (!d.lin || !o.use_pnum)
+            return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for inner iterations on a correction only");
+        dispatch_lin<31>(key | EF_LIN | (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0), s, a);
     }
     return check_launch(s->ctx, "k_t7_element");
 }
@@ -1229,11 +1244,23 @@ __global__ void k_pin_bc(int64_t n, const uint8_t* flag, const double* bx, const
     }
 }
 
-static int ensure_w32(fem_stokes* s) {
-    if (s->w32) return FEM_OK;
+static int ensure_w32(fem_stokes* s, bool delta) {
     fem_mesh* m = s->mesh;
     fem_ctx* c = s->ctx;
     const int64_t nv = m->nnodes, np = m->nnodesP, ne = m->nels;
+    if (s->w32) {
+        if (!delta || s->w32->xvx) return FEM_OK;
+        fem_stokes::W32& w = *s->w32;
+        CHK(dalloc_t(c, &w.xvx, nv));
+        CHK(dalloc_t(c, &w.xvy, nv));
+        CHK(dalloc_t(c, &w.xP, np));
+        // +16: 16-byte global->LDS loads of the last block read past the end
+        for (float** q : {&w.tau_old0, &w.tau_old1, &w.tau_old2}) {
+            CHK(dalloc_t(c, q, NQ * ne + 16));
+            HIPCHK(c, hipMemsetAsync(*q, 0, (NQ * ne + 16) * 4, c->stream));
+        }
+        return FEM_OK;
+    }
     s->w32 = new fem_stokes::W32();
     std::memset(s->w32, 0, sizeof(fem_stokes::W32));
     fem_stokes::W32& w = *s->w32;
@@ -1247,7 +1274,8 @@ static int ensure_w32(fem_stokes* s) {
     HIPCHK(c, hipMemsetAsync(w.zero_v, 0, nv * 4, c->stream));
     hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(5 * ne, 256)), dim3(256), 0, c->stream, 5 * ne,
                        (const double*)m->d_geo, w.geo, 1.0);
-    return check_launch(c, "k_cast");
+    CHK(check_launch(c, "k_cast"));
+    return delta ? ensure_w32(s, true) : FEM_OK;
 }
 
 static int cast_to32(fem_ctx* c, const double* x, float* y, int64_t n, double scale = 1.0) {
@@ -1273,36 +1301,68 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // reference. At every convergence check δv is folded into the FP64 state and
 // the correction restarts from the FP64 residual there (one FP64 residual
 // evaluation per check interval); the pressure update acts on δP, folded in
-// before the next outer residual. L is exact for viscous and
-// visco-elastic rheology; under Drucker-Prager yielding it is the elastic
-// predictor, whose error the next outer residual measures and corrects. The
-// preconditioner and λmax come from the Jacobian diagnostics of L, assembled
-// once. The solution x meets the Dirichlet values before the first iteration.
-static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpts& o, KernelTimer& timer,
+// before the next outer residual.
+//
+// Without plasticity L is the visco-elastic operator, exact, and its Jacobian
+// diagnostics (preconditioner, λmax) are assembled once. With a Drucker-
+// Prager rheology the correction operator keeps the return map: its stress
+// term is τ(x + δ) - τ(x) with x an FP32 copy of the state (refreshed at every
+// fold) and τ_old an FP32 copy of the stress history, and the Jacobian
+// diagnostics are re-assembled in FP64 at every check unless frozen, as the
+// reference does (DR.jl:726-745). The solution x meets the Dirichlet values
+// before the first iteration.
+#ifndef FEM_MIXED_RETURN_MAP
+#define FEM_MIXED_RETURN_MAP 1
+#endif
+static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpts& o, bool freeze, KernelTimer& timer,
                        fem_solve_stats* st) {
     fem_ctx* c = s->ctx;
     fem_mesh* m = s->mesh;
-    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
-    CHK(ensure_w32(s));
+    const int64_t nv = m->nnodes, np = m->nnodesP, nip = NQ * m->nels, ncheck = p->check_interval;
+    const bool delta = o.plastic && FEM_MIXED_RETURN_MAP;
+    CHK(ensure_w32(s, delta));
     fem_stokes::W32& w = *s->w32;
     const DRSet<double> d64 = set64(s);
-    const DRSet<float> d32 = set32(s);
+    DRSet<float> d32 = set32(s);
+    d32.delta = delta;
     CHK(cast_to32(c, d64.gammaP, w.gammaP, np));
     CHK(cast_to32(c, d64.MP, w.MP, np));
     CHK(cast_to32(c, s->d_pscale, w.pscale, np));
     hipLaunchKernelGGL(k_pin_bc, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
                        (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, d64.vx, d64.vy);
     CHK(check_launch(c, "k_pin_bc"));
+    // FP32 copies of the state the return-map operator is taken about
+    auto refresh_state = [&](bool pressure) -> int {
+        if (!delta) return FEM_OK;
+        CHK(cast_to32(c, d64.vx, w.xvx, nv));
+        CHK(cast_to32(c, d64.vy, w.xvy, nv));
+        if (pressure) {
+            // the yield pressure is P - Pf (momentum_residuals.jl:165-173)
+            hipLaunchKernelGGL(k_cast_sub, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, (const double*)d64.P,
+                               s->pf_nonzero ? (const double*)d64.Pf : nullptr, w.xP);
+            CHK(check_launch(c, "k_cast_sub"));
+        }
+        return FEM_OK;
+    };
+    if (delta && o.elastic) {
+        CHK(cast_to32(c, d64.tau_old[0], w.tau_old0, nip));
+        CHK(cast_to32(c, d64.tau_old[1], w.tau_old1, nip));
+        CHK(cast_to32(c, d64.tau_old[2], w.tau_old2, nip));
+    }
+    CHK(refresh_state(true));
+    // Jacobian diagnostics of the correction operator
     LaunchOpts oj = o;
-    oj.plastic = false;
+    oj.plastic = delta;
     CHK(launch_jacobian(s, oj));
     double lmax[2];
     CHK(checked_lmax(s, lmax));
+    double lmax_g = std::max(lmax[0], lmax[1]);
+    int64_t jac = 1;
     CHK(cast_to32(c, d64.PC_x, w.PC_x, nv));
     CHK(cast_to32(c, d64.PC_y, w.PC_y, nv));
     for (float* z : {w.vx, w.vy, w.rate_x, w.rate_y, w.Rv0_x, w.Rv0_y}) HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
-    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g) (visco-elastic operator)\n", lmax[0], lmax[1]);
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
 
     double al[2], be[2];
     for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
@@ -1314,8 +1374,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     bool converged = false;
     int rc = FEM_OK;
     LaunchOpts o32 = o;
-    o32.plastic = false;
-    o32.elastic = false;
+    o32.plastic = delta;
+    o32.elastic = delta && o.elastic;
     for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
         itPH_done = itPH;
         LaunchOpts oo = o;
@@ -1391,13 +1451,11 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                     const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
                     lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
                 }
-                for (int k = 0; k < 2; ++k)
-                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
                 // fold δv into the FP64 state and restart the correction from the
                 // FP64 residual there: FP32 rounding of L δ is relative to the
                 // size of δ, so the FP32 iterations only ever reduce the residual
                 // of the last ncheck steps, far above their rounding floor, and
-                // the linearisation is renewed at the current state
+                // the operator is renewed at the current state
                 CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
                 CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
                 HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
@@ -1407,6 +1465,18 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
                 CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
                 CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+                CHK(refresh_state(false));
+                if (delta && !freeze) {
+                    ++jac;
+                    CHK(launch_jacobian(s, o));
+                    rc = checked_lmax(s, lmax);
+                    if (rc != FEM_OK) break;
+                    lmax_g = std::max(lmax[0], lmax[1]);
+                    CHK(cast_to32(c, d64.PC_x, w.PC_x, nv));
+                    CHK(cast_to32(c, d64.PC_y, w.PC_y, nv));
+                }
+                for (int k = 0; k < 2; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
             }
             if (iter > p->max_iterations) { stop_total = true; break; }
         }
@@ -1421,6 +1491,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+        CHK(refresh_state(true));
         if (stop_total || iter > p->max_iterations) break;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1438,9 +1509,9 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     st->converged = converged;
     st->reached_total_iter = iter > p->max_iterations;
     st->lambda_max = std::max(lmax[0], lmax[1]);
-    st->lambda_max_gershgorin = st->lambda_max;
+    st->lambda_max_gershgorin = lmax_g;
     st->lambda_max_iterations = 0;
-    st->jacobian_assemblies = 1;
+    st->jacobian_assemblies = jac;
     st->element_launches = launches;
     return rc;
 }
@@ -1546,7 +1617,7 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
                            (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
     KernelTimer timer(c, p->time_kernels != 0);
     if (p->precision == 1) {
-        const int rc = solve_mixed(s, p, o, timer, st);
+        const int rc = solve_mixed(s, p, o, freeze, timer, st);
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rc != FEM_OK) return rc;
         if (!st->converged && p->throw_on_failure)

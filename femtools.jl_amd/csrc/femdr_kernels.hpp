@@ -170,6 +170,25 @@ __device__ inline void stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F
     }
 }
 
+// Plastic correction c = τ_trial - τ_returned of the plane-strain
+// Drucker-Prager return (rheology.jl:272-334) for a trial stress, zero where
+// the point does not yield; returns whether it yields.
+template <class F>
+__device__ inline bool dp_correction(F txx, F tyy, F txy, const IpMat<F>& m, F Pq, F& cxx, F& cyy, F& cxy) {
+    cxx = cyy = cxy = F(0);
+    const F tzz = -txx - tyy;
+    const F tII = dsqrt((txx * txx + tyy * tyy + tzz * tzz) * F(0.5) + txy * txy + eps2<F>());
+    const F F_ = (tII - m.yield0) - m.sphi * Pq;
+    if (!(F_ > F(0))) return false;
+    const F lam = F_ * m.invden;
+    if (!(lam > F(0))) return false;
+    const F h = F(2) * m.etave * lam * drcp((tII + eps2<F>()) * F(2));
+    cxx = (F(2) * txx + tyy) * h;
+    cyy = (txx + F(2) * tyy) * h;
+    cxy = (F(2) * txy) * h;
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
@@ -181,8 +200,11 @@ enum ElemFlags {
     EF_ITER = 8,      // inner pseudo-iteration: Pnum in the momentum, bubble update
     EF_GEN = 16,      // element-node phases, temperature, sources, fluid pressure
     EF_STRUCT = 32,   // structured box mesh: connectivity and geometry from the element index
-    EF_LIN = 64,      // linearised visco-elastic operator of a defect correction: no τ_old,
-                      // storage on δP alone, density linear in δP, plus right-hand sides
+    EF_LIN = 64,      // operator of a correction δ: R(x) + L δ with the right-hand side R(x)
+                      // given; storage on δP alone, density linear in δP. Alone: L is the
+                      // visco-elastic operator (no τ_old, no return map)
+    EF_DELTA = 128,   // with EF_LIN|EF_PLASTIC: the stress term is τ(x + δ) - τ(x), both with
+                      // the Drucker-Prager return, x given as an FP32 copy (xvx, xvy, xP)
 };
 
 // Structured T7 box mesh (connectivity.jl:111-140): element e = 2c + k of cell
@@ -236,6 +258,8 @@ struct ElemArgs {
     int32_t phase_same;        // phase_P holds the same cell phases as phase_v
     // EF_LIN right-hand sides R(x) of the correction problem (element DoFs / nodes)
     const F* rhs_P; const F* rhs_vx; const F* rhs_vy;
+    // EF_DELTA: the state x the correction is taken about
+    const F* xvx; const F* xvy; const F* xP;
     // EF_GEN extras
     int32_t nodeph;            // phases stored per element node
     int32_t bulk_all_inf;
@@ -272,7 +296,9 @@ template <class F, int FL, int BLOCK>
 __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
     constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
     constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT, LIN = FL & EF_LIN;
-    static_assert(!(LIN && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
+    constexpr bool DELTA = FL & EF_DELTA;
+    static_assert(!(LIN && !DELTA && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
+    static_assert(!DELTA || (LIN && PLASTIC), "EF_DELTA is the correction operator with the return map");
     constexpr bool TAU_LDS = ELASTIC && FEM_TAU_LDS;
     const int64_t ne = A.nels;
     const int64_t e0 = xcd_tile() * BLOCK;          // first element of this block
@@ -320,6 +346,13 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     F P[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) P[i] = A.P[e * NP + i];
+    F xvx[DELTA ? NV : 1], xvy[DELTA ? NV : 1], xP[DELTA ? NP : 1];
+    if (DELTA) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a) { xvx[a] = A.xvx[n[a]]; xvy[a] = A.xvy[n[a]]; }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) xP[i] = A.xP[e * NP + i];
+    }
     const int pc = A.phase_v[e];
 
     // ---- continuity residual (pressure_residual.jl:132-158). The element is
@@ -488,7 +521,38 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         const F gyx = bx_ * i00 + by_ * i10; // ∂vy/∂x
         const F gyy = bx_ * i01 + by_ * i11; // ∂vy/∂y
         F txx, tyy, txy;
-        stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy, txy);
+        if (DELTA) {
+            // τ(x + δ) - τ(x): the visco-elastic part is linear, 2 ηve ε̇'(δ), and
+            // exact; the returns of both states add their plastic corrections
+            F hxx = 0, hxy = 0, hyx = 0, hyy = 0, Pqx = 0;
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
+                const F cx = d0 * xvx[a], cy = d1 * xvx[a], dx = d0 * xvy[a], dy = d1 * xvy[a];
+                hxx += cx * i00 + cy * i10; hxy += cx * i01 + cy * i11;
+                hyx += dx * i00 + dy * i10; hyy += dx * i01 + dy * i11;
+            }
+#pragma unroll
+            for (int i = 0; i < NP; ++i) Pqx += (F)kT.NP1[q][i] * xP[i];
+            const F two_eve = F(2) * m.etave;
+            const F trd = (gxx + gyy) * F(1.0 / 3.0), trx = (hxx + hyy) * F(1.0 / 3.0);
+            txx = two_eve * (gxx - trd);
+            tyy = two_eve * (gyy - trd);
+            txy = two_eve * ((gxy + gyx) * F(0.5));
+            const F bxx = two_eve * ((hxx - trx) + txo * m.inv2Gdt);
+            const F byy = two_eve * ((hyy - trx) + tyo * m.inv2Gdt);
+            const F bxy = two_eve * ((hxy + hyx) * F(0.5) + txyo * m.inv2Gdt);
+            F cxx, cyy, cxy, dxx, dyy, dxy;
+            const bool yA = dp_correction<F>(bxx + txx, byy + tyy, bxy + txy, m, Pqx + Pq, cxx, cyy, cxy);
+            const bool yB = dp_correction<F>(bxx, byy, bxy, m, Pqx, dxx, dyy, dxy);
+            if (yA || yB) {
+                txx += dxx - cxx;
+                tyy += dyy - cyy;
+                txy += dxy - cxy;
+            }
+        } else {
+            stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy, txy);
+        }
         if (A.tau_out0) {
             A.tau_out0[e * NQ + q] = txx;
             A.tau_out1[e * NQ + q] = tyy;
@@ -1004,6 +1068,11 @@ template <class To, class Ti>
 __global__ void k_cast(int64_t n, const Ti* x, To* y, double scale) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = (To)(x[i] * scale);
+}
+// y = (float)(a - b), b nullable
+__global__ void k_cast_sub(int64_t n, const double* a, const double* b, float* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)(b ? a[i] - b[i] : a[i]);
 }
 // x64 += scale * δ32 (defect-correction update of the FP64 solution)
 __global__ void k_add_correction(int64_t n, double* x, const float* d, double scale) {
