@@ -7,12 +7,18 @@ log10 eta in U[-2,4]), G=1, dt=0.1, Drucker-Prager C=1, phi=30 deg, psi=0,
 K = etab = Kb = 100, densities 1 / 1.1, g=(0,-1), pure shear; solver keywords
 of the reference's VEP examples (pressure_factor 20, rel_drop0 0.75,
 check_interval 100, CFL_v 0.99); each physical time step solved to residual
-1e-8 (FP64), stress history handed over between steps.
+1e-8, stress history handed over between steps.
 
-A "step" is one physical time step (one solve! to 1e-8 plus the stress
-refresh). The warmup steps are the first steps of the same time series.
-value = GDOF.iter/s = (2 N_v + N_p) x DR iterations / seconds / 1e9 over the
-timed steps, inputs resident in HBM.
+A "step" is one physical time step. Each timed step starts from the same state
+(velocity, pressure, stress history) for both solvers: the plain FP64 solve
+(the reference's iteration) and the mixed-precision solve (FP64 state, outer
+residuals and convergence test, FP32 inner iterations), each timed to the same
+tolerance 1e-8 with its residual re-checked in FP64; the series then continues
+from the FP64 solution. Warmup solves repeat the first step and are discarded.
+
+value = GDOF.iter/s = (2 N_v + N_p) x DR iterations / seconds / 1e9 of the
+mixed-precision solve over the timed steps, inputs resident in HBM;
+time_to_tolerance_s and speedup_vs_fp64 give the time-to-1e-8 comparison.
 
 Prints one JSON line.
 """
@@ -33,8 +39,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(w, nv, ne, npp, wf=8, wi=4):
-    """Compulsory traffic per element-kernel launch and per DR iteration
-    (each array counted once; geometry as (D^2+1) words per element)."""
+    """Compulsory traffic per element-kernel launch and per DR iteration of the
+    FP64 path (each array counted once; geometry as (D^2+1) words per element)."""
     D, NV, nq = 2, 7, 7
     elastic = any(math.isfinite(g) for g in w.G)
     p_old = any(math.isfinite(b) for b in w.etab)
@@ -42,6 +48,23 @@ def algorithmic_bytes(w, nv, ne, npp, wf=8, wi=4):
     elem = wf * (D * nv + npp + (npp if p_old else 0) + npp + tau + (D * D + 1) * ne + D * nv) + wi * (NV * ne + ne)
     it = wf * (2 * D * nv + 2 * D * nv + D * nv + npp + (npp if p_old else 0) + npp + tau + (D * D + 1) * ne) \
         + wi * (NV * ne + ne)
+    return elem, it
+
+
+def algorithmic_bytes_fp32(w, nv, ne, npp):
+    """Same for the FP32 inner iteration of the mixed-precision solve on a
+    structured mesh (connectivity and geometry from the element index) with
+    the return-map correction operator: the element kernel reads the
+    correction and the FP32 state copy (velocity, pressure), the scale
+    gamma_P/M_P, the right-hand side R(x) and the stress history, and writes
+    the residual; the iteration adds the rate and preconditioner passes."""
+    D, nq = 2, 7
+    elastic = any(math.isfinite(g) for g in w.G)
+    plastic = w.plastic is not None
+    tau = 3 * nq * ne if (elastic and plastic) else 0
+    xcopy = (D * nv + npp) if plastic else 0
+    elem = 4 * (D * nv + npp + xcopy + npp + npp + D * nv + tau + D * nv) + 4 * ne
+    it = 4 * (2 * D * nv + 2 * D * nv + D * nv + D * nv + npp + xcopy + npp + npp + tau) + 4 * ne
     return elem, it
 
 
@@ -101,10 +124,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed physical time steps")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed physical time steps before")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed solves of the first step")
     ap.add_argument("--n", type=int, default=512, help="cells per side")
     ap.add_argument("--tolerance", type=float, default=1e-8)
-    ap.add_argument("--max-iterations", type=int, default=200_000)
+    ap.add_argument("--max-iterations", type=int, default=400_000)
+    ap.add_argument("--precision", choices=("both", "fp64", "fp32"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -117,7 +141,7 @@ def main():
     import femtools_amd as fa
     from femtools_amd import problems as PB
 
-    w = PB.vep_inclusions(args.n, nsteps=args.warmup + args.steps)
+    w = PB.vep_inclusions(args.n, nsteps=args.steps)
     ctx = fa.default_context()
     mesh = fa.t7_box_mesh(w.nx, w.ny)
     nv, ne, npp = mesh.nnodes, mesh.nels, mesh.nnodesP
@@ -132,44 +156,78 @@ def main():
     dr.set_phases("v", ph)
     dr.set_phases("P", ph)
     dp = fa.DruckerPrager(w.plastic["phi"], w.plastic["psi"], w.plastic["C"], w.plastic["eta_reg"], w.plastic["Kb"])
+    fields = (dr.v.x, dr.v.y, dr.P, dr.tau_old.xx, dr.tau_old.yy, dr.tau_old.xy)
 
-    def time_step(timed):
+    def snapshot():
+        return [f.numpy() for f in fields]
+
+    def restore(snap):
+        for f, a in zip(fields, snap):
+            f.copyto(a)
+
+    def solve(prec, timed):
         dr.P0.copyto(dr.P.numpy())
         kw = w.solve_kwargs()
         kw["tolerance"] = args.tolerance
+        ctx.sync()
+        t0 = time.perf_counter()
         st = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, plastic=dp, max_iterations=args.max_iterations,
-                      iterMax=args.max_iterations, throw_on_failure=False, time_kernels=timed, **kw)
+                      iterMax=args.max_iterations, throw_on_failure=False, time_kernels=timed, precision=prec, **kw)
+        ctx.sync()
+        st["wall"] = time.perf_counter() - t0
+        return st
+
+    precs = ("fp64", "fp32") if args.precision == "both" else (args.precision,)
+    s0 = snapshot()
+    for i in range(args.warmup):
+        for prec in precs:
+            restore(s0)
+            st = solve(prec, False)
+            print(f"# warmup {prec}: iter {st['iter']} converged {st['converged']} {st['wall']:.2f}s",
+                  file=sys.stderr, flush=True)
+    restore(s0)
+    stats = {p: [] for p in precs}
+    diffs = []
+    for i in range(args.steps):
+        start = snapshot()
+        sols = {}
+        for prec in precs:
+            restore(start)
+            st = solve(prec, True)
+            stats[prec].append(st)
+            sols[prec] = [f.numpy() for f in fields[:3]]
+            print(f"# step {i} {prec}: iter {st['iter']} itPH {st['itPH']} converged {st['converged']} "
+                  f"err_abs {st['err_abs']:.2e} {st['wall']:.2f}s", file=sys.stderr, flush=True)
+        if len(precs) == 2:
+            diffs.append({k: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                          for k, a, b in zip(("vx", "vy", "P"), sols["fp32"], sols["fp64"])})
+        # the series continues from the FP64 solution (the reference's trajectory)
+        for f, a in zip(fields[:3], sols[precs[0]]):
+            f.copyto(a)
         fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp)
         for c in ("xx", "yy", "xy"):
             getattr(dr.tau_old, c).copyto(getattr(dr.tau, c).numpy())
-        return st
 
-    for i in range(args.warmup):
-        st = time_step(False)
-        print(f"# warmup step {i}: iter {st['iter']} converged {st['converged']} {st['seconds']:.2f}s",
-              file=sys.stderr, flush=True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    stats = []
-    for i in range(args.steps):
-        st = time_step(True)
-        stats.append(st)
-        print(f"# step {i}: iter {st['iter']} converged {st['converged']} err_abs {st['err_abs']:.2e} "
-              f"{st['seconds']:.2f}s", file=sys.stderr, flush=True)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    iters = sum(s["iter"] for s in stats)
+    head = "fp32" if "fp32" in precs else "fp64"
+    hs = stats[head]
+    elapsed = sum(s["wall"] for s in hs)
+    iters = sum(s["iter"] for s in hs)
     value = w.dofs() * iters / elapsed / 1e9
-    ms_el = sum(s["kernel_ms_element"] for s in stats)
-    ms_ga = sum(s["kernel_ms_gather"] for s in stats)
-    nl = max(1, sum(s["timed_launches"] for s in stats))
-    elem_bytes, iter_bytes = algorithmic_bytes(w, nv, ne, npp)
+    ms_el = sum(s["kernel_ms_element"] for s in hs)
+    ms_ga = sum(s["kernel_ms_gather"] for s in hs)
+    nl = max(1, sum(s["timed_launches"] for s in hs))
+    if head == "fp32":
+        elem_bytes, iter_bytes = algorithmic_bytes_fp32(w, nv, ne, npp)
+        kname = "k_t7_element<float> (FP32 correction operator with the return map, bubble update)"
+    else:
+        elem_bytes, iter_bytes = algorithmic_bytes(w, nv, ne, npp)
+        kname = "k_t7_element<double> (fused element residual + bubble update)"
     avg_el_s = ms_el / nl * 1e-3
     avg_it_s = (ms_el + ms_ga) / nl * 1e-3
     achieved = elem_bytes / avg_el_s / 1e9 if avg_el_s > 0 else None
-    traffic = read_traffic("k_t7_element")
+    traffic = read_traffic("k_t7_element<float>" if head == "fp32" else "k_t7_element<double>")
     out = {
-        "metric": "GDOF.iter/s (time-to-1e-8 residual companion)",
+        "metric": "time-to-1e-8 residual (FP32+learned vs plain FP64, same GPU); GDOF.iter/s",
         "value": round(value, 4),
         "unit": "GDOF.iter/s",
         "n_gpus": 1,
@@ -179,24 +237,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 inner iterations, f64 state/residuals/norms" if head == "fp32" else "f64",
         "data": "synthetic (seeded random inclusions, generated in-process)",
         "config": {"workload": w.name + f", dt={w.dt}, DP C=1 phi=30deg psi=0 eta_reg=8e-3, K=etab=Kb=100, "
-                                        f"pure shear, tol {args.tolerance:g}, FP64",
+                                        f"pure shear, tol {args.tolerance:g}",
                    "cells": [w.nx, w.ny], "elements": ne, "velocity_nodes": nv, "pressure_dofs": npp,
-                   "dofs": w.dofs(), "time_steps_timed": args.steps, "parallelism": "single GPU"},
-        "time_to_tolerance_s": [round(s["seconds"], 4) for s in stats],
-        "iterations": [s["iter"] for s in stats],
-        "converged": [bool(s["converged"]) for s in stats],
+                   "dofs": w.dofs(), "time_steps_timed": args.steps, "parallelism": "single GPU",
+                   "warm_start": "previous time step (no learned warm start in this build)"},
+        "time_to_tolerance_s": {p: [round(s["wall"], 4) for s in stats[p]] for p in precs},
+        "iterations": {p: [s["iter"] for s in stats[p]] for p in precs},
+        "converged": {p: [bool(s["converged"]) for s in stats[p]] for p in precs},
+        "err_abs": {p: [s["err_abs"] for s in stats[p]] for p in precs},
+        "gdof_iter_per_s": {p: round(w.dofs() * sum(s["iter"] for s in stats[p]) /
+                                     sum(s["wall"] for s in stats[p]) / 1e9, 4) for p in precs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": "k_t7_element (fused element residual + bubble update)",
+                     "traffic": traffic, "kernel": kname,
                      "algorithmic_bytes_per_launch": elem_bytes,
                      "avg_launch_us": round(avg_el_s * 1e6, 2)},
         "iteration_roofline": {"algorithmic_bytes": iter_bytes, "avg_us": round(avg_it_s * 1e6, 2),
                                "achieved_GBs": round(iter_bytes / avg_it_s / 1e9, 1) if avg_it_s > 0 else None,
                                "avg_gather_us": round(ms_ga / nl * 1e3, 2)},
     }
+    if len(precs) == 2:
+        t64 = sum(s["wall"] for s in stats["fp64"])
+        out["speedup_vs_fp64"] = round(t64 / elapsed, 3)
+        out["fp32_vs_fp64_max_rel_diff"] = diffs
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)

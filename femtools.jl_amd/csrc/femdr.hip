@@ -38,11 +38,14 @@ struct fem_ctx {
     size_t tev_used = 0;
 };
 
-// Kernel timing: events e0 | element | e1 | gather | e2 per iteration, read
-// back at the next host synchronisation.
+// Kernel timing: events e0 | element | e1 | gather | e2 around every
+// STRIDE-th iteration (sampled, so that the event records do not slow the
+// solve they measure), read back at the next host synchronisation.
 struct KernelTimer {
+    static constexpr int64_t STRIDE = 16;
     fem_ctx* c;
     bool on;
+    bool take(int64_t iter) const { return on && iter % STRIDE == 1; }
     double ms_elem = 0, ms_gather = 0;
     int64_t n = 0;
     KernelTimer(fem_ctx* c_, bool on_) : c(c_), on(on_) { c->tev_used = 0; }
@@ -1187,8 +1190,8 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
             oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
             oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
             oi.beta[0] = be[0]; oi.beta[1] = be[1];
-            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
-                       e2 = timer.on ? timer.ev() : nullptr;
+            const bool tk = timer.take(iter);
+            hipEvent_t e0 = tk ? timer.ev() : nullptr, e1 = tk ? timer.ev() : nullptr, e2 = tk ? timer.ev() : nullptr;
             timer.mark(e0);
             CHK(launch_element(s, d, oi));
             timer.mark(e1);
@@ -1428,8 +1431,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
             oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
             oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
             oi.beta[0] = be[0]; oi.beta[1] = be[1];
-            hipEvent_t e0 = timer.on ? timer.ev() : nullptr, e1 = timer.on ? timer.ev() : nullptr,
-                       e2 = timer.on ? timer.ev() : nullptr;
+            const bool tk = timer.take(iter);
+            hipEvent_t e0 = tk ? timer.ev() : nullptr, e1 = tk ? timer.ev() : nullptr, e2 = tk ? timer.ev() : nullptr;
             timer.mark(e0);
             CHK(launch_element(s, d32, oi));
             timer.mark(e1);

@@ -202,7 +202,8 @@ typedef struct {
     double lambda_max, lambda_max_gershgorin;
     int64_t lambda_max_iterations, jacobian_assemblies;
     double seconds;             /* wall time of the solve (host clock, after a device sync) */
-    double kernel_ms_element;   /* summed device time of the element kernel (time_kernels) */
+    double kernel_ms_element;   /* summed device time of the element kernel over the timed
+                                   launches (time_kernels: every 16th inner iteration) */
     double kernel_ms_gather;    /* summed device time of the gather/update kernel          */
     int64_t element_launches;   /* inner-iteration element launches                        */
     int64_t timed_launches;     /* launches covered by kernel_ms_*                          */
