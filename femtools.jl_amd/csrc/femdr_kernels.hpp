@@ -1035,7 +1035,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_reduce_lmax(int64_t n, const F* J
 }
 
 // second stage: one block, K values; op 0 = sum, 1 = max
-__global__ __launch_bounds__(RED_BLOCK) void k_reduce_final(int nparts, int K, int op, const double* partial,
+static __global__ __launch_bounds__(RED_BLOCK) void k_reduce_final(int nparts, int K, int op, const double* partial,
                                                            double* out) {
     __shared__ double s[RED_BLOCK];
     for (int k = 0; k < K; ++k) {
@@ -1070,12 +1070,12 @@ __global__ void k_cast(int64_t n, const Ti* x, To* y, double scale) {
     if (i < n) y[i] = (To)(x[i] * scale);
 }
 // y = (float)(a - b), b nullable
-__global__ void k_cast_sub(int64_t n, const double* a, const double* b, float* y) {
+static __global__ void k_cast_sub(int64_t n, const double* a, const double* b, float* y) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = (float)(b ? a[i] - b[i] : a[i]);
 }
 // x64 += scale * δ32 (defect-correction update of the FP64 solution)
-__global__ void k_add_correction(int64_t n, double* x, const float* d, double scale) {
+static __global__ void k_add_correction(int64_t n, double* x, const float* d, double scale) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += scale * (double)d[i];
 }

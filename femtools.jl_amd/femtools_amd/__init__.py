@@ -102,6 +102,24 @@ class _SolveStats(C.Structure):
                 ("timed_launches", I64)]
 
 
+class _ThermalMaterial(C.Structure):
+    _fields_ = [("nphases", I32), ("k", D * MAXPH), ("Cp", D * MAXPH), ("rho0", D * MAXPH), ("alpha", D * MAXPH),
+                ("K", D * MAXPH)]
+
+
+class _ThermalOpts(C.Structure):
+    _fields_ = [("CFL", D), ("c_fact", D), ("eps", D)]
+
+
+class _ThermalParams(C.Structure):
+    _fields_ = [("dt", D), ("tolerance", D), ("max_iterations", I64), ("check_interval", I64), ("Tref", D),
+                ("throw_on_failure", I32), ("verbose", I32)]
+
+
+class _ThermalStats(C.Structure):
+    _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("seconds", D)]
+
+
 def _sig(name, restype, *argtypes):
     f = getattr(_lib, name)
     f.restype = restype
@@ -139,6 +157,11 @@ _sig("fem_stokes_update_current_stress", C.c_int, VP, _P(_DP), D)
 _sig("fem_stokes_residual", C.c_int, VP, _P(_DP), D, I32, I32)
 _sig("fem_stokes_jacobian_diagnostics", C.c_int, VP, _P(_DP), D, I32)
 _sig("fem_stokes_time_element_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
+_sig("fem_thermal_create", C.c_int, VP, VP, _P(_ThermalMaterial), _P(_ThermalOpts), _P(VP))
+_sig("fem_thermal_destroy", C.c_int, VP)
+_sig("fem_thermal_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
+_sig("fem_thermal_solve", C.c_int, VP, _P(_BC), _P(_ThermalParams), _P(_ThermalStats))
+_sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
 
 EXPORTED = [
     "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
@@ -147,7 +170,8 @@ EXPORTED = [
     "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
-    "fem_stokes_time_element_kernel",
+    "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
+    "fem_thermal_solve", "fem_thermal_residual",
 ]
 
 
@@ -497,7 +521,16 @@ def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, 
     dr.ctx.check(_lib.fem_stokes_pressure_scaling(dr.handle, float(pressure_factor), float(dt), int(finite_K)))
 
 
-def solve(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition, *,
+def solve(dr, mesh, *bcs, **kwargs):
+    """solve!(dr, mesh, bc...; kwargs...): dispatches on the state type like
+    the reference's solve! methods: StokesDR -> solve_stokes (DR.jl:302-337),
+    ThermalDiffusionDR -> solve_thermal (heat_diffusion/solvers/DR.jl:50-54)."""
+    if isinstance(dr, ThermalDiffusionDR):
+        return solve_thermal(dr, mesh, *bcs, **kwargs)
+    return solve_stokes(dr, mesh, *bcs, **kwargs)
+
+
+def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition, *,
           dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, check_interval=50, iterMax=50_000,
           max_ph_iterations=1000, rel_drop0=1e-2, plastic=None, phases_v=None, phases_P=None, finite_K=False,
           freeze_jacobian=None, throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False,
@@ -566,3 +599,96 @@ def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
     out = D()
     dr.ctx.check(_lib.fem_stokes_time_element_kernel(dr.handle, dp, float(dt), int(n), C.byref(out)))
     return out.value
+
+
+# ---------------------------------------------------------------------------
+# thermal diffusion (src/heat_diffusion/**)
+# ---------------------------------------------------------------------------
+class ThermalMaterial:
+    """ThermalMaterial(; k=(1.0,), Cp=(1.0,), ρ0=(1.0,), α=(1.0,), K=(1.0,))
+    (src/heat_diffusion/types/heat_diffusion_types.jl:15-41). The first tuple
+    given fixes the phase count; scalars apply to every phase; omitted
+    properties are unit-valued."""
+
+    def __init__(self, k=None, Cp=None, rho0=None, alpha=None, K=None):
+        given = [v for v in (k, Cp, rho0, alpha, K) if isinstance(v, (tuple, list, np.ndarray))]
+        n = len(given[0]) if given else 1
+        if n < 1 or n > MAXPH:
+            raise ValueError(f"the phase count must be in 1..{MAXPH}")
+
+        def prop(v):
+            t = _tup(1.0 if v is None else v, n)
+            if len(t) != n:
+                raise DimensionMismatch("material property tuples must have the same length")
+            return t
+
+        self.k, self.Cp, self.rho0, self.alpha, self.K = prop(k), prop(Cp), prop(rho0), prop(alpha), prop(K)
+
+    @property
+    def nphases(self):
+        return len(self.k)
+
+    def _c(self):
+        m = _ThermalMaterial()
+        m.nphases = self.nphases
+        for i in range(self.nphases):
+            m.k[i], m.Cp[i], m.rho0[i], m.alpha[i], m.K[i] = self.k[i], self.Cp[i], self.rho0[i], self.alpha[i], \
+                self.K[i]
+        return m
+
+
+class ThermalDiffusionDR:
+    """ThermalDiffusionDR(mesh, material; CFL=0.98, c_fact=0.9, ϵ=1e-6)
+    (heat_diffusion_types.jl:90-153): nodal device fields R, R0, dRdT (∂R∂T),
+    PC, T, T0, dTdtau (∂T∂τ), P, source and phases (Int32, 1-based, all 1)."""
+
+    _names = ("R", "R0", "dRdT", "PC", "T", "T0", "dTdtau", "P", "source", "phases")
+
+    def __init__(self, mesh: "MixedMesh", material: ThermalMaterial, CFL=0.98, c_fact=0.9, eps=1e-6):
+        self.ctx = mesh.ctx
+        self.mesh = mesh
+        self.material = material
+        self.CFL, self.c_fact, self.eps = float(CFL), float(c_fact), float(eps)
+        h = VP()
+        m = material._c()
+        o = _ThermalOpts(CFL=self.CFL, c_fact=self.c_fact, eps=self.eps)
+        self.ctx.check(_lib.fem_thermal_create(self.ctx.handle, mesh.handle, C.byref(m), C.byref(o), C.byref(h)))
+        self.handle = h
+        for k, name in enumerate(self._names):
+            p, n, dt = VP(), I64(), I32()
+            self.ctx.check(_lib.fem_thermal_field(h, k, C.byref(p), C.byref(n), C.byref(dt)))
+            setattr(self, name, DeviceArray(self.ctx, p.value, (n.value,), np.int32 if dt.value == 1 else np.float64,
+                                            owner=self))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.fem_thermal_destroy(h)
+            self.handle = None
+
+
+def solve_thermal(dr: ThermalDiffusionDR, mesh: "MixedMesh", bc: "DirichletBoundaryCondition" = None, *, dt,
+                  tolerance=None, max_iterations=10_000, check_interval=100, verbose=False, throw_on_failure=True,
+                  Tref=273.0):
+    """solve!(dr::ThermalDiffusionDR, mesh, bc; dt, tolerance=dr.ϵ,
+    max_iterations=10_000, check_interval=100, throw_on_failure=true, Tref=273)
+    (src/heat_diffusion/solvers/DR.jl:25-54). Returns (; converged,
+    iterations, residual) as a dict, plus the wall time."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    keep = []
+    bcp = C.pointer(bc._c(keep)) if bc is not None else C.cast(None, _P(_BC))
+    p = _ThermalParams(dt=float(dt), tolerance=-1.0 if tolerance is None else float(tolerance),
+                       max_iterations=int(max_iterations), check_interval=int(check_interval), Tref=float(Tref),
+                       throw_on_failure=int(bool(throw_on_failure)), verbose=int(bool(verbose)))
+    st = _ThermalStats()
+    rc = _lib.fem_thermal_solve(dr.handle, bcp, C.byref(p), C.byref(st))
+    dr.ctx.check(rc)
+    return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual,
+            "seconds": st.seconds}
+
+
+def thermal_residual(dr: ThermalDiffusionDR, dt, Tref=273.0, compute_jacobian=False):
+    """assemble_diffusion_matrices_atomix! (heat residual.jl:90-121): dr.R and,
+    with compute_jacobian, dr.dRdT and dr.PC."""
+    dr.ctx.check(_lib.fem_thermal_residual(dr.handle, float(dt), float(Tref), int(bool(compute_jacobian))))

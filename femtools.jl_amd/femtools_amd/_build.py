@@ -10,8 +10,9 @@ ROOT = os.path.dirname(PKG)                 # femtools.jl_amd/
 REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libfemdr.so")
-SOURCES = [os.path.join(CSRC, "femdr.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, "femdr_kernels.hpp"), os.path.join(REPO, "include", "femdr.h")]
+SOURCES = [os.path.join(CSRC, "femdr.hip"), os.path.join(CSRC, "femdr_thermal.hip")]
+DEPS = SOURCES + [os.path.join(CSRC, "femdr_kernels.hpp"), os.path.join(CSRC, "femdr_host.hpp"),
+                  os.path.join(REPO, "include", "femdr.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

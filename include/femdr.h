@@ -240,6 +240,70 @@ int fem_stokes_jacobian_diagnostics(fem_stokes* st, const fem_drucker_prager* pl
 int fem_stokes_time_element_kernel(fem_stokes* st, const fem_drucker_prager* plastic, double dt,
                                    int32_t n, double* avg_ms);
 
+/* ------------------------------------------------------- thermal diffusion
+ * ThermalMaterial (src/heat_diffusion/types/heat_diffusion_types.jl:15-41) */
+typedef struct {
+    int32_t nphases;
+    double k[FEM_MAX_PHASES];      /* conductivity          */
+    double Cp[FEM_MAX_PHASES];     /* heat capacity         */
+    double rho0[FEM_MAX_PHASES];   /* reference density     */
+    double alpha[FEM_MAX_PHASES];  /* thermal expansivity   */
+    double K[FEM_MAX_PHASES];      /* bulk modulus (Inf ok) */
+} fem_thermal_material;
+
+/* ThermalDiffusionDR(mesh, material; CFL, c_fact, ϵ) (heat_diffusion_types.jl:
+ * 90-153, mesh constructor :141-153): nodal fields, zero; phases all 1. */
+typedef struct {
+    double CFL;     /* 0.98 */
+    double c_fact;  /* 0.9  */
+    double eps;     /* 1e-6 (default tolerance) */
+} fem_thermal_opts;
+
+typedef struct fem_thermal fem_thermal;
+int fem_thermal_create(fem_ctx* ctx, fem_mesh* mesh, const fem_thermal_material* mat,
+                       const fem_thermal_opts* opts, fem_thermal** out);
+int fem_thermal_destroy(fem_thermal* th);
+
+/* dr.R, dr.R0, dr.∂R∂T, dr.PC, dr.T, dr.T0, dr.∂T∂τ, dr.P, dr.source (f64),
+ * dr.phases (i32, 1-based per node) */
+enum {
+    FEM_TFIELD_R = 0, FEM_TFIELD_R0, FEM_TFIELD_DRDT, FEM_TFIELD_PC, FEM_TFIELD_T, FEM_TFIELD_T0,
+    FEM_TFIELD_DTDTAU, FEM_TFIELD_P, FEM_TFIELD_SOURCE, FEM_TFIELD_PHASES, FEM_TFIELD_COUNT
+};
+/* device pointer and length of one field; dtype: 0 = f64, 1 = i32 */
+int fem_thermal_field(fem_thermal* th, int32_t field, void** dptr, int64_t* len, int32_t* dtype);
+
+/* solve!(dr::ThermalDiffusionDR, mesh, bc; dt, tolerance=dr.ϵ,
+ * max_iterations=10_000, check_interval=100, throw_on_failure=true, Tref=273)
+ * (heat_diffusion/solvers/DR.jl:25-54) */
+typedef struct {
+    double dt;
+    double tolerance;           /* < 0: dr.ϵ                                  */
+    int64_t max_iterations;     /* 10_000                                     */
+    int64_t check_interval;     /* 100                                        */
+    double Tref;                /* 273                                        */
+    int32_t throw_on_failure;   /* 1: non-convergence returns FEM_ENOCONV     */
+    int32_t verbose;
+} fem_thermal_params;
+
+/* (; converged, iterations, residual) of solve_dynamic_relaxation!
+ * (dynamic_relaxation/solver.jl:61-122) + wall time */
+typedef struct {
+    int32_t converged;
+    int64_t iterations;
+    double residual;            /* last relative residual ‖R‖/‖R⁰‖ */
+    double seconds;
+} fem_thermal_stats;
+
+/* solve_dynamic_relaxation! over the thermal residual; bc: one Dirichlet set
+ * (may be NULL or empty). */
+int fem_thermal_solve(fem_thermal* th, const fem_bc* bc, const fem_thermal_params* params,
+                      fem_thermal_stats* stats);
+
+/* assemble_diffusion_matrices_atomix! (heat_diffusion/assembly/residual.jl:90-121):
+ * dr.R, and dr.∂R∂T / dr.PC (|J| row sums / |J_ii|) when compute_jacobian. */
+int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t compute_jacobian);
+
 #ifdef __cplusplus
 }
 #endif

@@ -873,3 +873,164 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Thermal diffusion
+// ---------------------------------------------------------------------------
+namespace {
+// interp2ip_phase (src/utils.jl:37-44) over per-node phase ids (1-based)
+double interp_nodes(const double* N, int nn, const double* prop, const int32_t* ph) {
+    double s = 0;
+    for (int i = 0; i < nn; ++i) s += N[i] * prop[ph[i] - 1];
+    return s;
+}
+}  // namespace
+
+extern "C" {
+
+// integrate_residual, src/heat_diffusion/assembly/residual.jl:58-87. The
+// Jacobian is the exact derivative of the same expression: with
+// D = Δt/(ρ Cp) and ρ = ρ0 (1 - α (T - Tref) + β P), ∂D/∂T_j = D ρ0 α N_j / ρ.
+void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
+                           const double* T, const double* T0, const double* s, const int32_t* ph,
+                           const orc_thermal_material* m, const double* P, double dt, double Tref,
+                           double* Re, double* J) {
+    double beta[ORC_MAX_PHASES];
+    for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];   // β = map(inv, K) (residual.jl:62-64)
+    for (int i = 0; i < nn; ++i) Re[i] = 0;
+    if (J)
+        for (int i = 0; i < nn * nn; ++i) J[i] = 0;
+    for (int q = 0; q < nq; ++q) {
+        const double* Nv = N + (int64_t)q * nn;
+        const double* G = dNdx + (int64_t)q * nn * 2;
+        double Tq = 0, Pq = 0, T0q = 0, sq = 0, gx = 0, gy = 0;
+        for (int i = 0; i < nn; ++i) {
+            Tq += Nv[i] * T[i];
+            Pq += Nv[i] * P[i];
+            T0q += Nv[i] * T0[i];
+            sq += Nv[i] * s[i];
+            gx += G[2 * i] * T[i];
+            gy += G[2 * i + 1] * T[i];
+        }
+        const double kq = interp_nodes(Nv, nn, m->k, ph);
+        const double aq = interp_nodes(Nv, nn, m->alpha, ph);
+        const double bq = interp_nodes(Nv, nn, beta, ph);
+        const double r0 = interp_nodes(Nv, nn, m->rho0, ph);
+        const double rho = r0 * (1 - aq * (Tq - Tref) + bq * Pq);
+        const double D = dt / (rho * interp_nodes(Nv, nn, m->Cp, ph));
+        const double rate = T0q - Tq + D * sq;
+        for (int i = 0; i < nn; ++i) {
+            const double KTi = kq * (G[2 * i] * gx + G[2 * i + 1] * gy);
+            Re[i] += rate * Nv[i] * dOm[q] - D * KTi * dOm[q];
+        }
+        if (J) {
+            const double c = D * r0 * aq / rho;
+            for (int i = 0; i < nn; ++i) {
+                const double KTi = kq * (G[2 * i] * gx + G[2 * i + 1] * gy);
+                for (int j = 0; j < nn; ++j) {
+                    const double dD = c * Nv[j];
+                    const double Kij = kq * (G[2 * i] * G[2 * j] + G[2 * i + 1] * G[2 * j + 1]);
+                    J[i * nn + j] += ((-Nv[j] + dD * sq) * Nv[i] - dD * KTi - D * Kij) * dOm[q];
+                }
+            }
+        }
+    }
+}
+
+// assemble_diffusion_matrices_atomix! (heat residual.jl:90-121) via
+// assemble_dr_matrices_atomix! (dynamic_relaxation/assembly.jl:43-60); element
+// contributions are added in ascending element order.
+void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian) {
+    const Tables& tb = tables();
+    for (int64_t n = 0; n < t->nnodes; ++n) t->R[n] = 0;
+    if (compute_jacobian)
+        for (int64_t n = 0; n < t->nnodes; ++n) t->dRdT[n] = t->PC[n] = 0;
+    double N[7][7], dNdx[7][7][2], dOm[7], Tl[7], T0l[7], sl[7], Pl[7], Re[7], J[49];
+    int32_t ph[7];
+    for (int q = 0; q < 7; ++q)
+        for (int a = 0; a < 7; ++a) N[q][a] = tb.Nq[q][a];
+    for (int64_t e = 0; e < t->nels; ++e) {
+        const int32_t* nd = t->el2n + e * 7;
+        for (int q = 0; q < 7; ++q) {
+            const double* iJ = t->invJ + (e * 7 + q) * 4;
+            for (int a = 0; a < 7; ++a) {
+                dNdx[q][a][0] = tb.dNdxi[q][a][0] * iJ[0] + tb.dNdxi[q][a][1] * iJ[2];
+                dNdx[q][a][1] = tb.dNdxi[q][a][0] * iJ[1] + tb.dNdxi[q][a][1] * iJ[3];
+            }
+            dOm[q] = t->dOmega[e * 7 + q];
+        }
+        for (int a = 0; a < 7; ++a) {
+            Tl[a] = t->T[nd[a]]; T0l[a] = t->T0[nd[a]]; sl[a] = t->source[nd[a]]; Pl[a] = t->P[nd[a]];
+            ph[a] = t->phases[nd[a]];
+        }
+        orc_thermal_integrate(7, 7, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref, Re,
+                              compute_jacobian ? J : nullptr);
+        for (int a = 0; a < 7; ++a) t->R[nd[a]] += Re[a];
+        if (compute_jacobian) {
+            for (int a = 0; a < 7; ++a) {
+                double rs = 0;
+                for (int b = 0; b < 7; ++b) rs += std::fabs(J[a * 7 + b]);
+                t->dRdT[nd[a]] += rs;
+                t->PC[nd[a]] += std::fabs(J[a * 7 + a]);
+            }
+        }
+    }
+}
+
+// solve_dynamic_relaxation!, src/dynamic_relaxation/solver.jl:61-122
+int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out) {
+    g_err.clear();
+    const int64_t n = t->nnodes;
+    double a_dr = 0, b_dr = 0, nr0 = 0, last_rel = std::numeric_limits<double>::quiet_NaN(), lmax = 0;
+    for (int64_t it = 1; it <= p->max_iterations; ++it) {
+        const bool do_jac = (it % p->check_interval == 0) || it == 1;
+        if (do_jac)
+            for (int64_t i = 0; i < n; ++i) t->R0[i] = t->R[i];
+        orc_thermal_residual(t, p->dt, p->Tref, do_jac);
+        dirichlet(t->R, t->nbc, t->bc_dofs, nullptr);
+        dirichlet(t->rate, t->nbc, t->bc_dofs, nullptr);
+        if (do_jac && !checked_lmax(t->dRdT, t->PC, n, "thermal diffusion", &lmax)) {
+            out->status = -6;
+            return -6;
+        }
+        for (int64_t i = 0; i < n; ++i) t->rate[i] = t->R[i] / t->PC[i] + b_dr * t->rate[i];
+        for (int64_t i = 0; i < n; ++i) t->T[i] += a_dr * t->rate[i];
+        dirichlet(t->T, t->nbc, t->bc_dofs, t->bc_vals);
+        if (do_jac) {
+            const double nr = nrm2(t->R, n);
+            if (it == 1) nr0 = std::max(nr, julia_eps(nr));
+            if (std::isnan(nr / nr0)) {
+                g_err = "NaNs at PT iter " + std::to_string(it);
+                out->status = -4;
+                return -4;
+            }
+            const double dtau = 2 / std::sqrt(lmax) * t->CFL;
+            double den = 0, num = 0;
+            for (int64_t i = 0; i < n; ++i) {
+                const double r = dtau * t->rate[i];
+                den += r * r;
+                num += r * ((t->R[i] - t->R0[i]) / t->PC[i]);
+            }
+            const double lmin = (it == 1 || den == 0) ? 0.0 : std::fabs(num) / den;
+            const double c = 2 * std::sqrt(lmin) * t->c_fact;
+            a_dr = 2 * dtau * dtau / (2 + c * dtau);
+            b_dr = (2 - c * dtau) / (2 + c * dtau);
+            last_rel = nr / nr0;
+            if (p->verbose) printf("  PT %05lld  res = %6.2e\n", (long long)it, last_rel);
+            if (last_rel < p->tolerance) {
+                out->converged = 1;
+                out->iterations = it;
+                out->residual = last_rel;
+                out->status = 0;
+                return 0;
+            }
+        }
+    }
+    out->converged = 0;
+    out->iterations = p->max_iterations;
+    out->residual = last_rel;
+    out->status = 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -167,6 +167,63 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out);
 /* update_stokes_current_stress! (DR.jl:820-897): tau_store must be set */
 void orc_update_current_stress(orc_stokes* s, double dt);
 
+/* --- thermal diffusion (src/heat_diffusion/, src/dynamic_relaxation/solver.jl) --- */
+
+/* ThermalMaterial (src/heat_diffusion/types/heat_diffusion_types.jl:15-41) */
+typedef struct {
+    int32_t nphases;
+    double k[ORC_MAX_PHASES];
+    double Cp[ORC_MAX_PHASES];
+    double rho0[ORC_MAX_PHASES];
+    double alpha[ORC_MAX_PHASES];
+    double K[ORC_MAX_PHASES];
+} orc_thermal_material;
+
+/* ThermalDiffusionDR (heat_diffusion_types.jl:90-153) on a T7 mesh. */
+typedef struct {
+    int64_t nnodes, nels;
+    const int32_t* el2n;    /* [nels][7], 0-based */
+    const double* invJ;     /* [nels][7][4] */
+    const double* dOmega;   /* [nels][7]    */
+    const int32_t* phases;  /* [nnodes], 1-based per-node phases */
+    orc_thermal_material mat;
+    double *R, *R0, *dRdT, *PC, *T, *T0, *rate;
+    const double* P;        /* [nnodes] */
+    const double* source;   /* [nnodes] */
+    int64_t nbc;
+    const int32_t* bc_dofs; /* 0-based */
+    const double* bc_vals;
+    double CFL, c_fact;     /* 0.98, 0.9 */
+} orc_thermal;
+
+typedef struct {
+    double dt, tolerance, Tref;        /* tolerance defaults to dr.ϵ = 1e-6, Tref 273 */
+    int64_t max_iterations, check_interval;
+    int32_t verbose;
+} orc_thermal_params;
+
+typedef struct {
+    int32_t converged;
+    int64_t iterations;
+    double residual;                  /* last relative residual ‖R‖/‖R⁰‖ */
+    int32_t status;                   /* 0 ok, <0 error (orc_last_error)  */
+} orc_thermal_stats;
+
+/* integrate_residual (src/heat_diffusion/assembly/residual.jl:58-87) on caller
+ * tables N [nq][nn], dNdx [nq][nn][2], dOm [nq]; phases 1-based per node.
+ * J (nullable) receives dRe_i/dT_j [nn][nn], the matrix the reference takes
+ * with ForwardDiff (residual.jl:30-47). */
+void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
+                           const double* T, const double* T0, const double* s, const int32_t* phases,
+                           const orc_thermal_material* m, const double* P, double dt, double Tref,
+                           double* Re, double* J);
+/* assemble_diffusion_matrices_atomix! (residual.jl:90-121): R, and ∂R∂T (row
+ * sums of |J|) and PC (|J_ii|) when compute_jacobian */
+void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian);
+/* solve!(::ThermalDiffusionDR, ...) -> solve_dynamic_relaxation!
+ * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
+int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,7 +75,33 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ThermalMaterialC(C.Structure):
+    _fields_ = [("nphases", I32), ("k", D * MAXPH), ("Cp", D * MAXPH), ("rho0", D * MAXPH), ("alpha", D * MAXPH),
+                ("K", D * MAXPH)]
+
+
+class ThermalC(C.Structure):
+    _fields_ = [("nnodes", I64), ("nels", I64), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("phases", PI),
+                ("mat", ThermalMaterialC), ("R", PD), ("R0", PD), ("dRdT", PD), ("PC", PD), ("T", PD), ("T0", PD),
+                ("rate", PD), ("P", PD), ("source", PD), ("nbc", I64), ("bc_dofs", PI), ("bc_vals", PD),
+                ("CFL", D), ("c_fact", D)]
+
+
+class ThermalParamsC(C.Structure):
+    _fields_ = [("dt", D), ("tolerance", D), ("Tref", D), ("max_iterations", I64), ("check_interval", I64),
+                ("verbose", I32)]
+
+
+class ThermalStatsC(C.Structure):
+    _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("status", I32)]
+
+
 def _declare(L):
+    L.orc_thermal_integrate.argtypes = [I32, I32, PD, PD, PD, PD, PD, PD, PI, C.POINTER(ThermalMaterialC), PD, D, D,
+                                        PD, PD]
+    L.orc_thermal_residual.argtypes = [C.POINTER(ThermalC), D, D, I32]
+    L.orc_thermal_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(ThermalParamsC), C.POINTER(ThermalStatsC)]
+    L.orc_thermal_solve.restype = C.c_int
     L.orc_last_error.restype = C.c_char_p
     L.orc_t7_basis.argtypes = [D, D, PD, PD]
     L.orc_t7_rule.argtypes = [PD, PD, PD]
@@ -320,3 +346,71 @@ def pure_shear_bc(coords, box=(0.0, 1.0, 0.0, 1.0), edot=1.0):
     vx = edot * (coords[nx, 0] - xm)
     vy = -edot * (coords[ny, 1] - ym)
     return (nx, vx), (ny, vy)
+
+
+# ---------------------------------------------------------------------------
+# thermal diffusion
+def thermal_material(k=(1.0,), Cp=(1.0,), rho0=(1.0,), alpha=(1.0,), K=(1.0,)):
+    """ThermalMaterial (heat_diffusion_types.jl:15-41); defaults are the reference's unit values."""
+    m = ThermalMaterialC()
+    n = len(k)
+    m.nphases = n
+    for i in range(n):
+        m.k[i], m.Cp[i], m.rho0[i], m.alpha[i], m.K[i] = k[i], Cp[i], rho0[i], alpha[i], K[i]
+    return m
+
+
+def thermal_integrate(N, dNdx, dOm, T, T0, s, phases, mat, P, dt, Tref, jacobian=False):
+    """integrate_residual (heat residual.jl:58-87) on caller tables: N [nq][nn],
+    dNdx [nq][nn][2], dOm [nq]. Returns Re (and J [nn][nn])."""
+    N = f64(N)
+    nq, nn = N.shape
+    Re = np.zeros(nn)
+    J = np.zeros((nn, nn)) if jacobian else None
+    lib().orc_thermal_integrate(nn, nq, ptr(N), ptr(f64(dNdx)), ptr(f64(dOm)), ptr(f64(T)), ptr(f64(T0)),
+                                ptr(f64(s)), ptr(i32(phases), PI), C.byref(mat), ptr(f64(P)), dt, Tref, ptr(Re),
+                                ptr(J))
+    return (Re, J) if jacobian else Re
+
+
+class ThermalProblem:
+    """Host arrays of a ThermalDiffusionDR on a T7 mesh (heat_diffusion_types.jl:90-153)."""
+
+    def __init__(self, coords, el2n, mat, phases=None, bc=(np.zeros(0, np.int32), np.zeros(0)), CFL=0.98,
+                 c_fact=0.9):
+        self.coords = f64(coords)
+        self.el2n = i32(el2n)
+        nn, ne = len(self.coords), len(self.el2n)
+        self.nnodes, self.nels = nn, ne
+        self.invJ, self.dOmega, _ = t7_geometry(self.coords, self.el2n)
+        self.phases = i32(np.ones(nn) if phases is None else phases)
+        self.mat = mat
+        for k in ("R", "R0", "dRdT", "PC", "T", "T0", "rate", "P", "source"):
+            setattr(self, k, np.zeros(nn))
+        self.bc = (i32(bc[0]), f64(bc[1]))
+        self.CFL, self.c_fact = CFL, c_fact
+
+    def struct(self):
+        t = ThermalC()
+        t.nnodes, t.nels = self.nnodes, self.nels
+        t.el2n, t.invJ, t.dOmega, t.phases = ptr(self.el2n, PI), ptr(self.invJ), ptr(self.dOmega), ptr(self.phases, PI)
+        t.mat = self.mat
+        for k in ("R", "R0", "dRdT", "PC", "T", "T0", "rate", "P", "source"):
+            setattr(t, k, ptr(getattr(self, k)))
+        t.nbc = len(self.bc[0])
+        t.bc_dofs, t.bc_vals = ptr(self.bc[0], PI), ptr(self.bc[1])
+        t.CFL, t.c_fact = self.CFL, self.c_fact
+        self._struct = t
+        return t
+
+    def residual(self, dt, Tref=273.0, jacobian=False):
+        lib().orc_thermal_residual(C.byref(self.struct()), dt, Tref, int(jacobian))
+
+    def solve(self, dt, tolerance=1e-6, max_iterations=10_000, check_interval=100, Tref=273.0, verbose=False):
+        p = ThermalParamsC(dt=dt, tolerance=tolerance, Tref=Tref, max_iterations=max_iterations,
+                           check_interval=check_interval, verbose=int(verbose))
+        st = ThermalStatsC()
+        rc = lib().orc_thermal_solve(C.byref(self.struct()), C.byref(p), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual}

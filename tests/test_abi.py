@@ -39,7 +39,9 @@ def test_python_mirror_binds_every_declared_symbol():
 
 STRUCTS = {"fem_mesh_info": "_MeshInfo", "fem_material": "_Material", "fem_drucker_prager": "_DP",
            "fem_state_opts": "_StateOpts", "fem_bc": "_BC", "fem_solve_params": "_SolveParams",
-           "fem_solve_stats": "_SolveStats"}
+           "fem_solve_stats": "_SolveStats", "fem_thermal_material": "_ThermalMaterial",
+           "fem_thermal_opts": "_ThermalOpts", "fem_thermal_params": "_ThermalParams",
+           "fem_thermal_stats": "_ThermalStats"}
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libfemdr.so not built")
