@@ -30,7 +30,7 @@ struct fem_stokes {
     double* d_bc_stage_v = nullptr;
     int64_t bc_stage_cap = 0;
     double* d_ptab = nullptr;
-    int32_t* d_flag = nullptr;      // scratch yes/no flag
+    int32_t* d_flag = nullptr;      // [0] scratch yes/no flag, [1] yielding-element count
     bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
     bool q_nonzero = false;         // dr.Q holds a nonzero entry
     // FP32 work arrays of the mixed-precision solve (allocated by its first call):
@@ -399,7 +399,7 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         (r = dalloc_t(c, &s->d_Jrow, NSH * m->nels)) || (r = dalloc_t(c, &s->d_Jdiag, NSH * m->nels)) ||
         (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
         (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH)) ||
-        (r = dalloc_t(c, &s->d_flag, 1))) {
+        (r = dalloc_t(c, &s->d_flag, 2))) {
         fem_stokes_destroy(s);
         return r;
     }
@@ -580,6 +580,7 @@ struct LaunchOpts {
     bool store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false, store_tau = false;
     double dt = 1.0;
     double alpha[2] = {0, 0}, beta[2] = {0, 0};
+    int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
 };
 
 constexpr int EBLOCK = 128;
@@ -719,6 +720,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     a.xvx = d.xvx;
     a.xvy = d.xvy;
     a.xP = d.xP;
+    a.yield_count = o.yield_count;
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
@@ -1182,7 +1184,11 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // term is τ(x + δ) - τ(x) with x an FP32 copy of the state (refreshed at every
 // fold) and τ_old an FP32 copy of the stress history, and the Jacobian
 // diagnostics are re-assembled in FP64 at every check unless frozen, as the
-// reference does (DR.jl:726-745). The solution x meets the Dirichlet values
+// reference does (DR.jl:726-745). The FP64 residual evaluations count the
+// elements with a yielding point; an interval whose last known count is zero
+// runs the cheaper visco-elastic operator, exact there (the count arrives one
+// check interval late; a point that starts to yield inside an interval is
+// corrected by the next fold). The solution x meets the Dirichlet values
 // before the first iteration.
 #ifndef FEM_MIXED_RETURN_MAP
 #define FEM_MIXED_RETURN_MAP 1
@@ -1192,12 +1198,19 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     fem_ctx* c = s->ctx;
     fem_mesh* m = s->mesh;
     const int64_t nv = m->nnodes, np = m->nnodesP, nip = NQ * m->nels, ncheck = p->check_interval;
+    // the return-map operator is used on the check intervals that start from a
+    // state with yielding points; otherwise the visco-elastic operator is exact
     const bool delta = o.plastic && FEM_MIXED_RETURN_MAP;
     CHK(ensure_w32(s, delta));
     fem_stokes::W32& w = *s->w32;
     const DRSet<double> d64 = set64(s);
     DRSet<float> d32 = set32(s);
     d32.delta = delta;
+    // yielding elements of the last FP64 evaluation: counted on the device,
+    // copied to pinned memory behind it, read at the next synchronisation
+    int32_t* d_yield = s->d_flag + 1;
+    int32_t* h_yield = reinterpret_cast<int32_t*>(c->h_red + (RED_MAXK - 1));
+    *h_yield = 1;
     CHK(cast_to32(c, d64.gammaP, w.gammaP, np));
     CHK(cast_to32(c, d64.MP, w.MP, np));
     CHK(cast_to32(c, s->d_pscale, w.pscale, np));
@@ -1249,14 +1262,22 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     LaunchOpts o32 = o;
     o32.plastic = delta;
     o32.elastic = delta && o.elastic;
+    // FP64 residual at x; with plasticity it counts the yielding elements
+    auto residual64 = [&](const LaunchOpts& oo) -> int {
+        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
+        CHK(launch_element(s, d64, oo));
+        if (delta) HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 4, hipMemcpyDeviceToHost, c->stream));
+        return launch_gather(s, d64, oo);
+    };
     for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
         itPH_done = itPH;
         LaunchOpts oo = o;
         oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
-        CHK(launch_element(s, d64, oo));
-        CHK(launch_gather(s, d64, oo));
+        oo.yield_count = delta ? d_yield : nullptr;
+        CHK(residual64(oo));
         double r3[3];
         CHK(reduce_outer(s, d64, r3));
+        d32.delta = delta && *h_yield > 0;
         timer.collect();
         err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
         err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
@@ -1329,12 +1350,14 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 // size of δ, so the FP32 iterations only ever reduce the residual
                 // of the last ncheck steps, far above their rounding floor, and
                 // the operator is renewed at the current state
+                // the yield count of the previous fold has arrived with this
+                // check's synchronisation: it picks the next interval's operator
+                d32.delta = delta && *h_yield > 0;
                 CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
                 CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
                 HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
                 HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
-                CHK(launch_element(s, d64, oo));
-                CHK(launch_gather(s, d64, oo));
+                CHK(residual64(oo));
                 CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
                 CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
                 CHK(cast_to32(c, d64.RP, w.rhs_P, np));

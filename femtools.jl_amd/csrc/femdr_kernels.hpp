@@ -140,10 +140,12 @@ struct IpMat {
 };
 
 // Deviatoric stress with the plane-strain Drucker-Prager return
-// (rheology.jl:226-245, 272-334). S is F, or D3<F> for the tangent.
+// (rheology.jl:226-245, 272-334). S is F, or D3<F> for the tangent. Returns
+// whether the point yields.
 template <class S, class F, bool PLASTIC>
-__device__ inline void stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
+__device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
                                 S& txx, S& tyy, S& txy) {
+    bool yields = false;
     S tr = (exx + eyy) * F(1.0 / 3.0);
     const F two_eve = F(2) * m.etave;
     txx = two_eve * ((exx - tr) + txo * m.inv2Gdt);
@@ -165,9 +167,11 @@ __device__ inline void stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F
                 txx = txx - nxx;
                 tyy = tyy - nyy;
                 txy = txy - nxy;
+                yields = true;
             }
         }
     }
+    return yields;
 }
 
 // Plastic correction c = τ_trial - τ_returned of the plane-strain
@@ -260,6 +264,9 @@ struct ElemArgs {
     const F* rhs_P; const F* rhs_vx; const F* rhs_vy;
     // EF_DELTA: the state x the correction is taken about
     const F* xvx; const F* xvy; const F* xP;
+    // nullable: receives the number of elements with a yielding point (one
+    // atomic add per wave)
+    int32_t* yield_count;
     // EF_GEN extras
     int32_t nodeph;            // phases stored per element node
     int32_t bulk_all_inf;
@@ -459,6 +466,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         for (int a = 0; a < NV; ++a) pv[a] = A.phase_v[a * ne + e];
     }
     F Rx[NV], Ry[NV];
+    bool yel = false;
 #pragma unroll
     for (int a = 0; a < NV; ++a) { Rx[a] = 0; Ry[a] = 0; }
     const bool grav = (A.gx != F(0)) || (A.gy != F(0));
@@ -551,7 +559,8 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                 txy += dxy - cxy;
             }
         } else {
-            stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy, txy);
+            yel |= stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy,
+                                           txy);
         }
         if (A.tau_out0) {
             A.tau_out0[e * NQ + q] = txx;
@@ -598,6 +607,12 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                 Ry[a] -= Na * by;
             }
         }
+    }
+
+    if (PLASTIC && A.yield_count) {
+        const uint64_t mask = __ballot(yel);
+        if (mask && (threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1)
+            atomicAdd(A.yield_count, (int32_t)__popcll(mask));
     }
 
     // ---- shared nodes: element-slot residuals for the gather
