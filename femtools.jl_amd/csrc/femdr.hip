@@ -31,6 +31,7 @@ struct fem_stokes {
     int64_t bc_stage_cap = 0;
     double* d_ptab = nullptr;
     int32_t* d_flag = nullptr;      // [0] scratch yes/no flag, [1] yielding-element count
+    uint8_t* d_yflag = nullptr;     // near-yield flag per element (mixed-precision solve)
     bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
     bool q_nonzero = false;         // dr.Q holds a nonzero entry
     // FP32 work arrays of the mixed-precision solve (allocated by its first call):
@@ -40,8 +41,8 @@ struct fem_stokes {
         float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
         float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
         float2* Re;
-        // state copies for the correction operator with the return map
-        float *xvx, *xvy, *xP, *tau_old0, *tau_old1, *tau_old2;
+        // stress-history copy for the correction operator with the return map
+        float *tau_old0, *tau_old1, *tau_old2;
     };
     W32* w32 = nullptr;
 };
@@ -419,6 +420,7 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
     hipFree(s->d_flag);
+    hipFree(s->d_yflag);
     if (s->w32) {
         float** p = (float**)s->w32;
         for (size_t k = 0; k < sizeof(fem_stokes::W32) / sizeof(float*); ++k) hipFree(p[k]);
@@ -581,6 +583,7 @@ struct LaunchOpts {
     double dt = 1.0;
     double alpha[2] = {0, 0}, beta[2] = {0, 0};
     int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
+    uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
 };
 
 constexpr int EBLOCK = 128;
@@ -603,7 +606,8 @@ struct DRSet {
     F* geo = nullptr;
     typename Vec2<F>::T* Re = nullptr;
     const F *rhs_P = nullptr, *rhs_vx = nullptr, *rhs_vy = nullptr;
-    const F *xvx = nullptr, *xvy = nullptr, *xP = nullptr;
+    const double *xvx = nullptr, *xvy = nullptr, *xP = nullptr, *xPf = nullptr;
+    const uint8_t* hot = nullptr;   // near-yield flags of the elements (EF_DELTA)
     bool lin = false;     // correction operator (EF_LIN)
     bool delta = false;   // ... with the return map about the state x (EF_DELTA)
 };
@@ -638,7 +642,8 @@ static DRSet<float> set32(fem_stokes* s) {
     d.ptab = s->d_ptab; d.geo = w.geo;
     d.Re = w.Re;
     d.rhs_P = w.rhs_P; d.rhs_vx = w.rhs_vx; d.rhs_vy = w.rhs_vy;
-    d.xvx = w.xvx; d.xvy = w.xvy; d.xP = w.xP;
+    d.xvx = s->f[FEM_FIELD_V_X]; d.xvy = s->f[FEM_FIELD_V_Y]; d.xP = s->f[FEM_FIELD_P];
+    d.xPf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
     d.tau_old[0] = w.tau_old0; d.tau_old[1] = w.tau_old1; d.tau_old[2] = w.tau_old2;
     d.lin = true;
     return d;
@@ -720,7 +725,10 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     a.xvx = d.xvx;
     a.xvy = d.xvy;
     a.xP = d.xP;
+    a.xPf = d.xPf;
     a.yield_count = o.yield_count;
+    a.yield_flag = o.yield_flag;
+    a.hot = d.hot;
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
@@ -1124,11 +1132,10 @@ static int ensure_w32(fem_stokes* s, bool delta) {
     fem_ctx* c = s->ctx;
     const int64_t nv = m->nnodes, np = m->nnodesP, ne = m->nels;
     if (s->w32) {
-        if (!delta || s->w32->xvx) return FEM_OK;
+        if (!delta || s->w32->tau_old0) return FEM_OK;
         fem_stokes::W32& w = *s->w32;
-        CHK(dalloc_t(c, &w.xvx, nv));
-        CHK(dalloc_t(c, &w.xvy, nv));
-        CHK(dalloc_t(c, &w.xP, np));
+        CHK(dalloc_t(c, &s->d_yflag, ne));
+        HIPCHK(c, hipMemsetAsync(s->d_yflag, 1, ne, c->stream));
         // +16: 16-byte global->LDS loads of the last block read past the end
         for (float** q : {&w.tau_old0, &w.tau_old1, &w.tau_old2}) {
             CHK(dalloc_t(c, q, NQ * ne + 16));
@@ -1181,8 +1188,9 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // Without plasticity L is the visco-elastic operator, exact, and its Jacobian
 // diagnostics (preconditioner, λmax) are assembled once. With a Drucker-
 // Prager rheology the correction operator keeps the return map: its stress
-// term is τ(x + δ) - τ(x) with x an FP32 copy of the state (refreshed at every
-// fold) and τ_old an FP32 copy of the stress history, and the Jacobian
+// term is τ(x + δ) - τ(x), both returned in FP64 about the FP64 state (read in
+// blocks with points near yield only) with τ_old an FP32 copy of the stress
+// history, and the Jacobian
 // diagnostics are re-assembled in FP64 at every check unless frozen, as the
 // reference does (DR.jl:726-745). The FP64 residual evaluations count the
 // elements with a yielding point; an interval whose last known count is zero
@@ -1192,6 +1200,15 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // before the first iteration.
 #ifndef FEM_MIXED_RETURN_MAP
 #define FEM_MIXED_RETURN_MAP 1
+#endif
+// On intervals with yielding points δ is also folded every FEM_FOLD_INTERVAL
+// iterations between checks: the PH/DYREL trajectory under plasticity is
+// sensitive to perturbations, and smaller FP32 corrections keep the mixed
+// iteration closer to the FP64 one (512² VEP, plastic step: 203k iterations
+// folding at checks only, 151k every 25, 144k-165k every 10, 139k in FP64), for
+// one FP64 residual per fold.
+#ifndef FEM_FOLD_INTERVAL
+#define FEM_FOLD_INTERVAL 25
 #endif
 static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpts& o, bool freeze, KernelTimer& timer,
                        fem_solve_stats* st) {
@@ -1206,6 +1223,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     const DRSet<double> d64 = set64(s);
     DRSet<float> d32 = set32(s);
     d32.delta = delta;
+    d32.hot = s->d_yflag;
     // yielding elements of the last FP64 evaluation: counted on the device,
     // copied to pinned memory behind it, read at the next synchronisation
     int32_t* d_yield = s->d_flag + 1;
@@ -1217,25 +1235,11 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     hipLaunchKernelGGL(k_pin_bc, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
                        (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, d64.vx, d64.vy);
     CHK(check_launch(c, "k_pin_bc"));
-    // FP32 copies of the state the return-map operator is taken about
-    auto refresh_state = [&](bool pressure) -> int {
-        if (!delta) return FEM_OK;
-        CHK(cast_to32(c, d64.vx, w.xvx, nv));
-        CHK(cast_to32(c, d64.vy, w.xvy, nv));
-        if (pressure) {
-            // the yield pressure is P - Pf (momentum_residuals.jl:165-173)
-            hipLaunchKernelGGL(k_cast_sub, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, (const double*)d64.P,
-                               s->pf_nonzero ? (const double*)d64.Pf : nullptr, w.xP);
-            CHK(check_launch(c, "k_cast_sub"));
-        }
-        return FEM_OK;
-    };
     if (delta && o.elastic) {
         CHK(cast_to32(c, d64.tau_old[0], w.tau_old0, nip));
         CHK(cast_to32(c, d64.tau_old[1], w.tau_old1, nip));
         CHK(cast_to32(c, d64.tau_old[2], w.tau_old2, nip));
     }
-    CHK(refresh_state(true));
     // Jacobian diagnostics of the correction operator
     LaunchOpts oj = o;
     oj.plastic = delta;
@@ -1244,6 +1248,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     CHK(checked_lmax(s, lmax));
     double lmax_g = std::max(lmax[0], lmax[1]);
     int64_t jac = 1;
+    bool jac_elastic = false;   // the last assembly was at a state without yielding
     CHK(cast_to32(c, d64.PC_x, w.PC_x, nv));
     CHK(cast_to32(c, d64.PC_y, w.PC_y, nv));
     for (float* z : {w.vx, w.vy, w.rate_x, w.rate_y, w.Rv0_x, w.Rv0_y}) HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
@@ -1274,6 +1279,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         LaunchOpts oo = o;
         oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
         oo.yield_count = delta ? d_yield : nullptr;
+        oo.yield_flag = delta ? s->d_yflag : nullptr;
         CHK(residual64(oo));
         double r3[3];
         CHK(reduce_outer(s, d64, r3));
@@ -1352,7 +1358,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 // the operator is renewed at the current state
                 // the yield count of the previous fold has arrived with this
                 // check's synchronisation: it picks the next interval's operator
-                d32.delta = delta && *h_yield > 0;
+                const int32_t ycount = *h_yield;
+                d32.delta = delta && ycount > 0;
                 CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
                 CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
                 HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
@@ -1361,8 +1368,12 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
                 CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
                 CHK(cast_to32(c, d64.RP, w.rhs_P, np));
-                CHK(refresh_state(false));
-                if (delta && !freeze) {
+                // without yielding points the diagnostics are the visco-elastic
+                // ones: re-assemble only when yielding was seen since the last
+                // assembly
+                const bool skip_jac = ycount == 0 && jac_elastic;
+                if (delta && !freeze && !skip_jac) {
+                    jac_elastic = ycount == 0;
                     ++jac;
                     CHK(launch_jacobian(s, o));
                     rc = checked_lmax(s, lmax);
@@ -1373,6 +1384,16 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 }
                 for (int k = 0; k < 2; ++k)
                     cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+            } else if (FEM_FOLD_INTERVAL > 0 && d32.delta && iter % FEM_FOLD_INTERVAL == 0) {
+                // extra folds between checks on intervals with yielding points
+                CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
+                CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
+                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
+                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                CHK(residual64(oo));
+                CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
+                CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
+                CHK(cast_to32(c, d64.RP, w.rhs_P, np));
             }
             if (iter > p->max_iterations) { stop_total = true; break; }
         }
@@ -1387,7 +1408,6 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
-        CHK(refresh_state(true));
         if (stop_total || iter > p->max_iterations) break;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));

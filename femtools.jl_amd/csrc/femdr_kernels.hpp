@@ -144,7 +144,7 @@ struct IpMat {
 // whether the point yields.
 template <class S, class F, bool PLASTIC>
 __device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
-                                S& txx, S& tyy, S& txy) {
+                                S& txx, S& tyy, S& txy, F* fyield = nullptr) {
     bool yields = false;
     S tr = (exx + eyy) * F(1.0 / 3.0);
     const F two_eve = F(2) * m.etave;
@@ -155,6 +155,7 @@ __device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F
         S tzz = -txx - tyy;
         S tII = dsqrt((txx * txx + tyy * tyy + tzz * tzz) * F(0.5) + txy * txy + eps2<F>());
         S F_ = (tII - m.yield0) - m.sphi * Pq;
+        if (fyield) *fyield = val(F_);
         // whole waves without a yielding point skip the return (no if-conversion)
         if (__any(val(F_) > F(0)) && val(F_) > F(0)) {
             S lam = F_ * m.invden;
@@ -208,7 +209,7 @@ enum ElemFlags {
                       // given; storage on δP alone, density linear in δP. Alone: L is the
                       // visco-elastic operator (no τ_old, no return map)
     EF_DELTA = 128,   // with EF_LIN|EF_PLASTIC: the stress term is τ(x + δ) - τ(x), both with
-                      // the Drucker-Prager return, x given as an FP32 copy (xvx, xvy, xP)
+                      // the Drucker-Prager return about the FP64 state x (xvx, xvy, xP)
 };
 
 // Structured T7 box mesh (connectivity.jl:111-140): element e = 2c + k of cell
@@ -262,11 +263,18 @@ struct ElemArgs {
     int32_t phase_same;        // phase_P holds the same cell phases as phase_v
     // EF_LIN right-hand sides R(x) of the correction problem (element DoFs / nodes)
     const F* rhs_P; const F* rhs_vx; const F* rhs_vy;
-    // EF_DELTA: the state x the correction is taken about
-    const F* xvx; const F* xvy; const F* xP;
+    // EF_DELTA: the FP64 state x the correction is taken about (pressure,
+    // nullable fluid pressure)
+    const double* xvx; const double* xvy; const double* xP; const double* xPf;
     // nullable: receives the number of elements with a yielding point (one
     // atomic add per wave)
     int32_t* yield_count;
+    // nullable: per element, 1 if a point is within 20 % of its yield stress
+    // of the yield surface (or beyond); written with yield_count
+    uint8_t* yield_flag;
+    // EF_DELTA: those flags; blocks without a flagged element skip the return
+    // map (their operator is the exact visco-elastic one)
+    const uint8_t* hot;
     // EF_GEN extras
     int32_t nodeph;            // phases stored per element node
     int32_t bulk_all_inf;
@@ -290,6 +298,9 @@ __device__ inline int64_t xcd_tile() {
 #endif
 // τ_old staged through LDS by direct global->LDS loads (1) or read per
 // quadrature point (0)
+#ifndef FEM_ALL_HOT
+#define FEM_ALL_HOT 0   // 1: the return-map correction operator in every block
+#endif
 #ifndef FEM_TAU_LDS
 #define FEM_TAU_LDS 1
 #endif
@@ -318,7 +329,9 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     // stream it straight into LDS (no registers held) while the velocity
     // gathers and the continuity residual proceed.
     __shared__ F s_tau[TAU_LDS ? 3 * BLOCK * NQ : 1];
-    if (TAU_LDS && e0 < ne) {
+    bool hot = true;
+    if (DELTA) hot = FEM_ALL_HOT || __syncthreads_or(valid && A.hot[e]);
+    if (TAU_LDS && e0 < ne && hot) {
         const int64_t nvalid = ne - e0 < BLOCK ? ne - e0 : BLOCK;
         const int bytes = (int)(nvalid * NQ * sizeof(F));
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -353,13 +366,6 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     F P[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) P[i] = A.P[e * NP + i];
-    F xvx[DELTA ? NV : 1], xvy[DELTA ? NV : 1], xP[DELTA ? NP : 1];
-    if (DELTA) {
-#pragma unroll
-        for (int a = 0; a < NV; ++a) { xvx[a] = A.xvx[n[a]]; xvy[a] = A.xvy[n[a]]; }
-#pragma unroll
-        for (int i = 0; i < NP; ++i) xP[i] = A.xP[e * NP + i];
-    }
     const int pc = A.phase_v[e];
 
     // ---- continuity residual (pressure_residual.jl:132-158). The element is
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         for (int a = 0; a < NV; ++a) pv[a] = A.phase_v[a * ne + e];
     }
     F Rx[NV], Ry[NV];
-    bool yel = false;
+    bool yel = false, near = false;
 #pragma unroll
     for (int a = 0; a < NV; ++a) { Rx[a] = 0; Ry[a] = 0; }
     const bool grav = (A.gx != F(0)) || (A.gy != F(0));
@@ -532,35 +538,57 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         if (DELTA) {
             // τ(x + δ) - τ(x): the visco-elastic part is linear, 2 ηve ε̇'(δ), and
             // exact; the returns of both states add their plastic corrections
-            F hxx = 0, hxy = 0, hyx = 0, hyy = 0, Pqx = 0;
-#pragma unroll
-            for (int a = 0; a < NV; ++a) {
-                const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
-                const F cx = d0 * xvx[a], cy = d1 * xvx[a], dx = d0 * xvy[a], dy = d1 * xvy[a];
-                hxx += cx * i00 + cy * i10; hxy += cx * i01 + cy * i11;
-                hyx += dx * i00 + dy * i10; hyy += dx * i01 + dy * i11;
-            }
-#pragma unroll
-            for (int i = 0; i < NP; ++i) Pqx += (F)kT.NP1[q][i] * xP[i];
             const F two_eve = F(2) * m.etave;
-            const F trd = (gxx + gyy) * F(1.0 / 3.0), trx = (hxx + hyy) * F(1.0 / 3.0);
+            const F trd = (gxx + gyy) * F(1.0 / 3.0);
             txx = two_eve * (gxx - trd);
             tyy = two_eve * (gyy - trd);
             txy = two_eve * ((gxy + gyx) * F(0.5));
-            const F bxx = two_eve * ((hxx - trx) + txo * m.inv2Gdt);
-            const F byy = two_eve * ((hyy - trx) + tyo * m.inv2Gdt);
-            const F bxy = two_eve * ((hxy + hyx) * F(0.5) + txyo * m.inv2Gdt);
-            F cxx, cyy, cxy, dxx, dyy, dxy;
-            const bool yA = dp_correction<F>(bxx + txx, byy + tyy, bxy + txy, m, Pqx + Pq, cxx, cyy, cxy);
-            const bool yB = dp_correction<F>(bxx, byy, bxy, m, Pqx, dxx, dyy, dxy);
-            if (yA || yB) {
-                txx += dxx - cxx;
-                tyy += dyy - cyy;
-                txy += dxy - cxy;
+          if (hot) {
+            // x's strain rate and the two returns are evaluated in FP64 from the
+            // FP64 state, read through the caches in blocks with points near
+            // yield only: the stress of x then matches the FP64 residual's
+            // (same yield decisions), and the difference of two stresses of
+            // full size keeps an error relative to the correction, not to τ
+            typedef double H;
+            H hxx = 0, hxy = 0, hyx = 0, hyy = 0, Pqx = 0;
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                const H d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+                const H ux = A.xvx[n[a]], uy = A.xvy[n[a]];
+                const H cx = d0 * ux, cy = d1 * ux, dx = d0 * uy, dy = d1 * uy;
+                hxx += cx * (H)i00 + cy * (H)i10; hxy += cx * (H)i01 + cy * (H)i11;
+                hyx += dx * (H)i00 + dy * (H)i10; hyy += dx * (H)i01 + dy * (H)i11;
             }
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                Pqx += kT.NP1[q][i] * (A.xPf ? A.xP[e * NP + i] - A.xPf[e * NP + i] : A.xP[e * NP + i]);
+            IpMat<H> mh;
+            mh.etave = m.etave; mh.inv2Gdt = m.inv2Gdt; mh.yield0 = m.yield0; mh.sphi = m.sphi; mh.invden = m.invden;
+            const H two_h = H(2) * mh.etave;
+            const H trx = (hxx + hyy) * H(1.0 / 3.0);
+            const H bxx = two_h * ((hxx - trx) + (H)txo * mh.inv2Gdt);
+            const H byy = two_h * ((hyy - trx) + (H)tyo * mh.inv2Gdt);
+            const H bxy = two_h * ((hxy + hyx) * H(0.5) + (H)txyo * mh.inv2Gdt);
+            H cxx, cyy, cxy, dxx, dyy, dxy;
+            const bool yA = dp_correction<H>(bxx + (H)txx, byy + (H)tyy, bxy + (H)txy, mh, Pqx + (H)Pq, cxx, cyy, cxy);
+            const bool yB = dp_correction<H>(bxx, byy, bxy, mh, Pqx, dxx, dyy, dxy);
+            if (yA || yB) {
+                txx = (F)((H)txx + (dxx - cxx));
+                tyy = (F)((H)tyy + (dyy - cyy));
+                txy = (F)((H)txy + (dxy - cxy));
+            }
+          }
         } else {
-            yel |= stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx, tyy,
-                                           txy);
+            // the yield statistics are taken by residual evaluations outside the
+            // inner iteration only (no registers spent on them in EF_ITER)
+            constexpr bool STATS = PLASTIC && !ITER;
+            F fy = 0;
+            const bool y = stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx,
+                                                   tyy, txy, STATS ? &fy : nullptr);
+            if (STATS) {
+                yel |= y;
+                near |= fy > F(-0.2) * fabs(m.yield0 + m.sphi * (Pq - Pfq));
+            }
         }
         if (A.tau_out0) {
             A.tau_out0[e * NQ + q] = txx;
@@ -609,10 +637,11 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         }
     }
 
-    if (PLASTIC && A.yield_count) {
+    if (PLASTIC && !ITER && A.yield_count) {
         const uint64_t mask = __ballot(yel);
         if (mask && (threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1)
             atomicAdd(A.yield_count, (int32_t)__popcll(mask));
+        A.yield_flag[e] = near;
     }
 
     // ---- shared nodes: element-slot residuals for the gather
