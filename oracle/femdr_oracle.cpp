@@ -1034,3 +1034,561 @@ int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_s
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// 3-D: Hex27 velocity, P1-disc pressure with four modes per cell
+// ===========================================================================
+namespace {
+
+// QuadraticElement{3, 27} node order (src/elements/shape_functions.jl:411-446)
+const double kHex27Nodes[27][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+    {0, -1, -1},  {1, 0, -1},  {0, 1, -1}, {-1, 0, -1}, {0, -1, 1},  {1, 0, 1},  {0, 1, 1}, {-1, 0, 1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},  {-1, 1, 0},  {0, 0, -1},  {0, -1, 0}, {1, 0, 0}, {0, 1, 0},
+    {-1, 0, 0},   {0, 0, 1},   {0, 0, 0}};
+
+// one-dimensional quadratic Lagrange basis on -1, 0, +1 (shape_functions.jl:173-196)
+double lq(double x, double node) { return node < 0 ? x * (x - 1) * 0.5 : (node == 0 ? 1 - x * x : x * (x + 1) * 0.5); }
+double dlq(double x, double node) { return node < 0 ? x - 0.5 : (node == 0 ? -2 * x : x + 0.5); }
+
+void hex27_basis(double xi, double eta, double zeta, double N[27], double dN[27][3]) {
+    for (int a = 0; a < 27; ++a) {
+        const double* n = kHex27Nodes[a];
+        const double fx = lq(xi, n[0]), fy = lq(eta, n[1]), fz = lq(zeta, n[2]);
+        N[a] = fx * fy * fz;
+        dN[a][0] = dlq(xi, n[0]) * fy * fz;
+        dN[a][1] = fx * dlq(eta, n[1]) * fz;
+        dN[a][2] = fx * fy * dlq(zeta, n[2]);
+    }
+}
+
+// 3x3x3 Gauss rule, ξ fastest (integration_points.jl:321-337)
+void hex27_rule(double xi[27], double eta[27], double zeta[27], double w[27]) {
+    const double a = std::sqrt(3.0 / 5.0);
+    const double pts[3] = {-a, 0.0, a}, ws[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int i = 0; i < 27; ++i) {
+        const int ix = i % 3, iy = (i / 3) % 3, iz = i / 9;
+        xi[i] = pts[ix]; eta[i] = pts[iy]; zeta[i] = pts[iz];
+        w[i] = ws[ix] * ws[iy] * ws[iz];
+    }
+}
+
+// LinearElement{3, 4} modes (1 - ξ - η - ζ, ξ, η, ζ) evaluated on the cube
+// (shape_functions.jl:288-298, mixed_mesh.jl:399-417)
+void hexp1_basis(double xi, double eta, double zeta, double N[4]) {
+    N[0] = 1 - xi - eta - zeta; N[1] = xi; N[2] = eta; N[3] = zeta;
+}
+
+struct Tables3 {
+    double Nq[27][27], NqP[27][4], dNdxi[27][27][3], w[27];
+    Tables3() {
+        double xi[27], eta[27], zeta[27];
+        hex27_rule(xi, eta, zeta, w);
+        for (int q = 0; q < 27; ++q) {
+            hex27_basis(xi[q], eta[q], zeta[q], Nq[q], dNdxi[q]);
+            hexp1_basis(xi[q], eta[q], zeta[q], NqP[q]);
+        }
+    }
+};
+const Tables3& tables3() { static Tables3 t; return t; }
+
+// 3-D deviatoric stress with the Drucker-Prager return at one point
+// (rheology.jl:413-472); gv[i][j] = ∂v_i/∂x_j; tau = (xx, yy, zz, xy, xz, yz).
+template <class S>
+double stress_ip3(const S gv[3][3], const double* Nv, int NV, const int* ph, const Rheo& r, const double tau_o[6],
+                  double Pq, S tau[6]) {
+    S tr = (gv[0][0] + gv[1][1] + gv[2][2]) / 3.0;
+    S eps[6] = {gv[0][0] - tr, gv[1][1] - tr, gv[2][2] - tr, (gv[0][1] + gv[1][0]) / 2.0,
+                (gv[0][2] + gv[2][0]) / 2.0, (gv[1][2] + gv[2][1]) / 2.0};
+    double invG[ORC_MAX_PHASES];
+    for (int p = 0; p < r.mat->nphases; ++p) invG[p] = 1.0 / r.mat->G[p];
+    const double etaq = interp_phase(Nv, NV, r.eta, ph);
+    const double invGq = interp_phase(Nv, NV, invG, ph);
+    const double etave = 1.0 / (1.0 / etaq + invGq / r.dt);
+    const double inv2Gdt = invGq / (2 * r.dt);
+    for (int k = 0; k < 6; ++k) tau[k] = 2 * etave * (eps[k] + tau_o[k] * inv2Gdt);
+    double lambda = 0;
+    if (r.dp) {
+        const orc_dp* dp = r.dp;
+        const double cphi = interp_phase(Nv, NV, dp->cos_phi, ph);
+        const double sphi = interp_phase(Nv, NV, dp->sin_phi, ph);
+        const double spsi = interp_phase(Nv, NV, dp->sin_psi, ph);
+        const double C = interp_phase(Nv, NV, dp->C, ph);
+        const double ereg = interp_phase(Nv, NV, dp->eta_reg, ph);
+        const double Kb = interp_phase(Nv, NV, dp->Kb, ph);
+        // second_invariant, rheology.jl:395-400
+        S tII = dsqrt((tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]) / 2.0 + tau[3] * tau[3] +
+                      tau[4] * tau[4] + tau[5] * tau[5] + kEps * kEps);
+        S tIIs = tII + kEps * kEps;
+        S F = tII - cphi * C - sphi * Pq;
+        const double dFdP = -sphi, dQdP = -spsi;
+        S dQ[6] = {tau[0] / (2.0 * tIIs), tau[1] / (2.0 * tIIs), tau[2] / (2.0 * tIIs),
+                   tau[3] / tIIs, tau[4] / tIIs, tau[5] / tIIs};
+        if (val(F) > 0) {
+            S lam = F / (etave + ereg + Kb * r.dt * dQdP * dFdP);
+            lambda = val(lam);
+            if (lambda > 0)
+                for (int k = 0; k < 6; ++k) tau[k] = tau[k] - 2 * etave * lam * dQ[k];
+        }
+    }
+    return lambda;
+}
+
+// 3-D element momentum residual (momentum_residuals.jl:209-258,
+// accumulate_momentum :28-33, eos_density :11-16). dNdx [nq][NV][3],
+// tau_old [6][nq], T = 0 (no thermal field on this path).
+template <class S>
+void momentum_element3(int NV, int NP, int nq, const double* dNdx, const double* dOm, const double* Nq,
+                       const double* NqP, const S* const v[3], const double* P, const S* Pnum, const int* ph,
+                       const Rheo& r, const double g[3], const double* tau_old, S* const R[3], double* tau_out) {
+    const orc_material* m = r.mat;
+    double beta[ORC_MAX_PHASES];
+    for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < NV; ++a) R[c][a] = S(0.0);
+    for (int q = 0; q < nq; ++q) {
+        const double* G = dNdx + (size_t)q * NV * 3;
+        const double* Nv = Nq + (size_t)q * NV;
+        const double* Np = NqP + (size_t)q * NP;
+        S gv[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                S acc = S(0.0);
+                for (int a = 0; a < NV; ++a) acc = acc + G[a * 3 + j] * v[i][a];
+                gv[i][j] = acc;
+            }
+        double to[6] = {0, 0, 0, 0, 0, 0};
+        if (tau_old)
+            for (int k = 0; k < 6; ++k) to[k] = tau_old[k * nq + q];
+        double Pq = 0;
+        for (int i = 0; i < NP; ++i) Pq += Np[i] * P[i];
+        S tau[6];
+        stress_ip3<S>(gv, Nv, NV, ph, r, to, Pq, tau);
+        if (tau_out)
+            for (int k = 0; k < 6; ++k) tau_out[k * nq + q] = val(tau[k]);
+        S Ptot = S(Pq);
+        if (Pnum) {
+            S pn = S(0.0);
+            for (int i = 0; i < NP; ++i) pn = pn + Np[i] * Pnum[i];
+            Ptot = Ptot + pn;
+        }
+        const double rho = interp_phase(Nv, NV, m->rho0, ph) *
+                           (1 - interp_phase(Nv, NV, m->alpha, ph) * (0.0 - m->Tref) + interp_phase(Nv, NV, beta, ph) * Pq);
+        // τ as a symmetric matrix; column i holds τ_ji
+        const S* tm[3][3] = {{&tau[0], &tau[3], &tau[4]}, {&tau[3], &tau[1], &tau[5]}, {&tau[4], &tau[5], &tau[2]}};
+        for (int i = 0; i < 3; ++i)
+            for (int a = 0; a < NV; ++a) {
+                S acc = S(0.0);
+                for (int j = 0; j < 3; ++j) {
+                    S sji = *tm[j][i];
+                    if (i == j) sji = sji - Ptot;
+                    acc = acc + G[a * 3 + j] * sji;
+                }
+                R[i][a] = R[i][a] + (acc - Nv[a] * (rho * g[i])) * dOm[q];
+            }
+    }
+}
+
+// 3-D element continuity residual (pressure_residual.jl:132-158), T = 0, no Q
+template <class S>
+void pressure_element3(int NV, int NP, int nq, const double* dNdx, const double* dOmP, const double* NqP,
+                       const S* const v[3], const double* P, const double* P0, const int* phP, const double* bulk,
+                       int nphases, double dt, S* RP) {
+    bool all_inf = true;
+    for (int p = 0; p < nphases; ++p) all_inf = all_inf && std::isinf(bulk[p]);
+    for (int i = 0; i < NP; ++i) RP[i] = S(0.0);
+    for (int q = 0; q < nq; ++q) {
+        const double* G = dNdx + (size_t)q * NV * 3;
+        const double* Nv = NqP + (size_t)q * NP;
+        const double etabq = all_inf ? kInf : interp_phase(Nv, NP, bulk, phP);
+        double dP = 0;
+        for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
+        const double dPdt = dP / (etabq * dt);
+        S div = S(0.0);
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < NV; ++a) div = div + G[a * 3 + c] * v[c][a];
+        for (int i = 0; i < NP; ++i) RP[i] = RP[i] + Nv[i] * (((-div) - dPdt) + 0.0 + 0.0) * dOmP[q];
+    }
+}
+
+void elem_dNdx3(const orc_stokes3* s, int64_t e, double dNdx[27][27][3]) {
+    const Tables3& t = tables3();
+    for (int q = 0; q < 27; ++q) {
+        const double* iJ = s->invJ + (e * 27 + q) * 9;
+        for (int a = 0; a < 27; ++a)
+            for (int k = 0; k < 3; ++k)
+                dNdx[q][a][k] = t.dNdxi[q][a][0] * iJ[0 * 3 + k] + t.dNdxi[q][a][1] * iJ[1 * 3 + k] +
+                                t.dNdxi[q][a][2] * iJ[2 * 3 + k];
+    }
+}
+
+const double* pressure_bulk3(const orc_stokes3* s, int finite_K) { return finite_K ? s->mat.K : s->mat.etab; }
+
+void gather3(const orc_stokes3* s, const std::vector<double>& Re, double* out) {
+    // element contributions in ascending element order
+    std::fill(out, out + s->nnodes, 0.0);
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 27; ++a) out[s->el2n[e * 27 + a]] += Re[e * 27 + a];
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_hex27_basis(double xi, double eta, double zeta, double* N, double* dN) {
+    hex27_basis(xi, eta, zeta, N, reinterpret_cast<double(*)[3]>(dN));
+}
+void orc_hex27_rule(double* xi, double* eta, double* zeta, double* w) { hex27_rule(xi, eta, zeta, w); }
+void orc_hex27_tables(double* Nq, double* NqP, double* dNdxi, double* w) {
+    const Tables3& t = tables3();
+    std::memcpy(Nq, t.Nq, sizeof t.Nq);
+    std::memcpy(NqP, t.NqP, sizeof t.NqP);
+    std::memcpy(dNdxi, t.dNdxi, sizeof t.dNdxi);
+    std::memcpy(w, t.w, sizeof t.w);
+}
+
+void orc_hex27_mesh_sizes(int32_t nx, int32_t ny, int32_t nz, int64_t* nnodes, int64_t* nels) {
+    *nnodes = (int64_t)(2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1);
+    *nels = (int64_t)nx * ny * nz;
+}
+
+// generate_coordinates (mesh.jl:340-398) and generate_element2node
+// (connectivity.jl:273-322) for QuadraticElement{3, 27}
+void orc_hex27_mesh(int32_t nx, int32_t ny, int32_t nz, double x0, double x1, double y0, double y1, double z0,
+                    double z1, double* coords, int32_t* el2n) {
+    const int64_t sx = 2 * nx + 1, sy = 2 * ny + 1, sz = 2 * nz + 1;
+    for (int64_t iz = 0; iz < sz; ++iz)
+        for (int64_t iy = 0; iy < sy; ++iy)
+            for (int64_t ix = 0; ix < sx; ++ix) {
+                const int64_t n = (iz * sy + iy) * sx + ix;
+                coords[3 * n + 0] = x0 + (x1 - x0) * (double)ix / (double)(sx - 1);
+                coords[3 * n + 1] = y0 + (y1 - y0) * (double)iy / (double)(sy - 1);
+                coords[3 * n + 2] = z0 + (z1 - z0) * (double)iz / (double)(sz - 1);
+            }
+    auto node = [&](int64_t ix, int64_t iy, int64_t iz) { return (int32_t)(iz * sx * sy + iy * sx + ix); };
+    int64_t e = 0;
+    for (int64_t ez = 0; ez < nz; ++ez)
+        for (int64_t ey = 0; ey < ny; ++ey)
+            for (int64_t ex = 0; ex < nx; ++ex, ++e) {
+                const int64_t ix = 2 * ex, iy = 2 * ey, iz = 2 * ez;
+                for (int a = 0; a < 27; ++a) {
+                    const double* r = kHex27Nodes[a];
+                    el2n[e * 27 + a] = node(ix + 1 + (int64_t)r[0], iy + 1 + (int64_t)r[1], iz + 1 + (int64_t)r[2]);
+                }
+            }
+}
+
+// precompute_geometry (mesh.jl:550-553, 674-682): J = Σ_a x_a ⊗ ∂N_a/∂ξ at each
+// point, stored as J^-1 (row-major, [ξ-index][x-index]) and |det J| w
+void orc_hex27_geometry(int64_t nels, const double* coords, const int32_t* el2n, double* invJ, double* dOmega) {
+    const Tables3& t = tables3();
+    for (int64_t e = 0; e < nels; ++e)
+        for (int q = 0; q < 27; ++q) {
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // J[i][j] = ∂x_i/∂ξ_j
+            for (int a = 0; a < 27; ++a) {
+                const double* x = coords + 3 * (int64_t)el2n[e * 27 + a];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += x[i] * t.dNdxi[q][a][j];
+            }
+            const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            double inv[3][3];   // (J^-1)[j][i]: ∂ξ_j/∂x_i
+            inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+            inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+            inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+            inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) invJ[(e * 27 + q) * 9 + j * 3 + i] = inv[j][i];
+            dOmega[e * 27 + q] = std::fabs(det) * t.w[q];
+        }
+}
+
+void orc_momentum_element3(int32_t NV, int32_t NP, int32_t nq, const double* dNdx, const double* dOm, const double* Nq,
+                           const double* NqP, const double* vx, const double* vy, const double* vz, const double* P,
+                           const double* Pnum, const int32_t* phase, const orc_material* mat, const double* g,
+                           const orc_dp* dp, double dt, const double* tau_old, double* Rx, double* Ry, double* Rz,
+                           double* tau_out) {
+    std::vector<int> ph(NV);
+    for (int a = 0; a < NV; ++a) ph[a] = phase[a] - 1;
+    Rheo r{mat, mat->eta, dp, dt};
+    const double* v[3] = {vx, vy, vz};
+    double* R[3] = {Rx, Ry, Rz};
+    momentum_element3<double>(NV, NP, nq, dNdx, dOm, Nq, NqP, v, P, Pnum, ph.data(), r, g, tau_old, R, tau_out);
+}
+
+// assemble_viscosity_weighted_pressure_scaling! with the Hex27 Jacobi weight
+// N_a² dΩ (pressure_scaling.jl:166-216)
+void orc_pressure_scaling3(orc_stokes3* s, double gamma_fact, double dt, int32_t finite_K) {
+    const Tables3& t = tables3();
+    const double* bulk = pressure_bulk3(s, finite_K);
+    double invK[ORC_MAX_PHASES];
+    for (int p = 0; p < s->mat.nphases; ++p) invK[p] = 1.0 / bulk[p];
+    for (int64_t e = 0; e < s->nels; ++e) {
+        int ph[27];
+        for (int a = 0; a < 27; ++a) ph[a] = s->phases[e] - 1;
+        double MP[4] = {0, 0, 0, 0}, gP[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 27; ++q) {
+            const double dO = s->dOmega[e * 27 + q];
+            const double etaq = interp_phase(t.Nq[q], 27, s->mat.eta, ph);
+            const double gnum = gamma_fact * etaq;
+            const double bq = interp_phase(t.Nq[q], 27, invK, ph);
+            const double gphy = bq == 0 ? gnum : dt / bq;
+            const double gq = gphy * gnum / (gphy + gnum);
+            for (int a = 0; a < 4; ++a) {
+                const double w = t.NqP[q][a] * t.NqP[q][a] * dO;
+                MP[a] += w;
+                gP[a] += w * gq;
+            }
+        }
+        for (int a = 0; a < 4; ++a) {
+            s->MP[e * 4 + a] = MP[a];
+            s->gammaP[e * 4 + a] = MP[a] == 0 ? 0.0 : gP[a] / MP[a];
+        }
+    }
+}
+
+void orc_pressure_residual3(orc_stokes3* s, double dt, int32_t finite_K) {
+    const Tables3& t = tables3();
+    const double* bulk = pressure_bulk3(s, finite_K);
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        double vl[3][27];
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 27; ++a) vl[c][a] = s->v[c][s->el2n[e * 27 + a]];
+        const double* v[3] = {vl[0], vl[1], vl[2]};
+        int ph[4];
+        for (int a = 0; a < 4; ++a) ph[a] = s->phases[e] - 1;
+        double RPe[4];
+        pressure_element3<double>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.NqP[0][0], v, s->P + e * 4,
+                                  s->P0 + e * 4, ph, bulk, s->mat.nphases, dt, RPe);
+        for (int a = 0; a < 4; ++a) s->RP[e * 4 + a] = RPe[a];
+    }
+}
+
+static void momentum_residual3_impl(orc_stokes3* s, double dt, const double* Pnum, double* const* tau_store) {
+    const Tables3& t = tables3();
+    std::vector<double> Re[3];
+    for (int c = 0; c < 3; ++c) Re[c].assign((size_t)s->nels * 27, 0.0);
+    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+    const bool has_old = s->tau_old[0] != nullptr;
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        double vl[3][27];
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 27; ++a) vl[c][a] = s->v[c][s->el2n[e * 27 + a]];
+        const double* v[3] = {vl[0], vl[1], vl[2]};
+        int ph[27];
+        for (int a = 0; a < 27; ++a) ph[a] = s->phases[e] - 1;
+        double to[6 * 27], tout[6 * 27];
+        if (has_old)
+            for (int k = 0; k < 6; ++k)
+                for (int q = 0; q < 27; ++q) to[k * 27 + q] = s->tau_old[k][e * 27 + q];
+        double Rl[3][27];
+        double* R[3] = {Rl[0], Rl[1], Rl[2]};
+        momentum_element3<double>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.Nq[0][0], &t.NqP[0][0], v,
+                                  s->P + e * 4, Pnum ? Pnum + e * 4 : nullptr, ph, r, s->g, has_old ? to : nullptr, R,
+                                  tau_store ? tout : nullptr);
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 27; ++a) Re[c][e * 27 + a] = Rl[c][a];
+        if (tau_store)
+            for (int k = 0; k < 6; ++k)
+                for (int q = 0; q < 27; ++q) tau_store[k][e * 27 + q] = tout[k * 27 + q];
+    }
+    for (int c = 0; c < 3; ++c) gather3(s, Re[c], s->Rv[c]);
+}
+
+void orc_momentum_residual3(orc_stokes3* s, double dt, const double* Pnum) { momentum_residual3_impl(s, dt, Pnum, nullptr); }
+void orc_update_current_stress3(orc_stokes3* s, double dt) { momentum_residual3_impl(s, dt, nullptr, s->tau_store); }
+
+// augmented Jacobian diagnostics in 3-D: as orc_augmented_jacobian, one
+// forward-mode pass per velocity direction with 27 partials
+void orc_augmented_jacobian3(orc_stokes3* s, double dt, int32_t finite_K) {
+    const Tables3& t = tables3();
+    const double* bulk = pressure_bulk3(s, finite_K);
+    std::vector<double> Jr[3], Dg[3];
+    for (int c = 0; c < 3; ++c) { Jr[c].assign((size_t)s->nels * 27, 0.0); Dg[c].assign((size_t)s->nels * 27, 0.0); }
+    typedef Dual<27> D27;
+    const bool has_old = s->tau_old[0] != nullptr;
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        int ph[27], phP[4];
+        for (int a = 0; a < 27; ++a) ph[a] = s->phases[e] - 1;
+        for (int a = 0; a < 4; ++a) phP[a] = s->phases[e] - 1;
+        double eta_pc[ORC_MAX_PHASES];
+        for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = s->mat.eta[ph[0]];   // cell phase: element max
+        Rheo r{&s->mat, eta_pc, s->dp, dt};
+        double to[6 * 27];
+        if (has_old)
+            for (int k = 0; k < 6; ++k)
+                for (int q = 0; q < 27; ++q) to[k * 27 + q] = s->tau_old[k][e * 27 + q];
+        static thread_local double block[3][3][27][27];   // [i][j][row][col]
+        for (int j = 0; j < 3; ++j) {
+            static thread_local D27 u[3][27];
+            for (int c = 0; c < 3; ++c)
+                for (int a = 0; a < 27; ++a) {
+                    u[c][a] = D27(s->v[c][s->el2n[e * 27 + a]]);
+                    if (c == j) u[c][a].d[a] = 1.0;
+                }
+            const D27* uv[3] = {u[0], u[1], u[2]};
+            D27 RPe[4];
+            pressure_element3<D27>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.NqP[0][0], uv, s->P + e * 4,
+                                   s->P0 + e * 4, phP, bulk, s->mat.nphases, dt, RPe);
+            D27 Pn[4];
+            for (int a = 0; a < 4; ++a) Pn[a] = s->gammaP[e * 4 + a] * RPe[a] / s->MP[e * 4 + a];
+            static thread_local D27 Rl[3][27];
+            D27* R[3] = {Rl[0], Rl[1], Rl[2]};
+            momentum_element3<D27>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.Nq[0][0], &t.NqP[0][0], uv,
+                                   s->P + e * 4, Pn, ph, r, s->g, has_old ? to : nullptr, R, nullptr);
+            for (int i = 0; i < 3; ++i)
+                for (int row = 0; row < 27; ++row)
+                    for (int col = 0; col < 27; ++col) block[i][j][row][col] = Rl[i][row].d[col];
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int row = 0; row < 27; ++row) {
+                double sum = 0;
+                for (int j = 0; j < 3; ++j)
+                    for (int col = 0; col < 27; ++col) sum += std::fabs(block[i][j][row][col]);
+                Jr[i][e * 27 + row] = sum;
+                Dg[i][e * 27 + row] = std::fabs(block[i][i][row][row]);
+            }
+    }
+    for (int c = 0; c < 3; ++c) {
+        gather3(s, Jr[c], s->dRdv[c]);
+        gather3(s, Dg[c], s->PC[c]);
+    }
+}
+
+// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802), D = 3
+int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
+    std::memset(out, 0, sizeof *out);
+    g_err.clear();
+    const double dt = p->dt;
+    const int finite_K = p->finite_K;
+    const int64_t ncheck = p->check_interval;
+    const int64_t nv = s->nnodes, np = s->nnodesP;
+    const bool freeze = p->freeze_jacobian < 0 ? (s->dp == nullptr) : (p->freeze_jacobian != 0);
+    if (!p->skip_scaling) orc_pressure_scaling3(s, p->pressure_factor, dt, finite_K);
+    for (int c = 0; c < 3; ++c) {
+        std::fill(s->rate[c], s->rate[c] + nv, 0.0);
+        std::fill(s->Rv0[c], s->Rv0[c] + nv, 0.0);
+    }
+    const char* labels[3] = {"stokes vx", "stokes vy", "stokes vz"};
+    orc_augmented_jacobian3(s, dt, finite_K);
+    double lmax[3];
+    for (int c = 0; c < 3; ++c)
+        if (!checked_lmax(s->dRdv[c], s->PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
+    double lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    int64_t jac_assemblies = 1;
+    double dtau[3], al[3], be[3];
+    for (int c = 0; c < 3; ++c) {
+        dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
+        stokes_cheb(dtau[c], 0.0, p->c_fact, &al[c], &be[c]);
+    }
+    double err_min = kInf, eps_tol = p->tolerance, err = 2 * eps_tol;
+    double err_abs = kInf, err_rel = kInf, err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
+    double err_v = kInf, err_P = kInf;
+    int64_t iter = 0, itPH_done = 0;
+    bool converged = false;
+    double rel_drop = p->rel_drop0;
+    std::vector<double> tmp(np);
+    auto err_pressure = [&]() {
+        for (int64_t i = 0; i < np; ++i) tmp[i] = s->RP[i] / s->MP[i];
+        return nrm2(tmp.data(), np) / std::sqrt((double)np);
+    };
+    auto err_velocity = [&]() {
+        return std::max(nrm2(s->Rv[0], nv), std::max(nrm2(s->Rv[1], nv), nrm2(s->Rv[2], nv))) /
+               (2 * std::sqrt((double)nv));
+    };
+    auto update = [&]() {
+        for (int c = 0; c < 3; ++c)
+            for (int64_t i = 0; i < nv; ++i) {
+                s->rate[c][i] = be[c] * s->rate[c][i] + s->Rv[c][i] / s->PC[c][i];
+                s->v[c][i] += -al[c] * s->rate[c][i];
+            }
+    };
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        orc_pressure_residual3(s, dt, finite_K);
+        orc_momentum_residual3(s, dt, nullptr);
+        for (int c = 0; c < 3; ++c) dirichlet(s->Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+        err_P = err_pressure();
+        err_v = err_velocity();
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v / err_v0, err_P / err_P0);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            g_err = "NaN detected in outer loop at PH=" + std::to_string(itPH); out->status = -4; return -4;
+        }
+        if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in outer loop at PH=" + std::to_string(itPH); out->status = -5; return -5; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e\n", (long long)itPH, (long long)iter,
+                   err, err_abs, err_rel);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT; ++iter;
+            const bool do_check = iter % ncheck == 0;
+            if (do_check) for (int c = 0; c < 3; ++c) std::copy(s->Rv[c], s->Rv[c] + nv, s->Rv0[c]);
+            orc_pressure_residual3(s, dt, finite_K);
+            for (int64_t i = 0; i < np; ++i) s->Pnum[i] = s->gammaP[i] * s->RP[i] / s->MP[i];
+            orc_momentum_residual3(s, dt, s->Pnum);
+            for (int c = 0; c < 3; ++c) dirichlet(s->Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+            for (int c = 0; c < 3; ++c) dirichlet(s->rate[c], s->nbc[c], s->bc_dofs[c], nullptr);
+            update();
+            for (int c = 0; c < 3; ++c) dirichlet(s->v[c], s->nbc[c], s->bc_dofs[c], s->bc_vals[c]);
+            if (do_check) {
+                const double evi = err_velocity();
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { g_err = "NaN detected in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -4; return -4; }
+                if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -5; return -5; }
+                double lmin[3];
+                for (int c = 0; c < 3; ++c) lmin[c] = stokes_lmin(al[c], s->rate[c], s->Rv[c], s->Rv0[c], s->PC[c], nv);
+                if (!freeze) {
+                    ++jac_assemblies;
+                    orc_augmented_jacobian3(s, dt, finite_K);
+                    for (int c = 0; c < 3; ++c)
+                        if (!checked_lmax(s->dRdv[c], s->PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
+                    lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+                }
+                for (int c = 0; c < 3; ++c) {
+                    dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
+                    stokes_cheb(dtau[c], lmin[c], p->c_fact, &al[c], &be[c]);
+                }
+            }
+            if (iter > p->max_iterations) break;
+        }
+        for (int64_t i = 0; i < np; ++i) s->P[i] += s->gammaP[i] * s->RP[i] / s->MP[i];
+        if (iter > p->max_iterations) break;
+    }
+    out->itPH = itPH_done;
+    out->iter = iter;
+    out->err = err;
+    out->err_abs = err_abs;
+    out->err_rel = err_rel;
+    out->err_v = err_v;
+    out->err_P = err_P;
+    out->converged = converged;
+    out->reached_total_iter = iter > p->max_iterations;
+    out->lambda_max = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    out->lambda_max_gershgorin = lmax_g;
+    out->jacobian_assemblies = jac_assemblies;
+    out->status = 0;
+    return 0;
+}
+
+}  // extern "C"

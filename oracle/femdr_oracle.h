@@ -167,6 +167,53 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out);
 /* update_stokes_current_stress! (DR.jl:820-897): tau_store must be set */
 void orc_update_current_stress(orc_stokes* s, double dt);
 
+/* --- 3-D: Hex27 velocity, P1-disc pressure with four modes per cell --- */
+/* Whole 3-D Stokes problem. Cell phases (1-based, [nels]) serve velocity and
+ * pressure; no temperature, source or fluid-pressure fields (T = 0). Pressure
+ * DoF a of element e is 4 e + a; stress components (xx, yy, zz, xy, xz, yz). */
+typedef struct {
+    int64_t nnodes, nels, nnodesP;
+    const double* coords;   /* [nnodes][3] */
+    const int32_t* el2n;    /* [nels][27], 0-based */
+    const double* invJ;     /* [nels][27][9], (J^-1)[ξ-index][x-index] */
+    const double* dOmega;   /* [nels][27] |det J| w */
+    const int32_t* phases;  /* [nels] */
+    orc_material mat;       /* g and Tref of mat are not used: g below */
+    double g[3];
+    const orc_dp* dp;       /* nullable */
+    double *v[3], *rate[3], *Rv[3], *Rv0[3], *dRdv[3], *PC[3];
+    double *P, *P0, *gammaP, *RP, *MP, *Pnum;
+    const double* tau_old[6];   /* nullable (all or none): [nels][27] */
+    double* tau_store[6];       /* for orc_update_current_stress3 */
+    int64_t nbc[3];
+    const int32_t* bc_dofs[3];
+    const double* bc_vals[3];
+} orc_stokes3;
+
+/* reference Hex27 (shape_functions.jl:411-446, integration_points.jl:321-337) */
+void orc_hex27_basis(double xi, double eta, double zeta, double* N /*27*/, double* dN /*27x3*/);
+void orc_hex27_rule(double* xi, double* eta, double* zeta, double* w /*27 each*/);
+void orc_hex27_tables(double* Nq /*27x27*/, double* NqP /*27x4*/, double* dNdxi /*27x27x3*/, double* w /*27*/);
+/* structured Hex27 mesh (mesh.jl:340-398, connectivity.jl:273-322) */
+void orc_hex27_mesh_sizes(int32_t nx, int32_t ny, int32_t nz, int64_t* nnodes, int64_t* nels);
+void orc_hex27_mesh(int32_t nx, int32_t ny, int32_t nz, double x0, double x1, double y0, double y1, double z0,
+                    double z1, double* coords, int32_t* el2n);
+void orc_hex27_geometry(int64_t nels, const double* coords, const int32_t* el2n, double* invJ, double* dOmega);
+/* element momentum residual on caller tables (momentum_residuals.jl:209-258):
+ * dNdx [nq][NV][3], Nq [nq][NV], NqP [nq][NP], phase [NV] 1-based, g [3],
+ * tau_old [6][nq] nullable, tau_out [6][nq] nullable */
+void orc_momentum_element3(int32_t NV, int32_t NP, int32_t nq, const double* dNdx, const double* dOm, const double* Nq,
+                           const double* NqP, const double* vx, const double* vy, const double* vz, const double* P,
+                           const double* Pnum, const int32_t* phase, const orc_material* mat, const double* g,
+                           const orc_dp* dp, double dt, const double* tau_old, double* Rx, double* Ry, double* Rz,
+                           double* tau_out);
+void orc_pressure_scaling3(orc_stokes3* s, double gamma_fact, double dt, int32_t finite_K);
+void orc_pressure_residual3(orc_stokes3* s, double dt, int32_t finite_K);
+void orc_momentum_residual3(orc_stokes3* s, double dt, const double* Pnum /*nullable*/);
+void orc_augmented_jacobian3(orc_stokes3* s, double dt, int32_t finite_K);
+int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out);
+void orc_update_current_stress3(orc_stokes3* s, double dt);
+
 /* --- thermal diffusion (src/heat_diffusion/, src/dynamic_relaxation/solver.jl) --- */
 
 /* ThermalMaterial (src/heat_diffusion/types/heat_diffusion_types.jl:15-41) */

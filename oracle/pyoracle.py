@@ -96,7 +96,32 @@ class ThermalStatsC(C.Structure):
     _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("status", I32)]
 
 
+class Stokes3(C.Structure):
+    _fields_ = [("nnodes", I64), ("nels", I64), ("nnodesP", I64), ("coords", PD), ("el2n", PI), ("invJ", PD),
+                ("dOmega", PD), ("phases", PI), ("mat", Material), ("g", D * 3), ("dp", C.POINTER(DP)),
+                ("v", PD * 3), ("rate", PD * 3), ("Rv", PD * 3), ("Rv0", PD * 3), ("dRdv", PD * 3), ("PC", PD * 3),
+                ("P", PD), ("P0", PD), ("gammaP", PD), ("RP", PD), ("MP", PD), ("Pnum", PD),
+                ("tau_old", PD * 6), ("tau_store", PD * 6), ("nbc", I64 * 3), ("bc_dofs", PI * 3),
+                ("bc_vals", PD * 3)]
+
+
 def _declare(L):
+    L.orc_hex27_basis.argtypes = [D, D, D, PD, PD]
+    L.orc_hex27_rule.argtypes = [PD, PD, PD, PD]
+    L.orc_hex27_tables.argtypes = [PD, PD, PD, PD]
+    L.orc_hex27_mesh_sizes.argtypes = [I32, I32, I32, C.POINTER(I64), C.POINTER(I64)]
+    L.orc_hex27_mesh.argtypes = [I32, I32, I32, D, D, D, D, D, D, PD, PI]
+    L.orc_hex27_geometry.argtypes = [I64, PD, PI, PD, PD]
+    L.orc_momentum_element3.argtypes = [I32, I32, I32, PD, PD, PD, PD, PD, PD, PD, PD, PD, PI, C.POINTER(Material),
+                                        PD, C.POINTER(DP), D, PD, PD, PD, PD, PD]
+    for f in ("orc_pressure_scaling3", "orc_augmented_jacobian3"):
+        getattr(L, f).argtypes = [C.POINTER(Stokes3), D, D, I32] if f == "orc_pressure_scaling3" \
+            else [C.POINTER(Stokes3), D, I32]
+    L.orc_pressure_residual3.argtypes = [C.POINTER(Stokes3), D, I32]
+    L.orc_momentum_residual3.argtypes = [C.POINTER(Stokes3), D, PD]
+    L.orc_stokes_solve3.argtypes = [C.POINTER(Stokes3), C.POINTER(Params), C.POINTER(Stats)]
+    L.orc_stokes_solve3.restype = C.c_int
+    L.orc_update_current_stress3.argtypes = [C.POINTER(Stokes3), D]
     L.orc_thermal_integrate.argtypes = [I32, I32, PD, PD, PD, PD, PD, PD, PI, C.POINTER(ThermalMaterialC), PD, D, D,
                                         PD, PD]
     L.orc_thermal_residual.argtypes = [C.POINTER(ThermalC), D, D, I32]
@@ -414,3 +439,122 @@ class ThermalProblem:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual}
+
+
+# ---------------------------------------------------------------------------
+# 3-D: Hex27 / P1-disc (4 modes)
+def hex27_basis(xi, eta, zeta):
+    N = np.zeros(27)
+    dN = np.zeros((27, 3))
+    lib().orc_hex27_basis(xi, eta, zeta, ptr(N), ptr(dN))
+    return N, dN
+
+
+def hex27_rule():
+    xi, eta, zeta, w = (np.zeros(27) for _ in range(4))
+    lib().orc_hex27_rule(ptr(xi), ptr(eta), ptr(zeta), ptr(w))
+    return xi, eta, zeta, w
+
+
+def hex27_tables():
+    Nq, NqP, dN, w = np.zeros((27, 27)), np.zeros((27, 4)), np.zeros((27, 27, 3)), np.zeros(27)
+    lib().orc_hex27_tables(ptr(Nq), ptr(NqP), ptr(dN), ptr(w))
+    return Nq, NqP, dN, w
+
+
+def hex27_mesh(nx, ny, nz, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+    nn, ne = I64(), I64()
+    lib().orc_hex27_mesh_sizes(nx, ny, nz, C.byref(nn), C.byref(ne))
+    coords = np.zeros((nn.value, 3))
+    el2n = np.zeros((ne.value, 27), np.int32)
+    lib().orc_hex27_mesh(nx, ny, nz, *box, ptr(coords), ptr(el2n, PI))
+    return coords, el2n
+
+
+def hex27_geometry(coords, el2n):
+    ne = len(el2n)
+    invJ = np.zeros((ne, 27, 9))
+    dOm = np.zeros((ne, 27))
+    lib().orc_hex27_geometry(ne, ptr(f64(coords)), ptr(i32(el2n), PI), ptr(invJ), ptr(dOm))
+    return invJ, dOm
+
+
+def box_boundary_nodes3(coords, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tol=1e-12):
+    on = np.zeros(len(coords), bool)
+    for d in range(3):
+        on |= np.abs(coords[:, d] - box[2 * d]) <= tol
+        on |= np.abs(coords[:, d] - box[2 * d + 1]) <= tol
+    return np.nonzero(on)[0].astype(np.int32)
+
+
+class Problem3:
+    """Host arrays of one 3-D Hex27/P1-disc Stokes problem (cell phases)."""
+
+    def __init__(self, coords, el2n, mat, g=(0.0, 0.0, 0.0), dp=None, phases=None, tau_old=None, bc=None):
+        self.coords, self.el2n = f64(coords), i32(el2n)
+        nn, ne = len(self.coords), len(self.el2n)
+        self.nnodes, self.nels, self.nnodesP = nn, ne, 4 * ne
+        self.invJ, self.dOmega = hex27_geometry(self.coords, self.el2n)
+        self.phases = i32(np.ones(ne) if phases is None else phases)
+        self.mat, self.g, self.dp = mat, tuple(g), dp
+        for k in ("v", "rate", "Rv", "Rv0", "dRdv", "PC"):
+            setattr(self, k, [np.zeros(nn) for _ in range(3)])
+        for k in ("P", "P0", "gammaP", "RP", "MP", "Pnum"):
+            setattr(self, k, np.zeros(4 * ne))
+        self.tau_old = None if tau_old is None else [f64(t) for t in tau_old]
+        self.tau_store = [np.zeros((ne, 27)) for _ in range(6)]
+        e = (np.zeros(0, np.int32), np.zeros(0))
+        self.bc = tuple((i32(d), f64(v)) for d, v in (bc if bc is not None else (e, e, e)))
+
+    def struct(self):
+        s = Stokes3()
+        s.nnodes, s.nels, s.nnodesP = self.nnodes, self.nels, self.nnodesP
+        s.coords, s.el2n = ptr(self.coords), ptr(self.el2n, PI)
+        s.invJ, s.dOmega, s.phases = ptr(self.invJ), ptr(self.dOmega), ptr(self.phases, PI)
+        s.mat = self.mat
+        for i in range(3):
+            s.g[i] = self.g[i]
+        s.dp = C.pointer(self.dp) if self.dp is not None else C.cast(None, C.POINTER(DP))
+        for k in ("v", "rate", "Rv", "Rv0", "dRdv", "PC"):
+            arr = getattr(s, k)
+            for c in range(3):
+                arr[c] = ptr(getattr(self, k)[c])
+        for k in ("P", "P0", "gammaP", "RP", "MP", "Pnum"):
+            setattr(s, k, ptr(getattr(self, k)))
+        for k in range(6):
+            s.tau_old[k] = ptr(self.tau_old[k]) if self.tau_old is not None else ptr(None)
+            s.tau_store[k] = ptr(self.tau_store[k])
+        for c in range(3):
+            s.nbc[c] = len(self.bc[c][0])
+            s.bc_dofs[c], s.bc_vals[c] = ptr(self.bc[c][0], PI), ptr(self.bc[c][1])
+        self._struct = s
+        return s
+
+    def pressure_scaling(self, gamma_fact, dt, finite_K=False):
+        lib().orc_pressure_scaling3(C.byref(self.struct()), gamma_fact, dt, int(finite_K))
+
+    def pressure_residual(self, dt, finite_K=False):
+        lib().orc_pressure_residual3(C.byref(self.struct()), dt, int(finite_K))
+
+    def momentum_residual(self, dt, Pnum=None):
+        lib().orc_momentum_residual3(C.byref(self.struct()), dt, ptr(f64(Pnum)) if Pnum is not None else ptr(None))
+
+    def augmented_jacobian(self, dt, finite_K=False):
+        lib().orc_augmented_jacobian3(C.byref(self.struct()), dt, int(finite_K))
+
+    def update_current_stress(self, dt):
+        lib().orc_update_current_stress3(C.byref(self.struct()), dt)
+        return self.tau_store
+
+    def solve(self, dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
+              check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
+              finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False):
+        p = Params(dt=dt, pressure_factor=pressure_factor, tolerance=tolerance, max_iterations=max_iterations,
+                   iter_max=iter_max, check_interval=check_interval, max_ph_iterations=max_ph_iterations,
+                   rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
+                   freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
+        st = Stats()
+        rc = lib().orc_stokes_solve3(C.byref(self.struct()), C.byref(p), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return st.as_dict()
