@@ -8,44 +8,6 @@
 // scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
 #include "femdr_host.hpp"
 
-struct fem_stokes {
-    fem_ctx* ctx = nullptr;
-    fem_mesh* mesh = nullptr;
-    fem_material mat;
-    fem_state_opts opts;
-    double* f[FEM_FIELD_COUNT] = {};
-    int64_t len[FEM_FIELD_COUNT] = {};
-    int32_t* d_phase_v = nullptr;   // cell [nels] or element-node [7][nels], 0-based
-    int32_t* d_phase_P = nullptr;   // cell [nels] or element-node [3][nels]
-    int32_t layout_v = FEM_PHASE_CELL, layout_P = FEM_PHASE_CELL;
-    bool phase_same = true;         // cell phases of v and P identical
-    double* d_pscale = nullptr;
-    double2* d_Re = nullptr;
-    double2* d_Jrow = nullptr;
-    double2* d_Jdiag = nullptr;
-    uint8_t* d_bcflag = nullptr;
-    double* d_bcval_x = nullptr;
-    double* d_bcval_y = nullptr;
-    int32_t* d_bc_stage = nullptr;
-    double* d_bc_stage_v = nullptr;
-    int64_t bc_stage_cap = 0;
-    double* d_ptab = nullptr;
-    int32_t* d_flag = nullptr;      // [0] scratch yes/no flag, [1] yielding-element count
-    uint8_t* d_yflag = nullptr;     // near-yield flag per element (mixed-precision solve)
-    bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
-    bool q_nonzero = false;         // dr.Q holds a nonzero entry
-    // FP32 work arrays of the mixed-precision solve (allocated by its first call):
-    // the correction (δv, δP), its DR rates/residuals, the right-hand sides R(x)
-    // and FP32 copies of the scaling, preconditioner and geometry
-    struct W32 {
-        float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
-        float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
-        float2* Re;
-        // stress-history copy for the correction operator with the return map
-        float *tau_old0, *tau_old1, *tau_old2;
-    };
-    W32* w32 = nullptr;
-};
 
 // ---------------------------------------------------------------------------
 // reference-element tables (shape_functions.jl:115-171, 231-239;
@@ -341,7 +303,7 @@ int fem_mesh_t7_upload(fem_ctx* c, const double* host_coords, int64_t nnodes, co
 
 int fem_mesh_get_info(const fem_mesh* m, fem_mesh_info* o) {
     if (!m || !o) return FEM_EARG;
-    o->dim = 2; o->nv_local = 7; o->np_local = 3; o->nq = 7;
+    o->dim = m->dim; o->nv_local = m->nloc; o->np_local = m->dim == 3 ? 4 : 3; o->nq = m->nloc;
     o->nnodes = m->nnodes; o->nels = m->nels; o->nnodesP = m->nnodesP; o->n_shared = m->ns;
     o->n_boundary = (int64_t)m->h_boundary.size();
     return FEM_OK;
@@ -377,16 +339,23 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
     *out = nullptr;
     if (mat->nphases < 1 || mat->nphases > FEM_MAX_PHASES)
         return set_err(c, FEM_EARG, "nphases must be in 1..%d", FEM_MAX_PHASES);
-    if (mat->ndim != 2) return set_err(c, FEM_EDIM, "material gravity must have 2 components");
+    if (mat->ndim != m->dim)
+        return set_err(c, FEM_EDIM, "material gravity must have %d components for a %d-D mesh", m->dim, m->dim);
     HIPCHK(c, hipSetDevice(c->device));
     fem_stokes* s = new fem_stokes();
     s->ctx = c;
     s->mesh = m;
+    s->dim = m->dim;
     s->mat = *mat;
     s->opts = opts ? *opts : fem_state_opts{0.98, 0.98, 0.9, 1e-6};
-    const int64_t nv = m->nnodes, np = m->nnodesP, nip = m->nels * NQ;
+    const int64_t nv = m->nnodes, np = m->nnodesP, nip = m->nels * m->nloc;
     for (int k = 0; k < FEM_FIELD_COUNT; ++k) {
-        int64_t n = k <= FEM_FIELD_PC_V_Y ? nv : (k <= FEM_FIELD_TAU_OLD_XY ? nip : np);
+        int64_t n;
+        if (k <= FEM_FIELD_PC_V_Y) n = nv;
+        else if (k <= FEM_FIELD_TAU_OLD_XY) n = nip;
+        else if (k <= FEM_FIELD_PNUM) n = np;
+        else if (k <= FEM_FIELD_PC_V_Z) n = m->dim == 3 ? nv : 0;
+        else n = m->dim == 3 ? nip : 0;
         s->len[k] = n;
         // +16 entries: 16-byte global->LDS loads of the last block may read
         // 8 bytes past the final element of an integration-point array
@@ -401,6 +370,11 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
         (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH)) ||
         (r = dalloc_t(c, &s->d_flag, 2))) {
+        fem_stokes_destroy(s);
+        return r;
+    }
+    if (m->dim == 3 && ((r = dalloc_t(c, &s->d_bcval_z, nv)) || (r = dalloc_t(c, &s->d_slot3, 26 * 3 * m->nels)) ||
+                        (r = dalloc_t(c, &s->d_jslot3, 26 * 6 * m->nels)))) {
         fem_stokes_destroy(s);
         return r;
     }
@@ -421,6 +395,7 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
     hipFree(s->d_flag);
     hipFree(s->d_yflag);
+    hipFree(s->d_bcval_z); hipFree(s->d_slot3); hipFree(s->d_jslot3);
     if (s->w32) {
         float** p = (float**)s->w32;
         for (size_t k = 0; k < sizeof(fem_stokes::W32) / sizeof(float*); ++k) hipFree(p[k]);
@@ -443,6 +418,8 @@ int fem_stokes_set_phases(fem_stokes* s, int32_t which, int32_t layout, const in
     fem_ctx* c = s->ctx;
     const fem_mesh* m = s->mesh;
     const int64_t ne = m->nels;
+    if (m->dim == 3 && layout != FEM_PHASE_CELL)
+        return set_err(c, FEM_EARG, "the 3-D path takes one phase per cell (a 1 x nels phase matrix)");
     const int nloc = which == 0 ? 7 : 3;
     std::vector<int32_t> loc((size_t)nloc * ne);   // [nloc][ne] 0-based
     if (layout == FEM_PHASE_CELL) {
@@ -486,7 +463,7 @@ int fem_stokes_set_phases(fem_stokes* s, int32_t which, int32_t layout, const in
 // ---------------------------------------------------------------------------
 // per-phase table, boundary conditions, kernel launches
 // ---------------------------------------------------------------------------
-static int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
+int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
                        bool* thermal, bool* eos_p, bool* elastic) {
     fem_ctx* c = s->ctx;
     const fem_material& m = s->mat;
@@ -535,11 +512,12 @@ __global__ void k_set_bc(int64_t n, const int32_t* dofs, const double* vals, uin
     }
 }
 
-static int upload_bcs(fem_stokes* s, const fem_bc bcs[2]) {
+int upload_bcs(fem_stokes* s, const fem_bc* bcs) {
     fem_ctx* c = s->ctx;
     const int64_t nv = s->mesh->nnodes;
     HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, nv, c->stream));
-    for (int comp = 0; comp < 2; ++comp) {
+    double* bcval[3] = {s->d_bcval_x, s->d_bcval_y, s->d_bcval_z};
+    for (int comp = 0; comp < s->dim; ++comp) {
         const fem_bc& b = bcs[comp];
         if (b.n == 0) continue;
         if (b.n < 0 || !b.host_dofs_1based || !b.host_vals) return set_err(c, FEM_EARG, "invalid boundary condition");
@@ -569,7 +547,7 @@ static int upload_bcs(fem_stokes* s, const fem_bc bcs[2]) {
         HIPCHK(c, hipMemcpyAsync(s->d_bc_stage, d.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(s->d_bc_stage_v, v.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_set_bc, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, nb, s->d_bc_stage,
-                           s->d_bc_stage_v, (uint8_t)(1 << comp), s->d_bcflag, comp == 0 ? s->d_bcval_x : s->d_bcval_y);
+                           s->d_bc_stage_v, (uint8_t)(1 << comp), s->d_bcflag, bcval[comp]);
         CHK(check_launch(c, "k_set_bc"));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
@@ -940,19 +918,6 @@ static int checked_lmax(fem_stokes* s, double lmax[2]) {
                            "%s preconditioner produced invalid λmax = %g; check for zero or non-finite preconditioner entries",
                            labels[k], lmax[k]);
     return FEM_OK;
-}
-
-static double julia_eps(double x) {
-    const double a = std::fabs(x);
-    if (!std::isfinite(a)) return std::numeric_limits<double>::quiet_NaN();
-    return std::nextafter(a, std::numeric_limits<double>::infinity()) - a;
-}
-
-static void cheb(double dtau, double lmin, double c_fact, double* a, double* b) {
-    // _stokes_cheb, DR.jl:39-42
-    const double c = 2 * std::sqrt(lmin) * c_fact;
-    *a = 2 * dtau * dtau / (2 + c * dtau);
-    *b = (2 - c * dtau) / (2 + c * dtau);
 }
 
 static int any_nonzero(fem_stokes* s, const double* x, int64_t n, bool* out) {
@@ -1437,6 +1402,10 @@ extern "C" {
 int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt, int32_t finite_K) {
     if (!s) return FEM_EARG;
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (s->dim == 3) {
+        CHK(hex27_pressure_scaling(s, pressure_factor, dt, finite_K));
+        return fem_sync(s->ctx);
+    }
     LaunchOpts o;
     CHK(prepare(s, nullptr, dt, finite_K, o, true));
     CHK(scaling_impl(s, pressure_factor, dt));
@@ -1447,6 +1416,10 @@ int fem_stokes_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, 
     if (!s) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim == 3) {
+        CHK(hex27_residual(s, dp, dt, with_pnum != 0, finite_K));
+        return fem_sync(c);
+    }
     LaunchOpts o;
     CHK(prepare(s, dp, dt, finite_K, o, true));
     hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
@@ -1465,6 +1438,10 @@ int fem_stokes_jacobian_diagnostics(fem_stokes* s, const fem_drucker_prager* dp,
     if (!s) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim == 3) {
+        CHK(hex27_jacobian(s, dp, dt, finite_K));
+        return fem_sync(c);
+    }
     LaunchOpts o;
     CHK(prepare(s, dp, dt, finite_K, o, true));
     hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
@@ -1477,6 +1454,10 @@ int fem_stokes_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp
     if (!s) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim == 3) {
+        CHK(hex27_update_current_stress(s, dp, dt));
+        return fem_sync(c);
+    }
     LaunchOpts o;
     CHK(prepare(s, dp, dt, 0, o, true));
     HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
@@ -1491,6 +1472,7 @@ int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, 
     if (!s || n <= 0 || !avg_ms) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim == 3) return hex27_time_element(s, dp, dt, n, avg_ms);
     LaunchOpts o;
     CHK(prepare(s, dp, dt, 0, o, true));
     o.use_pnum = true;
@@ -1519,6 +1501,7 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
     if (p->precision != 0 && p->precision != 1)
         return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
                        p->precision);
+    if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
     const int64_t nv = m->nnodes, np = m->nnodesP;

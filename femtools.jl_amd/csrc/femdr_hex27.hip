@@ -1,0 +1,1155 @@
+// femdr_hex27.hip -- the 3-D path of the Stokes DR solve on structured Hex27
+// velocity / P1-disc pressure meshes (QuadraticElement{3,27} with the four
+// modes (1-ξ-η-ζ, ξ, η, ζ) per cell; src/mesh/mixed_mesh.jl:358-417).
+//
+// Work layout: each 64-lane wave serves two elements, 32 lanes each, 27 of
+// them active. A kernel runs in phases separated by block barriers, with the
+// element's data in LDS: lanes load the 27 nodal velocities (lane = node),
+// evaluate strain rate, continuity, stress and return map at the 27 Gauss
+// points (lane = point; the continuity residual is a lane reduction), and
+// finally assemble the nodal residuals (lane = node) from the point fluxes.
+// Shape functions are tensor products of the 1-D quadratic basis, so a lane
+// holds the 1-D values at its point or node and forms each 3-D value with two
+// multiplications. Structured boxes have one affine map: ∂ξ/∂x =
+// diag(2/hx, 2/hy, 2/hz), |det J| = hx hy hz / 8. As in 2-D the 26 shared
+// nodes of an element are summed by a gather over a node -> element-slot list
+// in ascending element order; the centre node is private and is updated by
+// its element.
+#include "femdr_host.hpp"
+
+namespace {
+
+constexpr int HNV = 27, HNQ = 27, HNP = 4, HNSH = 26;
+constexpr int HBLOCK = 256, HEPB = HBLOCK / 32;   // elements per block
+
+// QuadraticElement{3, 27} node order (src/elements/shape_functions.jl:411-446):
+// 1-D node index per axis (0: -1, 1: 0, 2: +1)
+constexpr int8_t kHexR[HNV][3] = {
+    {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2},
+    {1, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {1, 0, 2}, {2, 1, 2}, {1, 2, 2}, {0, 1, 2},
+    {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1},
+    {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
+
+// 1-D quadratic Lagrange basis at the 3-point Gauss rule
+// (shape_functions.jl:173-196, integration_points.jl:321-337), filled on the
+// host: L[node][point], dL[node][point], x[point], w[point]
+struct Hex1D {
+    double L[3][3], dL[3][3], x[3], w[3];
+};
+Hex1D make_hex1d() {
+    Hex1D t;
+    const double a = std::sqrt(3.0 / 5.0);
+    const double pts[3] = {-a, 0.0, a}, ws[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int p = 0; p < 3; ++p) {
+        const double x = pts[p];
+        t.L[0][p] = x * (x - 1) * 0.5; t.L[1][p] = 1 - x * x; t.L[2][p] = x * (x + 1) * 0.5;
+        t.dL[0][p] = x - 0.5; t.dL[1][p] = -2 * x; t.dL[2][p] = x + 0.5;
+        t.x[p] = x;
+        t.w[p] = ws[p];
+    }
+    return t;
+}
+
+struct HexGeo {
+    int64_t nels;
+    int32_t nx, ny, nz;
+    int64_t sx, sxy;                 // refined-grid strides
+    double jac[3], det;              // ∂ξ/∂x per axis, |det J|
+    Hex1D t;
+};
+
+HexGeo hex_geo(const fem_mesh* m) {
+    HexGeo g;
+    g.nels = m->nels;
+    g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
+    g.sx = 2 * (int64_t)m->nx + 1;
+    g.sxy = g.sx * (2 * (int64_t)m->ny + 1);
+    const double hx = (m->box[1] - m->box[0]) / m->nx, hy = (m->box[3] - m->box[2]) / m->ny,
+                 hz = (m->box[5] - m->box[4]) / m->nz;
+    g.jac[0] = 2 / hx; g.jac[1] = 2 / hy; g.jac[2] = 2 / hz;
+    g.det = (hx / 2) * (hy / 2) * (hz / 2);
+    g.t = make_hex1d();
+    return g;
+}
+
+// arguments of the residual / DR-step kernel (FP64 throughout)
+struct Hex27Args {
+    HexGeo g;
+    const double *vx, *vy, *vz;            // written for the centre node when do_update
+    const double *P, *Pold, *pscale;       // [nels][4]
+    const double* tau_old[6];              // [6][nels * 27]
+    double* tau_out[6];                    // nullable
+    const int32_t* phase;                  // cell phases, 0-based
+    const double* ptab;
+    double gx, gy, gz, Tref, dt;
+    double* slot;                          // [nels][26][3]
+    double *rate_x, *rate_y, *rate_z;
+    const double *PCx, *PCy, *PCz;
+    const uint8_t* bcflag;
+    const double *bcval_x, *bcval_y, *bcval_z;
+    double ax, ay, az, bx, by, bz;
+    int32_t do_update;
+    double *RP_out, *Pnum_out;             // nullable
+    double *Rv_x, *Rv_y, *Rv_z;            // nullable
+    double *Rv0_x, *Rv0_y, *Rv0_z;         // nullable
+};
+
+__device__ inline int64_t hex_node(const HexGeo& g, int64_t e, int a) {
+    const int64_t ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / ((int64_t)g.nx * g.ny);
+    return (2 * ez + kHexR[a][2]) * g.sxy + (2 * ey + kHexR[a][1]) * g.sx + (2 * ex + kHexR[a][0]);
+}
+
+// forward-mode dual with six partials (the strain-rate inputs of stress3d)
+struct D6 {
+    double v, d[6];
+    __device__ D6() : v(0) { for (int i = 0; i < 6; ++i) d[i] = 0; }
+    __device__ D6(double x) : v(x) { for (int i = 0; i < 6; ++i) d[i] = 0; }
+};
+__device__ inline D6 operator+(D6 a, const D6& b) { a.v += b.v; for (int i = 0; i < 6; ++i) a.d[i] += b.d[i]; return a; }
+__device__ inline D6 operator-(D6 a, const D6& b) { a.v -= b.v; for (int i = 0; i < 6; ++i) a.d[i] -= b.d[i]; return a; }
+__device__ inline D6 operator*(const D6& a, const D6& b) {
+    D6 r;
+    r.v = a.v * b.v;
+    for (int i = 0; i < 6; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+}
+__device__ inline D6 operator+(D6 a, double b) { a.v += b; return a; }
+__device__ inline D6 operator-(D6 a, double b) { a.v -= b; return a; }
+__device__ inline D6 operator*(D6 a, double b) { a.v *= b; for (int i = 0; i < 6; ++i) a.d[i] *= b; return a; }
+__device__ inline D6 operator*(double b, D6 a) { return a * b; }
+__device__ inline D6 dsqrt(const D6& a) {
+    D6 r;
+    r.v = sqrt(a.v);
+    const double h = 0.5 / r.v;
+    for (int i = 0; i < 6; ++i) r.d[i] = a.d[i] * h;
+    return r;
+}
+__device__ inline D6 drcp(const D6& a) {
+    D6 r;
+    r.v = 1.0 / a.v;
+    const double h = -r.v * r.v;
+    for (int i = 0; i < 6; ++i) r.d[i] = a.d[i] * h;
+    return r;
+}
+__device__ inline double val(const D6& a) { return a.v; }
+using fem::dsqrt;
+using fem::drcp;
+using fem::val;
+
+// 3-D deviatoric stress with the Drucker-Prager return (rheology.jl:413-472);
+// in = (∂vx/∂x, ∂vy/∂y, ∂vz/∂z, ε̇xy, ε̇xz, ε̇yz); tau = (xx, yy, zz, xy, xz, yz)
+template <class S, bool PLASTIC>
+__device__ inline void stress3d(const S in[6], const IpMat<double>& m, const double to[6], double Pq, S tau[6]) {
+    const S tr = (in[0] + in[1] + in[2]) * (1.0 / 3.0);
+    const S e[6] = {in[0] - tr, in[1] - tr, in[2] - tr, in[3], in[4], in[5]};
+    const double two_eve = 2 * m.etave;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau[k] = two_eve * (e[k] + to[k] * m.inv2Gdt);
+    if (PLASTIC) {
+        const S tII = dsqrt((tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]) * 0.5 + tau[3] * tau[3] +
+                            tau[4] * tau[4] + tau[5] * tau[5] + eps2<double>());
+        const S F_ = (tII - m.yield0) - m.sphi * Pq;
+        if (val(F_) > 0) {
+            const S lam = F_ * m.invden;
+            if (val(lam) > 0) {
+                // ∂Q/∂τ = (τ_nn / (2 τII_s), τ_shear / τII_s)
+                const S tIIs = tII + eps2<double>();
+                const S h = two_eve * lam;
+                const S r2 = drcp(tIIs * 2.0), r1 = drcp(tIIs);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) tau[k] = tau[k] - h * (tau[k] * r2);
+#pragma unroll
+                for (int k = 3; k < 6; ++k) tau[k] = tau[k] - h * (tau[k] * r1);
+            }
+        }
+    }
+}
+
+__device__ inline IpMat<double> hex_ipmat(const double* ptab, int p) {
+    IpMat<double> m;
+    m.etave = ptab[PC_ETAVE * MAXPH + p];
+    m.inv2Gdt = ptab[PC_INV2GDT * MAXPH + p];
+    m.yield0 = ptab[PC_YIELD0 * MAXPH + p];
+    m.sphi = ptab[PC_SINPHI * MAXPH + p];
+    m.invden = ptab[PC_INVDEN * MAXPH + p];
+    return m;
+}
+
+// sum over the 32 lanes of a lane group
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+
+// 1-D basis values at a point (lane = point) or of a node (lane = node)
+struct Lane1D {
+    double L0[3], L1[3], L2[3], d0[3], d1[3], d2[3];
+};
+// lane = point q: L0[r] = L_r(ξ_q) etc.
+__device__ inline Lane1D point_1d(const Hex1D& t, int q) {
+    Lane1D b;
+    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        b.L0[r] = t.L[r][qi]; b.L1[r] = t.L[r][qj]; b.L2[r] = t.L[r][qk];
+        b.d0[r] = t.dL[r][qi]; b.d1[r] = t.dL[r][qj]; b.d2[r] = t.dL[r][qk];
+    }
+    return b;
+}
+// lane = node a: L0[p] = L_{r0(a)}(x_p) etc.
+__device__ inline Lane1D node_1d(const Hex1D& t, int a) {
+    Lane1D b;
+    const int r0 = kHexR[a][0], r1 = kHexR[a][1], r2 = kHexR[a][2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        b.L0[p] = t.L[r0][p]; b.L1[p] = t.L[r1][p]; b.L2[p] = t.L[r2][p];
+        b.d0[p] = t.dL[r0][p]; b.d1[p] = t.dL[r1][p]; b.d2[p] = t.dL[r2][p];
+    }
+    return b;
+}
+
+// velocity gradient at the lane's point: gv[i][k] = ∂v_i/∂x_k (sv: [27][3])
+__device__ inline void point_gradient(const HexGeo& g, const Lane1D& b, const double* sv, double gv[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gv[i][0] = gv[i][1] = gv[i][2] = 0;
+#pragma unroll
+    for (int a = 0; a < HNV; ++a) {
+        const int r0 = kHexR[a][0], r1 = kHexR[a][1], r2 = kHexR[a][2];
+        const double G0 = (b.d0[r0] * b.L1[r1]) * b.L2[r2] * g.jac[0];
+        const double G1 = (b.L0[r0] * b.d1[r1]) * b.L2[r2] * g.jac[1];
+        const double G2 = (b.L0[r0] * b.L1[r1]) * b.d2[r2] * g.jac[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double v = sv[a * 3 + i];
+            gv[i][0] += G0 * v;
+            gv[i][1] += G1 * v;
+            gv[i][2] += G2 * v;
+        }
+    }
+}
+
+// P1 modes at a point: (1 - ξ - η - ζ, ξ, η, ζ)
+__device__ inline void point_modes(const Hex1D& t, int q, double NP[HNP], double& dOm, double det) {
+    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+    const double xi = t.x[qi], eta = t.x[qj], zeta = t.x[qk];
+    NP[0] = 1 - xi - eta - zeta; NP[1] = xi; NP[2] = eta; NP[3] = zeta;
+    dOm = det * ((t.w[qi] * t.w[qj]) * t.w[qk]);
+}
+
+// nodal residual of lane node a from the point fluxes in LDS
+// (accumulate_momentum, momentum_residuals.jl:28-33): sf[q][0..7] =
+// (τxx - P, τyy - P, τzz - P, τxy, τxz, τyz, ρ, dΩ)
+__device__ inline void node_residual(const HexGeo& g, const Lane1D& b, const double* sf, double gx, double gy,
+                                     double gz, double R[3]) {
+    R[0] = R[1] = R[2] = 0;
+#pragma unroll
+    for (int q = 0; q < HNQ; ++q) {
+        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+        const double N = (b.L0[qi] * b.L1[qj]) * b.L2[qk];
+        const double G0 = (b.d0[qi] * b.L1[qj]) * b.L2[qk] * g.jac[0];
+        const double G1 = (b.L0[qi] * b.d1[qj]) * b.L2[qk] * g.jac[1];
+        const double G2 = (b.L0[qi] * b.L1[qj]) * b.d2[qk] * g.jac[2];
+        const double* f = sf + q * 8;
+        const double rho = f[6], dOm = f[7];
+        R[0] += ((G0 * f[0] + G1 * f[3] + G2 * f[4]) - N * (rho * gx)) * dOm;
+        R[1] += ((G0 * f[3] + G1 * f[1] + G2 * f[5]) - N * (rho * gy)) * dOm;
+        R[2] += ((G0 * f[4] + G1 * f[5] + G2 * f[2]) - N * (rho * gz)) * dOm;
+    }
+}
+
+// element-group bookkeeping of a 256-thread block: 8 elements, 32 lanes each
+struct HexLane {
+    int es, l;
+    int64_t e;
+    bool valid;    // element exists
+    bool active;   // lane serves a node / point (l < 27)
+};
+__device__ inline HexLane hex_lane(int64_t nels) {
+    HexLane h;
+    h.es = threadIdx.x >> 5;
+    h.l = threadIdx.x & 31;
+    h.e = xcd_tile() * HEPB + h.es;
+    h.valid = h.e < nels;
+    h.active = h.valid && h.l < HNV;
+    return h;
+}
+
+// ---- residual: continuity (pressure_residual.jl:132-158), Pnum, momentum
+// (momentum_residuals.jl:209-258), Dirichlet and Chebyshev update of the
+// centre node
+template <int FL>
+__global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
+    constexpr bool ELASTIC = FL & 1, PLASTIC = FL & 2, BULK = FL & 4, ITER = FL & 8;
+    __shared__ double s_v[HEPB][HNV * 3];
+    __shared__ double s_f[HEPB][HNQ * 8];
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    const int64_t e = h.valid ? h.e : g.nels - 1;
+    // phase 0: nodal velocities (lane = node)
+    int64_t node = 0;
+    if (h.l < HNV) {
+        node = hex_node(g, e, h.l);
+        s_v[h.es][h.l * 3 + 0] = A.vx[node];
+        s_v[h.es][h.l * 3 + 1] = A.vy[node];
+        s_v[h.es][h.l * 3 + 2] = A.vz[node];
+    }
+    __syncthreads();
+    // phase A: point quantities (lane = point)
+    const int pc = A.phase[e];
+    double P[HNP], Pn[HNP] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) P[i] = A.P[e * HNP + i];
+    {
+        const int q = h.l < HNQ ? h.l : 0;
+        const Lane1D b = point_1d(g.t, q);
+        double gv[3][3];
+        point_gradient(g, b, s_v[h.es], gv);
+        double NP[HNP], dOm;
+        point_modes(g.t, q, NP, dOm, g.det);
+        // continuity contributions of this point, then the lane-group sum
+        const double div = gv[0][0] + gv[1][1] + gv[2][2];
+        double dPdt = 0;
+        if (BULK) {
+            double dP = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) dP += NP[i] * (P[i] - A.Pold[e * HNP + i]);
+            const double bulk = A.ptab[PC_BULK * MAXPH + pc];
+            double bq = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) bq += NP[i] * bulk;
+            dPdt = dP / (bq * A.dt);
+        }
+        double RP[HNP];
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) RP[i] = group_sum(h.l < HNQ ? NP[i] * ((-div) - dPdt) * dOm : 0.0);
+        if (ITER) {
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) Pn[i] = A.pscale[e * HNP + i] * RP[i];
+        }
+        if (h.valid && h.l < HNP) {
+            if (A.RP_out) A.RP_out[e * HNP + h.l] = RP[h.l];
+            if (A.Pnum_out) A.Pnum_out[e * HNP + h.l] = Pn[h.l];
+        }
+        // stress, return map, total pressure, density
+        const IpMat<double> m = hex_ipmat(A.ptab, pc);
+        double to[6] = {0, 0, 0, 0, 0, 0};
+        if (ELASTIC && h.l < HNQ) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) to[k] = A.tau_old[k][e * HNQ + q];
+        }
+        double Pq = 0, Pnq = 0;
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) { Pq += NP[i] * P[i]; Pnq += NP[i] * Pn[i]; }
+        const double in[6] = {gv[0][0], gv[1][1], gv[2][2], (gv[0][1] + gv[1][0]) * 0.5, (gv[0][2] + gv[2][0]) * 0.5,
+                              (gv[1][2] + gv[2][1]) * 0.5};
+        double tau[6];
+        stress3d<double, PLASTIC>(in, m, to, Pq, tau);
+        if (A.tau_out[0] && h.active) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) A.tau_out[k][e * HNQ + q] = tau[k];
+        }
+        const double Ptot = Pq + Pnq;
+        const double rho0 = A.ptab[PC_RHO0 * MAXPH + pc], al = A.ptab[PC_ALPHA * MAXPH + pc],
+                     be = A.ptab[PC_BETA * MAXPH + pc];
+        // eos_density (momentum_residuals.jl:11-16) with T = 0
+        const double rho = rho0 * ((1 - al * (0.0 - A.Tref)) + be * Pq);
+        if (h.l < HNQ) {
+            double* f = s_f[h.es] + q * 8;
+            f[0] = tau[0] - Ptot; f[1] = tau[1] - Ptot; f[2] = tau[2] - Ptot;
+            f[3] = tau[3]; f[4] = tau[4]; f[5] = tau[5];
+            f[6] = rho; f[7] = dOm;
+        }
+    }
+    __syncthreads();
+    // phase B: nodal residuals (lane = node)
+    if (!h.active) return;
+    const int a = h.l;
+    const Lane1D b = node_1d(g.t, a);
+    double R[3];
+    node_residual(g, b, s_f[h.es], A.gx, A.gy, A.gz, R);
+    if (a < HNSH) {
+        double* s = A.slot + ((e * HNSH + a) * 3);
+        s[0] = R[0]; s[1] = R[1]; s[2] = R[2];
+        return;
+    }
+    // centre node: private to the element
+    const uint8_t fl = A.bcflag[node];
+    const double rx = (fl & 1) ? 0.0 : R[0], ry = (fl & 2) ? 0.0 : R[1], rz = (fl & 4) ? 0.0 : R[2];
+    if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
+    if (A.Rv0_x) { A.Rv0_x[node] = rx; A.Rv0_y[node] = ry; A.Rv0_z[node] = rz; }
+    if (ITER && A.do_update) {
+        double nrx = A.bx * A.rate_x[node] + rx / A.PCx[node];
+        double nry = A.by * A.rate_y[node] + ry / A.PCy[node];
+        double nrz = A.bz * A.rate_z[node] + rz / A.PCz[node];
+        double nvx = s_v[h.es][a * 3 + 0] + (-A.ax) * nrx;
+        double nvy = s_v[h.es][a * 3 + 1] + (-A.ay) * nry;
+        double nvz = s_v[h.es][a * 3 + 2] + (-A.az) * nrz;
+        if (fl & 1) { nrx = 0; nvx = A.bcval_x[node]; }
+        if (fl & 2) { nry = 0; nvy = A.bcval_y[node]; }
+        if (fl & 4) { nrz = 0; nvz = A.bcval_z[node]; }
+        A.rate_x[node] = nrx; A.rate_y[node] = nry; A.rate_z[node] = nrz;
+        ((double*)A.vx)[node] = nvx; ((double*)A.vy)[node] = nvy; ((double*)A.vz)[node] = nvz;
+    }
+}
+
+struct Gather3Args {
+    int64_t ns;
+    const int32_t* node_of; const int32_t* off; const int32_t* ent;
+    const double* slot;
+    double *vx, *vy, *vz, *rate_x, *rate_y, *rate_z;
+    const double *PCx, *PCy, *PCz;
+    const uint8_t* bcflag;
+    const double *bcval_x, *bcval_y, *bcval_z;
+    double *Rv_x, *Rv_y, *Rv_z, *Rv0_x, *Rv0_y, *Rv0_z;
+    double ax, ay, az, bx, by, bz;
+    int32_t do_update;
+};
+
+__global__ __launch_bounds__(256) void k_gather3(Gather3Args A) {
+    const int64_t i = xcd_tile() * 256 + threadIdx.x;
+    if (i >= A.ns) return;
+    const int64_t node = A.node_of ? A.node_of[i] : i;
+    double rx = 0, ry = 0, rz = 0;
+    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
+        const double* s = A.slot + (int64_t)A.ent[k] * 3;
+        rx += s[0]; ry += s[1]; rz += s[2];
+    }
+    const uint8_t fl = A.bcflag[node];
+    if (fl & 1) rx = 0;
+    if (fl & 2) ry = 0;
+    if (fl & 4) rz = 0;
+    if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
+    if (A.Rv0_x) { A.Rv0_x[node] = rx; A.Rv0_y[node] = ry; A.Rv0_z[node] = rz; }
+    if (A.do_update) {
+        double nrx = A.bx * A.rate_x[node] + rx / A.PCx[node];
+        double nry = A.by * A.rate_y[node] + ry / A.PCy[node];
+        double nrz = A.bz * A.rate_z[node] + rz / A.PCz[node];
+        double nvx = A.vx[node] + (-A.ax) * nrx;
+        double nvy = A.vy[node] + (-A.ay) * nry;
+        double nvz = A.vz[node] + (-A.az) * nrz;
+        if (fl & 1) { nrx = 0; nvx = A.bcval_x[node]; }
+        if (fl & 2) { nry = 0; nvy = A.bcval_y[node]; }
+        if (fl & 4) { nrz = 0; nvz = A.bcval_z[node]; }
+        A.rate_x[node] = nrx; A.rate_y[node] = nry; A.rate_z[node] = nrz;
+        A.vx[node] = nvx; A.vy[node] = nvy; A.vz[node] = nvz;
+    }
+}
+
+// ---- augmented Jacobian diagnostics (momentum_residuals.jl:919-1117): per
+// column (comp j, node b) of the element operator, the directional
+// derivative of the momentum residual with Pnum(v) inline, from the exact
+// tangent of the (DP) stress at each point (6-partial duals)
+struct Jac3Args {
+    HexGeo g;
+    const double *vx, *vy, *vz, *P, *pscale;
+    const double* tau_old[6];
+    const int32_t* phase;
+    const double* ptab;
+    double* jslot;                   // [nels][26][6]: |J| row sums (x, y, z), |J_ii| (x, y, z)
+    double *dRdv_x, *dRdv_y, *dRdv_z, *PC_x, *PC_y, *PC_z;
+};
+
+template <bool ELASTIC, bool PLASTIC>
+__global__ __launch_bounds__(HBLOCK) void k_hex27_jacobian(Jac3Args A) {
+    __shared__ double s_v[HEPB][HNV * 3];
+    __shared__ double s_f[HEPB][HNQ * 8];
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    const int64_t e = h.valid ? h.e : g.nels - 1;
+    int64_t node = 0;
+    if (h.l < HNV) {
+        node = hex_node(g, e, h.l);
+        s_v[h.es][h.l * 3 + 0] = A.vx[node];
+        s_v[h.es][h.l * 3 + 1] = A.vy[node];
+        s_v[h.es][h.l * 3 + 2] = A.vz[node];
+    }
+    __syncthreads();
+    const int pc = A.phase[e];
+    const int q = h.l < HNQ ? h.l : 0;
+    const Lane1D bq = point_1d(g.t, q);
+    double NP[HNP], dOm;
+    point_modes(g.t, q, NP, dOm, g.det);
+    // tangent of the stress at this point: T[k][m] = ∂τ_k/∂in_m
+    double T[6][6];
+    {
+        double gv[3][3];
+        point_gradient(g, bq, s_v[h.es], gv);
+        const IpMat<double> m = hex_ipmat(A.ptab, pc);
+        double to[6] = {0, 0, 0, 0, 0, 0};
+        if (ELASTIC && h.l < HNQ) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) to[k] = A.tau_old[k][e * HNQ + q];
+        }
+        double Pq = 0;
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) Pq += NP[i] * A.P[e * HNP + i];
+        D6 in[6] = {D6(gv[0][0]), D6(gv[1][1]), D6(gv[2][2]), D6((gv[0][1] + gv[1][0]) * 0.5),
+                    D6((gv[0][2] + gv[2][0]) * 0.5), D6((gv[1][2] + gv[2][1]) * 0.5)};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) in[k].d[k] = 1.0;
+        D6 tau[6];
+        stress3d<D6, PLASTIC>(in, m, to, Pq, tau);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int mm = 0; mm < 6; ++mm) T[k][mm] = tau[k].d[mm];
+    }
+    double ps[HNP];
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) ps[i] = A.pscale[e * HNP + i];
+    const int a = h.l < HNV ? h.l : 0;
+    const Lane1D bn = node_1d(g.t, a);
+    double rs[3] = {0, 0, 0}, dg[3] = {0, 0, 0};
+    for (int col = 0; col < 3 * HNV; ++col) {
+        const int j = col / HNV, bnode = col % HNV;
+        // phase A (lane = point): derivative of the point flux
+        {
+            const int r0 = kHexR[bnode][0], r1 = kHexR[bnode][1], r2 = kHexR[bnode][2];
+            const double Gb[3] = {(bq.d0[r0] * bq.L1[r1]) * bq.L2[r2] * g.jac[0],
+                                  (bq.L0[r0] * bq.d1[r1]) * bq.L2[r2] * g.jac[1],
+                                  (bq.L0[r0] * bq.L1[r1]) * bq.d2[r2] * g.jac[2]};
+            double dgv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dgv[j][k] = Gb[k];
+            const double din[6] = {dgv[0][0], dgv[1][1], dgv[2][2], (dgv[0][1] + dgv[1][0]) * 0.5,
+                                   (dgv[0][2] + dgv[2][0]) * 0.5, (dgv[1][2] + dgv[2][1]) * 0.5};
+            const double ddiv = dgv[0][0] + dgv[1][1] + dgv[2][2];
+            double dPn = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i)
+                dPn += NP[i] * (ps[i] * group_sum(h.l < HNQ ? NP[i] * (-ddiv) * dOm : 0.0));
+            double dt_[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                double t = 0;
+#pragma unroll
+                for (int mm = 0; mm < 6; ++mm) t += T[k][mm] * din[mm];
+                dt_[k] = t;
+            }
+            if (h.l < HNQ) {
+                double* f = s_f[h.es] + q * 8;
+                f[0] = dt_[0] - dPn; f[1] = dt_[1] - dPn; f[2] = dt_[2] - dPn;
+                f[3] = dt_[3]; f[4] = dt_[4]; f[5] = dt_[5];
+                f[6] = 0.0; f[7] = dOm;
+            }
+        }
+        __syncthreads();
+        // phase B (lane = node): entries J[(i, a), (j, b)]
+        double Jc[3];
+        node_residual(g, bn, s_f[h.es], 0.0, 0.0, 0.0, Jc);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rs[i] += fabs(Jc[i]);
+        if (a == bnode) dg[j] = fabs(Jc[j]);
+        __syncthreads();
+    }
+    if (!h.active) return;
+    if (a < HNSH) {
+        double* s = A.jslot + (e * HNSH + a) * 6;
+        s[0] = rs[0]; s[1] = rs[1]; s[2] = rs[2];
+        s[3] = dg[0]; s[4] = dg[1]; s[5] = dg[2];
+        return;
+    }
+    A.dRdv_x[node] = rs[0]; A.dRdv_y[node] = rs[1]; A.dRdv_z[node] = rs[2];
+    A.PC_x[node] = dg[0]; A.PC_y[node] = dg[1]; A.PC_z[node] = dg[2];
+}
+
+struct GatherJ3Args {
+    int64_t ns;
+    const int32_t* node_of; const int32_t* off; const int32_t* ent;
+    const double* jslot;
+    double *dRdv_x, *dRdv_y, *dRdv_z, *PC_x, *PC_y, *PC_z;
+};
+
+__global__ __launch_bounds__(256) void k_gather_jac3(GatherJ3Args A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.ns) return;
+    const int64_t node = A.node_of ? A.node_of[i] : i;
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
+        const double* s = A.jslot + (int64_t)A.ent[k] * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t[c] += s[c];
+    }
+    A.dRdv_x[node] = t[0]; A.dRdv_y[node] = t[1]; A.dRdv_z[node] = t[2];
+    A.PC_x[node] = t[3]; A.PC_y[node] = t[4]; A.PC_z[node] = t[5];
+}
+
+// ---- assemble_viscosity_weighted_pressure_scaling! with the Hex27 Jacobi
+// weight N_a² dΩ (pressure_scaling.jl:166-216); one thread per element
+__global__ void k_hex27_scaling(HexGeo g, const int32_t* phase, const double* ptab, double gamma_fact, double dt,
+                                double* MP, double* gammaP, double* pscale) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nels) return;
+    const int pc = phase[e];
+    const double eta = ptab[PC_ETA * MAXPH + pc], bulk = ptab[PC_BULK * MAXPH + pc];
+    const double invK = 1.0 / bulk;
+    double M[HNP] = {0, 0, 0, 0}, G[HNP] = {0, 0, 0, 0};
+    for (int q = 0; q < HNQ; ++q) {
+        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+        // interp2ip_phase with the velocity basis (partition of unity)
+        double etaq = 0, bq = 0;
+#pragma unroll
+        for (int a = 0; a < HNV; ++a) {
+            const double N = (g.t.L[kHexR[a][0]][qi] * g.t.L[kHexR[a][1]][qj]) * g.t.L[kHexR[a][2]][qk];
+            etaq += N * eta;
+            bq += N * invK;
+        }
+        const double gnum = gamma_fact * etaq;
+        const double gphy = bq == 0 ? gnum : dt / bq;
+        const double gq = gphy * gnum / (gphy + gnum);
+        double NP[HNP], dOm;
+        point_modes(g.t, q, NP, dOm, g.det);
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) {
+            const double w = NP[i] * NP[i] * dOm;
+            M[i] += w;
+            G[i] += w * gq;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) {
+        MP[e * HNP + i] = M[i];
+        const double gp = M[i] == 0 ? 0.0 : G[i] / M[i];
+        gammaP[e * HNP + i] = gp;
+        pscale[e * HNP + i] = M[i] == 0 ? 0.0 : gp / M[i];
+    }
+}
+
+__global__ void k_pscale3(int64_t n, const double* g, const double* M, double* ps) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ps[i] = M[i] == 0.0 ? 0.0 : g[i] / M[i];
+}
+
+// ---- reductions for three components
+struct Red3 {
+    const double *R[3], *R0[3], *rate[3], *PC[3], *J[3];
+    double al[3];
+};
+
+__global__ __launch_bounds__(RED_BLOCK) void k_red3_check(int64_t n, Red3 r, double* partial) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double Ri = r.R[c][i];
+            acc[c] += Ri * Ri;
+            const double t = r.al[c] * r.rate[c][i];
+            acc[3 + 2 * c] += t * t;
+            acc[4 + 2 * c] += t * ((Ri - r.R0[c][i]) / r.PC[c][i]);
+        }
+    }
+    block_sum_store<9>(acc, partial);
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_red3_outer(int64_t nv, int64_t np, Red3 r, const double* RP,
+                                                          const double* MP, double* partial) {
+    double acc[4] = {0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < np; i += stride) {
+        const double x = RP[i] / MP[i];
+        acc[0] += x * x;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < nv; i += stride)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[1 + c] += r.R[c][i] * r.R[c][i];
+    block_sum_store<4>(acc, partial);
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_red3_lmax(int64_t n, Red3 r, double* partial) {
+    double mx[3] = {-INFINITY, -INFINITY, -INFINITY}, nan = 0;
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double a = r.J[c][i] / r.PC[c][i];
+            if (a != a) nan = 1;
+            mx[c] = fmax(mx[c], a);
+        }
+    __shared__ double s[RED_BLOCK / 64][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mx[c] = wave_max(mx[c]);
+    nan = wave_max(nan);
+    if (lane == 0) { s[w][0] = mx[0]; s[w][1] = mx[1]; s[w][2] = mx[2]; s[w][3] = nan; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4] = {s[0][0], s[0][1], s[0][2], s[0][3]};
+        for (int j = 1; j < RED_BLOCK / 64; ++j)
+            for (int c = 0; c < 4; ++c) t[c] = fmax(t[c], s[j][c]);
+        for (int c = 0; c < 4; ++c) partial[blockIdx.x * 4 + c] = t[c];
+    }
+}
+
+// ---- host side
+struct Opts3 {
+    bool elastic = false, plastic = false, bulk = false;
+    double dt = 1;
+    bool use_pnum = false, do_update = false, store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false,
+         store_tau = false;
+    double al[3] = {0, 0, 0}, be[3] = {0, 0, 0};
+};
+
+int check3(fem_ctx* c, const char* what) { return check_launch(c, what); }
+
+template <int FL>
+void launch_h27(fem_stokes* s, const Hex27Args& a) {
+    hipLaunchKernelGGL((k_hex27_element<FL>), dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0, s->ctx->stream,
+                       a);
+}
+template <int FL>
+void dispatch_h27(int key, fem_stokes* s, const Hex27Args& a) {
+    if (key == FL) launch_h27<FL>(s, a);
+    else if constexpr (FL > 0) dispatch_h27<FL - 1>(key, s, a);
+}
+
+int hex_element(fem_stokes* s, const Opts3& o, bool gather = true) {
+    fem_mesh* m = s->mesh;
+    double** f = s->f;
+    Hex27Args a;
+    std::memset(&a, 0, sizeof a);
+    a.g = hex_geo(m);
+    a.vx = f[FEM_FIELD_V_X]; a.vy = f[FEM_FIELD_V_Y]; a.vz = f[FEM_FIELD_V_Z];
+    a.P = f[FEM_FIELD_P]; a.Pold = f[FEM_FIELD_P0]; a.pscale = s->d_pscale;
+    const int tf[6] = {FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_ZZ,
+                       FEM_FIELD_TAU_OLD_XY, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ};
+    const int tc[6] = {FEM_FIELD_TAU_XX, FEM_FIELD_TAU_YY, FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XY, FEM_FIELD_TAU_XZ,
+                       FEM_FIELD_TAU_YZ};
+    for (int k = 0; k < 6; ++k) {
+        a.tau_old[k] = f[tf[k]];
+        a.tau_out[k] = o.store_tau ? f[tc[k]] : nullptr;
+    }
+    a.phase = s->d_phase_v;
+    a.ptab = s->d_ptab;
+    a.gx = s->mat.g[0]; a.gy = s->mat.g[1]; a.gz = s->mat.g[2];
+    a.Tref = s->mat.Tref;
+    a.dt = o.dt;
+    a.slot = s->d_slot3;
+    a.rate_x = f[FEM_FIELD_DVDTAU_X]; a.rate_y = f[FEM_FIELD_DVDTAU_Y]; a.rate_z = f[FEM_FIELD_DVDTAU_Z];
+    a.PCx = f[FEM_FIELD_PC_V_X]; a.PCy = f[FEM_FIELD_PC_V_Y]; a.PCz = f[FEM_FIELD_PC_V_Z];
+    a.bcflag = s->d_bcflag;
+    a.bcval_x = s->d_bcval_x; a.bcval_y = s->d_bcval_y; a.bcval_z = s->d_bcval_z;
+    a.ax = o.al[0]; a.ay = o.al[1]; a.az = o.al[2];
+    a.bx = o.be[0]; a.by = o.be[1]; a.bz = o.be[2];
+    a.do_update = o.do_update;
+    a.RP_out = o.store_RP ? f[FEM_FIELD_RP] : nullptr;
+    a.Pnum_out = o.store_Pnum ? f[FEM_FIELD_PNUM] : nullptr;
+    if (o.store_Rv) { a.Rv_x = f[FEM_FIELD_RV_X]; a.Rv_y = f[FEM_FIELD_RV_Y]; a.Rv_z = f[FEM_FIELD_RV_Z]; }
+    if (o.store_Rv0) { a.Rv0_x = f[FEM_FIELD_RV0_X]; a.Rv0_y = f[FEM_FIELD_RV0_Y]; a.Rv0_z = f[FEM_FIELD_RV0_Z]; }
+    const int key = (o.elastic ? 1 : 0) | (o.plastic ? 2 : 0) | (o.bulk ? 4 : 0) | (o.use_pnum ? 8 : 0);
+    dispatch_h27<15>(key, s, a);
+    CHK(check3(s->ctx, "k_hex27_element"));
+    if (!gather) return FEM_OK;
+    Gather3Args g;
+    std::memset(&g, 0, sizeof g);
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.slot = s->d_slot3;
+    g.vx = f[FEM_FIELD_V_X]; g.vy = f[FEM_FIELD_V_Y]; g.vz = f[FEM_FIELD_V_Z];
+    g.rate_x = a.rate_x; g.rate_y = a.rate_y; g.rate_z = a.rate_z;
+    g.PCx = a.PCx; g.PCy = a.PCy; g.PCz = a.PCz;
+    g.bcflag = s->d_bcflag;
+    g.bcval_x = s->d_bcval_x; g.bcval_y = s->d_bcval_y; g.bcval_z = s->d_bcval_z;
+    g.Rv_x = a.Rv_x; g.Rv_y = a.Rv_y; g.Rv_z = a.Rv_z;
+    g.Rv0_x = a.Rv0_x; g.Rv0_y = a.Rv0_y; g.Rv0_z = a.Rv0_z;
+    g.ax = a.ax; g.ay = a.ay; g.az = a.az; g.bx = a.bx; g.by = a.by; g.bz = a.bz;
+    g.do_update = o.do_update;
+    hipLaunchKernelGGL(k_gather3, dim3(grid_xcd(m->ns, 256)), dim3(256), 0, s->ctx->stream, g);
+    return check3(s->ctx, "k_gather3");
+}
+
+int hex_jacobian_impl(fem_stokes* s, const Opts3& o) {
+    fem_mesh* m = s->mesh;
+    double** f = s->f;
+    Jac3Args a;
+    std::memset(&a, 0, sizeof a);
+    a.g = hex_geo(m);
+    a.vx = f[FEM_FIELD_V_X]; a.vy = f[FEM_FIELD_V_Y]; a.vz = f[FEM_FIELD_V_Z];
+    a.P = f[FEM_FIELD_P]; a.pscale = s->d_pscale;
+    const int tf[6] = {FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_ZZ,
+                       FEM_FIELD_TAU_OLD_XY, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ};
+    for (int k = 0; k < 6; ++k) a.tau_old[k] = f[tf[k]];
+    a.phase = s->d_phase_v;
+    a.ptab = s->d_ptab;
+    a.jslot = s->d_jslot3;
+    a.dRdv_x = f[FEM_FIELD_DRVDV_X]; a.dRdv_y = f[FEM_FIELD_DRVDV_Y]; a.dRdv_z = f[FEM_FIELD_DRVDV_Z];
+    a.PC_x = f[FEM_FIELD_PC_V_X]; a.PC_y = f[FEM_FIELD_PC_V_Y]; a.PC_z = f[FEM_FIELD_PC_V_Z];
+    const dim3 grid(grid_xcd(m->nels, HEPB)), block(HBLOCK);
+    if (o.elastic && o.plastic) hipLaunchKernelGGL((k_hex27_jacobian<true, true>), grid, block, 0, s->ctx->stream, a);
+    else if (o.elastic) hipLaunchKernelGGL((k_hex27_jacobian<true, false>), grid, block, 0, s->ctx->stream, a);
+    else if (o.plastic) hipLaunchKernelGGL((k_hex27_jacobian<false, true>), grid, block, 0, s->ctx->stream, a);
+    else hipLaunchKernelGGL((k_hex27_jacobian<false, false>), grid, block, 0, s->ctx->stream, a);
+    CHK(check3(s->ctx, "k_hex27_jacobian"));
+    GatherJ3Args g;
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.jslot = s->d_jslot3;
+    g.dRdv_x = a.dRdv_x; g.dRdv_y = a.dRdv_y; g.dRdv_z = a.dRdv_z;
+    g.PC_x = a.PC_x; g.PC_y = a.PC_y; g.PC_z = a.PC_z;
+    hipLaunchKernelGGL(k_gather_jac3, dim3(grid_for(m->ns, 256)), dim3(256), 0, s->ctx->stream, g);
+    return check3(s->ctx, "k_gather_jac3");
+}
+
+Red3 red3(fem_stokes* s) {
+    Red3 r;
+    double** f = s->f;
+    r.R[0] = f[FEM_FIELD_RV_X]; r.R[1] = f[FEM_FIELD_RV_Y]; r.R[2] = f[FEM_FIELD_RV_Z];
+    r.R0[0] = f[FEM_FIELD_RV0_X]; r.R0[1] = f[FEM_FIELD_RV0_Y]; r.R0[2] = f[FEM_FIELD_RV0_Z];
+    r.rate[0] = f[FEM_FIELD_DVDTAU_X]; r.rate[1] = f[FEM_FIELD_DVDTAU_Y]; r.rate[2] = f[FEM_FIELD_DVDTAU_Z];
+    r.PC[0] = f[FEM_FIELD_PC_V_X]; r.PC[1] = f[FEM_FIELD_PC_V_Y]; r.PC[2] = f[FEM_FIELD_PC_V_Z];
+    r.J[0] = f[FEM_FIELD_DRVDV_X]; r.J[1] = f[FEM_FIELD_DRVDV_Y]; r.J[2] = f[FEM_FIELD_DRVDV_Z];
+    r.al[0] = r.al[1] = r.al[2] = 0;
+    return r;
+}
+
+// launches a two-stage reduction and copies K results to host
+template <class L>
+int reduce3(fem_stokes* s, int K, int op, int64_t n, L launch, double* out) {
+    fem_ctx* c = s->ctx;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(n, RED_BLOCK));
+    launch(g);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, K, op,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_red3"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, K * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_red, K * 8);
+    return FEM_OK;
+}
+
+int checked_lmax3(fem_stokes* s, double lmax[3]) {
+    fem_ctx* c = s->ctx;
+    const Red3 r = red3(s);
+    const int64_t nv = s->mesh->nnodes;
+    double out[4];
+    CHK(reduce3(s, 4, 1, nv, [&](unsigned g) {
+        hipLaunchKernelGGL(k_red3_lmax, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, r, c->red_partial);
+    }, out));
+    const char* labels[3] = {"stokes vx", "stokes vy", "stokes vz"};
+    for (int k = 0; k < 3; ++k) {
+        lmax[k] = out[3] > 0 ? std::numeric_limits<double>::quiet_NaN() : out[k];
+        if (!(std::isfinite(lmax[k]) && lmax[k] > 0))
+            return set_err(c, FEM_EPRECOND,
+                           "%s preconditioner produced invalid λmax = %g; check for zero or non-finite preconditioner entries",
+                           labels[k], lmax[k]);
+    }
+    return FEM_OK;
+}
+
+int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, Opts3& o, bool use_tau_old) {
+    bool bulk_inf, thermal, eos_p, elastic;
+    if (!(dt > 0) || !std::isfinite(dt)) return set_err(s->ctx, FEM_EARG, "dt must be positive and finite");
+    if (s->layout_v != FEM_PHASE_CELL || s->layout_P != FEM_PHASE_CELL || !s->phase_same)
+        return set_err(s->ctx, FEM_EARG, "the 3-D path takes one phase per cell (a 1 x nels phase matrix)");
+    CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
+    (void)thermal;   // no temperature field in 3-D: ρ = ρ0 (1 + α Tref + β P)
+    o.plastic = dp != nullptr;
+    o.elastic = elastic && use_tau_old;
+    o.bulk = !bulk_inf;
+    o.dt = dt;
+    return FEM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// entry points (behind fem_stokes_* for 3-D states)
+// ---------------------------------------------------------------------------
+// the scaling kernel alone, with the phase table already on the device
+static int hex_scaling_launch(fem_stokes* s, double gamma_fact, double dt) {
+    fem_mesh* m = s->mesh;
+    hipLaunchKernelGGL(k_hex27_scaling, dim3(grid_for(m->nels, 128)), dim3(128), 0, s->ctx->stream, hex_geo(m),
+                       (const int32_t*)s->d_phase_v, (const double*)s->d_ptab, gamma_fact, dt, s->f[FEM_FIELD_M_P],
+                       s->f[FEM_FIELD_GAMMA_P], s->d_pscale);
+    return check_launch(s->ctx, "k_hex27_scaling");
+}
+
+int hex27_pressure_scaling(fem_stokes* s, double gamma_fact, double dt, int finite_K) {
+    Opts3 o;
+    CHK(prepare3(s, nullptr, dt, finite_K, o, true));
+    return hex_scaling_launch(s, gamma_fact, dt);
+}
+
+static int hex_pscale(fem_stokes* s) {
+    const int64_t np = s->mesh->nnodesP;
+    hipLaunchKernelGGL(k_pscale3, dim3(grid_for(np, 256)), dim3(256), 0, s->ctx->stream, np,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    return check_launch(s->ctx, "k_pscale3");
+}
+
+int hex27_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, bool with_pnum, int finite_K) {
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, finite_K, o, true));
+    CHK(hex_pscale(s));
+    HIPCHK(s->ctx, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, s->ctx->stream));
+    o.use_pnum = with_pnum;
+    o.store_RP = true;
+    o.store_Pnum = with_pnum;
+    o.store_Rv = true;
+    return hex_element(s, o);
+}
+
+int hex27_jacobian(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K) {
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, finite_K, o, true));
+    CHK(hex_pscale(s));
+    return hex_jacobian_impl(s, o);
+}
+
+int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt) {
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, 0, o, true));
+    HIPCHK(s->ctx, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, s->ctx->stream));
+    o.store_tau = true;
+    o.store_Rv = true;
+    return hex_element(s, o);
+}
+
+// solve! + _solve_stokes_dyrel! (DR.jl:302-337, 348-802) with D = 3
+int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
+                fem_solve_stats* st) {
+    fem_ctx* c = s->ctx;
+    if (p->precision != 0) return set_err(c, FEM_EARG, "the 3-D path runs in FP64 (precision 0)");
+    const auto t0 = std::chrono::steady_clock::now();
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
+    const double dt = p->dt;
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
+    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
+    CHK(upload_bcs(s, bcs));
+    // the phase table uploaded by prepare3 (with the plastic parameters) is
+    // the one the scaling kernel reads
+    if (!p->skip_scaling) CHK(hex_scaling_launch(s, p->pressure_factor, dt));
+    else CHK(hex_pscale(s));
+    double** f = s->f;
+    for (int k : {FEM_FIELD_DVDTAU_X, FEM_FIELD_DVDTAU_Y, FEM_FIELD_DVDTAU_Z, FEM_FIELD_RV0_X, FEM_FIELD_RV0_Y,
+                  FEM_FIELD_RV0_Z})
+        HIPCHK(c, hipMemsetAsync(f[k], 0, nv * 8, c->stream));
+    CHK(hex_jacobian_impl(s, o));
+    double lmax[3];
+    CHK(checked_lmax3(s, lmax));
+    double lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    int64_t jac = 1;
+    double al[3], be[3];
+    for (int k = 0; k < 3; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g, %g)\n", lmax[0], lmax[1], lmax[2]);
+
+    const double eps_tol = p->tolerance;
+    double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
+    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
+    int64_t iter = 0, itPH_done = 0, launches = 0;
+    bool converged = false;
+    double rel_drop = p->rel_drop0;
+    int rc = FEM_OK;
+    const Red3 rbase = red3(s);
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        Opts3 oo = o;
+        oo.store_RP = true; oo.store_Rv = true;
+        CHK(hex_element(s, oo));
+        double r4[4];
+        CHK(reduce3(s, 4, 0, std::max(nv, np), [&](unsigned g) {
+            hipLaunchKernelGGL(k_red3_outer, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np, rbase,
+                               (const double*)f[FEM_FIELD_RP], (const double*)f[FEM_FIELD_M_P], c->red_partial);
+        }, r4));
+        err_P = std::sqrt(r4[0]) / std::sqrt((double)np);
+        err_v = std::max(std::sqrt(r4[1]), std::max(std::sqrt(r4[2]), std::sqrt(r4[3]))) / (2 * std::sqrt((double)nv));
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v / err_v0, err_P / err_P0);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
+            break;
+        }
+        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e\n", (long long)itPH, (long long)iter, err,
+                   err_abs, err_rel);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        bool stop_total = false;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT;
+            ++iter;
+            const bool do_check = iter % ncheck == 0;
+            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
+            if (do_check && ncheck == 1)
+                for (int k = 0; k < 3; ++k)
+                    HIPCHK(c, hipMemcpyAsync(f[FEM_FIELD_RV0_X + (k == 2 ? FEM_FIELD_RV0_Z - FEM_FIELD_RV0_X : k)],
+                                             f[FEM_FIELD_RV_X + (k == 2 ? FEM_FIELD_RV_Z - FEM_FIELD_RV_X : k)], nv * 8,
+                                             hipMemcpyDeviceToDevice, c->stream));
+            Opts3 oi = o;
+            oi.use_pnum = true;
+            oi.do_update = true;
+            oi.store_RP = last_possible;
+            oi.store_Pnum = last_possible;
+            oi.store_Rv = last_possible;
+            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
+            for (int k = 0; k < 3; ++k) { oi.al[k] = al[k]; oi.be[k] = be[k]; }
+            CHK(hex_element(s, oi));
+            ++launches;
+            if (do_check) {
+                Red3 r = rbase;
+                for (int k = 0; k < 3; ++k) r.al[k] = al[k];
+                double r9[9];
+                CHK(reduce3(s, 9, 0, nv, [&](unsigned g) {
+                    hipLaunchKernelGGL(k_red3_check, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, r, c->red_partial);
+                }, r9));
+                const double evi = std::max(std::sqrt(r9[0]), std::max(std::sqrt(r9[1]), std::sqrt(r9[2]))) /
+                                   (2 * std::sqrt((double)nv));
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                double lmin[3];
+                for (int k = 0; k < 3; ++k) {
+                    const double den = r9[3 + 2 * k], num = r9[4 + 2 * k];
+                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+                }
+                if (!freeze) {
+                    ++jac;
+                    CHK(hex_jacobian_impl(s, o));
+                    rc = checked_lmax3(s, lmax);
+                    if (rc != FEM_OK) break;
+                    lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+                }
+                for (int k = 0; k < 3; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+            }
+            if (iter > p->max_iterations) { stop_total = true; break; }
+        }
+        if (rc != FEM_OK) break;
+        hipLaunchKernelGGL(k_pressure_update<double>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                           f[FEM_FIELD_P], (const double*)f[FEM_FIELD_GAMMA_P], (const double*)f[FEM_FIELD_RP],
+                           (const double*)f[FEM_FIELD_M_P]);
+        CHK(check_launch(c, "k_pressure_update"));
+        if (stop_total || iter > p->max_iterations) break;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_abs = err_abs;
+    st->err_rel = err_rel;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged;
+    st->reached_total_iter = iter > p->max_iterations;
+    st->lambda_max = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    st->lambda_max_gershgorin = lmax_g;
+    st->lambda_max_iterations = 0;
+    st->jacobian_assemblies = jac;
+    st->element_launches = launches;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != FEM_OK) return rc;
+    if (!converged && p->throw_on_failure)
+        return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                       (long long)iter, err_abs, p->tolerance);
+    return FEM_OK;
+}
+
+// element kernel alone, n sweeps, for the bench's device-event timing
+int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms) {
+    fem_ctx* c = s->ctx;
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, 0, o, true));
+    o.use_pnum = true;
+    o.do_update = true;
+    for (int k = 0; k < 3; ++k) { o.al[k] = 0.0; o.be[k] = 1.0; }
+    CHK(hex_element(s, o, false));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < n; ++i) CHK(hex_element(s, o, false));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / n;
+    return FEM_OK;
+}
+
+extern "C" int fem_mesh_hex27_create(fem_ctx* c, int32_t nx, int32_t ny, int32_t nz, double x0, double x1, double y0,
+                                     double y1, double z0, double z1, fem_mesh** out) {
+    if (!c || !out) return FEM_EARG;
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1)
+        return set_err(c, FEM_EARG, "element counts must be positive, got (%d, %d, %d)", nx, ny, nz);
+    if (!(x1 > x0) || !(y1 > y0) || !(z1 > z0)) return set_err(c, FEM_EARG, "empty domain");
+    const int64_t sx = 2 * (int64_t)nx + 1, sy = 2 * (int64_t)ny + 1, sz = 2 * (int64_t)nz + 1;
+    const int64_t nn = sx * sy * sz, ne = (int64_t)nx * ny * nz;
+    if (nn > std::numeric_limits<int32_t>::max() || ne * HNSH > std::numeric_limits<int32_t>::max())
+        return set_err(c, FEM_EDIM, "mesh too large for 32-bit node and element-slot indices");
+    HIPCHK(c, hipSetDevice(c->device));
+    fem_mesh* m = new fem_mesh();
+    m->ctx = c;
+    m->dim = 3;
+    m->nloc = HNV;
+    m->structured = true;
+    m->nx = nx; m->ny = ny; m->nz = nz;
+    const double bx[6] = {x0, x1, y0, y1, z0, z1};
+    std::memcpy(m->box, bx, sizeof bx);
+    m->nnodes = nn;
+    m->nels = ne;
+    m->nnodesP = 4 * ne;
+    // generate_coordinates (mesh.jl:340-398): lexicographic, x fastest
+    m->h_coords.resize((size_t)nn * 3);
+    for (int64_t iz = 0; iz < sz; ++iz)
+        for (int64_t iy = 0; iy < sy; ++iy)
+            for (int64_t ix = 0; ix < sx; ++ix) {
+                const int64_t k = (iz * sy + iy) * sx + ix;
+                m->h_coords[3 * k + 0] = x0 + (x1 - x0) * (double)ix / (double)(sx - 1);
+                m->h_coords[3 * k + 1] = y0 + (y1 - y0) * (double)iy / (double)(sy - 1);
+                m->h_coords[3 * k + 2] = z0 + (z1 - z0) * (double)iz / (double)(sz - 1);
+                if (ix == 0 || iy == 0 || iz == 0 || ix == sx - 1 || iy == sy - 1 || iz == sz - 1)
+                    m->h_boundary.push_back((int32_t)k);
+            }
+    // generate_element2node (connectivity.jl:273-322), elements x fastest
+    m->h_el2n.resize((size_t)ne * HNV);
+    for (int64_t e = 0; e < ne; ++e) {
+        const int64_t ex = e % nx, ey = (e / nx) % ny, ez = e / ((int64_t)nx * ny);
+        for (int a = 0; a < HNV; ++a)
+            m->h_el2n[e * HNV + a] =
+                (int32_t)((2 * ez + kHexR[a][2]) * sx * sy + (2 * ey + kHexR[a][1]) * sx + (2 * ex + kHexR[a][0]));
+    }
+    // the 26 shared local nodes: node -> element-slot lists in ascending
+    // element order (generate_node2element, connectivity.jl:341-349); cell
+    // centres (odd, odd, odd grid indices) are private to their element
+    std::vector<int64_t> sid(nn, -1);
+    std::vector<int32_t> node_of;
+    for (int64_t k = 0; k < nn; ++k) {
+        const int64_t ix = k % sx, iy = (k / sx) % sy, iz = k / (sx * sy);
+        if ((ix & 1) && (iy & 1) && (iz & 1)) continue;
+        sid[k] = (int64_t)node_of.size();
+        node_of.push_back((int32_t)k);
+    }
+    m->ns = (int64_t)node_of.size();
+    std::vector<int32_t> off(m->ns + 1, 0);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < HNSH; ++a) off[sid[m->h_el2n[e * HNV + a]] + 1]++;
+    for (int64_t i = 0; i < m->ns; ++i) off[i + 1] += off[i];
+    std::vector<int32_t> ent(off[m->ns]);
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < HNSH; ++a) ent[fill[sid[m->h_el2n[e * HNV + a]]]++] = (int32_t)(e * HNSH + a);
+    int r = FEM_OK;
+    if ((r = dalloc_t(c, &m->d_off, off.size())) || (r = dalloc_t(c, &m->d_ent, ent.size())) ||
+        (r = dalloc_t(c, &m->d_node_of, node_of.size()))) {
+        fem_mesh_destroy(m);
+        return r;
+    }
+    hipError_t he = hipMemcpy(m->d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(m->d_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(m->d_node_of, node_of.data(), node_of.size() * 4, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        fem_mesh_destroy(m);
+        return set_err(c, FEM_EHIP, "hipMemcpy of the Hex27 node lists failed: %s", hipGetErrorString(he));
+    }
+    *out = m;
+    return FEM_OK;
+}

@@ -74,18 +74,62 @@ struct KernelTimer {
 
 struct fem_mesh {
     fem_ctx* ctx = nullptr;
+    int32_t dim = 2, nloc = 7;         // T7 (2-D) or Hex27 (3-D)
     int64_t nnodes = 0, nels = 0, nnodesP = 0, ns = 0;
     bool structured = false;
-    double box[4] = {0, 0, 0, 0};
-    std::vector<double> h_coords;      // [nnodes][2]
-    std::vector<int32_t> h_el2n;       // [nels][7] 0-based
+    double box[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> h_coords;      // [nnodes][dim]
+    std::vector<int32_t> h_el2n;       // [nels][nloc] 0-based
     std::vector<int32_t> h_boundary;   // 0-based
     int32_t* d_el2n = nullptr;         // [7][nels]
     double* d_geo = nullptr;           // [5][nels]
     int32_t* d_off = nullptr;          // [ns+1]
     int32_t* d_ent = nullptr;
     int32_t* d_node_of = nullptr;      // nullable
-    int32_t nx = 0, ny = 0;            // structured box mesh cell counts
+    int32_t nx = 0, ny = 0, nz = 0;    // structured box mesh cell counts
+};
+
+struct fem_stokes {
+    fem_ctx* ctx = nullptr;
+    fem_mesh* mesh = nullptr;
+    int32_t dim = 2;
+    fem_material mat;
+    fem_state_opts opts;
+    double* f[FEM_FIELD_COUNT] = {};
+    int64_t len[FEM_FIELD_COUNT] = {};
+    int32_t* d_phase_v = nullptr;   // cell [nels] or element-node [7][nels], 0-based
+    int32_t* d_phase_P = nullptr;   // cell [nels] or element-node [3][nels]
+    int32_t layout_v = FEM_PHASE_CELL, layout_P = FEM_PHASE_CELL;
+    bool phase_same = true;         // cell phases of v and P identical
+    double* d_pscale = nullptr;
+    double2* d_Re = nullptr;
+    double2* d_Jrow = nullptr;
+    double2* d_Jdiag = nullptr;
+    uint8_t* d_bcflag = nullptr;
+    double* d_bcval_x = nullptr;
+    double* d_bcval_y = nullptr;
+    double* d_bcval_z = nullptr;      // 3-D
+    double* d_slot3 = nullptr;        // 3-D element-slot residuals [nels][26][3]
+    double* d_jslot3 = nullptr;       // 3-D Jacobian slots [nels][26][6]: |J| row sums, |J_ii|
+    int32_t* d_bc_stage = nullptr;
+    double* d_bc_stage_v = nullptr;
+    int64_t bc_stage_cap = 0;
+    double* d_ptab = nullptr;
+    int32_t* d_flag = nullptr;      // [0] scratch yes/no flag, [1] yielding-element count
+    uint8_t* d_yflag = nullptr;     // near-yield flag per element (mixed-precision solve)
+    bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
+    bool q_nonzero = false;         // dr.Q holds a nonzero entry
+    // FP32 work arrays of the mixed-precision solve (allocated by its first call):
+    // the correction (δv, δP), its DR rates/residuals, the right-hand sides R(x)
+    // and FP32 copies of the scaling, preconditioner and geometry
+    struct W32 {
+        float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
+        float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
+        float2* Re;
+        // stress-history copy for the correction operator with the return map
+        float *tau_old0, *tau_old1, *tau_old2;
+    };
+    W32* w32 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -134,3 +178,30 @@ static inline unsigned grid_xcd(int64_t n, int block) {
     return (g + 7u) & ~7u;
 }
 
+// shared between the 2-D and 3-D translation units
+int upload_bcs(fem_stokes* s, const fem_bc* bcs);   // one fem_bc per velocity component
+// 3-D (Hex27) entry points behind the fem_stokes_* C ABI
+int hex27_pressure_scaling(fem_stokes* s, double gamma_fact, double dt, int finite_K);
+int hex27_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, bool with_pnum, int finite_K);
+int hex27_jacobian(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K);
+int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt);
+int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
+                fem_solve_stats* st);
+int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
+
+// eps(x) of Julia (spacing at |x|) and the Chebyshev step/momentum (DR.jl:39-42)
+static inline double julia_eps(double x) {
+    const double a = std::fabs(x);
+    if (!std::isfinite(a)) return std::numeric_limits<double>::quiet_NaN();
+    return std::nextafter(a, std::numeric_limits<double>::infinity()) - a;
+}
+
+static inline void cheb(double dtau, double lmin, double c_fact, double* a, double* b) {
+    // _stokes_cheb, DR.jl:39-42
+    const double c = 2 * std::sqrt(lmin) * c_fact;
+    *a = 2 * dtau * dtau / (2 + c * dtau);
+    *b = (2 - c * dtau) / (2 + c * dtau);
+}
+
+int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
+                bool* thermal, bool* eos_p, bool* elastic);

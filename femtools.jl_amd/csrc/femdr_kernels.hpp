@@ -988,7 +988,7 @@ __global__ void k_pressure_update(int64_t np, F* P, const F* gammaP, const F* RP
 // ---------------------------------------------------------------------------
 constexpr int RED_BLOCK = 256;
 constexpr int RED_GRID = 512;
-constexpr int RED_MAXK = 8;
+constexpr int RED_MAXK = 12;
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -142,6 +142,7 @@ _sig("fem_fill_f32", C.c_int, VP, VP, C.c_float, I64)
 _sig("fem_fill_i32", C.c_int, VP, VP, I32, I64)
 _sig("fem_mesh_t7_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
 _sig("fem_mesh_t7_upload", C.c_int, VP, _P(D), I64, _P(I32), I64, _P(VP))
+_sig("fem_mesh_hex27_create", C.c_int, VP, I32, I32, I32, D, D, D, D, D, D, _P(VP))
 _sig("fem_mesh_get_info", C.c_int, VP, _P(_MeshInfo))
 _sig("fem_mesh_get_coords", C.c_int, VP, _P(D))
 _sig("fem_mesh_get_el2n", C.c_int, VP, _P(I32))
@@ -166,7 +167,7 @@ _sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
 EXPORTED = [
     "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
     "fem_copy_h2d", "fem_copy_d2h", "fem_copy_d2d", "fem_fill_f64", "fem_fill_f32", "fem_fill_i32",
-    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
+    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
     "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
@@ -274,6 +275,41 @@ class VectorField2D:
     def __iter__(self):
         return iter((self.x, self.y))
 
+    def __getitem__(self, i):
+        return tuple(self)[i]
+
+
+@dataclass
+class VectorField3D:
+    """VectorField3D (src/vector_fields/vector_fields.jl:34-43)."""
+    x: DeviceArray
+    y: DeviceArray
+    z: DeviceArray
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+    def __getitem__(self, i):
+        return tuple(self)[i]
+
+
+@dataclass
+class SymmetricTensor3D:
+    """SymmetricTensor3D (src/tensor_fields/symmetric_tensor.jl:53-63), components
+    xx, yy, zz, xy, xz, yz; the invariant slot is not stored."""
+    xx: DeviceArray
+    yy: DeviceArray
+    zz: DeviceArray
+    xy: DeviceArray
+    xz: DeviceArray
+    yz: DeviceArray
+
+    def __iter__(self):
+        return iter((self.xx, self.yy, self.zz, self.xy, self.xz, self.yz))
+
+    def __getitem__(self, i):
+        return tuple(self)[i]
+
 
 @dataclass
 class SymmetricTensor2D:
@@ -285,6 +321,9 @@ class SymmetricTensor2D:
 
     def __iter__(self):
         return iter((self.xx, self.yy, self.xy))
+
+    def __getitem__(self, i):
+        return tuple(self)[i]
 
 
 def _tup(v, n=None):
@@ -364,8 +403,8 @@ class DruckerPrager:
 
 
 class MixedMesh:
-    """MixedMesh of T7 velocity / P1-disc pressure elements
-    (src/mesh/mixed_mesh.jl:173-211) resident on the device."""
+    """MixedMesh of T7 (2-D) or Hex27 (3-D) velocity / P1-disc pressure
+    elements (src/mesh/mixed_mesh.jl:173-211, 358-417) resident on the device."""
 
     def __init__(self, ctx: Context, handle, box=None):
         self.ctx = ctx
@@ -373,17 +412,18 @@ class MixedMesh:
         info = _MeshInfo()
         ctx.check(_lib.fem_mesh_get_info(handle, C.byref(info)))
         self.nnodes, self.nels, self.nnodesP = info.nnodes, info.nels, info.nnodesP
+        self.dim, self.nv_local, self.np_local = info.dim, info.nv_local, info.np_local
         self.n_shared = info.n_shared
         self.nq = info.nq
         self.box = box
-        coords = np.empty((self.nnodes, 2))
+        coords = np.empty((self.nnodes, self.dim))
         ctx.check(_lib.fem_mesh_get_coords(handle, coords.ctypes.data_as(_P(D))))
-        el2n = np.empty((self.nels, 7), np.int32)
+        el2n = np.empty((self.nels, self.nv_local), np.int32)
         ctx.check(_lib.fem_mesh_get_el2n(handle, el2n.ctypes.data_as(_P(I32))))
         self.coords = coords
         self.el2n = el2n - 1
-        self.el2nP = self.el2n[:, :3]
-        self.DoFsP = np.arange(self.nnodesP, dtype=np.int32).reshape(self.nels, 3)
+        self.el2nP = self.el2n[:, :self.np_local]
+        self.DoFsP = np.arange(self.nnodesP, dtype=np.int32).reshape(self.nels, self.np_local)
         nb = info.n_boundary
         b = np.empty(nb, np.int32)
         if nb:
@@ -404,6 +444,17 @@ def t7_box_mesh(nx: int, ny: int, domain=((0.0, 1.0), (0.0, 1.0)), ctx: Context 
     h = VP()
     ctx.check(_lib.fem_mesh_t7_create(ctx.handle, int(nx), int(ny), x0, x1, y0, y1, C.byref(h)))
     return MixedMesh(ctx, h, box=(x0, x1, y0, y1))
+
+
+def hex27_box_mesh(nx: int, ny: int, nz: int, domain=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)),
+                   ctx: Context | None = None) -> MixedMesh:
+    """MixedMesh(Mesh(backend, (x0..x1)×(y0..y1)×(z0..z1), Hex27, (nx, ny, nz)), P1)."""
+    ctx = ctx or default_context()
+    (x0, x1), (y0, y1), (z0, z1) = domain
+    h = VP()
+    ctx.check(_lib.fem_mesh_hex27_create(ctx.handle, int(nx), int(ny), int(nz), x0, x1, y0, y1, z0, z1,
+                                         C.byref(h)))
+    return MixedMesh(ctx, h, box=(x0, x1, y0, y1, z0, z1))
 
 
 def t7_mesh_from_arrays(coords, el2n, ctx: Context | None = None) -> MixedMesh:
@@ -439,23 +490,25 @@ class DirichletBoundaryCondition:
 
 _FIELDS = ["v.x", "v.y", "dvdtau.x", "dvdtau.y", "Rv.x", "Rv.y", "Rv0.x", "Rv0.y", "dRvdv.x", "dRvdv.y",
            "PC_v.x", "PC_v.y", "tau.xx", "tau.yy", "tau.xy", "tau_old.xx", "tau_old.yy", "tau_old.xy",
-           "P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum"]
+           "P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum",
+           "v.z", "dvdtau.z", "Rv.z", "Rv0.z", "dRvdv.z", "PC_v.z", "tau.zz", "tau.xz", "tau.yz",
+           "tau_old.zz", "tau_old.xz", "tau_old.yz"]
 
 
 class StokesDR:
     """StokesDR(mesh, material; CFL_v=0.98, CFL_P=0.98, c_fact=0.9, ϵ=1e-6)
     (stokes_types.jl:197-298, 359-369). Fields are device arrays:
     ``v``, ``dvdtau`` (∂v∂τ), ``Rv``, ``Rv0``, ``dRvdv`` (∂Rv∂v), ``PC_v``
-    (VectorField2D); ``tau``, ``tau_old`` (SymmetricTensor2D at integration
-    points, shape (nels, nq)); ``P``, ``P0``, ``gammaP``, ``T``, ``T0``, ``Q``,
-    ``Pf``, ``RP``, ``RP0``, ``M_P``, ``Pnum``."""
+    (VectorField2D / VectorField3D); ``tau``, ``tau_old`` (SymmetricTensor2D /
+    SymmetricTensor3D at integration points, shape (nels, nq)); ``P``, ``P0``,
+    ``gammaP``, ``T``, ``T0``, ``Q``, ``Pf``, ``RP``, ``RP0``, ``M_P``, ``Pnum``."""
 
     def __init__(self, mesh: MixedMesh, material: StokesMaterial, CFL_v=0.98, CFL_P=0.98, c_fact=0.9, eps=1e-6):
         self.ctx = mesh.ctx
         self.mesh = mesh
         self.material = material
-        if len(material.g) != 2:
-            raise DimensionMismatch("material gravity must have 2 components")
+        if len(material.g) != mesh.dim:
+            raise DimensionMismatch(f"material gravity must have {mesh.dim} components for a {mesh.dim}-D mesh")
         self._mat = material._c()
         opts = _StateOpts(CFL_v, CFL_P, c_fact, eps)
         h = VP()
@@ -467,17 +520,19 @@ class StokesDR:
         for k, name in enumerate(_FIELDS):
             p, n, dt = VP(), I64(), I32()
             self.ctx.check(_lib.fem_stokes_field(h, k, C.byref(p), C.byref(n), C.byref(dt)))
-            shape = (mesh.nels, mesh.nq) if name.startswith("tau") else (n.value,)
+            shape = (mesh.nels, mesh.nq) if name.startswith("tau") and n.value else (n.value,)
             arrs.append(DeviceArray(self.ctx, p.value, shape, np.float64, owner=self))
         a = dict(zip(_FIELDS, arrs))
-        self.v = VectorField2D(a["v.x"], a["v.y"])
-        self.dvdtau = VectorField2D(a["dvdtau.x"], a["dvdtau.y"])
-        self.Rv = VectorField2D(a["Rv.x"], a["Rv.y"])
-        self.Rv0 = VectorField2D(a["Rv0.x"], a["Rv0.y"])
-        self.dRvdv = VectorField2D(a["dRvdv.x"], a["dRvdv.y"])
-        self.PC_v = VectorField2D(a["PC_v.x"], a["PC_v.y"])
-        self.tau = SymmetricTensor2D(a["tau.xx"], a["tau.yy"], a["tau.xy"])
-        self.tau_old = SymmetricTensor2D(a["tau_old.xx"], a["tau_old.yy"], a["tau_old.xy"])
+        if mesh.dim == 3:
+            for name in ("v", "dvdtau", "Rv", "Rv0", "dRvdv", "PC_v"):
+                setattr(self, name, VectorField3D(a[name + ".x"], a[name + ".y"], a[name + ".z"]))
+            for name in ("tau", "tau_old"):
+                setattr(self, name, SymmetricTensor3D(*(a[f"{name}.{c}"] for c in ("xx", "yy", "zz", "xy", "xz", "yz"))))
+        else:
+            for name in ("v", "dvdtau", "Rv", "Rv0", "dRvdv", "PC_v"):
+                setattr(self, name, VectorField2D(a[name + ".x"], a[name + ".y"]))
+            for name in ("tau", "tau_old"):
+                setattr(self, name, SymmetricTensor2D(*(a[f"{name}.{c}"] for c in ("xx", "yy", "xy"))))
         for name in ("P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum"):
             setattr(self, name, a[name])
 
@@ -492,7 +547,7 @@ class StokesDR:
         phases, or a per-node (per-DoF) vector. 1-based ids as in the reference."""
         ph = np.asarray(phases)
         w = 0 if which == "v" else 1
-        nloc = 7 if w == 0 else 3
+        nloc = self.mesh.nv_local if w == 0 else self.mesh.np_local
         nnod = self.mesh.nnodes if w == 0 else self.mesh.nnodesP
         if ph.ndim == 2 and ph.shape == (self.mesh.nels, nloc):
             layout = 1
@@ -530,13 +585,15 @@ def solve(dr, mesh, *bcs, **kwargs):
     return solve_stokes(dr, mesh, *bcs, **kwargs)
 
 
-def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition, *,
+def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition,
+          bc_vz: DirichletBoundaryCondition | None = None, *,
           dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, check_interval=50, iterMax=50_000,
           max_ph_iterations=1000, rel_drop0=1e-2, plastic=None, phases_v=None, phases_P=None, finite_K=False,
           freeze_jacobian=None, throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False,
           time_kernels=False, precision="fp64"):
-    """solve!(dr, mesh, bc_vx, bc_vy; dt, ...) (src/stokes/solvers/DR.jl:302-337).
-    Mutates dr in place and returns the stats dict of DR.jl:783-801.
+    """solve!(dr, mesh, bc_vx, bc_vy[, bc_vz]; dt, ...) (src/stokes/solvers/DR.jl:302-337).
+    Mutates dr in place and returns the stats dict of DR.jl:783-801. A 3-D
+    state takes three boundary conditions and runs in FP64.
 
     precision="fp64" runs the reference's iteration in FP64. precision="fp32"
     keeps the state, the outer residuals, the norms and the convergence test
@@ -551,8 +608,11 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
         dr.set_phases("v", phases_v)
     if phases_P is not None:
         dr.set_phases("P", phases_P)
+    given = [b for b in (bc_vx, bc_vy, bc_vz) if b is not None]
+    if len(given) != mesh.dim:
+        raise DimensionMismatch(f"a {mesh.dim}-D solve takes {mesh.dim} boundary conditions, got {len(given)}")
     keep = []
-    bcs = (_BC * 2)(bc_vx._c(keep), bc_vy._c(keep))
+    bcs = (_BC * len(given))(*(b._c(keep) for b in given))
     d, dp = _dp_ptr(plastic)
     p = _SolveParams(dt=float(dt), pressure_factor=float(pressure_factor), tolerance=float(tolerance),
                      max_iterations=int(max_iterations), iter_max=int(iterMax), check_interval=int(check_interval),

@@ -10,7 +10,8 @@ ROOT = os.path.dirname(PKG)                 # femtools.jl_amd/
 REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libfemdr.so")
-SOURCES = [os.path.join(CSRC, "femdr.hip"), os.path.join(CSRC, "femdr_thermal.hip")]
+SOURCES = [os.path.join(CSRC, "femdr.hip"), os.path.join(CSRC, "femdr_thermal.hip"),
+           os.path.join(CSRC, "femdr_hex27.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, "femdr_kernels.hpp"), os.path.join(CSRC, "femdr_host.hpp"),
                   os.path.join(REPO, "include", "femdr.h")]
 

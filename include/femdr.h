@@ -63,10 +63,10 @@ int fem_fill_i32(fem_ctx* ctx, int32_t* dptr, int32_t value, int64_t n);
 
 /* ------------------------------------------------------------------- mesh */
 typedef struct {
-    int32_t dim;        /* 2                                         */
-    int32_t nv_local;   /* 7 (T7)                                    */
-    int32_t np_local;   /* 3 (P1-disc)                               */
-    int32_t nq;         /* 7 (Dunavant degree 5)                     */
+    int32_t dim;        /* 2 or 3                                    */
+    int32_t nv_local;   /* 7 (T7) or 27 (Hex27)                      */
+    int32_t np_local;   /* 3 (P1-disc) or 4 (P1-disc modes on Hex27) */
+    int32_t nq;         /* 7 (Dunavant degree 5) or 27 (3x3x3 Gauss) */
     int64_t nnodes;     /* velocity nodes                            */
     int64_t nels;       /* elements                                  */
     int64_t nnodesP;    /* pressure DoFs = 3 nels                    */
@@ -81,14 +81,20 @@ typedef struct {
  * and pressure DoFs on the device. */
 int fem_mesh_t7_create(fem_ctx* ctx, int32_t nx, int32_t ny, double x0, double x1,
                        double y0, double y1, fem_mesh** out);
+/* Mesh(backend, (x0..x1)x(y0..y1)x(z0..z1), ReferenceElement(QuadraticElement{3,27}), (nx,ny,nz))
+ * followed by MixedMesh(mesh, ReferenceElement(LinearElement{3,4})) (mesh.jl:340-398,
+ * connectivity.jl:273-322, mixed_mesh.jl:358-417): structured Hex27 velocity mesh,
+ * discontinuous pressure with the four modes (1-ξ-η-ζ, ξ, η, ζ) per cell (DoF 4e+a). */
+int fem_mesh_hex27_create(fem_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double x0, double x1,
+                          double y0, double y1, double z0, double z1, fem_mesh** out);
 /* Unstructured T7 mesh: Mesh(backend, coords, el2n, element) (mesh.jl:106-115)
  * + MixedMesh. host_el2n_1based is [nels][7]; local node 7 must be the
  * element's private centroid (bubble) node. */
 int fem_mesh_t7_upload(fem_ctx* ctx, const double* host_coords, int64_t nnodes,
                        const int32_t* host_el2n_1based, int64_t nels, fem_mesh** out);
 int fem_mesh_get_info(const fem_mesh* mesh, fem_mesh_info* out);
-int fem_mesh_get_coords(const fem_mesh* mesh, double* host_out /* [nnodes][2] */);
-int fem_mesh_get_el2n(const fem_mesh* mesh, int32_t* host_out_1based /* [nels][7] */);
+int fem_mesh_get_coords(const fem_mesh* mesh, double* host_out /* [nnodes][dim] */);
+int fem_mesh_get_el2n(const fem_mesh* mesh, int32_t* host_out_1based /* [nels][nv_local] */);
 /* Γnodes of a structured box mesh (mesh.jl:79-80), 1-based */
 int fem_mesh_get_boundary_nodes(const fem_mesh* mesh, int32_t* host_out_1based);
 int fem_mesh_destroy(fem_mesh* mesh);
@@ -146,6 +152,11 @@ enum {
     FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_XY,
     FEM_FIELD_P, FEM_FIELD_P0, FEM_FIELD_GAMMA_P, FEM_FIELD_T, FEM_FIELD_T0, FEM_FIELD_Q,
     FEM_FIELD_PF, FEM_FIELD_RP, FEM_FIELD_RP0, FEM_FIELD_M_P, FEM_FIELD_PNUM,
+    /* 3-D states only (zero length in 2-D): z components and the remaining
+     * stress components (VectorField3D, SymmetricTensor3D) */
+    FEM_FIELD_V_Z, FEM_FIELD_DVDTAU_Z, FEM_FIELD_RV_Z, FEM_FIELD_RV0_Z, FEM_FIELD_DRVDV_Z, FEM_FIELD_PC_V_Z,
+    FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XZ, FEM_FIELD_TAU_YZ,
+    FEM_FIELD_TAU_OLD_ZZ, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ,
     FEM_FIELD_COUNT
 };
 /* device pointer and length of one field; dtype: 0 = f64 */
