@@ -30,32 +30,29 @@ constexpr int8_t kHexR[HNV][3] = {
     {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1}, {1, 1, 0}, {1, 0, 1}, {2, 1, 1}, {1, 2, 1},
     {0, 1, 1}, {1, 1, 2}, {1, 1, 1}};
 
-// 1-D quadratic Lagrange basis at the 3-point Gauss rule
-// (shape_functions.jl:173-196, integration_points.jl:321-337), filled on the
-// host: L[node][point], dL[node][point], x[point], w[point]
-struct Hex1D {
-    double L[3][3], dL[3][3], x[3], w[3];
-};
-Hex1D make_hex1d() {
-    Hex1D t;
-    const double a = std::sqrt(3.0 / 5.0);
-    const double pts[3] = {-a, 0.0, a}, ws[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
-    for (int p = 0; p < 3; ++p) {
-        const double x = pts[p];
-        t.L[0][p] = x * (x - 1) * 0.5; t.L[1][p] = 1 - x * x; t.L[2][p] = x * (x + 1) * 0.5;
-        t.dL[0][p] = x - 0.5; t.dL[1][p] = -2 * x; t.dL[2][p] = x + 0.5;
-        t.x[p] = x;
-        t.w[p] = ws[p];
-    }
-    return t;
+// local node number of the node at tensor position t = i + 3 j + 9 k (the
+// inverse of kHexR); the centre node 27 sits at t = 13
+constexpr int8_t kHexA[HNV] = {0, 8, 1, 11, 20, 9, 3, 10, 2, 16, 21, 17, 24, 26, 22,
+                               19, 23, 18, 4, 12, 5, 15, 25, 13, 7, 14, 6};
+constexpr int HCENTRE_T = 13;
+
+// 3-point Gauss rule (integration_points.jl:321-337): ±√(3/5), 0; 5/9, 8/9
+__host__ __device__ inline double gauss_x(int p) {
+    return p == 0 ? -0x1.8c97ef43f7248p-1 : (p == 1 ? 0.0 : 0x1.8c97ef43f7248p-1);
 }
+__host__ __device__ inline double gauss_w(int p) { return p == 1 ? 8.0 / 9.0 : 5.0 / 9.0; }
+// 1-D quadratic Lagrange basis on -1, 0, +1 and its derivative
+// (shape_functions.jl:173-196)
+__host__ __device__ inline double lq(double x, int r) {
+    return r == 0 ? x * (x - 1) * 0.5 : (r == 1 ? 1 - x * x : x * (x + 1) * 0.5);
+}
+__host__ __device__ inline double dlq(double x, int r) { return r == 0 ? x - 0.5 : (r == 1 ? -2 * x : x + 0.5); }
 
 struct HexGeo {
     int64_t nels;
     int32_t nx, ny, nz;
     int64_t sx, sxy;                 // refined-grid strides
     double jac[3], det;              // ∂ξ/∂x per axis, |det J|
-    Hex1D t;
 };
 
 HexGeo hex_geo(const fem_mesh* m) {
@@ -68,7 +65,6 @@ HexGeo hex_geo(const fem_mesh* m) {
                  hz = (m->box[5] - m->box[4]) / m->nz;
     g.jac[0] = 2 / hx; g.jac[1] = 2 / hy; g.jac[2] = 2 / hz;
     g.det = (hx / 2) * (hy / 2) * (hz / 2);
-    g.t = make_hex1d();
     return g;
 }
 
@@ -94,9 +90,10 @@ struct Hex27Args {
     double *Rv0_x, *Rv0_y, *Rv0_z;         // nullable
 };
 
-__device__ inline int64_t hex_node(const HexGeo& g, int64_t e, int a) {
+// global node of element e at tensor position t = i + 3 j + 9 k
+__device__ inline int64_t hex_node(const HexGeo& g, int64_t e, int t) {
     const int64_t ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / ((int64_t)g.nx * g.ny);
-    return (2 * ez + kHexR[a][2]) * g.sxy + (2 * ey + kHexR[a][1]) * g.sx + (2 * ex + kHexR[a][0]);
+    return (2 * ez + t / 9) * g.sxy + (2 * ey + (t / 3) % 3) * g.sx + (2 * ex + t % 3);
 }
 
 // forward-mode dual with six partials (the strain-rate inputs of stress3d)
@@ -186,76 +183,106 @@ __device__ inline double group_sum(double v) {
 struct Lane1D {
     double L0[3], L1[3], L2[3], d0[3], d1[3], d2[3];
 };
-// lane = point q: L0[r] = L_r(ξ_q) etc.
-__device__ inline Lane1D point_1d(const Hex1D& t, int q) {
+// lane = point q = qi + 3 qj + 9 qk: L0[r] = L_r(x_qi) etc.
+__device__ inline Lane1D point_1d(int q) {
     Lane1D b;
-    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+    const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        b.L0[r] = t.L[r][qi]; b.L1[r] = t.L[r][qj]; b.L2[r] = t.L[r][qk];
-        b.d0[r] = t.dL[r][qi]; b.d1[r] = t.dL[r][qj]; b.d2[r] = t.dL[r][qk];
+        b.L0[r] = lq(x0, r); b.L1[r] = lq(x1, r); b.L2[r] = lq(x2, r);
+        b.d0[r] = dlq(x0, r); b.d1[r] = dlq(x1, r); b.d2[r] = dlq(x2, r);
     }
     return b;
 }
-// lane = node a: L0[p] = L_{r0(a)}(x_p) etc.
-__device__ inline Lane1D node_1d(const Hex1D& t, int a) {
+// lane = node at tensor position t = i + 3 j + 9 k: L0[p] = L_i(x_p) etc.
+__device__ inline Lane1D node_1d(int t) {
     Lane1D b;
-    const int r0 = kHexR[a][0], r1 = kHexR[a][1], r2 = kHexR[a][2];
+    const int i = t % 3, j = (t / 3) % 3, k = t / 9;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        b.L0[p] = t.L[r0][p]; b.L1[p] = t.L[r1][p]; b.L2[p] = t.L[r2][p];
-        b.d0[p] = t.dL[r0][p]; b.d1[p] = t.dL[r1][p]; b.d2[p] = t.dL[r2][p];
+        const double x = gauss_x(p);
+        b.L0[p] = lq(x, i); b.L1[p] = lq(x, j); b.L2[p] = lq(x, k);
+        b.d0[p] = dlq(x, i); b.d1[p] = dlq(x, j); b.d2[p] = dlq(x, k);
     }
     return b;
 }
 
-// velocity gradient at the lane's point: gv[i][k] = ∂v_i/∂x_k (sv: [27][3])
-__device__ inline void point_gradient(const HexGeo& g, const Lane1D& b, const double* sv, double gv[3][3]) {
+// velocity gradient at the lane's point, gv[c][d] = ∂v_c/∂x_d, from the
+// tensor-ordered nodal values sv[t][3], factorised over the three axes:
+// ∂/∂ξ = Σ_jk L_j L_k Σ_i L'_i v_ijk, ∂/∂η and ∂/∂ζ from Σ_i L_i v_ijk
+// (the ζ loop stays rolled, with its basis values recomputed, so that the
+// compiler does not hold all 81 nodal values in registers at once)
+__device__ inline void point_gradient(const HexGeo& g, const Lane1D& b, int q, const double* sv, double gv[3][3]) {
+    double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+    const double x2 = gauss_x(q / 9);
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        const double L2k = lq(x2, k), d2k = dlq(x2, k);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) gv[i][0] = gv[i][1] = gv[i][2] = 0;
+        for (int j = 0; j < 3; ++j) {
+            const double wLL = b.L1[j] * L2k, wdL = b.d1[j] * L2k, wLd = b.L1[j] * d2k;
+            const double* v = sv + (3 * j + 9 * k) * 3;
 #pragma unroll
-    for (int a = 0; a < HNV; ++a) {
-        const int r0 = kHexR[a][0], r1 = kHexR[a][1], r2 = kHexR[a][2];
-        const double G0 = (b.d0[r0] * b.L1[r1]) * b.L2[r2] * g.jac[0];
-        const double G1 = (b.L0[r0] * b.d1[r1]) * b.L2[r2] * g.jac[1];
-        const double G2 = (b.L0[r0] * b.L1[r1]) * b.d2[r2] * g.jac[2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const double v = sv[a * 3 + i];
-            gv[i][0] += G0 * v;
-            gv[i][1] += G1 * v;
-            gv[i][2] += G2 * v;
+            for (int c = 0; c < 3; ++c) {
+                const double s0 = b.d0[0] * v[c] + b.d0[1] * v[3 + c] + b.d0[2] * v[6 + c];
+                const double s1 = b.L0[0] * v[c] + b.L0[1] * v[3 + c] + b.L0[2] * v[6 + c];
+                a0[c] += wLL * s0;
+                a1[c] += wdL * s1;
+                a2[c] += wLd * s1;
+            }
         }
     }
-}
-
-// P1 modes at a point: (1 - ξ - η - ζ, ξ, η, ζ)
-__device__ inline void point_modes(const Hex1D& t, int q, double NP[HNP], double& dOm, double det) {
-    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
-    const double xi = t.x[qi], eta = t.x[qj], zeta = t.x[qk];
-    NP[0] = 1 - xi - eta - zeta; NP[1] = xi; NP[2] = eta; NP[3] = zeta;
-    dOm = det * ((t.w[qi] * t.w[qj]) * t.w[qk]);
-}
-
-// nodal residual of lane node a from the point fluxes in LDS
-// (accumulate_momentum, momentum_residuals.jl:28-33): sf[q][0..7] =
-// (τxx - P, τyy - P, τzz - P, τxy, τxz, τyz, ρ, dΩ)
-__device__ inline void node_residual(const HexGeo& g, const Lane1D& b, const double* sf, double gx, double gy,
-                                     double gz, double R[3]) {
-    R[0] = R[1] = R[2] = 0;
 #pragma unroll
-    for (int q = 0; q < HNQ; ++q) {
-        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
-        const double N = (b.L0[qi] * b.L1[qj]) * b.L2[qk];
-        const double G0 = (b.d0[qi] * b.L1[qj]) * b.L2[qk] * g.jac[0];
-        const double G1 = (b.L0[qi] * b.d1[qj]) * b.L2[qk] * g.jac[1];
-        const double G2 = (b.L0[qi] * b.L1[qj]) * b.d2[qk] * g.jac[2];
-        const double* f = sf + q * 8;
-        const double rho = f[6], dOm = f[7];
-        R[0] += ((G0 * f[0] + G1 * f[3] + G2 * f[4]) - N * (rho * gx)) * dOm;
-        R[1] += ((G0 * f[3] + G1 * f[1] + G2 * f[5]) - N * (rho * gy)) * dOm;
-        R[2] += ((G0 * f[4] + G1 * f[5] + G2 * f[2]) - N * (rho * gz)) * dOm;
+    for (int c = 0; c < 3; ++c) {
+        gv[c][0] = a0[c] * g.jac[0];
+        gv[c][1] = a1[c] * g.jac[1];
+        gv[c][2] = a2[c] * g.jac[2];
     }
+}
+
+// P1 modes at a point: (1 - ξ - η - ζ, ξ, η, ζ), and the point weight |det J| w
+__device__ inline void point_modes(int q, double NP[HNP], double& dOm, double det) {
+    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+    const double xi = gauss_x(qi), eta = gauss_x(qj), zeta = gauss_x(qk);
+    NP[0] = 1 - xi - eta - zeta; NP[1] = xi; NP[2] = eta; NP[3] = zeta;
+    dOm = det * ((gauss_w(qi) * gauss_w(qj)) * gauss_w(qk));
+}
+
+// nodal residual of the lane's node from the point fluxes in LDS
+// (accumulate_momentum, momentum_residuals.jl:28-33), factorised over the
+// axes like point_gradient. sf[q][0..6] = dΩ·(τxx - P, τyy - P, τzz - P,
+// τxy, τxz, τyz, ρ)
+__device__ inline void node_residual(const HexGeo& g, const Lane1D& b, int t, const double* sf, double gx, double gy,
+                                     double gz, double R[3]) {
+    double rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0}, rr = 0;   // ξ, η, ζ parts per row
+    const int kn = t / 9;
+#pragma unroll 1
+    for (int r = 0; r < 3; ++r) {
+        const double xr = gauss_x(r), L2r = lq(xr, kn), d2r = dlq(xr, kn);
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+            const double* f0 = sf + (3 * qq + 9 * r) * 8;
+            const double* f1 = f0 + 8;
+            const double* f2 = f0 + 16;
+            auto dsum = [&](int k) { return b.d0[0] * f0[k] + b.d0[1] * f1[k] + b.d0[2] * f2[k]; };
+            auto lsum = [&](int k) { return b.L0[0] * f0[k] + b.L0[1] * f1[k] + b.L0[2] * f2[k]; };
+            const double wLL = b.L1[qq] * L2r, wdL = b.d1[qq] * L2r, wLd = b.L1[qq] * d2r;
+            rx[0] += wLL * dsum(0);
+            ry[0] += wLL * dsum(3);
+            rz[0] += wLL * dsum(4);
+            const double lxy = lsum(3), lyz = lsum(5);
+            rx[1] += wdL * lxy;
+            ry[1] += wdL * lsum(1);
+            rz[1] += wdL * lyz;
+            rx[2] += wLd * lsum(4);
+            ry[2] += wLd * lyz;
+            rz[2] += wLd * lsum(2);
+            rr += wLL * lsum(6);
+        }
+    }
+    R[0] = (rx[0] * g.jac[0] + rx[1] * g.jac[1] + rx[2] * g.jac[2]) - gx * rr;
+    R[1] = (ry[0] * g.jac[0] + ry[1] * g.jac[1] + ry[2] * g.jac[2]) - gy * rr;
+    R[2] = (rz[0] * g.jac[0] + rz[1] * g.jac[1] + rz[2] * g.jac[2]) - gz * rr;
 }
 
 // element-group bookkeeping of a 256-thread block: 8 elements, 32 lanes each
@@ -286,7 +313,7 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
     const HexGeo& g = A.g;
     const HexLane h = hex_lane(g.nels);
     const int64_t e = h.valid ? h.e : g.nels - 1;
-    // phase 0: nodal velocities (lane = node)
+    // phase 0: nodal velocities (lane = node at tensor position h.l)
     int64_t node = 0;
     if (h.l < HNV) {
         node = hex_node(g, e, h.l);
@@ -302,11 +329,11 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
     for (int i = 0; i < HNP; ++i) P[i] = A.P[e * HNP + i];
     {
         const int q = h.l < HNQ ? h.l : 0;
-        const Lane1D b = point_1d(g.t, q);
+        const Lane1D b = point_1d(q);
         double gv[3][3];
-        point_gradient(g, b, s_v[h.es], gv);
+        point_gradient(g, b, q, s_v[h.es], gv);
         double NP[HNP], dOm;
-        point_modes(g.t, q, NP, dOm, g.det);
+        point_modes(q, NP, dOm, g.det);
         // continuity contributions of this point, then the lane-group sum
         const double div = gv[0][0] + gv[1][1] + gv[2][2];
         double dPdt = 0;
@@ -356,20 +383,20 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
         const double rho = rho0 * ((1 - al * (0.0 - A.Tref)) + be * Pq);
         if (h.l < HNQ) {
             double* f = s_f[h.es] + q * 8;
-            f[0] = tau[0] - Ptot; f[1] = tau[1] - Ptot; f[2] = tau[2] - Ptot;
-            f[3] = tau[3]; f[4] = tau[4]; f[5] = tau[5];
-            f[6] = rho; f[7] = dOm;
+            f[0] = (tau[0] - Ptot) * dOm; f[1] = (tau[1] - Ptot) * dOm; f[2] = (tau[2] - Ptot) * dOm;
+            f[3] = tau[3] * dOm; f[4] = tau[4] * dOm; f[5] = tau[5] * dOm;
+            f[6] = rho * dOm;
         }
     }
     __syncthreads();
-    // phase B: nodal residuals (lane = node)
+    // phase B: nodal residuals (lane = node at tensor position t)
     if (!h.active) return;
-    const int a = h.l;
-    const Lane1D b = node_1d(g.t, a);
+    const int t = h.l;
+    const Lane1D b = node_1d(t);
     double R[3];
-    node_residual(g, b, s_f[h.es], A.gx, A.gy, A.gz, R);
-    if (a < HNSH) {
-        double* s = A.slot + ((e * HNSH + a) * 3);
+    node_residual(g, b, t, s_f[h.es], A.gx, A.gy, A.gz, R);
+    if (t != HCENTRE_T) {
+        double* s = A.slot + ((e * HNSH + kHexA[t]) * 3);
         s[0] = R[0]; s[1] = R[1]; s[2] = R[2];
         return;
     }
@@ -382,9 +409,9 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
         double nrx = A.bx * A.rate_x[node] + rx / A.PCx[node];
         double nry = A.by * A.rate_y[node] + ry / A.PCy[node];
         double nrz = A.bz * A.rate_z[node] + rz / A.PCz[node];
-        double nvx = s_v[h.es][a * 3 + 0] + (-A.ax) * nrx;
-        double nvy = s_v[h.es][a * 3 + 1] + (-A.ay) * nry;
-        double nvz = s_v[h.es][a * 3 + 2] + (-A.az) * nrz;
+        double nvx = s_v[h.es][t * 3 + 0] + (-A.ax) * nrx;
+        double nvy = s_v[h.es][t * 3 + 1] + (-A.ay) * nry;
+        double nvz = s_v[h.es][t * 3 + 2] + (-A.az) * nrz;
         if (fl & 1) { nrx = 0; nvx = A.bcval_x[node]; }
         if (fl & 2) { nry = 0; nvy = A.bcval_y[node]; }
         if (fl & 4) { nrz = 0; nvz = A.bcval_z[node]; }
@@ -467,14 +494,14 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_jacobian(Jac3Args A) {
     __syncthreads();
     const int pc = A.phase[e];
     const int q = h.l < HNQ ? h.l : 0;
-    const Lane1D bq = point_1d(g.t, q);
+    const Lane1D bq = point_1d(q);
     double NP[HNP], dOm;
-    point_modes(g.t, q, NP, dOm, g.det);
+    point_modes(q, NP, dOm, g.det);
     // tangent of the stress at this point: T[k][m] = ∂τ_k/∂in_m
     double T[6][6];
     {
         double gv[3][3];
-        point_gradient(g, bq, s_v[h.es], gv);
+        point_gradient(g, bq, q, s_v[h.es], gv);
         const IpMat<double> m = hex_ipmat(A.ptab, pc);
         double to[6] = {0, 0, 0, 0, 0, 0};
         if (ELASTIC && h.l < HNQ) {
@@ -498,14 +525,15 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_jacobian(Jac3Args A) {
     double ps[HNP];
 #pragma unroll
     for (int i = 0; i < HNP; ++i) ps[i] = A.pscale[e * HNP + i];
-    const int a = h.l < HNV ? h.l : 0;
-    const Lane1D bn = node_1d(g.t, a);
+    const int t = h.l < HNV ? h.l : 0;   // lane's node (tensor position) in phase B
+    const Lane1D bn = node_1d(t);
     double rs[3] = {0, 0, 0}, dg[3] = {0, 0, 0};
+    // columns (velocity component j, node at tensor position tb)
     for (int col = 0; col < 3 * HNV; ++col) {
-        const int j = col / HNV, bnode = col % HNV;
+        const int j = col / HNV, tb = col % HNV;
         // phase A (lane = point): derivative of the point flux
         {
-            const int r0 = kHexR[bnode][0], r1 = kHexR[bnode][1], r2 = kHexR[bnode][2];
+            const int r0 = tb % 3, r1 = (tb / 3) % 3, r2 = tb / 9;
             const double Gb[3] = {(bq.d0[r0] * bq.L1[r1]) * bq.L2[r2] * g.jac[0],
                                   (bq.L0[r0] * bq.d1[r1]) * bq.L2[r2] * g.jac[1],
                                   (bq.L0[r0] * bq.L1[r1]) * bq.d2[r2] * g.jac[2]};
@@ -529,23 +557,23 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_jacobian(Jac3Args A) {
             }
             if (h.l < HNQ) {
                 double* f = s_f[h.es] + q * 8;
-                f[0] = dt_[0] - dPn; f[1] = dt_[1] - dPn; f[2] = dt_[2] - dPn;
-                f[3] = dt_[3]; f[4] = dt_[4]; f[5] = dt_[5];
-                f[6] = 0.0; f[7] = dOm;
+                f[0] = (dt_[0] - dPn) * dOm; f[1] = (dt_[1] - dPn) * dOm; f[2] = (dt_[2] - dPn) * dOm;
+                f[3] = dt_[3] * dOm; f[4] = dt_[4] * dOm; f[5] = dt_[5] * dOm;
+                f[6] = 0.0;
             }
         }
         __syncthreads();
         // phase B (lane = node): entries J[(i, a), (j, b)]
         double Jc[3];
-        node_residual(g, bn, s_f[h.es], 0.0, 0.0, 0.0, Jc);
+        node_residual(g, bn, t, s_f[h.es], 0.0, 0.0, 0.0, Jc);
 #pragma unroll
         for (int i = 0; i < 3; ++i) rs[i] += fabs(Jc[i]);
-        if (a == bnode) dg[j] = fabs(Jc[j]);
+        if (t == tb) dg[j] = fabs(Jc[j]);
         __syncthreads();
     }
     if (!h.active) return;
-    if (a < HNSH) {
-        double* s = A.jslot + (e * HNSH + a) * 6;
+    if (t != HCENTRE_T) {
+        double* s = A.jslot + (e * HNSH + kHexA[t]) * 6;
         s[0] = rs[0]; s[1] = rs[1]; s[2] = rs[2];
         s[3] = dg[0]; s[4] = dg[1]; s[5] = dg[2];
         return;
@@ -586,12 +614,13 @@ __global__ void k_hex27_scaling(HexGeo g, const int32_t* phase, const double* pt
     const double invK = 1.0 / bulk;
     double M[HNP] = {0, 0, 0, 0}, G[HNP] = {0, 0, 0, 0};
     for (int q = 0; q < HNQ; ++q) {
-        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
-        // interp2ip_phase with the velocity basis (partition of unity)
+        const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
+        // interp2ip_phase with the velocity basis (partition of unity), in
+        // the local node order
         double etaq = 0, bq = 0;
 #pragma unroll
         for (int a = 0; a < HNV; ++a) {
-            const double N = (g.t.L[kHexR[a][0]][qi] * g.t.L[kHexR[a][1]][qj]) * g.t.L[kHexR[a][2]][qk];
+            const double N = (lq(x0, kHexR[a][0]) * lq(x1, kHexR[a][1])) * lq(x2, kHexR[a][2]);
             etaq += N * eta;
             bq += N * invK;
         }
@@ -599,7 +628,7 @@ __global__ void k_hex27_scaling(HexGeo g, const int32_t* phase, const double* pt
         const double gphy = bq == 0 ? gnum : dt / bq;
         const double gq = gphy * gnum / (gphy + gnum);
         double NP[HNP], dOm;
-        point_modes(g.t, q, NP, dOm, g.det);
+        point_modes(q, NP, dOm, g.det);
 #pragma unroll
         for (int i = 0; i < HNP; ++i) {
             const double w = NP[i] * NP[i] * dOm;

@@ -134,3 +134,90 @@ def vep_inclusions(n=512, ninc=20, seed=0, nsteps=10, eta_reg=8e-3, Kb=100.0, K=
                                  eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph),
                     pressure_factor=20.0, rel_drop0=0.75, check_interval=100, CFL_v=0.99,
                     nsteps=nsteps, seed_pure_shear=True, extra=dict(seed=seed))
+
+
+# ---------------------------------------------------------------------------
+# 3-D: Hex27 velocity / P1-disc (four modes) pressure on the unit cube
+# ---------------------------------------------------------------------------
+@dataclass
+class Workload3:
+    name: str
+    n: int
+    eta: tuple
+    etab: tuple
+    G: tuple
+    rho0: tuple
+    K: tuple
+    g: tuple
+    dt: float
+    spheres: list
+    plastic: dict | None = None
+    tolerance: float = 1e-7
+    check_interval: int = 100
+    pressure_factor: float = 50.0
+    rel_drop0: float = 1e-2
+    extra: dict = field(default_factory=dict)
+
+    def solve_kwargs(self):
+        return dict(tolerance=self.tolerance, check_interval=self.check_interval,
+                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0)
+
+    def sizes(self):
+        s = 2 * self.n + 1
+        ne = self.n ** 3
+        return s ** 3, ne, 4 * ne
+
+    def dofs(self):
+        nn, ne, npp = self.sizes()
+        return 3 * nn + npp
+
+
+def hex27_sphere_phases(coords, el2n, spheres):
+    """1-based cell phases from the cell centre (local node 27): 1 = matrix,
+    k+1 = k-th sphere (later spheres win)."""
+    c = coords[el2n[:, 26]]
+    ph = np.ones(len(el2n), np.int32)
+    for k, (cx, cy, cz, r) in enumerate(spheres):
+        inside = (c[:, 0] - cx) ** 2 + (c[:, 1] - cy) ** 2 + (c[:, 2] - cz) ** 2 <= r * r
+        ph[inside] = k + 2
+    return ph
+
+
+def pure_shear_bc3(coords, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), edot=1.0):
+    """Every wall node carries the far-field pure shear (ε̇(x - xc), 0,
+    -ε̇(z - zc)) in all three components
+    (stokes_3D_viscous_inclusion.jl:237-243). Returns three (dofs, vals)."""
+    on = np.zeros(len(coords), bool)
+    tol = 32 * np.finfo(float).eps * max(1.0, *(abs(v) for v in box))
+    for d in range(3):
+        on |= np.abs(coords[:, d] - box[2 * d]) <= tol
+        on |= np.abs(coords[:, d] - box[2 * d + 1]) <= tol
+    w = np.nonzero(on)[0].astype(np.int32)
+    v = pure_shear_field3(coords[w], box, edot)
+    return tuple((w, v[c]) for c in range(3))
+
+
+def pure_shear_field3(coords, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), edot=1.0):
+    xm, zm = 0.5 * (box[0] + box[1]), 0.5 * (box[4] + box[5])
+    return edot * (coords[:, 0] - xm), np.zeros(len(coords)), -edot * (coords[:, 2] - zm)
+
+
+def hex27_inclusions(n=64, nsph=50, seed=0, plastic=False, eta_reg=8e-3, Kb=100.0, C=1.0) -> Workload3:
+    """Viscous (or Drucker-Prager) multi-sphere workload on an n^3 Hex27 grid:
+    spheres r ∈ U[0.02, 0.08], log10 η ∈ U[-1, 3], matrix η = 1, pure shear,
+    no gravity, incompressible (ηb = K = ∞); with plasticity G = 1, Δt = 0.1,
+    C = 1, ϕ = 30°, Ψ = 0, finite Kb."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 1, (nsph, 3))
+    r = rng.uniform(0.02, 0.08, nsph)
+    leta = rng.uniform(-1, 3, nsph)
+    inf = math.inf
+    nph = nsph + 1
+    eta = (1.0,) + tuple(10.0 ** leta)
+    G = (1.0,) * nph if plastic else (inf,) * nph
+    return Workload3(name=f"Hex27/P1-disc {n}^3 {'VEP' if plastic else 'viscous'} {nsph} spheres", n=n, eta=eta,
+                     etab=(inf,) * nph, G=G, rho0=(0.0,) * nph, K=(inf,) * nph, g=(0.0, 0.0, 0.0),
+                     dt=0.1 if plastic else 1.0, spheres=[(*c[k], r[k]) for k in range(nsph)],
+                     plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(C,) * nph,
+                                  eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph) if plastic else None,
+                     extra=dict(seed=seed))

@@ -226,7 +226,10 @@ def test_solve_parity(case):
     if case == "viscous_inclusion":
         ph = np.ones(ne, np.int32)
         ph[[4, 13]] = 2
-        args = dict(eta=(1.0, 20.0), rho0=(1.0, 1.5), g=(0.0, 0.0, -0.5), phases=ph)
+        # a contrast of 5: at 20 the oracle's own iteration count moves by a
+        # check interval under a 1e-16 perturbation of the start state (the
+        # exit test at a check is borderline there), at 5 it does not
+        args = dict(eta=(1.0, 5.0), rho0=(1.0, 1.5), g=(0.0, 0.0, -0.5), phases=ph)
         tau = None
     elif case == "ve":
         args = dict(eta=(1.0,), G=(2.0,))
