@@ -566,6 +566,18 @@ struct LaunchOpts {
 
 constexpr int EBLOCK = 128;
 constexpr int GBLOCK = 256;
+constexpr int TBLOCK = 256;   // EF_TILE blocks: 16 x 8 cells
+#ifndef FEM_TILES
+#define FEM_TILES 1           // 0: FP32 inner iterations use the element-slot gather everywhere
+#endif
+
+// FP32 inner iterations of structured 2-D meshes whose cell counts are
+// multiples of the 16 x 8 tile run the EF_TILE element kernel
+static bool tile_ok(const fem_mesh* m) {
+    return FEM_TILES && m->dim == 2 && m->structured && m->nx > 0 && m->nx % 16 == 0 && m->ny % (TBLOCK / 32) == 0;
+}
+static int64_t tile_count(const fem_mesh* m) { return (int64_t)(m->nx / 16) * (m->ny / (TBLOCK / 32)); }
+static int tile_perimeter_nodes() { return 2 * (32 + 1) + 2 * (2 * (TBLOCK / 32) - 1); }
 
 // One set of DR arrays in precision F. The FP64 solve runs on the state's own
 // fields; a correction solve of the mixed-precision path runs on FP32 work
@@ -586,6 +598,9 @@ struct DRSet {
     const F *rhs_P = nullptr, *rhs_vx = nullptr, *rhs_vy = nullptr;
     const double *xvx = nullptr, *xvy = nullptr, *xP = nullptr, *xPf = nullptr;
     const uint8_t* hot = nullptr;   // near-yield flags of the elements (EF_DELTA)
+    typename Vec2<F>::T* bpart = nullptr;   // EF_TILE perimeter partial sums
+    typename Vec2<F>::T* Rn = nullptr;      // EF_TILE interior-node sums
+    bool tile = false;    // inner iterations run the EF_TILE element kernel
     bool lin = false;     // correction operator (EF_LIN)
     bool delta = false;   // ... with the return map about the state x (EF_DELTA)
 };
@@ -623,14 +638,21 @@ static DRSet<float> set32(fem_stokes* s) {
     d.xvx = s->f[FEM_FIELD_V_X]; d.xvy = s->f[FEM_FIELD_V_Y]; d.xP = s->f[FEM_FIELD_P];
     d.xPf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
     d.tau_old[0] = w.tau_old0; d.tau_old[1] = w.tau_old1; d.tau_old[2] = w.tau_old2;
+    d.tile = w.bpart != nullptr;
+    d.bpart = w.bpart;
+    d.Rn = w.Rn;
     d.lin = true;
     return d;
 }
 
 template <class F, int FL>
 static void launch_elem_t(fem_stokes* s, const ElemArgs<F>& a) {
-    hipLaunchKernelGGL((k_t7_element<F, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
-                       s->ctx->stream, a);
+    if constexpr ((FL & EF_TILE) != 0)
+        hipLaunchKernelGGL((k_t7_element<F, FL, TBLOCK>), dim3(grid_xcd(a.ntiles, 1)), dim3(TBLOCK), 0,
+                           s->ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_t7_element<F, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
+                           s->ctx->stream, a);
 }
 
 template <int FL>
@@ -641,13 +663,14 @@ static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
 
 // inner iterations on a correction: EF_LIN|EF_ITER with any of EF_BULK,
 // EF_GEN, EF_STRUCT, and the return-map operator EF_DELTA|EF_PLASTIC with or
-// without EF_ELASTIC
+// without EF_ELASTIC; EF_TILE on structured meshes without EF_GEN
 template <int I>
 static void dispatch_lin(int key, fem_stokes* s, const ElemArgs<float>& a) {
-    constexpr bool el = I & 8, delta = I & 16;
+    constexpr bool el = I & 8, delta = I & 16, tile = I & 32;
     constexpr int FL = EF_LIN | EF_ITER | ((I & 1) ? EF_BULK : 0) | ((I & 2) ? EF_GEN : 0) |
-                       ((I & 4) ? EF_STRUCT : 0) | (el ? EF_ELASTIC : 0) | (delta ? (EF_DELTA | EF_PLASTIC) : 0);
-    if constexpr (!el || delta) {
+                       ((I & 4) ? EF_STRUCT : 0) | (el ? EF_ELASTIC : 0) | (delta ? (EF_DELTA | EF_PLASTIC) : 0) |
+                       (tile ? EF_TILE : 0);
+    if constexpr ((!el || delta) && (!tile || ((I & 4) && !(I & 2)))) {
         if (key == FL) { launch_elem_t<float, FL>(s, a); return; }
     }
     if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
@@ -737,7 +760,16 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     } else {
         if (!d.lin || !o.use_pnum)
             return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for inner iterations on a correction only");
-        dispatch_lin<31>(key | EF_LIN | (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0), s, a);
+        const bool tile = d.tile && strct && !gen;
+        if (tile) {
+            a.ntx = m->nx / 16;
+            a.ntiles = tile_count(m);
+            a.bpart = d.bpart;
+            a.Rn = d.Rn;
+        }
+        dispatch_lin<63>(key | EF_LIN | (tile ? EF_TILE : 0) |
+                             (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0),
+                         s, a);
     }
     return check_launch(s->ctx, "k_t7_element");
 }
@@ -768,6 +800,21 @@ static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) 
     g.do_update = o.do_update;
     g.rhs_vx = d.rhs_vx;
     g.rhs_vy = d.rhs_vy;
+    const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
+    if (d.tile && o.use_pnum && m->nx > 0 && !nodeph) {
+        // the tile kernel has summed every grid node's slots
+        TileUpdArgs<F> P;
+        P.g = g;
+        P.ntx = m->nx / 16; P.nty = m->ny / (TBLOCK / 32);
+        P.TH = 2 * (TBLOCK / 32);
+        P.sx = 2 * m->nx + 1;
+        P.nT6 = (int64_t)P.sx * (2 * m->ny + 1);
+        P.bpart = d.bpart;
+        P.Rn = d.Rn;
+        hipLaunchKernelGGL((k_tile_update<F, GBLOCK>), dim3(grid_xcd(P.nT6, GBLOCK)), dim3(GBLOCK), 0,
+                           s->ctx->stream, P);
+        return check_launch(s->ctx, "k_tile_update");
+    }
     // node -> element-slot lists are read from the CSR arrays for every mesh:
     // deriving them from indices on structured meshes lengthens the path to
     // the loads more than the list reads cost
@@ -1118,6 +1165,10 @@ static int ensure_w32(fem_stokes* s, bool delta) {
     for (float** q : pl) CHK(dalloc_t(c, q, np));
     CHK(dalloc_t(c, &w.geo, 5 * ne));
     CHK(dalloc_t(c, &w.Re, NSH * ne));
+    if (tile_ok(m)) {
+        CHK(dalloc_t(c, &w.bpart, tile_count(m) * tile_perimeter_nodes()));
+        CHK(dalloc_t(c, &w.Rn, (int64_t)(2 * m->nx + 1) * (2 * m->ny + 1)));
+    }
     HIPCHK(c, hipMemsetAsync(w.zero_v, 0, nv * 4, c->stream));
     hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(5 * ne, 256)), dim3(256), 0, c->stream, 5 * ne,
                        (const double*)m->d_geo, w.geo, 1.0);

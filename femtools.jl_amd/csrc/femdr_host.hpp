@@ -126,6 +126,8 @@ struct fem_stokes {
         float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
         float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
         float2* Re;
+        float2* bpart;   // per-tile perimeter partial sums (structured meshes with 16 x 8-cell tiles)
+        float2* Rn;      // summed residuals of the tiles' interior grid nodes
         // stress-history copy for the correction operator with the return map
         float *tau_old0, *tau_old1, *tau_old2;
     };

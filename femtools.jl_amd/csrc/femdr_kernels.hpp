@@ -210,7 +210,29 @@ enum ElemFlags {
                       // visco-elastic operator (no τ_old, no return map)
     EF_DELTA = 128,   // with EF_LIN|EF_PLASTIC: the stress term is τ(x + δ) - τ(x), both with
                       // the Drucker-Prager return about the FP64 state x (xvx, xvy, xP)
+    EF_TILE = 256,    // with EF_STRUCT|EF_ITER: a block is a tile of 16 x (BLOCK/32) cells; the
+                      // shared-node residuals are summed in LDS and stored per node for the
+                      // tile's interior nodes, as per-tile partial sums for its perimeter
+                      // nodes; k_tile_update finishes both
 };
+
+// Slot of the T6 grid node at offset pos = ox + 3 oy (ox, oy in 0..2) from a
+// cell's lower-left node in triangle tri of a cell of parity par, or -1
+// (the local node orders of t7_struct_element)
+__device__ constexpr int8_t kT7Slot[2][2][9] = {
+    // even cells: A = {BL, BR, TR, bot, rgt, ctr}, B = {BL, TR, TL, ctr, top, lft}
+    {{0, 3, 1, -1, 5, 4, -1, -1, 2}, {0, -1, -1, 5, 3, -1, 2, 4, 1}},
+    // odd cells:  A = {BL, BR, TL, bot, ctr, lft}, B = {BR, TR, TL, rgt, top, ctr}
+    {{0, 3, 1, 5, 4, -1, 2, -1, -1}, {-1, -1, 0, -1, 5, 3, 2, 4, 1}},
+};
+// perimeter index of tile node (jx, jy) of a tile with TW x TH grid intervals:
+// bottom row, top row, left column, right column
+__device__ inline int tile_perim(int jx, int jy, int TW, int TH) {
+    if (jy == 0) return jx;
+    if (jy == TH) return (TW + 1) + jx;
+    if (jx == 0) return 2 * (TW + 1) + (jy - 1);
+    return 2 * (TW + 1) + (TH - 1) + (jy - 1);
+}
 
 // Structured T7 box mesh (connectivity.jl:111-140): element e = 2c + k of cell
 // c = ey nx + ex; the cell's two triangles alternate diagonals with the cell
@@ -260,6 +282,9 @@ struct ElemArgs {
     F* tau_out0; F* tau_out1; F* tau_out2;
     // EF_STRUCT: nx cells per row, sx = 2 nx + 1 grid nodes per row, nT6 grid nodes
     int32_t nx, sx; int64_t nT6; F idx, idy, detc;
+    // EF_TILE: tiles per row and in all, per-tile partial sums of the perimeter
+    // nodes, summed residuals of the interior nodes (by grid node)
+    int32_t ntx; int64_t ntiles; typename Vec2<F>::T* bpart; typename Vec2<F>::T* Rn;
     int32_t phase_same;        // phase_P holds the same cell phases as phase_v
     // EF_LIN right-hand sides R(x) of the correction problem (element DoFs / nodes)
     const F* rhs_P; const F* rhs_vx; const F* rhs_vy;
@@ -310,18 +335,59 @@ __device__ inline int64_t xcd_tile() {
 #define FEM_ELEM_ATTR
 #endif
 
+// gathered residual (rx, ry) of a shared node: right-hand side, Dirichlet
+// pinning, optional stores and the Chebyshev update (helpers.jl:31-41,
+// dynamic_relaxation/solver.jl:139-166; DR.jl:686-697)
+template <class F>
+__device__ inline void node_update(int64_t node, F rx, F ry, const F* rhs_vx, const F* rhs_vy, const uint8_t* bcflag,
+                                   const F* bcval_x, const F* bcval_y, F* Rv_out_x, F* Rv_out_y, F* Rv0_out_x,
+                                   F* Rv0_out_y, int32_t do_update, F alpha_x, F alpha_y, F beta_x, F beta_y,
+                                   F* rate_x, F* rate_y, const F* PCx, const F* PCy, F* vx, F* vy) {
+    if (rhs_vx) { rx += rhs_vx[node]; ry += rhs_vy[node]; }
+    const uint8_t fl = bcflag[node];
+    if (fl & 1) rx = 0;
+    if (fl & 2) ry = 0;
+    if (Rv_out_x) { Rv_out_x[node] = rx; Rv_out_y[node] = ry; }
+    if (Rv0_out_x) { Rv0_out_x[node] = rx; Rv0_out_y[node] = ry; }
+    if (do_update) {
+        F nrx = beta_x * rate_x[node] + rx / PCx[node];
+        F nry = beta_y * rate_y[node] + ry / PCy[node];
+        F nvx = vx[node] + (-alpha_x) * nrx;
+        F nvy = vy[node] + (-alpha_y) * nry;
+        if (fl & 1) { nrx = 0; nvx = bcval_x[node]; }
+        if (fl & 2) { nry = 0; nvy = bcval_y[node]; }
+        rate_x[node] = nrx; rate_y[node] = nry;
+        vx[node] = nvx; vy[node] = nvy;
+    }
+}
+
 template <class F, int FL, int BLOCK>
 __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
     constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
     constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT, LIN = FL & EF_LIN;
-    constexpr bool DELTA = FL & EF_DELTA;
+    constexpr bool DELTA = FL & EF_DELTA, TILE = FL & EF_TILE;
     static_assert(!(LIN && !DELTA && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
     static_assert(!DELTA || (LIN && PLASTIC), "EF_DELTA is the correction operator with the return map");
+    static_assert(!TILE || (STRUCT && ITER && BLOCK % 64 == 0), "tiles are blocks of structured cells");
     constexpr bool TAU_LDS = ELASTIC && FEM_TAU_LDS;
+    constexpr int TCY = BLOCK / 32;                 // EF_TILE: rows of 16 cells per tile
     const int64_t ne = A.nels;
-    const int64_t e0 = xcd_tile() * BLOCK;          // first element of this block
-    const bool valid = e0 + threadIdx.x < ne;
-    const int64_t e = valid ? e0 + threadIdx.x : ne - 1;   // idle lanes recompute the last element, store nothing
+    int64_t e0, e, tile = 0, ttx = 0, tty = 0;
+    bool valid;
+    if (TILE) {
+        // thread t: triangle t & 1 of cell (t >> 1) & 15 in row t >> 5 of the tile
+        tile = xcd_tile();
+        if (tile >= A.ntiles) return;                // whole block
+        tty = tile / A.ntx;
+        ttx = tile - tty * A.ntx;
+        e0 = 2 * (tty * TCY * (int64_t)A.nx + ttx * 16);   // first element of the tile's first row
+        e = e0 + 2 * (int64_t)(threadIdx.x >> 5) * A.nx + (threadIdx.x & 31);
+        valid = true;
+    } else {
+        e0 = xcd_tile() * BLOCK;                     // first element of this block
+        valid = e0 + threadIdx.x < ne;
+        e = valid ? e0 + threadIdx.x : ne - 1;       // idle lanes recompute the last element, store nothing
+    }
     // per-phase coefficients: a few KB read through the caches
 #define PT(col, p) ((F)A.ptab[(col) * MAXPH + (p)])
     const bool nodeph = GEN && A.nodeph;
@@ -331,7 +397,22 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     __shared__ F s_tau[TAU_LDS ? 3 * BLOCK * NQ : 1];
     bool hot = true;
     if (DELTA) hot = FEM_ALL_HOT || __syncthreads_or(valid && A.hot[e]);
-    if (TAU_LDS && e0 < ne && hot) {
+    if (TAU_LDS && TILE && hot) {
+        // one contiguous chunk of 32 elements per tile row and component
+        constexpr int bytes = 32 * NQ * (int)sizeof(F);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const F* src[3] = {A.tau_old0, A.tau_old1, A.tau_old2};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            for (int r = wave; r < TCY; r += BLOCK / 64) {
+                const char* g = (const char*)(src[c] + (e0 + 2 * (int64_t)r * A.nx) * NQ);
+                char* l = (char*)(s_tau + c * BLOCK * NQ + r * 32 * NQ);
+                for (int off0 = 0; off0 < bytes; off0 += 64 * 16)
+                    if (off0 + lane * 16 < bytes)
+                        __builtin_amdgcn_global_load_lds((const void*)(g + off0 + lane * 16),
+                                                         (__attribute__((address_space(3))) void*)(l + off0), 16, 0, 0);
+            }
+    } else if (TAU_LDS && e0 < ne && hot) {
         const int64_t nvalid = ne - e0 < BLOCK ? ne - e0 : BLOCK;
         const int bytes = (int)(nvalid * NQ * sizeof(F));
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -646,10 +727,51 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
 
     // ---- shared nodes: element-slot residuals for the gather
     typedef typename Vec2<F>::T V2;
+    if (TILE) {
+        // sum each tile node's slots in LDS in ascending element order (cell
+        // rows, cells, triangles); k_tile_update adds the perimeter nodes'
+        // partial sums of the neighbouring tiles and updates every node
+        __shared__ V2 s_re[TILE ? BLOCK * NSH : 1];
 #pragma unroll
-    for (int a = 0; a < NSH; ++a) {
-        V2 r; r.x = Rx[a]; r.y = Ry[a];
-        A.Re[e * NSH + a] = r;
+        for (int a = 0; a < NSH; ++a) {
+            V2 r; r.x = Rx[a]; r.y = Ry[a];
+            s_re[threadIdx.x * NSH + a] = r;
+        }
+        __syncthreads();
+        constexpr int TW = 32, TH = 2 * TCY, NTN = (TW + 1) * (TH + 1), NPER = 2 * (TW + 1) + 2 * (TH - 1);
+        for (int k = threadIdx.x; k < NTN; k += BLOCK) {
+            const int jx = k % (TW + 1), jy = k / (TW + 1);
+            const int lx0 = max((jx - 1) >> 1, 0), lx1 = min(jx >> 1, 15);
+            const int ly0 = max((jy - 1) >> 1, 0), ly1 = min(jy >> 1, TCY - 1);
+            F rx = 0, ry = 0;
+            for (int ly = ly0; ly <= ly1; ++ly)
+                for (int lx = lx0; lx <= lx1; ++lx) {
+                    // tiles start on even cells (16 columns, an even number of rows)
+                    const int par = (lx + ly) & 1, pos = (jx - 2 * lx) + 3 * (jy - 2 * ly);
+#pragma unroll
+                    for (int tri = 0; tri < 2; ++tri) {
+                        const int sl = kT7Slot[par][tri][pos];
+                        if (sl >= 0) {
+                            const V2 r = s_re[(ly * 32 + lx * 2 + tri) * NSH + sl];
+                            rx += r.x;
+                            ry += r.y;
+                        }
+                    }
+                }
+            if (jx == 0 || jx == TW || jy == 0 || jy == TH) {
+                V2 r; r.x = rx; r.y = ry;
+                A.bpart[tile * NPER + tile_perim(jx, jy, TW, TH)] = r;
+                continue;
+            }
+            V2 r; r.x = rx; r.y = ry;
+            A.Rn[(tty * TH + jy) * (int64_t)A.sx + (ttx * TW + jx)] = r;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < NSH; ++a) {
+            V2 r; r.x = Rx[a]; r.y = Ry[a];
+            A.Re[e * NSH + a] = r;
+        }
     }
     // ---- private bubble node: Dirichlet, residual store, Chebyshev update
     {
@@ -706,22 +828,52 @@ __global__ __launch_bounds__(BLOCK) void k_gather_update(GatherArgs<F> A) {
         rx += r.x;
         ry += r.y;
     }
-    if (A.rhs_vx) { rx += A.rhs_vx[node]; ry += A.rhs_vy[node]; }
-    const uint8_t fl = A.bcflag[node];
-    if (fl & 1) rx = 0;
-    if (fl & 2) ry = 0;
-    if (A.Rv_out_x) { A.Rv_out_x[node] = rx; A.Rv_out_y[node] = ry; }
-    if (A.Rv0_out_x) { A.Rv0_out_x[node] = rx; A.Rv0_out_y[node] = ry; }
-    if (A.do_update) {
-        F nrx = A.beta_x * A.rate_x[node] + rx / A.PCx[node];
-        F nry = A.beta_y * A.rate_y[node] + ry / A.PCy[node];
-        F nvx = A.vx[node] + (-A.alpha_x) * nrx;
-        F nvy = A.vy[node] + (-A.alpha_y) * nry;
-        if (fl & 1) { nrx = 0; nvx = A.bcval_x[node]; }
-        if (fl & 2) { nry = 0; nvy = A.bcval_y[node]; }
-        A.rate_x[node] = nrx; A.rate_y[node] = nry;
-        A.vx[node] = nvx; A.vy[node] = nvy;
+    node_update<F>(node, rx, ry, A.rhs_vx, A.rhs_vy, A.bcflag, A.bcval_x, A.bcval_y, A.Rv_out_x, A.Rv_out_y,
+                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.alpha_x, A.alpha_y, A.beta_x, A.beta_y, A.rate_x,
+                   A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
+}
+
+// Grid nodes after the EF_TILE element kernel, one thread per T6 grid node:
+// an interior node of a tile reads its summed residual, a node on a tile line
+// adds the partial sums of the (two or four) tiles holding it in ascending
+// tile order; then the update of k_gather_update.
+template <class F>
+struct TileUpdArgs {
+    GatherArgs<F> g;                  // node data (off, ent, Re, node_of unused)
+    int32_t ntx, nty, TH, sx;         // tiles per row / column, grid rows per tile, 2 nx + 1
+    int64_t nT6;                      // grid nodes
+    const typename Vec2<F>::T* bpart; // [ntiles][perimeter nodes]
+    const typename Vec2<F>::T* Rn;    // [nT6], interior nodes of the tiles
+};
+
+template <class F, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tile_update(TileUpdArgs<F> P) {
+    const int64_t node = xcd_tile() * BLOCK + threadIdx.x;
+    if (node >= P.nT6) return;
+    constexpr int TW = 32;
+    const int TH = P.TH, NPER = 2 * (TW + 1) + 2 * (TH - 1);
+    const int64_t iy = node / P.sx, ix = node - iy * P.sx;
+    F rx = 0, ry = 0;
+    if (ix % TW != 0 && iy % TH != 0) {
+        const typename Vec2<F>::T r = P.Rn[node];
+        rx = r.x;
+        ry = r.y;
+    } else {
+        const int64_t tx1 = ix / TW, ty1 = iy / TH;
+        const int64_t tx0 = (ix % TW == 0 && tx1 > 0) ? tx1 - 1 : tx1;
+        const int64_t ty0 = (iy % TH == 0 && ty1 > 0) ? ty1 - 1 : ty1;
+        for (int64_t ty = ty0; ty <= ty1 && ty < P.nty; ++ty)
+            for (int64_t tx = tx0; tx <= tx1 && tx < P.ntx; ++tx) {
+                const typename Vec2<F>::T r =
+                    P.bpart[(ty * P.ntx + tx) * NPER + tile_perim((int)(ix - tx * TW), (int)(iy - ty * TH), TW, TH)];
+                rx += r.x;
+                ry += r.y;
+            }
     }
+    const GatherArgs<F>& A = P.g;
+    node_update<F>(node, rx, ry, A.rhs_vx, A.rhs_vy, A.bcflag, A.bcval_x, A.bcval_y, A.Rv_out_x, A.Rv_out_y,
+                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.alpha_x, A.alpha_y, A.beta_x, A.beta_y, A.rate_x,
+                   A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
 }
 
 // ---------------------------------------------------------------------------
