@@ -395,8 +395,14 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     // stream it straight into LDS (no registers held) while the velocity
     // gathers and the continuity residual proceed.
     __shared__ F s_tau[TAU_LDS ? 3 * BLOCK * NQ : 1];
-    bool hot = true;
-    if (DELTA) hot = FEM_ALL_HOT || __syncthreads_or(valid && A.hot[e]);
+    // hot: the block stages τ_old (some element is near yield); hot_e: this
+    // element takes the FP64 returns (an element away from yield has none in
+    // either state, so its correction operator is the visco-elastic one)
+    bool hot = true, hot_e = true;
+    if (DELTA) {
+        hot_e = FEM_ALL_HOT || (valid && A.hot[e]);
+        hot = __syncthreads_or(hot_e);
+    }
     if (TAU_LDS && TILE && hot) {
         // one contiguous chunk of 32 elements per tile row and component
         constexpr int bytes = 32 * NQ * (int)sizeof(F);
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             txx = two_eve * (gxx - trd);
             tyy = two_eve * (gyy - trd);
             txy = two_eve * ((gxy + gyx) * F(0.5));
-          if (hot) {
+          if (hot_e) {
             // x's strain rate and the two returns are evaluated in FP64 from the
             // FP64 state, read through the caches in blocks with points near
             // yield only: the stress of x then matches the FP64 residual's
