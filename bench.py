@@ -120,6 +120,50 @@ def cpu_baseline(w, budget_s=15.0):
                       f"threads, {t:.1f} s"}
 
 
+# FP64 flops the factorised Hex27 evaluation executes per element in the
+# viscous DR-step kernel: 27 active lanes x 1234 (velocity gradient at the
+# lane's point 480, continuity / stress / fluxes ~106, nodal residual 648)
+HEX27_FLOPS_PER_ELEMENT = 27 * 1234
+FP64_PEAK_TFLOPS = 78.6
+
+
+def hex27_measure(n, iters):
+    """The 3-D path on its own configuration: n^3 Hex27 cells, 50 seeded
+    spheres, viscous, pure shear; iters DR iterations from the far-field
+    guess (not to convergence) and the element kernel alone timed with HIP
+    events."""
+    import femtools_amd as fa
+    from femtools_amd import problems as PB
+    w = PB.hex27_inclusions(n)
+    mesh = fa.hex27_box_mesh(n, n, n)
+    ph = PB.hex27_sphere_phases(mesh.coords, mesh.el2n, w.spheres)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
+    dr.set_phases("v", ph)
+    dr.set_phases("P", ph)
+    bcs = [fa.DirichletBoundaryCondition(d, v) for d, v in PB.pure_shear_bc3(mesh.coords)]
+    v0 = PB.pure_shear_field3(mesh.coords)
+    fa.assemble_viscosity_weighted_pressure_scaling(dr, w.pressure_factor, w.dt)
+    el_ms = fa.time_element_kernel(dr, w.dt, n=20)
+    for f, v in zip(dr.v, v0):
+        f.copyto(v)
+    dr.P.fill(0.0)
+    mesh.ctx.sync()
+    t0 = time.perf_counter()
+    st = fa.solve(dr, mesh, *bcs, dt=w.dt, max_iterations=iters, iterMax=iters, throw_on_failure=False,
+                  **w.solve_kwargs())
+    mesh.ctx.sync()
+    wall = time.perf_counter() - t0
+    flops = HEX27_FLOPS_PER_ELEMENT * mesh.nels
+    tf = flops / (el_ms * 1e-3) / 1e12
+    return {"workload": w.name + f", {iters} DR iterations from the far-field guess",
+            "elements": mesh.nels, "velocity_nodes": mesh.nnodes, "pressure_dofs": mesh.nnodesP, "dofs": w.dofs(),
+            "iterations": st["iter"], "seconds": round(wall, 4), "ms_per_iter": round(wall / st["iter"] * 1e3, 4),
+            "gdof_iter_per_s": round(w.dofs() * st["iter"] / wall / 1e9, 4),
+            "roofline": {"bound": "fp64 alu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tf / FP64_PEAK_TFLOPS, 4), "kernel": "k_hex27_element (viscous DR step)",
+                         "flops_per_launch": flops, "avg_launch_us": round(el_ms * 1e3, 2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +175,8 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp64", "fp32"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--hex27-n", type=int, default=64, help="cells per side of the 3-D measurement (0: skip)")
+    ap.add_argument("--hex27-iters", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
@@ -263,6 +309,8 @@ def main():
         t64 = sum(s["wall"] for s in stats["fp64"])
         out["speedup_vs_fp64"] = round(t64 / elapsed, 3)
         out["fp32_vs_fp64_max_rel_diff"] = diffs
+    if args.hex27_n > 0:
+        out["hex27"] = hex27_measure(args.hex27_n, args.hex27_iters)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
