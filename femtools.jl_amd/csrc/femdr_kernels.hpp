@@ -563,6 +563,11 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
 #pragma unroll
     for (int a = 0; a < NV; ++a) { Rx[a] = 0; Ry[a] = 0; }
     const bool grav = (A.gx != F(0)) || (A.gy != F(0));
+    // FP32 (correction operator): chained FMAs and the body force summed
+    // once per element after the point loop (ρ dΩ per point in bw); the FP64
+    // path keeps the reference's per-point summation order
+    constexpr bool F32 = std::is_same<F, float>::value;
+    F bw[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         if (nodeph) {
@@ -710,17 +715,37 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             const F rho = LIN ? rho_c * (beta_c * Pq) : rho_c * ((F(1) - alpha_c * (Tq - A.Tref)) + beta_c * Pq);
             bx = rho * A.gx * dOm;
             by = rho * A.gy * dOm;
+            bw[q] = rho * dOm;
         }
+        if (F32) {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
+                Rx[a] = fmaf(d1, s1x, fmaf(d0, s0x, Rx[a]));
+                Ry[a] = fmaf(d1, s1y, fmaf(d0, s0y, Ry[a]));
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < NV; ++a) {
+                const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
+                Rx[a] += d0 * s0x + d1 * s1x;
+                Ry[a] += d0 * s0y + d1 * s1y;
+                if (grav) {
+                    const F Na = (F)kT.N[q][a];
+                    Rx[a] -= Na * bx;
+                    Ry[a] -= Na * by;
+                }
+            }
+        }
+    }
+    if (F32 && grav) {
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
-            const F d0 = (F)kT.dN[q][a][0], d1 = (F)kT.dN[q][a][1];
-            Rx[a] += d0 * s0x + d1 * s1x;
-            Ry[a] += d0 * s0y + d1 * s1y;
-            if (grav) {
-                const F Na = (F)kT.N[q][a];
-                Rx[a] -= Na * bx;
-                Ry[a] -= Na * by;
-            }
+            F m = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) m = fmaf((F)kT.N[q][a], bw[q], m);
+            Rx[a] = fmaf(-A.gx, m, Rx[a]);
+            Ry[a] = fmaf(-A.gy, m, Ry[a]);
         }
     }
 
