@@ -164,6 +164,62 @@ def hex27_measure(n, iters):
                          "flops_per_launch": flops, "avg_launch_us": round(el_ms * 1e3, 2)}}
 
 
+def batch_measure(count, nthreads, precision):
+    """Independent small solves one at a time and then all at once through
+    fem_stokes_solve_batch (one context and stream per problem), each from
+    the same initial guess to residual 1e-8."""
+    import femtools_amd as fa
+    from femtools_amd import problems as PB
+    probs = []
+    for w in PB.batch_workloads(count):
+        ctx = fa.Context(0)
+        mesh = fa.t7_box_mesh(w.nx, w.ny, ctx=ctx)
+        ph = PB.inclusion_phases(mesh.coords, mesh.el2n, w.circles)
+        (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
+        dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
+        dr.set_phases("v", ph)
+        dr.set_phases("P", ph)
+        bcs = [fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)]
+        kw = dict(dt=w.dt, tolerance=1e-8, check_interval=50, max_iterations=400_000, iterMax=400_000,
+                  throw_on_failure=False, precision=precision)
+        probs.append((w, dr, mesh, bcs, kw, PB.pure_shear_field(mesh.coords)))
+
+    def reset():
+        for w, dr, mesh, bcs, kw, v0 in probs:
+            dr.v.x.copyto(v0[0])
+            dr.v.y.copyto(v0[1])
+            dr.P.fill(0.0)
+            mesh.ctx.sync()
+
+    def one(dr, mesh, bcs, kw):
+        try:
+            return fa.solve(dr, mesh, *bcs, **kw)
+        except fa.FemError as e:   # a diverging problem counts as a finished, unconverged solve
+            return {"iter": -1, "converged": False, "error": str(e)}
+
+    reset()
+    t0 = time.perf_counter()
+    seq = [one(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw, v0 in probs]
+    t_seq = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    bat = fa.solve_batch([(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw, v0 in probs], nthreads=nthreads,
+                         raise_errors=False)
+    t_bat = time.perf_counter() - t0
+    dof_it = sum(w.dofs() * max(st["iter"], 0) for (w, *_r), st in zip(probs, bat))
+    return {"workload": f"{count} independent T7/P1-disc viscous problems, n in 32..128, 1-10 inclusions, "
+                        f"log10 eta in U[-2,4], tol 1e-8, {precision}",
+            "sizes": [w.nx for w, *_r in probs], "nthreads": nthreads,
+            "converged": sum(bool(st["converged"]) for st in bat),
+            "failed": sum(st["status"] != 0 for st in bat),
+            "same_iterations_as_sequential": all(a["iter"] == b["iter"] for a, b in zip(seq, bat)
+                                                 if b["status"] == 0 and "error" not in a),
+            "sequential_s": round(t_seq, 3), "batched_s": round(t_bat, 3),
+            "solves_per_s": {"sequential": round(count / t_seq, 3), "batched": round(count / t_bat, 3)},
+            "batched_gdof_iter_per_s": round(dof_it / t_bat / 1e9, 3),
+            "speedup": round(t_seq / t_bat, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +233,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--hex27-n", type=int, default=64, help="cells per side of the 3-D measurement (0: skip)")
     ap.add_argument("--hex27-iters", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=0, help="independent small solves to run batched (0: skip)")
+    ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--batch-precision", choices=("fp64", "fp32"), default="fp32")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
@@ -311,6 +370,8 @@ def main():
         out["fp32_vs_fp64_max_rel_diff"] = diffs
     if args.hex27_n > 0:
         out["hex27"] = hex27_measure(args.hex27_n, args.hex27_iters)
+    if args.batch > 0:
+        out["batched"] = batch_measure(args.batch, args.batch_threads, args.batch_precision)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)

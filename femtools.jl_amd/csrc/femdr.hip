@@ -1541,6 +1541,47 @@ int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, 
     return FEM_OK;
 }
 
+int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
+                     fem_solve_stats* st);
+
+// Independent solves, concurrently: host threads take states off a shared
+// counter, largest mesh first, and run fem_stokes_solve on each; every state
+// has its own context, so each solve runs on its own stream and the GPU
+// overlaps their launches.
+int fem_stokes_solve_batch(int32_t n, fem_stokes* const* states, const fem_bc* const* bcs,
+                           const fem_drucker_prager* const* plastic, const fem_solve_params* params,
+                           fem_solve_stats* stats, int32_t* status, int32_t nthreads) {
+    if (n < 0) return FEM_EARG;
+    if (n == 0) return FEM_OK;
+    if (!states || !bcs || !params || !stats || !status) return FEM_EARG;
+    std::set<fem_ctx*> ctxs;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!states[i] || !bcs[i]) return FEM_EARG;
+        if (!ctxs.insert(states[i]->ctx).second)
+            return set_err(states[i]->ctx, FEM_EARG,
+                           "batched solves need one context (stream) per state; state %d shares its context", i);
+    }
+    std::vector<int32_t> order(n);
+    for (int32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return states[a]->mesh->nels > states[b]->mesh->nels; });
+    const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(nthreads <= 0 ? n : nthreads, n));
+    std::atomic<int32_t> next(0);
+    auto worker = [&]() {
+        for (int32_t k; (k = next.fetch_add(1)) < n;) {
+            const int32_t i = order[k];
+            status[i] = fem_stokes_solve(states[i], bcs[i], plastic ? plastic[i] : nullptr, &params[i], &stats[i]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    for (int32_t i = 0; i < n; ++i)
+        if (status[i] != FEM_OK) return status[i];
+    return FEM_OK;
+}
+
 // solve! + _solve_stokes_dyrel! (DR.jl:302-337, 348-802)
 int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
                      fem_solve_stats* st) {

@@ -26,7 +26,9 @@ __all__ = [
     "FemError", "DimensionMismatch", "Context", "DeviceArray", "VectorField2D", "SymmetricTensor2D",
     "StokesMaterial", "DruckerPrager", "MixedMesh", "t7_box_mesh", "t7_mesh_from_arrays",
     "DirichletBoundaryCondition", "StokesDR", "solve", "update_stokes_current_stress",
-    "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context",
+    "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
+    "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
+    "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -153,6 +155,8 @@ _sig("fem_stokes_destroy", C.c_int, VP)
 _sig("fem_stokes_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
 _sig("fem_stokes_set_phases", C.c_int, VP, I32, I32, _P(I32), I64)
 _sig("fem_stokes_solve", C.c_int, VP, _P(_BC), _P(_DP), _P(_SolveParams), _P(_SolveStats))
+_sig("fem_stokes_solve_batch", C.c_int, I32, _P(VP), _P(_P(_BC)), _P(_P(_DP)), _P(_SolveParams), _P(_SolveStats),
+     _P(I32), I32)
 _sig("fem_stokes_pressure_scaling", C.c_int, VP, D, D, I32)
 _sig("fem_stokes_update_current_stress", C.c_int, VP, _P(_DP), D)
 _sig("fem_stokes_residual", C.c_int, VP, _P(_DP), D, I32, I32)
@@ -169,7 +173,8 @@ EXPORTED = [
     "fem_copy_h2d", "fem_copy_d2h", "fem_copy_d2d", "fem_fill_f64", "fem_fill_f32", "fem_fill_i32",
     "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
     "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
-    "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_pressure_scaling",
+    "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
+    "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
     "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual",
@@ -585,21 +590,12 @@ def solve(dr, mesh, *bcs, **kwargs):
     return solve_stokes(dr, mesh, *bcs, **kwargs)
 
 
-def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition,
-          bc_vz: DirichletBoundaryCondition | None = None, *,
-          dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, check_interval=50, iterMax=50_000,
-          max_ph_iterations=1000, rel_drop0=1e-2, plastic=None, phases_v=None, phases_P=None, finite_K=False,
-          freeze_jacobian=None, throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False,
-          time_kernels=False, precision="fp64"):
-    """solve!(dr, mesh, bc_vx, bc_vy[, bc_vz]; dt, ...) (src/stokes/solvers/DR.jl:302-337).
-    Mutates dr in place and returns the stats dict of DR.jl:783-801. A 3-D
-    state takes three boundary conditions and runs in FP64.
-
-    precision="fp64" runs the reference's iteration in FP64. precision="fp32"
-    keeps the state, the outer residuals, the norms and the convergence test
-    in FP64 (same criterion and tolerance) and runs the inner pseudo-transient
-    iterations in FP32 on a correction of the linearised visco-elastic
-    problem, folded into the FP64 state at every outer iteration."""
+def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_factor=50.0, tolerance=1e-6,
+                 max_iterations=50_000, check_interval=50, iterMax=50_000, max_ph_iterations=1000, rel_drop0=1e-2,
+                 plastic=None, phases_v=None, phases_P=None, finite_K=False, freeze_jacobian=None,
+                 throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False, time_kernels=False,
+                 precision="fp64"):
+    """Checks and C arguments of one solve! call: (bc array, plastic pointer, params)."""
     if precision not in ("fp64", "fp32"):
         raise ValueError(f"precision must be 'fp64' or 'fp32', got {precision!r}")
     if mesh is not dr.mesh:
@@ -608,12 +604,13 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
         dr.set_phases("v", phases_v)
     if phases_P is not None:
         dr.set_phases("P", phases_P)
-    given = [b for b in (bc_vx, bc_vy, bc_vz) if b is not None]
+    given = [b for b in bcs if b is not None]
     if len(given) != mesh.dim:
         raise DimensionMismatch(f"a {mesh.dim}-D solve takes {mesh.dim} boundary conditions, got {len(given)}")
-    keep = []
-    bcs = (_BC * len(given))(*(b._c(keep) for b in given))
+    arr = (_BC * len(given))(*(b._c(keep) for b in given))
+    keep.append(arr)
     d, dp = _dp_ptr(plastic)
+    keep.append(d)
     p = _SolveParams(dt=float(dt), pressure_factor=float(pressure_factor), tolerance=float(tolerance),
                      max_iterations=int(max_iterations), iter_max=int(iterMax), check_interval=int(check_interval),
                      max_ph_iterations=int(max_ph_iterations), rel_drop0=float(rel_drop0), finite_K=int(finite_K),
@@ -621,15 +618,82 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
                      throw_on_failure=int(bool(throw_on_failure)), use_tau_old=int(bool(tau_old)),
                      skip_scaling=int(bool(skip_scaling)), verbose=int(verbose),
                      precision=0 if precision == "fp64" else 1, time_kernels=int(bool(time_kernels)))
-    st = _SolveStats()
-    rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
+    return arr, dp, p
+
+
+def _stats_dict(st):
     stats = {k: getattr(st, k) for k, _ in _SolveStats._fields_}
     stats["converged"] = bool(stats["converged"])
     stats["reached_total_iter"] = bool(stats["reached_total_iter"])
     stats["iterations"] = stats["iter"]
     stats["residual"] = stats["err_abs"]
+    return stats
+
+
+def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition, bc_vy: DirichletBoundaryCondition,
+                 bc_vz: DirichletBoundaryCondition | None = None, **kwargs):
+    """solve!(dr, mesh, bc_vx, bc_vy[, bc_vz]; dt, ...) (src/stokes/solvers/DR.jl:302-337).
+    Mutates dr in place and returns the stats dict of DR.jl:783-801. A 3-D
+    state takes three boundary conditions and runs in FP64. Keywords: dt,
+    pressure_factor=50, tolerance=1e-6, max_iterations=50_000,
+    check_interval=50, iterMax=50_000, max_ph_iterations=1000,
+    rel_drop0=1e-2, plastic, phases_v, phases_P, finite_K, freeze_jacobian,
+    throw_on_failure=True, tau_old=True, skip_scaling, verbose,
+    time_kernels, precision.
+
+    precision="fp64" runs the reference's iteration in FP64. precision="fp32"
+    keeps the state, the outer residuals, the norms and the convergence test
+    in FP64 (same criterion and tolerance) and runs the inner pseudo-transient
+    iterations in FP32 on a correction of the linearised visco-elastic
+    problem, folded into the FP64 state at every outer iteration."""
+    keep = []
+    bcs, dp, p = _stokes_call(dr, mesh, (bc_vx, bc_vy, bc_vz), keep, **kwargs)
+    st = _SolveStats()
+    rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
+    stats = _stats_dict(st)
     dr.ctx.check(rc)
     return stats
+
+
+def solve_batch(problems, nthreads: int = 0, raise_errors: bool = True):
+    """Independent Stokes solves run concurrently (fem_stokes_solve_batch):
+    ``problems`` is a list of (dr, mesh, bcs, kwargs) with the keywords of
+    solve_stokes; every dr must have its own Context. Returns the list of
+    stats dicts (with "status"); raises the first failure after all ran
+    unless raise_errors is False (then a failed solve shows as a nonzero
+    "status" and its message as "error")."""
+    n = len(problems)
+    keep = []
+    states = (VP * n)()
+    bcp = (_P(_BC) * n)()
+    dps = (_P(_DP) * n)()
+    params = (_SolveParams * n)()
+    stats = (_SolveStats * n)()
+    status = (I32 * n)()
+    for i, (dr, mesh, bcs, kw) in enumerate(problems):
+        arr, dp, p = _stokes_call(dr, mesh, tuple(bcs) + (None,) * (3 - len(bcs)), keep, **kw)
+        states[i] = dr.handle
+        bcp[i] = C.cast(arr, _P(_BC))
+        dps[i] = dp
+        params[i] = p
+    rc = _lib.fem_stokes_solve_batch(n, states, bcp, dps, params, stats, status, int(nthreads))
+    out = []
+    for i in range(n):
+        d = _stats_dict(stats[i])
+        d["status"] = status[i]
+        if status[i] != FEM_OK:
+            d["error"] = problems[i][0].ctx.error()
+        out.append(d)
+    if rc != FEM_OK and (raise_errors or not any(st != FEM_OK for st in status)):
+        for i, (dr, *_rest) in enumerate(problems):
+            if status[i] != FEM_OK:
+                dr.ctx.check(status[i])
+        # rejected before any solve ran (the message is on a context of the batch)
+        for dr, *_rest in problems:
+            if dr.ctx.error():
+                dr.ctx.check(rc)
+        raise FemError(rc, "batched solve rejected")
+    return out
 
 
 def update_stokes_current_stress(dr: StokesDR, mesh: MixedMesh, dt, plastic=None):

@@ -221,3 +221,22 @@ def hex27_inclusions(n=64, nsph=50, seed=0, plastic=False, eta_reg=8e-3, Kb=100.
                      plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(C,) * nph,
                                   eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph) if plastic else None,
                      extra=dict(seed=seed))
+
+
+def batch_workloads(count=32, seed=0, sizes=(32, 48, 64, 80, 96, 112, 128)):
+    """Many small independent viscous problems: n x n T7 cells with n drawn
+    from sizes, 1-10 circular inclusions (r ∈ U[0.02, 0.1], log10 η ∈
+    U[-2, 4], later inclusion wins), pure shear, incompressible."""
+    rng = np.random.default_rng(seed)
+    out = []
+    inf = math.inf
+    for k in range(count):
+        n = int(rng.choice(sizes))
+        ninc = int(rng.integers(1, 11))
+        circles = list(zip(rng.uniform(0.1, 0.9, ninc), rng.uniform(0.1, 0.9, ninc), rng.uniform(0.02, 0.1, ninc)))
+        eta = (1.0,) + tuple(10.0 ** rng.uniform(-2, 4, ninc))
+        nph = ninc + 1
+        out.append(Workload(name=f"T7/P1-disc {n}x{n} viscous {ninc} inclusions", nx=n, ny=n, eta=eta,
+                            etab=(inf,) * nph, G=(inf,) * nph, rho0=(1.0,) * nph, K=(inf,) * nph, g=(0.0, 0.0),
+                            dt=1.0, circles=circles, extra=dict(seed=seed, index=k)))
+    return out

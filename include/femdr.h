@@ -226,6 +226,18 @@ typedef struct {
 int fem_stokes_solve(fem_stokes* st, const fem_bc bcs[2], const fem_drucker_prager* plastic,
                      const fem_solve_params* params, fem_solve_stats* stats);
 
+/* n independent solve! calls (DR.jl:302-337) run concurrently. No counterpart
+ * in the reference, which solves one state per call; for callers with many
+ * small problems (parameter sweeps, ensembles). Each state must own its
+ * fem_ctx (one HIP stream per solve); nthreads host threads (<= 0: n) take
+ * the states largest mesh first. bcs[i] holds the state's dim conditions,
+ * plastic may be NULL or hold NULL entries, params/stats/status are arrays
+ * of n. status[i] is the return code of solve i; the call returns the first
+ * nonzero status in state order, FEM_OK if all succeed. */
+int fem_stokes_solve_batch(int32_t n, fem_stokes* const* states, const fem_bc* const* bcs,
+                           const fem_drucker_prager* const* plastic, const fem_solve_params* params,
+                           fem_solve_stats* stats, int32_t* status, int32_t nthreads);
+
 /* assemble_viscosity_weighted_pressure_scaling! (pressure_scaling.jl:118-216):
  * fills M_P and γP. */
 int fem_stokes_pressure_scaling(fem_stokes* st, double pressure_factor, double dt, int32_t finite_K);
