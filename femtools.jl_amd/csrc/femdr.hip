@@ -59,6 +59,7 @@ int fem_ctx_create(int device, fem_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return FEM_EHIP; }
     if (dalloc_t(c, &c->red_partial, RED_GRID * RED_MAXK) || dalloc_t(c, &c->red_out, RED_MAXK) ||
         hipHostMalloc((void**)&c->h_red, RED_MAXK * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_cheb, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return FEM_EHIP;
@@ -76,6 +77,7 @@ int fem_ctx_destroy(fem_ctx* c) {
     hipFree(c->red_partial);
     hipFree(c->red_out);
     hipHostFree(c->h_red);
+    hipHostFree(c->h_cheb);
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipStreamDestroy(c->stream);
@@ -369,7 +371,7 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         (r = dalloc_t(c, &s->d_Jrow, NSH * m->nels)) || (r = dalloc_t(c, &s->d_Jdiag, NSH * m->nels)) ||
         (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
         (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH)) ||
-        (r = dalloc_t(c, &s->d_flag, 2))) {
+        (r = dalloc_t(c, &s->d_flag, 2)) || (r = dalloc_t(c, &s->d_cheb, 8))) {
         fem_stokes_destroy(s);
         return r;
     }
@@ -394,6 +396,7 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
     hipFree(s->d_flag);
+    hipFree(s->d_cheb);
     hipFree(s->d_yflag);
     hipFree(s->d_bcval_z); hipFree(s->d_slot3); hipFree(s->d_jslot3);
     if (s->w32) {
@@ -560,6 +563,7 @@ struct LaunchOpts {
     bool store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false, store_tau = false;
     double dt = 1.0;
     double alpha[2] = {0, 0}, beta[2] = {0, 0};
+    const void* cheb = nullptr;       // nullable: the Chebyshev scalars (element type) in device memory
     int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
     uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
 };
@@ -710,6 +714,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     a.bcval_y = d.bcval_y;
     a.alpha_x = (F)o.alpha[0]; a.alpha_y = (F)o.alpha[1];
     a.beta_x = (F)o.beta[0]; a.beta_y = (F)o.beta[1];
+    a.cheb = (const F*)o.cheb;
     a.do_update = o.do_update;
     a.RP_out = o.store_RP ? d.RP : nullptr;
     a.Pnum_out = o.store_Pnum ? d.Pnum : nullptr;
@@ -797,6 +802,7 @@ static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) 
     if (o.store_Rv0) { g.Rv0_out_x = d.Rv0_x; g.Rv0_out_y = d.Rv0_y; }
     g.alpha_x = (F)o.alpha[0]; g.alpha_y = (F)o.alpha[1];
     g.beta_x = (F)o.beta[0]; g.beta_y = (F)o.beta[1];
+    g.cheb = (const F*)o.cheb;
     g.do_update = o.do_update;
     g.rhs_vx = d.rhs_vx;
     g.rhs_vy = d.rhs_vy;
@@ -995,6 +1001,47 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
     return FEM_OK;
 }
 
+#ifndef FEM_GRAPHS
+#define FEM_GRAPHS 1   // 0: every inner iteration is launched from the host
+#endif
+
+// One check interval (ncheck inner iterations: element + gather launches with
+// the interval's fixed store pattern) captured once per solve and replayed
+// from a graph. The Chebyshev scalars change at every check, so replayed
+// launches read them from device memory, staged by a stream-ordered copy
+// before each replay (the host writes the pinned stage only after the
+// previous check's synchronisation, when that copy has completed).
+struct IntervalGraph {
+    hipGraphExec_t exec = nullptr;
+    ~IntervalGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+    template <class F, class L>
+    int run(fem_stokes* s, int64_t ncheck, const double al[2], const double be[2], L launch_iter) {
+        fem_ctx* c = s->ctx;
+        F* h = reinterpret_cast<F*>(c->h_cheb);
+        h[0] = (F)al[0]; h[1] = (F)al[1]; h[2] = (F)be[0]; h[3] = (F)be[1];
+        HIPCHK(c, hipMemcpyAsync(s->d_cheb, h, 4 * sizeof(F), hipMemcpyHostToDevice, c->stream));
+        if (!exec) {
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = FEM_OK;
+            for (int64_t k = 1; k <= ncheck && rc == FEM_OK; ++k) rc = launch_iter(k);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rc != FEM_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            HIPCHK(c, e);
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(c, ei);
+        }
+        HIPCHK(c, hipGraphLaunch(exec, c->stream));
+        return FEM_OK;
+    }
+};
+
 // Control and result of one PH/DYREL run (_solve_stokes_dyrel!, DR.jl:348-802).
 struct DRControl {
     double tol = 0;            // converged when min(err_abs, err_rel) < tol
@@ -1030,6 +1077,7 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
     double rel_drop = ctl.rel_drop0;
     int64_t iter = 0;
     int rc = FEM_OK;
+    IntervalGraph graph;
 
     for (int64_t itPH = 1; itPH <= ctl.max_ph; ++itPH) {
         r.itPH = itPH;
@@ -1064,6 +1112,27 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
         int64_t itPT = 0;
         bool stop_total = false;
         while (err > eps_vel && itPT <= ctl.iter_max) {
+            // a whole check interval that cannot end early is replayed from the
+            // graph (every 16th interval stays on the host when kernels are timed)
+            const bool whole = FEM_GRAPHS && ncheck > 1 && iter % ncheck == 0 &&
+                               itPT + ncheck - 1 <= ctl.iter_max && iter + ncheck - 1 <= ctl.max_iterations &&
+                               !(timer.on && (iter / ncheck) % 16 == 0);
+            if (whole) {
+                auto launch_iter = [&](int64_t k) -> int {
+                    LaunchOpts oi = o;
+                    oi.use_pnum = true;
+                    oi.do_update = true;
+                    oi.store_RP = oi.store_Pnum = oi.store_Rv = k == ncheck;
+                    oi.store_Rv0 = k == ncheck - 1;
+                    oi.cheb = s->d_cheb;
+                    CHK(launch_element(s, d, oi));
+                    return launch_gather(s, d, oi);
+                };
+                CHK(graph.template run<F>(s, ncheck, al, be, launch_iter));
+                itPT += ncheck;
+                iter += ncheck;
+                r.launches += ncheck;
+            } else {
             ++itPT;
             ++iter;
             const bool do_check = iter % ncheck == 0;
@@ -1090,6 +1159,8 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
             CHK(launch_gather(s, d, oi));
             timer.mark(e2);
             ++r.launches;
+            }
+            const bool do_check = iter % ncheck == 0;
             if (do_check) {
                 double r6[6];
                 CHK(reduce_check(s, d, al, r6));
@@ -1283,6 +1354,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     LaunchOpts o32 = o;
     o32.plastic = delta;
     o32.elastic = delta && o.elastic;
+    IntervalGraph graph;
     // FP64 residual at x; with plasticity it counts the yielding elements
     auto residual64 = [&](const LaunchOpts& oo) -> int {
         if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
@@ -1327,6 +1399,28 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         int64_t itPT = 0;
         bool stop_total = false;
         while (err > eps_vel && itPT <= p->iter_max) {
+            // whole visco-elastic check intervals are replayed from a graph
+            // (intervals with the return map fold between checks; every 16th
+            // interval stays on the host when kernels are timed)
+            const bool whole = FEM_GRAPHS && ncheck > 1 && !d32.delta && iter % ncheck == 0 &&
+                               itPT + ncheck - 1 <= p->iter_max && iter + ncheck - 1 <= p->max_iterations &&
+                               !(timer.on && (iter / ncheck) % 16 == 0);
+            if (whole) {
+                auto launch_iter = [&](int64_t k) -> int {
+                    LaunchOpts oi = o32;
+                    oi.use_pnum = true;
+                    oi.do_update = true;
+                    oi.store_RP = oi.store_Pnum = oi.store_Rv = k == ncheck;
+                    oi.store_Rv0 = k == ncheck - 1;
+                    oi.cheb = s->d_cheb;
+                    CHK(launch_element(s, d32, oi));
+                    return launch_gather(s, d32, oi);
+                };
+                CHK(graph.run<float>(s, ncheck, al, be, launch_iter));
+                itPT += ncheck;
+                iter += ncheck;
+                launches += ncheck;
+            } else {
             ++itPT;
             ++iter;
             const bool do_check = iter % ncheck == 0;
@@ -1352,6 +1446,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
             CHK(launch_gather(s, d32, oi));
             timer.mark(e2);
             ++launches;
+            }
+            const bool do_check = iter % ncheck == 0;
             if (do_check) {
                 double r6[6];
                 CHK(reduce_check(s, d32, al, r6));

@@ -31,6 +31,7 @@ struct fem_ctx {
     double* red_partial = nullptr;   // [RED_GRID * RED_MAXK]
     double* red_out = nullptr;       // [RED_MAXK]
     double* h_red = nullptr;         // pinned [RED_MAXK]
+    double* h_cheb = nullptr;        // pinned [8]: Chebyshev scalars staged for graph replays
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;     // kernel-timing events (3 per timed iteration)
     size_t tev_used = 0;
@@ -118,6 +119,7 @@ struct fem_stokes {
     int64_t bc_stage_cap = 0;
     double* d_ptab = nullptr;
     int32_t* d_flag = nullptr;      // [0] scratch yes/no flag, [1] yielding-element count
+    double* d_cheb = nullptr;       // [8]: the Chebyshev scalars read by graph-replayed launches
     uint8_t* d_yflag = nullptr;     // near-yield flag per element (mixed-precision solve)
     bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
     bool q_nonzero = false;         // dr.Q holds a nonzero entry

@@ -275,6 +275,8 @@ struct ElemArgs {
     F* rate_x; F* rate_y; const F* PCx; const F* PCy;
     const uint8_t* bcflag; const F* bcval_x; const F* bcval_y;
     F alpha_x, alpha_y, beta_x, beta_y;  // positive Chebyshev step and momentum
+    const F* cheb;             // nullable: (alpha_x, alpha_y, beta_x, beta_y) in device memory
+                               // instead (launches replayed from a graph)
     int32_t do_update;         // EF_ITER launches inside the DR loop update the bubble node
     // optional outputs (null unless stored this launch)
     F* RP_out; F* Pnum_out;
@@ -814,10 +816,12 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         if (A.Rv_out_x) { A.Rv_out_x[b] = rx; A.Rv_out_y[b] = ry; }
         if (A.Rv0_out_x) { A.Rv0_out_x[b] = rx; A.Rv0_out_y[b] = ry; }
         if (ITER && A.do_update) {
-            F nrx = A.beta_x * A.rate_x[b] + rx / A.PCx[b];
-            F nry = A.beta_y * A.rate_y[b] + ry / A.PCy[b];
-            F nvx = vx[NSH] + (-A.alpha_x) * nrx;
-            F nvy = vy[NSH] + (-A.alpha_y) * nry;
+            const F cax = A.cheb ? A.cheb[0] : A.alpha_x, cay = A.cheb ? A.cheb[1] : A.alpha_y;
+            const F cbx = A.cheb ? A.cheb[2] : A.beta_x, cby = A.cheb ? A.cheb[3] : A.beta_y;
+            F nrx = cbx * A.rate_x[b] + rx / A.PCx[b];
+            F nry = cby * A.rate_y[b] + ry / A.PCy[b];
+            F nvx = vx[NSH] + (-cax) * nrx;
+            F nvy = vy[NSH] + (-cay) * nry;
             if (fl & 1) { nrx = 0; nvx = A.bcval_x[b]; }
             if (fl & 2) { nry = 0; nvy = A.bcval_y[b]; }
             A.rate_x[b] = nrx; A.rate_y[b] = nry;
@@ -842,6 +846,7 @@ struct GatherArgs {
     const uint8_t* bcflag; const F* bcval_x; const F* bcval_y;
     F* Rv_out_x; F* Rv_out_y; F* Rv0_out_x; F* Rv0_out_y;
     F alpha_x, alpha_y, beta_x, beta_y;
+    const F* cheb;              // nullable: the four Chebyshev scalars in device memory instead
     int32_t do_update;
     const F* rhs_vx; const F* rhs_vy;   // nullable: nodal right-hand side added to the gathered sum
 };
@@ -860,8 +865,9 @@ __global__ __launch_bounds__(BLOCK) void k_gather_update(GatherArgs<F> A) {
         ry += r.y;
     }
     node_update<F>(node, rx, ry, A.rhs_vx, A.rhs_vy, A.bcflag, A.bcval_x, A.bcval_y, A.Rv_out_x, A.Rv_out_y,
-                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.alpha_x, A.alpha_y, A.beta_x, A.beta_y, A.rate_x,
-                   A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
+                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.cheb ? A.cheb[0] : A.alpha_x,
+                   A.cheb ? A.cheb[1] : A.alpha_y, A.cheb ? A.cheb[2] : A.beta_x, A.cheb ? A.cheb[3] : A.beta_y,
+                   A.rate_x, A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
 }
 
 // Grid nodes after the EF_TILE element kernel, one thread per T6 grid node:
@@ -903,8 +909,9 @@ __global__ __launch_bounds__(BLOCK) void k_tile_update(TileUpdArgs<F> P) {
     }
     const GatherArgs<F>& A = P.g;
     node_update<F>(node, rx, ry, A.rhs_vx, A.rhs_vy, A.bcflag, A.bcval_x, A.bcval_y, A.Rv_out_x, A.Rv_out_y,
-                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.alpha_x, A.alpha_y, A.beta_x, A.beta_y, A.rate_x,
-                   A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
+                   A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.cheb ? A.cheb[0] : A.alpha_x,
+                   A.cheb ? A.cheb[1] : A.alpha_y, A.cheb ? A.cheb[2] : A.beta_x, A.cheb ? A.cheb[3] : A.beta_y,
+                   A.rate_x, A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
 }
 
 // ---------------------------------------------------------------------------
