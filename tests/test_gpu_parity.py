@@ -335,3 +335,46 @@ def test_structured_and_unstructured_paths_agree():
     assert outs[0][0] == outs[1][0]
     for a, b in zip(outs[0][1:], outs[1][1:]):
         assert rel_err(a, b) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_gpu_nan_in_state_is_reported_like_the_oracle(precision):
+    # "NaN detected in outer loop at PH=1" (DR.jl:574): a NaN in the velocity
+    # reaches the first outer residual; the oracle stops with the same message
+    say(f"[gpu] NaN path {precision}")
+    w, pr, dr, mesh, (bx, vx, by, vy) = _viscous_pair(8)
+    free = np.setdiff1d(np.arange(mesh.nnodes), np.union1d(bx, by))[0]
+    v = dr.v.x.numpy()
+    v[free] = np.nan
+    dr.v.x.copyto(v)
+    pr.vx[free] = np.nan
+    with pytest.raises(RuntimeError, match="NaN detected in outer loop at PH=1") as eo:
+        pr.solve(w.dt, tolerance=1e-8, check_interval=w.check_interval)
+    with pytest.raises(fa.FemError, match="NaN detected in outer loop at PH=1") as eg:
+        fa.solve(dr, mesh, fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy),
+                 dt=w.dt, tolerance=1e-8, check_interval=w.check_interval, precision=precision)
+    assert str(eo.value) == str(eg.value)
+
+
+def test_gpu_3d_nan_and_non_convergence_paths():
+    say("[gpu] 3-D NaN and iteration-cap paths")
+    coords, el2n = O.hex27_mesh(2, 2, 1)
+    gam = O.box_boundary_nodes3(coords)
+    z = np.zeros(len(gam))
+    mesh = fa.hex27_box_mesh(2, 2, 1)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=(1.0,), etab=(math.inf,), g=(0.0, 0.0, 1.0)))
+    bcs = [fa.DirichletBoundaryCondition(gam, z) for _ in range(3)]
+    pr = O.Problem3(coords, el2n, O.material(eta=(1.0,), etab=(math.inf,)), g=(0.0, 0.0, 1.0),
+                    bc=tuple((gam, z) for _ in range(3)))
+    st_o = pr.solve(1.0, tolerance=1e-12, check_interval=1, max_iterations=3)
+    st = fa.solve(dr, mesh, *bcs, dt=1.0, tolerance=1e-12, check_interval=1, max_iterations=3,
+                  throw_on_failure=False)
+    assert not st["converged"] and st["reached_total_iter"] and st["iter"] == st_o["iter"]
+    with pytest.raises(fa.FemError, match="did not converge"):
+        fa.solve(dr, mesh, *bcs, dt=1.0, tolerance=1e-12, check_interval=1, max_iterations=3)
+    free = np.setdiff1d(np.arange(mesh.nnodes), gam)[0]
+    v = dr.v.z.numpy()
+    v[free] = np.nan
+    dr.v.z.copyto(v)
+    with pytest.raises(fa.FemError, match="NaN detected in outer loop at PH=1"):
+        fa.solve(dr, mesh, *bcs, dt=1.0, tolerance=1e-8)
