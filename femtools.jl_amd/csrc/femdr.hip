@@ -1005,6 +1005,7 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
 #define FEM_GRAPHS 1   // 0: every inner iteration is launched from the host
 #endif
 
+
 // One check interval (ncheck inner iterations: element + gather launches with
 // the interval's fixed store pattern) captured once per solve and replayed
 // from a graph. The Chebyshev scalars change at every check, so replayed
@@ -1039,6 +1040,11 @@ struct IntervalGraph {
         }
         HIPCHK(c, hipGraphLaunch(exec, c->stream));
         return FEM_OK;
+    }
+    // the same with the launches of the whole segment in one call
+    template <class F, class L>
+    int run_once(fem_stokes* s, const double al[2], const double be[2], L launch_all) {
+        return run<F>(s, 1, al, be, [&](int64_t) { return launch_all(); });
     }
 };
 
@@ -1153,6 +1159,7 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
             oi.beta[0] = be[0]; oi.beta[1] = be[1];
             const bool tk = timer.take(iter);
             hipEvent_t e0 = tk ? timer.ev() : nullptr, e1 = tk ? timer.ev() : nullptr, e2 = tk ? timer.ev() : nullptr;
+            if (tk) timer.weight.push_back(1);
             timer.mark(e0);
             CHK(launch_element(s, d, oi));
             timer.mark(e1);
@@ -1354,7 +1361,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     LaunchOpts o32 = o;
     o32.plastic = delta;
     o32.elastic = delta && o.elastic;
-    IntervalGraph graph;
+    std::vector<std::pair<int64_t, std::unique_ptr<IntervalGraph>>> graphs;   // by segment shape
+    hipEvent_t timer_e1 = nullptr;
     // FP64 residual at x; with plasticity it counts the yielding elements
     auto residual64 = [&](const LaunchOpts& oo) -> int {
         if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
@@ -1399,54 +1407,69 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         int64_t itPT = 0;
         bool stop_total = false;
         while (err > eps_vel && itPT <= p->iter_max) {
-            // whole visco-elastic check intervals are replayed from a graph
-            // (intervals with the return map fold between checks; every 16th
-            // interval stays on the host when kernels are timed)
-            const bool whole = FEM_GRAPHS && ncheck > 1 && !d32.delta && iter % ncheck == 0 &&
-                               itPT + ncheck - 1 <= p->iter_max && iter + ncheck - 1 <= p->max_iterations &&
-                               !(timer.on && (iter / ncheck) % 16 == 0);
-            if (whole) {
-                auto launch_iter = [&](int64_t k) -> int {
-                    LaunchOpts oi = o32;
-                    oi.use_pnum = true;
-                    oi.do_update = true;
-                    oi.store_RP = oi.store_Pnum = oi.store_Rv = k == ncheck;
-                    oi.store_Rv0 = k == ncheck - 1;
-                    oi.cheb = s->d_cheb;
-                    CHK(launch_element(s, d32, oi));
-                    return launch_gather(s, d32, oi);
-                };
-                CHK(graph.run<float>(s, ncheck, al, be, launch_iter));
-                itPT += ncheck;
-                iter += ncheck;
-                launches += ncheck;
-            } else {
-            ++itPT;
-            ++iter;
-            const bool do_check = iter % ncheck == 0;
-            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
-            if (do_check && ncheck == 1) {
+            // the next segment: the inner iterations up to the next check, or
+            // fold on intervals with the return map, or iteration cap. Segments
+            // are replayed from graphs; every 16th interval runs one iteration
+            // at a time from the host, timed.
+            const bool timed = timer.on && (iter / ncheck) % 16 == 0;
+            int64_t nseg = 1;
+            if (ncheck > 1 && !timed) {
+                nseg = ncheck - iter % ncheck;
+                if (d32.delta && FEM_FOLD_INTERVAL > 0)
+                    nseg = std::min<int64_t>(nseg, FEM_FOLD_INTERVAL - iter % FEM_FOLD_INTERVAL);
+                nseg = std::max<int64_t>(1, std::min(nseg, std::min(p->iter_max - itPT + 1, p->max_iterations - iter + 1)));
+            }
+            const int64_t iter_end = iter + nseg;
+            // an iteration after which the loop may end stores what the end needs
+            const bool lastp = iter_end % ncheck == 0 || itPT + nseg > p->iter_max || iter_end > p->max_iterations;
+            // position in the segment of the iteration before a check (its residual is Rv0)
+            const int64_t mj = ncheck - 1 - iter % ncheck;
+            const int64_t mpos = ncheck > 1 && mj >= 1 && mj <= nseg ? mj : 0;
+            if (ncheck == 1 && iter_end % ncheck == 0) {
                 HIPCHK(c, hipMemcpyAsync(w.Rv0_x, w.Rv_x, nv * 4, hipMemcpyDeviceToDevice, c->stream));
                 HIPCHK(c, hipMemcpyAsync(w.Rv0_y, w.Rv_y, nv * 4, hipMemcpyDeviceToDevice, c->stream));
             }
-            LaunchOpts oi = o32;
-            oi.use_pnum = true;
-            oi.do_update = true;
-            oi.store_RP = last_possible;
-            oi.store_Pnum = last_possible;
-            oi.store_Rv = last_possible;
-            oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
-            oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
-            oi.beta[0] = be[0]; oi.beta[1] = be[1];
-            const bool tk = timer.take(iter);
-            hipEvent_t e0 = tk ? timer.ev() : nullptr, e1 = tk ? timer.ev() : nullptr, e2 = tk ? timer.ev() : nullptr;
-            timer.mark(e0);
-            CHK(launch_element(s, d32, oi));
-            timer.mark(e1);
-            CHK(launch_gather(s, d32, oi));
-            timer.mark(e2);
-            ++launches;
+            auto seg = [&](const void* chb) -> int {
+                LaunchOpts oi = o32;
+                oi.use_pnum = true;
+                oi.do_update = true;
+                oi.alpha[0] = al[0]; oi.alpha[1] = al[1];
+                oi.beta[0] = be[0]; oi.beta[1] = be[1];
+                oi.cheb = chb;
+                for (int64_t j = 1; j <= nseg; ++j) {
+                    LaunchOpts oj = oi;
+                    oj.store_RP = oj.store_Pnum = oj.store_Rv = lastp && j == nseg;
+                    oj.store_Rv0 = j == mpos;
+                    CHK(launch_element(s, d32, oj));
+                    if (timed) timer.mark(timer_e1);
+                    CHK(launch_gather(s, d32, oj));
+                }
+                return FEM_OK;
+            };
+            if (timed) {
+                hipEvent_t e0 = timer.ev(), e2;
+                timer_e1 = timer.ev();
+                e2 = timer.ev();
+                timer.weight.push_back(nseg);
+                timer.mark(e0);
+                CHK(seg(nullptr));
+                timer.mark(e2);
+            } else if (FEM_GRAPHS && nseg > 1) {
+                const int64_t key = ((nseg * 2 + (lastp ? 1 : 0)) * (ncheck + 1) + mpos) * 2 + (d32.delta ? 1 : 0);
+                IntervalGraph* gph = nullptr;
+                for (auto& kv : graphs)
+                    if (kv.first == key) gph = kv.second.get();
+                if (!gph) {
+                    graphs.emplace_back(key, std::unique_ptr<IntervalGraph>(new IntervalGraph()));
+                    gph = graphs.back().second.get();
+                }
+                CHK(gph->run_once<float>(s, al, be, [&]() { return seg(s->d_cheb); }));
+            } else {
+                CHK(seg(nullptr));
             }
+            itPT += nseg;
+            iter = iter_end;
+            launches += nseg;
             const bool do_check = iter % ncheck == 0;
             if (do_check) {
                 double r6[6];

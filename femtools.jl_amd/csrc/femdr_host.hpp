@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <set>
 #include <string>
 #include <thread>
@@ -47,6 +48,7 @@ struct KernelTimer {
     bool take(int64_t iter) const { return on && iter % STRIDE == 1; }
     double ms_elem = 0, ms_gather = 0;
     int64_t n = 0;
+    std::vector<int64_t> weight;   // inner iterations covered by each timed triple
     KernelTimer(fem_ctx* c_, bool on_) : c(c_), on(on_) { c->tev_used = 0; }
     hipEvent_t ev() {
         if (c->tev_used == c->tev.size()) {
@@ -68,10 +70,11 @@ struct KernelTimer {
                 hipEventElapsedTime(&b, c->tev[i + 1], c->tev[i + 2]) == hipSuccess) {
                 ms_elem += a;
                 ms_gather += b;
-                ++n;
+                n += i / 3 < weight.size() ? weight[i / 3] : 1;
             }
         }
         c->tev_used = 0;
+        weight.clear();
     }
 };
 
