@@ -202,11 +202,13 @@ def pure_shear_field3(coords, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), edot=1.0):
     return edot * (coords[:, 0] - xm), np.zeros(len(coords)), -edot * (coords[:, 2] - zm)
 
 
-def hex27_inclusions(n=64, nsph=50, seed=0, plastic=False, eta_reg=8e-3, Kb=100.0, C=1.0) -> Workload3:
+def hex27_inclusions(n=64, nsph=50, seed=0, plastic=False, eta_reg=8e-3, Kb=100.0, C=1.0,
+                     elastic=True) -> Workload3:
     """Viscous (or Drucker-Prager) multi-sphere workload on an n^3 Hex27 grid:
     spheres r ∈ U[0.02, 0.08], log10 η ∈ U[-1, 3], matrix η = 1, pure shear,
-    no gravity, incompressible (ηb = K = ∞); with plasticity G = 1, Δt = 0.1,
-    C = 1, ϕ = 30°, Ψ = 0, finite Kb."""
+    no gravity, incompressible (ηb = K = ∞); with plasticity C = 1, ϕ = 30°,
+    Ψ = 0, finite Kb, and G = 1, Δt = 0.1 unless elastic is False (viscoplastic,
+    G = ∞)."""
     rng = np.random.default_rng(seed)
     c = rng.uniform(0, 1, (nsph, 3))
     r = rng.uniform(0.02, 0.08, nsph)
@@ -214,10 +216,12 @@ def hex27_inclusions(n=64, nsph=50, seed=0, plastic=False, eta_reg=8e-3, Kb=100.
     inf = math.inf
     nph = nsph + 1
     eta = (1.0,) + tuple(10.0 ** leta)
-    G = (1.0,) * nph if plastic else (inf,) * nph
-    return Workload3(name=f"Hex27/P1-disc {n}^3 {'VEP' if plastic else 'viscous'} {nsph} spheres", n=n, eta=eta,
+    ve = plastic and elastic
+    G = (1.0,) * nph if ve else (inf,) * nph
+    kind = ("VEP" if ve else "VP") if plastic else "viscous"
+    return Workload3(name=f"Hex27/P1-disc {n}^3 {kind} {nsph} spheres", n=n, eta=eta,
                      etab=(inf,) * nph, G=G, rho0=(0.0,) * nph, K=(inf,) * nph, g=(0.0, 0.0, 0.0),
-                     dt=0.1 if plastic else 1.0, spheres=[(*c[k], r[k]) for k in range(nsph)],
+                     dt=0.1 if ve else 1.0, spheres=[(*c[k], r[k]) for k in range(nsph)],
                      plastic=dict(phi=(math.radians(30),) * nph, psi=(0.0,) * nph, C=(C,) * nph,
                                   eta_reg=(eta_reg,) * nph, Kb=(Kb,) * nph) if plastic else None,
                      extra=dict(seed=seed))
