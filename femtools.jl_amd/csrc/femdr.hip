@@ -8,6 +8,8 @@
 // scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
 #include "femdr_host.hpp"
 
+#include <functional>
+
 
 // ---------------------------------------------------------------------------
 // reference-element tables (shape_functions.jl:115-171, 231-239;
@@ -566,6 +568,7 @@ struct LaunchOpts {
     const void* cheb = nullptr;       // nullable: the Chebyshev scalars (element type) in device memory
     int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
     uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
+    double* phi_out = nullptr;        // nullable (FP64 residual launches): shear dissipation [nels][7]
 };
 
 constexpr int EBLOCK = 128;
@@ -761,6 +764,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
                     (strct ? EF_STRUCT : 0);
     if constexpr (std::is_same<F, double>::value) {
         if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
+        a.phi_out = o.phi_out;
         dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
     } else {
         if (!d.lin || !o.use_pnum)
@@ -1056,6 +1060,10 @@ struct DRControl {
     bool freeze = true;
     int verbose = 0;
     const char* tag = "";
+    // coupled solve (DR.jl:597-661): run before the Stokes launches of every
+    // inner iteration; both loop tests also wait for *thermal_converged
+    std::function<int()> thermal_step;
+    const bool* thermal_converged = nullptr;
 };
 struct DRResult {
     int64_t itPH = 0, iter = 0, launches = 0, jac = 0;
@@ -1084,6 +1092,8 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
     int64_t iter = 0;
     int rc = FEM_OK;
     IntervalGraph graph;
+    const bool coupled = (bool)ctl.thermal_step;
+    auto thermal_ok = [&]() { return !coupled || *ctl.thermal_converged; };
 
     for (int64_t itPH = 1; itPH <= ctl.max_ph; ++itPH) {
         r.itPH = itPH;
@@ -1111,16 +1121,16 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
             printf("%sitPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
                    ctl.tag, (long long)itPH, (long long)iter, err, r.err_abs, r.err_rel, r.err_v, r.err_v / err_v0,
                    r.err_P, r.err_P / err_P0);
-        if (err < ctl.tol) { r.converged = true; break; }
+        if (err < ctl.tol && thermal_ok()) { r.converged = true; break; }
         if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
         err_min = std::min(err_min, err);
         const double eps_vel = err * rel_drop;
         int64_t itPT = 0;
         bool stop_total = false;
-        while (err > eps_vel && itPT <= ctl.iter_max) {
+        while ((err > eps_vel || !thermal_ok()) && itPT <= ctl.iter_max) {
             // a whole check interval that cannot end early is replayed from the
             // graph (every 16th interval stays on the host when kernels are timed)
-            const bool whole = FEM_GRAPHS && ncheck > 1 && iter % ncheck == 0 &&
+            const bool whole = FEM_GRAPHS && !coupled && ncheck > 1 && iter % ncheck == 0 &&
                                itPT + ncheck - 1 <= ctl.iter_max && iter + ncheck - 1 <= ctl.max_iterations &&
                                !(timer.on && (iter / ncheck) % 16 == 0);
             if (whole) {
@@ -1143,11 +1153,14 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
             ++iter;
             const bool do_check = iter % ncheck == 0;
             // an iteration after which the loop may end stores what the end needs
-            const bool last_possible = do_check || itPT > ctl.iter_max || iter > ctl.max_iterations;
+            // (a coupled loop can also end after the thermal check of iteration 1)
+            const bool last_possible = do_check || itPT > ctl.iter_max || iter > ctl.max_iterations ||
+                                       (coupled && iter == 1);
             if (do_check && ncheck == 1) {
                 HIPCHK(c, hipMemcpyAsync(d.Rv0_x, d.Rv_x, nv * sizeof(F), hipMemcpyDeviceToDevice, c->stream));
                 HIPCHK(c, hipMemcpyAsync(d.Rv0_y, d.Rv_y, nv * sizeof(F), hipMemcpyDeviceToDevice, c->stream));
             }
+            if (coupled) CHK(ctl.thermal_step());
             LaunchOpts oi = o;
             oi.use_pnum = true;
             oi.do_update = true;
@@ -1567,6 +1580,148 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     return rc;
 }
 
+// The thermal half of a coupled solve (solve_coupled!, DR.jl:81-131)
+struct CoupledSide {
+    ThermalCoupling tc;
+    const fem_bc* bc_T = nullptr;
+    bool shear = false;
+    double* d_phi = nullptr;   // [nels][7] shear dissipation, with shear heating
+};
+
+// _transfer_temperature_kernel! (DR.jl:44-47): T_P[DoFsP[i]] = T[el2nP[i]];
+// the pressure DoFs 3e + a of element e sit at its vertex nodes a = 0..2
+__global__ void k_transfer_T(int64_t nels, const int32_t* el2n, const double* T, double* TP) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nels) return;
+    const int64_t e = i / 3;
+    const int a = (int)(i - 3 * e);
+    TP[i] = T[el2n[a * nels + e]];
+}
+
+static int transfer_T(fem_stokes* s, const double* T, double* TP) {
+    const int64_t ne = s->mesh->nels;
+    hipLaunchKernelGGL(k_transfer_T, dim3(grid_for(3 * ne, 256)), dim3(256), 0, s->ctx->stream, ne,
+                       (const int32_t*)s->mesh->d_el2n, T, TP);
+    return check_launch(s->ctx, "k_transfer_T");
+}
+
+// Shear dissipation at the velocity quadrature points: one residual launch
+// of the element kernel (v, P without Pnum, τ_old) storing Φ only; the slot
+// residuals it also writes are overwritten by the next assembly
+static int shear_heating_launch(fem_stokes* s, const LaunchOpts& o, double* phi) {
+    LaunchOpts oo = o;
+    oo.use_pnum = false;
+    oo.do_update = false;
+    oo.store_RP = oo.store_Pnum = oo.store_Rv = oo.store_Rv0 = oo.store_tau = false;
+    oo.yield_count = nullptr;
+    oo.yield_flag = nullptr;
+    oo.phi_out = phi;
+    return launch_element(s, set64(s), oo);
+}
+
+// solve! on a 2-D state (DR.jl:302-337 and the loop of DR.jl:348-802); with
+// cp, the coupled loop of solve_coupled! (DR.jl:514-661)
+static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
+                    fem_solve_stats* st, CoupledSide* cp) {
+    fem_ctx* c = s->ctx;
+    std::memset(st, 0, sizeof *st);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
+    if (p->precision != 0 && p->precision != 1)
+        return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
+                       p->precision);
+    if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
+    const auto t0 = std::chrono::steady_clock::now();
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP;
+    const double dt = p->dt;
+    if (cp) {
+        // the thermal field on the pressure DoFs before the Stokes set-up (DR.jl:123-124)
+        CHK(transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T), s->f[FEM_FIELD_T]));
+        CHK(transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T0), s->f[FEM_FIELD_T0]));
+    }
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
+    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
+    CHK(upload_bcs(s, bcs));
+    if (!p->skip_scaling) CHK(scaling_impl(s, p->pressure_factor, dt));
+    else
+        hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                           (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    KernelTimer timer(c, p->time_kernels != 0);
+    if (p->precision == 1) {
+        const int rc = solve_mixed(s, p, o, freeze, timer, st);
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != FEM_OK) return rc;
+        if (!st->converged && p->throw_on_failure)
+            return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                           (long long)st->iter, st->err_abs, p->tolerance);
+        return FEM_OK;
+    }
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_X], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_Y], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_X], 0, nv * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_Y], 0, nv * 8, c->stream));
+
+    CHK(launch_jacobian(s, o));
+    double lmax[2];
+    CHK(checked_lmax(s, lmax));
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
+
+    DRControl ctl;
+    ctl.tol = p->tolerance;
+    ctl.max_iterations = p->max_iterations;
+    ctl.iter_max = p->iter_max;
+    ctl.max_ph = p->max_ph_iterations;
+    ctl.ncheck = p->check_interval;
+    ctl.rel_drop0 = p->rel_drop0;
+    ctl.freeze = freeze;
+    ctl.verbose = p->verbose;
+    if (cp) {
+        // ∂T∂τ = R0 = 0 and the dissipation of the initial velocity (DR.jl:524-530)
+        CHK(thermal_coupling_begin(cp->tc, cp->bc_T));
+        if (cp->shear) CHK(shear_heating_launch(s, o, cp->d_phi));
+        double* Tth = thermal_field_ptr(cp->tc.t, FEM_TFIELD_T);
+        ctl.thermal_step = [&, cp, Tth]() -> int {
+            CHK(thermal_coupling_step(cp->tc));
+            CHK(transfer_T(s, Tth, s->f[FEM_FIELD_T]));
+            // refreshed after the thermal spectral estimate (DR.jl:659-661)
+            if (cp->shear && cp->tc.checked) CHK(shear_heating_launch(s, o, cp->d_phi));
+            if (p->verbose > 1 && cp->tc.checked)
+                printf("  thermal iter = %lld, err_T = %.3e\n", (long long)cp->tc.iter, cp->tc.err_T);
+            return FEM_OK;
+        };
+        ctl.thermal_converged = &cp->tc.converged;
+    }
+    DRResult r;
+    const int rc = ph_dyrel<double>(s, set64(s), o, ctl, lmax, timer, r);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timer.collect();
+    st->kernel_ms_element = timer.ms_elem;
+    st->kernel_ms_gather = timer.ms_gather;
+    st->timed_launches = timer.n;
+    st->itPH = r.itPH;
+    st->iter = r.iter;
+    st->err = r.err;
+    st->err_abs = r.err_abs;
+    st->err_rel = r.err_rel;
+    st->err_v = r.err_v;
+    st->err_P = r.err_P;
+    st->converged = r.converged;
+    st->reached_total_iter = r.iter > p->max_iterations;
+    st->lambda_max = std::max(lmax[0], lmax[1]);
+    st->lambda_max_gershgorin = r.lmax_g;
+    st->lambda_max_iterations = 0;
+    st->jacobian_assemblies = 1 + r.jac;
+    st->element_launches = r.launches;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != FEM_OK) return rc;
+    if (!r.converged && p->throw_on_failure)
+        return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                       (long long)r.iter, r.err_abs, p->tolerance);
+    return FEM_OK;
+}
+
 extern "C" {
 
 int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt, int32_t finite_K) {
@@ -1705,82 +1860,57 @@ int fem_stokes_solve_batch(int32_t n, fem_stokes* const* states, const fem_bc* c
 int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
                      fem_solve_stats* st) {
     if (!s || !bcs || !p || !st) return FEM_EARG;
+    return solve_2d(s, bcs, dp, p, st, nullptr);
+}
+
+// assemble_shear_heating! (shear_heating.jl:18-46)
+int fem_stokes_shear_heating(fem_stokes* s, const fem_drucker_prager* dp, double dt, double* phi) {
+    if (!s || !phi) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim != 2) return set_err(c, FEM_EARG, "shear heating is assembled for 2-D states");
+    LaunchOpts o;
+    CHK(prepare(s, dp, dt, 0, o, true));
+    CHK(shear_heating_launch(s, o, phi));
+    return fem_sync(c);
+}
+
+// solve_coupled! (DR.jl:81-131) over the coupled loop of _solve_stokes_dyrel!
+int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const fem_bc bc_v[2],
+                      const fem_drucker_prager* dp, const fem_solve_params* p, const fem_coupled_params* cp,
+                      fem_solve_stats* st, fem_coupled_stats* cst) {
+    if (!th || !s || !bc_v || !p || !st || !cst) return FEM_EARG;
     fem_ctx* c = s->ctx;
     std::memset(st, 0, sizeof *st);
+    std::memset(cst, 0, sizeof *cst);
+    cst->err_T = INFINITY;
+    if (thermal_ctx(th) != c)
+        return set_err(c, FEM_EARG, "the thermal and Stokes states must share one context (stream)");
+    if (s->dim != 2) return set_err(c, FEM_EARG, "coupled solves take 2-D states");
+    fem_mesh* tm = thermal_mesh(th);
+    if (tm->nnodes != s->mesh->nnodes || tm->nels != s->mesh->nels)
+        return set_err(c, FEM_EDIM, "thermal mesh nodes must match the Stokes velocity-node layout");
+    if (p->precision != 0)
+        return set_err(c, FEM_EARG, "coupled solves run in FP64 (precision 0), got precision %d", p->precision);
     HIPCHK(c, hipSetDevice(c->device));
-    if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
-    if (p->precision != 0 && p->precision != 1)
-        return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
-                       p->precision);
-    if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
-    const auto t0 = std::chrono::steady_clock::now();
-    fem_mesh* m = s->mesh;
-    const int64_t nv = m->nnodes, np = m->nnodesP;
-    const double dt = p->dt;
-    LaunchOpts o;
-    CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
-    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
-    CHK(upload_bcs(s, bcs));
-    if (!p->skip_scaling) CHK(scaling_impl(s, p->pressure_factor, dt));
-    else
-        hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
-                           (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
-    KernelTimer timer(c, p->time_kernels != 0);
-    if (p->precision == 1) {
-        const int rc = solve_mixed(s, p, o, freeze, timer, st);
-        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (rc != FEM_OK) return rc;
-        if (!st->converged && p->throw_on_failure)
-            return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
-                           (long long)st->iter, st->err_abs, p->tolerance);
-        return FEM_OK;
+    CoupledSide side;
+    side.tc.t = th;
+    side.tc.dt = p->dt;
+    side.tc.Tref = cp ? cp->Tref : 273.0;
+    side.tc.nout = p->check_interval;
+    side.bc_T = bc_T;
+    side.shear = cp && cp->shear_heating;
+    if (side.shear) {
+        CHK(dalloc_t(c, &side.d_phi, NV * s->mesh->nels));
+        side.tc.phi = side.d_phi;
     }
-    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_X], 0, nv * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_DVDTAU_Y], 0, nv * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_X], 0, nv * 8, c->stream));
-    HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_Y], 0, nv * 8, c->stream));
-
-    CHK(launch_jacobian(s, o));
-    double lmax[2];
-    CHK(checked_lmax(s, lmax));
-    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
-
-    DRControl ctl;
-    ctl.tol = p->tolerance;
-    ctl.max_iterations = p->max_iterations;
-    ctl.iter_max = p->iter_max;
-    ctl.max_ph = p->max_ph_iterations;
-    ctl.ncheck = p->check_interval;
-    ctl.rel_drop0 = p->rel_drop0;
-    ctl.freeze = freeze;
-    ctl.verbose = p->verbose;
-    DRResult r;
-    const int rc = ph_dyrel<double>(s, set64(s), o, ctl, lmax, timer, r);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    timer.collect();
-    st->kernel_ms_element = timer.ms_elem;
-    st->kernel_ms_gather = timer.ms_gather;
-    st->timed_launches = timer.n;
-    st->itPH = r.itPH;
-    st->iter = r.iter;
-    st->err = r.err;
-    st->err_abs = r.err_abs;
-    st->err_rel = r.err_rel;
-    st->err_v = r.err_v;
-    st->err_P = r.err_P;
-    st->converged = r.converged;
-    st->reached_total_iter = r.iter > p->max_iterations;
-    st->lambda_max = std::max(lmax[0], lmax[1]);
-    st->lambda_max_gershgorin = r.lmax_g;
-    st->lambda_max_iterations = 0;
-    st->jacobian_assemblies = 1 + r.jac;
-    st->element_launches = r.launches;
-    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (rc != FEM_OK) return rc;
-    if (!r.converged && p->throw_on_failure)
-        return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
-                       (long long)r.iter, r.err_abs, p->tolerance);
-    return FEM_OK;
+    const int rc = solve_2d(s, bc_v, dp, p, st, &side);
+    hipStreamSynchronize(c->stream);
+    hipFree(side.d_phi);
+    cst->err_T = side.tc.err_T;
+    cst->thermal_iterations = side.tc.iter;
+    cst->thermal_converged = side.tc.converged;
+    return rc;
 }
 
 }  // extern "C"

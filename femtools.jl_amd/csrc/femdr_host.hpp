@@ -214,3 +214,23 @@ static inline void cheb(double dtau, double lmin, double c_fact, double* a, doub
 
 int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
                 bool* thermal, bool* eos_p, bool* elastic);
+
+// The thermal side of a coupled thermal-Stokes solve (DR.jl:514-529,
+// 604-661): one thermal DR iteration per Stokes inner iteration, on the
+// thermal state's stream, with the Chebyshev scalars and the convergence
+// test carried between iterations.
+struct ThermalCoupling {
+    fem_thermal* t = nullptr;
+    double dt = 1, Tref = 273;
+    int64_t nout = 1;                // check cadence: the Stokes check_interval
+    const double* phi = nullptr;     // nullable: heat source at the velocity quadrature points [nels][7]
+    int64_t iter = 0;
+    double alpha = 0, beta = 0, nr0 = 0, err_T = INFINITY;
+    bool converged = false;
+    bool checked = false;            // the last iteration was a check
+};
+int thermal_coupling_begin(ThermalCoupling& tc, const fem_bc* bc);   // tables, Dirichlet set, ∂T∂τ = R0 = 0
+int thermal_coupling_step(ThermalCoupling& tc);
+fem_mesh* thermal_mesh(const fem_thermal* t);
+fem_ctx* thermal_ctx(const fem_thermal* t);
+double* thermal_field_ptr(fem_thermal* t, int32_t field);

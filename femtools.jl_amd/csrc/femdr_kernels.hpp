@@ -307,6 +307,9 @@ struct ElemArgs {
     int32_t bulk_all_inf;
     int32_t thermal;
     const F* T; const F* T0; const F* Q; const F* Pf;   // nullable
+    // nullable, residual launches only: the shear dissipation τ : (ε̇ − ε̇ᵉˡ)
+    // per quadrature point [nels][7]
+    F* phi_out;
 };
 
 // Workgroups are dealt round-robin over the 8 XCDs (one L2 each). The grid
@@ -688,6 +691,15 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             A.tau_out0[e * NQ + q] = txx;
             A.tau_out1[e * NQ + q] = tyy;
             A.tau_out2[e * NQ + q] = txy;
+        }
+        if (!ITER && !LIN && A.phi_out) {
+            // stress power less the elastic power (τ − τ_old) : τ / (2GΔt), the
+            // out-of-plane deviator τzz = −(τxx + τyy) included (shear_heating.jl:77-88)
+            const F tzz = -(txx + tyy), tzzo = -(txo + tyo);
+            const F power = txx * gxx + tyy * gyy + txy * (gxy + gyx);
+            const F elastic = (txx * (txx - txo) + tyy * (tyy - tyo) + tzz * (tzz - tzzo) +
+                               F(2) * txy * (txy - txyo)) * m.inv2Gdt;
+            A.phi_out[e * NQ + q] = power - elastic;
         }
         const F dOm = (F)kT.w[q] * det;
         const F Ptot = Pq + Pnq;

@@ -31,6 +31,7 @@ struct ThermArgs {
     const uint8_t* bcflag; const double* bcval;
     double alpha, beta;
     int32_t update;
+    const double* phi;                                       // nullable: source at the quadrature points [nels][7]
 };
 
 template <bool STRUCT, bool JAC>
@@ -84,7 +85,9 @@ __global__ __launch_bounds__(128) void k_thermal_element(ThermArgs A) {
         }
         const double rho = rq * (1 - aq * (Tq - A.Tref) + bq * Pq);
         const double D = A.dt / (rho * cq);
-        const double rate = T0q - Tq + D * sq;
+        double rate = T0q - Tq + D * sq;
+        const double phq = A.phi ? A.phi[e * NQ + q] : 0.0;
+        if (A.phi) rate += D * phq;   // Δt/(ρ Cp) Φ_q (residual.jl:75-79)
         const double dOm = kT.w[q] * det;
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
@@ -100,7 +103,9 @@ __global__ __launch_bounds__(128) void k_thermal_element(ThermArgs A) {
                 for (int j = 0; j < NV; ++j) {
                     const double dD = c * kT.N[q][j];
                     const double Kij = kq * (G[i][0] * G[j][0] + G[i][1] * G[j][1]);
-                    J[i * NV + j] += ((-kT.N[q][j] + dD * sq) * kT.N[q][i] - dD * KTi - D * Kij) * dOm;
+                    double dr = -kT.N[q][j] + dD * sq;
+                    if (A.phi) dr += dD * phq;
+                    J[i * NV + j] += (dr * kT.N[q][i] - dD * KTi - D * Kij) * dOm;
                 }
             }
         }
@@ -350,7 +355,96 @@ int thermal_assemble(fem_thermal* t, ThermArgs& a, bool jac) {
     return check_launch(c, "k_thermal_gather");
 }
 
+// The rest of a check iteration after its Jacobian assembly: λmax before the
+// update (solver.jl:97), the rate and temperature update with the current
+// Chebyshev scalars (a.alpha, a.beta), then ‖R‖ and the two sums of the
+// Rayleigh-quotient λmin (solver.jl:107-117), with Δτ = 2/√λmax CFL.
+int thermal_check(fem_thermal* t, const ThermArgs& a, double* nr, double* den, double* num, double* dtau) {
+    fem_ctx* c = t->ctx;
+    const int64_t nn = t->mesh->nnodes;
+    const unsigned rg = std::min<unsigned>(RED_GRID, grid_for(nn, RED_BLOCK));
+    hipLaunchKernelGGL(k_reduce_lmax<double>, dim3(rg), dim3(RED_BLOCK), 0, c->stream, nn,
+                       (const double*)a.dRdT, (const double*)a.dRdT, (const double*)a.PC, (const double*)a.PC,
+                       c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)rg, 3, 1,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_reduce_lmax"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double lmax = c->h_red[2] > 0 ? std::numeric_limits<double>::quiet_NaN() : c->h_red[0];
+    if (!(std::isfinite(lmax) && lmax > 0))
+        return set_err(c, FEM_EPRECOND,
+                       "thermal diffusion preconditioner produced invalid λmax = %g; check for zero or "
+                       "non-finite preconditioner entries", lmax);
+    hipLaunchKernelGGL(k_thermal_update, dim3(grid_for(nn, 256)), dim3(256), 0, c->stream, nn,
+                       (const double*)a.R, (const double*)a.PC, a.rate, a.Tw, (const uint8_t*)a.bcflag,
+                       (const double*)a.bcval, a.alpha, a.beta);
+    CHK(check_launch(c, "k_thermal_update"));
+    *dtau = 2 / std::sqrt(lmax) * t->opts.CFL;
+    hipLaunchKernelGGL(k_reduce_check<double>, dim3(rg), dim3(RED_BLOCK), 0, c->stream, nn, (const double*)a.R,
+                       (const double*)a.R, (const double*)a.R0, (const double*)a.R0, (const double*)a.rate,
+                       (const double*)a.rate, (const double*)a.PC, (const double*)a.PC, *dtau, *dtau,
+                       c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)rg, 6, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_reduce_check"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *nr = std::sqrt(c->h_red[0]);
+    *den = c->h_red[2];
+    *num = c->h_red[3];
+    return FEM_OK;
+}
+
 }  // namespace
+
+// One thermal iteration of the coupled inner loop (DR.jl:604-661): R0 <- R
+// and the Jacobian diagnostics at checks (thermal_iter == 1 or a multiple of
+// the check cadence), the rate/temperature update, and at checks the
+// relative residual, λmin and the next Chebyshev scalars.
+int thermal_coupling_begin(ThermalCoupling& tc, const fem_bc* bc) {
+    fem_thermal* t = tc.t;
+    fem_ctx* c = t->ctx;
+    const int64_t nn = t->mesh->nnodes;
+    CHK(thermal_prepare(t));
+    CHK(thermal_upload_bc(t, bc));
+    HIPCHK(c, hipMemsetAsync(t->f[FEM_TFIELD_DTDTAU], 0, nn * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(t->f[FEM_TFIELD_R0], 0, nn * 8, c->stream));
+    tc.iter = 0;
+    tc.alpha = tc.beta = tc.nr0 = 0;
+    tc.err_T = INFINITY;
+    tc.converged = tc.checked = false;
+    return FEM_OK;
+}
+
+int thermal_coupling_step(ThermalCoupling& tc) {
+    fem_thermal* t = tc.t;
+    fem_ctx* c = t->ctx;
+    ++tc.iter;
+    const bool chk = tc.iter == 1 || tc.iter % tc.nout == 0;
+    tc.checked = chk;
+    ThermArgs a = thermal_args(t, tc.dt, tc.Tref);
+    a.phi = tc.phi;
+    a.alpha = tc.alpha;
+    a.beta = tc.beta;
+    a.update = !chk;
+    CHK(thermal_assemble(t, a, chk));
+    if (!chk) return FEM_OK;
+    double nr, den, num, dtau;
+    CHK(thermal_check(t, a, &nr, &den, &num, &dtau));
+    if (tc.iter == 1) tc.nr0 = std::max(nr, julia_eps(nr));
+    tc.err_T = nr / tc.nr0;
+    if (!std::isfinite(tc.err_T))
+        return set_err(c, FEM_ENAN, "non-finite thermal residual at coupled iteration %lld", (long long)tc.iter);
+    const double lmin = (tc.iter == 1 || den == 0) ? 0.0 : std::fabs(num) / den;
+    cheb(dtau, lmin, t->opts.c_fact, &tc.alpha, &tc.beta);
+    tc.converged = tc.err_T < t->opts.eps;
+    return FEM_OK;
+}
+
+fem_mesh* thermal_mesh(const fem_thermal* t) { return t->mesh; }
+fem_ctx* thermal_ctx(const fem_thermal* t) { return t->ctx; }
+double* thermal_field_ptr(fem_thermal* t, int32_t field) { return t->f[field]; }
 
 extern "C" {
 
@@ -428,13 +522,11 @@ int fem_thermal_solve(fem_thermal* t, const fem_bc* bc, const fem_thermal_params
     if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
     if (!(p->dt > 0) || !std::isfinite(p->dt)) return set_err(c, FEM_EARG, "dt must be positive and finite");
     const auto t0 = std::chrono::steady_clock::now();
-    const int64_t nn = t->mesh->nnodes;
     CHK(thermal_prepare(t));
     CHK(thermal_upload_bc(t, bc));
     const double tol = p->tolerance >= 0 ? p->tolerance : t->opts.eps;
     ThermArgs a = thermal_args(t, p->dt, p->Tref);
     double a_dr = 0, b_dr = 0, nr0 = 0, last_rel = std::numeric_limits<double>::quiet_NaN();
-    const unsigned rg = std::min<unsigned>(RED_GRID, grid_for(nn, RED_BLOCK));
     bool converged = false;
     int64_t it_done = 0;
     for (int64_t it = 1; it <= p->max_iterations; ++it) {
@@ -445,41 +537,13 @@ int fem_thermal_solve(fem_thermal* t, const fem_bc* bc, const fem_thermal_params
         a.update = !do_jac;
         CHK(thermal_assemble(t, a, do_jac));
         if (!do_jac) continue;
-        // λmax before the update (solver.jl:97)
-        hipLaunchKernelGGL(k_reduce_lmax<double>, dim3(rg), dim3(RED_BLOCK), 0, c->stream, nn,
-                           (const double*)a.dRdT, (const double*)a.dRdT, (const double*)a.PC, (const double*)a.PC,
-                           c->red_partial);
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)rg, 3, 1,
-                           (const double*)c->red_partial, c->red_out);
-        CHK(check_launch(c, "k_reduce_lmax"));
-        HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        double lmax = c->h_red[2] > 0 ? std::numeric_limits<double>::quiet_NaN() : c->h_red[0];
-        if (!(std::isfinite(lmax) && lmax > 0))
-            return set_err(c, FEM_EPRECOND,
-                           "thermal diffusion preconditioner produced invalid λmax = %g; check for zero or "
-                           "non-finite preconditioner entries", lmax);
-        hipLaunchKernelGGL(k_thermal_update, dim3(grid_for(nn, 256)), dim3(256), 0, c->stream, nn,
-                           (const double*)a.R, (const double*)a.PC, a.rate, a.Tw, (const uint8_t*)a.bcflag,
-                           (const double*)a.bcval, a_dr, b_dr);
-        CHK(check_launch(c, "k_thermal_update"));
-        const double dtau = 2 / std::sqrt(lmax) * t->opts.CFL;
-        hipLaunchKernelGGL(k_reduce_check<double>, dim3(rg), dim3(RED_BLOCK), 0, c->stream, nn, (const double*)a.R,
-                           (const double*)a.R, (const double*)a.R0, (const double*)a.R0, (const double*)a.rate,
-                           (const double*)a.rate, (const double*)a.PC, (const double*)a.PC, dtau, dtau,
-                           c->red_partial);
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)rg, 6, 0,
-                           (const double*)c->red_partial, c->red_out);
-        CHK(check_launch(c, "k_reduce_check"));
-        HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        const double nr = std::sqrt(c->h_red[0]);
+        double nr, den, num, dtau;
+        CHK(thermal_check(t, a, &nr, &den, &num, &dtau));
         if (it == 1) {
             const double a_ = std::fabs(nr);
             nr0 = std::max(nr, std::nextafter(a_, std::numeric_limits<double>::infinity()) - a_);
         }
         if (std::isnan(nr / nr0)) return set_err(c, FEM_ENAN, "NaNs at PT iter %lld", (long long)it);
-        const double den = c->h_red[2], num = c->h_red[3];
         const double lmin = (it == 1 || den == 0) ? 0.0 : std::fabs(num) / den;
         const double cc = 2 * std::sqrt(lmin) * t->opts.c_fact;
         a_dr = 2 * dtau * dtau / (2 + cc * dtau);

@@ -29,6 +29,7 @@ __all__ = [
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
+    "solve_coupled", "assemble_shear_heating",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -122,6 +123,14 @@ class _ThermalStats(C.Structure):
     _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("seconds", D)]
 
 
+class _CoupledParams(C.Structure):
+    _fields_ = [("Tref", D), ("shear_heating", I32)]
+
+
+class _CoupledStats(C.Structure):
+    _fields_ = [("err_T", D), ("thermal_iterations", I64), ("thermal_converged", I32)]
+
+
 def _sig(name, restype, *argtypes):
     f = getattr(_lib, name)
     f.restype = restype
@@ -167,6 +176,9 @@ _sig("fem_thermal_destroy", C.c_int, VP)
 _sig("fem_thermal_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
 _sig("fem_thermal_solve", C.c_int, VP, _P(_BC), _P(_ThermalParams), _P(_ThermalStats))
 _sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
+_sig("fem_stokes_shear_heating", C.c_int, VP, _P(_DP), D, VP)
+_sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
+     _P(_SolveStats), _P(_CoupledStats))
 
 EXPORTED = [
     "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
@@ -177,7 +189,7 @@ EXPORTED = [
     "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
     "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
-    "fem_thermal_solve", "fem_thermal_residual",
+    "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
 ]
 
 
@@ -816,3 +828,61 @@ def thermal_residual(dr: ThermalDiffusionDR, dt, Tref=273.0, compute_jacobian=Fa
     """assemble_diffusion_matrices_atomix! (heat residual.jl:90-121): dr.R and,
     with compute_jacobian, dr.dRdT and dr.PC."""
     dr.ctx.check(_lib.fem_thermal_residual(dr.handle, float(dt), float(Tref), int(bool(compute_jacobian))))
+
+
+# ---------------------------------------------------------------------------
+# coupled thermal--Stokes (src/stokes/solvers/DR.jl:44-165, 514-661)
+# ---------------------------------------------------------------------------
+def assemble_shear_heating(dr: StokesDR, mesh: MixedMesh, dt, plastic=None) -> np.ndarray:
+    """assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
+    Φ = τ : (ε̇ − (τ − τ_old)/(2GΔt)) at the velocity quadrature points for the
+    current dr.v, dr.P and dr.tau_old, as an (nels, 7) array."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    d, dp = _dp_ptr(plastic)
+    buf = VP()
+    dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, mesh.nels * 7 * 8, C.byref(buf)))
+    try:
+        phi = DeviceArray(dr.ctx, buf.value, (mesh.nels, 7))
+        dr.ctx.check(_lib.fem_stokes_shear_heating(dr.handle, dp, float(dt), phi.ptr))
+        return phi.numpy()
+    finally:
+        _lib.fem_buf_free(dr.ctx.handle, buf)
+
+
+def solve_coupled(thermal: ThermalDiffusionDR, stokes: StokesDR, thermal_mesh: MixedMesh, stokes_mesh: MixedMesh,
+                  bc_T: DirichletBoundaryCondition, bc_vx: DirichletBoundaryCondition,
+                  bc_vy: DirichletBoundaryCondition, *, dt, Tref=273.0, shear_heating=False, **kwargs):
+    """solve_coupled!(thermal, stokes, thermal_mesh, stokes_mesh, bc_T, bc_vx,
+    bc_vy; dt, Tref=273, shear_heating=false, kwargs...) (DR.jl:81-165): one
+    coupled step of size dt. Every inner Stokes iteration first advances one
+    thermal DR iteration and gathers the thermal field at the pressure DoFs'
+    vertex nodes into stokes.T. The remaining keywords are solve_stokes's
+    (check_interval sets the check cadence of both relaxations). Returns the
+    Stokes stats dict plus err_T and thermal_iterations; converged requires
+    the thermal test (err_T < thermal.eps) too. thermal.source is unchanged;
+    shear_heating adds τ : (ε̇ − ε̇ᵉˡ) at the quadrature points to it inside
+    the thermal residual, refreshed at every thermal check."""
+    if thermal_mesh.nnodes != stokes_mesh.nnodes:
+        raise DimensionMismatch("thermal mesh nodes must match the Stokes velocity-node layout")
+    if len(thermal.T) != thermal_mesh.nnodes:
+        raise DimensionMismatch("thermal state size must match thermal mesh nodes")
+    if len(stokes.T) != stokes_mesh.nnodesP:
+        raise DimensionMismatch("Stokes temperature size must match pressure DoFs")
+    if thermal_mesh is not thermal.mesh:
+        raise ValueError("thermal was built on a different mesh")
+    if thermal.ctx is not stokes.ctx:
+        raise ValueError("the thermal and Stokes states must share one Context")
+    keep = []
+    bcs, dp, p = _stokes_call(stokes, stokes_mesh, (bc_vx, bc_vy, None), keep, dt=dt, **kwargs)
+    bct = C.pointer(bc_T._c(keep)) if bc_T is not None else C.cast(None, _P(_BC))
+    cp = _CoupledParams(Tref=float(Tref), shear_heating=int(bool(shear_heating)))
+    st, cst = _SolveStats(), _CoupledStats()
+    rc = _lib.fem_coupled_solve(thermal.handle, stokes.handle, bct, bcs, dp, C.byref(p), C.byref(cp), C.byref(st),
+                                C.byref(cst))
+    stats = _stats_dict(st)
+    stats["err_T"] = cst.err_T
+    stats["thermal_iterations"] = cst.thermal_iterations
+    stats["thermal_converged"] = bool(cst.thermal_converged)
+    stokes.ctx.check(rc)
+    return stats

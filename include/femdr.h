@@ -327,6 +327,39 @@ int fem_thermal_solve(fem_thermal* th, const fem_bc* bc, const fem_thermal_param
  * dr.R, and dr.∂R∂T / dr.PC (|J| row sums / |J_ii|) when compute_jacobian. */
 int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t compute_jacobian);
 
+/* ------------------------------------------------- coupled thermal--Stokes */
+/* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
+ * phi (device, nq x nels = [nels][7] f64) receives τ : (ε̇ − (τ − τ_old)/(2GΔt))
+ * at the velocity quadrature points for the state's current v, P, τ_old.
+ * 2-D states. */
+int fem_stokes_shear_heating(fem_stokes* st, const fem_drucker_prager* plastic, double dt, double* phi);
+
+/* solve_coupled! keywords beyond those of solve! (DR.jl:81-91) */
+typedef struct {
+    double Tref;                /* thermal reference temperature, 273            */
+    int32_t shear_heating;      /* 1: τ : (ε̇ − ε̇ᵉˡ) added to the heat source,
+                                   refreshed at every thermal check             */
+} fem_coupled_params;
+
+/* the fields solve_coupled! merges into the Stokes stats (DR.jl:801) */
+typedef struct {
+    double err_T;               /* last thermal ‖R‖/‖R⁰‖ (inf before the first check) */
+    int64_t thermal_iterations;
+    int32_t thermal_converged;
+} fem_coupled_stats;
+
+/* solve_coupled!(thermal, stokes, thermal_mesh, stokes_mesh, bc_T, bc_v; dt, ...)
+ * (src/stokes/solvers/DR.jl:81-131, coupled loop DR.jl:514-661): the thermal
+ * field is transferred to the Stokes pressure DoFs (T and T0), then every
+ * inner Stokes iteration first advances one thermal DR iteration (tolerance
+ * th's ϵ, check cadence params->check_interval) and transfers T again. Both
+ * loop tests also wait for the thermal test; stats->converged requires it.
+ * Both states must share one fem_ctx and a node layout; 2-D, precision 0.
+ * The thermal source field is left unchanged. */
+int fem_coupled_solve(fem_thermal* th, fem_stokes* st, const fem_bc* bc_T, const fem_bc bc_v[2],
+                      const fem_drucker_prager* plastic, const fem_solve_params* params,
+                      const fem_coupled_params* coupled, fem_solve_stats* stats, fem_coupled_stats* cstats);
+
 #ifdef __cplusplus
 }
 #endif

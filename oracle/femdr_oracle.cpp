@@ -376,6 +376,22 @@ void stokes_cheb(double dtau, double lmin, double c_fact, double* a, double* b) 
     *b = (2 - c * dtau) / (2 + c * dtau);
 }
 
+// The thermal side of a coupled solve (DR.jl:514-529, 604-661): the thermal
+// state, its Chebyshev scalars and convergence, the shear-heating source.
+struct ThermalHook {
+    orc_thermal* t = nullptr;
+    orc_stokes* s = nullptr;
+    double dt = 1, Tref = 273, eps = 1e-6;
+    int64_t nout = 1;
+    bool shear = false;
+    std::vector<double> phi;
+    int64_t iter = 0;
+    double alpha = 0, beta = 0, lmax = 0, nr0 = 0, err_T = kInf;
+    bool converged = false;
+};
+void thermal_hook_begin(ThermalHook& h);
+int thermal_hook_step(ThermalHook& h);
+
 }  // namespace
 
 // ===========================================================================
@@ -747,8 +763,9 @@ void orc_dr_iterations(orc_stokes* s, double dt, int32_t finite_K, const double*
     }
 }
 
-// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802).
-int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
+// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802); th is the
+// thermal side of solve_coupled! (null for solve!).
+static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out, ThermalHook* th) {
     std::memset(out, 0, sizeof *out);
     g_err.clear();
     const double dt = p->dt;
@@ -778,6 +795,9 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
         dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
         stokes_cheb(dtau[c], 0.0, p->c_fact, &al[c], &be[c]);
     }
+
+    if (th) thermal_hook_begin(*th);
+    auto thermal_ok = [&]() { return !th || th->converged; };
 
     double err_min = kInf, eps_tol = p->tolerance, err = 2 * eps_tol;
     double err_abs = kInf, err_rel = kInf, err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
@@ -815,15 +835,19 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
         if (p->verbose)
             printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e - norm[Rv=%.3e %.3e, Rp=%.3e %.3e]\n",
                    (long long)itPH, (long long)iter, err, err_abs, err_rel, err_v, err_v_rel, err_P, err_P_rel);
-        if (err < eps_tol) { converged = true; break; }
+        if (err < eps_tol && thermal_ok()) { converged = true; break; }
         if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
         err_min = std::min(err_min, err);
         const double eps_vel = err * rel_drop;
         int64_t itPT = 0;
-        while (err > eps_vel && itPT <= p->iter_max) {
+        while ((err > eps_vel || !thermal_ok()) && itPT <= p->iter_max) {
             ++itPT; ++iter;
             const bool do_check = iter % ncheck == 0;
             if (do_check) for (int c = 0; c < 2; ++c) std::copy(Rv[c], Rv[c] + nv, Rv0[c]);
+            if (th) {
+                const int rc = thermal_hook_step(*th);
+                if (rc != 0) { out->status = rc; return rc; }
+            }
             orc_pressure_residual(s, dt, finite_K, s->P0);
             for (int64_t i = 0; i < np; ++i) s->Pnum[i] = s->gammaP[i] * s->RP[i] / s->MP[i];
             orc_momentum_residual(s, dt, s->Pnum);
@@ -872,6 +896,8 @@ int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) {
     return 0;
 }
 
+int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out) { return stokes_solve_impl(s, p, out, nullptr); }
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -891,10 +917,12 @@ extern "C" {
 // integrate_residual, src/heat_diffusion/assembly/residual.jl:58-87. The
 // Jacobian is the exact derivative of the same expression: with
 // D = Δt/(ρ Cp) and ρ = ρ0 (1 - α (T - Tref) + β P), ∂D/∂T_j = D ρ0 α N_j / ρ.
-void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
-                           const double* T, const double* T0, const double* s, const int32_t* ph,
-                           const orc_thermal_material* m, const double* P, double dt, double Tref,
-                           double* Re, double* J) {
+// The quadrature-point source phi (nullable, [nq]) enters as Δt/(ρ Cp) Φ_q
+// (residual.jl:75-79), in the Jacobian through ∂D/∂T_j.
+static void thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
+                              const double* T, const double* T0, const double* s, const int32_t* ph,
+                              const orc_thermal_material* m, const double* P, double dt, double Tref,
+                              const double* phi, double* Re, double* J) {
     double beta[ORC_MAX_PHASES];
     for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];   // β = map(inv, K) (residual.jl:62-64)
     for (int i = 0; i < nn; ++i) Re[i] = 0;
@@ -918,7 +946,8 @@ void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double
         const double r0 = interp_nodes(Nv, nn, m->rho0, ph);
         const double rho = r0 * (1 - aq * (Tq - Tref) + bq * Pq);
         const double D = dt / (rho * interp_nodes(Nv, nn, m->Cp, ph));
-        const double rate = T0q - Tq + D * sq;
+        double rate = T0q - Tq + D * sq;
+        if (phi) rate += D * phi[q];
         for (int i = 0; i < nn; ++i) {
             const double KTi = kq * (G[2 * i] * gx + G[2 * i + 1] * gy);
             Re[i] += rate * Nv[i] * dOm[q] - D * KTi * dOm[q];
@@ -930,11 +959,20 @@ void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double
                 for (int j = 0; j < nn; ++j) {
                     const double dD = c * Nv[j];
                     const double Kij = kq * (G[2 * i] * G[2 * j] + G[2 * i + 1] * G[2 * j + 1]);
-                    J[i * nn + j] += ((-Nv[j] + dD * sq) * Nv[i] - dD * KTi - D * Kij) * dOm[q];
+                    double dr = -Nv[j] + dD * sq;
+                    if (phi) dr += dD * phi[q];
+                    J[i * nn + j] += (dr * Nv[i] - dD * KTi - D * Kij) * dOm[q];
                 }
             }
         }
     }
+}
+
+void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
+                           const double* T, const double* T0, const double* s, const int32_t* ph,
+                           const orc_thermal_material* m, const double* P, double dt, double Tref,
+                           double* Re, double* J) {
+    thermal_integrate(nn, nq, N, dNdx, dOm, T, T0, s, ph, m, P, dt, Tref, nullptr, Re, J);
 }
 
 // assemble_diffusion_matrices_atomix! (heat residual.jl:90-121) via
@@ -963,8 +1001,8 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
             Tl[a] = t->T[nd[a]]; T0l[a] = t->T0[nd[a]]; sl[a] = t->source[nd[a]]; Pl[a] = t->P[nd[a]];
             ph[a] = t->phases[nd[a]];
         }
-        orc_thermal_integrate(7, 7, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref, Re,
-                              compute_jacobian ? J : nullptr);
+        thermal_integrate(7, 7, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref,
+                          t->source_ip ? t->source_ip + e * 7 : nullptr, Re, compute_jacobian ? J : nullptr);
         for (int a = 0; a < 7; ++a) t->R[nd[a]] += Re[a];
         if (compute_jacobian) {
             for (int a = 0; a < 7; ++a) {
@@ -1031,6 +1069,147 @@ int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_s
     out->residual = last_rel;
     out->status = 0;
     return 0;
+}
+
+// assemble_shear_heating! (shear_heating.jl:18-92), plane strain: the stress
+// power τ : ∇v less the elastic power, whose sum includes τzz = −(τxx + τyy)
+// (shear_dissipation, shear_heating.jl:77-88).
+void orc_shear_heating(const orc_stokes* s, double dt, double* phi) {
+    const Tables& tb = tables();
+    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+    double invG[ORC_MAX_PHASES];
+    for (int p = 0; p < s->mat.nphases; ++p) invG[p] = 1.0 / s->mat.G[p];
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        int ph[7];
+        elem_phases_v(s, e, ph);
+        for (int q = 0; q < 7; ++q) {
+            double gvx[2] = {0, 0}, gvy[2] = {0, 0};
+            for (int a = 0; a < 7; ++a) {
+                const int32_t g = s->el2n[e * 7 + a];
+                gvx[0] += dNdx[q][a][0] * s->vx[g];
+                gvx[1] += dNdx[q][a][1] * s->vx[g];
+                gvy[0] += dNdx[q][a][0] * s->vy[g];
+                gvy[1] += dNdx[q][a][1] * s->vy[g];
+            }
+            double to[3] = {0, 0, 0};
+            for (int c = 0; c < 3; ++c)
+                if (s->tau_old[c]) to[c] = s->tau_old[c][e * 7 + q];
+            double Pq = 0, Pfq = 0;
+            for (int i = 0; i < 3; ++i) {
+                Pq += tb.NqP[q][i] * s->P[e * 3 + i];
+                if (s->Pf) Pfq += tb.NqP[q][i] * s->Pf[e * 3 + i];
+            }
+            double tau[3];
+            stress_ip<double>(gvx, gvy, tb.Nq[q], 7, ph, r, to, Pq - Pfq, tau);
+            const double inv2Gdt = interp_phase(tb.Nq[q], 7, invG, ph) / (2 * dt);
+            const double tzz = -(tau[0] + tau[1]), tzzo = -(to[0] + to[1]);
+            const double power = tau[0] * gvx[0] + tau[1] * gvy[1] + tau[2] * (gvx[1] + gvy[0]);
+            const double elastic = (tau[0] * (tau[0] - to[0]) + tau[1] * (tau[1] - to[1]) + tzz * (tzz - tzzo) +
+                                    2 * tau[2] * (tau[2] - to[2])) * inv2Gdt;
+            phi[e * 7 + q] = power - elastic;
+        }
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+// _transfer_temperature_kernel! (DR.jl:44-47): T_P[DoFsP[i]] = T[el2nP[i]];
+// the P1 DoFs of element e sit at its three vertex nodes
+void transfer_T(const orc_stokes* s, const double* T, double* TP) {
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 3; ++a) TP[e * 3 + a] = T[s->el2n[e * 7 + a]];
+}
+
+// DR.jl:524-530: zero ∂T∂τ and R0, initial dissipation
+void thermal_hook_begin(ThermalHook& h) {
+    orc_thermal* t = h.t;
+    std::fill(t->rate, t->rate + t->nnodes, 0.0);
+    std::fill(t->R0, t->R0 + t->nnodes, 0.0);
+    if (h.shear) orc_shear_heating(h.s, h.dt, h.phi.data());
+}
+
+// One thermal DR iteration of the coupled inner loop (DR.jl:604-661)
+int thermal_hook_step(ThermalHook& h) {
+    orc_thermal* t = h.t;
+    const int64_t n = t->nnodes;
+    ++h.iter;
+    const bool chk = h.iter == 1 || h.iter % h.nout == 0;
+    if (chk) std::copy(t->R, t->R + n, t->R0);
+    orc_thermal_residual(t, h.dt, h.Tref, chk);
+    dirichlet(t->R, t->nbc, t->bc_dofs, nullptr);
+    dirichlet(t->rate, t->nbc, t->bc_dofs, nullptr);
+    if (chk && !checked_lmax(t->dRdT, t->PC, n, "thermal diffusion", &h.lmax)) return -6;
+    for (int64_t i = 0; i < n; ++i) t->rate[i] = t->R[i] / t->PC[i] + h.beta * t->rate[i];
+    for (int64_t i = 0; i < n; ++i) t->T[i] += h.alpha * t->rate[i];
+    dirichlet(t->T, t->nbc, t->bc_dofs, t->bc_vals);
+    transfer_T(h.s, t->T, h.s->T);
+    if (chk) {
+        const double nr = nrm2(t->R, n);
+        if (h.iter == 1) h.nr0 = std::max(nr, julia_eps(nr));
+        h.err_T = nr / h.nr0;
+        if (!std::isfinite(h.err_T)) {
+            g_err = "non-finite thermal residual at coupled iteration " + std::to_string(h.iter);
+            return -4;
+        }
+        const double dtau = 2 / std::sqrt(h.lmax) * t->CFL;
+        double den = 0, num = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double r = dtau * t->rate[i];
+            den += r * r;
+        }
+        if (!(h.iter == 1 || den == 0))
+            for (int64_t i = 0; i < n; ++i) num += (dtau * t->rate[i]) * ((t->R[i] - t->R0[i]) / t->PC[i]);
+        const double lmin = (h.iter == 1 || den == 0) ? 0.0 : std::fabs(num) / den;
+        stokes_cheb(dtau, lmin, t->c_fact, &h.alpha, &h.beta);
+        h.converged = h.err_T < h.eps;
+        // the dissipation is refreshed after the spectral estimate (DR.jl:659-661)
+        if (h.shear) orc_shear_heating(h.s, h.dt, h.phi.data());
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// solve_coupled! (DR.jl:81-131) with the coupled loop of _solve_stokes_dyrel!
+int orc_coupled_solve(orc_thermal* t, orc_stokes* s, const orc_params* p, const orc_coupled_params* cp,
+                      orc_stats* out, orc_coupled_stats* cst) {
+    std::memset(out, 0, sizeof *out);
+    std::memset(cst, 0, sizeof *cst);
+    g_err.clear();
+    if (t->nnodes != s->nnodes) {
+        g_err = "thermal mesh nodes must match the Stokes velocity-node layout";
+        out->status = -2;
+        return -2;
+    }
+    transfer_T(s, t->T, s->T);
+    transfer_T(s, t->T0, s->T0);
+    ThermalHook h;
+    h.t = t;
+    h.s = s;
+    h.dt = p->dt;
+    h.Tref = cp->Tref;
+    h.eps = cp->tolerance_T;
+    h.nout = p->check_interval;
+    h.shear = cp->shear_heating != 0;
+    const double* saved = t->source_ip;
+    if (h.shear) {
+        h.phi.assign((size_t)s->nels * 7, 0.0);
+        t->source_ip = h.phi.data();
+    }
+    const int rc = stokes_solve_impl(s, p, out, &h);
+    t->source_ip = saved;
+    cst->err_T = h.err_T;
+    cst->thermal_iterations = h.iter;
+    cst->thermal_converged = h.converged;
+    if (rc == 0) out->converged = out->converged && h.converged;
+    return rc;
 }
 
 }  // extern "C"

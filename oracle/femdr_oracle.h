@@ -241,6 +241,8 @@ typedef struct {
     const int32_t* bc_dofs; /* 0-based */
     const double* bc_vals;
     double CFL, c_fact;     /* 0.98, 0.9 */
+    const double* source_ip; /* nullable [nels][7]: heat source at the quadrature points
+                                (source_ip of residual.jl:7-47, the shear heating of a coupled solve) */
 } orc_thermal;
 
 typedef struct {
@@ -270,6 +272,34 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
 /* solve!(::ThermalDiffusionDR, ...) -> solve_dynamic_relaxation!
  * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
 int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
+
+/* --- coupled thermal--Stokes step (src/stokes/solvers/DR.jl:44-131, 514-661) --- */
+
+/* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-92):
+ * phi [nels][7] = τ : (ε̇ − (τ − τ_old)/(2GΔt)) at the velocity quadrature
+ * points, τ the stress of the momentum residual for the current v and P */
+void orc_shear_heating(const orc_stokes* s, double dt, double* phi);
+
+typedef struct {
+    double Tref;              /* thermal Tref keyword (default 273)            */
+    double tolerance_T;       /* thermal dr.ϵ                                  */
+    int32_t shear_heating;
+} orc_coupled_params;
+
+typedef struct {
+    double err_T;             /* last thermal ‖R‖/‖R⁰‖                         */
+    int64_t thermal_iterations;
+    int32_t thermal_converged;
+} orc_coupled_stats;
+
+/* solve_coupled!: one thermal DR iteration per Stokes inner iteration, the
+ * thermal field transferred to the Stokes pressure DoFs (T and T0), the
+ * convergence tests of both loops extended by the thermal test. The
+ * thermal state's R, T and ∂T∂τ (rate) carry the iteration; t->source_ip is
+ * set by the call when shear heating is on. Stokes statistics in out;
+ * out->converged requires the thermal test too. */
+int orc_coupled_solve(orc_thermal* t, orc_stokes* s, const orc_params* p, const orc_coupled_params* cp,
+                      orc_stats* out, orc_coupled_stats* cst);
 
 #ifdef __cplusplus
 }

@@ -84,7 +84,15 @@ class ThermalC(C.Structure):
     _fields_ = [("nnodes", I64), ("nels", I64), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("phases", PI),
                 ("mat", ThermalMaterialC), ("R", PD), ("R0", PD), ("dRdT", PD), ("PC", PD), ("T", PD), ("T0", PD),
                 ("rate", PD), ("P", PD), ("source", PD), ("nbc", I64), ("bc_dofs", PI), ("bc_vals", PD),
-                ("CFL", D), ("c_fact", D)]
+                ("CFL", D), ("c_fact", D), ("source_ip", PD)]
+
+
+class CoupledParamsC(C.Structure):
+    _fields_ = [("Tref", D), ("tolerance_T", D), ("shear_heating", I32)]
+
+
+class CoupledStatsC(C.Structure):
+    _fields_ = [("err_T", D), ("thermal_iterations", I64), ("thermal_converged", I32)]
 
 
 class ThermalParamsC(C.Structure):
@@ -127,6 +135,10 @@ def _declare(L):
     L.orc_thermal_residual.argtypes = [C.POINTER(ThermalC), D, D, I32]
     L.orc_thermal_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(ThermalParamsC), C.POINTER(ThermalStatsC)]
     L.orc_thermal_solve.restype = C.c_int
+    L.orc_shear_heating.argtypes = [C.POINTER(Stokes), D, PD]
+    L.orc_coupled_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(Stokes), C.POINTER(Params),
+                                    C.POINTER(CoupledParamsC), C.POINTER(Stats), C.POINTER(CoupledStatsC)]
+    L.orc_coupled_solve.restype = C.c_int
     L.orc_last_error.restype = C.c_char_p
     L.orc_t7_basis.argtypes = [D, D, PD, PD]
     L.orc_t7_rule.argtypes = [PD, PD, PD]
@@ -344,13 +356,23 @@ class Problem:
         lib().orc_update_current_stress(C.byref(self.struct()), dt)
         return self.tau_store
 
-    def solve(self, dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
-              check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
-              finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False):
-        p = Params(dt=dt, pressure_factor=pressure_factor, tolerance=tolerance, max_iterations=max_iterations,
-                   iter_max=iter_max, check_interval=check_interval, max_ph_iterations=max_ph_iterations,
-                   rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
-                   freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
+    def shear_heating(self, dt):
+        """assemble_shear_heating! (shear_heating.jl:18-92): Φ [nels][7]."""
+        phi = np.zeros((self.nels, 7))
+        lib().orc_shear_heating(C.byref(self.struct()), dt, ptr(phi))
+        return phi
+
+    @staticmethod
+    def params(dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
+               check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
+               finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False):
+        return Params(dt=dt, pressure_factor=pressure_factor, tolerance=tolerance, max_iterations=max_iterations,
+                      iter_max=iter_max, check_interval=check_interval, max_ph_iterations=max_ph_iterations,
+                      rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
+                      freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
+
+    def solve(self, dt, **kw):
+        p = self.params(dt, **kw)
         st = Stats()
         rc = lib().orc_stokes_solve(C.byref(self.struct()), C.byref(p), C.byref(st))
         if rc != 0:
@@ -425,6 +447,7 @@ class ThermalProblem:
         t.nbc = len(self.bc[0])
         t.bc_dofs, t.bc_vals = ptr(self.bc[0], PI), ptr(self.bc[1])
         t.CFL, t.c_fact = self.CFL, self.c_fact
+        t.source_ip = ptr(None)
         self._struct = t
         return t
 
@@ -439,6 +462,24 @@ class ThermalProblem:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual}
+
+
+def coupled_solve(thermal: ThermalProblem, stokes: Problem, dt, Tref=273.0, tolerance_T=1e-6, shear_heating=False,
+                  **kw):
+    """solve_coupled! (src/stokes/solvers/DR.jl:81-131): one thermal DR
+    iteration per Stokes inner iteration. Keywords as Problem.solve; returns
+    the Stokes stats with err_T, thermal_iterations and thermal_converged."""
+    p = Problem.params(dt, **kw)
+    cp = CoupledParamsC(Tref=Tref, tolerance_T=tolerance_T, shear_heating=int(bool(shear_heating)))
+    st, cst = Stats(), CoupledStatsC()
+    ts, ss = thermal.struct(), stokes.struct()
+    rc = lib().orc_coupled_solve(C.byref(ts), C.byref(ss), C.byref(p), C.byref(cp), C.byref(st), C.byref(cst))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    out = st.as_dict()
+    out.update(err_T=cst.err_T, thermal_iterations=cst.thermal_iterations,
+               thermal_converged=bool(cst.thermal_converged))
+    return out
 
 
 # ---------------------------------------------------------------------------
