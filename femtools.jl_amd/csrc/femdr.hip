@@ -1580,6 +1580,44 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     return rc;
 }
 
+// rotate_stress! (src/stokes/stress_rotation.jl:28-61) in plane strain: at
+// every point ω = ½(∂vx/∂y − ∂vy/∂x) and τ_old = R(θ) τ R(θ)ᵀ with θ = ωΔt,
+// R = [cos θ  −sin θ; sin θ  cos θ] (GeoParams' 2-D convention). One thread
+// per element and its seven points.
+__global__ __launch_bounds__(128) void k_rotate_stress2(int64_t ne, const int32_t* el2n, const double* geo,
+                                                        const double* vx, const double* vy, const double* t0,
+                                                        const double* t1, const double* t2, double* o0, double* o1,
+                                                        double* o2, double dt) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double ux[NV], uy[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+        const int32_t n = el2n[a * ne + e];
+        ux[a] = vx[n];
+        uy[a] = vy[n];
+    }
+    const double i00 = geo[0 * ne + e], i01 = geo[1 * ne + e], i10 = geo[2 * ne + e], i11 = geo[3 * ne + e];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double ax = 0, ay = 0, bx = 0, by = 0;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+            ax += d0 * ux[a]; ay += d1 * ux[a];
+            bx += d0 * uy[a]; by += d1 * uy[a];
+        }
+        const double dvx_dy = ax * i01 + ay * i11, dvy_dx = bx * i00 + by * i10;
+        const double th = 0.5 * (dvx_dy - dvy_dx) * dt;
+        const double c = cos(th), sn = sin(th);
+        const int64_t k = e * NQ + q;
+        const double xx = t0[k], yy = t1[k], xy = t2[k];
+        o0[k] = c * c * xx - 2 * c * sn * xy + sn * sn * yy;
+        o1[k] = sn * sn * xx + 2 * c * sn * xy + c * c * yy;
+        o2[k] = c * sn * (xx - yy) + (c * c - sn * sn) * xy;
+    }
+}
+
 // The thermal half of a coupled solve (solve_coupled!, DR.jl:81-131)
 struct CoupledSide {
     ThermalCoupling tc;
@@ -1861,6 +1899,27 @@ int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prage
                      fem_solve_stats* st) {
     if (!s || !bcs || !p || !st) return FEM_EARG;
     return solve_2d(s, bcs, dp, p, st, nullptr);
+}
+
+// rotate_stress!(dr, mesh, Δt) (stress_rotation.jl:16-19)
+int fem_stokes_rotate_stress(fem_stokes* s, double dt) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!std::isfinite(dt)) return set_err(c, FEM_EARG, "dt must be finite");
+    if (s->dim == 3) {
+        CHK(hex27_rotate_stress(s, dt));
+        return fem_sync(c);
+    }
+    const int64_t ne = s->mesh->nels;
+    double** f = s->f;
+    hipLaunchKernelGGL(k_rotate_stress2, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, ne,
+                       (const int32_t*)s->mesh->d_el2n, (const double*)s->mesh->d_geo, (const double*)f[FEM_FIELD_V_X],
+                       (const double*)f[FEM_FIELD_V_Y], (const double*)f[FEM_FIELD_TAU_XX],
+                       (const double*)f[FEM_FIELD_TAU_YY], (const double*)f[FEM_FIELD_TAU_XY], f[FEM_FIELD_TAU_OLD_XX],
+                       f[FEM_FIELD_TAU_OLD_YY], f[FEM_FIELD_TAU_OLD_XY], dt);
+    CHK(check_launch(c, "k_rotate_stress2"));
+    return fem_sync(c);
 }
 
 // assemble_shear_heating! (shear_heating.jl:18-46)

@@ -926,6 +926,90 @@ int hex27_jacobian(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
     return hex_jacobian_impl(s, o);
 }
 
+// rotate_stress! (src/stokes/stress_rotation.jl:28-61) in 3-D: at every
+// Gauss point ω = curl v and τ_old = R τ Rᵀ with R the rotation by the
+// vector ωΔt/2 (Rodrigues' formula; no rotation where ω = 0). One thread per
+// (element, point): the 27 nodal velocities are read per point through the
+// caches; the step runs once per time step, outside the DR loop.
+struct RotArgs3 {
+    HexGeo g;
+    const double *vx, *vy, *vz;
+    const double* tau[6];   // assembler order xx, yy, zz, xy, xz, yz
+    double* out[6];
+    double dt;
+};
+
+__global__ __launch_bounds__(256) void k_hex27_rotate(RotArgs3 A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.g.nels * HNQ) return;
+    const int64_t e = i / HNQ;
+    const int q = (int)(i - e * HNQ);
+    const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
+    double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // G[c][k] = ∂v_c/∂x_k
+    for (int t = 0; t < HNV; ++t) {
+        const int r0 = t % 3, r1 = (t / 3) % 3, r2 = t / 9;
+        const double L0 = lq(x0, r0), L1 = lq(x1, r1), L2 = lq(x2, r2);
+        const double d[3] = {dlq(x0, r0) * L1 * L2 * A.g.jac[0], L0 * dlq(x1, r1) * L2 * A.g.jac[1],
+                             L0 * L1 * dlq(x2, r2) * A.g.jac[2]};
+        const int64_t n = hex_node(A.g, e, t);
+        const double v[3] = {A.vx[n], A.vy[n], A.vz[n]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) G[c][k] += d[k] * v[c];
+    }
+    const double r[3] = {0.5 * A.dt * (G[2][1] - G[1][2]), 0.5 * A.dt * (G[0][2] - G[2][0]),
+                         0.5 * A.dt * (G[1][0] - G[0][1])};
+    const double th = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    double t[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t[c] = A.tau[c][i];
+    if (th == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) A.out[c][i] = t[c];
+        return;
+    }
+    const double n0 = r[0] / th, n1 = r[1] / th, n2 = r[2] / th;
+    const double sn = sin(th), cm = 1 - cos(th);
+    // R = I + sin θ K + (1 − cos θ) K², K the cross-product matrix of n (K² = n nᵀ − I)
+    const double R[3][3] = {{1 + cm * (n0 * n0 - 1), -sn * n2 + cm * n0 * n1, sn * n1 + cm * n0 * n2},
+                            {sn * n2 + cm * n1 * n0, 1 + cm * (n1 * n1 - 1), -sn * n0 + cm * n1 * n2},
+                            {-sn * n1 + cm * n2 * n0, sn * n0 + cm * n2 * n1, 1 + cm * (n2 * n2 - 1)}};
+    const double T[3][3] = {{t[0], t[3], t[4]}, {t[3], t[1], t[5]}, {t[4], t[5], t[2]}};
+    double RT[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) RT[a][b] = R[a][0] * T[0][b] + R[a][1] * T[1][b] + R[a][2] * T[2][b];
+    auto M = [&](int a, int b) { return RT[a][0] * R[b][0] + RT[a][1] * R[b][1] + RT[a][2] * R[b][2]; };
+    A.out[0][i] = M(0, 0);
+    A.out[1][i] = M(1, 1);
+    A.out[2][i] = M(2, 2);
+    A.out[3][i] = M(0, 1);
+    A.out[4][i] = M(0, 2);
+    A.out[5][i] = M(1, 2);
+}
+
+int hex27_rotate_stress(fem_stokes* s, double dt) {
+    RotArgs3 a;
+    a.g = hex_geo(s->mesh);
+    a.vx = s->f[FEM_FIELD_V_X];
+    a.vy = s->f[FEM_FIELD_V_Y];
+    a.vz = s->f[FEM_FIELD_V_Z];
+    const int tc[6] = {FEM_FIELD_TAU_XX, FEM_FIELD_TAU_YY, FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XY, FEM_FIELD_TAU_XZ,
+                       FEM_FIELD_TAU_YZ};
+    const int tf[6] = {FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_ZZ,
+                       FEM_FIELD_TAU_OLD_XY, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ};
+    for (int k = 0; k < 6; ++k) {
+        a.tau[k] = s->f[tc[k]];
+        a.out[k] = s->f[tf[k]];
+    }
+    a.dt = dt;
+    const int64_t n = a.g.nels * HNQ;
+    hipLaunchKernelGGL(k_hex27_rotate, dim3(grid_for(n, 256)), dim3(256), 0, s->ctx->stream, a);
+    return check_launch(s->ctx, "k_hex27_rotate");
+}
+
 int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt) {
     Opts3 o;
     CHK(prepare3(s, dp, dt, 0, o, true));

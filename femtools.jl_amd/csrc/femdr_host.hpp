@@ -197,6 +197,7 @@ int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, dou
 int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
                 fem_solve_stats* st);
 int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
+int hex27_rotate_stress(fem_stokes* s, double dt);
 
 // eps(x) of Julia (spacing at |x|) and the Chebyshev step/momentum (DR.jl:39-42)
 static inline double julia_eps(double x) {

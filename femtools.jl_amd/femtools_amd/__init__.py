@@ -29,7 +29,7 @@ __all__ = [
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
-    "solve_coupled", "assemble_shear_heating",
+    "solve_coupled", "assemble_shear_heating", "rotate_stress",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -177,6 +177,7 @@ _sig("fem_thermal_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
 _sig("fem_thermal_solve", C.c_int, VP, _P(_BC), _P(_ThermalParams), _P(_ThermalStats))
 _sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
 _sig("fem_stokes_shear_heating", C.c_int, VP, _P(_DP), D, VP)
+_sig("fem_stokes_rotate_stress", C.c_int, VP, D)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
 
@@ -190,6 +191,7 @@ EXPORTED = [
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
     "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
+    "fem_stokes_rotate_stress",
 ]
 
 
@@ -726,6 +728,16 @@ def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False):
     """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal)."""
     d, dp = _dp_ptr(plastic)
     dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
+
+
+def rotate_stress(dr: StokesDR, mesh: MixedMesh, dt):
+    """rotate_stress!(dr, mesh_stokes, Δt) (src/stokes/stress_rotation.jl:16-61):
+    dr.tau_old becomes dr.tau rotated by the local vorticity over Δt (2-D:
+    θ = ωΔt, ω = ½(∂vx/∂y − ∂vy/∂x); 3-D: the rotation vector curl(v)Δt/2)."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    dr.ctx.check(_lib.fem_stokes_rotate_stress(dr.handle, float(dt)))
+    return dr.tau_old
 
 
 def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:

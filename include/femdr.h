@@ -327,6 +327,13 @@ int fem_thermal_solve(fem_thermal* th, const fem_bc* bc, const fem_thermal_param
  * dr.R, and dr.∂R∂T / dr.PC (|J| row sums / |J_ii|) when compute_jacobian. */
 int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t compute_jacobian);
 
+/* rotate_stress!(dr, mesh_stokes, Δt) (src/stokes/stress_rotation.jl:16-61):
+ * τ_old at every velocity quadrature point becomes the current τ rotated by
+ * the local vorticity over Δt (GeoParams.rotate_elastic_stress: θ = ωΔt with
+ * ω = ½(∂vx/∂y − ∂vy/∂x) in 2-D; the rotation vector curl(v)Δt/2 in 3-D).
+ * 2-D and 3-D states. */
+int fem_stokes_rotate_stress(fem_stokes* st, double dt);
+
 /* ------------------------------------------------- coupled thermal--Stokes */
 /* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
  * phi (device, nq x nels = [nels][7] f64) receives τ : (ε̇ − (τ − τ_old)/(2GΔt))

@@ -1114,6 +1114,46 @@ void orc_shear_heating(const orc_stokes* s, double dt, double* phi) {
     }
 }
 
+// GeoParams.rotate_elastic_stress, 2-D (GeoParams 0.7.21, the version
+// Project.toml pins; restated from its documented behaviour and the
+// reference's tests, test/test_stress_rotation.jl:48-77): the stress turns by
+// θ = ωΔt, τ' = R(θ) τ R(θ)ᵀ with R = [cos θ  −sin θ; sin θ  cos θ].
+void orc_rotate_elastic_stress2(double omega, const double* tau, double dt, double* out) {
+    const double th = omega * dt;
+    const double c = std::cos(th), sn = std::sin(th);
+    const double R[2][2] = {{c, -sn}, {sn, c}};
+    const double T[2][2] = {{tau[0], tau[2]}, {tau[2], tau[1]}};
+    double RT[2][2], M[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) RT[i][j] = R[i][0] * T[0][j] + R[i][1] * T[1][j];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) M[i][j] = RT[i][0] * R[j][0] + RT[i][1] * R[j][1];
+    out[0] = M[0][0];
+    out[1] = M[1][1];
+    out[2] = M[0][1];
+}
+
+// rotate_stress! (src/stokes/stress_rotation.jl:28-61), 2-D: at every
+// quadrature point ω = ½(∂vx/∂y − ∂vy/∂x) and tau_old = rotated tau
+void orc_rotate_stress(const orc_stokes* s, double dt, const double* const* tau, double* const* tau_old) {
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        for (int q = 0; q < 7; ++q) {
+            double dvx_dy = 0, dvy_dx = 0;
+            for (int a = 0; a < 7; ++a) {
+                const int32_t g = s->el2n[e * 7 + a];
+                dvx_dy += dNdx[q][a][1] * s->vx[g];
+                dvy_dx += dNdx[q][a][0] * s->vy[g];
+            }
+            const double t[3] = {tau[0][e * 7 + q], tau[1][e * 7 + q], tau[2][e * 7 + q]};
+            double r[3];
+            orc_rotate_elastic_stress2((dvx_dy - dvy_dx) / 2, t, dt, r);
+            for (int c = 0; c < 3; ++c) tau_old[c][e * 7 + q] = r[c];
+        }
+    }
+}
+
 }  // extern "C"
 
 namespace {
@@ -1586,6 +1626,56 @@ static void momentum_residual3_impl(orc_stokes3* s, double dt, const double* Pnu
 
 void orc_momentum_residual3(orc_stokes3* s, double dt, const double* Pnum) { momentum_residual3_impl(s, dt, Pnum, nullptr); }
 void orc_update_current_stress3(orc_stokes3* s, double dt) { momentum_residual3_impl(s, dt, nullptr, s->tau_store); }
+
+// GeoParams.rotate_elastic_stress, 3-D (restated; test/test_stress_rotation.jl
+// :112-160): ω = curl v; the stress co-rotates by the rotation vector ωΔt/2,
+// R = I + sin θ K + (1 − cos θ) K² (Rodrigues; K the cross-product matrix of
+// the unit axis, θ = |ω|Δt/2), τ' = R τ Rᵀ; no rotation when ω = 0. tau in
+// assembler order (xx, yy, zz, xy, xz, yz).
+void orc_rotate_elastic_stress3(const double* omega, const double* tau, double dt, double* out) {
+    const double r[3] = {omega[0] * dt / 2, omega[1] * dt / 2, omega[2] * dt / 2};
+    const double th = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (th == 0) {
+        for (int c = 0; c < 6; ++c) out[c] = tau[c];
+        return;
+    }
+    const double n[3] = {r[0] / th, r[1] / th, r[2] / th};
+    const double K[3][3] = {{0, -n[2], n[1]}, {n[2], 0, -n[0]}, {-n[1], n[0], 0}};
+    double K2[3][3], R[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) K2[i][j] = K[i][0] * K[0][j] + K[i][1] * K[1][j] + K[i][2] * K[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i][j] = (i == j ? 1.0 : 0.0) + std::sin(th) * K[i][j] + (1 - std::cos(th)) * K2[i][j];
+    const double T[3][3] = {{tau[0], tau[3], tau[4]}, {tau[3], tau[1], tau[5]}, {tau[4], tau[5], tau[2]}};
+    double RT[3][3], M[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) RT[i][j] = R[i][0] * T[0][j] + R[i][1] * T[1][j] + R[i][2] * T[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[i][j] = RT[i][0] * R[j][0] + RT[i][1] * R[j][1] + RT[i][2] * R[j][2];
+    out[0] = M[0][0]; out[1] = M[1][1]; out[2] = M[2][2];
+    out[3] = M[0][1]; out[4] = M[0][2]; out[5] = M[1][2];
+}
+
+// rotate_stress! (stress_rotation.jl:28-61), 3-D
+void orc_rotate_stress3(const orc_stokes3* s, double dt, const double* const* tau, double* const* tau_old) {
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        for (int q = 0; q < 27; ++q) {
+            double G[3][3] = {};   // G[c][k] = ∂v_c/∂x_k
+            for (int a = 0; a < 27; ++a) {
+                const int32_t g = s->el2n[e * 27 + a];
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < 3; ++k) G[c][k] += dNdx[q][a][k] * s->v[c][g];
+            }
+            const double w[3] = {G[2][1] - G[1][2], G[0][2] - G[2][0], G[1][0] - G[0][1]};
+            double t[6], o[6];
+            for (int c = 0; c < 6; ++c) t[c] = tau[c][e * 27 + q];
+            orc_rotate_elastic_stress3(w, t, dt, o);
+            for (int c = 0; c < 6; ++c) tau_old[c][e * 27 + q] = o[c];
+        }
+    }
+}
 
 // augmented Jacobian diagnostics in 3-D: as orc_augmented_jacobian, one
 // forward-mode pass per velocity direction with 27 partials

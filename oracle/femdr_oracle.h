@@ -273,6 +273,15 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
  * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
 int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
 
+/* --- rotate_stress! (src/stokes/stress_rotation.jl:28-61) and the GeoParams
+ *     0.7.21 rotate_elastic_stress it calls (restated, not vendored) --- */
+void orc_rotate_elastic_stress2(double omega, const double* tau /*xx,yy,xy*/, double dt, double* out);
+void orc_rotate_elastic_stress3(const double* omega /*curl v*/, const double* tau /*xx,yy,zz,xy,xz,yz*/, double dt,
+                                double* out);
+/* tau, tau_old: [3][nels*7] (2-D) and [6][nels*27] (3-D, assembler order) */
+void orc_rotate_stress(const orc_stokes* s, double dt, const double* const* tau, double* const* tau_old);
+void orc_rotate_stress3(const orc_stokes3* s, double dt, const double* const* tau, double* const* tau_old);
+
 /* --- coupled thermal--Stokes step (src/stokes/solvers/DR.jl:44-131, 514-661) --- */
 
 /* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-92):

@@ -136,6 +136,10 @@ def _declare(L):
     L.orc_thermal_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(ThermalParamsC), C.POINTER(ThermalStatsC)]
     L.orc_thermal_solve.restype = C.c_int
     L.orc_shear_heating.argtypes = [C.POINTER(Stokes), D, PD]
+    L.orc_rotate_elastic_stress2.argtypes = [D, PD, D, PD]
+    L.orc_rotate_elastic_stress3.argtypes = [PD, PD, D, PD]
+    L.orc_rotate_stress.argtypes = [C.POINTER(Stokes), D, C.POINTER(PD), C.POINTER(PD)]
+    L.orc_rotate_stress3.argtypes = [C.POINTER(Stokes3), D, C.POINTER(PD), C.POINTER(PD)]
     L.orc_coupled_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(Stokes), C.POINTER(Params),
                                     C.POINTER(CoupledParamsC), C.POINTER(Stats), C.POINTER(CoupledStatsC)]
     L.orc_coupled_solve.restype = C.c_int
@@ -356,6 +360,14 @@ class Problem:
         lib().orc_update_current_stress(C.byref(self.struct()), dt)
         return self.tau_store
 
+    def rotate_stress(self, dt, tau):
+        """rotate_stress! (stress_rotation.jl:28-61): the rotated tau (3 arrays [nels][7])."""
+        tau = [f64(t).reshape(self.nels, 7) for t in tau]
+        out = [np.zeros((self.nels, 7)) for _ in range(3)]
+        lib().orc_rotate_stress(C.byref(self.struct()), dt, (PD * 3)(*[ptr(t) for t in tau]),
+                                (PD * 3)(*[ptr(o) for o in out]))
+        return out
+
     def shear_heating(self, dt):
         """assemble_shear_heating! (shear_heating.jl:18-92): Φ [nels][7]."""
         phi = np.zeros((self.nels, 7))
@@ -462,6 +474,18 @@ class ThermalProblem:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual}
+
+
+def rotate_elastic_stress(omega, tau, dt):
+    """GeoParams.rotate_elastic_stress as restated: 2-D (scalar ω, τ = xx, yy,
+    xy) or 3-D (ω = curl v, τ in assembler order xx, yy, zz, xy, xz, yz)."""
+    tau = f64(tau)
+    out = np.zeros(len(tau))
+    if np.ndim(omega) == 0:
+        lib().orc_rotate_elastic_stress2(float(omega), ptr(tau), dt, ptr(out))
+    else:
+        lib().orc_rotate_elastic_stress3(ptr(f64(omega)), ptr(tau), dt, ptr(out))
+    return out
 
 
 def coupled_solve(thermal: ThermalProblem, stokes: Problem, dt, Tref=273.0, tolerance_T=1e-6, shear_heating=False,
@@ -586,6 +610,14 @@ class Problem3:
     def update_current_stress(self, dt):
         lib().orc_update_current_stress3(C.byref(self.struct()), dt)
         return self.tau_store
+
+    def rotate_stress(self, dt, tau):
+        """rotate_stress! (stress_rotation.jl:28-61), 3-D: 6 arrays [nels][27] in assembler order."""
+        tau = [f64(t).reshape(self.nels, 27) for t in tau]
+        out = [np.zeros((self.nels, 27)) for _ in range(6)]
+        lib().orc_rotate_stress3(C.byref(self.struct()), dt, (PD * 6)(*[ptr(t) for t in tau]),
+                                 (PD * 6)(*[ptr(o) for o in out]))
+        return out
 
     def solve(self, dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
               check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
