@@ -29,7 +29,7 @@ __all__ = [
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
-    "solve_coupled", "assemble_shear_heating", "rotate_stress",
+    "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -267,7 +267,15 @@ class DeviceArray:
         return a.astype(dtype) if dtype is not None else a
 
     def copyto(self, src):
-        """copyto!(dest, src)"""
+        """copyto!(dest, src); a DeviceArray source is copied on the device."""
+        if isinstance(src, DeviceArray):
+            if src.size != self.size or src.dtype != self.dtype:
+                raise DimensionMismatch(f"copyto! from {src} into {self}")
+            if src.ctx is not self.ctx:
+                raise ValueError("copyto! between device arrays of different contexts")
+            if self.size:
+                self.ctx.check(_lib.fem_copy_d2d(self.ctx.handle, self.ptr, src.ptr, self.nbytes))
+            return
         a = np.ascontiguousarray(np.asarray(src, dtype=self.dtype).reshape(self.shape))
         if self.size:
             self.ctx.check(_lib.fem_copy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, self.nbytes))
@@ -297,6 +305,15 @@ class VectorField2D:
     def __getitem__(self, i):
         return tuple(self)[i]
 
+    def copyto(self, src):
+        """copyto!(dst, src), component by component on the device
+        (symmetric_tensor.jl:133-150 advances the stress history this way)."""
+        mine, theirs = tuple(self), tuple(src)
+        if len(mine) != len(theirs):
+            raise DimensionMismatch("copyto! between fields of different dimension")
+        for d, c in zip(mine, theirs):
+            d.copyto(c)
+
 
 @dataclass
 class VectorField3D:
@@ -310,6 +327,15 @@ class VectorField3D:
 
     def __getitem__(self, i):
         return tuple(self)[i]
+
+    def copyto(self, src):
+        """copyto!(dst, src), component by component on the device
+        (symmetric_tensor.jl:133-150 advances the stress history this way)."""
+        mine, theirs = tuple(self), tuple(src)
+        if len(mine) != len(theirs):
+            raise DimensionMismatch("copyto! between fields of different dimension")
+        for d, c in zip(mine, theirs):
+            d.copyto(c)
 
 
 @dataclass
@@ -329,6 +355,15 @@ class SymmetricTensor3D:
     def __getitem__(self, i):
         return tuple(self)[i]
 
+    def copyto(self, src):
+        """copyto!(dst, src), component by component on the device
+        (symmetric_tensor.jl:133-150 advances the stress history this way)."""
+        mine, theirs = tuple(self), tuple(src)
+        if len(mine) != len(theirs):
+            raise DimensionMismatch("copyto! between fields of different dimension")
+        for d, c in zip(mine, theirs):
+            d.copyto(c)
+
 
 @dataclass
 class SymmetricTensor2D:
@@ -343,6 +378,15 @@ class SymmetricTensor2D:
 
     def __getitem__(self, i):
         return tuple(self)[i]
+
+    def copyto(self, src):
+        """copyto!(dst, src), component by component on the device
+        (symmetric_tensor.jl:133-150 advances the stress history this way)."""
+        mine, theirs = tuple(self), tuple(src)
+        if len(mine) != len(theirs):
+            raise DimensionMismatch("copyto! between fields of different dimension")
+        for d, c in zip(mine, theirs):
+            d.copyto(c)
 
 
 def _tup(v, n=None):
@@ -728,6 +772,14 @@ def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False):
     """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal)."""
     d, dp = _dp_ptr(plastic)
     dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
+
+
+def copyto(dst, src):
+    """copyto!(dst, src) for device arrays and the field containers (dr.v,
+    dr.tau, ...): e.g. copyto(dr.tau_old, dr.tau) and copyto(dr.P0, dr.P) at a
+    time-step boundary, on the device."""
+    dst.copyto(src)
+    return dst
 
 
 def rotate_stress(dr: StokesDR, mesh: MixedMesh, dt):

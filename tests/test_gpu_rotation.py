@@ -123,3 +123,46 @@ def test_gpu_rotate_stress_3d_parity_and_rigid_rotation():
     fa.rotate_stress(dr, mesh, 0.25)
     for name, t in zip(TAU3, t0):
         assert np.allclose(getattr(dr.tau_old, name).numpy(), t, rtol=1e-14)
+
+
+def test_gpu_time_step_handoff_on_device():
+    # the step boundary of a visco-elastic series: τ at the points from the
+    # converged state (update_stokes_current_stress!, DR.jl:820-897), then
+    # copyto!(τ_old, τ) (symmetric_tensor.jl:133-150) and P0 <- P, all on the
+    # device; the next step's oracle solve from the same history agrees
+    say("[gpu] time-step handoff on the device")
+    from test_gpu_parity import make_pair
+    n = 8
+    coords, el2n = O.t7_mesh(n, n)
+    ne = len(el2n)
+    (bx, vx), (by, vy) = O.pure_shear_bc(coords)
+    zero = tuple(np.zeros((ne, 7)) for _ in range(3))
+    cen = coords[el2n[:, :3]].mean(axis=1)
+    ph = np.where(((cen - 0.5) ** 2).sum(axis=1) < 0.2 ** 2, 2, 1).astype(np.int32)
+    pr, dr, mesh, _, _ = make_pair(n, n, (1.0, 5.0), G=(1.0, 2.0), etab=(10.0, 10.0), phases=ph, tau_old=zero,
+                                   bc=((bx, vx), (by, vy)), random_state=False)
+    # a start that meets the boundary values (pure shear)
+    pr.vx[:], pr.vy[:] = coords[:, 0] - 0.5, -(coords[:, 1] - 0.5)
+    dr.v.x.copyto(pr.vx)
+    dr.v.y.copyto(pr.vy)
+    bcs = (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy))
+    dt = 0.3
+    kw = dict(dt=dt, tolerance=1e-10, check_interval=20)
+    for step in range(2):
+        st = fa.solve(dr, mesh, *bcs, **kw)
+        so = pr.solve(dt, tolerance=1e-10, check_interval=20)
+        say(f"  step {step}: gpu iter {st['iter']} oracle iter {so['iter']}")
+        assert st["converged"] and st["iter"] == so["iter"] > 0
+        assert rel(dr.v.x.numpy(), pr.vx) < 1e-10
+        fa.update_stokes_current_stress(dr, mesh, dt)
+        tau = [t.copy() for t in pr.update_current_stress(dt)]
+        fa.copyto(dr.tau_old, dr.tau)
+        fa.copyto(dr.P0, dr.P)
+        for name, t in zip(TAU2, tau):
+            assert np.array_equal(getattr(dr.tau_old, name).numpy(), getattr(dr.tau, name).numpy())
+            assert rel(getattr(dr.tau_old, name).numpy(), t) < 1e-10
+        assert np.array_equal(dr.P0.numpy(), dr.P.numpy())
+        pr.tau_old = tuple(tau)
+        pr.arrays["P0"][:] = pr.P
+    with pytest.raises(fa.DimensionMismatch):
+        fa.copyto(dr.P0, dr.v.x)
