@@ -143,6 +143,8 @@ struct Rheo {
     const double* eta;   // per-phase viscosity actually used (mat->eta or element max)
     const orc_dp* dp;
     double dt;
+    const orc_cap* cap = nullptr;     // tensile cap in place of dp
+    const double* gamma = nullptr;    // the element's accepted plastic strain [nq] (softening)
 };
 
 template <class S>
@@ -193,6 +195,236 @@ double stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, const
     return lambda;
 }
 
+// ---------------------------------------------------------------------------
+// Drucker-Prager tensile cap, src/stokes/assemblers/rheology.jl:510-787
+// (Popov, Berlie & Kaus 2025, Eqs. 13-18, 42). Pressure is compression-
+// positive; pT <= 0.
+// ---------------------------------------------------------------------------
+struct CapGeom { double a, b, p_y, R_y, p_d, tau_d, p_q; };
+
+// cap_geometry (rheology.jl:546-557)
+CapGeom cap_geometry(double k, double kq, double c, double pT) {
+    CapGeom g;
+    g.a = std::sqrt(1 + k * k);
+    g.b = std::sqrt(1 + kq * kq);
+    g.p_y = (g.a * pT + c) / (g.a - k);
+    g.R_y = g.p_y - pT;
+    g.p_d = g.p_y - g.R_y * k / g.a;
+    g.tau_d = g.R_y / g.a;
+    g.p_q = g.p_d + kq * g.tau_d;
+    return g;
+}
+
+template <class S> S sq(const S& x) { return x * x; }
+
+// cap_yield_function (rheology.jl:582-589): shear line or a·(R̂ − R_y)
+template <class S>
+S cap_yield(const S& s, const S& p, double k, double c, const CapGeom& g) {
+    if (val(p) + k * val(s) >= g.p_y) return s - k * p - c;
+    const double e = std::numeric_limits<double>::epsilon();
+    return g.a * (dsqrt(s * s + sq(p - g.p_y) + e * e) - g.R_y);
+}
+
+// cap_invariants (rheology.jl:600-609): F, Aτ = (∂Q/∂s)/2, Ap = −∂Q/∂p
+template <class S>
+void cap_invariants(const S& s, const S& p, double k, double kq, double c, double pT, S& F, S& At, S& Ap) {
+    const CapGeom g = cap_geometry(k, kq, c, pT);
+    F = cap_yield(s, p, k, c, g);
+    if (val(p) + kq * val(s) >= g.p_q) { At = S(0.5); Ap = S(kq); return; }
+    const S Rq = dsqrt(s * s + sq(p - g.p_q));
+    if (val(Rq) == 0) { At = S(0.0); Ap = S(0.0); return; }
+    At = g.b * s / (2.0 * Rq);
+    Ap = -g.b * (p - g.p_q) / Rq;
+}
+
+// the local system of cap_return_map and its Jacobian in x = (s, p, λ)
+// (rheology.jl:611-615); the partials of F, Aτ, Ap by branch
+struct CapSys {
+    double s_tr, p_tr, eve, KDt, k, kq, c, pT, ereg;
+    void residual(const double x[3], double r[3]) const {
+        double F, At, Ap;
+        cap_invariants<double>(x[0], x[1], k, kq, c, pT, F, At, Ap);
+        r[0] = x[0] - s_tr + 2 * eve * x[2] * At;
+        r[1] = x[1] - p_tr - KDt * x[2] * Ap;
+        r[2] = F - ereg * x[2];
+    }
+    void jacobian(const double x[3], double J[3][3]) const {
+        typedef Dual<2> D2;
+        D2 s(x[0]), p(x[1]);
+        s.d[0] = 1; p.d[1] = 1;
+        D2 F, At, Ap;
+        cap_invariants<D2>(s, p, k, kq, c, pT, F, At, Ap);
+        const double lam = x[2];
+        J[0][0] = 1 + 2 * eve * lam * At.d[0]; J[0][1] = 2 * eve * lam * At.d[1]; J[0][2] = 2 * eve * At.v;
+        J[1][0] = -KDt * lam * Ap.d[0]; J[1][1] = 1 - KDt * lam * Ap.d[1]; J[1][2] = -KDt * Ap.v;
+        J[2][0] = F.d[0]; J[2][1] = F.d[1]; J[2][2] = -ereg;
+    }
+};
+
+// 3x3 solve by Gaussian elimination with partial pivoting; false if singular
+bool solve3(double A[3][3], double b[3], double x[3]) {
+    int piv[3] = {0, 1, 2};
+    for (int col = 0; col < 3; ++col) {
+        int best = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (std::fabs(A[piv[r]][col]) > std::fabs(A[piv[best]][col])) best = r;
+        std::swap(piv[col], piv[best]);
+        const double d = A[piv[col]][col];
+        if (d == 0 || !std::isfinite(d)) return false;
+        for (int r = col + 1; r < 3; ++r) {
+            const double f = A[piv[r]][col] / d;
+            for (int cc = col; cc < 3; ++cc) A[piv[r]][cc] -= f * A[piv[col]][cc];
+            b[piv[r]] -= f * b[piv[col]];
+        }
+    }
+    for (int i = 2; i >= 0; --i) {
+        double acc = b[piv[i]];
+        for (int cc = i + 1; cc < 3; ++cc) acc -= A[piv[i]][cc] * x[cc];
+        x[i] = acc / A[piv[i]][i];
+    }
+    return true;
+}
+
+struct CapRet {
+    double s, p, lam, At, Ap;
+    bool converged;
+    double dx[3];   // ∂(s, p, λ)/∂s_trial at the solution (implicit function)
+};
+
+// cap_return_map (rheology.jl:652-690): Newton on the residual scaled by
+// max(|s_trial|, |p_trial|, |F_trial|, eps), halving backtracking (24 tries)
+// subject to s >= 0, λ >= 0, a finite residual and the Armijo decrease
+// ‖r_new‖² <= (1 − α/10⁴)‖r‖², convergence at ‖r‖∞ <= 100 eps, then one full
+// step; dx is the derivative that full step carries through dual numbers.
+CapRet cap_return(double s_tr, double p_tr, double eve, double KDt, double k, double kq, double c, double pT,
+                  double ereg, int maxiter = 40) {
+    CapSys sys{s_tr, p_tr, eve, KDt, k, kq, c, pT, ereg};
+    CapRet out;
+    double F0, At0, Ap0;
+    cap_invariants<double>(s_tr, p_tr, k, kq, c, pT, F0, At0, Ap0);
+    out.s = s_tr; out.p = p_tr; out.lam = 0; out.At = At0; out.Ap = Ap0;
+    out.dx[0] = 1; out.dx[1] = 0; out.dx[2] = 0;
+    if (F0 <= 0) { out.converged = true; return out; }
+    out.converged = false;
+    if (!(std::isfinite(KDt) && KDt > 0 && std::isfinite(eve) && eve > 0)) return out;
+    const double e = std::numeric_limits<double>::epsilon();
+    const double scale = std::max(std::max(std::fabs(s_tr), std::fabs(p_tr)), std::max(std::fabs(F0), e));
+    const double tol = 100 * e;
+    double x[3] = {s_tr, p_tr, 0.0}, r[3];
+    auto resid = [&](const double* y, double* rr) {
+        sys.residual(y, rr);
+        for (int i = 0; i < 3; ++i) rr[i] /= scale;
+    };
+    auto inf_norm = [](const double* v) { return std::max(std::fabs(v[0]), std::max(std::fabs(v[1]), std::fabs(v[2]))); };
+    resid(x, r);
+    bool conv = inf_norm(r) <= tol;
+    for (int it = 0; it < maxiter && !conv; ++it) {
+        double J[3][3], rhs[3] = {r[0], r[1], r[2]}, step[3];
+        sys.jacobian(x, J);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] /= scale;
+        if (!solve3(J, rhs, step)) return out;
+        if (!(std::isfinite(step[0]) && std::isfinite(step[1]) && std::isfinite(step[2]))) return out;
+        double alpha = 1.0;
+        bool accepted = false;
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        for (int ls = 0; ls < 24; ++ls) {
+            const double cand[3] = {x[0] - alpha * step[0], x[1] - alpha * step[1], x[2] - alpha * step[2]};
+            if (cand[0] >= 0 && cand[2] >= 0) {
+                double rn[3];
+                resid(cand, rn);
+                if (std::isfinite(rn[0]) && std::isfinite(rn[1]) && std::isfinite(rn[2]) &&
+                    rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2] <= (1 - alpha / 10000) * r2) {
+                    for (int i = 0; i < 3; ++i) { x[i] = cand[i]; r[i] = rn[i]; }
+                    accepted = true;
+                    break;
+                }
+            }
+            alpha /= 2;
+        }
+        if (!accepted) return out;
+        conv = inf_norm(r) <= tol;
+    }
+    if (!conv) return out;
+    // the final full step (value unchanged to round-off); its derivative with
+    // respect to s_trial is J⁻¹ e₁ (∂r/∂s_trial = −e₁)
+    double J[3][3], Jc[3][3], rhs[3] = {r[0], r[1], r[2]}, step[3];
+    sys.jacobian(x, J);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { J[i][j] /= scale; Jc[i][j] = J[i][j] * scale; }
+    if (!solve3(J, rhs, step)) return out;
+    for (int i = 0; i < 3; ++i) x[i] -= step[i];
+    double e1[3] = {1, 0, 0};
+    if (!solve3(Jc, e1, out.dx)) return out;
+    double F, At, Ap;
+    cap_invariants<double>(x[0], x[1], k, kq, c, pT, F, At, Ap);
+    out.s = x[0]; out.p = x[1]; out.lam = x[2]; out.At = At; out.Ap = Ap;
+    out.converged = true;
+    return out;
+}
+
+// second_invariant with the eps floor (rheology.jl:206-212)
+template <class S>
+S second_invariant2(const S t[3]) {
+    const double e = std::numeric_limits<double>::epsilon();
+    const S tzz = -t[0] - t[1];
+    return dsqrt((t[0] * t[0] + t[1] * t[1] + tzz * tzz) / 2.0 + t[2] * t[2] + e * e);
+}
+
+// cap_local_update (rheology.jl:708-717): radial deviatoric return and the
+// corrected pressure; S carries the implicit derivative of the return with
+// respect to the trial stress (P_trial is a constant of the velocity)
+template <class S>
+void cap_local_update(const S tt[3], double P_tr, double eve, double KDt, double k, double kq, double c, double pT,
+                      double ereg, S tau[3], S& P, S* gdot = nullptr, S* thdot = nullptr) {
+    const S s_tr = second_invariant2(tt);
+    const CapRet ret = cap_return(val(s_tr), P_tr, eve, KDt, k, kq, c, pT, ereg);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    if (!ret.converged) {
+        for (int i = 0; i < 3; ++i) tau[i] = S(nan);
+        P = S(nan);
+        if (gdot) *gdot = S(nan);
+        if (thdot) *thdot = S(nan);
+        return;
+    }
+    const S ds = s_tr - val(s_tr);   // the derivative part of s_trial
+    const S s = ret.s + ret.dx[0] * ds, p = ret.p + ret.dx[1] * ds, lam = ret.lam + ret.dx[2] * ds;
+    S F, At, Ap;
+    if (ret.lam == 0) { At = S(ret.At); Ap = S(ret.Ap); }
+    else cap_invariants<S>(s, p, k, kq, c, pT, F, At, Ap);
+    const S g = lam * At, th = lam * Ap;
+    for (int i = 0; i < 3; ++i) tau[i] = tt[i] - 2.0 * eve * g * tt[i] / s_tr;
+    P = P_tr + KDt * th;
+    if (gdot) *gdot = g;
+    if (thdot) *thdot = th;
+}
+
+// _cap_ip_update (rheology.jl:722-741): VE trial stress, cap parameters and
+// the softened cohesion interpolated to the point, the local update
+template <class S>
+void cap_stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, const int* ph, const Rheo& r,
+                   const double tau_o[3], double Peq, const double* gamma, S tau[3], S& P_loc, S* gdot = nullptr,
+                   S* thdot = nullptr) {
+    Rheo rv = r;
+    rv.dp = nullptr;
+    S tt[3];
+    stress_ip<S>(gvx, gvy, Nv, NV, ph, rv, tau_o, Peq, tt);
+    double invG[ORC_MAX_PHASES];
+    for (int p = 0; p < r.mat->nphases; ++p) invG[p] = 1.0 / r.mat->G[p];
+    const double eve = 1.0 / (1.0 / interp_phase(Nv, NV, r.eta, ph) + interp_phase(Nv, NV, invG, ph) / r.dt);
+    const orc_cap* cp = r.cap;
+    double C = interp_phase(Nv, NV, cp->C, ph);
+    if (gamma) {
+        const double Cmin = interp_phase(Nv, NV, cp->C_min, ph), HC = interp_phase(Nv, NV, cp->H_C, ph);
+        const double x = C + HC * *gamma;
+        C = x > C ? C : (x < Cmin ? Cmin : x);   // clamp(C + H_C γ, C_min, C)
+    }
+    cap_local_update<S>(tt, Peq, eve, interp_phase(Nv, NV, cp->Kb, ph) * r.dt, interp_phase(Nv, NV, cp->sin_phi, ph),
+                        interp_phase(Nv, NV, cp->sin_psi, ph), C * interp_phase(Nv, NV, cp->cos_phi, ph),
+                        interp_phase(Nv, NV, cp->pT, ph), interp_phase(Nv, NV, cp->eta_reg, ph), tau, P_loc, gdot,
+                        thdot);
+}
+
 // Element momentum residual, src/stokes/assemblers/momentum_residuals.jl:136-189
 // (2-D with Pnum, T and gravity); accumulate_momentum :28-33; eos_density :11-16.
 // Generic in NV/NP/nq; dNdx [nq][NV][2], Nq [nq][NV], NqP [nq][NP].
@@ -227,11 +459,13 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
         if (Pf) for (int i = 0; i < NP; ++i) Pfq += Np[i] * Pf[i];
         const double Peq = Pq - Pfq;
         S tau[3];
-        stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, tau);
-        const double Pq_local = Peq + Pfq;
+        S Pe = S(Peq);   // the cap corrects the pressure the momentum balance uses
+        if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, r.gamma ? r.gamma + q : nullptr, tau, Pe);
+        else stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, tau);
+        const S Pq_local = Pe + Pfq;
         if (tau_out) { tau_out[0 * nq + q] = val(tau[0]); tau_out[1 * nq + q] = val(tau[1]); tau_out[2 * nq + q] = val(tau[2]); }
-        if (Pq_out) Pq_out[q] = Pq_local;
-        S Ptot = S(Pq_local);
+        if (Pq_out) Pq_out[q] = val(Pq_local);
+        S Ptot = Pq_local;
         if (Pnum) { S pn = S(0.0); for (int i = 0; i < NP; ++i) pn = pn + Np[i] * Pnum[i]; Ptot = Ptot + pn; }
         double Tq = 0;
         for (int i = 0; i < NP; ++i) Tq += Np[i] * T[i];
@@ -620,9 +854,12 @@ void orc_pressure_residual(orc_stokes* s, double dt, int32_t finite_K, const dou
 static void momentum_residual_impl(orc_stokes* s, double dt, const double* Pnum, double* tau_store[3]) {
     const Tables& t = tables();
     std::vector<double> Rex((size_t)s->nels * 7), Rey((size_t)s->nels * 7);
-    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+    Rheo r0{&s->mat, s->mat.eta, s->cap ? nullptr : s->dp, dt};
+    r0.cap = s->cap;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < s->nels; ++e) {
+        Rheo r = r0;
+        if (s->cap && s->gamma_hist) r.gamma = s->gamma_hist + e * 7;
         double dNdx[7][7][2];
         elem_dNdx(s, e, dNdx);
         double vx[7], vy[7];
@@ -637,7 +874,8 @@ static void momentum_residual_impl(orc_stokes* s, double dt, const double* Pnum,
         momentum_element<double>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], vx, vy,
                                  s->P + e * 3, Pnum ? Pnum + e * 3 : nullptr, s->T + e * 3,
                                  s->Pf ? s->Pf + e * 3 : nullptr, ph, r, has_old ? to : nullptr,
-                                 &Rex[e * 7], &Rey[e * 7], tau_store ? tout : nullptr, nullptr);
+                                 &Rex[e * 7], &Rey[e * 7], tau_store ? tout : nullptr,
+                                 tau_store && s->P_store ? s->P_store + e * 7 : nullptr);
         if (tau_store)
             for (int c = 0; c < 3; ++c)
                 if (tau_store[c])
@@ -680,7 +918,9 @@ void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K) {
         for (int a = 1; a < 7; ++a) emax = std::max(emax, s->mat.eta[ph[a]]);
         double eta_pc[ORC_MAX_PHASES];
         for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = emax;
-        Rheo r{&s->mat, eta_pc, s->dp, dt};
+        Rheo r{&s->mat, eta_pc, s->cap ? nullptr : s->dp, dt};
+        r.cap = s->cap;
+        if (s->cap && s->gamma_hist) r.gamma = s->gamma_hist + e * 7;
         double to[21];
         bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
         for (int c = 0; c < 3; ++c)
@@ -1076,10 +1316,85 @@ int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_s
 // (shear_dissipation, shear_heating.jl:77-88).
 void orc_shear_heating(const orc_stokes* s, double dt, double* phi) {
     const Tables& tb = tables();
-    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+    Rheo r0{&s->mat, s->mat.eta, s->cap ? nullptr : s->dp, dt};
+    r0.cap = s->cap;
     double invG[ORC_MAX_PHASES];
     for (int p = 0; p < s->mat.nphases; ++p) invG[p] = 1.0 / s->mat.G[p];
 #pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        Rheo r = r0;
+        double dNdx[7][7][2];
+        elem_dNdx(s, e, dNdx);
+        int ph[7];
+        elem_phases_v(s, e, ph);
+        for (int q = 0; q < 7; ++q) {
+            double gvx[2] = {0, 0}, gvy[2] = {0, 0};
+            for (int a = 0; a < 7; ++a) {
+                const int32_t g = s->el2n[e * 7 + a];
+                gvx[0] += dNdx[q][a][0] * s->vx[g];
+                gvx[1] += dNdx[q][a][1] * s->vx[g];
+                gvy[0] += dNdx[q][a][0] * s->vy[g];
+                gvy[1] += dNdx[q][a][1] * s->vy[g];
+            }
+            double to[3] = {0, 0, 0};
+            for (int c = 0; c < 3; ++c)
+                if (s->tau_old[c]) to[c] = s->tau_old[c][e * 7 + q];
+            double Pq = 0, Pfq = 0;
+            for (int i = 0; i < 3; ++i) {
+                Pq += tb.NqP[q][i] * s->P[e * 3 + i];
+                if (s->Pf) Pfq += tb.NqP[q][i] * s->Pf[e * 3 + i];
+            }
+            double tau[3], Pe;
+            if (r.cap)
+                cap_stress_ip<double>(gvx, gvy, tb.Nq[q], 7, ph, r, to, Pq - Pfq,
+                                      s->gamma_hist ? s->gamma_hist + e * 7 + q : nullptr, tau, Pe);
+            else
+                stress_ip<double>(gvx, gvy, tb.Nq[q], 7, ph, r, to, Pq - Pfq, tau);
+            const double inv2Gdt = interp_phase(tb.Nq[q], 7, invG, ph) / (2 * dt);
+            const double tzz = -(tau[0] + tau[1]), tzzo = -(to[0] + to[1]);
+            const double power = tau[0] * gvx[0] + tau[1] * gvy[1] + tau[2] * (gvx[1] + gvy[0]);
+            const double elastic = (tau[0] * (tau[0] - to[0]) + tau[1] * (tau[1] - to[1]) + tzz * (tzz - tzzo) +
+                                    2 * tau[2] * (tau[2] - to[2])) * inv2Gdt;
+            phi[e * 7 + q] = power - elastic;
+        }
+    }
+}
+
+void orc_cap_geometry(double k, double kq, double c, double pT, double* out) {
+    const CapGeom g = cap_geometry(k, kq, c, pT);
+    out[0] = g.a; out[1] = g.b; out[2] = g.p_y; out[3] = g.R_y; out[4] = g.p_d; out[5] = g.tau_d; out[6] = g.p_q;
+}
+
+double orc_cap_yield_function(double tII, double P, double k, double kq, double c, double pT) {
+    return cap_yield<double>(tII, P, k, c, cap_geometry(k, kq, c, pT));
+}
+
+void orc_cap_invariants(double s, double p, double k, double kq, double c, double pT, double* out) {
+    cap_invariants<double>(s, p, k, kq, c, pT, out[0], out[1], out[2]);
+}
+
+int orc_cap_return_map(double s_trial, double p_trial, double eta_ve, double KDt, double k, double kq, double c,
+                       double pT, double eta_reg, int32_t maxiter, double* out) {
+    const CapRet r = cap_return(s_trial, p_trial, eta_ve, KDt, k, kq, c, pT, eta_reg, maxiter);
+    out[0] = r.s; out[1] = r.p; out[2] = r.lam; out[3] = r.At; out[4] = r.Ap;
+    return r.converged ? 1 : 0;
+}
+
+void orc_cap_local_update(const double* tau_trial, double P_trial, double eta_ve, double KDt, double k, double kq,
+                          double c, double pT, double eta_reg, double* tau_out, double* out) {
+    double P, g, th;
+    cap_local_update<double>(tau_trial, P_trial, eta_ve, KDt, k, kq, c, pT, eta_reg, tau_out, P, &g, &th);
+    out[0] = P; out[1] = g; out[2] = th;
+}
+
+// update_stokes_plastic_history_kernel! (momentum_residuals.jl:584-610) with
+// the finiteness check of DR.jl:950-952
+int orc_commit_plastic_history(orc_stokes* s, double dt) {
+    if (!s->cap || !s->gamma_hist || !s->theta_hist) return 0;
+    const Tables& tb = tables();
+    Rheo r0{&s->mat, s->mat.eta, nullptr, dt};
+    r0.cap = s->cap;
+    bool bad = false;
     for (int64_t e = 0; e < s->nels; ++e) {
         double dNdx[7][7][2];
         elem_dNdx(s, e, dNdx);
@@ -1102,16 +1417,19 @@ void orc_shear_heating(const orc_stokes* s, double dt, double* phi) {
                 Pq += tb.NqP[q][i] * s->P[e * 3 + i];
                 if (s->Pf) Pfq += tb.NqP[q][i] * s->Pf[e * 3 + i];
             }
-            double tau[3];
-            stress_ip<double>(gvx, gvy, tb.Nq[q], 7, ph, r, to, Pq - Pfq, tau);
-            const double inv2Gdt = interp_phase(tb.Nq[q], 7, invG, ph) / (2 * dt);
-            const double tzz = -(tau[0] + tau[1]), tzzo = -(to[0] + to[1]);
-            const double power = tau[0] * gvx[0] + tau[1] * gvy[1] + tau[2] * (gvx[1] + gvy[0]);
-            const double elastic = (tau[0] * (tau[0] - to[0]) + tau[1] * (tau[1] - to[1]) + tzz * (tzz - tzzo) +
-                                    2 * tau[2] * (tau[2] - to[2])) * inv2Gdt;
-            phi[e * 7 + q] = power - elastic;
+            double tau[3], Pe, gd, thd;
+            const double gam = s->gamma_hist[e * 7 + q];
+            cap_stress_ip<double>(gvx, gvy, tb.Nq[q], 7, ph, r0, to, Pq - Pfq, &gam, tau, Pe, &gd, &thd);
+            s->gamma_hist[e * 7 + q] += dt * gd;
+            s->theta_hist[e * 7 + q] += dt * thd;
+            bad |= !std::isfinite(s->gamma_hist[e * 7 + q]) || !std::isfinite(s->theta_hist[e * 7 + q]);
         }
     }
+    if (bad) {
+        g_err = "non-finite plastic history: a tensile-cap local return map failed";
+        return -1;
+    }
+    return 0;
 }
 
 // GeoParams.rotate_elastic_stress, 2-D (GeoParams 0.7.21, the version

@@ -50,6 +50,22 @@ typedef struct {
     double Kb[ORC_MAX_PHASES];
 } orc_dp;
 
+/* DruckerPragerCap (src/stokes/types/stokes_types.jl:449-544): the
+ * Drucker-Prager shear line closed on the tensile side by the circular cap of
+ * Popov, Berlie & Kaus (2025); pT <= 0 (compression-positive pressure). */
+typedef struct {
+    int32_t nphases;
+    double cos_phi[ORC_MAX_PHASES];
+    double sin_phi[ORC_MAX_PHASES];
+    double sin_psi[ORC_MAX_PHASES];
+    double C[ORC_MAX_PHASES];
+    double pT[ORC_MAX_PHASES];
+    double eta_reg[ORC_MAX_PHASES];
+    double Kb[ORC_MAX_PHASES];
+    double C_min[ORC_MAX_PHASES];
+    double H_C[ORC_MAX_PHASES];
+} orc_cap;
+
 /* Phase layouts (src/utils.jl:70-74): */
 #define ORC_PHASE_CELL 0      /* one phase per element (1 x nels matrix)        */
 #define ORC_PHASE_ELEMENT 1   /* one phase per element-local node (NV x nels)  */
@@ -79,6 +95,12 @@ typedef struct {
     int64_t nbc[2];
     const int32_t* bc_dofs[2];
     const double* bc_vals[2];
+    /* tensile-cap yield model (nullable; replaces dp when set) and its
+     * integration-point history (rheology.jl:689-760, DR.jl:899-958) */
+    const orc_cap* cap;
+    double* gamma_hist;       /* nullable [nels][7]: accepted plastic strain γ (softens C) */
+    double* theta_hist;       /* nullable [nels][7]: accepted volumetric plastic strain θ */
+    double* P_store;          /* nullable [nels][7]: pressure of the stress update (cap-corrected) */
 } orc_stokes;
 
 /* solve!/_solve_stokes_dyrel! keyword parameters (src/stokes/solvers/DR.jl:302-386) */
@@ -272,6 +294,22 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
 /* solve!(::ThermalDiffusionDR, ...) -> solve_dynamic_relaxation!
  * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
 int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
+
+/* --- Drucker-Prager tensile cap (src/stokes/assemblers/rheology.jl:510-787) --- */
+/* cap_geometry: out = (a, b, p_y, R_y, p_d, τ_d, p_q) (rheology.jl:546-557) */
+void orc_cap_geometry(double k, double kq, double c, double pT, double* out);
+double orc_cap_yield_function(double tII, double P, double k, double kq, double c, double pT);
+/* cap_invariants: out = (F, Aτ, Ap) (rheology.jl:600-609) */
+void orc_cap_invariants(double s, double p, double k, double kq, double c, double pT, double* out);
+/* cap_return_map: out = (τII, P, λ, Aτ, Ap); returns converged (rheology.jl:652-690) */
+int orc_cap_return_map(double s_trial, double p_trial, double eta_ve, double KDt, double k, double kq, double c,
+                       double pT, double eta_reg, int32_t maxiter, double* out);
+/* cap_local_update: tau_out (3), out = (P, γdot, θdot) (rheology.jl:708-717) */
+void orc_cap_local_update(const double* tau_trial, double P_trial, double eta_ve, double KDt, double k, double kq,
+                          double c, double pT, double eta_reg, double* tau_out, double* out);
+/* commit_stokes_plastic_history! (DR.jl:899-958, momentum_residuals.jl:570-610):
+ * γ += Δt γdot, θ += Δt θdot at every point; returns -1 if a value is non-finite */
+int orc_commit_plastic_history(orc_stokes* s, double dt);
 
 /* --- rotate_stress! (src/stokes/stress_rotation.jl:28-61) and the GeoParams
  *     0.7.21 rotate_elastic_stress it calls (restated, not vendored) --- */

@@ -47,6 +47,12 @@ class DP(C.Structure):
                 ("C", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH)]
 
 
+class Cap(C.Structure):
+    _fields_ = [("nphases", I32), ("cos_phi", D * MAXPH), ("sin_phi", D * MAXPH), ("sin_psi", D * MAXPH),
+                ("C", D * MAXPH), ("pT", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH),
+                ("C_min", D * MAXPH), ("H_C", D * MAXPH)]
+
+
 class Stokes(C.Structure):
     _fields_ = [("nnodes", I64), ("nels", I64), ("nnodesP", I64),
                 ("coords", PD), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("dOmegaP", PD),
@@ -56,7 +62,7 @@ class Stokes(C.Structure):
                 ("Rv0x", PD), ("Rv0y", PD), ("dRdvx", PD), ("dRdvy", PD), ("PCx", PD), ("PCy", PD),
                 ("P", PD), ("P0", PD), ("gammaP", PD), ("T", PD), ("T0", PD), ("Q", PD), ("RP", PD),
                 ("MP", PD), ("Pnum", PD), ("Pf", PD), ("tau_old", PD * 3), ("tau_store", PD * 3),
-                ("nbc", I64 * 2), ("bc_dofs", PI * 2), ("bc_vals", PD * 2)]
+                ("nbc", I64 * 2), ("bc_dofs", PI * 2), ("bc_vals", PD * 2), ("cap", C.POINTER(Cap)), ("gamma_hist", PD), ("theta_hist", PD), ("P_store", PD)]
 
 
 class Params(C.Structure):
@@ -136,6 +142,15 @@ def _declare(L):
     L.orc_thermal_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(ThermalParamsC), C.POINTER(ThermalStatsC)]
     L.orc_thermal_solve.restype = C.c_int
     L.orc_shear_heating.argtypes = [C.POINTER(Stokes), D, PD]
+    L.orc_cap_geometry.argtypes = [D, D, D, D, PD]
+    L.orc_cap_yield_function.argtypes = [D, D, D, D, D, D]
+    L.orc_cap_yield_function.restype = D
+    L.orc_cap_invariants.argtypes = [D, D, D, D, D, D, PD]
+    L.orc_cap_return_map.argtypes = [D, D, D, D, D, D, D, D, D, I32, PD]
+    L.orc_cap_return_map.restype = C.c_int
+    L.orc_cap_local_update.argtypes = [PD, D, D, D, D, D, D, D, D, PD, PD]
+    L.orc_commit_plastic_history.argtypes = [C.POINTER(Stokes), D]
+    L.orc_commit_plastic_history.restype = C.c_int
     L.orc_rotate_elastic_stress2.argtypes = [D, PD, D, PD]
     L.orc_rotate_elastic_stress3.argtypes = [PD, PD, D, PD]
     L.orc_rotate_stress.argtypes = [C.POINTER(Stokes), D, C.POINTER(PD), C.POINTER(PD)]
@@ -285,6 +300,8 @@ class Problem:
     tau_old: tuple | None = None
     bc: tuple = ((np.zeros(0, np.int32), np.zeros(0)), (np.zeros(0, np.int32), np.zeros(0)))
     arrays: dict = field(default_factory=dict)
+    cap: Cap | None = None
+    history: bool = False
 
     def __post_init__(self):
         self.coords = f64(self.coords)
@@ -304,6 +321,9 @@ class Problem:
         for k in ("P", "P0", "gammaP", "T", "T0", "Q", "RP", "MP", "Pnum"):
             self.arrays[k] = np.zeros(npp)
         self.tau_store = [np.zeros((ne, 7)) for _ in range(3)]
+        self.P_store = np.zeros((ne, 7))
+        self.gamma_hist = np.zeros((ne, 7)) if self.history else None
+        self.theta_hist = np.zeros((ne, 7)) if self.history else None
         if self.tau_old is not None:
             self.tau_old = tuple(f64(t) for t in self.tau_old)
         self.bc = tuple((i32(d), f64(v)) for d, v in self.bc)
@@ -335,8 +355,17 @@ class Problem:
             s.nbc[c] = len(d)
             s.bc_dofs[c] = ptr(d, PI)
             s.bc_vals[c] = ptr(v)
+        s.cap = C.pointer(self.cap) if self.cap is not None else C.cast(None, C.POINTER(Cap))
+        s.gamma_hist = ptr(self.gamma_hist) if self.gamma_hist is not None else ptr(None)
+        s.theta_hist = ptr(self.theta_hist) if self.theta_hist is not None else ptr(None)
+        s.P_store = ptr(self.P_store)
         self._struct = s
         return s
+
+    def commit_plastic_history(self, dt):
+        """commit_stokes_plastic_history! (DR.jl:899-958)."""
+        if lib().orc_commit_plastic_history(C.byref(self.struct()), dt) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
 
     # assemblers -------------------------------------------------------------
     def pressure_scaling(self, gamma_fact, dt, finite_K=False):
@@ -390,6 +419,50 @@ class Problem:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return st.as_dict()
+
+
+def drucker_prager_cap(phi, psi, C_, pT, eta_reg, Kb, C_min=None, H_C=None):
+    """DruckerPragerCap(ϕ, Ψ, C, pT, η_reg, Kb; C_min=C, H_C=0)
+    (stokes_types.jl:492-544); angles in radians, no validation."""
+    n = len(phi)
+    cp = Cap()
+    cp.nphases = n
+    C_min = C_ if C_min is None else C_min
+    H_C = (0.0,) * n if H_C is None else H_C
+    for i in range(n):
+        cp.cos_phi[i], cp.sin_phi[i], cp.sin_psi[i] = math.cos(phi[i]), math.sin(phi[i]), math.sin(psi[i])
+        cp.C[i], cp.pT[i], cp.eta_reg[i], cp.Kb[i] = C_[i], pT[i], eta_reg[i], Kb[i]
+        cp.C_min[i], cp.H_C[i] = C_min[i], H_C[i]
+    return cp
+
+
+def cap_geometry(k, kq, c, pT):
+    out = np.zeros(7)
+    lib().orc_cap_geometry(k, kq, c, pT, ptr(out))
+    return dict(zip(("a", "b", "p_y", "R_y", "p_d", "tau_d", "p_q"), out))
+
+
+def cap_yield_function(tII, P, k, kq, c, pT):
+    return lib().orc_cap_yield_function(tII, P, k, kq, c, pT)
+
+
+def cap_invariants(s, p, k, kq, c, pT):
+    out = np.zeros(3)
+    lib().orc_cap_invariants(s, p, k, kq, c, pT, ptr(out))
+    return tuple(out)
+
+
+def cap_return_map(s_trial, p_trial, eta_ve, KDt, k, kq, c, pT, eta_reg, maxiter=40):
+    out = np.zeros(5)
+    conv = lib().orc_cap_return_map(s_trial, p_trial, eta_ve, KDt, k, kq, c, pT, eta_reg, maxiter, ptr(out))
+    return dict(tII=out[0], P=out[1], lam=out[2], At=out[3], Ap=out[4], converged=bool(conv))
+
+
+def cap_local_update(tau_trial, P_trial, eta_ve, KDt, k, kq, c, pT, eta_reg):
+    tau = np.zeros(3)
+    out = np.zeros(3)
+    lib().orc_cap_local_update(ptr(f64(tau_trial)), P_trial, eta_ve, KDt, k, kq, c, pT, eta_reg, ptr(tau), ptr(out))
+    return dict(tau=tau, P=out[0], gdot=out[1], thdot=out[2])
 
 
 def pure_shear_bc(coords, box=(0.0, 1.0, 0.0, 1.0), edot=1.0):
