@@ -359,7 +359,8 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         else if (k <= FEM_FIELD_TAU_OLD_XY) n = nip;
         else if (k <= FEM_FIELD_PNUM) n = np;
         else if (k <= FEM_FIELD_PC_V_Z) n = m->dim == 3 ? nv : 0;
-        else n = m->dim == 3 ? nip : 0;
+        else if (k <= FEM_FIELD_TAU_OLD_YZ) n = m->dim == 3 ? nip : 0;
+        else n = m->dim == 2 ? nip : 0;
         s->len[k] = n;
         // +16 entries: 16-byte global->LDS loads of the last block may read
         // 8 bytes past the final element of an integration-point array
@@ -469,11 +470,18 @@ int fem_stokes_set_phases(fem_stokes* s, int32_t which, int32_t layout, const in
 // per-phase table, boundary conditions, kernel launches
 // ---------------------------------------------------------------------------
 int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
-                       bool* thermal, bool* eos_p, bool* elastic) {
+                       bool* thermal, bool* eos_p, bool* elastic, const fem_drucker_prager_cap* cap) {
     fem_ctx* c = s->ctx;
     const fem_material& m = s->mat;
     if (dp && dp->nphases != m.nphases)
         return set_err(c, FEM_EDIM, "DruckerPrager has %d phases, material has %d", dp->nphases, m.nphases);
+    if (cap && cap->nphases != m.nphases)
+        return set_err(c, FEM_EDIM, "DruckerPragerCap has %d phases, material has %d", cap->nphases, m.nphases);
+    if (cap)
+        for (int p = 0; p < m.nphases; ++p)
+            if (!(cap->Kb[p] > 0 && std::isfinite(cap->Kb[p])))
+                return set_err(c, FEM_EARG, "phase %d: dilatant plasticity requires a positive finite bulk modulus, got %g",
+                               p + 1, cap->Kb[p]);
     std::vector<double> t((size_t)PC_NCOL * MAXPH, 0.0);
     *bulk_inf = true; *thermal = false; *eos_p = false; *elastic = false;
     for (int p = 0; p < m.nphases; ++p) {
@@ -497,6 +505,17 @@ int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int fini
             t[PC_C * MAXPH + p] = dp->C[p];
             t[PC_EREG * MAXPH + p] = dp->eta_reg[p];
             t[PC_KB * MAXPH + p] = dp->Kb[p];
+        }
+        if (cap) {
+            t[PC_COSPHI * MAXPH + p] = cap->cos_phi[p];
+            t[PC_SINPHI * MAXPH + p] = cap->sin_phi[p];
+            t[PC_SINPSI * MAXPH + p] = cap->sin_psi[p];
+            t[PC_C * MAXPH + p] = cap->C[p];
+            t[PC_EREG * MAXPH + p] = cap->eta_reg[p];
+            t[PC_KB * MAXPH + p] = cap->Kb[p];
+            t[PC_PT * MAXPH + p] = cap->pT[p];
+            t[PC_CMIN * MAXPH + p] = cap->C_min[p];
+            t[PC_HC * MAXPH + p] = cap->H_C[p];
         }
         if (!std::isinf(bulk)) *bulk_inf = false;
         if (m.alpha[p] != 0) *thermal = true;
@@ -569,6 +588,8 @@ struct LaunchOpts {
     int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
     uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
     double* phi_out = nullptr;        // nullable (FP64 residual launches): shear dissipation [nels][7]
+    bool cap = false;                 // the tensile cap is the yield model (FP64, EF_GEN launches)
+    bool commit = false;              // a cap launch that commits the plastic history
 };
 
 constexpr int EBLOCK = 128;
@@ -741,7 +762,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
-    const bool gen = nodeph || (!d.lin && (o.thermal || s->pf_nonzero || s->q_nonzero));
+    const bool gen = nodeph || (!d.lin && (o.thermal || s->pf_nonzero || s->q_nonzero || o.cap));
     a.nodeph = nodeph;
     a.bulk_all_inf = o.bulk_inf;
     a.thermal = o.thermal;
@@ -765,6 +786,15 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     if constexpr (std::is_same<F, double>::value) {
         if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
         a.phi_out = o.phi_out;
+        if (o.cap) {
+            a.cap = 1;
+            a.gamma = s->f[FEM_FIELD_GAMMA_PL];
+            if (o.commit) {
+                a.hist_gamma = s->f[FEM_FIELD_GAMMA_PL];
+                a.hist_theta = s->f[FEM_FIELD_THETA_PL];
+            }
+        }
+        if (o.store_tau && s->dim == 2) a.P_out = s->f[FEM_FIELD_P_IP];
         dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
     } else {
         if (!d.lin || !o.use_pnum)
@@ -864,6 +894,8 @@ static int launch_jacobian(fem_stokes* s, const LaunchOpts& o) {
     a.dRdv_y = s->f[FEM_FIELD_DRVDV_Y];
     a.PC_x = s->f[FEM_FIELD_PC_V_X];
     a.PC_y = s->f[FEM_FIELD_PC_V_Y];
+    a.cap = o.cap;
+    a.gamma = o.cap ? s->f[FEM_FIELD_GAMMA_PL] : nullptr;
     const bool np_ = s->layout_v == FEM_PHASE_ELEMENT;
     const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
     switch (key) {
@@ -990,12 +1022,14 @@ static int any_nonzero(fem_stokes* s, const double* x, int64_t n, bool* out) {
 }
 
 static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, LaunchOpts& o,
-                   bool use_tau_old) {
+                   bool use_tau_old, const fem_drucker_prager_cap* cap = nullptr) {
     bool bulk_inf, thermal, eos_p, elastic;
+    if (cap && s->dim != 2) return set_err(s->ctx, FEM_EARG, "the tensile cap is built for 2-D states");
     CHK(any_nonzero(s, s->f[FEM_FIELD_PF], s->mesh->nnodesP, &s->pf_nonzero));
     CHK(any_nonzero(s, s->f[FEM_FIELD_Q], s->mesh->nnodesP, &s->q_nonzero));
-    CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
-    o.plastic = dp != nullptr;
+    CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic, cap));
+    o.plastic = dp != nullptr || cap != nullptr;
+    o.cap = cap != nullptr;
     o.elastic = elastic && use_tau_old;
     o.bulk_inf = bulk_inf;
     o.thermal = thermal;
@@ -1660,13 +1694,17 @@ static int shear_heating_launch(fem_stokes* s, const LaunchOpts& o, double* phi)
 // solve! on a 2-D state (DR.jl:302-337 and the loop of DR.jl:348-802); with
 // cp, the coupled loop of solve_coupled! (DR.jl:514-661)
 static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
-                    fem_solve_stats* st, CoupledSide* cp) {
+                    fem_solve_stats* st, CoupledSide* cp, const fem_drucker_prager_cap* cap = nullptr) {
     fem_ctx* c = s->ctx;
     std::memset(st, 0, sizeof *st);
     HIPCHK(c, hipSetDevice(c->device));
     if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
     if (p->precision != 0 && p->precision != 1)
         return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
+                       p->precision);
+    if (cap && s->dim != 2) return set_err(c, FEM_EARG, "the tensile cap is built for 2-D states");
+    if (cap && p->precision != 0)
+        return set_err(c, FEM_EARG, "solves with the tensile cap run in FP64 (precision 0), got precision %d",
                        p->precision);
     if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
     const auto t0 = std::chrono::steady_clock::now();
@@ -1679,8 +1717,8 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
         CHK(transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T0), s->f[FEM_FIELD_T0]));
     }
     LaunchOpts o;
-    CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
-    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
+    CHK(prepare(s, dp, dt, p->finite_K, o, p->use_tau_old != 0, cap));
+    const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr && cap == nullptr) : (p->freeze_jacobian != 0);
     CHK(upload_bcs(s, bcs));
     if (!p->skip_scaling) CHK(scaling_impl(s, p->pressure_factor, dt));
     else
@@ -1920,6 +1958,80 @@ int fem_stokes_rotate_stress(fem_stokes* s, double dt) {
                        f[FEM_FIELD_TAU_OLD_YY], f[FEM_FIELD_TAU_OLD_XY], dt);
     CHK(check_launch(c, "k_rotate_stress2"));
     return fem_sync(c);
+}
+
+// ---- the tensile cap (rheology.jl:510-787) through the 2-D entry points
+int fem_stokes_solve_cap(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager_cap* cap,
+                         const fem_solve_params* p, fem_solve_stats* st) {
+    if (!s || !bcs || !cap || !p || !st) return FEM_EARG;
+    return solve_2d(s, bcs, nullptr, p, st, nullptr, cap);
+}
+
+int fem_stokes_residual_cap(fem_stokes* s, const fem_drucker_prager_cap* cap, double dt, int32_t with_pnum,
+                            int32_t finite_K) {
+    if (!s || !cap) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, finite_K, o, true, cap));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    o.use_pnum = with_pnum != 0;
+    o.store_RP = true;
+    o.store_Pnum = with_pnum != 0;
+    o.store_Rv = true;
+    CHK(launch_element(s, set64(s), o));
+    CHK(launch_gather(s, set64(s), o));
+    return fem_sync(c);
+}
+
+int fem_stokes_jacobian_diagnostics_cap(fem_stokes* s, const fem_drucker_prager_cap* cap, double dt,
+                                        int32_t finite_K) {
+    if (!s || !cap) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, finite_K, o, true, cap));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(s->mesh->nnodesP, 256)), dim3(256), 0, c->stream, s->mesh->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    CHK(launch_jacobian(s, o));
+    return fem_sync(c);
+}
+
+int fem_stokes_update_current_stress_cap(fem_stokes* s, const fem_drucker_prager_cap* cap, double dt) {
+    if (!s || !cap) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, 1, o, true, cap));
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    o.store_tau = true;
+    o.store_Rv = true;
+    CHK(launch_element(s, set64(s), o));
+    CHK(launch_gather(s, set64(s), o));
+    return fem_sync(c);
+}
+
+// commit_stokes_plastic_history! (DR.jl:899-958): one residual launch of the
+// element kernel that writes γ + Δt γdot and θ + Δt θdot at every point
+int fem_stokes_commit_plastic_history(fem_stokes* s, const fem_drucker_prager_cap* cap, double dt) {
+    if (!s || !cap) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, 1, o, true, cap));
+    o.commit = true;
+    CHK(launch_element(s, set64(s), o));
+    const int64_t n = s->mesh->nels * NQ;
+    std::vector<double> h(2 * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), s->f[FEM_FIELD_GAMMA_PL], n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data() + n, s->f[FEM_FIELD_THETA_PL], n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (double x : h)
+        if (!std::isfinite(x))
+            return set_err(c, FEM_ENAN, "non-finite plastic history: a tensile-cap local return map failed");
+    return FEM_OK;
 }
 
 // assemble_shear_heating! (shear_heating.jl:18-46)

@@ -214,7 +214,7 @@ static inline void cheb(double dtau, double lmin, double c_fact, double* a, doub
 }
 
 int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
-                bool* thermal, bool* eos_p, bool* elastic);
+                bool* thermal, bool* eos_p, bool* elastic, const fem_drucker_prager_cap* cap = nullptr);
 
 // The thermal side of a coupled thermal-Stokes solve (DR.jl:514-529,
 // 604-661): one thermal DR iteration per Stokes inner iteration, on the

@@ -127,6 +127,7 @@ enum PhaseCol {
     PC_ETA = 0, PC_INVG, PC_RHO0, PC_ALPHA, PC_BETA, PC_BULK,
     PC_COSPHI, PC_SINPHI, PC_SINPSI, PC_C, PC_EREG, PC_KB,
     PC_ETAVE, PC_INV2GDT, PC_YIELD0, PC_INVDEN,   // dt-dependent, element-constant phases
+    PC_PT, PC_CMIN, PC_HC,                        // tensile cap: pT, C_min, H_C
     PC_NCOL
 };
 
@@ -192,6 +193,200 @@ __device__ inline bool dp_correction(F txx, F tyy, F txy, const IpMat<F>& m, F P
     cyy = (txx + F(2) * tyy) * h;
     cxy = (F(2) * txy) * h;
     return true;
+}
+
+// ---------------------------------------------------------------------------
+// Drucker-Prager tensile cap (rheology.jl:510-760; Popov, Berlie & Kaus 2025,
+// Eqs. 13-18, 42), FP64. Pressure is compression-positive, pT <= 0.
+// ---------------------------------------------------------------------------
+struct CapGeo { double a, b, p_y, R_y, p_d, tau_d, p_q; };
+
+// cap_geometry (rheology.jl:546-557)
+__device__ inline CapGeo cap_geo(double k, double kq, double c, double pT) {
+    CapGeo g;
+    g.a = sqrt(1 + k * k);
+    g.b = sqrt(1 + kq * kq);
+    g.p_y = (g.a * pT + c) / (g.a - k);
+    g.R_y = g.p_y - pT;
+    g.p_d = g.p_y - g.R_y * k / g.a;
+    g.tau_d = g.R_y / g.a;
+    g.p_q = g.p_d + kq * g.tau_d;
+    return g;
+}
+
+// cap_yield_function and cap_invariants (rheology.jl:582-609) with their
+// partials in (s, p): the yield and flow-potential branches are chosen
+// independently by the half-plane tests of the reference
+struct CapInv { double F, At, Ap, dF[2], dAt[2], dAp[2]; };
+__device__ inline CapInv cap_inv(double s, double p, double k, double kq, double c, const CapGeo& g) {
+    CapInv r;
+    if (p + k * s >= g.p_y) {
+        r.F = s - k * p - c;
+        r.dF[0] = 1; r.dF[1] = -k;
+    } else {
+        const double dy = p - g.p_y;
+        const double R = sqrt(s * s + dy * dy + eps2<double>());
+        r.F = g.a * (R - g.R_y);
+        r.dF[0] = g.a * s / R; r.dF[1] = g.a * dy / R;
+    }
+    r.dAt[0] = r.dAt[1] = r.dAp[0] = r.dAp[1] = 0;
+    if (p + kq * s >= g.p_q) {
+        r.At = 0.5; r.Ap = kq;
+        return r;
+    }
+    const double d = p - g.p_q;
+    const double Rq = sqrt(s * s + d * d);
+    if (Rq == 0) { r.At = 0; r.Ap = 0; return r; }
+    const double i3 = 1 / (Rq * Rq * Rq);
+    r.At = g.b * s / (2 * Rq);
+    r.Ap = -g.b * d / Rq;
+    r.dAt[0] = 0.5 * g.b * d * d * i3; r.dAt[1] = -0.5 * g.b * s * d * i3;
+    r.dAp[0] = g.b * d * s * i3;       r.dAp[1] = -g.b * s * s * i3;
+    return r;
+}
+
+// 3x3 solve, Gaussian elimination with partial pivoting
+__device__ inline bool cap_solve3(double A[3][3], double b[3], double x[3]) {
+    int p0 = 0, p1 = 1, p2 = 2;
+    int piv[3] = {p0, p1, p2};
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        int best = col;
+        for (int r = col + 1; r < 3; ++r)
+            if (fabs(A[piv[r]][col]) > fabs(A[piv[best]][col])) best = r;
+        const int t = piv[col]; piv[col] = piv[best]; piv[best] = t;
+        const double dg = A[piv[col]][col];
+        if (dg == 0 || !isfinite(dg)) return false;
+        for (int r = col + 1; r < 3; ++r) {
+            const double f = A[piv[r]][col] / dg;
+            for (int cc = col; cc < 3; ++cc) A[piv[r]][cc] -= f * A[piv[col]][cc];
+            b[piv[r]] -= f * b[piv[col]];
+        }
+    }
+    for (int i = 2; i >= 0; --i) {
+        double acc = b[piv[i]];
+        for (int cc = i + 1; cc < 3; ++cc) acc -= A[piv[i]][cc] * x[cc];
+        x[i] = acc / A[piv[i]][i];
+    }
+    return true;
+}
+
+struct CapPoint {
+    double s_tr, p_tr, eve, KDt, k, kq, c, pT, ereg;
+    CapGeo g;
+    __device__ void residual(const double x[3], double r[3]) const {
+        const CapInv v = cap_inv(x[0], x[1], k, kq, c, g);
+        r[0] = x[0] - s_tr + 2 * eve * x[2] * v.At;
+        r[1] = x[1] - p_tr - KDt * x[2] * v.Ap;
+        r[2] = v.F - ereg * x[2];
+    }
+    __device__ void jacobian(const double x[3], double J[3][3]) const {
+        const CapInv v = cap_inv(x[0], x[1], k, kq, c, g);
+        const double lam = x[2];
+        J[0][0] = 1 + 2 * eve * lam * v.dAt[0]; J[0][1] = 2 * eve * lam * v.dAt[1]; J[0][2] = 2 * eve * v.At;
+        J[1][0] = -KDt * lam * v.dAp[0]; J[1][1] = 1 - KDt * lam * v.dAp[1]; J[1][2] = -KDt * v.Ap;
+        J[2][0] = v.dF[0]; J[2][1] = v.dF[1]; J[2][2] = -ereg;
+    }
+};
+
+// cap_return_map (rheology.jl:652-690): x = (s, p, λ); Newton on residuals
+// scaled by max(|s_trial|, |p_trial|, |F_trial|, eps), halving backtracking
+// (24 tries) under s >= 0, λ >= 0, a finite residual and the Armijo test,
+// ‖r‖∞ <= 100 eps, one final full step. dx = ∂x/∂s_trial = J⁻¹ e₁ at the
+// solution (the derivative the final step carries through dual numbers).
+struct CapRes { double s, p, lam, At, Ap, dx[3]; bool ok; };
+__device__ inline CapRes cap_return(const CapPoint& cp) {
+    CapRes o;
+    const CapInv v0 = cap_inv(cp.s_tr, cp.p_tr, cp.k, cp.kq, cp.c, cp.g);
+    o.s = cp.s_tr; o.p = cp.p_tr; o.lam = 0; o.At = v0.At; o.Ap = v0.Ap;
+    o.dx[0] = 1; o.dx[1] = 0; o.dx[2] = 0;
+    o.ok = true;
+    if (v0.F <= 0) return o;
+    o.ok = false;
+    if (!(isfinite(cp.KDt) && cp.KDt > 0 && isfinite(cp.eve) && cp.eve > 0)) return o;
+    const double eps = 2.220446049250313e-16;
+    const double scale = fmax(fmax(fabs(cp.s_tr), fabs(cp.p_tr)), fmax(fabs(v0.F), eps));
+    const double isc = 1 / scale, tol = 100 * eps;
+    double x[3] = {cp.s_tr, cp.p_tr, 0.0}, r[3];
+    cp.residual(x, r);
+    for (int i = 0; i < 3; ++i) r[i] *= isc;
+    bool conv = fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2]))) <= tol;
+    for (int it = 0; it < 40 && !conv; ++it) {
+        double J[3][3], rhs[3] = {r[0], r[1], r[2]}, st[3];
+        cp.jacobian(x, J);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[i][j] *= isc;
+        if (!cap_solve3(J, rhs, st) || !(isfinite(st[0]) && isfinite(st[1]) && isfinite(st[2]))) return o;
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double alpha = 1;
+        bool acc = false;
+        for (int ls = 0; ls < 24; ++ls) {
+            const double cand[3] = {x[0] - alpha * st[0], x[1] - alpha * st[1], x[2] - alpha * st[2]};
+            if (cand[0] >= 0 && cand[2] >= 0) {
+                double rn[3];
+                cp.residual(cand, rn);
+                for (int i = 0; i < 3; ++i) rn[i] *= isc;
+                if (isfinite(rn[0]) && isfinite(rn[1]) && isfinite(rn[2]) &&
+                    rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2] <= (1 - alpha / 10000) * r2) {
+                    for (int i = 0; i < 3; ++i) { x[i] = cand[i]; r[i] = rn[i]; }
+                    acc = true;
+                    break;
+                }
+            }
+            alpha *= 0.5;
+        }
+        if (!acc) return o;
+        conv = fmax(fabs(r[0]), fmax(fabs(r[1]), fabs(r[2]))) <= tol;
+    }
+    if (!conv) return o;
+    double J[3][3], Jc[3][3], rhs[3] = {r[0], r[1], r[2]}, st[3];
+    cp.jacobian(x, J);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { Jc[i][j] = J[i][j]; J[i][j] *= isc; }
+    if (!cap_solve3(J, rhs, st)) return o;
+    for (int i = 0; i < 3; ++i) x[i] -= st[i];
+    double e1[3] = {1, 0, 0};
+    if (!cap_solve3(Jc, e1, o.dx)) return o;
+    const CapInv v = cap_inv(x[0], x[1], cp.k, cp.kq, cp.c, cp.g);
+    o.s = x[0]; o.p = x[1]; o.lam = x[2]; o.At = v.At; o.Ap = v.Ap;
+    o.ok = true;
+    return o;
+}
+
+// cap_local_update (rheology.jl:708-717): radial deviatoric return of the
+// trial stress and the corrected pressure P_trial + KΔt λ Ap. S = double, or
+// D3<double> carrying the implicit derivative with respect to the strain
+// rate (P_trial does not depend on the velocity). A failed local solve
+// gives NaN, which the solver's residual checks report.
+template <class S>
+__device__ inline void cap_update(S txx, S tyy, S txy, const CapPoint& cp0, S& oxx, S& oyy, S& oxy, S& P,
+                                  double* gdot = nullptr, double* thdot = nullptr) {
+    const S tzz = -txx - tyy;
+    const S s_tr = dsqrt((txx * txx + tyy * tyy + tzz * tzz) * 0.5 + txy * txy + eps2<double>());
+    CapPoint cp = cp0;
+    cp.s_tr = val(s_tr);
+    const CapRes r = cap_return(cp);
+    if (!r.ok) {
+        const double nan = __builtin_nan("");
+        oxx = oyy = oxy = P = S(nan);
+        if (gdot) { *gdot = nan; *thdot = nan; }
+        return;
+    }
+    const S ds = s_tr - val(s_tr);
+    const S s = ds * r.dx[0] + r.s, p = ds * r.dx[1] + r.p, lam = ds * r.dx[2] + r.lam;
+    S At = S(r.At), Ap = S(r.Ap);
+    if (r.lam != 0) {
+        const CapInv v = cap_inv(r.s, r.p, cp.k, cp.kq, cp.c, cp.g);
+        At = (s - r.s) * v.dAt[0] + (p - r.p) * v.dAt[1] + r.At;
+        Ap = (s - r.s) * v.dAp[0] + (p - r.p) * v.dAp[1] + r.Ap;
+    }
+    const S g = lam * At, th = lam * Ap;
+    const S f = g * (2 * cp.eve) * drcp(s_tr);
+    oxx = txx - f * txx;
+    oyy = tyy - f * tyy;
+    oxy = txy - f * txy;
+    P = th * cp.KDt + cp.p_tr;
+    if (gdot) { *gdot = val(g); *thdot = val(th); }
 }
 
 // ---------------------------------------------------------------------------
@@ -310,6 +505,14 @@ struct ElemArgs {
     // nullable, residual launches only: the shear dissipation τ : (ε̇ − ε̇ᵉˡ)
     // per quadrature point [nels][7]
     F* phi_out;
+    // tensile cap (FP64, EF_GEN|EF_PLASTIC launches): the yield model, the
+    // accepted plastic strain γ that softens the cohesion (nullable), the
+    // history written by a commit (γ += Δt γdot, θ += Δt θdot; nullable) and
+    // the pressure of the momentum balance per point (nullable)
+    int32_t cap;
+    const F* gamma;
+    F* hist_gamma; F* hist_theta;
+    F* P_out;
 };
 
 // Workgroups are dealt round-robin over the 8 XCDs (one L2 each). The grid
@@ -620,6 +823,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
 #pragma unroll
             for (int i = 0; i < NP; ++i) Pfq += (F)kT.NP1[q][i] * A.Pf[e * NP + i];
         }
+        F Pmom = Pq;   // the pressure of the momentum balance (the cap corrects it)
         F ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
         for (int a = 0; a < NV; ++a) {
@@ -676,21 +880,65 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             }
           }
         } else {
-            // the yield statistics are taken by residual evaluations outside the
-            // inner iteration only (no registers spent on them in EF_ITER)
-            constexpr bool STATS = PLASTIC && !ITER;
-            F fy = 0;
-            const bool y = stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq, txx,
-                                                   tyy, txy, STATS ? &fy : nullptr);
-            if (STATS) {
-                yel |= y;
-                near |= fy > F(-0.2) * fabs(m.yield0 + m.sphi * (Pq - Pfq));
+            bool capped = false;
+            if constexpr (GEN && PLASTIC && !LIN && std::is_same<F, double>::value) {
+                if (A.cap) {
+                    // the tensile cap: the visco-elastic trial stress, the cap
+                    // parameters and softened cohesion at the point, the local
+                    // update (rheology.jl:722-741); the momentum balance uses the
+                    // corrected pressure
+                    capped = true;
+                    F ttxx, ttyy, ttxy;
+                    stress2d<F, F, false>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, F(0), ttxx, ttyy, ttxy);
+                    auto at = [&](int col) -> double {
+                        if (!nodeph) return PT(col, pc);
+                        double acc = 0;
+#pragma unroll
+                        for (int a = 0; a < NV; ++a) acc += kT.N[q][a] * PT(col, pv[a]);
+                        return acc;
+                    };
+                    double C = at(PC_C);
+                    const double gam = A.gamma ? A.gamma[e * NQ + q] : 0.0;
+                    if (A.gamma) C = fmin(fmax(C + at(PC_HC) * gam, at(PC_CMIN)), C);
+                    CapPoint cp;
+                    cp.p_tr = Pq - Pfq;
+                    cp.eve = m.etave;
+                    cp.KDt = at(PC_KB) * A.dt;
+                    cp.k = at(PC_SINPHI);
+                    cp.kq = at(PC_SINPSI);
+                    cp.c = C * at(PC_COSPHI);
+                    cp.pT = at(PC_PT);
+                    cp.ereg = at(PC_EREG);
+                    cp.g = cap_geo(cp.k, cp.kq, cp.c, cp.pT);
+                    F Pe;
+                    double gd, thd;
+                    cap_update<F>(ttxx, ttyy, ttxy, cp, txx, tyy, txy, Pe, &gd, &thd);
+                    Pmom = Pe + Pfq;
+                    if (PLASTIC && !ITER) yel |= gd != 0;
+                    if (A.hist_gamma) {
+                        A.hist_gamma[e * NQ + q] = gam + A.dt * gd;
+                        A.hist_theta[e * NQ + q] += A.dt * thd;
+                    }
+                }
+            }
+            if (!capped) {
+                // the yield statistics are taken by residual evaluations outside
+                // the inner iteration only (no registers spent on them in EF_ITER)
+                constexpr bool STATS = PLASTIC && !ITER;
+                F fy = 0;
+                const bool y = stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq,
+                                                       txx, tyy, txy, STATS ? &fy : nullptr);
+                if (STATS) {
+                    yel |= y;
+                    near |= fy > F(-0.2) * fabs(m.yield0 + m.sphi * (Pq - Pfq));
+                }
             }
         }
         if (A.tau_out0) {
             A.tau_out0[e * NQ + q] = txx;
             A.tau_out1[e * NQ + q] = tyy;
             A.tau_out2[e * NQ + q] = txy;
+            if (A.P_out) A.P_out[e * NQ + q] = Pmom;
         }
         if (!ITER && !LIN && A.phi_out) {
             // stress power less the elastic power (τ − τ_old) : τ / (2GΔt), the
@@ -702,7 +950,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             A.phi_out[e * NQ + q] = power - elastic;
         }
         const F dOm = (F)kT.w[q] * det;
-        const F Ptot = Pq + Pnq;
+        const F Ptot = Pmom + Pnq;
         const F sxx = txx - Ptot, syy = tyy - Ptot;
         // reference-space stress: R_i[a] = Σ_j ∂N_a/∂ξ_j s_ji
         const F s0x = (i00 * sxx + i01 * txy) * dOm;
@@ -947,6 +1195,8 @@ struct JacArgs {
     typename Vec2<F>::T* Jrow;   // [nels][6] (row sums x,y)
     typename Vec2<F>::T* Jdiag;  // [nels][6] (diag x,y)
     F* dRdv_x; F* dRdv_y; F* PC_x; F* PC_y;   // bubble entries written directly
+    int32_t cap;                 // PLASTIC: the tensile cap instead of Drucker-Prager
+    const F* gamma;              // nullable: accepted plastic strain (cohesion softening)
 };
 
 template <class F, bool ELASTIC, bool PLASTIC, bool NODEPH, int BLOCK>
@@ -1027,7 +1277,39 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
         }
         D3<F> exx(gxx, F(1), F(0), F(0)), eyy(gyy, F(0), F(1), F(0)), exy((gxy + gyx) * F(0.5), F(0), F(0), F(1));
         D3<F> txx, tyy, txy;
-        stress2d<D3<F>, F, PLASTIC>(exx, eyy, exy, m, txo, tyo, txyo, Pq, txx, tyy, txy);
+        bool capped = false;
+        if constexpr (PLASTIC && std::is_same<F, double>::value) {
+            if (A.cap) {
+                // the cap's tangent carries the corrected pressure, which also
+                // depends on the strain rate: C holds ∂(τ − P_local I)/∂ε
+                capped = true;
+                D3<F> ttxx, ttyy, ttxy;
+                stress2d<D3<F>, F, false>(exx, eyy, exy, m, txo, tyo, txyo, F(0), ttxx, ttyy, ttxy);
+                auto at = [&](int col) -> double {
+                    double acc = 0;
+#pragma unroll
+                    for (int a = 0; a < NV; ++a) acc += kT.N[q][a] * PT(col, pv[a]);
+                    return acc;
+                };
+                double C = at(PC_C);
+                if (A.gamma) C = fmin(fmax(C + at(PC_HC) * A.gamma[e * NQ + q], at(PC_CMIN)), C);
+                CapPoint cp;
+                cp.p_tr = Pq;
+                cp.eve = m.etave;
+                cp.KDt = at(PC_KB) * A.dt;
+                cp.k = at(PC_SINPHI);
+                cp.kq = at(PC_SINPSI);
+                cp.c = C * at(PC_COSPHI);
+                cp.pT = at(PC_PT);
+                cp.ereg = at(PC_EREG);
+                cp.g = cap_geo(cp.k, cp.kq, cp.c, cp.pT);
+                D3<F> Pe;
+                cap_update<D3<F>>(ttxx, ttyy, ttxy, cp, txx, tyy, txy, Pe);
+                txx = txx - Pe;
+                tyy = tyy - Pe;
+            }
+        }
+        if (!capped) stress2d<D3<F>, F, PLASTIC>(exx, eyy, exy, m, txo, tyo, txyo, Pq, txx, tyy, txy);
         Cq[q][0][0] = txx.d0; Cq[q][0][1] = txx.d1; Cq[q][0][2] = txx.d2;
         Cq[q][1][0] = tyy.d0; Cq[q][1][1] = tyy.d1; Cq[q][1][2] = tyy.d2;
         Cq[q][2][0] = txy.d0; Cq[q][2][1] = txy.d1; Cq[q][2][2] = txy.d2;

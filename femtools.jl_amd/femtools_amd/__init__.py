@@ -29,7 +29,8 @@ __all__ = [
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
-    "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto",
+    "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap",
+    "commit_stokes_plastic_history", "PlasticHistory",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -75,6 +76,12 @@ class _Material(C.Structure):
 class _DP(C.Structure):
     _fields_ = [("nphases", I32), ("cos_phi", D * MAXPH), ("sin_phi", D * MAXPH), ("sin_psi", D * MAXPH),
                 ("C", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH)]
+
+
+class _Cap(C.Structure):
+    _fields_ = [("nphases", I32), ("cos_phi", D * MAXPH), ("sin_phi", D * MAXPH), ("sin_psi", D * MAXPH),
+                ("C", D * MAXPH), ("pT", D * MAXPH), ("eta_reg", D * MAXPH), ("Kb", D * MAXPH),
+                ("C_min", D * MAXPH), ("H_C", D * MAXPH)]
 
 
 class _MeshInfo(C.Structure):
@@ -178,6 +185,11 @@ _sig("fem_thermal_solve", C.c_int, VP, _P(_BC), _P(_ThermalParams), _P(_ThermalS
 _sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
 _sig("fem_stokes_shear_heating", C.c_int, VP, _P(_DP), D, VP)
 _sig("fem_stokes_rotate_stress", C.c_int, VP, D)
+_sig("fem_stokes_solve_cap", C.c_int, VP, _P(_BC), _P(_Cap), _P(_SolveParams), _P(_SolveStats))
+_sig("fem_stokes_residual_cap", C.c_int, VP, _P(_Cap), D, I32, I32)
+_sig("fem_stokes_jacobian_diagnostics_cap", C.c_int, VP, _P(_Cap), D, I32)
+_sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
+_sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
 
@@ -191,7 +203,9 @@ EXPORTED = [
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
     "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
-    "fem_stokes_rotate_stress",
+    "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
+    "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
+    "fem_stokes_commit_plastic_history",
 ]
 
 
@@ -366,6 +380,14 @@ class SymmetricTensor3D:
 
 
 @dataclass
+class PlasticHistory:
+    """CapPlasticHistory (stokes_types.jl:83-96): accepted deviatoric (γ) and
+    volumetric (θ) plastic strain at the integration points."""
+    gamma: DeviceArray
+    theta: DeviceArray
+
+
+@dataclass
 class SymmetricTensor2D:
     """SymmetricTensor2D (src/tensor_fields/symmetric_tensor.jl:44-51); the
     invariant slot is not stored (stokes_types.jl:71-79)."""
@@ -465,6 +487,61 @@ class DruckerPrager:
         return d
 
 
+class DruckerPragerCap:
+    """DruckerPragerCap(ϕ, Ψ, C, pT, η_reg, Kb; C_min=C, H_C=0)
+    (stokes_types.jl:449-544): Drucker-Prager closed on the tensile side by
+    the circular cap of Popov, Berlie & Kaus (2025); angles in radians, pT <= 0
+    (compression-positive pressure), optional linear cohesion softening with
+    the accepted plastic strain. Validated per phase as the reference does."""
+
+    def __init__(self, phi, psi, C_, pT, eta_reg, Kb, C_min=None, H_C=None):
+        phi, psi, C_, pT, eta_reg, Kb = (_tup(x) for x in (phi, psi, C_, pT, eta_reg, Kb))
+        n = len(phi)
+        C_min = C_ if C_min is None else _tup(C_min)
+        H_C = (0.0,) * n if H_C is None else _tup(H_C)
+        if not all(len(t) == n for t in (psi, C_, pT, eta_reg, Kb, C_min, H_C)):
+            raise DimensionMismatch("DruckerPragerCap parameter tuples must have the same length")
+        for i in range(n):
+            vals = (phi[i], psi[i], C_[i], pT[i], eta_reg[i], Kb[i], C_min[i], H_C[i])
+            if not all(math.isfinite(x) for x in vals):
+                raise ValueError(f"phase {i + 1}: tensile-cap parameters must all be finite")
+            if pT[i] > 0:
+                raise ValueError(f"phase {i + 1}: tensile strength pT must be ≤ 0 with compression-positive "
+                                 f"pressure, got {pT[i]}")
+            r = C_[i] * math.cos(phi[i]) + math.sin(phi[i]) * pT[i]
+            if not r > 0:
+                raise ValueError(f"phase {i + 1}: cap radius is not positive; need C·cos(ϕ) + sin(ϕ)·pT > 0, "
+                                 f"got {r}. Cohesion must exceed sin(ϕ)·|pT|.")
+            if not Kb[i] > 0:
+                raise ValueError(f"phase {i + 1}: dilatant plasticity requires a positive finite bulk modulus, "
+                                 f"got {Kb[i]}")
+            if not eta_reg[i] >= 0:
+                raise ValueError(f"phase {i + 1}: η_reg must be ≥ 0, got {eta_reg[i]}")
+            if not C_min[i] >= 0:
+                raise ValueError(f"phase {i + 1}: C_min must be ≥ 0, got {C_min[i]}")
+            if not C_min[i] <= C_[i]:
+                raise ValueError(f"phase {i + 1}: C_min must be ≤ C, got C_min={C_min[i]}, C={C_[i]}")
+            if not H_C[i] <= 0:
+                raise ValueError(f"phase {i + 1}: H_C must be ≤ 0 for softening, got {H_C[i]}")
+            if not C_min[i] * math.cos(phi[i]) + math.sin(phi[i]) * pT[i] > 0:
+                raise ValueError(f"phase {i + 1}: softened cap radius must remain positive; need "
+                                 "C_min·cos(ϕ) + sin(ϕ)·pT > 0")
+        self.cos_phi = tuple(math.cos(x) for x in phi)
+        self.sin_phi = tuple(math.sin(x) for x in phi)
+        self.sin_psi = tuple(math.sin(x) for x in psi)
+        self.C, self.pT, self.eta_reg, self.Kb, self.C_min, self.H_C = C_, pT, eta_reg, Kb, C_min, H_C
+        self.nphases = n
+
+    def _c(self):
+        d = _Cap()
+        d.nphases = self.nphases
+        for name in ("cos_phi", "sin_phi", "sin_psi", "C", "pT", "eta_reg", "Kb", "C_min", "H_C"):
+            arr = getattr(d, name)
+            for i, x in enumerate(getattr(self, name)):
+                arr[i] = x
+        return d
+
+
 class MixedMesh:
     """MixedMesh of T7 (2-D) or Hex27 (3-D) velocity / P1-disc pressure
     elements (src/mesh/mixed_mesh.jl:173-211, 358-417) resident on the device."""
@@ -555,7 +632,7 @@ _FIELDS = ["v.x", "v.y", "dvdtau.x", "dvdtau.y", "Rv.x", "Rv.y", "Rv0.x", "Rv0.y
            "PC_v.x", "PC_v.y", "tau.xx", "tau.yy", "tau.xy", "tau_old.xx", "tau_old.yy", "tau_old.xy",
            "P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum",
            "v.z", "dvdtau.z", "Rv.z", "Rv0.z", "dRvdv.z", "PC_v.z", "tau.zz", "tau.xz", "tau.yz",
-           "tau_old.zz", "tau_old.xz", "tau_old.yz"]
+           "tau_old.zz", "tau_old.xz", "tau_old.yz", "plastic_history.gamma", "plastic_history.theta", "P_ip"]
 
 
 class StokesDR:
@@ -583,7 +660,8 @@ class StokesDR:
         for k, name in enumerate(_FIELDS):
             p, n, dt = VP(), I64(), I32()
             self.ctx.check(_lib.fem_stokes_field(h, k, C.byref(p), C.byref(n), C.byref(dt)))
-            shape = (mesh.nels, mesh.nq) if name.startswith("tau") and n.value else (n.value,)
+            ip = name.startswith(("tau", "plastic_history", "P_ip"))
+            shape = (mesh.nels, mesh.nq) if ip and n.value else (n.value,)
             arrs.append(DeviceArray(self.ctx, p.value, shape, np.float64, owner=self))
         a = dict(zip(_FIELDS, arrs))
         if mesh.dim == 3:
@@ -598,6 +676,10 @@ class StokesDR:
                 setattr(self, name, SymmetricTensor2D(*(a[f"{name}.{c}"] for c in ("xx", "yy", "xy"))))
         for name in ("P", "P0", "gammaP", "T", "T0", "Q", "Pf", "RP", "RP0", "M_P", "Pnum"):
             setattr(self, name, a[name])
+        # the tensile cap's integration-point history (zero: no softening) and
+        # the pressure of the last stress update (cap-corrected); 2-D states
+        self.plastic_history = PlasticHistory(a["plastic_history.gamma"], a["plastic_history.theta"])
+        self.P_ip = a["P_ip"]
 
     def __del__(self):
         try:
@@ -633,6 +715,10 @@ def _dp_ptr(plastic):
     return d, C.pointer(d)
 
 
+def _is_cap(plastic):
+    return isinstance(plastic, DruckerPragerCap)
+
+
 def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, dt, finite_K=False):
     """assemble_viscosity_weighted_pressure_scaling! (pressure_scaling.jl:8-149):
     fills dr.M_P and dr.gammaP."""
@@ -650,7 +736,7 @@ def solve(dr, mesh, *bcs, **kwargs):
 
 def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_factor=50.0, tolerance=1e-6,
                  max_iterations=50_000, check_interval=50, iterMax=50_000, max_ph_iterations=1000, rel_drop0=1e-2,
-                 plastic=None, phases_v=None, phases_P=None, finite_K=False, freeze_jacobian=None,
+                 plastic=None, phases_v=None, phases_P=None, finite_K=None, freeze_jacobian=None,
                  throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False, time_kernels=False,
                  precision="fp64"):
     """Checks and C arguments of one solve! call: (bc array, plastic pointer, params)."""
@@ -667,6 +753,8 @@ def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_facto
         raise DimensionMismatch(f"a {mesh.dim}-D solve takes {mesh.dim} boundary conditions, got {len(given)}")
     arr = (_BC * len(given))(*(b._c(keep) for b in given))
     keep.append(arr)
+    if finite_K is None:   # DR.jl:312: finite_K = plastic isa DruckerPragerCap
+        finite_K = _is_cap(plastic)
     d, dp = _dp_ptr(plastic)
     keep.append(d)
     p = _SolveParams(dt=float(dt), pressure_factor=float(pressure_factor), tolerance=float(tolerance),
@@ -707,7 +795,10 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
     keep = []
     bcs, dp, p = _stokes_call(dr, mesh, (bc_vx, bc_vy, bc_vz), keep, **kwargs)
     st = _SolveStats()
-    rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
+    if _is_cap(kwargs.get("plastic")):
+        rc = _lib.fem_stokes_solve_cap(dr.handle, bcs, C.cast(dp, _P(_Cap)), C.byref(p), C.byref(st))
+    else:
+        rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
     stats = _stats_dict(st)
     dr.ctx.check(rc)
     return stats
@@ -729,6 +820,8 @@ def solve_batch(problems, nthreads: int = 0, raise_errors: bool = True):
     stats = (_SolveStats * n)()
     status = (I32 * n)()
     for i, (dr, mesh, bcs, kw) in enumerate(problems):
+        if _is_cap(kw.get("plastic")):
+            raise ValueError("batched solves take DruckerPrager or no yield model, not the tensile cap")
         arr, dp, p = _stokes_call(dr, mesh, tuple(bcs) + (None,) * (3 - len(bcs)), keep, **kw)
         states[i] = dr.handle
         bcp[i] = C.cast(arr, _P(_BC))
@@ -758,20 +851,44 @@ def update_stokes_current_stress(dr: StokesDR, mesh: MixedMesh, dt, plastic=None
     """update_stokes_current_stress!(dr, mesh, τ, Δt; plastic) (DR.jl:820-897):
     writes dr.tau at the integration points."""
     d, dp = _dp_ptr(plastic)
-    dr.ctx.check(_lib.fem_stokes_update_current_stress(dr.handle, dp, float(dt)))
+    if _is_cap(plastic):
+        dr.ctx.check(_lib.fem_stokes_update_current_stress_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt)))
+    else:
+        dr.ctx.check(_lib.fem_stokes_update_current_stress(dr.handle, dp, float(dt)))
     return dr.tau
+
+
+def commit_stokes_plastic_history(dr: StokesDR, mesh: MixedMesh, dt, *, plastic):
+    """commit_stokes_plastic_history!(dr, mesh, Δt; plastic) (DR.jl:899-958):
+    γ += Δt γdot, θ += Δt θdot in dr.plastic_history from the tensile cap's
+    local update at the current dr.v, dr.P (a no-op for other yield models)."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    if not _is_cap(plastic):
+        return dr
+    d, dp = _dp_ptr(plastic)
+    dr.ctx.check(_lib.fem_stokes_commit_plastic_history(dr.handle, C.cast(dp, _P(_Cap)), float(dt)))
+    return dr
 
 
 def residual(dr: StokesDR, dt, plastic=None, with_pnum=False, finite_K=False):
     """Element residual sweep: dr.RP (and dr.Pnum) and dr.Rv at the current state."""
     d, dp = _dp_ptr(plastic)
-    dr.ctx.check(_lib.fem_stokes_residual(dr.handle, dp, float(dt), int(with_pnum), int(finite_K)))
+    if _is_cap(plastic):
+        dr.ctx.check(_lib.fem_stokes_residual_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt), int(with_pnum),
+                                                  int(finite_K)))
+    else:
+        dr.ctx.check(_lib.fem_stokes_residual(dr.handle, dp, float(dt), int(with_pnum), int(finite_K)))
 
 
 def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False):
     """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal)."""
     d, dp = _dp_ptr(plastic)
-    dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
+    if _is_cap(plastic):
+        dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt),
+                                                              int(finite_K)))
+    else:
+        dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
 
 
 def copyto(dst, src):
@@ -795,6 +912,8 @@ def rotate_stress(dr: StokesDR, mesh: MixedMesh, dt):
 def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
     """Average device time (ms) of the fused element kernel over n launches
     (HIP events on the context stream). Advances bubble-node velocities."""
+    if _is_cap(plastic):
+        raise ValueError("the element-kernel timer takes DruckerPrager or no yield model")
     d, dp = _dp_ptr(plastic)
     out = D()
     dr.ctx.check(_lib.fem_stokes_time_element_kernel(dr.handle, dp, float(dt), int(n), C.byref(out)))
@@ -903,6 +1022,8 @@ def assemble_shear_heating(dr: StokesDR, mesh: MixedMesh, dt, plastic=None) -> n
     current dr.v, dr.P and dr.tau_old, as an (nels, 7) array."""
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
+    if _is_cap(plastic):
+        raise ValueError("shear heating is assembled for DruckerPrager or no yield model")
     d, dp = _dp_ptr(plastic)
     buf = VP()
     dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, mesh.nels * 7 * 8, C.byref(buf)))
@@ -937,6 +1058,8 @@ def solve_coupled(thermal: ThermalDiffusionDR, stokes: StokesDR, thermal_mesh: M
         raise ValueError("thermal was built on a different mesh")
     if thermal.ctx is not stokes.ctx:
         raise ValueError("the thermal and Stokes states must share one Context")
+    if _is_cap(kwargs.get("plastic")):
+        raise ValueError("coupled solves take DruckerPrager or no yield model, not the tensile cap")
     keep = []
     bcs, dp, p = _stokes_call(stokes, stokes_mesh, (bc_vx, bc_vy, None), keep, dt=dt, **kwargs)
     bct = C.pointer(bc_T._c(keep)) if bc_T is not None else C.cast(None, _P(_BC))

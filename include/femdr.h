@@ -125,6 +125,23 @@ typedef struct {
     double Kb[FEM_MAX_PHASES];
 } fem_drucker_prager;
 
+/* DruckerPragerCap (stokes_types.jl:449-544): Drucker-Prager closed on the
+ * tensile side by the circular cap of Popov, Berlie & Kaus (2025). Stored as
+ * cos(ϕ), sin(ϕ), sin(Ψ), C, pT (<= 0, compression-positive pressure), η_reg,
+ * Kb (> 0, finite), C_min and H_C (<= 0: linear softening with γ). */
+typedef struct {
+    int32_t nphases;
+    double cos_phi[FEM_MAX_PHASES];
+    double sin_phi[FEM_MAX_PHASES];
+    double sin_psi[FEM_MAX_PHASES];
+    double C[FEM_MAX_PHASES];
+    double pT[FEM_MAX_PHASES];
+    double eta_reg[FEM_MAX_PHASES];
+    double Kb[FEM_MAX_PHASES];
+    double C_min[FEM_MAX_PHASES];
+    double H_C[FEM_MAX_PHASES];
+} fem_drucker_prager_cap;
+
 /* ------------------------------------------------------------------ state */
 /* StokesDR(mesh, material; CFL_v, CFL_P, c_fact, ϵ) (stokes_types.jl:244-298,
  * 359-369): all fields zero, phases 1, stress at integration points (nq x nels). */
@@ -157,6 +174,11 @@ enum {
     FEM_FIELD_V_Z, FEM_FIELD_DVDTAU_Z, FEM_FIELD_RV_Z, FEM_FIELD_RV0_Z, FEM_FIELD_DRVDV_Z, FEM_FIELD_PC_V_Z,
     FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XZ, FEM_FIELD_TAU_YZ,
     FEM_FIELD_TAU_OLD_ZZ, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ,
+    /* 2-D states (zero length in 3-D), at integration points: the tensile
+     * cap's plastic history dr.plastic_history.γ / .θ (stokes_types.jl:83-96,
+     * zero: no softening) and the pressure the momentum balance used at the
+     * last stress update (the fourth output of update_stokes_current_stress!) */
+    FEM_FIELD_GAMMA_PL, FEM_FIELD_THETA_PL, FEM_FIELD_P_IP,
     FEM_FIELD_COUNT
 };
 /* device pointer and length of one field; dtype: 0 = f64 */
@@ -333,6 +355,26 @@ int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t comput
  * ω = ½(∂vx/∂y − ∂vy/∂x) in 2-D; the rotation vector curl(v)Δt/2 in 3-D).
  * 2-D and 3-D states. */
 int fem_stokes_rotate_stress(fem_stokes* st, double dt);
+
+/* ------------------------------------------------------ tensile cap (2-D) */
+/* The entry points above with plastic::DruckerPragerCap (rheology.jl:510-787):
+ * the visco-elastic trial stress returned by the local Newton solve of the
+ * cap (Popov et al. 2025, Eq. 42), the momentum balance with the corrected
+ * pressure, the Jacobian diagnostics with the return's implicit derivative.
+ * The cohesion is softened by FEM_FIELD_GAMMA_PL. 2-D states, FP64
+ * (precision 0). solve!'s default finite_K for the cap is 1 (DR.jl:312). */
+int fem_stokes_solve_cap(fem_stokes* st, const fem_bc bcs[2], const fem_drucker_prager_cap* cap,
+                         const fem_solve_params* params, fem_solve_stats* stats);
+int fem_stokes_residual_cap(fem_stokes* st, const fem_drucker_prager_cap* cap, double dt, int32_t with_pnum,
+                            int32_t finite_K);
+int fem_stokes_jacobian_diagnostics_cap(fem_stokes* st, const fem_drucker_prager_cap* cap, double dt,
+                                        int32_t finite_K);
+/* τ at the points and the corrected pressure into FEM_FIELD_P_IP */
+int fem_stokes_update_current_stress_cap(fem_stokes* st, const fem_drucker_prager_cap* cap, double dt);
+/* commit_stokes_plastic_history! (DR.jl:899-958): γ += Δt γdot, θ += Δt θdot
+ * at every point from the local update at the current v, P (softened by the
+ * γ before the call); FEM_ENAN if a committed value is non-finite. */
+int fem_stokes_commit_plastic_history(fem_stokes* st, const fem_drucker_prager_cap* cap, double dt);
 
 /* ------------------------------------------------- coupled thermal--Stokes */
 /* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):

@@ -1012,7 +1012,7 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
     const int finite_K = p->finite_K;
     const int64_t ncheck = p->check_interval;
     const int64_t nv = s->nnodes, np = s->nnodesP;
-    const bool freeze = p->freeze_jacobian < 0 ? (s->dp == nullptr) : (p->freeze_jacobian != 0);
+    const bool freeze = p->freeze_jacobian < 0 ? (s->dp == nullptr && s->cap == nullptr) : (p->freeze_jacobian != 0);
     if (!p->skip_scaling) orc_pressure_scaling(s, p->pressure_factor, dt, finite_K);
 
     double* rate[2] = {s->rate_x, s->rate_y};
