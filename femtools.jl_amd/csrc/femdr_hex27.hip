@@ -1010,6 +1010,431 @@ int hex27_rotate_stress(fem_stokes* s, double dt) {
     return check_launch(s->ctx, "k_hex27_rotate");
 }
 
+// ===========================================================================
+// CellPressureStokesDR (stokes_types.jl:545-613, DR.jl:960-1057): the viscous
+// Hex27 system with the four cell-local pressure modes (1, ξ, η, ζ), relaxed
+// by an Arrow-Hurwicz pressure step (the constant mode's mass-weighted mean
+// removed) and a Jacobi velocity step. The state is a 3-D fem_stokes: P and
+// RP hold the modes per cell, PC_v the Jacobi diagonal, M_P the mode mass.
+// One iteration: k_cp3_pressure (RP, P += ps RP/M, partial sums of P_0 M_0),
+// the deterministic reduction of those sums, k_cp3_momentum (the mean
+// removed from P_0 as it is read, the residual, the centre node's step) and
+// k_gather3 (the shared nodes' step, v -= vs R/D).
+// ===========================================================================
+struct CP3Args {
+    HexGeo g;
+    double *vx, *vy, *vz;
+    double* P;                     // [nels][4]
+    double* RP;                    // [nels][4]
+    double* M;                     // [nels][4] mode mass (written by the preconditioner)
+    const int32_t* phase;          // cell phases, 0-based
+    const double* ptab;
+    double gx, gy, gz;
+    double ps, vs;                 // pressure_step, velocity_step
+    double* part;                  // [tiles]: Σ P_0 M_0 of each tile's cells (pressure pass)
+    const double* psum;            // nullable: Σ_cells P_0 M_0 (momentum pass); the mean is removed
+    double inv_sumM;               // 1 / Σ_cells M_0
+    double* slot;                  // [nels][26][3]
+    double *Dx, *Dy, *Dz;          // Jacobi diagonal
+    const uint8_t* bcflag;
+    const double *bcval_x, *bcval_y, *bcval_z;
+    double *Rv_x, *Rv_y, *Rv_z;    // nullable: the residual of the centre nodes
+    int32_t update;                // pressure pass: P += ps RP/M; momentum pass: the centre node's step
+};
+
+// the modes (1, ξ, η, ζ) at a point (pressure_mode_table, momentum_residuals.jl:618-621)
+__device__ inline void point_modes_cp(int q, double NP[HNP], double& dOm, double det) {
+    const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+    NP[0] = 1; NP[1] = gauss_x(qi); NP[2] = gauss_x(qj); NP[3] = gauss_x(qk);
+    dOm = det * ((gauss_w(qi) * gauss_w(qj)) * gauss_w(qk));
+}
+
+// stokes_preconditioner_3d_kernel! (momentum_residuals.jl:724-745): lane =
+// node t, the Jacobi diagonal η (|∇N_t|² + (∂N_t/∂x_c)²/3) dΩ over the 27
+// points into the element slot (the centre node directly); lanes 0-3 the
+// mode mass Σ_q mode² dΩ
+__global__ __launch_bounds__(HBLOCK) void k_cp3_precond(CP3Args A) {
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    if (!h.active) return;
+    const int64_t e = h.e;
+    const double eta = A.ptab[PC_ETA * MAXPH + A.phase[e]];
+    const int t = h.l;
+    const Lane1D b = node_1d(t);
+    double D[3] = {0, 0, 0};
+    for (int p2 = 0; p2 < 3; ++p2)
+        for (int p1 = 0; p1 < 3; ++p1)
+#pragma unroll
+            for (int p0 = 0; p0 < 3; ++p0) {
+                const double g0 = b.d0[p0] * b.L1[p1] * b.L2[p2] * g.jac[0];
+                const double g1 = b.L0[p0] * b.d1[p1] * b.L2[p2] * g.jac[1];
+                const double g2 = b.L0[p0] * b.L1[p1] * b.d2[p2] * g.jac[2];
+                const double dOm = g.det * ((gauss_w(p0) * gauss_w(p1)) * gauss_w(p2));
+                const double n2 = g0 * g0 + g1 * g1 + g2 * g2;
+                D[0] += eta * (n2 + g0 * g0 / 3) * dOm;
+                D[1] += eta * (n2 + g1 * g1 / 3) * dOm;
+                D[2] += eta * (n2 + g2 * g2 / 3) * dOm;
+            }
+    if (t != HCENTRE_T) {
+        double* sl = A.slot + ((e * HNSH + kHexA[t]) * 3);
+        sl[0] = D[0]; sl[1] = D[1]; sl[2] = D[2];
+    } else {
+        const int64_t node = hex_node(g, e, t);
+        A.Dx[node] = D[0]; A.Dy[node] = D[1]; A.Dz[node] = D[2];
+    }
+    if (t < HNP) {
+        double m = 0;
+        for (int q = 0; q < HNQ; ++q) {
+            double NP[HNP], dOm;
+            point_modes_cp(q, NP, dOm, g.det);
+            m += NP[t] * NP[t] * dOm;
+        }
+        A.M[e * HNP + t] = m;
+    }
+}
+
+// stokes_pressure_residual_3d_kernel! (pressure_residual.jl:337-366) and the
+// Arrow-Hurwicz step of DR.jl:1025-1029: RP = −Σ_q modes (∇·v) dΩ per cell
+// (a lane-group sum over the points), P += ps RP / M, and the tile's Σ P_0 M_0
+__global__ __launch_bounds__(HBLOCK) void k_cp3_pressure(CP3Args A) {
+    __shared__ double s_v[HEPB][HNV * 3];
+    __shared__ double s_part[HEPB];
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    const int64_t e = h.valid ? h.e : g.nels - 1;
+    if (h.l < HNV) {
+        const int64_t node = hex_node(g, e, h.l);
+        s_v[h.es][h.l * 3 + 0] = A.vx[node];
+        s_v[h.es][h.l * 3 + 1] = A.vy[node];
+        s_v[h.es][h.l * 3 + 2] = A.vz[node];
+    }
+    __syncthreads();
+    const int q = h.l < HNQ ? h.l : 0;
+    const Lane1D b = point_1d(q);
+    double gv[3][3];
+    point_gradient(g, b, q, s_v[h.es], gv);
+    double NP[HNP], dOm;
+    point_modes_cp(q, NP, dOm, g.det);
+    const double div = gv[0][0] + gv[1][1] + gv[2][2];
+    double RP[HNP];
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) RP[i] = group_sum(h.l < HNQ ? -(NP[i] * (div * dOm)) : 0.0);
+    if (h.l == 0) s_part[h.es] = 0;
+    if (h.valid && h.l < HNP) {
+        const int64_t k = e * HNP + h.l;
+        A.RP[k] = RP[h.l];
+        if (A.update) {
+            const double Pn = A.P[k] + A.ps * RP[h.l] / A.M[k];
+            A.P[k] = Pn;
+            if (h.l == 0) s_part[h.es] = Pn * A.M[k];
+        }
+    }
+    __syncthreads();
+    if (A.update && threadIdx.x == 0) {
+        double acc = 0;
+#pragma unroll
+        for (int i = 0; i < HEPB; ++i) acc += s_part[i];
+        A.part[xcd_tile()] = acc;
+    }
+}
+
+// stokes_momentum_residual_3d_kernel! (momentum_residuals.jl:645-691): the
+// viscous residual ∫ ∇N·(2η ε̇′ − P I) − N ρ0 g with P from the modes (the
+// constant mode's mean removed, and written back, when psum is set); the
+// shared nodes' residuals go to the element slots, the centre node takes its
+// Jacobi step v -= vs R / D here (DR.jl:1031-1038)
+__global__ __launch_bounds__(HBLOCK) void k_cp3_momentum(CP3Args A) {
+    __shared__ double s_v[HEPB][HNV * 3];
+    __shared__ double s_f[HEPB][HNQ * 8];
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    const int64_t e = h.valid ? h.e : g.nels - 1;
+    int64_t node = 0;
+    if (h.l < HNV) {
+        node = hex_node(g, e, h.l);
+        s_v[h.es][h.l * 3 + 0] = A.vx[node];
+        s_v[h.es][h.l * 3 + 1] = A.vy[node];
+        s_v[h.es][h.l * 3 + 2] = A.vz[node];
+    }
+    double P[HNP];
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) P[i] = A.P[e * HNP + i];
+    if (A.psum) P[0] -= A.psum[0] * A.inv_sumM;
+    __syncthreads();
+    if (A.psum && h.valid && h.l == 0) A.P[e * HNP] = P[0];
+    const int pc = A.phase[e];
+    const double eta = A.ptab[PC_ETA * MAXPH + pc], rho0 = A.ptab[PC_RHO0 * MAXPH + pc];
+    {
+        const int q = h.l < HNQ ? h.l : 0;
+        const Lane1D b = point_1d(q);
+        double gv[3][3];
+        point_gradient(g, b, q, s_v[h.es], gv);
+        double NP[HNP], dOm;
+        point_modes_cp(q, NP, dOm, g.det);
+        const double tr = (gv[0][0] + gv[1][1] + gv[2][2]) / 3.0;
+        double Pq = 0;
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) Pq += NP[i] * P[i];
+        const double two = 2 * eta;
+        if (h.l < HNQ) {
+            double* f = s_f[h.es] + q * 8;
+            f[0] = (two * (gv[0][0] - tr) - Pq) * dOm;
+            f[1] = (two * (gv[1][1] - tr) - Pq) * dOm;
+            f[2] = (two * (gv[2][2] - tr) - Pq) * dOm;
+            f[3] = eta * (gv[0][1] + gv[1][0]) * dOm;
+            f[4] = eta * (gv[0][2] + gv[2][0]) * dOm;
+            f[5] = eta * (gv[1][2] + gv[2][1]) * dOm;
+            f[6] = rho0 * dOm;
+        }
+    }
+    __syncthreads();
+    if (!h.active) return;
+    const int t = h.l;
+    const Lane1D b = node_1d(t);
+    double R[3];
+    node_residual(g, b, t, s_f[h.es], A.gx, A.gy, A.gz, R);
+    if (t != HCENTRE_T) {
+        double* sl = A.slot + ((e * HNSH + kHexA[t]) * 3);
+        sl[0] = R[0]; sl[1] = R[1]; sl[2] = R[2];
+        return;
+    }
+    const uint8_t fl = A.bcflag[node];
+    const double rx = (fl & 1) ? 0.0 : R[0], ry = (fl & 2) ? 0.0 : R[1], rz = (fl & 4) ? 0.0 : R[2];
+    if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
+    if (A.update) {
+        double nvx = s_v[h.es][t * 3 + 0] - A.vs * rx / A.Dx[node];
+        double nvy = s_v[h.es][t * 3 + 1] - A.vs * ry / A.Dy[node];
+        double nvz = s_v[h.es][t * 3 + 2] - A.vs * rz / A.Dz[node];
+        if (fl & 1) nvx = A.bcval_x[node];
+        if (fl & 2) nvy = A.bcval_y[node];
+        if (fl & 4) nvz = A.bcval_z[node];
+        A.vx[node] = nvx; A.vy[node] = nvy; A.vz[node] = nvz;
+    }
+}
+
+// Σ RP², Σ Rx², Σ Ry², Σ Rz² (the norms of the check, DR.jl:1040-1042)
+__global__ __launch_bounds__(RED_BLOCK) void k_cp3_norms(int64_t nv, int64_t np, const double* Rx, const double* Ry,
+                                                         const double* Rz, const double* RP, double* partial) {
+    double acc[4] = {0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < np; i += stride) acc[0] += RP[i] * RP[i];
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < nv; i += stride) {
+        acc[1] += Rx[i] * Rx[i];
+        acc[2] += Ry[i] * Ry[i];
+        acc[3] += Rz[i] * Rz[i];
+    }
+    block_sum_store<4>(acc, partial);
+}
+
+__global__ void k_pin3(int64_t n, const uint8_t* flag, const double* bx, const double* by, const double* bz,
+                       double* vx, double* vy, double* vz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        if (flag[i] & 1) vx[i] = bx[i];
+        if (flag[i] & 2) vy[i] = by[i];
+        if (flag[i] & 4) vz[i] = bz[i];
+    }
+}
+
+namespace {
+
+CP3Args cp3_args(fem_stokes* s) {
+    CP3Args a;
+    std::memset(&a, 0, sizeof a);
+    double** f = s->f;
+    a.g = hex_geo(s->mesh);
+    a.vx = f[FEM_FIELD_V_X]; a.vy = f[FEM_FIELD_V_Y]; a.vz = f[FEM_FIELD_V_Z];
+    a.P = f[FEM_FIELD_P];
+    a.RP = f[FEM_FIELD_RP];
+    a.M = f[FEM_FIELD_M_P];
+    a.phase = s->d_phase_v;
+    a.ptab = s->d_ptab;
+    a.gx = s->mat.g[0]; a.gy = s->mat.g[1]; a.gz = s->mat.g[2];
+    a.slot = s->d_slot3;
+    a.Dx = f[FEM_FIELD_PC_V_X]; a.Dy = f[FEM_FIELD_PC_V_Y]; a.Dz = f[FEM_FIELD_PC_V_Z];
+    a.bcflag = s->d_bcflag;
+    a.bcval_x = s->d_bcval_x; a.bcval_y = s->d_bcval_y; a.bcval_z = s->d_bcval_z;
+    return a;
+}
+
+// the state must be a purely viscous, incompressible 3-D one
+// (CellPressureStokesDR, stokes_types.jl:581-613); the phase table carries η, ρ0
+int cp3_prepare(fem_stokes* s) {
+    fem_ctx* c = s->ctx;
+    if (s->dim != 3 || s->mesh->nloc != 27)
+        return set_err(c, FEM_EARG, "CellPressureStokesDR requires a Hex27 mesh");
+    for (int p = 0; p < s->mat.nphases; ++p)
+        if (!(std::isinf(s->mat.G[p]) && std::isinf(s->mat.K[p]) && s->mat.alpha[p] == 0))
+            return set_err(c, FEM_EARG,
+                           "CellPressureStokesDR is purely viscous and incompressible; G and K must be Inf and α zero");
+    bool bulk_inf, thermal, eos_p, elastic;
+    return upload_ptab(s, nullptr, 1.0, 0, &bulk_inf, &thermal, &eos_p, &elastic);
+}
+
+// the shared nodes' sums of the element slots: into the diagonal
+// (preconditioner) or, with the Jacobi step, the velocity
+int cp3_gather(fem_stokes* s, double* const out[3], bool step, double vs, bool store_R) {
+    fem_mesh* m = s->mesh;
+    double** f = s->f;
+    Gather3Args g;
+    std::memset(&g, 0, sizeof g);
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.slot = s->d_slot3;
+    g.bcflag = s->d_bcflag;
+    if (step) {
+        g.vx = f[FEM_FIELD_V_X]; g.vy = f[FEM_FIELD_V_Y]; g.vz = f[FEM_FIELD_V_Z];
+        g.rate_x = f[FEM_FIELD_DVDTAU_X]; g.rate_y = f[FEM_FIELD_DVDTAU_Y]; g.rate_z = f[FEM_FIELD_DVDTAU_Z];
+        g.PCx = f[FEM_FIELD_PC_V_X]; g.PCy = f[FEM_FIELD_PC_V_Y]; g.PCz = f[FEM_FIELD_PC_V_Z];
+        g.bcval_x = s->d_bcval_x; g.bcval_y = s->d_bcval_y; g.bcval_z = s->d_bcval_z;
+        g.ax = g.ay = g.az = vs;
+        g.bx = g.by = g.bz = 0;
+        g.do_update = 1;
+    }
+    if (store_R) { g.Rv_x = out[0]; g.Rv_y = out[1]; g.Rv_z = out[2]; }
+    hipLaunchKernelGGL(k_gather3, dim3(grid_xcd(m->ns, 256)), dim3(256), 0, s->ctx->stream, g);
+    return check_launch(s->ctx, "k_gather3");
+}
+
+int cp3_preconditioner(fem_stokes* s) {
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    CP3Args a = cp3_args(s);
+    hipLaunchKernelGGL(k_cp3_precond, dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0, c->stream, a);
+    CHK(check_launch(c, "k_cp3_precond"));
+    double* D[3] = {s->f[FEM_FIELD_PC_V_X], s->f[FEM_FIELD_PC_V_Y], s->f[FEM_FIELD_PC_V_Z]};
+    return cp3_gather(s, D, false, 0, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fem_cell_stokes_preconditioner(fem_stokes* s) {
+    if (!s) return FEM_EARG;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    CHK(cp3_prepare(s));
+    CHK(cp3_preconditioner(s));
+    return fem_sync(s->ctx);
+}
+
+int fem_cell_stokes_pressure_residual(fem_stokes* s) {
+    if (!s) return FEM_EARG;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    CHK(cp3_prepare(s));
+    CP3Args a = cp3_args(s);
+    hipLaunchKernelGGL(k_cp3_pressure, dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0, s->ctx->stream, a);
+    CHK(check_launch(s->ctx, "k_cp3_pressure"));
+    return fem_sync(s->ctx);
+}
+
+int fem_cell_stokes_momentum_residual(fem_stokes* s) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    CHK(cp3_prepare(s));
+    HIPCHK(c, hipMemsetAsync(s->d_bcflag, 0, s->mesh->nnodes, c->stream));
+    CP3Args a = cp3_args(s);
+    double* R[3] = {s->f[FEM_FIELD_RV_X], s->f[FEM_FIELD_RV_Y], s->f[FEM_FIELD_RV_Z]};
+    a.Rv_x = R[0]; a.Rv_y = R[1]; a.Rv_z = R[2];
+    hipLaunchKernelGGL(k_cp3_momentum, dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0, c->stream, a);
+    CHK(check_launch(c, "k_cp3_momentum"));
+    CHK(cp3_gather(s, R, false, 0, true));
+    return fem_sync(c);
+}
+
+// solve!(dr::CellPressureStokesDR, mesh, bc_v; ...) (DR.jl:980-1057)
+int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_solve_params* p,
+                          fem_cell_solve_stats* st, double* history, int64_t history_cap, int64_t* history_len) {
+    if (!s || !bcs || !p || !st) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    std::memset(st, 0, sizeof *st);
+    if (history_len) *history_len = 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
+    const auto t0 = std::chrono::steady_clock::now();
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, ne = m->nels, np = HNP * ne;
+    CHK(cp3_prepare(s));
+    CHK(cp3_preconditioner(s));
+    CHK(upload_bcs(s, bcs));
+    double** f = s->f;
+    hipLaunchKernelGGL(k_pin3, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
+                       (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, (const double*)s->d_bcval_z,
+                       f[FEM_FIELD_V_X], f[FEM_FIELD_V_Y], f[FEM_FIELD_V_Z]);
+    CHK(check_launch(c, "k_pin3"));
+    // Σ_cells M_0 in cell order on the host (a constant of the solve)
+    std::vector<double> hM(np);
+    HIPCHK(c, hipMemcpyAsync(hM.data(), f[FEM_FIELD_M_P], np * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double sumM = 0;
+    for (int64_t e = 0; e < ne; ++e) sumM += hM[e * HNP];
+    const unsigned nb = grid_xcd(ne, HEPB);
+    double* part = nullptr;
+    CHK(dalloc_t(c, &part, nb + 1));
+    CP3Args a = cp3_args(s);
+    a.ps = p->pressure_step;
+    a.vs = p->velocity_step;
+    a.part = part;
+    a.inv_sumM = 1.0 / sumM;
+    double* R[3] = {f[FEM_FIELD_RV_X], f[FEM_FIELD_RV_Y], f[FEM_FIELD_RV_Z]};
+    double err_v = INFINITY, err_P = INFINITY, err = INFINITY;
+    int64_t iter = 0, nh = 0;
+    int rc = FEM_OK;
+    for (int64_t it = 1; it <= p->max_iterations && rc == FEM_OK; ++it) {
+        iter = it;
+        const bool chk = it % p->check_interval == 0 || it == p->max_iterations;
+        a.update = 1;
+        a.psum = nullptr;
+        a.Rv_x = a.Rv_y = a.Rv_z = nullptr;
+        hipLaunchKernelGGL(k_cp3_pressure, dim3(nb), dim3(HBLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)nb, 1, 0, (const double*)part,
+                           part + nb);
+        a.psum = part + nb;
+        if (chk) { a.Rv_x = R[0]; a.Rv_y = R[1]; a.Rv_z = R[2]; }
+        hipLaunchKernelGGL(k_cp3_momentum, dim3(nb), dim3(HBLOCK), 0, c->stream, a);
+        rc = check_launch(c, "k_cp3_momentum");
+        if (rc == FEM_OK) rc = cp3_gather(s, R, true, p->velocity_step, chk);
+        if (rc != FEM_OK || !chk) continue;
+        const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, np), RED_BLOCK));
+        hipLaunchKernelGGL(k_cp3_norms, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np, (const double*)R[0],
+                           (const double*)R[1], (const double*)R[2], (const double*)f[FEM_FIELD_RP], c->red_partial);
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 4, 0,
+                           (const double*)c->red_partial, c->red_out);
+        rc = check_launch(c, "k_cp3_norms");
+        if (rc != FEM_OK) break;
+        HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        err_P = std::sqrt(c->h_red[0]);
+        err_v = std::max(std::sqrt(c->h_red[1]), std::max(std::sqrt(c->h_red[2]), std::sqrt(c->h_red[3])));
+        err = std::max(err_v, err_P);
+        if (!std::isfinite(err)) { rc = set_err(c, FEM_ENAN, "non-finite residual in 3-D Stokes solve"); break; }
+        if (history && nh < history_cap) {
+            history[nh * 3] = (double)it; history[nh * 3 + 1] = err_v; history[nh * 3 + 2] = err_P;
+            ++nh;
+        }
+        if (p->verbose) printf("iter = %06lld err = %.3e - norm[Rv=%.3e, Rp=%.3e]\n", (long long)it, err, err_v, err_P);
+        if (err < p->tolerance) break;
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(part);
+    if (history_len) *history_len = nh;
+    st->converged = err < p->tolerance;
+    st->iterations = iter;
+    st->residual = err;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->reached_total_iter = iter >= p->max_iterations && !st->converged;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != FEM_OK) return rc;
+    if (!st->converged && p->throw_on_failure)
+        return set_err(c, FEM_ENOCONV, "3-D Stokes solve did not converge after %lld iterations (residual = %g, "
+                       "tolerance = %g)", (long long)iter, err, p->tolerance);
+    return FEM_OK;
+}
+
+}  // extern "C"
+
 int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt) {
     Opts3 o;
     CHK(prepare3(s, dp, dt, 0, o, true));

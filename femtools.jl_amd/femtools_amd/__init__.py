@@ -30,7 +30,8 @@ __all__ = [
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap",
-    "commit_stokes_plastic_history", "PlasticHistory",
+    "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
+    "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -130,6 +131,16 @@ class _ThermalStats(C.Structure):
     _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("seconds", D)]
 
 
+class _CellParams(C.Structure):
+    _fields_ = [("tolerance", D), ("max_iterations", I64), ("check_interval", I64), ("velocity_step", D),
+                ("pressure_step", D), ("verbose", I32), ("throw_on_failure", I32)]
+
+
+class _CellStats(C.Structure):
+    _fields_ = [("converged", I32), ("reached_total_iter", I32), ("iterations", I64), ("residual", D),
+                ("err_v", D), ("err_P", D), ("seconds", D)]
+
+
 class _CoupledParams(C.Structure):
     _fields_ = [("Tref", D), ("shear_heating", I32)]
 
@@ -186,6 +197,10 @@ _sig("fem_thermal_residual", C.c_int, VP, D, D, I32)
 _sig("fem_stokes_shear_heating", C.c_int, VP, _P(_DP), D, VP)
 _sig("fem_stokes_rotate_stress", C.c_int, VP, D)
 _sig("fem_stokes_solve_cap", C.c_int, VP, _P(_BC), _P(_Cap), _P(_SolveParams), _P(_SolveStats))
+_sig("fem_cell_stokes_preconditioner", C.c_int, VP)
+_sig("fem_cell_stokes_pressure_residual", C.c_int, VP)
+_sig("fem_cell_stokes_momentum_residual", C.c_int, VP)
+_sig("fem_cell_stokes_solve", C.c_int, VP, _P(_BC), _P(_CellParams), _P(_CellStats), _P(D), I64, _P(I64))
 _sig("fem_stokes_residual_cap", C.c_int, VP, _P(_Cap), D, I32, I32)
 _sig("fem_stokes_jacobian_diagnostics_cap", C.c_int, VP, _P(_Cap), D, I32)
 _sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
@@ -205,7 +220,8 @@ EXPORTED = [
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
     "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
     "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
-    "fem_stokes_commit_plastic_history",
+    "fem_stokes_commit_plastic_history", "fem_cell_stokes_preconditioner", "fem_cell_stokes_pressure_residual",
+    "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve",
 ]
 
 
@@ -728,9 +744,12 @@ def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, 
 def solve(dr, mesh, *bcs, **kwargs):
     """solve!(dr, mesh, bc...; kwargs...): dispatches on the state type like
     the reference's solve! methods: StokesDR -> solve_stokes (DR.jl:302-337),
-    ThermalDiffusionDR -> solve_thermal (heat_diffusion/solvers/DR.jl:50-54)."""
+    ThermalDiffusionDR -> solve_thermal (heat_diffusion/solvers/DR.jl:50-54),
+    CellPressureStokesDR -> solve_cell_pressure (DR.jl:980-996)."""
     if isinstance(dr, ThermalDiffusionDR):
         return solve_thermal(dr, mesh, *bcs, **kwargs)
+    if isinstance(dr, CellPressureStokesDR):
+        return solve_cell_pressure(dr, mesh, *bcs, **kwargs)
     return solve_stokes(dr, mesh, *bcs, **kwargs)
 
 
@@ -1073,3 +1092,92 @@ def solve_coupled(thermal: ThermalDiffusionDR, stokes: StokesDR, thermal_mesh: M
     stats["thermal_converged"] = bool(cst.thermal_converged)
     stokes.ctx.check(rc)
     return stats
+
+
+# ---------------------------------------------------------------------------
+# CellPressureStokesDR: viscous Hex27 with cell pressure modes (1, ξ, η, ζ)
+# ---------------------------------------------------------------------------
+class CellPressureStokesDR:
+    """CellPressureStokesDR(mesh, material; phases=None) (stokes_types.jl:545-613):
+    the viscous, incompressible 3-D Hex27 state with four cell-local pressure
+    modes (1, ξ, η, ζ). Device fields: v, Rv, diagonal (VectorField3D), P, RP,
+    pressure_mass ((nels, 4)); phases one per cell (1-based)."""
+
+    def __init__(self, mesh: MixedMesh, material: StokesMaterial, phases=None):
+        if mesh.dim != 3 or mesh.nv_local != 27:
+            raise ValueError("CellPressureStokesDR requires a Hex27 mesh")
+        if len(material.g) != 3:
+            raise DimensionMismatch("material gravity must have 3 components")
+        if not (all(math.isinf(x) for x in material.G) and all(math.isinf(x) for x in material.K)
+                and all(x == 0 for x in material.alpha)):
+            raise ValueError("CellPressureStokesDR is purely viscous and incompressible; G and K must be Inf and "
+                             "α zero")
+        n = len(material.eta)
+        if phases is not None:
+            ph = np.asarray(phases).astype(np.int64).ravel()
+            if len(ph) != mesh.nels:
+                raise DimensionMismatch(f"phases has {len(ph)} entries but the mesh has {mesh.nels} cells")
+            lo, hi = int(ph.min()), int(ph.max())
+            if not (1 <= lo and hi <= n):
+                raise ValueError(f"phases must lie in 1:{n}, got {lo}:{hi}")
+        self.mesh, self.material, self.ctx = mesh, material, mesh.ctx
+        self._st = StokesDR(mesh, material)
+        if phases is not None:
+            self._st.set_phases("v", ph.astype(np.int32))
+        self.handle = self._st.handle
+        st = self._st
+        self.v, self.Rv, self.diagonal = st.v, st.Rv, st.PC_v
+        self.P = DeviceArray(self.ctx, st.P.ptr, (mesh.nels, 4), owner=st)
+        self.RP = DeviceArray(self.ctx, st.RP.ptr, (mesh.nels, 4), owner=st)
+        self.pressure_mass = DeviceArray(self.ctx, st.M_P.ptr, (mesh.nels, 4), owner=st)
+
+
+def stokes_preconditioner_3d(dr: CellPressureStokesDR):
+    """stokes_preconditioner_3d! (momentum_residuals.jl:715-745): fills
+    dr.diagonal and dr.pressure_mass."""
+    dr.ctx.check(_lib.fem_cell_stokes_preconditioner(dr.handle))
+    return dr.diagonal, dr.pressure_mass
+
+
+def assemble_stokes_pressure_residual_3d(dr: CellPressureStokesDR):
+    """assemble_stokes_pressure_residual_3d! (pressure_residual.jl:337-366): dr.RP from dr.v."""
+    dr.ctx.check(_lib.fem_cell_stokes_pressure_residual(dr.handle))
+    return dr.RP
+
+
+def assemble_stokes_momentum_residual_3d(dr: CellPressureStokesDR):
+    """assemble_stokes_momentum_residual_3d! (momentum_residuals.jl:645-691): dr.Rv from dr.v, dr.P."""
+    dr.ctx.check(_lib.fem_cell_stokes_momentum_residual(dr.handle))
+    return dr.Rv
+
+
+def solve_cell_pressure(dr: CellPressureStokesDR, mesh: MixedMesh, bc_vx, bc_vy, bc_vz, *, tolerance=1e-5,
+                        max_iterations=3000, check_interval=100, velocity_step=0.6, pressure_step=0.2, verbose=False,
+                        collect_history=False, throw_on_failure=True):
+    """solve!(dr::CellPressureStokesDR, mesh, bc_v; tolerance=1e-5,
+    max_iterations=3000, check_interval=100, velocity_step=0.6,
+    pressure_step=0.2, verbose, collect_history, throw_on_failure)
+    (DR.jl:980-1057). Returns (converged, iterations, residual, history,
+    err_v, err_P, iter, err, reached_total_iter) as a dict."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    keep = []
+    arr = (_BC * 3)(*(b._c(keep) for b in (bc_vx, bc_vy, bc_vz)))
+    p = _CellParams(tolerance=float(tolerance), max_iterations=int(max_iterations),
+                    check_interval=int(check_interval), velocity_step=float(velocity_step),
+                    pressure_step=float(pressure_step), verbose=int(bool(verbose)),
+                    throw_on_failure=int(bool(throw_on_failure)))
+    st = _CellStats()
+    cap = (int(max_iterations) // max(int(check_interval), 1) + 2) if collect_history else 0
+    hist = np.zeros((max(cap, 1), 3))
+    nh = I64(0)
+    rc = _lib.fem_cell_stokes_solve(dr.handle, arr, C.byref(p), C.byref(st),
+                                    hist.ctypes.data_as(_P(D)) if cap else C.cast(None, _P(D)), cap, C.byref(nh))
+    out = {k: getattr(st, k) for k, _ in _CellStats._fields_}
+    out["converged"] = bool(out["converged"])
+    out["reached_total_iter"] = bool(out["reached_total_iter"])
+    out["iter"] = out["iterations"]
+    out["err"] = out["residual"]
+    out["history"] = [dict(iter=int(h[0]), err_v=h[1], err_P=h[2]) for h in hist[:nh.value]]
+    dr.ctx.check(rc)
+    return out

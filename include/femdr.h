@@ -376,6 +376,45 @@ int fem_stokes_update_current_stress_cap(fem_stokes* st, const fem_drucker_prage
  * γ before the call); FEM_ENAN if a committed value is non-finite. */
 int fem_stokes_commit_plastic_history(fem_stokes* st, const fem_drucker_prager_cap* cap, double dt);
 
+/* ------------------------------------------- CellPressureStokesDR (Hex27) */
+/* CellPressureStokesDR(mesh, material; phases) (stokes_types.jl:545-613) is a
+ * 3-D fem_stokes on a Hex27 mesh with a purely viscous, incompressible
+ * material (G, K infinite, α zero; cell phases): dr.v is the velocity, dr.P
+ * and dr.RP hold the cell modes (1, ξ, η, ζ) [nels][4], dr.PC_v the Jacobi
+ * diagonal and dr.M_P the mode mass. */
+/* stokes_preconditioner_3d! (momentum_residuals.jl:715-745): PC_v, M_P */
+int fem_cell_stokes_preconditioner(fem_stokes* st);
+/* assemble_stokes_pressure_residual_3d! (pressure_residual.jl:337-366): RP from v */
+int fem_cell_stokes_pressure_residual(fem_stokes* st);
+/* assemble_stokes_momentum_residual_3d! (momentum_residuals.jl:645-691): Rv from v, P */
+int fem_cell_stokes_momentum_residual(fem_stokes* st);
+
+/* solve!(dr::CellPressureStokesDR, mesh, bc_v; ...) keywords (DR.jl:960-996) */
+typedef struct {
+    double tolerance;           /* 1e-5  */
+    int64_t max_iterations;     /* 3000  */
+    int64_t check_interval;     /* 100   */
+    double velocity_step;       /* 0.6   */
+    double pressure_step;       /* 0.2   */
+    int32_t verbose;
+    int32_t throw_on_failure;   /* 1: non-convergence returns FEM_ENOCONV */
+} fem_cell_solve_params;
+
+/* (; converged, iterations, residual, err_v, err_P, reached_total_iter) + wall time */
+typedef struct {
+    int32_t converged, reached_total_iter;
+    int64_t iterations;
+    double residual, err_v, err_P;
+    double seconds;
+} fem_cell_solve_stats;
+
+/* the Arrow-Hurwicz / Jacobi relaxation of _relax_cell_pressure_stokes!
+ * (DR.jl:997-1057) from the state's v and P; bcs[3] one Dirichlet set per
+ * component. history (nullable, history_cap rows of (iter, err_v, err_P))
+ * records every check (collect_history); *history_len receives the count. */
+int fem_cell_stokes_solve(fem_stokes* st, const fem_bc bcs[3], const fem_cell_solve_params* params,
+                          fem_cell_solve_stats* stats, double* history, int64_t history_cap, int64_t* history_len);
+
 /* ------------------------------------------------- coupled thermal--Stokes */
 /* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
  * phi (device, nq x nels = [nels][7] f64) receives τ : (ε̇ − (τ − τ_old)/(2GΔt))

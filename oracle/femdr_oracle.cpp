@@ -2179,3 +2179,171 @@ int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// CellPressureStokesDR: the viscous 3-D Hex27 solver with the four cell-local
+// pressure modes (1, ξ, η, ζ) (stokes_types.jl:545-613, DR.jl:960-1057)
+// ===========================================================================
+namespace {
+
+// pressure_mode_table (momentum_residuals.jl:618-621): the modes at the points
+struct Modes3 {
+    double m[27][4];
+    Modes3() {
+        double xi[27], eta[27], zeta[27], w[27];
+        hex27_rule(xi, eta, zeta, w);
+        for (int q = 0; q < 27; ++q) { m[q][0] = 1; m[q][1] = xi[q]; m[q][2] = eta[q]; m[q][3] = zeta[q]; }
+    }
+};
+const Modes3& modes3() { static Modes3 t; return t; }
+
+}  // namespace
+
+extern "C" {
+
+// stokes_pressure_residual_3d_kernel! (pressure_residual.jl:337-366):
+// RP[4e + m] = −Σ_q mode_m(q) (∇·v)_q dΩ_q
+void orc_cp3_pressure_residual(const orc_stokes3* s, double* RP) {
+    const Modes3& md = modes3();
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        double r[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 27; ++q) {
+            double div = 0;
+            for (int a = 0; a < 27; ++a) {
+                const int32_t g = s->el2n[e * 27 + a];
+                div += dNdx[q][a][0] * s->v[0][g] + dNdx[q][a][1] * s->v[1][g] + dNdx[q][a][2] * s->v[2][g];
+            }
+            const double dOm = s->dOmega[e * 27 + q];
+            for (int m = 0; m < 4; ++m) r[m] -= md.m[q][m] * (div * dOm);
+        }
+        for (int m = 0; m < 4; ++m) RP[e * 4 + m] = r[m];
+    }
+}
+
+// stokes_momentum_residual_3d_kernel! (momentum_residuals.jl:645-691): the
+// viscous residual ∫ ∇N·(2η ε̇′ − P I) − N ρ0 g with P from the modes; the
+// element contributions added in ascending element order
+void orc_cp3_momentum_residual(const orc_stokes3* s, double* const* R) {
+    const Tables3& t = tables3();
+    const Modes3& md = modes3();
+    std::vector<double> Re((size_t)s->nels * 27 * 3);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        static thread_local double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        const int p = s->phases[e] - 1;
+        const double eta = s->mat.eta[p], rho = s->mat.rho0[p];
+        double* re = &Re[(size_t)e * 81];
+        for (int i = 0; i < 81; ++i) re[i] = 0;
+        for (int q = 0; q < 27; ++q) {
+            double G[3][3] = {};
+            for (int a = 0; a < 27; ++a) {
+                const int32_t g = s->el2n[e * 27 + a];
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < 3; ++k) G[c][k] += dNdx[q][a][k] * s->v[c][g];
+            }
+            const double tr = (G[0][0] + G[1][1] + G[2][2]) / 3.0;
+            double Pq = 0;
+            for (int m = 0; m < 4; ++m) Pq += md.m[q][m] * s->P[e * 4 + m];
+            double sig[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) sig[i][j] = eta * (G[i][j] + G[j][i]) - (i == j ? 2 * eta * tr + Pq : 0.0);
+            const double dOm = s->dOmega[e * 27 + q];
+            for (int a = 0; a < 27; ++a)
+                for (int i = 0; i < 3; ++i) {
+                    double acc = 0;
+                    for (int j = 0; j < 3; ++j) acc += dNdx[q][a][j] * sig[i][j];
+                    re[a * 3 + i] += (acc - t.Nq[q][a] * rho * s->g[i]) * dOm;
+                }
+        }
+    }
+    for (int c = 0; c < 3; ++c) std::fill(R[c], R[c] + s->nnodes, 0.0);
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 27; ++a)
+            for (int c = 0; c < 3; ++c) R[c][s->el2n[e * 27 + a]] += Re[(size_t)e * 81 + a * 3 + c];
+}
+
+// stokes_preconditioner_3d_kernel! (momentum_residuals.jl:724-745): Jacobi
+// diagonal η (|∇N_a|² + (∂N_a/∂x_c)²/3) dΩ and the lumped mode mass Σ mode² dΩ
+void orc_cp3_preconditioner(const orc_stokes3* s, double* const* diag, double* M) {
+    const Modes3& md = modes3();
+    for (int c = 0; c < 3; ++c) std::fill(diag[c], diag[c] + s->nnodes, 0.0);
+    static double dNdx[27][27][3];
+    for (int64_t e = 0; e < s->nels; ++e) {
+        elem_dNdx3(s, e, dNdx);
+        const double eta = s->mat.eta[s->phases[e] - 1];
+        double m4[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 27; ++q) {
+            const double dOm = s->dOmega[e * 27 + q];
+            for (int a = 0; a < 27; ++a) {
+                const double* gr = dNdx[q][a];
+                const double n2 = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+                for (int c = 0; c < 3; ++c) diag[c][s->el2n[e * 27 + a]] += eta * (n2 + gr[c] * gr[c] / 3) * dOm;
+            }
+            for (int m = 0; m < 4; ++m) m4[m] += md.m[q][m] * md.m[q][m] * dOm;
+        }
+        for (int m = 0; m < 4; ++m) M[e * 4 + m] = m4[m];
+    }
+}
+
+// _relax_cell_pressure_stokes! (DR.jl:997-1057): Arrow-Hurwicz pressure step
+// with the mean of the constant mode removed, Jacobi velocity step,
+// Dirichlet values before the first residual and after every update, the
+// absolute test max(max_c ‖Rv_c‖, ‖RP‖) < tolerance every check_interval
+// iterations and at the last. s->PC holds the diagonal, s->MP the mode mass.
+int orc_cp3_solve(orc_stokes3* s, const orc_cp3_params* p, orc_cp3_stats* out, double* history,
+                  int64_t history_cap, int64_t* history_len) {
+    std::memset(out, 0, sizeof *out);
+    g_err.clear();
+    const int64_t nv = s->nnodes, np = 4 * s->nels;
+    orc_cp3_preconditioner(s, s->PC, s->MP);
+    for (int c = 0; c < 3; ++c) dirichlet(s->v[c], s->nbc[c], s->bc_dofs[c], s->bc_vals[c]);
+    double sumM = 0;
+    for (int64_t e = 0; e < s->nels; ++e) sumM += s->MP[e * 4];
+    double err_v = kInf, err_P = kInf, err = kInf;
+    int64_t iter = 0, nh = 0;
+    for (int64_t it = 1; it <= p->max_iterations; ++it) {
+        iter = it;
+        orc_cp3_pressure_residual(s, s->RP);
+        for (int64_t i = 0; i < np; ++i) s->P[i] += p->pressure_step * s->RP[i] / s->MP[i];
+        double pm = 0;
+        for (int64_t e = 0; e < s->nels; ++e) pm += s->P[e * 4] * s->MP[e * 4];
+        pm /= sumM;
+        for (int64_t e = 0; e < s->nels; ++e) s->P[e * 4] -= pm;
+        orc_cp3_momentum_residual(s, s->Rv);
+        for (int c = 0; c < 3; ++c) {
+            dirichlet(s->Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
+            for (int64_t i = 0; i < nv; ++i) s->v[c][i] -= p->velocity_step * s->Rv[c][i] / s->PC[c][i];
+            dirichlet(s->v[c], s->nbc[c], s->bc_dofs[c], s->bc_vals[c]);
+        }
+        if (!(it % p->check_interval == 0 || it == p->max_iterations)) continue;
+        err_v = std::max(nrm2(s->Rv[0], nv), std::max(nrm2(s->Rv[1], nv), nrm2(s->Rv[2], nv)));
+        err_P = nrm2(s->RP, np);
+        err = std::max(err_v, err_P);
+        if (!std::isfinite(err)) {
+            g_err = "non-finite residual in 3-D Stokes solve";
+            out->status = -4;
+            return -4;
+        }
+        if (history && nh < history_cap) {
+            history[nh * 3] = (double)it; history[nh * 3 + 1] = err_v; history[nh * 3 + 2] = err_P;
+            ++nh;
+        }
+        if (p->verbose) printf("iter = %06lld err = %.3e - norm[Rv=%.3e, Rp=%.3e]\n", (long long)it, err, err_v, err_P);
+        if (err < p->tolerance) break;
+    }
+    if (history_len) *history_len = nh;
+    out->converged = err < p->tolerance;
+    out->iterations = iter;
+    out->residual = err;
+    out->err_v = err_v;
+    out->err_P = err_P;
+    out->reached_total_iter = iter >= p->max_iterations && !out->converged;
+    out->status = 0;
+    return 0;
+}
+
+}  // extern "C"

@@ -236,6 +236,30 @@ void orc_augmented_jacobian3(orc_stokes3* s, double dt, int32_t finite_K);
 int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out);
 void orc_update_current_stress3(orc_stokes3* s, double dt);
 
+/* --- CellPressureStokesDR (stokes_types.jl:545-613, DR.jl:960-1057): viscous
+ *     Hex27 with the cell-local pressure modes (1, ξ, η, ζ); an orc_stokes3
+ *     whose PC holds the Jacobi diagonal and MP the mode mass --- */
+typedef struct {
+    double tolerance;          /* 1e-5 */
+    int64_t max_iterations;    /* 3000 */
+    int64_t check_interval;    /* 100  */
+    double velocity_step;      /* 0.6  */
+    double pressure_step;      /* 0.2  */
+    int32_t verbose;
+} orc_cp3_params;
+typedef struct {
+    int32_t converged, reached_total_iter;
+    int64_t iterations;
+    double residual, err_v, err_P;
+    int32_t status;
+} orc_cp3_stats;
+void orc_cp3_pressure_residual(const orc_stokes3* s, double* RP /* [nels][4] */);
+void orc_cp3_momentum_residual(const orc_stokes3* s, double* const* R /* 3 x [nnodes] */);
+void orc_cp3_preconditioner(const orc_stokes3* s, double* const* diag, double* M);
+/* history (nullable): [history_cap][3] (iter, err_v, err_P) per check */
+int orc_cp3_solve(orc_stokes3* s, const orc_cp3_params* p, orc_cp3_stats* out, double* history,
+                  int64_t history_cap, int64_t* history_len);
+
 /* --- thermal diffusion (src/heat_diffusion/, src/dynamic_relaxation/solver.jl) --- */
 
 /* ThermalMaterial (src/heat_diffusion/types/heat_diffusion_types.jl:15-41) */

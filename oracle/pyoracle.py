@@ -110,6 +110,16 @@ class ThermalStatsC(C.Structure):
     _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("status", I32)]
 
 
+class CP3Params(C.Structure):
+    _fields_ = [("tolerance", D), ("max_iterations", I64), ("check_interval", I64), ("velocity_step", D),
+                ("pressure_step", D), ("verbose", I32)]
+
+
+class CP3Stats(C.Structure):
+    _fields_ = [("converged", I32), ("reached_total_iter", I32), ("iterations", I64), ("residual", D),
+                ("err_v", D), ("err_P", D), ("status", I32)]
+
+
 class Stokes3(C.Structure):
     _fields_ = [("nnodes", I64), ("nels", I64), ("nnodesP", I64), ("coords", PD), ("el2n", PI), ("invJ", PD),
                 ("dOmega", PD), ("phases", PI), ("mat", Material), ("g", D * 3), ("dp", C.POINTER(DP)),
@@ -136,6 +146,12 @@ def _declare(L):
     L.orc_stokes_solve3.argtypes = [C.POINTER(Stokes3), C.POINTER(Params), C.POINTER(Stats)]
     L.orc_stokes_solve3.restype = C.c_int
     L.orc_update_current_stress3.argtypes = [C.POINTER(Stokes3), D]
+    L.orc_cp3_pressure_residual.argtypes = [C.POINTER(Stokes3), PD]
+    L.orc_cp3_momentum_residual.argtypes = [C.POINTER(Stokes3), C.POINTER(PD)]
+    L.orc_cp3_preconditioner.argtypes = [C.POINTER(Stokes3), C.POINTER(PD), PD]
+    L.orc_cp3_solve.argtypes = [C.POINTER(Stokes3), C.POINTER(CP3Params), C.POINTER(CP3Stats), PD, I64,
+                                C.POINTER(I64)]
+    L.orc_cp3_solve.restype = C.c_int
     L.orc_thermal_integrate.argtypes = [I32, I32, PD, PD, PD, PD, PD, PD, PI, C.POINTER(ThermalMaterialC), PD, D, D,
                                         PD, PD]
     L.orc_thermal_residual.argtypes = [C.POINTER(ThermalC), D, D, I32]
@@ -683,6 +699,38 @@ class Problem3:
     def update_current_stress(self, dt):
         lib().orc_update_current_stress3(C.byref(self.struct()), dt)
         return self.tau_store
+
+    # CellPressureStokesDR (DR.jl:960-1057): P holds the modes (1, ξ, η, ζ)
+    def cp3_pressure_residual(self):
+        RP = np.zeros(4 * self.nels)
+        lib().orc_cp3_pressure_residual(C.byref(self.struct()), ptr(RP))
+        return RP
+
+    def cp3_momentum_residual(self):
+        R = [np.zeros(self.nnodes) for _ in range(3)]
+        lib().orc_cp3_momentum_residual(C.byref(self.struct()), (PD * 3)(*[ptr(r) for r in R]))
+        return R
+
+    def cp3_preconditioner(self):
+        dg = [np.zeros(self.nnodes) for _ in range(3)]
+        M = np.zeros(4 * self.nels)
+        lib().orc_cp3_preconditioner(C.byref(self.struct()), (PD * 3)(*[ptr(d) for d in dg]), ptr(M))
+        return dg, M
+
+    def cp3_solve(self, tolerance=1e-5, max_iterations=3000, check_interval=100, velocity_step=0.6,
+                  pressure_step=0.2, verbose=False, history_cap=0):
+        prm = CP3Params(tolerance=tolerance, max_iterations=max_iterations, check_interval=check_interval,
+                        velocity_step=velocity_step, pressure_step=pressure_step, verbose=int(verbose))
+        st = CP3Stats()
+        hist = np.zeros((max(history_cap, 1), 3))
+        nh = I64(0)
+        rc = lib().orc_cp3_solve(C.byref(self.struct()), C.byref(prm), C.byref(st),
+                                 ptr(hist) if history_cap else ptr(None), int(history_cap), C.byref(nh))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        out = {k: getattr(st, k) for k, _ in CP3Stats._fields_}
+        out["history"] = hist[:nh.value].copy()
+        return out
 
     def rotate_stress(self, dt, tau):
         """rotate_stress! (stress_rotation.jl:28-61), 3-D: 6 arrays [nels][27] in assembler order."""
