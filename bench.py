@@ -178,6 +178,193 @@ def hex27_measure(n, iters):
                          "flops_per_launch": flops, "avg_launch_us": round(el_ms * 1e3, 2)}}
 
 
+def _free_slip3(coords):
+    import numpy as np
+    out = []
+    for c in range(3):
+        d = np.where((np.abs(coords[:, c]) < 1e-12) | (np.abs(coords[:, c] - 1) < 1e-12))[0].astype(np.int32)
+        out.append((d, np.zeros(len(d))))
+    return out
+
+
+def _block_phases3(coords, el2n):
+    import numpy as np
+    cen = coords[el2n].mean(axis=1)
+    return np.where(np.all(np.abs(cen - 0.5) < 0.26, axis=1), 2, 1).astype(np.int32)
+
+
+# FP64 flops per element of one CellPressureStokesDR iteration (pressure pass:
+# 27 points x ~500 for the gradient and the mode sums; momentum pass: 27
+# lanes x ~1160 for the gradient, stress and nodal residual)
+CP3_FLOPS_PER_ELEMENT = 27 * (500 + 1160)
+
+
+def cell_pressure_measure(n, iters):
+    """CellPressureStokesDR on n^3 Hex27 cells: a stiff, dense block (η 50,
+    ρ 2) in a free-slip unit box under gravity; iters relaxation iterations
+    (tolerance 0: the loop runs them all), checks every 100."""
+    import femtools_amd as fa
+    mesh = fa.hex27_box_mesh(n, n, n)
+    ph = _block_phases3(mesh.coords, mesh.el2n)
+    dr = fa.CellPressureStokesDR(mesh, fa.StokesMaterial(eta=(1.0, 50.0), rho0=(1.0, 2.0), g=(0.0, 0.0, -1.0)),
+                                 phases=ph)
+    bcs = [fa.DirichletBoundaryCondition(d, v) for d, v in _free_slip3(mesh.coords)]
+    fa.solve(dr, mesh, *bcs, max_iterations=20, check_interval=20, tolerance=0.0, throw_on_failure=False)
+    for f in dr.v:
+        f.fill(0.0)
+    dr.P.fill(0.0)
+    mesh.ctx.sync()
+    t0 = time.perf_counter()
+    st = fa.solve(dr, mesh, *bcs, max_iterations=iters, check_interval=100, tolerance=0.0, throw_on_failure=False)
+    mesh.ctx.sync()
+    wall = time.perf_counter() - t0
+    dofs = 3 * mesh.nnodes + 4 * mesh.nels
+    tf = CP3_FLOPS_PER_ELEMENT * mesh.nels * st["iterations"] / wall / 1e12
+    return {"workload": f"CellPressureStokesDR, {n}^3 Hex27 cells, sinking block (eta 50, rho 2), free slip, "
+                        f"{iters} Arrow-Hurwicz/Jacobi iterations",
+            "elements": mesh.nels, "dofs": dofs, "iterations": st["iterations"], "seconds": round(wall, 4),
+            "ms_per_iter": round(wall / st["iterations"] * 1e3, 4),
+            "gdof_iter_per_s": round(dofs * st["iterations"] / wall / 1e9, 4), "residual": st["residual"],
+            "fp64_tflops_whole_iteration": round(tf, 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS}
+
+
+def _cap_setup(fa, n, cap):
+    """2-D extension problem with a weak inclusion: the tensile cap (or plain
+    Drucker-Prager) under pure-shear boundary velocities."""
+    import math
+    import numpy as np
+    mesh = fa.t7_box_mesh(n, n)
+    coords, el2n = mesh.coords, mesh.el2n
+    x, y = coords[:, 0], coords[:, 1]
+    gam = mesh.boundary_nodes
+    nx = gam[(np.abs(x[gam]) < 1e-12) | (np.abs(x[gam] - 1) < 1e-12)]
+    ny = gam[(np.abs(y[gam]) < 1e-12) | (np.abs(y[gam] - 1) < 1e-12)]
+    cen = coords[el2n[:, :3]].mean(axis=1)
+    ph = np.where(((cen - 0.5) ** 2).sum(axis=1) < 0.15 ** 2, 2, 1).astype(np.int32)
+    mat = fa.StokesMaterial(eta=(1.0, 0.1), etab=(1.0, 1.0), G=(1.0, 1.0), K=(4.0, 4.0))
+    dr = fa.StokesDR(mesh, mat)
+    dr.set_phases("v", ph)
+    dr.set_phases("P", ph)
+    dr.v.x.copyto(x - 0.5)
+    dr.v.y.copyto(-(y - 0.5))
+    bcs = (fa.DirichletBoundaryCondition(nx, x[nx] - 0.5), fa.DirichletBoundaryCondition(ny, -(y[ny] - 0.5)))
+    ang = (math.radians(30),) * 2
+    if cap:
+        plastic = fa.DruckerPragerCap(ang, (math.radians(5),) * 2, (1.0, 0.5), (-0.3, -0.15), (0.1, 0.1), (4.0, 4.0))
+    else:
+        plastic = fa.DruckerPrager(ang, (math.radians(5),) * 2, (1.0, 0.5), (0.1, 0.1), (4.0, 4.0))
+    return mesh, dr, bcs, plastic
+
+
+def cap_measure(n, iters):
+    """The tensile cap against plain Drucker-Prager on the same 2-D extension
+    problem (n^2 T7 cells, FP64, Jacobian re-assembled at every check):
+    iters DR iterations each from the same start."""
+    import femtools_amd as fa
+    out = {"workload": f"{n}x{n} T7 pure shear with a weak inclusion, VE + yield, {iters} FP64 DR iterations, "
+                       "check interval 50"}
+    for name, cap in (("cap", True), ("drucker_prager", False)):
+        mesh, dr, bcs, plastic = _cap_setup(fa, n, cap)
+        kw = dict(dt=0.5, plastic=plastic, pressure_factor=1.0, tolerance=1e-30, check_interval=50,
+                  max_iterations=iters, iterMax=iters, throw_on_failure=False, finite_K=True)
+        st = fa.solve(dr, mesh, *bcs, **kw)
+        out[name] = {"iterations": st["iter"], "seconds": round(st["seconds"], 4),
+                     "us_per_iter": round(st["seconds"] / max(1, st["iter"]) * 1e6, 2),
+                     "jacobian_assemblies": st["jacobian_assemblies"]}
+    out["cap_over_drucker_prager"] = round(out["cap"]["us_per_iter"] / out["drucker_prager"]["us_per_iter"], 3)
+    return out
+
+
+def _buoyancy_setup(fa, n, coupled):
+    import math
+    import numpy as np
+    mesh = fa.t7_box_mesh(n, n)
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    gam = mesh.boundary_nodes
+    zero = np.zeros(len(gam))
+    mat = fa.StokesMaterial(eta=(1.0,), etab=(1.0,), G=(math.inf,), alpha=(0.05,), rho0=(1.0,), K=(math.inf,),
+                            g=(0.0, -10.0))
+    dr = fa.StokesDR(mesh, mat)
+    th = None
+    T0 = 0.2 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    if coupled:
+        th = fa.ThermalDiffusionDR(mesh, fa.ThermalMaterial(k=(1.0,), Cp=(1.0,), rho0=(1.0,), alpha=(0.0,),
+                                                             K=(math.inf,)), eps=1e-9)
+        th.source.copyto(50.0 * np.exp(-((x - 0.4) ** 2 + (y - 0.5) ** 2) / 0.02))
+        th.T0.copyto(T0)
+        th.T.copyto(T0)
+    bcv = tuple(fa.DirichletBoundaryCondition(gam, zero) for _ in range(2))
+    return mesh, dr, th, fa.DirichletBoundaryCondition(gam, zero), bcv
+
+
+def coupled_measure(n, iters):
+    """solve_coupled! against the Stokes solve alone on the same 2-D buoyancy
+    problem (n^2 T7 cells, FP64): iters inner iterations each (one thermal DR
+    iteration and the temperature transfer per Stokes iteration)."""
+    import femtools_amd as fa
+    out = {"workload": f"{n}x{n} T7 hot blob under gravity (alpha 0.05), {iters} FP64 inner iterations, "
+                       "check interval 50"}
+    kw = dict(dt=0.01, pressure_factor=1.0, tolerance=1e-30, check_interval=50, max_iterations=iters, iterMax=iters,
+              throw_on_failure=False)
+    mesh, dr, th, bcT, bcv = _buoyancy_setup(fa, n, True)
+    t0 = time.perf_counter()
+    st = fa.solve_coupled(th, dr, mesh, mesh, bcT, *bcv, Tref=0.0, **kw)
+    wall = time.perf_counter() - t0
+    out["coupled"] = {"iterations": st["iter"], "thermal_iterations": st["thermal_iterations"],
+                      "seconds": round(wall, 4), "us_per_iter": round(wall / max(1, st["iter"]) * 1e6, 2)}
+    mesh, dr, _, _, bcv = _buoyancy_setup(fa, n, False)
+    t0 = time.perf_counter()
+    st = fa.solve(dr, mesh, *bcv, **kw)
+    wall = time.perf_counter() - t0
+    out["stokes_alone"] = {"iterations": st["iter"], "seconds": round(wall, 4),
+                           "us_per_iter": round(wall / max(1, st["iter"]) * 1e6, 2)}
+    out["coupled_over_stokes"] = round(out["coupled"]["us_per_iter"] / out["stokes_alone"]["us_per_iter"], 3)
+    return out
+
+
+def cpu_rows(budget_s=4.0):
+    """The oracle on small samples of the rows measured beside the headline
+    (cell-pressure 3-D, tensile cap, coupled step): per-iteration times."""
+    import math
+    import numpy as np
+    import pyoracle as O
+    rows = {}
+    # CellPressureStokesDR, 12^3 cells
+    n = 12
+    coords, el2n = O.hex27_mesh(n, n, n)
+    pr = O.Problem3(coords, el2n, O.material(eta=(1.0, 50.0), rho0=(1.0, 2.0)), g=(0.0, 0.0, -1.0),
+                    phases=_block_phases3(coords, el2n), bc=tuple(_free_slip3(coords)))
+    t0 = time.perf_counter()
+    st = pr.cp3_solve(tolerance=0.0, max_iterations=20, check_interval=20)
+    t = time.perf_counter() - t0
+    rows["cell_pressure"] = {"sample": f"{n}^3 cells, {st['iterations']} iterations",
+                             "us_per_element_iter": round(t / st["iterations"] / len(el2n) * 1e6, 4)}
+    # tensile cap, 64^2
+    n = 64
+    coords, el2n = O.t7_mesh(n, n)
+    x, y = coords[:, 0], coords[:, 1]
+    gam = O.box_boundary_nodes(coords)
+    nx = gam[(np.abs(x[gam]) < 1e-12) | (np.abs(x[gam] - 1) < 1e-12)]
+    ny = gam[(np.abs(y[gam]) < 1e-12) | (np.abs(y[gam] - 1) < 1e-12)]
+    cen = coords[el2n[:, :3]].mean(axis=1)
+    ph = np.where(((cen - 0.5) ** 2).sum(axis=1) < 0.15 ** 2, 2, 1).astype(np.int32)
+    cap = O.drucker_prager_cap((math.radians(30),) * 2, (math.radians(5),) * 2, (1.0, 0.5), (-0.3, -0.15),
+                               (0.1, 0.1), (4.0, 4.0))
+    ne = len(el2n)
+    pr = O.Problem(coords, el2n, O.material(eta=(1.0, 0.1), etab=(1.0, 1.0), G=(1.0, 1.0), K=(4.0, 4.0)),
+                   phases_v=ph, phases_P=ph, tau_old=tuple(np.zeros((ne, 7)) for _ in range(3)), cap=cap,
+                   bc=((nx, x[nx] - 0.5), (ny, -(y[ny] - 0.5))))
+    pr.vx[:], pr.vy[:] = x - 0.5, -(y - 0.5)
+    k = 100
+    t0 = time.perf_counter()
+    st = pr.solve(0.5, pressure_factor=1.0, tolerance=1e-30, check_interval=50, max_iterations=k, iter_max=k,
+                  finite_K=True)
+    t = time.perf_counter() - t0
+    rows["cap"] = {"sample": f"{n}x{n} cells, {st['iter']} iterations",
+                   "us_per_element_iter": round(t / max(1, st["iter"]) / ne * 1e6, 4)}
+    return rows
+
+
 def batch_measure(count, nthreads, precision):
     """Independent small solves one at a time and then all at once through
     fem_stokes_solve_batch (one context and stream per problem), each from
@@ -247,6 +434,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--hex27-n", type=int, default=64, help="cells per side of the 3-D measurement (0: skip)")
     ap.add_argument("--hex27-iters", type=int, default=2000)
+    ap.add_argument("--rows-n", type=int, default=256, help="cells per side of the 2-D cap / coupled rows (0: skip)")
+    ap.add_argument("--rows-iters", type=int, default=2000)
+    ap.add_argument("--cell-n", type=int, default=64, help="cells per side of the CellPressureStokesDR row (0: skip)")
+    ap.add_argument("--cell-iters", type=int, default=2000)
     ap.add_argument("--batch", type=int, default=0, help="independent small solves to run batched (0: skip)")
     ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--batch-precision", choices=("fp64", "fp32"), default="fp32")
@@ -402,11 +593,17 @@ def main():
         out["fp32_vs_fp64_max_rel_diff"] = diffs
     if args.hex27_n > 0:
         out["hex27"] = hex27_measure(args.hex27_n, args.hex27_iters)
+    if args.cell_n > 0:
+        out["cell_pressure"] = cell_pressure_measure(args.cell_n, args.cell_iters)
+    if args.rows_n > 0:
+        out["cap"] = cap_measure(args.rows_n, args.rows_iters)
+        out["coupled"] = coupled_measure(args.rows_n, args.rows_iters)
     if args.batch > 0:
         out["batched"] = batch_measure(args.batch, args.batch_threads, args.batch_precision)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+            out["cpu_baseline"]["rows"] = cpu_rows()
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     print(json.dumps(out), flush=True)
