@@ -8,6 +8,7 @@
 // scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
 #include "femdr_host.hpp"
 
+#include <cstdlib>
 #include <functional>
 
 
@@ -1043,6 +1044,16 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
 #define FEM_GRAPHS 1   // 0: every inner iteration is launched from the host
 #endif
 
+// graph replay on (FEM_GRAPHS) unless the environment sets FEMDR_GRAPHS=0,
+// e.g. for tracing tools that do not follow graph launches
+static bool graphs_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("FEMDR_GRAPHS");
+        return FEM_GRAPHS && !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 
 // One check interval (ncheck inner iterations: element + gather launches with
 // the interval's fixed store pattern) captured once per solve and replayed
@@ -1164,7 +1175,7 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
         while ((err > eps_vel || !thermal_ok()) && itPT <= ctl.iter_max) {
             // a whole check interval that cannot end early is replayed from the
             // graph (every 16th interval stays on the host when kernels are timed)
-            const bool whole = FEM_GRAPHS && !coupled && ncheck > 1 && iter % ncheck == 0 &&
+            const bool whole = graphs_on() && !coupled && ncheck > 1 && iter % ncheck == 0 &&
                                itPT + ncheck - 1 <= ctl.iter_max && iter + ncheck - 1 <= ctl.max_iterations &&
                                !(timer.on && (iter / ncheck) % 16 == 0);
             if (whole) {
@@ -1501,7 +1512,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 timer.mark(e0);
                 CHK(seg(nullptr));
                 timer.mark(e2);
-            } else if (FEM_GRAPHS && nseg > 1) {
+            } else if (graphs_on() && nseg > 1) {
                 const int64_t key = ((nseg * 2 + (lastp ? 1 : 0)) * (ncheck + 1) + mpos) * 2 + (d32.delta ? 1 : 0);
                 IntervalGraph* gph = nullptr;
                 for (auto& kv : graphs)
