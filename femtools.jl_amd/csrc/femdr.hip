@@ -642,7 +642,9 @@ static DRSet<double> set64(fem_stokes* s) {
     d.Rv_x = f[FEM_FIELD_RV_X]; d.Rv_y = f[FEM_FIELD_RV_Y];
     d.Rv0_x = f[FEM_FIELD_RV0_X]; d.Rv0_y = f[FEM_FIELD_RV0_Y];
     d.PC_x = f[FEM_FIELD_PC_V_X]; d.PC_y = f[FEM_FIELD_PC_V_Y];
-    d.P = f[FEM_FIELD_P]; d.Pold = f[FEM_FIELD_P0]; d.RP = f[FEM_FIELD_RP]; d.Pnum = f[FEM_FIELD_PNUM];
+    d.P = f[FEM_FIELD_P]; d.RP = f[FEM_FIELD_RP]; d.Pnum = f[FEM_FIELD_PNUM];
+    // solve!'s P_old keyword (a nodal field) in place of dr.P0 (DR.jl:220-226)
+    d.Pold = (s->in.P_old && !s->in.P_old_at_points) ? const_cast<double*>(s->in.P_old) : f[FEM_FIELD_P0];
     d.gammaP = f[FEM_FIELD_GAMMA_P]; d.MP = f[FEM_FIELD_M_P]; d.pscale = s->d_pscale;
     d.tau_old[0] = f[FEM_FIELD_TAU_OLD_XX]; d.tau_old[1] = f[FEM_FIELD_TAU_OLD_YY];
     d.tau_old[2] = f[FEM_FIELD_TAU_OLD_XY];
@@ -763,7 +765,8 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
-    const bool gen = nodeph || (!d.lin && (o.thermal || s->pf_nonzero || s->q_nonzero || o.cap));
+    const bool gen = nodeph || (!d.lin && (o.thermal || s->pf_nonzero || s->q_nonzero || o.cap ||
+                                          s->in.viscosity || (s->in.P_old && s->in.P_old_at_points)));
     a.nodeph = nodeph;
     a.bulk_all_inf = o.bulk_inf;
     a.thermal = o.thermal;
@@ -796,6 +799,10 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
             }
         }
         if (o.store_tau && s->dim == 2) a.P_out = s->f[FEM_FIELD_P_IP];
+        a.bf_x = s->in.body_force[0];
+        a.bf_y = s->in.body_force[1];
+        a.eta_ip = s->in.viscosity;
+        a.Pold_ip = s->in.P_old_at_points ? s->in.P_old : nullptr;
         dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
     } else {
         if (!d.lin || !o.use_pnum)
@@ -841,6 +848,10 @@ static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) 
     g.do_update = o.do_update;
     g.rhs_vx = d.rhs_vx;
     g.rhs_vy = d.rhs_vy;
+    if (!d.lin) {
+        g.bf_x = (const F*)s->in.body_force[0];
+        g.bf_y = (const F*)s->in.body_force[1];
+    }
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     if (d.tile && o.use_pnum && m->nx > 0 && !nodeph) {
         // the tile kernel has summed every grid node's slots
@@ -897,6 +908,7 @@ static int launch_jacobian(fem_stokes* s, const LaunchOpts& o) {
     a.PC_y = s->f[FEM_FIELD_PC_V_Y];
     a.cap = o.cap;
     a.gamma = o.cap ? s->f[FEM_FIELD_GAMMA_PL] : nullptr;
+    a.eta_ip = s->in.viscosity;
     const bool np_ = s->layout_v == FEM_PHASE_ELEMENT;
     const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
     switch (key) {
@@ -929,16 +941,18 @@ static int launch_jacobian(fem_stokes* s, const LaunchOpts& o) {
 static int scaling_impl(fem_stokes* s, double gamma_fact, double dt) {
     fem_mesh* m = s->mesh;
     fem_ctx* c = s->ctx;
+    // solve!'s scaling_viscosity defaults to the viscosity it solves with (DR.jl:315-325)
+    const double* eta_ip = s->in.scaling_viscosity ? s->in.scaling_viscosity : s->in.viscosity;
     if (s->layout_v == FEM_PHASE_ELEMENT)
         hipLaunchKernelGGL((k_pressure_scaling<double, true, 256>), dim3(grid_for(m->nels, 256)), dim3(256), 0,
                            c->stream, m->nels, (const double*)m->d_geo, (const int32_t*)s->d_phase_v,
                            (const double*)s->d_ptab, s->mat.nphases, gamma_fact, dt, s->f[FEM_FIELD_M_P],
-                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale);
+                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale, eta_ip);
     else
         hipLaunchKernelGGL((k_pressure_scaling<double, false, 256>), dim3(grid_for(m->nels, 256)), dim3(256), 0,
                            c->stream, m->nels, (const double*)m->d_geo, (const int32_t*)s->d_phase_v,
                            (const double*)s->d_ptab, s->mat.nphases, gamma_fact, dt, s->f[FEM_FIELD_M_P],
-                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale);
+                           s->f[FEM_FIELD_GAMMA_P], s->d_pscale, eta_ip);
     return check_launch(c, "k_pressure_scaling");
 }
 
@@ -1010,6 +1024,58 @@ static int checked_lmax(fem_stokes* s, double lmax[2]) {
     return FEM_OK;
 }
 
+// entries that are not finite (or, with positive, not > 0)
+__global__ void k_count_bad(int64_t n, const double* x, int32_t positive, int32_t* flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double v = x[i];
+        if (!std::isfinite(v) || (positive && !(v > 0))) atomicAdd(flag, 1);
+    }
+}
+
+// per element: Σ_q (N^P_q · Qq) w_q |det J| and the count of negative or
+// non-finite nodal and point values (_pressure_source_area_kernel!)
+__global__ __launch_bounds__(RED_BLOCK) void k_source_area(int64_t ne, const double* Qq, const double* geo,
+                                                           double* partial) {
+    double acc[2] = {0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t e = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < ne; e += stride) {
+        const double det = geo[4 * ne + e];
+        double Ql[3];
+        for (int i = 0; i < 3; ++i) {
+            Ql[i] = Qq[e * 3 + i];
+            if (!std::isfinite(Ql[i]) || Ql[i] < 0) acc[1] += 1;
+        }
+        for (int q = 0; q < NQ; ++q) {
+            double qv = 0;
+            for (int i = 0; i < 3; ++i) qv += kT.NP1[q][i] * Ql[i];
+            if (!std::isfinite(qv) || qv < 0) acc[1] += 1;
+            acc[0] += qv * (kT.w[q] * det);
+        }
+    }
+    block_sum_store<2>(acc, partial);
+}
+
+__global__ void k_scale_copy(int64_t n, const double* x, double a, double* y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] * a;
+}
+
+// number of bad entries of a device array (-1 when the check itself fails)
+static int64_t count_bad(fem_stokes* s, const double* x, int64_t n, bool positive) {
+    fem_ctx* c = s->ctx;
+    if (n <= 0) return 0;
+    int32_t h = 0;
+    if (hipMemsetAsync(s->d_flag, 0, 4, c->stream) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k_count_bad, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, (int32_t)positive,
+                       s->d_flag);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&h, s->d_flag, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return -1;
+    return h;
+}
+
 static int any_nonzero(fem_stokes* s, const double* x, int64_t n, bool* out) {
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipMemsetAsync(s->d_flag, 0, 4, c->stream));
@@ -1026,6 +1092,7 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
                    bool use_tau_old, const fem_drucker_prager_cap* cap = nullptr) {
     bool bulk_inf, thermal, eos_p, elastic;
     if (cap && s->dim != 2) return set_err(s->ctx, FEM_EARG, "the tensile cap is built for 2-D states");
+    CHK(check_solve_inputs(s, dp != nullptr || cap != nullptr));
     CHK(any_nonzero(s, s->f[FEM_FIELD_PF], s->mesh->nnodesP, &s->pf_nonzero));
     CHK(any_nonzero(s, s->f[FEM_FIELD_Q], s->mesh->nnodesP, &s->q_nonzero));
     CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic, cap));
@@ -1117,6 +1184,16 @@ struct DRResult {
     bool converged = false;
 };
 
+// one row (iter, err_v, err_v_x, err_v_y, err_v_z, err_P) of collect_history
+// (DR.jl:558-561, 722-727) into the caller's buffer; rows past its capacity are dropped
+static void history_row(fem_stokes* s, int64_t iter, double ev, double ex, double ey, double eP) {
+    const fem_solve_inputs& in = s->in;
+    if (!in.history || !in.history_len || *in.history_len >= in.history_cap) return;
+    double* h = in.history + 6 * *in.history_len;
+    h[0] = (double)iter; h[1] = ev; h[2] = ex; h[3] = ey; h[4] = 0.0; h[5] = eP;
+    ++*in.history_len;
+}
+
 // The outer Powell-Hestenes loop (outer residuals without Pnum, errors,
 // rel_drop, P += γP RP / M_P) around the inner DYREL loop (Chebyshev-damped
 // pseudo-transient steps, λmin from the residual change at each check, λmax
@@ -1152,6 +1229,8 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
         timer.collect();
         r.err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
         r.err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
+        history_row(s, iter, r.err_v, std::sqrt(r3[1]) / (2 * std::sqrt((double)nv)),
+                    std::sqrt(r3[2]) / (2 * std::sqrt((double)nv)), r.err_P);
         if (itPH == 1) { err_P0 = r.err_P + julia_eps(r.err_P); err_v0 = r.err_v + julia_eps(r.err_v); }
         else if (itPH == 2) { err_P0 = r.err_P + julia_eps(r.err_P); }
         r.err_abs = std::max(r.err_v, r.err_P);
@@ -1235,6 +1314,13 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
                 err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
                 if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
                 if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (s->in.history) {
+                    // the pressure residual of this check's iteration (stored by the check launch)
+                    double h3[3];
+                    CHK(reduce_outer(s, d, h3));
+                    history_row(s, iter, evi, std::sqrt(r6[0]) / (2 * std::sqrt((double)nv)),
+                                std::sqrt(r6[1]) / (2 * std::sqrt((double)nv)), std::sqrt(h3[0]) / std::sqrt((double)np));
+                }
                 if (ctl.verbose > 1) printf("%s  it = %lld, iter = %lld, err = %.3e\n", ctl.tag, (long long)itPT, (long long)iter, err);
                 double lmin[2];
                 for (int k = 0; k < 2; ++k) {
@@ -1717,6 +1803,12 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     if (cap && p->precision != 0)
         return set_err(c, FEM_EARG, "solves with the tensile cap run in FP64 (precision 0), got precision %d",
                        p->precision);
+    if (s->in.viscosity && cp) return set_err(c, FEM_EARG, "quadrature viscosity is not supported by coupled solves");
+    if (s->in.viscosity && p->precision != 0)
+        return set_err(c, FEM_EARG, "quadrature viscosity is solved in FP64 (precision 0)");
+    if (s->in.history && (p->precision != 0 || s->dim != 2))
+        return set_err(c, FEM_EARG, "collect_history is recorded by 2-D FP64 solves (precision 0)");
+    if (s->in.history_len) *s->in.history_len = 0;
     if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
@@ -1822,6 +1914,73 @@ int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt
     CHK(prepare(s, nullptr, dt, finite_K, o, true));
     CHK(scaling_impl(s, pressure_factor, dt));
     return fem_sync(s->ctx);
+}
+
+int fem_stokes_set_solve_inputs(fem_stokes* s, const fem_solve_inputs* in) {
+    if (!s) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    if (!in) { s->in = fem_solve_inputs{}; return FEM_OK; }
+    HIPCHK(c, hipSetDevice(c->device));
+    const fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, nip = m->nels * (s->dim == 2 ? NQ : 27);
+    // 0: clean, 1: bad entries, -1: the device check failed
+    auto bad = [&](const double* x, int64_t n, bool positive) -> int {
+        const int64_t k = count_bad(s, x, n, positive);
+        return k < 0 ? -1 : (k > 0 ? 1 : 0);
+    };
+    const int nbf = (in->body_force[0] != nullptr) + (in->body_force[1] != nullptr) + (in->body_force[2] != nullptr);
+    if (nbf != 0 && (nbf != s->dim || in->body_force[s->dim - 1] == nullptr))
+        return set_err(c, FEM_EARG, "body_force must contain one assembled load vector per velocity component");
+    int b = 0;
+    for (int k = 0; k < nbf && b == 0; ++k) b = bad(in->body_force[k], nv, false);
+    if (b < 0) return set_err(c, FEM_EHIP, "device check of body_force failed");
+    if (b > 0) return set_err(c, FEM_EARG, "body_force must contain finite values");
+    for (const double* eta : {in->viscosity, in->scaling_viscosity}) {
+        if (!eta) continue;
+        b = bad(eta, nip, true);
+        if (b < 0) return set_err(c, FEM_EHIP, "device check of the viscosity failed");
+        if (b > 0) return set_err(c, FEM_EARG, "viscosity must contain finite positive values");
+    }
+    if (in->history && (in->history_cap < 0 || !in->history_len))
+        return set_err(c, FEM_EARG, "a history buffer needs a nonnegative capacity and a length output");
+    s->in = *in;
+    return FEM_OK;
+}
+
+int fem_stokes_set_pressure_source(fem_stokes* s, const double* Qq, double Q2D, int32_t has_Q2D) {
+    if (!s || !Qq) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim != 2) return set_err(c, FEM_EARG, "the pressure source is built for 2-D states");
+    const int64_t np = s->mesh->nnodesP, ne = s->mesh->nels;
+    double* Q = s->f[FEM_FIELD_Q];
+    if (!has_Q2D) {
+        const int64_t k = count_bad(s, Qq, np, false);
+        if (k < 0) return set_err(c, FEM_EHIP, "device check of Qq failed");
+        if (k > 0) return set_err(c, FEM_EARG, "Qq must be finite");
+        HIPCHK(c, hipMemcpyAsync(Q, Qq, np * 8, hipMemcpyDeviceToDevice, c->stream));
+        return fem_sync(c);
+    }
+    if (!std::isfinite(Q2D)) return set_err(c, FEM_EARG, "Q2D must be finite");
+    // ∫ N^P·Qq dΩ and the count of negative or non-finite values (pressure_residual.jl:32-47)
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(ne, RED_BLOCK));
+    hipLaunchKernelGGL(k_source_area, dim3(g), dim3(RED_BLOCK), 0, c->stream, ne, Qq,
+                       (const double*)s->mesh->d_geo, c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 2, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_source_area"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 2 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double area = c->h_red[0], invalid = c->h_red[1];
+    if (invalid != 0) return set_err(c, FEM_EARG, "Qq must be finite and nonnegative");
+    if (!(std::isfinite(area) && area > 0)) {
+        if (Q2D != 0) return set_err(c, FEM_EARG, "nonzero Q2D requires nonempty positive Qq support");
+        HIPCHK(c, hipMemsetAsync(Q, 0, np * 8, c->stream));
+        return fem_sync(c);
+    }
+    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, Qq, Q2D / area, Q);
+    CHK(check_launch(c, "k_scale_copy"));
+    return fem_sync(c);
 }
 
 int fem_stokes_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t with_pnum, int32_t finite_K) {

@@ -88,6 +88,7 @@ struct Hex27Args {
     double *RP_out, *Pnum_out;             // nullable
     double *Rv_x, *Rv_y, *Rv_z;            // nullable
     double *Rv0_x, *Rv0_y, *Rv0_z;         // nullable
+    const double *bf_x, *bf_y, *bf_z;      // nullable: nodal loads subtracted from the residual
 };
 
 // global node of element e at tensor position t = i + 3 j + 9 k
@@ -402,6 +403,7 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
     }
     // centre node: private to the element
     const uint8_t fl = A.bcflag[node];
+    if (A.bf_x) { R[0] -= A.bf_x[node]; R[1] -= A.bf_y[node]; R[2] -= A.bf_z[node]; }
     const double rx = (fl & 1) ? 0.0 : R[0], ry = (fl & 2) ? 0.0 : R[1], rz = (fl & 4) ? 0.0 : R[2];
     if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
     if (A.Rv0_x) { A.Rv0_x[node] = rx; A.Rv0_y[node] = ry; A.Rv0_z[node] = rz; }
@@ -431,6 +433,7 @@ struct Gather3Args {
     double *Rv_x, *Rv_y, *Rv_z, *Rv0_x, *Rv0_y, *Rv0_z;
     double ax, ay, az, bx, by, bz;
     int32_t do_update;
+    const double *bf_x, *bf_y, *bf_z;   // nullable: nodal loads subtracted from the gathered sum
 };
 
 __global__ __launch_bounds__(256) void k_gather3(Gather3Args A) {
@@ -442,6 +445,7 @@ __global__ __launch_bounds__(256) void k_gather3(Gather3Args A) {
         const double* s = A.slot + (int64_t)A.ent[k] * 3;
         rx += s[0]; ry += s[1]; rz += s[2];
     }
+    if (A.bf_x) { rx -= A.bf_x[node]; ry -= A.bf_y[node]; rz -= A.bf_z[node]; }
     const uint8_t fl = A.bcflag[node];
     if (fl & 1) rx = 0;
     if (fl & 2) ry = 0;
@@ -740,7 +744,10 @@ int hex_element(fem_stokes* s, const Opts3& o, bool gather = true) {
     std::memset(&a, 0, sizeof a);
     a.g = hex_geo(m);
     a.vx = f[FEM_FIELD_V_X]; a.vy = f[FEM_FIELD_V_Y]; a.vz = f[FEM_FIELD_V_Z];
-    a.P = f[FEM_FIELD_P]; a.Pold = f[FEM_FIELD_P0]; a.pscale = s->d_pscale;
+    a.P = f[FEM_FIELD_P];
+    a.Pold = (s->in.P_old && !s->in.P_old_at_points) ? s->in.P_old : f[FEM_FIELD_P0];
+    a.pscale = s->d_pscale;
+    a.bf_x = s->in.body_force[0]; a.bf_y = s->in.body_force[1]; a.bf_z = s->in.body_force[2];
     const int tf[6] = {FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_ZZ,
                        FEM_FIELD_TAU_OLD_XY, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ};
     const int tc[6] = {FEM_FIELD_TAU_XX, FEM_FIELD_TAU_YY, FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XY, FEM_FIELD_TAU_XZ,
@@ -784,6 +791,7 @@ int hex_element(fem_stokes* s, const Opts3& o, bool gather = true) {
     g.Rv0_x = a.Rv0_x; g.Rv0_y = a.Rv0_y; g.Rv0_z = a.Rv0_z;
     g.ax = a.ax; g.ay = a.ay; g.az = a.az; g.bx = a.bx; g.by = a.by; g.bz = a.bz;
     g.do_update = o.do_update;
+    g.bf_x = a.bf_x; g.bf_y = a.bf_y; g.bf_z = a.bf_z;
     hipLaunchKernelGGL(k_gather3, dim3(grid_xcd(m->ns, 256)), dim3(256), 0, s->ctx->stream, g);
     return check3(s->ctx, "k_gather3");
 }
@@ -871,6 +879,7 @@ int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_
     if (!(dt > 0) || !std::isfinite(dt)) return set_err(s->ctx, FEM_EARG, "dt must be positive and finite");
     if (s->layout_v != FEM_PHASE_CELL || s->layout_P != FEM_PHASE_CELL || !s->phase_same)
         return set_err(s->ctx, FEM_EARG, "the 3-D path takes one phase per cell (a 1 x nels phase matrix)");
+    CHK(check_solve_inputs(s, dp != nullptr));
     CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
     (void)thermal;   // no temperature field in 3-D: ρ = ρ0 (1 + α Tref + β P)
     o.plastic = dp != nullptr;

@@ -126,6 +126,7 @@ struct fem_stokes {
     uint8_t* d_yflag = nullptr;     // near-yield flag per element (mixed-precision solve)
     bool pf_nonzero = false;        // dr.Pf holds a nonzero entry (checked at each entry point)
     bool q_nonzero = false;         // dr.Q holds a nonzero entry
+    fem_solve_inputs in{};          // array keywords of the next calls (fem_stokes_set_solve_inputs)
     // FP32 work arrays of the mixed-precision solve (allocated by its first call):
     // the correction (δv, δP), its DR rates/residuals, the right-hand sides R(x)
     // and FP32 copies of the scaling, preconditioner and geometry
@@ -152,6 +153,22 @@ static int set_err(fem_ctx* c, int code, const char* fmt, ...) {
     va_end(ap);
     if (c) c->err = buf;
     return code;
+}
+// The array keywords attached by fem_stokes_set_solve_inputs that a call with
+// this yield model may use: a viscosity at the points is for purely viscous
+// 2-D flow without a yield model (DR.jl:388-395)
+static inline int check_solve_inputs(fem_stokes* s, bool plastic) {
+    const fem_solve_inputs& in = s->in;
+    if (in.viscosity) {
+        bool ginf = true;
+        for (int p = 0; p < s->mat.nphases; ++p) ginf = ginf && std::isinf(s->mat.G[p]);
+        if (s->dim != 2 || !ginf)
+            return set_err(s->ctx, FEM_EARG, "quadrature viscosity currently requires purely viscous 2-D flow");
+        if (plastic) return set_err(s->ctx, FEM_EARG, "quadrature viscosity requires plastic=nothing");
+    }
+    if (s->dim != 2 && (in.scaling_viscosity || (in.P_old && in.P_old_at_points)))
+        return set_err(s->ctx, FEM_EARG, "integration-point P_old and scaling_viscosity are built for 2-D states");
+    return FEM_OK;
 }
 #define HIPCHK(ctx, x)                                                                           \
     do {                                                                                         \

@@ -497,6 +497,11 @@ struct ElemArgs {
     // EF_DELTA: those flags; blocks without a flagged element skip the return
     // map (their operator is the exact visco-elastic one)
     const uint8_t* hot;
+    // solve! array keywords (FP64 launches; nullable): nodal loads subtracted
+    // from the momentum residual, with EF_GEN the viscosity at the points
+    // [nels][NQ] and the previous-step pressure at the points [nels][NQ]
+    const F* bf_x; const F* bf_y;
+    const F* eta_ip; const F* Pold_ip;
     // EF_GEN extras
     int32_t nodeph;            // phases stored per element node
     int32_t bulk_all_inf;
@@ -691,7 +696,8 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
 #pragma unroll
             for (int i = 0; i < NP; ++i) Qv[i] = A.Q[e * NP + i];
         }
-        if (!nodeph) {
+        const bool ipold = GEN && BULK && !LIN && A.Pold_ip != nullptr;
+        if (!nodeph && !ipold) {
             const int p = A.phase_same ? pc : A.phase_P[e];
             const F bulk = (GEN && A.bulk_all_inf) || !BULK ? (F)INFINITY : PT(PC_BULK, p);
             const F al = PT(PC_ALPHA, p);
@@ -707,11 +713,12 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
             }
         } else {
             int pp[NP];
+            const int pcP = A.phase_same ? pc : A.phase_P[e];
 #pragma unroll
-            for (int i = 0; i < NP; ++i) pp[i] = A.phase_P[i * ne + e];
+            for (int i = 0; i < NP; ++i) pp[i] = nodeph ? A.phase_P[i * ne + e] : pcP;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                F bulk = 0, alq = 0, dPq = 0, dTq = 0, Qq = 0;
+                F bulk = 0, alq = 0, dPq = 0, dTq = 0, Qq = 0, Pq = 0;
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
                     const F Ni = (F)kT.NP1[q][i];
@@ -720,7 +727,10 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                     dPq += Ni * dP[i];
                     dTq += Ni * dT[i];
                     Qq += Ni * Qv[i];
+                    Pq += Ni * P[i];
                 }
+                // pressure_increment_at_ip (pressure_residual.jl:90-92): N·P − P_old[q]
+                if (ipold) dPq = Pq - A.Pold_ip[e * NQ + q];
                 if (A.bulk_all_inf || !BULK) bulk = (F)INFINITY;
                 const F src = (-(dPq / (bulk * A.dt)) + alq * dTq / A.dt + Qq) * ((F)kT.w[q] * det);
 #pragma unroll
@@ -802,6 +812,8 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
                 m.invden = F(1) / ((m.etave + ereg) + ((Kb * A.dt) * (-spsi)) * (-sphi));
             }
         }
+        // a viscosity given at the points (purely viscous: G = Inf, so ηve = η)
+        if (GEN && !LIN && A.eta_ip) m.etave = A.eta_ip[e * NQ + q];
         F txo = 0, tyo = 0, txyo = 0;
         if (TAU_LDS) {
             txo = s_tau[0 * BLOCK * NQ + threadIdx.x * NQ + q];
@@ -1071,6 +1083,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         const int32_t b = n[NSH];
         const uint8_t fl = A.bcflag[b];
         if (LIN) { Rx[NSH] += A.rhs_vx[b]; Ry[NSH] += A.rhs_vy[b]; }
+        if (!LIN && A.bf_x) { Rx[NSH] -= A.bf_x[b]; Ry[NSH] -= A.bf_y[b]; }
         const F rx = (fl & 1) ? F(0) : Rx[NSH];
         const F ry = (fl & 2) ? F(0) : Ry[NSH];
         if (A.Rv_out_x) { A.Rv_out_x[b] = rx; A.Rv_out_y[b] = ry; }
@@ -1109,6 +1122,7 @@ struct GatherArgs {
     const F* cheb;              // nullable: the four Chebyshev scalars in device memory instead
     int32_t do_update;
     const F* rhs_vx; const F* rhs_vy;   // nullable: nodal right-hand side added to the gathered sum
+    const F* bf_x; const F* bf_y;       // nullable: nodal loads subtracted from the gathered sum
 };
 
 template <class F, bool STRUCT, int BLOCK>
@@ -1124,6 +1138,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_update(GatherArgs<F> A) {
         rx += r.x;
         ry += r.y;
     }
+    if (A.bf_x) { rx -= A.bf_x[node]; ry -= A.bf_y[node]; }
     node_update<F>(node, rx, ry, A.rhs_vx, A.rhs_vy, A.bcflag, A.bcval_x, A.bcval_y, A.Rv_out_x, A.Rv_out_y,
                    A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.cheb ? A.cheb[0] : A.alpha_x,
                    A.cheb ? A.cheb[1] : A.alpha_y, A.cheb ? A.cheb[2] : A.beta_x, A.cheb ? A.cheb[3] : A.beta_y,
@@ -1197,6 +1212,8 @@ struct JacArgs {
     F* dRdv_x; F* dRdv_y; F* PC_x; F* PC_y;   // bubble entries written directly
     int32_t cap;                 // PLASTIC: the tensile cap instead of Drucker-Prager
     const F* gamma;              // nullable: accepted plastic strain (cohesion softening)
+    const F* eta_ip;             // nullable: viscosity at the points [nels][NQ]; the element
+                                 // maximum replaces the phase maximum (momentum_residuals.jl:43)
 };
 
 template <class F, bool ELASTIC, bool PLASTIC, bool NODEPH, int BLOCK>
@@ -1234,6 +1251,11 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
 #pragma unroll
         for (int a = 0; a < NV; ++a) pv[a] = p;
         etamax = PT(PC_ETA, p);
+    }
+    if (A.eta_ip) {
+        etamax = A.eta_ip[e * NQ];
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) etamax = fmax(etamax, A.eta_ip[e * NQ + q]);
     }
     // tangents at the quadrature points: C[q][t][m] = ∂τ_t/∂ε_m
     F Cq[NQ][3][3];
@@ -1420,7 +1442,8 @@ __global__ __launch_bounds__(BLOCK) void k_gather_jac(GatherJacArgs<F> A) {
 template <class F, bool NODEPH, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pressure_scaling(int64_t nels, const F* geo, const int32_t* phase_v,
                                                            const double* ptab, int32_t nphases, double gamma_fact,
-                                                           double dt, F* MP, F* gammaP, F* pscale) {
+                                                           double dt, F* MP, F* gammaP, F* pscale,
+                                                           const double* eta_ip) {
     __shared__ double s_tab[PC_NCOL * MAXPH];
     for (int i = threadIdx.x; i < PC_NCOL * MAXPH; i += BLOCK)
         if ((i % MAXPH) < nphases) s_tab[i] = ptab[i];
@@ -1441,6 +1464,7 @@ __global__ __launch_bounds__(BLOCK) void k_pressure_scaling(int64_t nels, const 
             eta += kT.N[q][a] * s_tab[PC_ETA * MAXPH + pv[a]];
             bq += kT.N[q][a] * (1.0 / s_tab[PC_BULK * MAXPH + pv[a]]);
         }
+        if (eta_ip) eta = eta_ip[e * NQ + q];   // a viscosity field at the points (pressure_scaling.jl:185)
         const double gnum = gamma_fact * eta;
         const double gphy = bq == 0.0 ? gnum : dt / bq;
         const double gq = gphy * gnum / (gphy + gnum);

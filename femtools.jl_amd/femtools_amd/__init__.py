@@ -29,7 +29,7 @@ __all__ = [
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
-    "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap",
+    "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
 ]
@@ -103,6 +103,12 @@ class _SolveParams(C.Structure):
                 ("check_interval", I64), ("max_ph_iterations", I64), ("rel_drop0", D), ("finite_K", I32),
                 ("freeze_jacobian", I32), ("throw_on_failure", I32), ("use_tau_old", I32), ("skip_scaling", I32),
                 ("verbose", I32), ("precision", I32), ("time_kernels", I32)]
+
+
+class _SolveInputs(C.Structure):
+    _fields_ = [("body_force", VP * 3), ("viscosity", VP), ("scaling_viscosity", VP), ("P_old", VP),
+                ("P_old_at_points", I32), ("history", C.POINTER(D)), ("history_cap", I64),
+                ("history_len", C.POINTER(I64))]
 
 
 class _SolveStats(C.Structure):
@@ -205,6 +211,8 @@ _sig("fem_stokes_residual_cap", C.c_int, VP, _P(_Cap), D, I32, I32)
 _sig("fem_stokes_jacobian_diagnostics_cap", C.c_int, VP, _P(_Cap), D, I32)
 _sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
+_sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
+_sig("fem_stokes_set_pressure_source", C.c_int, VP, VP, D, I32)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
 
@@ -221,7 +229,8 @@ EXPORTED = [
     "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
     "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
     "fem_stokes_commit_plastic_history", "fem_cell_stokes_preconditioner", "fem_cell_stokes_pressure_residual",
-    "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve",
+    "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve", "fem_stokes_set_solve_inputs",
+    "fem_stokes_set_pressure_source",
 ]
 
 
@@ -735,10 +744,137 @@ def _is_cap(plastic):
     return isinstance(plastic, DruckerPragerCap)
 
 
-def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, dt, finite_K=False):
+class _Inputs:
+    """The array keywords of one solve!/residual call (DR.jl:183-186, 217-226,
+    302-337, 386-418), attached with fem_stokes_set_solve_inputs on entry and
+    cleared on exit. Host arrays are staged in device buffers freed on exit;
+    DeviceArrays are used in place. Shapes follow this package's layouts:
+    nodal vectors, point fields (nels, nq), pressure (nnodesP,) or (nels, np)."""
+
+    def __init__(self, dr, *, body_force=None, viscosity=None, scaling_viscosity=None, P_old=None,
+                 history_cap=0):
+        self.dr, self.ctx = dr, dr.ctx
+        self.bufs, self.keep = [], []
+        m = dr.mesh
+        nq = 7 if m.dim == 2 else 27
+        c = _SolveInputs()
+        self.history = None
+        self.history_len = I64(0)
+        if body_force is not None:
+            if not isinstance(body_force, (tuple, list)) or len(body_force) != m.dim:
+                raise ValueError("body_force must contain one assembled load vector per velocity component")
+            for k, f in enumerate(body_force):
+                if _size(f) != m.nnodes:
+                    raise DimensionMismatch("body_force must match velocity nodes")
+                c.body_force[k] = self._dev(f, (m.nnodes,))
+        for name in ("viscosity", "scaling_viscosity"):
+            eta = locals()[name]
+            if eta is None:
+                continue
+            if isinstance(eta, (tuple, list)):
+                raise ValueError(f"{name} takes a viscosity at the points, an (nels, {nq}) array; per-phase "
+                                 "viscosities are set on the StokesMaterial")
+            if _shape(eta) not in ((m.nels, nq), (m.nels * nq,)):
+                raise DimensionMismatch(f"viscosity must be nq × nels = ({nq}, {m.nels}) (here (nels, nq))")
+            if _dtype(eta) != np.float64:
+                raise ValueError("viscosity must match the state precision")
+            setattr(c, name, self._dev(eta, (m.nels, nq)))
+        if P_old is not None:
+            sh = _shape(P_old)
+            npl = m.nnodesP // m.nels
+            if len(sh) == 2 and sh != (m.nels, npl):
+                if sh != (m.nels, nq):
+                    raise DimensionMismatch(f"integration-point P_old must be nq × nels = ({nq}, {m.nels}), got "
+                                            f"{sh[::-1]}")
+                c.P_old = self._dev(P_old, sh)
+                c.P_old_at_points = 1
+            elif _size(P_old) == m.nnodesP:
+                c.P_old = self._dev(P_old, (m.nnodesP,))
+            else:
+                raise DimensionMismatch("nodal P_old must match the pressure field shape")
+        if history_cap:
+            self.history = np.zeros((int(history_cap), 6))
+            c.history = self.history.ctypes.data_as(C.POINTER(D))
+            c.history_cap = int(history_cap)
+            c.history_len = C.pointer(self.history_len)
+        self.c = c
+
+    def _dev(self, a, shape):
+        if isinstance(a, DeviceArray):
+            if a.ctx is not self.ctx:
+                raise ValueError("keyword arrays and Stokes state must use the same device context")
+            return a.ptr
+        h = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+        buf = VP()
+        self.ctx.check(_lib.fem_buf_alloc(self.ctx.handle, max(h.nbytes, 8), C.byref(buf)))
+        self.bufs.append(buf)
+        if h.nbytes:
+            self.ctx.check(_lib.fem_copy_h2d(self.ctx.handle, buf, h.ctypes.data, h.nbytes))
+        return buf.value
+
+    def rows(self):
+        """collect_history rows as the reference's NamedTuples (DR.jl:558-561, 722-727)."""
+        h = self.history[:self.history_len.value]
+        nd = self.dr.mesh.dim
+        return [dict(iter=int(r[0]), err_v=r[1], err_v_components=tuple(r[2:2 + nd]), err_P=r[5]) for r in h]
+
+    def __enter__(self):
+        try:
+            self.ctx.check(_lib.fem_stokes_set_solve_inputs(self.dr.handle, C.byref(self.c)))
+        except Exception:
+            self._free()
+            raise
+        return self
+
+    def __exit__(self, *exc):
+        _lib.fem_stokes_set_solve_inputs(self.dr.handle, None)
+        self._free()
+        return False
+
+    def _free(self):
+        for b in self.bufs:
+            _lib.fem_buf_free(self.ctx.handle, b)
+        self.bufs = []
+
+
+def _shape(a):
+    return tuple(a.shape) if isinstance(a, DeviceArray) else np.shape(a)
+
+
+def _size(a):
+    return a.size if isinstance(a, DeviceArray) else int(np.size(a))
+
+
+def _dtype(a):
+    return a.dtype if isinstance(a, DeviceArray) else np.asarray(a).dtype
+
+
+_INPUT_KEYS = ("body_force", "viscosity", "scaling_viscosity", "P_old")
+
+
+def _pop_inputs(kwargs):
+    return {k: kwargs.pop(k) for k in _INPUT_KEYS if k in kwargs and kwargs[k] is not None}
+
+
+def set_pressure_source(dr: StokesDR, Qq, Q2D=None):
+    """_set_pressure_source!(dr.Q, Qq, Q2D, ...) (pressure_residual.jl:32-77),
+    as solve_stokes_dyrel! runs it for its Qq/Q2D keywords (DR.jl:451-457):
+    dr.Q = Qq scaled so that ∫ Q dΩ = Q2D; without Q2D, Qq is copied."""
+    if _size(Qq) != dr.mesh.nnodesP:
+        raise DimensionMismatch("Qq must match the pressure field shape")
+    with _Inputs(dr) as io:
+        ptr = io._dev(Qq, (dr.mesh.nnodesP,))
+        dr.ctx.check(_lib.fem_stokes_set_pressure_source(dr.handle, ptr, 0.0 if Q2D is None else float(Q2D),
+                                                          int(Q2D is not None)))
+
+
+def assemble_viscosity_weighted_pressure_scaling(dr: StokesDR, pressure_factor, dt, finite_K=False, eta=None):
     """assemble_viscosity_weighted_pressure_scaling! (pressure_scaling.jl:8-149):
-    fills dr.M_P and dr.gammaP."""
-    dr.ctx.check(_lib.fem_stokes_pressure_scaling(dr.handle, float(pressure_factor), float(dt), int(finite_K)))
+    fills dr.M_P and dr.gammaP; eta (nels, nq) samples the viscosity at the
+    points in place of the phase viscosities (pressure_scaling.jl:60, 185)."""
+    with _Inputs(dr, scaling_viscosity=eta):
+        dr.ctx.check(_lib.fem_stokes_pressure_scaling(dr.handle, float(pressure_factor), float(dt),
+                                                      int(finite_K)))
 
 
 def solve(dr, mesh, *bcs, **kwargs):
@@ -812,13 +948,24 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
     iterations in FP32 on a correction of the linearised visco-elastic
     problem, folded into the FP64 state at every outer iteration."""
     keep = []
+    inputs = _pop_inputs(kwargs)
+    collect = bool(kwargs.pop("collect_history", False))
+    Qq, Q2D = kwargs.pop("Qq", None), kwargs.pop("Q2D", None)
+    if Q2D is not None and Qq is None:
+        raise ValueError("Qq is required when Q2D is supplied")
     bcs, dp, p = _stokes_call(dr, mesh, (bc_vx, bc_vy, bc_vz), keep, **kwargs)
+    if Qq is not None:
+        set_pressure_source(dr, Qq, Q2D)
+    hcap = min(2 * (p.max_iterations // max(p.check_interval, 1) + p.max_ph_iterations) + 4, 1_000_000) \
+        if collect else 0
     st = _SolveStats()
-    if _is_cap(kwargs.get("plastic")):
-        rc = _lib.fem_stokes_solve_cap(dr.handle, bcs, C.cast(dp, _P(_Cap)), C.byref(p), C.byref(st))
-    else:
-        rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
-    stats = _stats_dict(st)
+    with _Inputs(dr, history_cap=hcap, **inputs) as io:
+        if _is_cap(kwargs.get("plastic")):
+            rc = _lib.fem_stokes_solve_cap(dr.handle, bcs, C.cast(dp, _P(_Cap)), C.byref(p), C.byref(st))
+        else:
+            rc = _lib.fem_stokes_solve(dr.handle, bcs, dp, C.byref(p), C.byref(st))
+        stats = _stats_dict(st)
+        stats["history"] = io.rows() if collect else []
     dr.ctx.check(rc)
     return stats
 
@@ -890,24 +1037,32 @@ def commit_stokes_plastic_history(dr: StokesDR, mesh: MixedMesh, dt, *, plastic)
     return dr
 
 
-def residual(dr: StokesDR, dt, plastic=None, with_pnum=False, finite_K=False):
-    """Element residual sweep: dr.RP (and dr.Pnum) and dr.Rv at the current state."""
+def residual(dr: StokesDR, dt, plastic=None, with_pnum=False, finite_K=False, **inputs):
+    """Element residual sweep: dr.RP (and dr.Pnum) and dr.Rv at the current
+    state; body_force, viscosity and P_old as solve! takes them (the loads are
+    subtracted from dr.Rv before the Dirichlet rows)."""
     d, dp = _dp_ptr(plastic)
-    if _is_cap(plastic):
-        dr.ctx.check(_lib.fem_stokes_residual_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt), int(with_pnum),
-                                                  int(finite_K)))
-    else:
-        dr.ctx.check(_lib.fem_stokes_residual(dr.handle, dp, float(dt), int(with_pnum), int(finite_K)))
+    kw = _pop_inputs(inputs)
+    if inputs:
+        raise TypeError(f"unexpected keywords {sorted(inputs)}")
+    with _Inputs(dr, **kw):
+        if _is_cap(plastic):
+            dr.ctx.check(_lib.fem_stokes_residual_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt), int(with_pnum),
+                                                      int(finite_K)))
+        else:
+            dr.ctx.check(_lib.fem_stokes_residual(dr.handle, dp, float(dt), int(with_pnum), int(finite_K)))
 
 
-def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False):
-    """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal)."""
+def jacobian_diagnostics(dr: StokesDR, dt, plastic=None, finite_K=False, viscosity=None):
+    """Augmented-Jacobian diagnostics dr.dRvdv (row sums) and dr.PC_v (diagonal);
+    a viscosity at the points enters as its element maximum."""
     d, dp = _dp_ptr(plastic)
-    if _is_cap(plastic):
-        dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt),
-                                                              int(finite_K)))
-    else:
-        dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
+    with _Inputs(dr, viscosity=viscosity):
+        if _is_cap(plastic):
+            dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics_cap(dr.handle, C.cast(dp, _P(_Cap)), float(dt),
+                                                                  int(finite_K)))
+        else:
+            dr.ctx.check(_lib.fem_stokes_jacobian_diagnostics(dr.handle, dp, float(dt), int(finite_K)))
 
 
 def copyto(dst, src):

@@ -248,6 +248,44 @@ typedef struct {
 int fem_stokes_solve(fem_stokes* st, const fem_bc bcs[2], const fem_drucker_prager* plastic,
                      const fem_solve_params* params, fem_solve_stats* stats);
 
+/* The optional array keywords of solve! / solve_stokes_dyrel! (DR.jl:183-186,
+ * 217-226, 302-337, 366-418, 556-561, 722-727). Device arrays (f64, NULL when
+ * absent) stay attached to the state until the next fem_stokes_set_solve_inputs
+ * call (NULL clears them) and are read by fem_stokes_solve, fem_stokes_solve_cap,
+ * fem_stokes_residual(_cap), fem_stokes_jacobian_diagnostics(_cap) and
+ * fem_stokes_pressure_scaling, so a binding sets them around one call:
+ *   body_force[c]      nodal loads f_i = ∫N_i b dΩ (nnodes each), subtracted from
+ *                      the momentum residual before the Dirichlet rows (2-D, 3-D)
+ *   viscosity          nq x nels = [nels][nq] η at the velocity quadrature points
+ *                      in place of the phase viscosities (2-D, purely viscous:
+ *                      G = Inf, no yield model, FP64, not coupled)
+ *   scaling_viscosity  [nels][nq] viscosity of the pressure scale only; NULL:
+ *                      `viscosity` (the phase viscosities when that is NULL too)
+ *   P_old              accepted previous-step pressure in the continuity rate in
+ *                      place of dr.P0: nodal (nnodesP) or, with P_old_at_points,
+ *                      [nels][nq] at the quadrature points (2-D)
+ *   history            host buffer of history_cap rows (iter, err_v, err_v_x,
+ *                      err_v_y, err_v_z, err_P) filled at every outer and inner
+ *                      check (collect_history); *history_len receives the count
+ *                      (FP64 solves; rows beyond history_cap are dropped) */
+typedef struct {
+    const double* body_force[3];
+    const double* viscosity;
+    const double* scaling_viscosity;
+    const double* P_old;
+    int32_t P_old_at_points;
+    double* history;
+    int64_t history_cap;
+    int64_t* history_len;
+} fem_solve_inputs;
+int fem_stokes_set_solve_inputs(fem_stokes* st, const fem_solve_inputs* inputs);
+
+/* _set_pressure_source!(dr.Q, Qq, Q2D, ...) (pressure_residual.jl:32-77), run by
+ * solve_stokes_dyrel! when Qq is given (DR.jl:451-457): dr.Q = Qq (device,
+ * nnodesP) scaled so that ∫ Q dΩ = Q2D; has_Q2D = 0 copies Qq unscaled. Qq must
+ * be finite and nonnegative; zero support with a nonzero Q2D is an error. 2-D. */
+int fem_stokes_set_pressure_source(fem_stokes* st, const double* Qq, double Q2D, int32_t has_Q2D);
+
 /* n independent solve! calls (DR.jl:302-337) run concurrently. No counterpart
  * in the reference, which solves one state per call; for callers with many
  * small problems (parameter sweeps, ensembles). Each state must own its

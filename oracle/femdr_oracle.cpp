@@ -145,6 +145,8 @@ struct Rheo {
     double dt;
     const orc_cap* cap = nullptr;     // tensile cap in place of dp
     const double* gamma = nullptr;    // the element's accepted plastic strain [nq] (softening)
+    const double* eta_ip = nullptr;   // the element's viscosity at the points [nq] (_viscosity_at_q)
+    double eta_pt = 0;                // ... at the current point (set per point from eta_ip)
 };
 
 template <class S>
@@ -156,7 +158,8 @@ double stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, const
     S tr = (exx + eyy) / 3.0;
     double invG[ORC_MAX_PHASES];
     for (int p = 0; p < r.mat->nphases; ++p) invG[p] = 1.0 / r.mat->G[p];
-    const double etaq = interp_phase(Nv, NV, r.eta, ph);
+    // a sampled viscosity is used as given, phase tuples are interpolated (rheology.jl:1-9)
+    const double etaq = r.eta_ip ? r.eta_pt : interp_phase(Nv, NV, r.eta, ph);
     const double invGq = interp_phase(Nv, NV, invG, ph);
     const double etave = 1.0 / (1.0 / etaq + invGq / r.dt);
     const double inv2Gdt = invGq / (2 * r.dt);
@@ -460,8 +463,10 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
         const double Peq = Pq - Pfq;
         S tau[3];
         S Pe = S(Peq);   // the cap corrects the pressure the momentum balance uses
-        if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, r.gamma ? r.gamma + q : nullptr, tau, Pe);
-        else stress_ip<S>(gvx, gvy, Nv, NV, ph, r, to, Peq, tau);
+        Rheo rq = r;
+        if (r.eta_ip) rq.eta_pt = r.eta_ip[q];
+        if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, Peq, r.gamma ? r.gamma + q : nullptr, tau, Pe);
+        else stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, Peq, tau);
         const S Pq_local = Pe + Pfq;
         if (tau_out) { tau_out[0 * nq + q] = val(tau[0]); tau_out[1 * nq + q] = val(tau[1]); tau_out[2 * nq + q] = val(tau[2]); }
         if (Pq_out) Pq_out[q] = val(Pq_local);
@@ -486,7 +491,7 @@ void pressure_element(int NV, int NP, int nq, const double* dNdx, const double* 
                       const double* NqP, const S* vx, const S* vy, const double* P,
                       const double* P0, const double* T, const double* T0, const double* Q,
                       const int* phP, const double* alpha, const double* bulk, int nphases,
-                      double dt, S* RP) {
+                      double dt, S* RP, const double* P0q = nullptr /* [nq]: P_old at the points */) {
     bool all_inf = true;
     for (int p = 0; p < nphases; ++p) all_inf = all_inf && std::isinf(bulk[p]);
     for (int i = 0; i < NP; ++i) RP[i] = S(0.0);
@@ -498,7 +503,12 @@ void pressure_element(int NV, int NP, int nq, const double* dNdx, const double* 
         const double etabq = all_inf ? kInf : interp_phase(Nv, NP, bulk, phP);
         const double aq = interp_phase(Nv, NP, alpha, phP);
         double dP = 0, dT = 0;
-        for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
+        if (P0q) {   // pressure_increment_at_ip with IntegrationPointPressure (pressure_residual.jl:90-92)
+            for (int i = 0; i < NP; ++i) dP += Nv[i] * P[i];
+            dP -= P0q[q];
+        } else {
+            for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
+        }
         for (int i = 0; i < NP; ++i) dT += Nv[i] * (T[i] - T0[i]);
         const double dPdt = dP / (etabq * dt);
         const double dTdt = aq * dT / dt;
@@ -811,7 +821,8 @@ void orc_pressure_scaling(orc_stokes* s, double gamma_fact, double dt, int32_t f
         double MP[3] = {0, 0, 0}, gP[3] = {0, 0, 0};
         for (int q = 0; q < 7; ++q) {
             const double dO = s->dOmegaP[e * 7 + q];
-            const double etaq = interp_phase(t.Nq[q], 7, s->mat.eta, ph);
+            const double* eip = s->eta_scale_ip ? s->eta_scale_ip : s->eta_ip;
+            const double etaq = eip ? eip[e * 7 + q] : interp_phase(t.Nq[q], 7, s->mat.eta, ph);
             const double gnum = gamma_fact * etaq;
             const double bq = interp_phase(t.Nq[q], 7, invK, ph);
             const double gphy = bq == 0 ? gnum : dt / bq;
@@ -834,6 +845,7 @@ void orc_pressure_residual(orc_stokes* s, double dt, int32_t finite_K, const dou
     const Tables& t = tables();
     const double* bulk = pressure_bulk(s, finite_K);
     if (!P_old) P_old = s->P0;
+    const double* Pq_old = P_old == s->P0 ? s->Pold_ip : nullptr;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < s->nels; ++e) {
         double dNdx[7][7][2];
@@ -845,7 +857,8 @@ void orc_pressure_residual(orc_stokes* s, double dt, int32_t finite_K, const dou
         double RPe[3];
         pressure_element<double>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], vx, vy,
                                  s->P + e * 3, P_old + e * 3, s->T + e * 3, s->T0 + e * 3,
-                                 s->Q ? s->Q + e * 3 : nullptr, ph, s->mat.alpha, bulk, s->mat.nphases, dt, RPe);
+                                 s->Q ? s->Q + e * 3 : nullptr, ph, s->mat.alpha, bulk, s->mat.nphases, dt, RPe,
+                                 Pq_old ? Pq_old + e * 7 : nullptr);
         for (int a = 0; a < 3; ++a) s->RP[e * 3 + a] = RPe[a];
     }
 }
@@ -860,6 +873,7 @@ static void momentum_residual_impl(orc_stokes* s, double dt, const double* Pnum,
     for (int64_t e = 0; e < s->nels; ++e) {
         Rheo r = r0;
         if (s->cap && s->gamma_hist) r.gamma = s->gamma_hist + e * 7;
+        if (s->eta_ip) r.eta_ip = s->eta_ip + e * 7;
         double dNdx[7][7][2];
         elem_dNdx(s, e, dNdx);
         double vx[7], vy[7];
@@ -887,6 +901,43 @@ static void momentum_residual_impl(orc_stokes* s, double dt, const double* Pnum,
 
 void orc_momentum_residual(orc_stokes* s, double dt, const double* Pnum) {
     momentum_residual_impl(s, dt, Pnum, nullptr);
+    // _subtract_body_force! (DR.jl:1-7), before the Dirichlet rows (DR.jl:553, 686)
+    double* R[2] = {s->Rvx, s->Rvy};
+    for (int c = 0; c < 2; ++c)
+        if (s->body_force[c])
+            for (int64_t n = 0; n < s->nnodes; ++n) R[c][n] -= s->body_force[c][n];
+}
+
+// _set_pressure_source! and its area kernel (pressure_residual.jl:32-77)
+int orc_set_pressure_source(orc_stokes* s, const double* Qq, double Q2D, int32_t has_Q2D) {
+    const Tables& t = tables();
+    if (!has_Q2D) {
+        for (int64_t i = 0; i < s->nnodesP; ++i)
+            if (!std::isfinite(Qq[i])) { g_err = "Qq must be finite"; return -1; }
+        std::copy(Qq, Qq + s->nnodesP, s->Q);
+        return 0;
+    }
+    if (!std::isfinite(Q2D)) { g_err = "Q2D must be finite"; return -1; }
+    double area = 0;
+    int64_t invalid = 0;
+    for (int64_t e = 0; e < s->nels; ++e) {
+        const double* Ql = Qq + e * 3;
+        for (int i = 0; i < 3; ++i) invalid += !std::isfinite(Ql[i]) || Ql[i] < 0;
+        for (int q = 0; q < 7; ++q) {
+            double qv = 0;
+            for (int i = 0; i < 3; ++i) qv += t.NqP[q][i] * Ql[i];
+            invalid += !std::isfinite(qv) || qv < 0;
+            area += qv * s->dOmegaP[e * 7 + q];
+        }
+    }
+    if (invalid) { g_err = "Qq must be finite and nonnegative"; return -1; }
+    if (!(std::isfinite(area) && area > 0)) {
+        if (Q2D != 0) { g_err = "nonzero Q2D requires nonempty positive Qq support"; return -1; }
+        std::fill(s->Q, s->Q + s->nnodesP, 0.0);
+        return 0;
+    }
+    for (int64_t i = 0; i < s->nnodesP; ++i) s->Q[i] = Qq[i] * (Q2D / area);
+    return 0;
 }
 
 // update_stokes_current_stress! (DR.jl:820-897): momentum assembly with
@@ -916,6 +967,10 @@ void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K) {
         // _element_max_phase_property: every phase gets the element max.
         double emax = s->mat.eta[ph[0]];
         for (int a = 1; a < 7; ++a) emax = std::max(emax, s->mat.eta[ph[a]]);
+        if (s->eta_ip) {   // a sampled field: maximum over the element's points (momentum_residuals.jl:43)
+            emax = s->eta_ip[e * 7];
+            for (int q = 1; q < 7; ++q) emax = std::max(emax, s->eta_ip[e * 7 + q]);
+        }
         double eta_pc[ORC_MAX_PHASES];
         for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = emax;
         Rheo r{&s->mat, eta_pc, s->cap ? nullptr : s->dp, dt};
@@ -1054,6 +1109,16 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
     auto err_velocity = [&]() {
         return std::max(nrm2(Rv[0], nv), nrm2(Rv[1], nv)) / (2 * std::sqrt((double)nv));
     };
+    // collect_history (DR.jl:558-561, 722-727): (iter, err_v, per-component, 0, err_P)
+    if (s->history) s->history_len = 0;
+    auto history = [&](int64_t it, double ev) {
+        if (!s->history || s->history_len >= s->history_cap) return;
+        double* h = s->history + 6 * s->history_len++;
+        h[0] = (double)it; h[1] = ev;
+        h[2] = nrm2(Rv[0], nv) / (2 * std::sqrt((double)nv));
+        h[3] = nrm2(Rv[1], nv) / (2 * std::sqrt((double)nv));
+        h[4] = 0.0; h[5] = err_pressure();
+    };
 
     for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
         itPH_done = itPH;
@@ -1062,6 +1127,7 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
         for (int c = 0; c < 2; ++c) dirichlet(Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
         err_P = err_pressure();
         err_v = err_velocity();
+        history(iter, err_v);
         if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
         else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
         const double err_v_rel = err_v / err_v0, err_P_rel = err_P / err_P0;
@@ -1099,6 +1165,7 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
                 const double evi = err_velocity();
                 if (iter == ncheck) err_v00 = evi + julia_eps(evi);
                 err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                history(iter, evi);
                 if (std::isnan(err)) { g_err = "NaN detected in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -4; return -4; }
                 if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -5; return -5; }
                 double lmin[2];

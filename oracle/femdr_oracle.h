@@ -101,6 +101,13 @@ typedef struct {
     double* gamma_hist;       /* nullable [nels][7]: accepted plastic strain γ (softens C) */
     double* theta_hist;       /* nullable [nels][7]: accepted volumetric plastic strain θ */
     double* P_store;          /* nullable [nels][7]: pressure of the stress update (cap-corrected) */
+    /* array keywords of solve! (DR.jl:183-186, 217-226, 302-337, 556-561, 722-727), nullable */
+    const double* body_force[2];  /* nodal loads subtracted from Rv before the Dirichlet rows */
+    const double* eta_ip;         /* [nels][7] viscosity at the points (purely viscous flow) */
+    const double* eta_scale_ip;   /* [nels][7] viscosity of the pressure scale; null: eta_ip */
+    const double* Pold_ip;        /* [nels][7] previous-step pressure at the points (continuity) */
+    double* history;              /* rows (iter, err_v, err_v_x, err_v_y, 0, err_P) of every check */
+    int64_t history_cap, history_len;
 } orc_stokes;
 
 /* solve!/_solve_stokes_dyrel! keyword parameters (src/stokes/solvers/DR.jl:302-386) */
@@ -186,6 +193,9 @@ void orc_dr_iterations(orc_stokes* s, double dt, int32_t finite_K, const double*
 
 /* --- full solve (src/stokes/solvers/DR.jl:302-337, 348-802) --- */
 int orc_stokes_solve(orc_stokes* s, const orc_params* p, orc_stats* out);
+/* _set_pressure_source! (pressure_residual.jl:32-77): s->Q = Qq scaled to
+ * ∫ Q dΩ = Q2D (has_Q2D = 0: copied); returns 0, or -1 with orc_last_error() */
+int orc_set_pressure_source(orc_stokes* s, const double* Qq, double Q2D, int32_t has_Q2D);
 /* update_stokes_current_stress! (DR.jl:820-897): tau_store must be set */
 void orc_update_current_stress(orc_stokes* s, double dt);
 

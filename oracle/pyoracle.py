@@ -62,7 +62,9 @@ class Stokes(C.Structure):
                 ("Rv0x", PD), ("Rv0y", PD), ("dRdvx", PD), ("dRdvy", PD), ("PCx", PD), ("PCy", PD),
                 ("P", PD), ("P0", PD), ("gammaP", PD), ("T", PD), ("T0", PD), ("Q", PD), ("RP", PD),
                 ("MP", PD), ("Pnum", PD), ("Pf", PD), ("tau_old", PD * 3), ("tau_store", PD * 3),
-                ("nbc", I64 * 2), ("bc_dofs", PI * 2), ("bc_vals", PD * 2), ("cap", C.POINTER(Cap)), ("gamma_hist", PD), ("theta_hist", PD), ("P_store", PD)]
+                ("nbc", I64 * 2), ("bc_dofs", PI * 2), ("bc_vals", PD * 2), ("cap", C.POINTER(Cap)), ("gamma_hist", PD), ("theta_hist", PD), ("P_store", PD),
+                ("body_force", PD * 2), ("eta_ip", PD), ("eta_scale_ip", PD), ("Pold_ip", PD),
+                ("history", PD), ("history_cap", I64), ("history_len", I64)]
 
 
 class Params(C.Structure):
@@ -144,6 +146,8 @@ def _declare(L):
     L.orc_pressure_residual3.argtypes = [C.POINTER(Stokes3), D, I32]
     L.orc_momentum_residual3.argtypes = [C.POINTER(Stokes3), D, PD]
     L.orc_stokes_solve3.argtypes = [C.POINTER(Stokes3), C.POINTER(Params), C.POINTER(Stats)]
+    L.orc_set_pressure_source.argtypes = [C.POINTER(Stokes), PD, D, I32]
+    L.orc_set_pressure_source.restype = C.c_int
     L.orc_stokes_solve3.restype = C.c_int
     L.orc_update_current_stress3.argtypes = [C.POINTER(Stokes3), D]
     L.orc_cp3_pressure_residual.argtypes = [C.POINTER(Stokes3), PD]
@@ -318,6 +322,13 @@ class Problem:
     arrays: dict = field(default_factory=dict)
     cap: Cap | None = None
     history: bool = False
+    # array keywords of solve! (DR.jl:183-186, 217-226, 302-337): nodal loads
+    # (bf_x, bf_y), viscosity at the points [nels][7], viscosity of the pressure
+    # scale [nels][7], previous-step pressure at the points [nels][7]
+    body_force: tuple | None = None
+    eta_ip: np.ndarray | None = None
+    eta_scale_ip: np.ndarray | None = None
+    Pold_ip: np.ndarray | None = None
 
     def __post_init__(self):
         self.coords = f64(self.coords)
@@ -375,8 +386,25 @@ class Problem:
         s.gamma_hist = ptr(self.gamma_hist) if self.gamma_hist is not None else ptr(None)
         s.theta_hist = ptr(self.theta_hist) if self.theta_hist is not None else ptr(None)
         s.P_store = ptr(self.P_store)
+        self._kw = [None if a is None else f64(a) for a in
+                    (*(self.body_force or (None, None)), self.eta_ip, self.eta_scale_ip, self.Pold_ip)]
+        bfx, bfy, eip, esc, pold = self._kw
+        s.body_force[0], s.body_force[1] = ptr(bfx) if bfx is not None else ptr(None), \
+            ptr(bfy) if bfy is not None else ptr(None)
+        s.eta_ip = ptr(eip) if eip is not None else ptr(None)
+        s.eta_scale_ip = ptr(esc) if esc is not None else ptr(None)
+        s.Pold_ip = ptr(pold) if pold is not None else ptr(None)
+        s.history, s.history_cap, s.history_len = ptr(None), 0, 0
         self._struct = s
         return s
+
+    def set_pressure_source(self, Qq, Q2D=None):
+        """_set_pressure_source! (pressure_residual.jl:32-77) into Q."""
+        q = f64(Qq)
+        rc = lib().orc_set_pressure_source(C.byref(self.struct()), ptr(q), 0.0 if Q2D is None else float(Q2D),
+                                           int(Q2D is not None))
+        if rc != 0:
+            raise ValueError(lib().orc_last_error().decode())
 
     def commit_plastic_history(self, dt):
         """commit_stokes_plastic_history! (DR.jl:899-958)."""
@@ -428,13 +456,22 @@ class Problem:
                       rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
                       freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
 
-    def solve(self, dt, **kw):
+    def solve(self, dt, collect_history=False, **kw):
         p = self.params(dt, **kw)
         st = Stats()
-        rc = lib().orc_stokes_solve(C.byref(self.struct()), C.byref(p), C.byref(st))
+        s = self.struct()
+        hist = None
+        if collect_history:
+            cap = 2 * (p.max_iterations // max(p.check_interval, 1) + p.max_ph_iterations) + 4
+            hist = np.zeros((min(cap, 1_000_000), 6))
+            s.history, s.history_cap = ptr(hist), len(hist)
+        rc = lib().orc_stokes_solve(C.byref(s), C.byref(p), C.byref(st))
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
-        return st.as_dict()
+        out = st.as_dict()
+        if collect_history:
+            out["history"] = hist[:s.history_len].copy()
+        return out
 
 
 def drucker_prager_cap(phi, psi, C_, pT, eta_reg, Kb, C_min=None, H_C=None):
