@@ -1049,6 +1049,7 @@ struct CP3Args {
     const double *bcval_x, *bcval_y, *bcval_z;
     double *Rv_x, *Rv_y, *Rv_z;    // nullable: the residual of the centre nodes
     int32_t update;                // pressure pass: P += ps RP/M; momentum pass: the centre node's step
+    const double *bf_x, *bf_y, *bf_z;   // nullable: a momentum load subtracted before the Dirichlet mask
 };
 
 // the modes (1, ξ, η, ζ) at a point (pressure_mode_table, momentum_residuals.jl:618-621)
@@ -1208,6 +1209,7 @@ __global__ __launch_bounds__(HBLOCK) void k_cp3_momentum(CP3Args A) {
         return;
     }
     const uint8_t fl = A.bcflag[node];
+    if (A.bf_x) { R[0] -= A.bf_x[node]; R[1] -= A.bf_y[node]; R[2] -= A.bf_z[node]; }
     const double rx = (fl & 1) ? 0.0 : R[0], ry = (fl & 2) ? 0.0 : R[1], rz = (fl & 4) ? 0.0 : R[2];
     if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
     if (A.update) {
@@ -1282,7 +1284,8 @@ int cp3_prepare(fem_stokes* s) {
 
 // the shared nodes' sums of the element slots: into the diagonal
 // (preconditioner) or, with the Jacobi step, the velocity
-int cp3_gather(fem_stokes* s, double* const out[3], bool step, double vs, bool store_R) {
+int cp3_gather(fem_stokes* s, double* const out[3], bool step, double vs, bool store_R,
+               double* const* v = nullptr, const double* const* load = nullptr) {
     fem_mesh* m = s->mesh;
     double** f = s->f;
     Gather3Args g;
@@ -1292,7 +1295,8 @@ int cp3_gather(fem_stokes* s, double* const out[3], bool step, double vs, bool s
     g.slot = s->d_slot3;
     g.bcflag = s->d_bcflag;
     if (step) {
-        g.vx = f[FEM_FIELD_V_X]; g.vy = f[FEM_FIELD_V_Y]; g.vz = f[FEM_FIELD_V_Z];
+        g.vx = v ? v[0] : f[FEM_FIELD_V_X]; g.vy = v ? v[1] : f[FEM_FIELD_V_Y]; g.vz = v ? v[2] : f[FEM_FIELD_V_Z];
+        if (load) { g.bf_x = load[0]; g.bf_y = load[1]; g.bf_z = load[2]; }
         g.rate_x = f[FEM_FIELD_DVDTAU_X]; g.rate_y = f[FEM_FIELD_DVDTAU_Y]; g.rate_z = f[FEM_FIELD_DVDTAU_Z];
         g.PCx = f[FEM_FIELD_PC_V_X]; g.PCy = f[FEM_FIELD_PC_V_Y]; g.PCz = f[FEM_FIELD_PC_V_Z];
         g.bcval_x = s->d_bcval_x; g.bcval_y = s->d_bcval_y; g.bcval_z = s->d_bcval_z;
@@ -1352,10 +1356,26 @@ int fem_cell_stokes_momentum_residual(fem_stokes* s) {
     return fem_sync(c);
 }
 
-// solve!(dr::CellPressureStokesDR, mesh, bc_v; ...) (DR.jl:980-1057)
-int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_solve_params* p,
-                          fem_cell_solve_stats* st, double* history, int64_t history_cap, int64_t* history_len) {
-    if (!s || !bcs || !p || !st) return FEM_EARG;
+}  // extern "C"
+
+// Σ_i a_x b_x + a_y b_y + a_z b_z over the nodes (deterministic block partials)
+__global__ __launch_bounds__(RED_BLOCK) void k_dot3(int64_t n, const double* ax, const double* ay, const double* az,
+                                                    const double* bx, const double* by, const double* bz,
+                                                    double* partial) {
+    double acc[1] = {0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride)
+        acc[0] += ax[i] * bx[i] + ay[i] * by[i] + az[i] * bz[i];
+    block_sum_store<1>(acc, partial);
+}
+
+// _relax_cell_pressure_stokes! (DR.jl:997-1057) on the arrays v (3 nodal) and
+// P ([nels][4]) with the state's viscosity, phases and scratch: gravity on or
+// off (the adjoint runs with zero density), an optional momentum load
+// subtracted before the Dirichlet rows, the Dirichlet values of bcs
+static int cp3_relax(fem_stokes* s, double* const v[3], double* P, bool gravity, const double* const load[3],
+                     const fem_bc bcs[3], const fem_cell_solve_params* p, fem_cell_solve_stats* st, double* history,
+                     int64_t history_cap, int64_t* history_len, const char* what) {
     fem_ctx* c = s->ctx;
     std::memset(st, 0, sizeof *st);
     if (history_len) *history_len = 0;
@@ -1370,7 +1390,7 @@ int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_sol
     double** f = s->f;
     hipLaunchKernelGGL(k_pin3, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
                        (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, (const double*)s->d_bcval_z,
-                       f[FEM_FIELD_V_X], f[FEM_FIELD_V_Y], f[FEM_FIELD_V_Z]);
+                       v[0], v[1], v[2]);
     CHK(check_launch(c, "k_pin3"));
     // Σ_cells M_0 in cell order on the host (a constant of the solve)
     std::vector<double> hM(np);
@@ -1382,6 +1402,10 @@ int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_sol
     double* part = nullptr;
     CHK(dalloc_t(c, &part, nb + 1));
     CP3Args a = cp3_args(s);
+    a.vx = v[0]; a.vy = v[1]; a.vz = v[2];
+    a.P = P;
+    if (!gravity) a.gx = a.gy = a.gz = 0;
+    if (load) { a.bf_x = load[0]; a.bf_y = load[1]; a.bf_z = load[2]; }
     a.ps = p->pressure_step;
     a.vs = p->velocity_step;
     a.part = part;
@@ -1403,7 +1427,7 @@ int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_sol
         if (chk) { a.Rv_x = R[0]; a.Rv_y = R[1]; a.Rv_z = R[2]; }
         hipLaunchKernelGGL(k_cp3_momentum, dim3(nb), dim3(HBLOCK), 0, c->stream, a);
         rc = check_launch(c, "k_cp3_momentum");
-        if (rc == FEM_OK) rc = cp3_gather(s, R, true, p->velocity_step, chk);
+        if (rc == FEM_OK) rc = cp3_gather(s, R, true, p->velocity_step, chk, v, load);
         if (rc != FEM_OK || !chk) continue;
         const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, np), RED_BLOCK));
         hipLaunchKernelGGL(k_cp3_norms, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np, (const double*)R[0],
@@ -1417,7 +1441,7 @@ int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_sol
         err_P = std::sqrt(c->h_red[0]);
         err_v = std::max(std::sqrt(c->h_red[1]), std::max(std::sqrt(c->h_red[2]), std::sqrt(c->h_red[3])));
         err = std::max(err_v, err_P);
-        if (!std::isfinite(err)) { rc = set_err(c, FEM_ENAN, "non-finite residual in 3-D Stokes solve"); break; }
+        if (!std::isfinite(err)) { rc = set_err(c, FEM_ENAN, "non-finite residual in %s", what); break; }
         if (history && nh < history_cap) {
             history[nh * 3] = (double)it; history[nh * 3 + 1] = err_v; history[nh * 3 + 2] = err_P;
             ++nh;
@@ -1437,10 +1461,109 @@ int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_sol
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc != FEM_OK) return rc;
     if (!st->converged && p->throw_on_failure)
-        return set_err(c, FEM_ENOCONV, "3-D Stokes solve did not converge after %lld iterations (residual = %g, "
-                       "tolerance = %g)", (long long)iter, err, p->tolerance);
+        return set_err(c, FEM_ENOCONV, "%s did not converge after %lld iterations (residual = %g, "
+                       "tolerance = %g)", what, (long long)iter, err, p->tolerance);
     return FEM_OK;
 }
+
+extern "C" {
+
+// solve!(dr::CellPressureStokesDR, mesh, bc_v; ...) (DR.jl:980-996)
+int fem_cell_stokes_solve(fem_stokes* s, const fem_bc bcs[3], const fem_cell_solve_params* p,
+                          fem_cell_solve_stats* st, double* history, int64_t history_cap, int64_t* history_len) {
+    if (!s || !bcs || !p || !st) return FEM_EARG;
+    double* v[3] = {s->f[FEM_FIELD_V_X], s->f[FEM_FIELD_V_Y], s->f[FEM_FIELD_V_Z]};
+    return cp3_relax(s, v, s->f[FEM_FIELD_P], true, nullptr, bcs, p, st, history, history_cap, history_len,
+                     "3-D Stokes solve");
+}
+
+// stokes_material_gradient_3d (DR_adjoint.jl:483-531): for every phase p,
+// −λ·R(v = 0, P = 0, ρ = e_p, g) and −λ·R(v_fwd, P = 0, η = e_p, ρ = 0), R the
+// assembled momentum residual without Dirichlet rows; v_fwd NULL: the state's v
+int fem_cell_stokes_material_gradient(fem_stokes* s, const double* const v_fwd[3], const double* const lam_v[3],
+                                      double* density_gradient, double* viscosity_gradient) {
+    if (!s || !lam_v || !density_gradient || !viscosity_gradient) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    CHK(cp3_prepare(s));
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = HNP * m->nels;
+    double** f = s->f;
+    const double* vf[3] = {v_fwd ? v_fwd[0] : f[FEM_FIELD_V_X], v_fwd ? v_fwd[1] : f[FEM_FIELD_V_Y],
+                           v_fwd ? v_fwd[2] : f[FEM_FIELD_V_Z]};
+    double *zv = nullptr, *zP = nullptr;
+    CHK(dalloc_t(c, &zv, nv));
+    int rc = dalloc_t(c, &zP, np);
+    if (rc == FEM_OK && (hipMemsetAsync(zv, 0, nv * 8, c->stream) != hipSuccess ||
+                         hipMemsetAsync(zP, 0, np * 8, c->stream) != hipSuccess ||
+                         hipMemsetAsync(s->d_bcflag, 0, nv, c->stream) != hipSuccess))
+        rc = set_err(c, FEM_EHIP, "clearing the gradient work arrays failed");
+    const fem_material saved = s->mat;
+    double* R[3] = {f[FEM_FIELD_RV_X], f[FEM_FIELD_RV_Y], f[FEM_FIELD_RV_Z]};
+    const int nph = saved.nphases;
+    for (int which = 0; which < 2 && rc == FEM_OK; ++which)
+        for (int p = 0; p < nph && rc == FEM_OK; ++p) {
+            s->mat = saved;
+            for (int q = 0; q < nph; ++q) {
+                if (which == 0) s->mat.rho0[q] = q == p ? 1.0 : 0.0;
+                else { s->mat.eta[q] = q == p ? 1.0 : 0.0; s->mat.rho0[q] = 0.0; }
+            }
+            rc = cp3_prepare(s);
+            if (rc != FEM_OK) break;
+            CP3Args a = cp3_args(s);
+            a.P = zP;
+            if (which == 0) { a.vx = zv; a.vy = zv; a.vz = zv; }
+            else {
+                a.vx = const_cast<double*>(vf[0]); a.vy = const_cast<double*>(vf[1]); a.vz = const_cast<double*>(vf[2]);
+                a.gx = a.gy = a.gz = 0;
+            }
+            a.Rv_x = R[0]; a.Rv_y = R[1]; a.Rv_z = R[2];
+            hipLaunchKernelGGL(k_cp3_momentum, dim3(grid_xcd(m->nels, HEPB)), dim3(HBLOCK), 0, c->stream, a);
+            rc = check_launch(c, "k_cp3_momentum");
+            if (rc == FEM_OK) rc = cp3_gather(s, R, false, 0, true);
+            if (rc != FEM_OK) break;
+            const unsigned g = std::min<unsigned>(RED_GRID, grid_for(nv, RED_BLOCK));
+            hipLaunchKernelGGL(k_dot3, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, lam_v[0], lam_v[1], lam_v[2],
+                               (const double*)R[0], (const double*)R[1], (const double*)R[2], c->red_partial);
+            hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 1, 0,
+                               (const double*)c->red_partial, c->red_out);
+            rc = check_launch(c, "k_dot3");
+            if (rc != FEM_OK) break;
+            if (hipMemcpyAsync(c->h_red, c->red_out, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = set_err(c, FEM_EHIP, "reading the gradient reduction failed");
+                break;
+            }
+            (which == 0 ? density_gradient : viscosity_gradient)[p] = -c->h_red[0];
+        }
+    s->mat = saved;
+    const int rc2 = cp3_prepare(s);
+    hipStreamSynchronize(c->stream);
+    hipFree(zv);
+    hipFree(zP);
+    return rc != FEM_OK ? rc : rc2;
+}
+
+// solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP) (DR_adjoint.jl:446-480)
+int fem_cell_stokes_solve_adjoint(fem_stokes* s, const fem_bc bcs[3], const double* const objective[3],
+                                  double* const lam_v[3], double* lam_P, const fem_cell_solve_params* p,
+                                  fem_cell_solve_stats* st, double* history, int64_t history_cap,
+                                  int64_t* history_len) {
+    if (!s || !bcs || !objective || !lam_v || !lam_P || !p || !st) return FEM_EARG;
+    for (int k = 0; k < 3; ++k)
+        if (!objective[k] || !lam_v[k]) return set_err(s->ctx, FEM_EARG, "objective_v and λv take three arrays");
+    // homogeneous conditions on the same nodes
+    std::vector<double> zeros[3];
+    fem_bc zb[3];
+    for (int k = 0; k < 3; ++k) {
+        zeros[k].assign((size_t)std::max<int64_t>(bcs[k].n, 0), 0.0);
+        zb[k] = bcs[k];
+        zb[k].host_vals = zeros[k].data();
+    }
+    return cp3_relax(s, lam_v, lam_P, false, objective, zb, p, st, history, history_cap, history_len,
+                     "3-D Stokes adjoint solve");
+}
+
 
 }  // extern "C"
 

@@ -30,6 +30,7 @@ __all__ = [
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
+    "solve_adjoint", "stokes_material_gradient_3d",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
 ]
@@ -212,6 +213,9 @@ _sig("fem_stokes_jacobian_diagnostics_cap", C.c_int, VP, _P(_Cap), D, I32)
 _sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
+_sig("fem_cell_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(VP), _P(VP), VP, _P(_CellParams), _P(_CellStats),
+     _P(D), I64, _P(I64))
+_sig("fem_cell_stokes_material_gradient", C.c_int, VP, _P(VP), _P(VP), _P(D), _P(D))
 _sig("fem_stokes_set_pressure_source", C.c_int, VP, VP, D, I32)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
@@ -230,7 +234,7 @@ EXPORTED = [
     "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
     "fem_stokes_commit_plastic_history", "fem_cell_stokes_preconditioner", "fem_cell_stokes_pressure_residual",
     "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve", "fem_stokes_set_solve_inputs",
-    "fem_stokes_set_pressure_source",
+    "fem_stokes_set_pressure_source", "fem_cell_stokes_solve_adjoint", "fem_cell_stokes_material_gradient",
 ]
 
 
@@ -1336,3 +1340,75 @@ def solve_cell_pressure(dr: CellPressureStokesDR, mesh: MixedMesh, bc_vx, bc_vy,
     out["history"] = [dict(iter=int(h[0]), err_v=h[1], err_P=h[2]) for h in hist[:nh.value]]
     dr.ctx.check(rc)
     return out
+
+
+def _cell_result(st, hist, nh):
+    out = {k: getattr(st, k) for k, _ in _CellStats._fields_}
+    out["converged"] = bool(out["converged"])
+    out["reached_total_iter"] = bool(out["reached_total_iter"])
+    out["iter"] = out["iterations"]
+    out["err"] = out["residual"]
+    out["history"] = [dict(iter=int(h[0]), err_v=h[1], err_P=h[2]) for h in hist[:nh.value]]
+    return out
+
+
+def solve_adjoint(dr: CellPressureStokesDR, mesh: MixedMesh, bc_vx, bc_vy, bc_vz, *, objective_v, lam_v, lam_P,
+                  tolerance=1e-5, max_iterations=3000, check_interval=100, velocity_step=0.6, pressure_step=0.2,
+                  verbose=False, collect_history=False, throw_on_failure=True):
+    """solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP, ...)
+    (DR_adjoint.jl:446-480): the viscous operator is symmetric, so the forward
+    relaxation runs on (lam_v, lam_P) with objective_v as momentum load, zero
+    density and gravity and homogeneous conditions on the nodes of the bcs.
+    lam_v (three nodal arrays) and lam_P ((nels, 4)) are DeviceArrays updated
+    in place (their values are the initial iterate), or host arrays (then
+    the results are returned). dr.v and dr.P are unchanged."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    nv, ne = mesh.nnodes, mesh.nels
+    if len(objective_v) != 3 or any(_size(o) != nv for o in objective_v):
+        raise DimensionMismatch("objective_v must have mesh.nnodes entries per component")
+    if len(lam_v) != 3 or any(_size(o) != nv for o in lam_v) or _size(lam_P) != 4 * ne:
+        raise DimensionMismatch("velocity must have mesh.nnodes entries per component and pressure must be 4 × nels")
+    keep = []
+    arr = (_BC * 3)(*(b._c(keep) for b in (bc_vx, bc_vy, bc_vz)))
+    p = _CellParams(tolerance=float(tolerance), max_iterations=int(max_iterations),
+                    check_interval=int(check_interval), velocity_step=float(velocity_step),
+                    pressure_step=float(pressure_step), verbose=int(bool(verbose)),
+                    throw_on_failure=int(bool(throw_on_failure)))
+    st = _CellStats()
+    cap = (int(max_iterations) // max(int(check_interval), 1) + 2) if collect_history else 0
+    hist = np.zeros((max(cap, 1), 3))
+    nh = I64(0)
+    with _Inputs(dr) as io:
+        obj = (VP * 3)(*(io._dev(o, (nv,)) for o in objective_v))
+        lv = [a if isinstance(a, DeviceArray) else None for a in lam_v]
+        lvp = (VP * 3)(*(io._dev(a, (nv,)) for a in lam_v))
+        lp = io._dev(lam_P, (ne, 4))
+        rc = _lib.fem_cell_stokes_solve_adjoint(dr.handle, arr, obj, lvp, lp, C.byref(p), C.byref(st),
+                                                hist.ctypes.data_as(_P(D)) if cap else C.cast(None, _P(D)), cap,
+                                                C.byref(nh))
+        out = _cell_result(st, hist, nh)
+        if rc == FEM_OK or rc == FEM_ENOCONV:
+            out["lam_v"] = tuple(a if a is not None else DeviceArray(dr.ctx, lvp[k], (nv,)).numpy()
+                                 for k, a in enumerate(lv))
+            out["lam_P"] = lam_P if isinstance(lam_P, DeviceArray) else DeviceArray(dr.ctx, lp, (ne, 4)).numpy()
+    dr.ctx.check(rc)
+    return out
+
+
+def stokes_material_gradient_3d(dr: CellPressureStokesDR, mesh: MixedMesh, adjoint_velocity, forward_velocity=None):
+    """stokes_material_gradient_3d(dr, mesh, adjoint_velocity) (DR_adjoint.jl:483-531):
+    dict(density_gradient, viscosity_gradient), one entry per phase; the forward
+    velocity is dr.v unless forward_velocity (three nodal arrays) is given."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    nv = mesh.nnodes
+    nph = dr.material.nphases if hasattr(dr.material, "nphases") else len(dr.material.eta)
+    dens = np.zeros(nph)
+    visc = np.zeros(nph)
+    with _Inputs(dr) as io:
+        lam = (VP * 3)(*(io._dev(a, (nv,)) for a in adjoint_velocity))
+        fwd = (VP * 3)(*(io._dev(a, (nv,)) for a in forward_velocity)) if forward_velocity is not None else None
+        dr.ctx.check(_lib.fem_cell_stokes_material_gradient(dr.handle, fwd, lam, dens.ctypes.data_as(_P(D)),
+                                                            visc.ctypes.data_as(_P(D))))
+    return dict(density_gradient=tuple(dens), viscosity_gradient=tuple(visc))

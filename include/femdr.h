@@ -453,6 +453,24 @@ typedef struct {
 int fem_cell_stokes_solve(fem_stokes* st, const fem_bc bcs[3], const fem_cell_solve_params* params,
                           fem_cell_solve_stats* stats, double* history, int64_t history_cap, int64_t* history_len);
 
+/* solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP, ...)
+ * (src/stokes/solvers/DR_adjoint.jl:446-480): the same relaxation on the
+ * caller's device arrays lam_v (three nodal) and lam_P ([nels][4]), whose
+ * values are the initial iterate, with objective (three nodal device arrays)
+ * as momentum load, zero density and gravity and homogeneous conditions on
+ * the nodes of bcs (their values are not used). dr.v and dr.P are untouched;
+ * the state's scratch (Rv, RP, PC_v, M_P) is overwritten. */
+int fem_cell_stokes_solve_adjoint(fem_stokes* st, const fem_bc bcs[3], const double* const objective[3],
+                                  double* const lam_v[3], double* lam_P, const fem_cell_solve_params* params,
+                                  fem_cell_solve_stats* stats, double* history, int64_t history_cap,
+                                  int64_t* history_len);
+/* stokes_material_gradient_3d (DR_adjoint.jl:483-531): per phase p,
+ * density_gradient[p] = −λ·R(v = 0, P = 0, ρ = e_p) and viscosity_gradient[p] =
+ * −λ·R(v_fwd, P = 0, η = e_p, ρ = 0) with R the assembled momentum residual;
+ * v_fwd (three nodal device arrays) NULL: dr.v. Host outputs of nphases. */
+int fem_cell_stokes_material_gradient(fem_stokes* st, const double* const v_fwd[3], const double* const lam_v[3],
+                                      double* density_gradient, double* viscosity_gradient);
+
 /* ------------------------------------------------- coupled thermal--Stokes */
 /* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
  * phi (device, nq x nels = [nels][7] f64) receives τ : (ε̇ − (τ − τ_old)/(2GΔt))

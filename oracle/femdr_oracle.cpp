@@ -2381,6 +2381,9 @@ int orc_cp3_solve(orc_stokes3* s, const orc_cp3_params* p, orc_cp3_stats* out, d
         pm /= sumM;
         for (int64_t e = 0; e < s->nels; ++e) s->P[e * 4] -= pm;
         orc_cp3_momentum_residual(s, s->Rv);
+        for (int c = 0; c < 3; ++c)
+            if (s->body_force[c])
+                for (int64_t i = 0; i < nv; ++i) s->Rv[c][i] -= s->body_force[c][i];
         for (int c = 0; c < 3; ++c) {
             dirichlet(s->Rv[c], s->nbc[c], s->bc_dofs[c], nullptr);
             for (int64_t i = 0; i < nv; ++i) s->v[c][i] -= p->velocity_step * s->Rv[c][i] / s->PC[c][i];

@@ -220,6 +220,9 @@ typedef struct {
     int64_t nbc[3];
     const int32_t* bc_dofs[3];
     const double* bc_vals[3];
+    /* nullable: a momentum load subtracted in the cell-pressure relaxation
+     * (_subtract_body_force!, DR.jl:1034; the adjoint's objective) */
+    const double* body_force[3];
 } orc_stokes3;
 
 /* reference Hex27 (shape_functions.jl:411-446, integration_points.jl:321-337) */

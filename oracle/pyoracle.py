@@ -128,7 +128,7 @@ class Stokes3(C.Structure):
                 ("v", PD * 3), ("rate", PD * 3), ("Rv", PD * 3), ("Rv0", PD * 3), ("dRdv", PD * 3), ("PC", PD * 3),
                 ("P", PD), ("P0", PD), ("gammaP", PD), ("RP", PD), ("MP", PD), ("Pnum", PD),
                 ("tau_old", PD * 6), ("tau_store", PD * 6), ("nbc", I64 * 3), ("bc_dofs", PI * 3),
-                ("bc_vals", PD * 3)]
+                ("bc_vals", PD * 3), ("body_force", PD * 3)]
 
 
 def _declare(L):
@@ -696,6 +696,7 @@ class Problem3:
         self.tau_store = [np.zeros((ne, 27)) for _ in range(6)]
         e = (np.zeros(0, np.int32), np.zeros(0))
         self.bc = tuple((i32(d), f64(v)) for d, v in (bc if bc is not None else (e, e, e)))
+        self.body_force = None   # nullable: 3 nodal loads of the cell-pressure relaxation
 
     def struct(self):
         s = Stokes3()
@@ -718,8 +719,58 @@ class Problem3:
         for c in range(3):
             s.nbc[c] = len(self.bc[c][0])
             s.bc_dofs[c], s.bc_vals[c] = ptr(self.bc[c][0], PI), ptr(self.bc[c][1])
+        self._bf = None if self.body_force is None else [f64(b) for b in self.body_force]
+        for c in range(3):
+            s.body_force[c] = ptr(self._bf[c]) if self._bf is not None else ptr(None)
         self._struct = s
         return s
+
+    def cp3_solve_adjoint(self, objective, lam_v=None, lam_P=None, **kw):
+        """solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP)
+        (DR_adjoint.jl:446-480): the relaxation on (λv, λP) with the objective as
+        momentum load, zero density and gravity, homogeneous conditions on the
+        constrained nodes; v, P and g of this problem are left as they were.
+        Returns (λv, λP, stats)."""
+        lam_v = [np.zeros(self.nnodes) if lam_v is None else f64(lam_v[c]).copy() for c in range(3)]
+        lam_P = np.zeros(4 * self.nels) if lam_P is None else f64(lam_P).reshape(-1).copy()
+        saved = (self.v, self.P, self.g, self.bc, self.body_force)
+        try:
+            self.v, self.P, self.g = lam_v, lam_P, (0.0, 0.0, 0.0)
+            self.bc = tuple((d, np.zeros(len(d))) for d, _ in self.bc)
+            self.body_force = [f64(o) for o in objective]
+            st = self.cp3_solve(**kw)
+        finally:
+            self.v, self.P, self.g, self.bc, self.body_force = saved
+        return lam_v, lam_P, st
+
+    def cp3_material_gradient(self, v_fwd, lam_v):
+        """stokes_material_gradient_3d (DR_adjoint.jl:483-531): per phase
+        -λ·R(v=0, P=0, ρ=e_p) and -λ·R(v_fwd, P=0, η=e_p, ρ=0, g=0)."""
+        n = self.mat.nphases
+        saved = (self.v, self.P, self.g, Material.from_buffer_copy(self.mat))
+        dens, visc = [], []
+        try:
+            self.P = np.zeros(4 * self.nels)
+            for which in ("rho", "eta"):
+                for p in range(n):
+                    m = Material.from_buffer_copy(saved[3])
+                    for q in range(n):
+                        if which == "rho":
+                            m.rho0[q] = 1.0 if q == p else 0.0
+                        else:
+                            m.eta[q] = 1.0 if q == p else 0.0
+                            m.rho0[q] = 0.0
+                    self.mat = m
+                    if which == "rho":
+                        self.v, self.g = [np.zeros(self.nnodes) for _ in range(3)], saved[2]
+                    else:
+                        self.v, self.g = [f64(x) for x in v_fwd], (0.0, 0.0, 0.0)
+                    R = self.cp3_momentum_residual()
+                    val = -sum(float(np.dot(lam_v[c], R[c])) for c in range(3))
+                    (dens if which == "rho" else visc).append(val)
+        finally:
+            self.v, self.P, self.g, self.mat = saved
+        return dens, visc
 
     def pressure_scaling(self, gamma_fact, dt, finite_K=False):
         lib().orc_pressure_scaling3(C.byref(self.struct()), gamma_fact, dt, int(finite_K))

@@ -129,3 +129,45 @@ def test_gpu_cell_pressure_constructor_checks():
         fa.CellPressureStokesDR(mesh, fa.StokesMaterial(eta=1.0, G=1.0, g=(0.0, 0.0, 0.0)))
     with pytest.raises(ValueError, match="requires a Hex27 mesh"):
         fa.CellPressureStokesDR(fa.t7_box_mesh(1, 1), fa.StokesMaterial(eta=1.0))
+
+
+def test_gpu_cell_pressure_adjoint_and_material_gradient():
+    # solve_adjoint! / stokes_material_gradient_3d (DR_adjoint.jl:446-531)
+    # against the oracle (itself pinned to the transposed system and finite
+    # differences): iteration counts, λ to 1e-10, gradients to 1e-10
+    say("[gpu] cell-pressure adjoint and material gradient")
+    n = (4, 3, 3)
+    pr, dr, mesh, bcs = pair(n)
+    kw = dict(tolerance=1e-9, max_iterations=60_000, check_interval=100)
+    so = pr.cp3_solve(**kw)
+    st = fa.solve(dr, mesh, *bcs, **kw)
+    assert so["converged"] and st["converged"] and st["iterations"] == so["iterations"]
+    v_fwd = [getattr(dr.v, c).numpy() for c in "xyz"]
+    P_fwd = dr.P.numpy()
+    rng = np.random.default_rng(5)
+    obj = [rng.normal(size=pr.nnodes) for _ in range(3)]
+    lo, lPo, sao = pr.cp3_solve_adjoint(obj, **kw)
+    sa = fa.solve_adjoint(dr, mesh, *bcs, objective_v=obj, lam_v=[np.zeros(pr.nnodes)] * 3,
+                          lam_P=np.zeros((pr.nels, 4)), **kw)
+    errs = {f"lam_{c}": rel(sa["lam_v"][k], lo[k]) for k, c in enumerate("xyz")}
+    errs["lam_P"] = rel(sa["lam_P"], lPo.reshape(-1, 4))
+    say(f"  adjoint oracle iter {sao['iterations']} | gpu iter {sa['iterations']}; "
+        + " ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert sao["converged"] and sa["converged"] and sa["iterations"] == sao["iterations"]
+    assert max(errs.values()) < 1e-10
+    # the forward state is untouched
+    for k, c in enumerate("xyz"):
+        assert np.array_equal(getattr(dr.v, c).numpy(), v_fwd[k])
+    assert np.array_equal(dr.P.numpy(), P_fwd)
+    go = pr.cp3_material_gradient(pr.v, lo)
+    gg = fa.stokes_material_gradient_3d(dr, mesh, sa["lam_v"])
+    say(f"  gradients oracle {go} | gpu {gg}")
+    for p in range(2):
+        assert gg["density_gradient"][p] == pytest.approx(go[0][p], rel=1e-10)
+        assert gg["viscosity_gradient"][p] == pytest.approx(go[1][p], rel=1e-10)
+    # a warm start from the converged adjoint converges at the first check
+    with pytest.raises(fa.DimensionMismatch, match="objective_v"):
+        fa.solve_adjoint(dr, mesh, *bcs, objective_v=obj[:2], lam_v=[np.zeros(pr.nnodes)] * 3,
+                         lam_P=np.zeros((pr.nels, 4)))
+    warm = fa.solve_adjoint(dr, mesh, *bcs, objective_v=obj, lam_v=list(sa["lam_v"]), lam_P=sa["lam_P"], **kw)
+    assert warm["converged"] and warm["iterations"] <= kw["check_interval"]

@@ -134,3 +134,53 @@ def test_prescribed_velocities_enter_the_first_residual():
     assert np.allclose(pc.P, pr.P)
     for c in range(3):
         assert np.allclose(pc.v[c], pr.v[c])
+
+
+def test_adjoint_is_the_transposed_system_and_gradients_match_finite_differences():
+    # test_stokes_3d_reference.jl:165-215: the adjoint of the relaxation is
+    # A_ffᵀ λ = objective; the material gradients −λ·∂R/∂m equal centred
+    # differences of J = objective·v on the (independently assembled) system
+    eta, rho, g = (1.0, 50.0), (1.0, 2.0), (0.0, 0.0, -1.0)
+    pr, ph = block_problem((3, 3, 2))
+    st = pr.cp3_solve(tolerance=1e-11, max_iterations=80000)
+    assert st["converged"]
+    v_fwd = [v.copy() for v in pr.v]
+    P_fwd = pr.P.copy()
+    A, rhs = dense_system(pr, eta, rho, g, ph)
+    nv = 3 * pr.nnodes
+    fixed = {nv}   # pressure gauge
+    for c in range(3):
+        fixed |= {3 * int(d) + c for d in pr.bc[c][0]}
+    free = np.array(sorted(set(range(len(rhs))) - fixed))
+    rng = np.random.default_rng(2)
+    obj = [rng.normal(size=pr.nnodes) for _ in range(3)]
+    lam_v, lam_P, sa = pr.cp3_solve_adjoint(obj, tolerance=1e-11, max_iterations=80000)
+    assert sa["converged"] and sa["iterations"] > 0
+    # the forward state is untouched
+    for c in range(3):
+        assert np.array_equal(pr.v[c], v_fwd[c])
+    assert np.array_equal(pr.P, P_fwd)
+    load = np.concatenate([np.stack(obj, axis=1).ravel(), np.zeros(4 * pr.nels)])
+    Aff = A[np.ix_(free, free)]
+    exact = np.zeros(len(rhs))
+    exact[free] = np.linalg.solve(Aff.T, load[free])
+    lam = np.stack(lam_v, axis=1).ravel()
+    assert np.abs(lam - exact[:nv]).max() < 1e-7 * np.abs(exact[:nv]).max()
+    for c in range(3):
+        assert not lam_v[c][pr.bc[c][0]].any()   # homogeneous conditions
+    dens, visc = pr.cp3_material_gradient(v_fwd, lam_v)
+    h = 1e-5
+    for p in range(2):
+        e_p = tuple(1.0 if q == p else 0.0 for q in range(2))
+        drhs = dense_system(pr, eta, e_p, g, ph)[1][free]
+        plus = np.linalg.solve(Aff, rhs[free] + h * drhs)
+        minus = np.linalg.solve(Aff, rhs[free] - h * drhs)
+        fd_rho = load[free] @ (plus - minus) / (2 * h)
+        dA = (dense_system(pr, e_p, rho, g, ph)[0] - dense_system(pr, (0.0, 0.0), rho, g, ph)[0])[np.ix_(free, free)]
+        step = h * eta[p]
+        plus = np.linalg.solve(Aff + step * dA, rhs[free])
+        minus = np.linalg.solve(Aff - step * dA, rhs[free])
+        fd_eta = load[free] @ (plus - minus) / (2 * step)
+        assert dens[p] == pytest.approx(fd_rho, rel=1e-6)
+        assert visc[p] == pytest.approx(fd_eta, rel=1e-5)
+        assert abs(dens[p]) > 0 and abs(visc[p]) > 0
