@@ -1024,6 +1024,45 @@ static int checked_lmax(fem_stokes* s, double lmax[2]) {
     return FEM_OK;
 }
 
+// compute_strain_rate_stress_postprocess (postprocess.jl:17-100), one thread per
+// element: dΩ-weighted element averages of ε̇xx, ε̇yy, ε̇zz = 0, ε̇xy, ε̇II (of the
+// deviatoric rate) and of the point stresses τxx, τyy, τzz = −(τxx + τyy), τxy,
+// τII; out is [10][nels]
+__global__ __launch_bounds__(256) void k_postprocess2(int64_t ne, const int32_t* el2n, const double* geo,
+                                                      const double* vx, const double* vy, const double* txx,
+                                                      const double* tyy, const double* txy, double* out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const double i00 = geo[0 * ne + e], i01 = geo[1 * ne + e], i10 = geo[2 * ne + e], i11 = geo[3 * ne + e];
+    const double det = geo[4 * ne + e];
+    double ux[NV], uy[NV];
+    for (int a = 0; a < NV; ++a) {
+        const int32_t n = el2n[a * ne + e];
+        ux[a] = vx[n];
+        uy[a] = vy[n];
+    }
+    double tot[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, vol = 0;
+    for (int q = 0; q < NQ; ++q) {
+        double ax = 0, ay = 0, bx = 0, by = 0;
+        for (int a = 0; a < NV; ++a) {
+            const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+            ax += d0 * ux[a]; ay += d1 * ux[a]; bx += d0 * uy[a]; by += d1 * uy[a];
+        }
+        const double exx = ax * i00 + ay * i10, exy_x = ax * i01 + ay * i11;
+        const double eyx = bx * i00 + by * i10, eyy = bx * i01 + by * i11;
+        const double exy = (exy_x + eyx) / 2;
+        const double tr = (exx + eyy) / 3;
+        const double d0 = exx - tr, d1 = eyy - tr, d2 = -tr;
+        const double tx = txx[e * NQ + q], ty = tyy[e * NQ + q], tz = -(tx + ty), txy_q = txy[e * NQ + q];
+        const double val[10] = {exx, eyy, 0.0, exy, sqrt((d0 * d0 + d1 * d1 + d2 * d2) / 2 + exy * exy),
+                                tx, ty, tz, txy_q, sqrt((tx * tx + ty * ty + tz * tz) / 2 + txy_q * txy_q)};
+        const double dO = kT.w[q] * det;
+        for (int k = 0; k < 10; ++k) tot[k] += val[k] * dO;
+        vol += dO;
+    }
+    for (int k = 0; k < 10; ++k) out[k * ne + e] = tot[k] / vol;
+}
+
 // entries that are not finite (or, with positive, not > 0)
 __global__ void k_count_bad(int64_t n, const double* x, int32_t positive, int32_t* flag) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1914,6 +1953,21 @@ int fem_stokes_pressure_scaling(fem_stokes* s, double pressure_factor, double dt
     CHK(prepare(s, nullptr, dt, finite_K, o, true));
     CHK(scaling_impl(s, pressure_factor, dt));
     return fem_sync(s->ctx);
+}
+
+int fem_stokes_postprocess(fem_stokes* s, double* out) {
+    if (!s || !out) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim != 2) return set_err(c, FEM_EARG, "the strain-rate and stress diagnostics are built for 2-D states");
+    const fem_mesh* m = s->mesh;
+    double** f = s->f;
+    hipLaunchKernelGGL(k_postprocess2, dim3(grid_for(m->nels, 256)), dim3(256), 0, c->stream, m->nels,
+                       (const int32_t*)m->d_el2n, (const double*)m->d_geo, (const double*)f[FEM_FIELD_V_X],
+                       (const double*)f[FEM_FIELD_V_Y], (const double*)f[FEM_FIELD_TAU_XX],
+                       (const double*)f[FEM_FIELD_TAU_YY], (const double*)f[FEM_FIELD_TAU_XY], out);
+    CHK(check_launch(c, "k_postprocess2"));
+    return fem_sync(c);
 }
 
 int fem_stokes_set_solve_inputs(fem_stokes* s, const fem_solve_inputs* in) {

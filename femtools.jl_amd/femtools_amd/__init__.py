@@ -30,7 +30,8 @@ __all__ = [
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
-    "solve_adjoint", "stokes_material_gradient_3d",
+    "solve_adjoint", "stokes_material_gradient_3d", "compute_strain_rate_stress_postprocess", "write_vtk",
+    "write_stokes_vtk", "VtkMesh",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
 ]
@@ -213,6 +214,7 @@ _sig("fem_stokes_jacobian_diagnostics_cap", C.c_int, VP, _P(_Cap), D, I32)
 _sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
+_sig("fem_stokes_postprocess", C.c_int, VP, VP)
 _sig("fem_cell_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(VP), _P(VP), VP, _P(_CellParams), _P(_CellStats),
      _P(D), I64, _P(I64))
 _sig("fem_cell_stokes_material_gradient", C.c_int, VP, _P(VP), _P(VP), _P(D), _P(D))
@@ -235,6 +237,7 @@ EXPORTED = [
     "fem_stokes_commit_plastic_history", "fem_cell_stokes_preconditioner", "fem_cell_stokes_pressure_residual",
     "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve", "fem_stokes_set_solve_inputs",
     "fem_stokes_set_pressure_source", "fem_cell_stokes_solve_adjoint", "fem_cell_stokes_material_gradient",
+    "fem_stokes_postprocess",
 ]
 
 
@@ -1412,3 +1415,27 @@ def stokes_material_gradient_3d(dr: CellPressureStokesDR, mesh: MixedMesh, adjoi
         dr.ctx.check(_lib.fem_cell_stokes_material_gradient(dr.handle, fwd, lam, dens.ctypes.data_as(_P(D)),
                                                             visc.ctypes.data_as(_P(D))))
     return dict(density_gradient=tuple(dens), viscosity_gradient=tuple(visc))
+
+
+_POST_KEYS = ("exx", "eyy", "ezz", "exy", "eII", "txx", "tyy", "tzz", "txy", "tII")
+
+
+def compute_strain_rate_stress_postprocess(dr: StokesDR, mesh: MixedMesh):
+    """compute_strain_rate_stress_postprocess (postprocess.jl:84-100) on the
+    device from dr.v and the point stresses dr.tau (as left by
+    update_stokes_current_stress): element averages keyed exx, eyy, ezz, exy,
+    eII, txx, tyy, tzz, txy, tII (the reference's εxx ... tauII)."""
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    buf = VP()
+    n = 10 * mesh.nels
+    dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, n * 8, C.byref(buf)))
+    try:
+        dr.ctx.check(_lib.fem_stokes_postprocess(dr.handle, buf))
+        a = DeviceArray(dr.ctx, buf.value, (10, mesh.nels)).numpy()
+    finally:
+        _lib.fem_buf_free(dr.ctx.handle, buf)
+    return dict(zip(_POST_KEYS, a))
+
+
+from .vtk import Mesh as VtkMesh, write_vtk, write_stokes_vtk  # noqa: E402

@@ -387,6 +387,13 @@ int fem_thermal_solve(fem_thermal* th, const fem_bc* bc, const fem_thermal_param
  * dr.R, and dr.∂R∂T / dr.PC (|J| row sums / |J_ii|) when compute_jacobian. */
 int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t compute_jacobian);
 
+/* compute_strain_rate_stress_postprocess(vx, vy, el2n_v, geo_v, τ_ip, element_v)
+ * (src/postprocess/postprocess.jl:17-100) on the state's v and τ (the point
+ * stresses of the last update_stokes_current_stress!): per element the
+ * dΩ-weighted averages of (ε̇xx, ε̇yy, ε̇zz, ε̇xy, ε̇II, τxx, τyy, τzz, τxy, τII)
+ * into out (device, [10][nels]). 2-D states. */
+int fem_stokes_postprocess(fem_stokes* st, double* out);
+
 /* rotate_stress!(dr, mesh_stokes, Δt) (src/stokes/stress_rotation.jl:16-61):
  * τ_old at every velocity quadrature point becomes the current τ rotated by
  * the local vorticity over Δt (GeoParams.rotate_elastic_stress: θ = ωΔt with

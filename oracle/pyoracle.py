@@ -840,3 +840,29 @@ class Problem3:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return st.as_dict()
+
+
+def postprocess2(coords, el2n, vx, vy, tau):
+    """compute_strain_rate_stress_postprocess (postprocess.jl:17-100): per
+    element the dΩ-weighted averages of (ε̇xx, ε̇yy, ε̇zz = 0, ε̇xy, ε̇II of the
+    deviatoric rate, τxx, τyy, τzz = −(τxx + τyy), τxy, τII) from the nodal
+    velocity and the point stresses tau = (τxx, τyy, τxy), each [nels][7].
+    Returns a dict keyed exx, eyy, ezz, exy, eII, txx, tyy, tzz, txy, tII."""
+    _, _, dNdxi, _ = t7_tables()
+    invJ, dO, _ = t7_geometry(coords, el2n)
+    ux, uy = np.asarray(vx)[el2n], np.asarray(vy)[el2n]           # [ne][7]
+    # dN/dx = dN/dξ J⁻¹ per point: [ne][q][a][x]
+    dNdx = np.einsum("qak,eqkx->eqax", dNdxi, invJ.reshape(-1, 7, 2, 2))
+    gx = np.einsum("eqax,ea->eqx", dNdx, ux)
+    gy = np.einsum("eqax,ea->eqx", dNdx, uy)
+    exx, eyy = gx[..., 0], gy[..., 1]
+    exy = (gx[..., 1] + gy[..., 0]) / 2
+    tr = (exx + eyy) / 3
+    d0, d1, d2 = exx - tr, eyy - tr, -tr
+    txx, tyy, txy = (np.asarray(t).reshape(-1, 7) for t in tau)
+    tzz = -(txx + tyy)
+    fields = dict(exx=exx, eyy=eyy, ezz=np.zeros_like(exx), exy=exy,
+                  eII=np.sqrt((d0 ** 2 + d1 ** 2 + d2 ** 2) / 2 + exy ** 2),
+                  txx=txx, tyy=tyy, tzz=tzz, txy=txy, tII=np.sqrt((txx ** 2 + tyy ** 2 + tzz ** 2) / 2 + txy ** 2))
+    vol = dO.sum(axis=1)
+    return {k: (v * dO).sum(axis=1) / vol for k, v in fields.items()}
