@@ -1959,7 +1959,10 @@ int fem_stokes_postprocess(fem_stokes* s, double* out) {
     if (!s || !out) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    if (s->dim != 2) return set_err(c, FEM_EARG, "the strain-rate and stress diagnostics are built for 2-D states");
+    if (s->dim == 3) {
+        CHK(hex27_postprocess(s, out));
+        return fem_sync(c);
+    }
     const fem_mesh* m = s->mesh;
     double** f = s->f;
     hipLaunchKernelGGL(k_postprocess2, dim3(grid_for(m->nels, 256)), dim3(256), 0, c->stream, m->nels,

@@ -1019,6 +1019,69 @@ int hex27_rotate_stress(fem_stokes* s, double dt) {
     return check_launch(s->ctx, "k_hex27_rotate");
 }
 
+// compute_strain_rate_stress_postprocess(v, el2n_v, geo_v, τ_ip, element_v)
+// (postprocess.jl:169-212), one thread per cell: the dΩ-weighted cell averages
+// of ε̇ (xx, yy, zz, xy, xz, yz), ε̇II of its deviatoric part, the six point
+// stresses and τII, the invariants with the eps² floor of second_invariant
+// (rheology.jl:395-399); out is [14][nels]
+struct Post3Args {
+    HexGeo g;
+    const double *vx, *vy, *vz;
+    const double* tau[6];   // [nels][27] each, (xx, yy, zz, xy, xz, yz)
+    double* out;
+};
+
+__global__ __launch_bounds__(128) void k_postprocess3(Post3Args A) {
+    const HexGeo& g = A.g;
+    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (e >= g.nels) return;
+    double sv[HNV * 3];
+    for (int t = 0; t < HNV; ++t) {
+        const int64_t n = hex_node(g, e, t);
+        sv[t * 3 + 0] = A.vx[n]; sv[t * 3 + 1] = A.vy[n]; sv[t * 3 + 2] = A.vz[n];
+    }
+    constexpr double eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+    double tot[14];
+    for (int k = 0; k < 14; ++k) tot[k] = 0;
+    double vol = 0;
+    for (int q = 0; q < HNQ; ++q) {
+        const Lane1D b = point_1d(q);
+        double gv[3][3];
+        point_gradient(g, b, q, sv, gv);
+        const double ex = gv[0][0], ey = gv[1][1], ez = gv[2][2];
+        const double exy = (gv[0][1] + gv[1][0]) / 2, exz = (gv[0][2] + gv[2][0]) / 2, eyz = (gv[1][2] + gv[2][1]) / 2;
+        const double tr = (ex + ey + ez) / 3;
+        const double dx = ex - tr, dy = ey - tr, dz = ez - tr;
+        double t6[6];
+        for (int c = 0; c < 6; ++c) t6[c] = A.tau[c][e * HNQ + q];
+        const double val[14] = {
+            ex, ey, ez, exy, exz, eyz,
+            sqrt((dx * dx + dy * dy + dz * dz) / 2 + exy * exy + exz * exz + eyz * eyz + eps2),
+            t6[0], t6[1], t6[2], t6[3], t6[4], t6[5],
+            sqrt((t6[0] * t6[0] + t6[1] * t6[1] + t6[2] * t6[2]) / 2 + t6[3] * t6[3] + t6[4] * t6[4] +
+                 t6[5] * t6[5] + eps2)};
+        const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
+        const double dOm = g.det * ((gauss_w(qi) * gauss_w(qj)) * gauss_w(qk));
+        for (int k = 0; k < 14; ++k) tot[k] += val[k] * dOm;
+        vol += dOm;
+    }
+    for (int k = 0; k < 14; ++k) A.out[k * g.nels + e] = tot[k] / vol;
+}
+
+int hex27_postprocess(fem_stokes* s, double* out) {
+    Post3Args a;
+    std::memset(&a, 0, sizeof a);
+    a.g = hex_geo(s->mesh);
+    double** f = s->f;
+    a.vx = f[FEM_FIELD_V_X]; a.vy = f[FEM_FIELD_V_Y]; a.vz = f[FEM_FIELD_V_Z];
+    const int tc[6] = {FEM_FIELD_TAU_XX, FEM_FIELD_TAU_YY, FEM_FIELD_TAU_ZZ, FEM_FIELD_TAU_XY, FEM_FIELD_TAU_XZ,
+                       FEM_FIELD_TAU_YZ};
+    for (int k = 0; k < 6; ++k) a.tau[k] = f[tc[k]];
+    a.out = out;
+    hipLaunchKernelGGL(k_postprocess3, dim3(grid_for(a.g.nels, 128)), dim3(128), 0, s->ctx->stream, a);
+    return check_launch(s->ctx, "k_postprocess3");
+}
+
 // ===========================================================================
 // CellPressureStokesDR (stokes_types.jl:545-613, DR.jl:960-1057): the viscous
 // Hex27 system with the four cell-local pressure modes (1, ξ, η, ζ), relaxed

@@ -215,6 +215,7 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
                 fem_solve_stats* st);
 int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
 int hex27_rotate_stress(fem_stokes* s, double dt);
+int hex27_postprocess(fem_stokes* s, double* out);
 
 // eps(x) of Julia (spacing at |x|) and the Chebyshev step/momentum (DR.jl:39-42)
 static inline double julia_eps(double x) {

@@ -1418,24 +1418,27 @@ def stokes_material_gradient_3d(dr: CellPressureStokesDR, mesh: MixedMesh, adjoi
 
 
 _POST_KEYS = ("exx", "eyy", "ezz", "exy", "eII", "txx", "tyy", "tzz", "txy", "tII")
+_POST_KEYS3 = ("exx", "eyy", "ezz", "exy", "exz", "eyz", "eII", "txx", "tyy", "tzz", "txy", "txz", "tyz", "tII")
 
 
 def compute_strain_rate_stress_postprocess(dr: StokesDR, mesh: MixedMesh):
     """compute_strain_rate_stress_postprocess (postprocess.jl:84-100) on the
     device from dr.v and the point stresses dr.tau (as left by
     update_stokes_current_stress): element averages keyed exx, eyy, ezz, exy,
-    eII, txx, tyy, tzz, txy, tII (the reference's εxx ... tauII)."""
+    eII, txx, tyy, tzz, txy, tII (the reference's εxx ... tauII); 3-D states
+    (postprocess.jl:169-212) add exz, eyz, txz, tyz."""
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
+    keys = _POST_KEYS if mesh.dim == 2 else _POST_KEYS3
     buf = VP()
-    n = 10 * mesh.nels
+    n = len(keys) * mesh.nels
     dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, n * 8, C.byref(buf)))
     try:
         dr.ctx.check(_lib.fem_stokes_postprocess(dr.handle, buf))
-        a = DeviceArray(dr.ctx, buf.value, (10, mesh.nels)).numpy()
+        a = DeviceArray(dr.ctx, buf.value, (len(keys), mesh.nels)).numpy()
     finally:
         _lib.fem_buf_free(dr.ctx.handle, buf)
-    return dict(zip(_POST_KEYS, a))
+    return dict(zip(keys, a))
 
 
 from .vtk import Mesh as VtkMesh, write_vtk, write_stokes_vtk  # noqa: E402

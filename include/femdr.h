@@ -391,7 +391,9 @@ int fem_thermal_residual(fem_thermal* th, double dt, double Tref, int32_t comput
  * (src/postprocess/postprocess.jl:17-100) on the state's v and τ (the point
  * stresses of the last update_stokes_current_stress!): per element the
  * dΩ-weighted averages of (ε̇xx, ε̇yy, ε̇zz, ε̇xy, ε̇II, τxx, τyy, τzz, τxy, τII)
- * into out (device, [10][nels]). 2-D states. */
+ * into out (device, [10][nels]) for 2-D states; for 3-D states (postprocess.jl:
+ * 169-212) the 14 averages (ε̇xx, ε̇yy, ε̇zz, ε̇xy, ε̇xz, ε̇yz, ε̇II, τxx, τyy, τzz,
+ * τxy, τxz, τyz, τII) into [14][nels]. */
 int fem_stokes_postprocess(fem_stokes* st, double* out);
 
 /* rotate_stress!(dr, mesh_stokes, Δt) (src/stokes/stress_rotation.jl:16-61):

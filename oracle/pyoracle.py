@@ -866,3 +866,27 @@ def postprocess2(coords, el2n, vx, vy, tau):
                   txx=txx, tyy=tyy, tzz=tzz, txy=txy, tII=np.sqrt((txx ** 2 + tyy ** 2 + tzz ** 2) / 2 + txy ** 2))
     vol = dO.sum(axis=1)
     return {k: (v * dO).sum(axis=1) / vol for k, v in fields.items()}
+
+
+def postprocess3(coords, el2n, v, tau):
+    """compute_strain_rate_stress_postprocess, 3-D (postprocess.jl:169-212):
+    per cell the dΩ-weighted averages of ε̇ (xx, yy, zz, xy, xz, yz), ε̇II of the
+    deviatoric rate, the six point stresses tau ([nels][27] each) and τII; the
+    invariants carry the eps² floor of second_invariant (rheology.jl:395-399)."""
+    _, _, dNdxi, _ = hex27_tables()
+    invJ, dO = hex27_geometry(coords, el2n)
+    ne = len(el2n)
+    dNdx = np.einsum("qak,eqkx->eqax", dNdxi, invJ.reshape(ne, 27, 3, 3))
+    G = [np.einsum("eqax,ea->eqx", dNdx, np.asarray(v[c])[el2n]) for c in range(3)]   # G[c][e,q,x]
+    e = dict(exx=G[0][..., 0], eyy=G[1][..., 1], ezz=G[2][..., 2], exy=(G[0][..., 1] + G[1][..., 0]) / 2,
+             exz=(G[0][..., 2] + G[2][..., 0]) / 2, eyz=(G[1][..., 2] + G[2][..., 1]) / 2)
+    tr = (e["exx"] + e["eyy"] + e["ezz"]) / 3
+    eps2 = np.finfo(float).eps ** 2
+    d = [e["exx"] - tr, e["eyy"] - tr, e["ezz"] - tr]
+    e["eII"] = np.sqrt((d[0] ** 2 + d[1] ** 2 + d[2] ** 2) / 2 + e["exy"] ** 2 + e["exz"] ** 2 + e["eyz"] ** 2 + eps2)
+    t = [np.asarray(x).reshape(ne, 27) for x in tau]
+    for k, x in zip(("txx", "tyy", "tzz", "txy", "txz", "tyz"), t):
+        e[k] = x
+    e["tII"] = np.sqrt((t[0] ** 2 + t[1] ** 2 + t[2] ** 2) / 2 + t[3] ** 2 + t[4] ** 2 + t[5] ** 2 + eps2)
+    vol = dO.sum(axis=1)
+    return {k: (x * dO).sum(axis=1) / vol for k, x in e.items()}

@@ -70,3 +70,23 @@ def test_gpu_solved_state_to_vtk():
     corner = int(el2nP[0, 0])
     k = np.searchsorted(np.unique(el2nP), corner)
     assert v["point_data"]["P"][k] == pytest.approx(P[DoFsP][el2nP == corner].mean(), rel=1e-14)
+
+
+def test_gpu_postprocess_3d_matches_the_oracle():
+    say("[gpu] 3-D strain-rate / stress diagnostics")
+    coords, el2n = O.hex27_mesh(3, 2, 2)
+    mesh = fa.hex27_box_mesh(3, 2, 2)
+    assert np.array_equal(mesh.coords, coords)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(g=(0.0, 0.0, 0.0)))
+    rng = np.random.default_rng(4)
+    v = tuple(rng.normal(size=len(coords)) for _ in range(3))
+    tau = tuple(rng.normal(size=(len(el2n), 27)) for _ in range(6))
+    for c, a in zip("xyz", v):
+        getattr(dr.v, c).copyto(a)
+    for name, a in zip(("xx", "yy", "zz", "xy", "xz", "yz"), tau):
+        getattr(dr.tau, name).copyto(a)
+    got = fa.compute_strain_rate_stress_postprocess(dr, mesh)
+    ref = O.postprocess3(coords, el2n, v, tau)
+    errs = {k: np.abs(got[k] - ref[k]).max() / np.abs(ref[k]).max() for k in ref}
+    say("  " + " ".join(f"{k} {e:.1e}" for k, e in errs.items()))
+    assert max(errs.values()) < 1e-12

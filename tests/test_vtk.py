@@ -186,3 +186,22 @@ def test_diagnostics_closed_forms():
     p = O.postprocess2(coords, el2n, coords[:, 0] ** 2, 0 * vy, tau)
     cen = coords[el2n[:, :3]].mean(axis=1)
     assert np.allclose(p["exx"], 2 * cen[:, 0], rtol=1e-13)
+
+
+def test_diagnostics_3d_closed_forms():
+    # postprocess.jl:169-212 on a linear velocity field and uniform stresses
+    coords, el2n = O.hex27_mesh(2, 2, 1)
+    x, y, z = coords[:, 0], coords[:, 1], coords[:, 2]
+    v = (0.2 * x - 0.5 * y + 0.1 * z, 0.3 * x + 0.4 * y, -0.6 * z + 0.7 * x)
+    ne = len(el2n)
+    tau = tuple(np.full((ne, 27), c) for c in (1.0, -2.0, 1.0, 0.5, -0.25, 0.75))
+    p = O.postprocess3(coords, el2n, v, tau)
+    exx, eyy, ezz = 0.2, 0.4, -0.6
+    exy, exz, eyz = (-0.5 + 0.3) / 2, (0.1 + 0.7) / 2, 0.0
+    tr = (exx + eyy + ezz) / 3
+    eII = math.sqrt(((exx - tr) ** 2 + (eyy - tr) ** 2 + (ezz - tr) ** 2) / 2 + exy ** 2 + exz ** 2 + eyz ** 2)
+    tII = math.sqrt((1 + 4 + 1) / 2 + 0.25 + 0.0625 + 0.5625)
+    expect = dict(exx=exx, eyy=eyy, ezz=ezz, exy=exy, exz=exz, eyz=eyz, eII=eII, txx=1.0, tyy=-2.0, tzz=1.0,
+                  txy=0.5, txz=-0.25, tyz=0.75, tII=tII)
+    for k, val in expect.items():
+        assert np.allclose(p[k], val, rtol=1e-13, atol=1e-14), k
