@@ -957,6 +957,8 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
     keep = []
     inputs = _pop_inputs(kwargs)
     collect = bool(kwargs.pop("collect_history", False))
+    if kwargs.pop("verbose_inner", False):   # the inner-check trace (DR.jl:729) is verbose level 2
+        kwargs["verbose"] = 2
     Qq, Q2D = kwargs.pop("Qq", None), kwargs.pop("Q2D", None)
     if Q2D is not None and Qq is None:
         raise ValueError("Qq is required when Q2D is supplied")
