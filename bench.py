@@ -443,8 +443,8 @@ def main():
     ap.add_argument("--batch-precision", choices=("fp64", "fp32"), default="fp32")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on sharding); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a single solve would shard by element "
+              "strips with an interface-node exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -105,7 +105,7 @@ def test_geometry_affine_closed_form():
 
 def test_node_valence_structured_t7():
     # Valence: vertices 4 or 8 (alternating diagonals, connectivity.jl:129-135)
-    # in the interior, mid-edge nodes 2, bubbles 1 (SURVEY appendix facts).
+    # in the interior, mid-edge nodes 2, bubbles 1.
     coords, el2n = O.t7_mesh(4, 4)
     val = np.bincount(el2n.ravel(), minlength=len(coords))
     bubbles = el2n[:, 6]
