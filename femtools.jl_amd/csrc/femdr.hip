@@ -1634,6 +1634,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 timer_e1 = timer.ev();
                 e2 = timer.ev();
                 timer.weight.push_back(nseg);
+                timer.kind.push_back(d32.delta ? 1 : 0);
                 timer.mark(e0);
                 CHK(seg(nullptr));
                 timer.mark(e2);
@@ -1733,6 +1734,9 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     st->kernel_ms_element = timer.ms_elem;
     st->kernel_ms_gather = timer.ms_gather;
     st->timed_launches = timer.n;
+    st->kernel_ms_element_rm = timer.ms_elem_rm;
+    st->kernel_ms_gather_rm = timer.ms_gather_rm;
+    st->timed_launches_rm = timer.n_rm;
     st->itPH = itPH_done;
     st->iter = iter;
     st->err = err;
@@ -2309,6 +2313,87 @@ int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const 
     cst->thermal_iterations = side.tc.iter;
     cst->thermal_converged = side.tc.converged;
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Kernel-level residual norms of a DR check (DR.jl:556-557, 700): block b sums
+// the squares of entries [b*NORM_CHUNK, (b+1)*NORM_CHUNK) of RP./M_P, Rv_x and
+// Rv_y, one block then adds the block partials in block order, so the result
+// is run-to-run reproducible and the same for every entry point that uses this
+// partition.
+// ---------------------------------------------------------------------------
+constexpr int64_t NORM_CHUNK = (int64_t)RED_BLOCK * 16;
+
+__global__ __launch_bounds__(RED_BLOCK) void k_norms_chunk(int64_t nv, int64_t np, const double* Rx,
+                                                           const double* Ry, const double* RP, const double* MP,
+                                                           double* partial) {
+    double acc[3] = {0, 0, 0};
+    const int64_t base = (int64_t)blockIdx.x * NORM_CHUNK + threadIdx.x;
+#pragma unroll 4
+    for (int j = 0; j < (int)(NORM_CHUNK / RED_BLOCK); ++j) {
+        const int64_t i = base + (int64_t)j * RED_BLOCK;
+        if (i < np) {
+            const double r = RP[i] / MP[i];
+            acc[0] += r * r;
+        }
+        if (i < nv) {
+            const double a = Rx[i], b = Ry[i];
+            acc[1] += a * a;
+            acc[2] += b * b;
+        }
+    }
+    block_sum_store<3>(acc, partial);
+}
+
+// the number of block partials of a norms launch over nv velocity and np pressure entries
+static int64_t norms_blocks(int64_t nv, int64_t np) { return (std::max(nv, np) + NORM_CHUNK - 1) / NORM_CHUNK; }
+
+// the two launches of the norms: partials into `partial` ([nb][3]), sums into out[3]
+// (Σ(RP/M_P)², ΣRv_x², ΣRv_y²) on `stream`
+static hipError_t norms_launch(hipStream_t stream, int64_t nv, int64_t np, const double* Rx, const double* Ry,
+                               const double* RP, const double* MP, double* partial, double* out) {
+    const int64_t nb = norms_blocks(nv, np);
+    hipLaunchKernelGGL(k_norms_chunk, dim3((unsigned)nb), dim3(RED_BLOCK), 0, stream, nv, np, Rx, Ry, RP, MP,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, stream, (int)nb, 3, 0,
+                       (const double*)partial, out);
+    return hipGetLastError();
+}
+
+extern "C" {
+
+int fem_norms(fem_ctx* c, const double* Rv_x, const double* Rv_y, int64_t nv, const double* RP, const double* MP,
+              int64_t np, double* host_out) {
+    if (!c || !host_out || nv < 0 || np < 0 || (nv > 0 && (!Rv_x || !Rv_y)) || (np > 0 && (!RP || !MP)))
+        return set_err(c, FEM_EARG, "fem_norms: null argument or negative size");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nb = norms_blocks(nv, np);
+    double sums[3] = {0, 0, 0};
+    if (nb > 0) {
+        // the context's reduction workspace holds RED_GRID * RED_MAXK partial
+        // values; larger inputs get a workspace of their own for this call
+        double* partial = c->red_partial;
+        const bool own = nb * 3 > (int64_t)RED_GRID * RED_MAXK;
+        if (own) CHK(dalloc_t(c, &partial, nb * 3));
+        const hipError_t e = norms_launch(c->stream, nv, np, Rv_x, Rv_y, RP, MP, partial, c->red_out);
+        hipError_t e2 = hipSuccess;
+        if (e == hipSuccess)
+            e2 = hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream);
+        const hipError_t e3 = hipStreamSynchronize(c->stream);
+        if (own) (void)hipFree(partial);
+        HIPCHK(c, e);
+        HIPCHK(c, e2);
+        HIPCHK(c, e3);
+        std::memcpy(sums, c->h_red, 3 * 8);
+    }
+    host_out[0] = std::sqrt(sums[1]);
+    host_out[1] = std::sqrt(sums[2]);
+    host_out[2] = std::sqrt(sums[0]);
+    host_out[3] = nv > 0 ? std::max(host_out[0], host_out[1]) / (2 * std::sqrt((double)nv)) : 0.0;
+    host_out[4] = np > 0 ? host_out[2] / std::sqrt((double)np) : 0.0;
+    return FEM_OK;
 }
 
 }  // extern "C"

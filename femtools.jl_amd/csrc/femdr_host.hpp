@@ -48,7 +48,12 @@ struct KernelTimer {
     bool take(int64_t iter) const { return on && iter % STRIDE == 1; }
     double ms_elem = 0, ms_gather = 0;
     int64_t n = 0;
+    // the subset of the above timed on return-map correction intervals of the
+    // mixed-precision solve (kind 1)
+    double ms_elem_rm = 0, ms_gather_rm = 0;
+    int64_t n_rm = 0;
     std::vector<int64_t> weight;   // inner iterations covered by each timed triple
+    std::vector<int8_t> kind;      // 1: a return-map correction segment
     KernelTimer(fem_ctx* c_, bool on_) : c(c_), on(on_) { c->tev_used = 0; }
     hipEvent_t ev() {
         if (c->tev_used == c->tev.size()) {
@@ -68,13 +73,20 @@ struct KernelTimer {
             float a = 0, b = 0;
             if (hipEventElapsedTime(&a, c->tev[i], c->tev[i + 1]) == hipSuccess &&
                 hipEventElapsedTime(&b, c->tev[i + 1], c->tev[i + 2]) == hipSuccess) {
+                const int64_t w = i / 3 < weight.size() ? weight[i / 3] : 1;
                 ms_elem += a;
                 ms_gather += b;
-                n += i / 3 < weight.size() ? weight[i / 3] : 1;
+                n += w;
+                if (i / 3 < kind.size() && kind[i / 3]) {
+                    ms_elem_rm += a;
+                    ms_gather_rm += b;
+                    n_rm += w;
+                }
             }
         }
         c->tev_used = 0;
         weight.clear();
+        kind.clear();
     }
 };
 

@@ -34,6 +34,7 @@ __all__ = [
     "write_stokes_vtk", "VtkMesh",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
+    "norms", "device_array",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -118,7 +119,8 @@ class _SolveStats(C.Structure):
                 ("err_P", D), ("converged", I32), ("reached_total_iter", I32), ("lambda_max", D),
                 ("lambda_max_gershgorin", D), ("lambda_max_iterations", I64), ("jacobian_assemblies", I64),
                 ("seconds", D), ("kernel_ms_element", D), ("kernel_ms_gather", D), ("element_launches", I64),
-                ("timed_launches", I64)]
+                ("timed_launches", I64), ("kernel_ms_element_rm", D), ("kernel_ms_gather_rm", D),
+                ("timed_launches_rm", I64)]
 
 
 class _ThermalMaterial(C.Structure):
@@ -215,6 +217,16 @@ _sig("fem_stokes_update_current_stress_cap", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
 _sig("fem_stokes_postprocess", C.c_int, VP, VP)
+_sig("fem_norms", C.c_int, VP, VP, VP, I64, VP, VP, I64, _P(D))
+_sig("fem_cnn_create", C.c_int, VP, I32, I32, I32, _P(VP))
+_sig("fem_cnn_destroy", C.c_int, VP)
+_sig("fem_cnn_parameter_count", C.c_int, _P(I64), _P(I64))
+_sig("fem_cnn_set_weights", C.c_int, VP, VP, VP)
+_sig("fem_cnn_get_weights", C.c_int, VP, VP, VP)
+_sig("fem_cnn_get_grads", C.c_int, VP, VP, VP)
+_sig("fem_cnn_forward", C.c_int, VP, VP, I32, VP)
+_sig("fem_cnn_train_step", C.c_int, VP, VP, VP, I32, D, I32, _P(D))
+_sig("fem_cnn_gemm_bf16", C.c_int, VP, I64, I64, I64, VP, VP, VP)
 _sig("fem_cell_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(VP), _P(VP), VP, _P(_CellParams), _P(_CellStats),
      _P(D), I64, _P(I64))
 _sig("fem_cell_stokes_material_gradient", C.c_int, VP, _P(VP), _P(VP), _P(D), _P(D))
@@ -237,7 +249,9 @@ EXPORTED = [
     "fem_stokes_commit_plastic_history", "fem_cell_stokes_preconditioner", "fem_cell_stokes_pressure_residual",
     "fem_cell_stokes_momentum_residual", "fem_cell_stokes_solve", "fem_stokes_set_solve_inputs",
     "fem_stokes_set_pressure_source", "fem_cell_stokes_solve_adjoint", "fem_cell_stokes_material_gradient",
-    "fem_stokes_postprocess",
+    "fem_stokes_postprocess", "fem_norms", "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_parameter_count",
+    "fem_cnn_set_weights", "fem_cnn_get_weights", "fem_cnn_get_grads", "fem_cnn_forward", "fem_cnn_train_step",
+    "fem_cnn_gemm_bf16",
 ]
 
 
@@ -1106,6 +1120,42 @@ def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
 # ---------------------------------------------------------------------------
 # thermal diffusion (src/heat_diffusion/**)
 # ---------------------------------------------------------------------------
+class _OwnedBuf:
+    """A fem_buf_alloc allocation, freed with its owner."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        buf = VP()
+        ctx.check(_lib.fem_buf_alloc(ctx.handle, max(int(nbytes), 8), C.byref(buf)))
+        self.ptr = buf.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib.fem_buf_free(self.ctx.handle, VP(self.ptr))
+            self.ptr = None
+
+
+def device_array(ctx: Context, host=None, shape=None, dtype=np.float64) -> DeviceArray:
+    """A new device array owned by the caller (freed when it is collected),
+    filled from host when given."""
+    if host is not None:
+        host = np.ascontiguousarray(np.asarray(host, dtype=dtype))
+        shape = host.shape
+    own = _OwnedBuf(ctx, int(np.prod(shape)) * np.dtype(dtype).itemsize)
+    a = DeviceArray(ctx, own.ptr, shape, dtype, owner=own)
+    if host is not None:
+        a.copyto(host)
+    return a
+
+
+def norms(ctx: Context, Rv_x, Rv_y, RP, M_P):
+    """The residual norms of a DR check (DR.jl:556-557, 700) on device arrays
+    (DeviceArray, FP64): returns (‖Rv_x‖₂, ‖Rv_y‖₂, ‖RP./M_P‖₂, err_v, err_P)."""
+    out = (D * 5)()
+    ctx.check(_lib.fem_norms(ctx.handle, Rv_x.ptr, Rv_y.ptr, len(Rv_x), RP.ptr, M_P.ptr, len(RP), out))
+    return tuple(out)
+
+
 class ThermalMaterial:
     """ThermalMaterial(; k=(1.0,), Cp=(1.0,), ρ0=(1.0,), α=(1.0,), K=(1.0,))
     (src/heat_diffusion/types/heat_diffusion_types.jl:15-41). The first tuple

@@ -240,6 +240,10 @@ typedef struct {
     double kernel_ms_gather;    /* summed device time of the gather/update kernel          */
     int64_t element_launches;   /* inner-iteration element launches                        */
     int64_t timed_launches;     /* launches covered by kernel_ms_*                          */
+    double kernel_ms_element_rm;  /* mixed precision: the part of kernel_ms_element timed on
+                                     check intervals with the return-map correction operator */
+    double kernel_ms_gather_rm;
+    int64_t timed_launches_rm;
 } fem_solve_stats;
 
 /* solve!(::StokesDR, ::MixedMesh, bc_v; dt, ...) DR.jl:302-337: assembles
@@ -512,6 +516,51 @@ typedef struct {
 int fem_coupled_solve(fem_thermal* th, fem_stokes* st, const fem_bc* bc_T, const fem_bc bc_v[2],
                       const fem_drucker_prager* plastic, const fem_solve_params* params,
                       const fem_coupled_params* coupled, fem_solve_stats* stats, fem_coupled_stats* cstats);
+
+/* Residual norms of a DR convergence check, kernel level (DR.jl:556-557,
+ * 700: err_P = norm(RP ./ M_P)/sqrt(nnodesP), err_v = maximum(norm, Rv)/(2
+ * sqrt(nnodes))). Device inputs Rv_x, Rv_y (nv each), RP, M_P (np each);
+ * host_out[5] = (‖Rv_x‖₂, ‖Rv_y‖₂, ‖RP./M_P‖₂, err_v, err_P). The sums are
+ * FP64 in a fixed block partition (run-to-run reproducible). Runs on the
+ * context's stream and synchronises it. */
+int fem_norms(fem_ctx* ctx, const double* Rv_x, const double* Rv_y, int64_t nv, const double* RP,
+              const double* M_P, int64_t np, double* host_out);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference, which takes any initial
+ * guess and lets the DR iteration decide the answer,
+ * examples/miniapps/stokes/stokes_2D_pure_shear_triangle/
+ * stokes_2D_pure_shear_triangle.jl:159-163). A 4-level encoder-decoder of 14
+ * 3x3 convolutions (channels 32, 64, 128, 128; ReLU after all but the head)
+ * on H x W images of 16 input features (NHWC, FP32 on the device), three
+ * outputs per pixel. BF16 weights and activations, FP32 accumulation on the
+ * matrix cores, FP32 master weights and Adam moments. Layer l's weight is
+ * [Co][3][3][Ci] (tap (ky, kx) reads pixel (y + ky - 1, x + kx - 1)); host
+ * weight arrays concatenate the 14 layers in forward order
+ * (e0a e0b e1a e1b e2a e2b e3a e3b d2a d2b d1a d1b d0a head).
+ * --------------------------------------------------------------------------- */
+typedef struct fem_cnn fem_cnn;
+/* H, W multiples of 8; buffers for batches up to max_batch. Weights start at zero. */
+int fem_cnn_create(fem_ctx* ctx, int32_t max_batch, int32_t H, int32_t W, fem_cnn** out);
+int fem_cnn_destroy(fem_cnn* net);
+/* total weight and bias counts of the 14 layers */
+int fem_cnn_parameter_count(int64_t* weights, int64_t* biases);
+/* FP32 master weights/biases to/from the host (set_weights also resets the Adam moments) */
+int fem_cnn_set_weights(fem_cnn* net, const float* host_W, const float* host_b);
+int fem_cnn_get_weights(fem_cnn* net, float* host_W, float* host_b);
+/* gradients of the last fem_cnn_train_step */
+int fem_cnn_get_grads(fem_cnn* net, float* host_gW, float* host_gb);
+/* y[B][H][W][3] = net(x[B][H][W][16]), device pointers, on the context stream (not synchronised) */
+int fem_cnn_forward(fem_cnn* net, const float* x, int32_t B, float* y);
+/* forward, loss = mean over B, pixels and 3 channels of (y - t)^2 with t
+ * [B][H][W][3] (device), backward, and with lr > 0 one Adam step (step >= 1
+ * is the step count of the bias correction). Synchronises; *loss on the host. */
+int fem_cnn_train_step(fem_cnn* net, const float* x, const float* t, int32_t B, double lr, int32_t step,
+                       double* loss);
+/* C[M][N] = sum_k A[m][k] Bt[n][k] of bfloat16 (raw bits) device matrices, FP32
+ * out: the matrix-core product every convolution of the network uses. K a
+ * multiple of 16. Not synchronised. */
+int fem_cnn_gemm_bf16(fem_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint16_t* A, const uint16_t* Bt, float* C);
 
 #ifdef __cplusplus
 }

@@ -620,6 +620,17 @@ void stokes_cheb(double dtau, double lmin, double c_fact, double* a, double* b) 
     *b = (2 - c * dtau) / (2 + c * dtau);
 }
 
+extern "C" {
+// the two scalar helpers above, exported for their closed-form checks
+double orc_stokes_lmin(double step, const double* rate, const double* R, const double* R0, const double* PC,
+                       int64_t n) {
+    return stokes_lmin(step, rate, R, R0, PC, n);
+}
+void orc_stokes_cheb(double dtau, double lmin, double c_fact, double* a, double* b) {
+    stokes_cheb(dtau, lmin, c_fact, a, b);
+}
+}  // extern "C"
+
 // The thermal side of a coupled solve (DR.jl:514-529, 604-661): the thermal
 // state, its Chebyshev scalars and convergence, the shear-heating source.
 struct ThermalHook {
@@ -1166,6 +1177,9 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
                 if (iter == ncheck) err_v00 = evi + julia_eps(evi);
                 err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
                 history(iter, evi);
+                if (p->verbose > 1)
+                    printf("  it = %lld, iter = %lld, err = %.3e abs = %.3e err_v00 = %.3e lmax = %.4e %.4e\n",
+                           (long long)itPT, (long long)iter, err, evi, err_v00, lmax[0], lmax[1]);
                 if (std::isnan(err)) { g_err = "NaN detected in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -4; return -4; }
                 if (err > 1.0e10) { g_err = "Kaboom! Error > 1e10 in inner loop PH=" + std::to_string(itPH) + " PT=" + std::to_string(itPT); out->status = -5; return -5; }
                 double lmin[2];

@@ -132,6 +132,9 @@ class Stokes3(C.Structure):
 
 
 def _declare(L):
+    L.orc_stokes_lmin.argtypes = [D, PD, PD, PD, PD, I64]
+    L.orc_stokes_lmin.restype = D
+    L.orc_stokes_cheb.argtypes = [D, D, D, PD, PD]
     L.orc_hex27_basis.argtypes = [D, D, D, PD, PD]
     L.orc_hex27_rule.argtypes = [PD, PD, PD, PD]
     L.orc_hex27_tables.argtypes = [PD, PD, PD, PD]
@@ -217,6 +220,19 @@ def f64(a):
 
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def stokes_lmin(step, rate, R, R0, PC):
+    """_stokes_λmin (DR.jl:32-37) over one velocity component."""
+    a = [f64(x) for x in (rate, R, R0, PC)]
+    return lib().orc_stokes_lmin(float(step), *(ptr(x) for x in a), len(a[0]))
+
+
+def stokes_cheb(dtau, lmin, c_fact):
+    """_stokes_cheb (DR.jl:39-42): (α, β)."""
+    a, b = C.c_double(), C.c_double()
+    lib().orc_stokes_cheb(float(dtau), float(lmin), float(c_fact), C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def set_threads(n):
