@@ -8,6 +8,8 @@
 // scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
 #include "femdr_host.hpp"
 
+#include <mutex>
+
 #include <cstdlib>
 #include <functional>
 
@@ -48,6 +50,27 @@ static int fill_t(fem_ctx* c, T* p, T v, int64_t n) {
 // ---------------------------------------------------------------------------
 // context and buffers
 // ---------------------------------------------------------------------------
+// Device scratch of fem_norms_noalloc: the block partials and the three sums
+// of one norms evaluation. Each GPU holds its own copy for the life of the
+// process; its device address is resolved once per device at context creation
+// (with that device current) and only that address is used from the host.
+constexpr int64_t NORMS_SCRATCH_BLOCKS = 2048;
+static __device__ __align__(256) double scratch[NORMS_SCRATCH_BLOCKS * 3 + 8];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device `dev`
+static hipError_t resolve_scratch(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (g_scratch_ptr[dev]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[dev] = static_cast<double*>(p);
+    return e;
+}
+
 extern "C" {
 
 int fem_ctx_create(int device, fem_ctx** out) {
@@ -63,7 +86,8 @@ int fem_ctx_create(int device, fem_ctx** out) {
     if (dalloc_t(c, &c->red_partial, RED_GRID * RED_MAXK) || dalloc_t(c, &c->red_out, RED_MAXK) ||
         hipHostMalloc((void**)&c->h_red, RED_MAXK * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&c->h_cheb, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        resolve_scratch(device) != hipSuccess) {
         delete c;
         return FEM_EHIP;
     }
@@ -2393,6 +2417,46 @@ int fem_norms(fem_ctx* c, const double* Rv_x, const double* Rv_y, int64_t nv, co
     host_out[2] = std::sqrt(sums[0]);
     host_out[3] = nv > 0 ? std::max(host_out[0], host_out[1]) / (2 * std::sqrt((double)nv)) : 0.0;
     host_out[4] = np > 0 ? host_out[2] / std::sqrt((double)np) : 0.0;
+    return FEM_OK;
+}
+
+}  // extern "C"
+
+// out[5] from the three sums (Σ(RP/M_P)², ΣRv_x², ΣRv_y²), as fem_norms forms them on the host
+__global__ void k_norms_finish(int64_t nv, int64_t np, const double* sums, double* out) {
+    if (threadIdx.x != 0) return;
+    const double nx = sqrt(sums[1]), ny = sqrt(sums[2]), nP = sqrt(sums[0]);
+    out[0] = nx;
+    out[1] = ny;
+    out[2] = nP;
+    out[3] = nv > 0 ? fmax(nx, ny) / (2 * sqrt((double)nv)) : 0.0;
+    out[4] = np > 0 ? nP / sqrt((double)np) : 0.0;
+}
+
+extern "C" {
+
+int fem_norms_noalloc(fem_ctx* c, const double* Rv_x, const double* Rv_y, int64_t nv, const double* RP,
+                      const double* MP, int64_t np, double* dev_out) {
+    if (!c || !dev_out || nv < 0 || np < 0 || (nv > 0 && (!Rv_x || !Rv_y)) || (np > 0 && (!RP || !MP)))
+        return set_err(c, FEM_EARG, "fem_norms_noalloc: null argument or negative size");
+    const int64_t nb = norms_blocks(nv, np);
+    if (nb > NORMS_SCRATCH_BLOCKS)
+        return set_err(c, FEM_EARG,
+                       "fem_norms_noalloc: %lld entries exceed the static scratch limit of %lld per array "
+                       "(use fem_norms)",
+                       (long long)std::max(nv, np), (long long)(NORMS_SCRATCH_BLOCKS * NORM_CHUNK));
+    HIPCHK(c, hipSetDevice(c->device));
+    double* ptr = g_scratch_ptr[c->device];
+    if (!ptr) return set_err(c, FEM_EHIP, "fem_norms_noalloc: device scratch not resolved");
+    double* sums = ptr + NORMS_SCRATCH_BLOCKS * 3;
+    if (nb > 0) {
+        HIPCHK(c, norms_launch(c->stream, nv, np, Rv_x, Rv_y, RP, MP, ptr, sums));
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, c->stream, (int64_t)3, sums, 0.0);
+        HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_norms_finish, dim3(1), dim3(64), 0, c->stream, nv, np, (const double*)sums, dev_out);
+    HIPCHK(c, hipGetLastError());
     return FEM_OK;
 }
 

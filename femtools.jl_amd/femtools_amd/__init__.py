@@ -218,6 +218,7 @@ _sig("fem_stokes_commit_plastic_history", C.c_int, VP, _P(_Cap), D)
 _sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
 _sig("fem_stokes_postprocess", C.c_int, VP, VP)
 _sig("fem_norms", C.c_int, VP, VP, VP, I64, VP, VP, I64, _P(D))
+_sig("fem_norms_noalloc", C.c_int, VP, VP, VP, I64, VP, VP, I64, VP)
 _sig("fem_cnn_create", C.c_int, VP, I32, I32, I32, _P(VP))
 _sig("fem_cnn_destroy", C.c_int, VP)
 _sig("fem_cnn_parameter_count", C.c_int, _P(I64), _P(I64))
@@ -251,7 +252,7 @@ EXPORTED = [
     "fem_stokes_set_pressure_source", "fem_cell_stokes_solve_adjoint", "fem_cell_stokes_material_gradient",
     "fem_stokes_postprocess", "fem_norms", "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_parameter_count",
     "fem_cnn_set_weights", "fem_cnn_get_weights", "fem_cnn_get_grads", "fem_cnn_forward", "fem_cnn_train_step",
-    "fem_cnn_gemm_bf16",
+    "fem_cnn_gemm_bf16", "fem_norms_noalloc",
 ]
 
 

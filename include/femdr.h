@@ -526,6 +526,20 @@ int fem_coupled_solve(fem_thermal* th, fem_stokes* st, const fem_bc* bc_T, const
 int fem_norms(fem_ctx* ctx, const double* Rv_x, const double* Rv_y, int64_t nv, const double* RP,
               const double* M_P, int64_t np, double* host_out);
 
+/* fem_norms without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (work under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The block
+ * partials live in a static device array of 2048 x 3 doubles per GPU (48 KB,
+ * held until the process ends), so nv and np are limited to 2048 * 4096 =
+ * 8,388,608 entries each (larger inputs return FEM_EARG; fem_norms takes
+ * them). Same partition and summation order as fem_norms; dev_out[5] (device)
+ * receives (‖Rv_x‖₂, ‖Rv_y‖₂, ‖RP./M_P‖₂, err_v, err_P). Enqueued on the
+ * context's stream, not synchronised; every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (several streams or host threads)
+ * race. */
+int fem_norms_noalloc(fem_ctx* ctx, const double* Rv_x, const double* Rv_y, int64_t nv, const double* RP,
+                      const double* M_P, int64_t np, double* dev_out);
+
 /* ---------------------------------------------------------------------------
  * Learned warm start (no counterpart in the reference, which takes any initial
  * guess and lets the DR iteration decide the answer,
