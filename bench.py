@@ -239,6 +239,45 @@ def hex27_measure(n, iters):
                          "flops_per_launch": flops, "avg_launch_us": round(el_ms * 1e3, 2)}}
 
 
+def batch_measure(count, seed=0, max_iterations=400_000):
+    """The batched workload: count independent viscous T7 problems, n drawn
+    from 32..256 in steps of 16 cells, 1-10 inclusions (r in U[0.02, 0.1],
+    log10 eta in U[-2, 4]), pure shear, FP32 bulk, each to 1e-8, solved in
+    one fem_stokes_solve_batch call (grouped inner iterations replayed from
+    graphs). Each problem's time is when it finished, from the call's start."""
+    import femtools_amd as fa
+    from femtools_amd import problems as PB
+    ctx = fa.default_context()
+    probs = []
+    for w in PB.batch_workloads(count, seed=seed, sizes=tuple(range(32, 257, 16)), log_eta=(-2.0, 4.0)):
+        mesh = fa.t7_box_mesh(w.nx, w.ny, ctx=ctx)
+        ph = PB.inclusion_phases(mesh.coords, mesh.el2n, w.circles)
+        (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
+        dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
+        dr.set_phases("v", ph)
+        dr.set_phases("P", ph)
+        v0 = PB.pure_shear_field(mesh.coords)
+        dr.v.x.copyto(v0[0])
+        dr.v.y.copyto(v0[1])
+        bcs = [fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)]
+        kw = dict(dt=w.dt, tolerance=1e-8, check_interval=50, max_iterations=max_iterations, iterMax=max_iterations,
+                  throw_on_failure=False, precision="fp32")
+        probs.append((w, dr, mesh, bcs, kw))
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = fa.solve_batch([(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw in probs], raise_errors=False)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    dof_it = sum(w.dofs() * max(st["iter"], 0) for (w, *_r), st in zip(probs, res))
+    per = [{"n": w.nx, "inclusions": len(w.circles), "iterations": st["iter"], "converged": bool(st["converged"]),
+            "finished_s": round(st["seconds"], 3), "status": st["status"]} for (w, *_r), st in zip(probs, res)]
+    return {"workload": f"{count} independent T7/P1-disc viscous problems, n in 32..256 (steps of 16), 1-10 "
+                        f"inclusions, log10 eta in U[-2,4], FP32 bulk, tol 1e-8, one batched call",
+            "seconds": round(wall, 3), "solves_per_s": round(count / wall, 3),
+            "converged": sum(p["converged"] for p in per), "gdof_iter_per_s": round(dof_it / wall / 1e9, 3),
+            "problems": per}
+
+
 class Series:
     """The 512^2 VEP time series: state, boundary conditions and the two solvers."""
 
@@ -323,7 +362,10 @@ def main():
     ap.add_argument("--tolerance", type=float, default=1e-8)
     ap.add_argument("--max-iterations", type=int, default=400_000)
     ap.add_argument("--eta-reg", type=float, default=None, help="Perzyna viscosity (default problems.VEP_ETA_REG)")
-    ap.add_argument("--budget-s", type=float, default=420.0, help="wall-time budget of the timed series")
+    ap.add_argument("--deadline-s", type=float, default=400.0,
+                    help="no new series step starts when it could end after this many seconds of the run")
+    ap.add_argument("--end-s", type=float, default=540.0, help="the measurements after the series fit in this")
+    ap.add_argument("--batch", type=int, default=16, help="problems of the batched measurement (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="fast form without the learned warm start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -369,10 +411,12 @@ def main():
 
     stats = {"fp64": [], "fp32": []}
     diffs, errors, guess_s = [], [], []
-    t_series = time.perf_counter()
+    # a step costs at most max_iterations of each solver (~160 us per FP64 +
+    # FP32 iteration pair at 512^2); no step starts that could end past the deadline
+    worst = args.max_iterations * 160e-6 * (w.nx * w.ny) / 512 ** 2
     for i in range(steps):
-        if time.perf_counter() - t_series > args.budget_s:
-            log(f"series budget {args.budget_s:.0f}s spent after {i} steps")
+        if time.perf_counter() - T_START + worst > args.deadline_s:
+            log(f"deadline {args.deadline_s:.0f}s: the series stops after {i} steps")
             break
         start = S.snapshot()
         sols = {}
@@ -479,12 +523,27 @@ def main():
     }
     if errors:
         out["errors"] = errors
-    if args.extras and time.perf_counter() - T_START < 480:
+    def left():
+        return args.end_s - (time.perf_counter() - T_START)
+
+    if args.extras and left() > 60:
         try:
             out["viscous64"] = viscous64_gpu()
             out["hex27"] = hex27_measure(64, 2000)
         except Exception as e:   # reported, never required
             out["extras_error"] = repr(e)
+    if model is not None and left() > 60:
+        from femtools_amd import warmstart as WS
+        try:
+            rows = WS.evaluate_gain(model, 256, [20_000_000], log=log)
+            out["warm_start_gain"] = {"n256": rows, "summary": WS.summarize_gain(rows)}
+        except Exception as e:
+            out["warm_start_gain"] = {"error": repr(e)}
+    if args.batch > 0 and left() > 60:
+        try:
+            out["batched"] = batch_measure(args.batch)
+        except Exception as e:
+            out["batched"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)

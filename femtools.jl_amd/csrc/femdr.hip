@@ -8,6 +8,7 @@
 // scalars of a check (norms, λmin sums, λmax) and of an outer iteration.
 #include "femdr_host.hpp"
 
+#include <map>
 #include <mutex>
 
 #include <cstdlib>
@@ -731,10 +732,10 @@ static void dispatch_lin(int key, fem_stokes* s, const ElemArgs<float>& a) {
     if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
 }
 
+// The element kernel's arguments for one launch and its template key.
 template <class F>
-static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
+static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, ElemArgs<F>& a, int& fl) {
     fem_mesh* m = s->mesh;
-    ElemArgs<F> a;
     std::memset(&a, 0, sizeof a);
     a.nels = m->nels;
     a.el2n = m->d_el2n;
@@ -827,7 +828,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
         a.bf_y = s->in.body_force[1];
         a.eta_ip = s->in.viscosity;
         a.Pold_ip = s->in.P_old_at_points ? s->in.P_old : nullptr;
-        dispatch_elem<63>(key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0), s, a);
+        fl = key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0);
     } else {
         if (!d.lin || !o.use_pnum)
             return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for inner iterations on a correction only");
@@ -838,17 +839,25 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
             a.bpart = d.bpart;
             a.Rn = d.Rn;
         }
-        dispatch_lin<63>(key | EF_LIN | (tile ? EF_TILE : 0) |
-                             (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0),
-                         s, a);
+        fl = key | EF_LIN | (tile ? EF_TILE : 0) |
+             (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0);
     }
+    return FEM_OK;
+}
+
+template <class F>
+static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
+    ElemArgs<F> a;
+    int fl = 0;
+    CHK(build_elem_args(s, d, o, a, fl));
+    if constexpr (std::is_same<F, double>::value) dispatch_elem<63>(fl, s, a);
+    else dispatch_lin<63>(fl, s, a);
     return check_launch(s->ctx, "k_t7_element");
 }
 
 template <class F>
-static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
+static void build_gather_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, GatherArgs<F>& g) {
     fem_mesh* m = s->mesh;
-    GatherArgs<F> g;
     std::memset(&g, 0, sizeof g);
     g.ns = m->ns;
     g.node_of = m->d_node_of;
@@ -876,17 +885,31 @@ static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) 
         g.bf_x = (const F*)s->in.body_force[0];
         g.bf_y = (const F*)s->in.body_force[1];
     }
+}
+
+// the grid-node update after an EF_TILE element launch (the tile kernel has
+// summed every grid node's slots)
+template <class F>
+static void build_tile_update_args(fem_stokes* s, const DRSet<F>& d, const GatherArgs<F>& g, TileUpdArgs<F>& P) {
+    fem_mesh* m = s->mesh;
+    P.g = g;
+    P.ntx = m->nx / 16; P.nty = m->ny / (TBLOCK / 32);
+    P.TH = 2 * (TBLOCK / 32);
+    P.sx = 2 * m->nx + 1;
+    P.nT6 = (int64_t)P.sx * (2 * m->ny + 1);
+    P.bpart = d.bpart;
+    P.Rn = d.Rn;
+}
+
+template <class F>
+static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
+    fem_mesh* m = s->mesh;
+    GatherArgs<F> g;
+    build_gather_args(s, d, o, g);
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     if (d.tile && o.use_pnum && m->nx > 0 && !nodeph) {
-        // the tile kernel has summed every grid node's slots
         TileUpdArgs<F> P;
-        P.g = g;
-        P.ntx = m->nx / 16; P.nty = m->ny / (TBLOCK / 32);
-        P.TH = 2 * (TBLOCK / 32);
-        P.sx = 2 * m->nx + 1;
-        P.nT6 = (int64_t)P.sx * (2 * m->ny + 1);
-        P.bpart = d.bpart;
-        P.Rn = d.Rn;
+        build_tile_update_args(s, d, g, P);
         hipLaunchKernelGGL((k_tile_update<F, GBLOCK>), dim3(grid_xcd(P.nT6, GBLOCK)), dim3(GBLOCK), 0,
                            s->ctx->stream, P);
         return check_launch(s->ctx, "k_tile_update");
@@ -2149,39 +2172,575 @@ int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, 
 int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
                      fem_solve_stats* st);
 
-// Independent solves, concurrently: host threads take states off a shared
-// counter, largest mesh first, and run fem_stokes_solve on each; every state
-// has its own context, so each solve runs on its own stream and the GPU
-// overlaps their launches.
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched independent solves on one stream. Every problem keeps the control
+// flow of the single solve (the mixed-precision solve: FP64 state and outer
+// residuals, FP32 inner iterations on a correction; its decisions, criteria
+// and iteration counts are those of solve_mixed), but the inner iterations of
+// all problems run as grouped launches: one element launch and one grid-node
+// update launch per iteration cover every problem's tiles (k_t7_element_group,
+// k_tile_update_group, per-problem arguments and Chebyshev scalars in device
+// tables), and one check interval of those launches is captured once per
+// active set and replayed as a graph. The per-problem work at checks and
+// outer iterations (reductions, folds, FP64 residuals) is enqueued problem by
+// problem on the same stream with one host synchronisation per round for all
+// of them. Problems leave the group as they finish.
+// ---------------------------------------------------------------------------
+struct BatchItem {
+    int32_t idx = 0;
+    fem_stokes* s = nullptr;
+    const fem_solve_params* p = nullptr;
+    LaunchOpts o, oo;              // inner (correction) and outer-residual options
+    DRSet<double> d64;
+    DRSet<float> d32;
+    double lmax[2] = {0, 0}, al[2] = {0, 0}, be[2] = {0, 0};
+    double err_min = INFINITY, err = 0, err_abs = INFINITY, err_rel = INFINITY, err_v0 = 1, err_P0 = 1, err_v00 = 1;
+    double err_v = INFINITY, err_P = INFINITY, rel_drop = 0, eps_vel = 0;
+    int64_t iter = 0, itPH = 0, itPT = 0, launches = 0;
+    bool need_outer = true, done = false, converged = false, stop_total = false;
+    int rc = FEM_OK;
+    double seconds = 0;
+};
+
+// the problems a grouped round can take: 2-D structured meshes with the FP32
+// tile kernels, mixed precision, no yield model, no array keywords, one context
+static bool batch_groupable(int32_t n, fem_stokes* const* states, const fem_drucker_prager* const* plastic,
+                            const fem_solve_params* params) {
+    for (int32_t i = 0; i < n; ++i) {
+        fem_stokes* s = states[i];
+        const fem_solve_inputs& in = s->in;
+        if (s->ctx != states[0]->ctx || s->dim != 2 || !tile_ok(s->mesh) || params[i].precision != 1 ||
+            (plastic && plastic[i]) || params[i].check_interval < 2 || in.viscosity || in.scaling_viscosity ||
+            in.P_old || in.history || in.body_force[0] || in.body_force[1] ||
+            s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT)
+            return false;
+    }
+    return true;
+}
+
+// set-up of one problem: solve_2d's and solve_mixed's work before the loop
+static int batch_setup(BatchItem& b, const fem_bc* bcs) {
+    fem_stokes* s = b.s;
+    const fem_solve_params* p = b.p;
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP;
+    CHK(prepare(s, nullptr, p->dt, p->finite_K, b.o, p->use_tau_old != 0));
+    CHK(upload_bcs(s, bcs));
+    if (!p->skip_scaling) CHK(scaling_impl(s, p->pressure_factor, p->dt));
+    else
+        hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                           (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    CHK(ensure_w32(s, false));
+    fem_stokes::W32& w = *s->w32;
+    b.d64 = set64(s);
+    b.d32 = set32(s);
+    CHK(cast_to32(c, b.d64.gammaP, w.gammaP, np));
+    CHK(cast_to32(c, b.d64.MP, w.MP, np));
+    CHK(cast_to32(c, s->d_pscale, w.pscale, np));
+    hipLaunchKernelGGL(k_pin_bc, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
+                       (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, b.d64.vx, b.d64.vy);
+    CHK(check_launch(c, "k_pin_bc"));
+    LaunchOpts oj = b.o;
+    oj.plastic = false;
+    CHK(launch_jacobian(s, oj));
+    CHK(checked_lmax(s, b.lmax));
+    CHK(cast_to32(c, b.d64.PC_x, w.PC_x, nv));
+    CHK(cast_to32(c, b.d64.PC_y, w.PC_y, nv));
+    for (float* z : {w.vx, w.vy, w.rate_x, w.rate_y, w.Rv0_x, w.Rv0_y}) HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+    for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(b.lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &b.al[k], &b.be[k]);
+    b.err = 2 * p->tolerance;
+    b.rel_drop = p->rel_drop0;
+    b.oo = b.o;
+    b.oo.use_pnum = false; b.oo.do_update = false; b.oo.store_RP = true; b.oo.store_Rv = true;
+    b.o.plastic = false;
+    b.o.elastic = false;   // the visco-elastic correction operator reads no stress history
+    return FEM_OK;
+}
+
+// FP64 residual at x and its right-hand-side casts (solve_mixed's residual64 + casts)
+static int batch_residual64(BatchItem& b) {
+    fem_stokes* s = b.s;
+    fem_ctx* c = s->ctx;
+    fem_stokes::W32& w = *s->w32;
+    const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP;
+    CHK(launch_element(s, b.d64, b.oo));
+    CHK(launch_gather(s, b.d64, b.oo));
+    CHK(cast_to32(c, b.d64.Rv_x, w.rhs_vx, nv));
+    CHK(cast_to32(c, b.d64.Rv_y, w.rhs_vy, nv));
+    return cast_to32(c, b.d64.RP, w.rhs_P, np);
+}
+
+// the inner-loop segment of one problem launched on its own (segments cut
+// short by an iteration cap); solve_mixed's seg without graphs
+static int batch_segment_single(BatchItem& b, int64_t nseg) {
+    fem_stokes* s = b.s;
+    const int64_t ncheck = b.p->check_interval;
+    const int64_t iter_end = b.iter + nseg;
+    const bool lastp = iter_end % ncheck == 0 || b.itPT + nseg > b.p->iter_max || iter_end > b.p->max_iterations;
+    const int64_t mj = ncheck - 1 - b.iter % ncheck;
+    const int64_t mpos = mj >= 1 && mj <= nseg ? mj : 0;
+    LaunchOpts oi = b.o;
+    oi.use_pnum = true;
+    oi.do_update = true;
+    oi.alpha[0] = b.al[0]; oi.alpha[1] = b.al[1];
+    oi.beta[0] = b.be[0]; oi.beta[1] = b.be[1];
+    for (int64_t j = 1; j <= nseg; ++j) {
+        LaunchOpts oj = oi;
+        oj.store_RP = oj.store_Pnum = oj.store_Rv = lastp && j == nseg;
+        oj.store_Rv0 = j == mpos;
+        CHK(launch_element(s, b.d32, oj));
+        CHK(launch_gather(s, b.d32, oj));
+    }
+    return FEM_OK;
+}
+
+// device tables of one grouped round: element and update arguments of the
+// three launch variants (plain, the iteration before a check that stores Rv0,
+// the check iteration that stores RP, Pnum and Rv), block offsets, and the
+// per-problem Chebyshev scalars the launches read
+struct BatchGroup {
+    int fl = 0;
+    std::vector<int32_t> members;          // indices into the item list
+    ElemArgs<float>* d_ea = nullptr;       // [3][n]
+    TileUpdArgs<float>* d_ua = nullptr;    // [3][n]
+    int32_t* d_eoff = nullptr;             // [n + 1] tile offsets
+    int32_t* d_uoff = nullptr;             // [n + 1] update-block offsets
+    int64_t eblocks = 0, ublocks = 0;
+    hipGraphExec_t exec = nullptr;
+    int64_t ncheck = 0;
+};
+
+// the grouped FP32 tile kernels: the visco-elastic correction operator with or without finite bulk terms
+static bool group_key_ok(int key) {
+    return key == (EF_LIN | EF_ITER | EF_STRUCT | EF_TILE) || key == (EF_LIN | EF_ITER | EF_STRUCT | EF_TILE | EF_BULK);
+}
+template <int I>
+static void launch_group_lin(int key, hipStream_t st, unsigned grid, const ElemArgs<float>* ea, const int32_t* off,
+                             int32_t n) {
+    constexpr int FL = EF_LIN | EF_ITER | EF_STRUCT | EF_TILE | ((I & 1) ? EF_BULK : 0);
+    if (key == FL) {
+        hipLaunchKernelGGL((k_t7_element_group<float, FL, TBLOCK>), dim3(grid), dim3(TBLOCK), 0, st, ea, off, n);
+        return;
+    }
+    if constexpr (I > 0) launch_group_lin<I - 1>(key, st, grid, ea, off, n);
+}
+
+static void free_group(BatchGroup& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (void* q : {(void*)g.d_ea, (void*)g.d_ua, (void*)g.d_eoff, (void*)g.d_uoff})
+        if (q) (void)hipFree(q);
+    g = BatchGroup();
+}
+
+// builds the tables and the interval graph of the active inner problems with key fl
+static int build_group(fem_ctx* c, std::vector<BatchItem>& items, const std::vector<int32_t>& mem, int fl,
+                       float* d_cheb, int64_t ncheck, BatchGroup& g) {
+    free_group(g);
+    if (!group_key_ok(fl)) return set_err(c, FEM_EARG, "no grouped element kernel for key %d", fl);
+    g.fl = fl;
+    g.members = mem;
+    g.ncheck = ncheck;
+    const int32_t n = (int32_t)mem.size();
+    std::vector<ElemArgs<float>> ea(3 * n);
+    std::vector<TileUpdArgs<float>> ua(3 * n);
+    std::vector<int32_t> eoff(n + 1, 0), uoff(n + 1, 0);
+    for (int32_t k = 0; k < n; ++k) {
+        BatchItem& b = items[mem[k]];
+        for (int v = 0; v < 3; ++v) {
+            LaunchOpts oj = b.o;
+            oj.use_pnum = true;
+            oj.do_update = true;
+            oj.store_Rv0 = v == 1;
+            oj.store_RP = oj.store_Pnum = oj.store_Rv = v == 2;
+            oj.cheb = d_cheb + 4 * (int64_t)b.idx;
+            int key = 0;
+            CHK(build_elem_args(b.s, b.d32, oj, ea[v * n + k], key));
+            if (key != fl) return set_err(c, FEM_EARG, "batched problem %d has another element kernel", b.idx);
+            GatherArgs<float> ga;
+            build_gather_args(b.s, b.d32, oj, ga);
+            build_tile_update_args(b.s, b.d32, ga, ua[v * n + k]);
+        }
+        eoff[k + 1] = eoff[k] + (int32_t)ea[k].ntiles;
+        uoff[k + 1] = uoff[k] + (int32_t)grid_for(ua[k].nT6, GBLOCK);
+    }
+    g.eblocks = eoff[n];
+    g.ublocks = uoff[n];
+    CHK(dalloc_t(c, &g.d_ea, 3 * n));
+    CHK(dalloc_t(c, &g.d_ua, 3 * n));
+    CHK(dalloc_t(c, &g.d_eoff, n + 1));
+    CHK(dalloc_t(c, &g.d_uoff, n + 1));
+    HIPCHK(c, hipMemcpyAsync(g.d_ea, ea.data(), ea.size() * sizeof(ea[0]), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.d_ua, ua.data(), ua.size() * sizeof(ua[0]), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.d_eoff, eoff.data(), eoff.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.d_uoff, uoff.data(), uoff.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+    // one check interval: ncheck (element, update) pairs
+    const unsigned ge = grid_xcd(g.eblocks, 1), gu = grid_xcd(g.ublocks, 1);
+    auto launches = [&]() -> int {
+        for (int64_t j = 1; j <= ncheck; ++j) {
+            const int v = j == ncheck ? 2 : (j == ncheck - 1 ? 1 : 0);
+            launch_group_lin<1>(fl, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
+            hipLaunchKernelGGL((k_tile_update_group<float, GBLOCK>), dim3(gu), dim3(GBLOCK), 0, c->stream,
+                               (const TileUpdArgs<float>*)(g.d_ua + v * n), (const int32_t*)g.d_uoff, n);
+        }
+        return check_launch(c, "k_t7_element_group");
+    };
+    if (!graphs_on()) return FEM_OK;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launches();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != FEM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHK(c, e);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(c, ei);
+    return FEM_OK;
+}
+
+static int run_group(fem_ctx* c, BatchGroup& g) {
+    if (g.exec) {
+        HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+        return FEM_OK;
+    }
+    const int32_t n = (int32_t)g.members.size();
+    const unsigned ge = grid_xcd(g.eblocks, 1), gu = grid_xcd(g.ublocks, 1);
+    for (int64_t j = 1; j <= g.ncheck; ++j) {
+        const int v = j == g.ncheck ? 2 : (j == g.ncheck - 1 ? 1 : 0);
+        launch_group_lin<1>(g.fl, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
+        hipLaunchKernelGGL((k_tile_update_group<float, GBLOCK>), dim3(gu), dim3(GBLOCK), 0, c->stream,
+                           (const TileUpdArgs<float>*)(g.d_ua + v * n), (const int32_t*)g.d_uoff, n);
+    }
+    return check_launch(c, "k_t7_element_group");
+}
+
+static void batch_finish(BatchItem& b, fem_solve_stats* st, double t) {
+    std::memset(st, 0, sizeof *st);
+    st->itPH = b.itPH;
+    st->iter = b.iter;
+    st->err = b.err;
+    st->err_abs = b.err_abs;
+    st->err_rel = b.err_rel;
+    st->err_v = b.err_v;
+    st->err_P = b.err_P;
+    st->converged = b.converged;
+    st->reached_total_iter = b.iter > b.p->max_iterations;
+    st->lambda_max = std::max(b.lmax[0], b.lmax[1]);
+    st->lambda_max_gershgorin = st->lambda_max;
+    st->jacobian_assemblies = 1;
+    st->element_launches = b.launches;
+    st->seconds = t;
+    b.done = true;
+}
+
+static int solve_batch_grouped(int32_t n, fem_stokes* const* states, const fem_bc* const* bcs,
+                               const fem_solve_params* params, fem_solve_stats* stats, int32_t* status) {
+    fem_ctx* c = states[0]->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    std::vector<BatchItem> items(n);
+    for (int32_t i = 0; i < n; ++i) {
+        BatchItem& b = items[i];
+        b.idx = i;
+        b.s = states[i];
+        b.p = &params[i];
+        std::memset(&stats[i], 0, sizeof stats[i]);
+        status[i] = batch_setup(b, bcs[i]);
+        if (status[i] != FEM_OK) { b.rc = status[i]; b.done = true; }
+    }
+    // per-problem Chebyshev scalars (α_x, α_y, β_x, β_y), staged in pinned memory
+    float* d_cheb = nullptr;
+    float* h_cheb = nullptr;
+    double* d_red = nullptr;   // per-problem partial sums (RED_GRID x 6) and results (6)
+    double* h_red = nullptr;
+    CHK(dalloc_t(c, &d_cheb, 4 * (int64_t)n));
+    CHK(dalloc_t(c, &d_red, (int64_t)n * (RED_GRID * 6 + 6)));
+    if (hipHostMalloc((void**)&h_cheb, 16 * (size_t)n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_red, 48 * (size_t)n, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(d_cheb);
+        (void)hipFree(d_red);
+        if (h_cheb) (void)hipHostFree(h_cheb);
+        return set_err(c, FEM_EHIP, "hipHostMalloc failed");
+    }
+    std::vector<BatchGroup> groups;
+    std::vector<int32_t> group_key_members;   // last active grouped set (for rebuilds)
+    int rc = FEM_OK;
+    auto fail = [&](BatchItem& b, int r) {
+        b.rc = r;
+        status[b.idx] = r;
+        batch_finish(b, &stats[b.idx], now());
+    };
+    // the reductions of one phase: every listed problem's partial and final
+    // kernels, one copy of all results, one synchronisation
+    auto reduce_all = [&](const std::vector<int32_t>& which, bool outer) -> int {
+        for (int32_t i : which) {
+            BatchItem& b = items[i];
+            fem_stokes* s = b.s;
+            const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP;
+            double* part = d_red + (int64_t)i * (RED_GRID * 6 + 6);
+            double* out = part + RED_GRID * 6;
+            if (outer) {
+                const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, np), RED_BLOCK));
+                hipLaunchKernelGGL(k_reduce_outer<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np,
+                                   (const double*)b.d64.Rv_x, (const double*)b.d64.Rv_y, (const double*)b.d64.RP,
+                                   (const double*)b.d64.MP, part);
+                hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
+                                   (const double*)part, out);
+            } else {
+                const DRSet<float>& d = b.d32;
+                const unsigned g = std::min<unsigned>(RED_GRID, grid_for(nv, RED_BLOCK));
+                hipLaunchKernelGGL(k_reduce_check<float>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv,
+                                   (const float*)d.Rv_x, (const float*)d.Rv_y, (const float*)d.Rv0_x,
+                                   (const float*)d.Rv0_y, (const float*)d.rate_x, (const float*)d.rate_y,
+                                   (const float*)d.PC_x, (const float*)d.PC_y, b.al[0], b.al[1], part);
+                hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 6, 0,
+                                   (const double*)part, out);
+            }
+            CHK(check_launch(c, "batched reductions"));
+            HIPCHK(c, hipMemcpyAsync(h_red + 6 * i, out, 48, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return FEM_OK;
+    };
+    while (rc == FEM_OK) {
+        // --- outer iterations (solve_mixed's itPH head) of the problems that need one
+        std::vector<int32_t> outer;
+        for (int32_t i = 0; i < n; ++i)
+            if (!items[i].done && items[i].need_outer) outer.push_back(i);
+        for (int32_t i : outer) {
+            BatchItem& b = items[i];
+            const int r = batch_residual64(b);
+            if (r != FEM_OK) fail(b, r);
+        }
+        if ((rc = reduce_all(outer, true)) != FEM_OK) break;
+        for (int32_t i : outer) {
+            BatchItem& b = items[i];
+            if (b.done) continue;
+            const double* r3 = h_red + 6 * i;
+            const int64_t nv = b.s->mesh->nnodes, np = b.s->mesh->nnodesP;
+            ++b.itPH;
+            b.err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
+            b.err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
+            if (b.itPH == 1) { b.err_P0 = b.err_P + julia_eps(b.err_P); b.err_v0 = b.err_v + julia_eps(b.err_v); }
+            else if (b.itPH == 2) { b.err_P0 = b.err_P + julia_eps(b.err_P); }
+            b.err_abs = std::max(b.err_v, b.err_P);
+            b.err_rel = std::max(b.err_v / b.err_v0, b.err_P / b.err_P0);
+            b.err = std::min(b.err_abs, b.err_rel);
+            if (std::isnan(b.err_v) || std::isnan(b.err_P) || std::isnan(b.err)) {
+                fail(b, set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)b.itPH));
+                continue;
+            }
+            if (b.err > 1.0e10) {
+                fail(b, set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)b.itPH));
+                continue;
+            }
+            if (b.err < b.p->tolerance) {
+                b.converged = true;
+                batch_finish(b, &stats[i], now());
+                continue;
+            }
+            if (b.err > b.err_min * 1.05) b.rel_drop = std::max(b.rel_drop * 0.1, 1.0e-3);
+            b.err_min = std::min(b.err_min, b.err);
+            b.eps_vel = b.err * b.rel_drop;
+            b.itPT = 0;
+            b.need_outer = false;   // the right-hand sides were cast with the residual
+        }
+        // --- one check interval of inner iterations for every problem in its inner loop
+        std::vector<int32_t> inner, grouped;
+        for (int32_t i = 0; i < n; ++i)
+            if (!items[i].done && !items[i].need_outer) inner.push_back(i);
+        if (inner.empty()) {
+            bool all = true;
+            for (auto& b : items) all = all && b.done;
+            if (all) break;
+            continue;
+        }
+        std::vector<int64_t> nseg(n, 0);
+        for (int32_t i : inner) {
+            BatchItem& b = items[i];
+            const int64_t ncheck = b.p->check_interval;
+            int64_t k = ncheck - b.iter % ncheck;
+            k = std::max<int64_t>(1, std::min(k, std::min(b.p->iter_max - b.itPT + 1, b.p->max_iterations - b.iter + 1)));
+            nseg[i] = k;
+            if (k == ncheck && ncheck == items[inner[0]].p->check_interval) grouped.push_back(i);
+        }
+        // Chebyshev scalars of this interval
+        for (int32_t i : inner) {
+            const BatchItem& b = items[i];
+            h_cheb[4 * i] = (float)b.al[0]; h_cheb[4 * i + 1] = (float)b.al[1];
+            h_cheb[4 * i + 2] = (float)b.be[0]; h_cheb[4 * i + 3] = (float)b.be[1];
+        }
+        if ((rc = [&]() -> int {
+                HIPCHK(c, hipMemcpyAsync(d_cheb, h_cheb, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+                return FEM_OK;
+            }()) != FEM_OK)
+            break;
+        if (!grouped.empty()) {
+            // problems of one element-kernel key share a group; rebuild when the set changed
+            std::map<int, std::vector<int32_t>> bykey;
+            for (int32_t i : grouped) {
+                ElemArgs<float> a;
+                int key = 0;
+                LaunchOpts oj = items[i].o;
+                oj.use_pnum = true;
+                if ((rc = build_elem_args(items[i].s, items[i].d32, oj, a, key)) != FEM_OK) break;
+                bykey[key].push_back(i);
+            }
+            if (rc != FEM_OK) break;
+            bool same = groups.size() == bykey.size();
+            if (same) {
+                size_t gi = 0;
+                for (auto& kv : bykey) same = same && groups[gi].fl == kv.first && groups[gi++].members == kv.second;
+            }
+            if (!same) {
+                for (auto& g : groups) free_group(g);
+                groups.assign(bykey.size(), BatchGroup());
+                size_t gi = 0;
+                for (auto& kv : bykey) {
+                    if ((rc = build_group(c, items, kv.second, kv.first, d_cheb, items[kv.second[0]].p->check_interval,
+                                          groups[gi++])) != FEM_OK)
+                        break;
+                }
+                if (rc != FEM_OK) break;
+                if (graphs_on())
+                    HIPCHK(c, hipMemcpyAsync(d_cheb, h_cheb, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+            }
+            for (auto& g : groups)
+                if ((rc = run_group(c, g)) != FEM_OK) break;
+            if (rc != FEM_OK) break;
+        }
+        for (int32_t i : inner) {
+            if (std::find(grouped.begin(), grouped.end(), i) != grouped.end()) continue;
+            const int r = batch_segment_single(items[i], nseg[i]);
+            if (r != FEM_OK) fail(items[i], r);
+        }
+        for (int32_t i : inner) {
+            BatchItem& b = items[i];
+            if (b.done) continue;
+            b.itPT += nseg[i];
+            b.iter += nseg[i];
+            b.launches += nseg[i];
+        }
+        // --- checks
+        std::vector<int32_t> checks;
+        for (int32_t i : inner)
+            if (!items[i].done && items[i].iter % items[i].p->check_interval == 0) checks.push_back(i);
+        if ((rc = reduce_all(checks, false)) != FEM_OK) break;
+        for (int32_t i : checks) {
+            BatchItem& b = items[i];
+            fem_stokes* s = b.s;
+            fem_stokes::W32& w = *s->w32;
+            const int64_t nv = s->mesh->nnodes;
+            const double* r6 = h_red + 6 * i;
+            const double evi = std::max(std::sqrt(r6[0]), std::sqrt(r6[1])) / (2 * std::sqrt((double)nv));
+            if (b.iter == b.p->check_interval) b.err_v00 = evi + julia_eps(evi);
+            b.err = evi <= b.p->tolerance ? evi : std::max(evi / b.err_v00, evi);
+            if (std::isnan(b.err)) {
+                fail(b, set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)b.itPH,
+                                (long long)b.itPT));
+                continue;
+            }
+            if (b.err > 1.0e10) {
+                fail(b, set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld",
+                                (long long)b.itPH, (long long)b.itPT));
+                continue;
+            }
+            double lmin[2];
+            for (int k = 0; k < 2; ++k) {
+                const double den = r6[2 + 2 * k], num = r6[3 + 2 * k];
+                lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+            }
+            const int r = [&]() -> int {
+                CHK(add_correction(c, b.d64.vx, w.vx, nv, 1.0));
+                CHK(add_correction(c, b.d64.vy, w.vy, nv, 1.0));
+                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
+                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                return batch_residual64(b);
+            }();
+            if (r != FEM_OK) { fail(b, r); continue; }
+            for (int k = 0; k < 2; ++k)
+                cheb(2 / std::sqrt(b.lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &b.al[k], &b.be[k]);
+        }
+        // --- ends of inner loops: P += γP RP / M_P on the correction, fold δ
+        for (int32_t i : inner) {
+            BatchItem& b = items[i];
+            if (b.done) continue;
+            if (b.iter > b.p->max_iterations) b.stop_total = true;
+            if (!b.stop_total && b.err > b.eps_vel && b.itPT <= b.p->iter_max) continue;
+            fem_stokes* s = b.s;
+            fem_stokes::W32& w = *s->w32;
+            const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP;
+            const int r = [&]() -> int {
+                hipLaunchKernelGGL(k_pressure_update<float>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
+                                   w.P, (const float*)w.gammaP, (const float*)w.RP, (const float*)w.MP);
+                CHK(check_launch(c, "k_pressure_update"));
+                CHK(add_correction(c, b.d64.vx, w.vx, nv, 1.0));
+                CHK(add_correction(c, b.d64.vy, w.vy, nv, 1.0));
+                CHK(add_correction(c, b.d64.P, w.P, np, 1.0));
+                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
+                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+                return FEM_OK;
+            }();
+            if (r != FEM_OK) { fail(b, r); continue; }
+            if (b.stop_total || b.itPH >= b.p->max_ph_iterations) batch_finish(b, &stats[i], now());
+            else b.need_outer = true;
+        }
+    }
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    for (auto& g : groups) free_group(g);
+    (void)hipFree(d_cheb);
+    (void)hipFree(d_red);
+    (void)hipHostFree(h_cheb);
+    (void)hipHostFree(h_red);
+    if (rc != FEM_OK) return rc;
+    HIPCHK(c, es);
+    for (int32_t i = 0; i < n; ++i) {
+        BatchItem& b = items[i];
+        if (status[i] == FEM_OK && !b.converged && b.p->throw_on_failure)
+            status[i] = set_err(c, FEM_ENOCONV,
+                                "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                                (long long)b.iter, b.err_abs, b.p->tolerance);
+    }
+    return FEM_OK;
+}
+
+extern "C" {
+
+int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,
+                     fem_solve_stats* st);
+
 int fem_stokes_solve_batch(int32_t n, fem_stokes* const* states, const fem_bc* const* bcs,
                            const fem_drucker_prager* const* plastic, const fem_solve_params* params,
                            fem_solve_stats* stats, int32_t* status, int32_t nthreads) {
+    (void)nthreads;
     if (n < 0) return FEM_EARG;
     if (n == 0) return FEM_OK;
     if (!states || !bcs || !params || !stats || !status) return FEM_EARG;
-    std::set<fem_ctx*> ctxs;
     for (int32_t i = 0; i < n; ++i) {
         if (!states[i] || !bcs[i]) return FEM_EARG;
-        if (!ctxs.insert(states[i]->ctx).second)
+        if (states[i]->ctx != states[0]->ctx)
             return set_err(states[i]->ctx, FEM_EARG,
-                           "batched solves need one context (stream) per state; state %d shares its context", i);
+                           "batched solves share one context (stream): state %d has another", i);
+        status[i] = FEM_OK;
     }
-    std::vector<int32_t> order(n);
-    for (int32_t i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return states[a]->mesh->nels > states[b]->mesh->nels; });
-    const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(nthreads <= 0 ? n : nthreads, n));
-    std::atomic<int32_t> next(0);
-    auto worker = [&]() {
-        for (int32_t k; (k = next.fetch_add(1)) < n;) {
-            const int32_t i = order[k];
+    int rc = FEM_OK;
+    if (batch_groupable(n, states, plastic, params)) {
+        rc = solve_batch_grouped(n, states, bcs, params, stats, status);
+    } else {
+        // the other problem kinds run one after the other on the shared stream
+        for (int32_t i = 0; i < n; ++i)
             status[i] = fem_stokes_solve(states[i], bcs[i], plastic ? plastic[i] : nullptr, &params[i], &stats[i]);
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int32_t t = 1; t < nt; ++t) pool.emplace_back(worker);
-    worker();
-    for (auto& t : pool) t.join();
+    }
+    if (rc != FEM_OK) return rc;
     for (int32_t i = 0; i < n; ++i)
         if (status[i] != FEM_OK) return status[i];
     return FEM_OK;
@@ -2458,6 +3017,71 @@ int fem_norms_noalloc(fem_ctx* c, const double* Rv_x, const double* Rv_y, int64_
     hipLaunchKernelGGL(k_norms_finish, dim3(1), dim3(64), 0, c->stream, nv, np, (const double*)sums, dev_out);
     HIPCHK(c, hipGetLastError());
     return FEM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Kernel-level pieces of the inner DR step, for tests and for callers that
+// assemble their own loop.
+// ---------------------------------------------------------------------------
+// update_stokes_velocity_kernel! (helpers.jl:31-41) with the negated step the
+// solver passes (DR.jl:690-695): rate = β rate + R / PC, v -= α rate
+__global__ void k_dr_velocity_update(int64_t n, double* rate, double* v, const double* R, const double* PC,
+                                     double alpha, double beta) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = beta * rate[i] + R[i] / PC[i];
+    rate[i] = r;
+    v[i] += (-alpha) * r;
+}
+
+// the visco-elasto-plastic stress with the plane-strain Drucker-Prager return
+// at independent points (rheology.jl:35-42, 226-245, 272-334): each point
+// brings its strain rate, old stress, pressure and material
+__global__ void k_dp_points(int64_t n, double dt, fem_dp_points P) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double invG = isinf(P.G[i]) ? 0.0 : 1.0 / P.G[i];
+    IpMat<double> m;
+    m.etave = 1.0 / (1.0 / P.eta[i] + invG / dt);
+    m.inv2Gdt = invG / (2.0 * dt);
+    const double sphi = sin(P.phi[i]), spsi = sin(P.psi[i]);
+    m.yield0 = cos(P.phi[i]) * P.C[i];
+    m.sphi = sphi;
+    m.invden = 1.0 / ((m.etave + P.eta_reg[i]) + ((P.Kb[i] * dt) * (-spsi)) * (-sphi));
+    double txx, tyy, txy;
+    const bool y = stress2d<double, double, true>(P.exx[i], P.eyy[i], P.exy[i], m, P.txx_old[i], P.tyy_old[i],
+                                                  P.txy_old[i], P.P[i], txx, tyy, txy);
+    P.txx[i] = txx;
+    P.tyy[i] = tyy;
+    P.txy[i] = txy;
+    if (P.yields) P.yields[i] = y ? 1 : 0;
+}
+
+extern "C" {
+
+int fem_dr_velocity_update(fem_ctx* c, int64_t n, double* rate, double* v, const double* R, const double* PC,
+                           double alpha, double beta) {
+    if (!c || n < 0 || (n > 0 && (!rate || !v || !R || !PC))) return set_err(c, FEM_EARG, "fem_dr_velocity_update: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return FEM_OK;
+    hipLaunchKernelGGL(k_dr_velocity_update, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, rate, v, R, PC,
+                       alpha, beta);
+    return check_launch(c, "k_dr_velocity_update");
+}
+
+int fem_dp_return_map_batch(fem_ctx* c, int64_t n, double dt, const fem_dp_points* pts) {
+    if (!c || !pts || n < 0) return set_err(c, FEM_EARG, "fem_dp_return_map_batch: bad arguments");
+    if (!(dt > 0) || !std::isfinite(dt)) return set_err(c, FEM_EARG, "dt must be positive and finite");
+    const void* need[] = {pts->exx, pts->eyy, pts->exy, pts->txx_old, pts->tyy_old, pts->txy_old, pts->P, pts->eta,
+                          pts->G, pts->C, pts->phi, pts->psi, pts->eta_reg, pts->Kb, pts->txx, pts->tyy, pts->txy};
+    for (const void* q : need)
+        if (n > 0 && !q) return set_err(c, FEM_EARG, "fem_dp_return_map_batch: a required array is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n == 0) return FEM_OK;
+    hipLaunchKernelGGL(k_dp_points, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, dt, *pts);
+    return check_launch(c, "k_dp_points");
 }
 
 }  // extern "C"

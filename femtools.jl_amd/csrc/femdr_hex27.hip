@@ -889,6 +889,139 @@ int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_
     return FEM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// ThermalDiffusionDR on a Hex27 box: the backward-Euler heat residual of
+// heat_diffusion/assembly/residual.jl:58-87 with per-node phases interpolated
+// at the 27 Gauss points, and on check iterations the Jacobian row sums
+// Σ_j |J_ij| and |J_ii| of the exact derivative (residual.jl:30-47 takes it with
+// ForwardDiff): with D = Δt/(ρ Cp), ρ = ρ0 (1 - α (T - Tref) + β P),
+// ∂D/∂T_j = D ρ0 α N_j / ρ. 32 lanes per element as in the Stokes kernel:
+// lane = node (gather), lane = point (point quantities), lane = node (nodal
+// rows); the 26 shared nodes write element slots, the private centre node is
+// finished (and with `update` relaxed) by its element.
+// ---------------------------------------------------------------------------
+template <bool JAC>
+__global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g, ThermHexArgs A) {
+    const int es = threadIdx.x >> 5, l = threadIdx.x & 31;
+    const int64_t e = xcd_tile() * HEPB + es;
+    const bool ev = e < g.nels;
+    __shared__ double sT[HEPB][HNV], sT0[HEPB][HNV], sS[HEPB][HNV], sP[HEPB][HNV];
+    __shared__ double sK[HEPB][HNV], sCp[HEPB][HNV], sR0[HEPB][HNV], sAl[HEPB][HNV], sBe[HEPB][HNV];
+    __shared__ double qRate[HEPB][HNQ], qD[HEPB][HNQ], qC[HEPB][HNQ], qK[HEPB][HNQ], qS[HEPB][HNQ];
+    __shared__ double qW[HEPB][HNQ], qG[HEPB][HNQ][3];
+    if (ev && l < HNV) {
+        const int64_t n = hex_node(g, e, l);
+        sT[es][l] = A.T[n]; sT0[es][l] = A.T0[n]; sS[es][l] = A.src[n]; sP[es][l] = A.P[n];
+        const int p = A.phases[n] - 1;
+        sK[es][l] = A.tab[TC_K * MAXPH + p]; sCp[es][l] = A.tab[TC_CP * MAXPH + p];
+        sR0[es][l] = A.tab[TC_RHO0 * MAXPH + p]; sAl[es][l] = A.tab[TC_ALPHA * MAXPH + p];
+        sBe[es][l] = A.tab[TC_BETA * MAXPH + p];
+    }
+    __syncthreads();
+    if (ev && l < HNQ) {
+        const int q = l;
+        const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
+        double Tq = 0, T0q = 0, sq = 0, Pq = 0, kq = 0, cq = 0, rq = 0, aq = 0, bq = 0;
+        double gT[3] = {0, 0, 0};
+        for (int t = 0; t < HNV; ++t) {
+            const int i = t % 3, j = (t / 3) % 3, k = t / 9;
+            const double L0 = lq(x0, i), L1 = lq(x1, j), L2 = lq(x2, k);
+            const double N = L0 * L1 * L2;
+            const double v = sT[es][t];
+            Tq += N * v; T0q += N * sT0[es][t]; sq += N * sS[es][t]; Pq += N * sP[es][t];
+            kq += N * sK[es][t]; cq += N * sCp[es][t]; rq += N * sR0[es][t]; aq += N * sAl[es][t];
+            bq += N * sBe[es][t];
+            gT[0] += dlq(x0, i) * L1 * L2 * v;
+            gT[1] += L0 * dlq(x1, j) * L2 * v;
+            gT[2] += L0 * L1 * dlq(x2, k) * v;
+        }
+        for (int d = 0; d < 3; ++d) gT[d] *= g.jac[d];
+        const double rho = rq * (1 - aq * (Tq - A.Tref) + bq * Pq);
+        const double D = A.dt / (rho * cq);
+        const double phq = A.phi ? A.phi[e * HNQ + q] : 0.0;
+        double rate = T0q - Tq + D * sq;
+        if (A.phi) rate += D * phq;   // Δt/(ρ Cp) Φ_q (residual.jl:75-79)
+        const double dOm = g.det * gauss_w(q % 3) * gauss_w((q / 3) % 3) * gauss_w(q / 9);
+        qRate[es][q] = rate;
+        qD[es][q] = D;
+        qC[es][q] = D * rq * aq / rho;
+        qK[es][q] = kq;
+        qS[es][q] = sq + phq;
+        qW[es][q] = dOm;
+        qG[es][q][0] = gT[0]; qG[es][q][1] = gT[1]; qG[es][q][2] = gT[2];
+    }
+    __syncthreads();
+    if (!ev || l >= HNV) return;
+    const int t = l, ti = t % 3, tj = (t / 3) % 3, tk = t / 9;
+    double r = 0;
+    for (int q = 0; q < HNQ; ++q) {
+        const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
+        const double L0 = lq(x0, ti), L1 = lq(x1, tj), L2 = lq(x2, tk);
+        const double N = L0 * L1 * L2;
+        const double g0 = dlq(x0, ti) * L1 * L2 * g.jac[0], g1 = L0 * dlq(x1, tj) * L2 * g.jac[1],
+                     g2 = L0 * L1 * dlq(x2, tk) * g.jac[2];
+        const double KTi = qK[es][q] * (g0 * qG[es][q][0] + g1 * qG[es][q][1] + g2 * qG[es][q][2]);
+        r += (qRate[es][q] * N - qD[es][q] * KTi) * qW[es][q];
+    }
+    double rs = 0, dg = 0;
+    if (JAC) {
+        for (int u = 0; u < HNV; ++u) {
+            const int ui = u % 3, uj = (u / 3) % 3, uk = u / 9;
+            double J = 0;
+            for (int q = 0; q < HNQ; ++q) {
+                const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
+                const double L0 = lq(x0, ti), L1 = lq(x1, tj), L2 = lq(x2, tk);
+                const double M0 = lq(x0, ui), M1 = lq(x1, uj), M2 = lq(x2, uk);
+                const double Nt = L0 * L1 * L2, Nu = M0 * M1 * M2;
+                const double gt0 = dlq(x0, ti) * L1 * L2 * g.jac[0], gt1 = L0 * dlq(x1, tj) * L2 * g.jac[1],
+                             gt2 = L0 * L1 * dlq(x2, tk) * g.jac[2];
+                const double gu0 = dlq(x0, ui) * M1 * M2 * g.jac[0], gu1 = M0 * dlq(x1, uj) * M2 * g.jac[1],
+                             gu2 = M0 * M1 * dlq(x2, uk) * g.jac[2];
+                const double k = qK[es][q];
+                const double KTi = k * (gt0 * qG[es][q][0] + gt1 * qG[es][q][1] + gt2 * qG[es][q][2]);
+                const double Kij = k * (gt0 * gu0 + gt1 * gu1 + gt2 * gu2);
+                const double dD = qC[es][q] * Nu;
+                J += ((-Nu + dD * qS[es][q]) * Nt - dD * KTi - qD[es][q] * Kij) * qW[es][q];
+            }
+            rs += fabs(J);
+            if (u == t) dg = fabs(J);
+        }
+    }
+    if (t != HCENTRE_T) {
+        const int64_t sl = e * HNSH + kHexA[t];
+        A.Rs[sl] = r;
+        if (JAC) A.Js[sl] = make_double2(rs, dg);
+        return;
+    }
+    // the private centre node
+    const int64_t b = hex_node(g, e, t);
+    const bool fixed = A.bcflag[b] & 1;
+    const double rr = fixed ? 0.0 : r;
+    if (JAC) {
+        A.R0[b] = A.R[b];           // R0 <- R before the assembly (solver.jl:95)
+        A.dRdT[b] = rs;
+        A.PC[b] = dg;
+    }
+    A.R[b] = rr;
+    if (A.update) {
+        const double nr = fixed ? 0.0 : rr / A.PC[b] + A.beta * A.rate[b];
+        A.rate[b] = nr;
+        A.Tw[b] = fixed ? A.bcval[b] : sT[es][t] + A.alpha * nr;
+    }
+}
+
+}  // namespace
+
+int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermHexArgs& a, bool jac) {
+    const HexGeo g = hex_geo(m);
+    const dim3 grid(grid_xcd(m->nels, HEPB));
+    if (jac) hipLaunchKernelGGL(k_thermal_hex27<true>, grid, dim3(HBLOCK), 0, c->stream, g, a);
+    else hipLaunchKernelGGL(k_thermal_hex27<false>, grid, dim3(HBLOCK), 0, c->stream, g, a);
+    return check_launch(c, "k_thermal_hex27");
+}
+
+namespace {
+
 }  // namespace
 
 // ---------------------------------------------------------------------------

@@ -246,6 +246,25 @@ static inline void cheb(double dtau, double lmin, double c_fact, double* a, doub
 int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K, bool* bulk_inf,
                 bool* thermal, bool* eos_p, bool* elastic, const fem_drucker_prager_cap* cap = nullptr);
 
+// per-phase table columns of the thermal state (heat_diffusion_types.jl:15-41);
+// β = 1/K is interpolated rather than K (residual.jl:62-64)
+enum ThermCol { TC_K = 0, TC_CP, TC_RHO0, TC_ALPHA, TC_BETA, TC_NCOL };
+
+// arguments of the thermal element kernel on a structured Hex27 mesh
+struct ThermHexArgs {
+    const double *T, *T0, *src, *P;
+    const int32_t* phases;                 // per node, 1-based
+    const double* tab;
+    double dt, Tref;
+    double* Rs; double2* Js;               // [nels][26] slots of the shared nodes
+    double *R, *R0, *dRdT, *PC, *rate, *Tw;
+    const uint8_t* bcflag; const double* bcval;
+    double alpha, beta;
+    int32_t update;
+    const double* phi;                     // nullable: source at the quadrature points [nels][27]
+};
+int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermHexArgs& a, bool jac);
+
 // The thermal side of a coupled thermal-Stokes solve (DR.jl:514-529,
 // 604-661): one thermal DR iteration per Stokes inner iteration, on the
 // thermal state's stream, with the Chebyshev scalars and the convergence

@@ -574,8 +574,9 @@ __device__ inline void node_update(int64_t node, F rx, F ry, const F* rhs_vx, co
     }
 }
 
+// The element kernel's body: block blk of the launch of one problem.
 template <class F, int FL, int BLOCK>
-__global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
+__device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int64_t blk) {
     constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
     constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT, LIN = FL & EF_LIN;
     constexpr bool DELTA = FL & EF_DELTA, TILE = FL & EF_TILE;
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
     bool valid;
     if (TILE) {
         // thread t: triangle t & 1 of cell (t >> 1) & 15 in row t >> 5 of the tile
-        tile = xcd_tile();
+        tile = blk;
         if (tile >= A.ntiles) return;                // whole block
         tty = tile / A.ntx;
         ttx = tile - tty * A.ntx;
@@ -597,7 +598,7 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
         e = e0 + 2 * (int64_t)(threadIdx.x >> 5) * A.nx + (threadIdx.x & 31);
         valid = true;
     } else {
-        e0 = xcd_tile() * BLOCK;                     // first element of this block
+        e0 = blk * BLOCK;                            // first element of this block
         valid = e0 + threadIdx.x < ne;
         e = valid ? e0 + threadIdx.x : ne - 1;       // idle lanes recompute the last element, store nothing
     }
@@ -1104,6 +1105,38 @@ __global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> 
 #undef PT
 }
 
+template <class F, int FL, int BLOCK>
+__global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element(ElemArgs<F> A) {
+    t7_element_body<F, FL, BLOCK>(A, xcd_tile());
+}
+
+// The problem of a grouped launch that owns block b: the launch covers the
+// blocks of nprob problems back to back, problem p from boff[p] (ascending,
+// boff[0] = 0) to boff[p + 1]; -1 for the padding blocks past boff[nprob].
+__device__ inline int group_of(int64_t b, const int32_t* __restrict__ boff, int32_t nprob) {
+    if (b >= boff[nprob]) return -1;
+    int lo = 0, hi = nprob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (boff[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One launch of the element kernel for many independent problems (batched
+// solves): each problem keeps its own arguments (arrays, scalars, Chebyshev
+// pointer) in args[p] and its own block range.
+template <class F, int FL, int BLOCK>
+__global__ __launch_bounds__(BLOCK) FEM_ELEM_ATTR void k_t7_element_group(const ElemArgs<F>* __restrict__ args,
+                                                                          const int32_t* __restrict__ boff,
+                                                                          int32_t nprob) {
+    const int64_t b = xcd_tile();
+    const int p = group_of(b, boff, nprob);
+    if (p < 0) return;
+    t7_element_body<F, FL, BLOCK>(args[p], b - boff[p]);
+}
+
 // ---------------------------------------------------------------------------
 // Node-centric gather + Dirichlet + Chebyshev update (helpers.jl:31-41,
 // dynamic_relaxation/solver.jl:139-166; DR.jl:686-697)
@@ -1159,8 +1192,8 @@ struct TileUpdArgs {
 };
 
 template <class F, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_tile_update(TileUpdArgs<F> P) {
-    const int64_t node = xcd_tile() * BLOCK + threadIdx.x;
+__device__ __forceinline__ void tile_update_body(const TileUpdArgs<F>& P, const int64_t blk) {
+    const int64_t node = blk * BLOCK + threadIdx.x;
     if (node >= P.nT6) return;
     constexpr int TW = 32;
     const int TH = P.TH, NPER = 2 * (TW + 1) + 2 * (TH - 1);
@@ -1187,6 +1220,21 @@ __global__ __launch_bounds__(BLOCK) void k_tile_update(TileUpdArgs<F> P) {
                    A.Rv0_out_x, A.Rv0_out_y, A.do_update, A.cheb ? A.cheb[0] : A.alpha_x,
                    A.cheb ? A.cheb[1] : A.alpha_y, A.cheb ? A.cheb[2] : A.beta_x, A.cheb ? A.cheb[3] : A.beta_y,
                    A.rate_x, A.rate_y, A.PCx, A.PCy, A.vx, A.vy);
+}
+
+template <class F, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tile_update(TileUpdArgs<F> P) {
+    tile_update_body<F, BLOCK>(P, xcd_tile());
+}
+
+// grouped launch of k_tile_update (see k_t7_element_group)
+template <class F, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tile_update_group(const TileUpdArgs<F>* __restrict__ args,
+                                                             const int32_t* __restrict__ boff, int32_t nprob) {
+    const int64_t b = xcd_tile();
+    const int p = group_of(b, boff, nprob);
+    if (p < 0) return;
+    tile_update_body<F, BLOCK>(args[p], b - boff[p]);
 }
 
 // ---------------------------------------------------------------------------

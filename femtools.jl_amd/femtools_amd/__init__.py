@@ -34,7 +34,7 @@ __all__ = [
     "write_stokes_vtk", "VtkMesh",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
-    "norms", "device_array",
+    "norms", "device_array", "dr_velocity_update", "dp_return_map_batch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -219,6 +219,15 @@ _sig("fem_stokes_set_solve_inputs", C.c_int, VP, _P(_SolveInputs))
 _sig("fem_stokes_postprocess", C.c_int, VP, VP)
 _sig("fem_norms", C.c_int, VP, VP, VP, I64, VP, VP, I64, _P(D))
 _sig("fem_norms_noalloc", C.c_int, VP, VP, VP, I64, VP, VP, I64, VP)
+_sig("fem_dr_velocity_update", C.c_int, VP, I64, VP, VP, VP, VP, D, D)
+
+
+class _DPPoints(C.Structure):
+    _fields_ = [(k, VP) for k in ("exx", "eyy", "exy", "txx_old", "tyy_old", "txy_old", "P", "eta", "G", "C", "phi",
+                                  "psi", "eta_reg", "Kb", "txx", "tyy", "txy", "yields")]
+
+
+_sig("fem_dp_return_map_batch", C.c_int, VP, I64, D, _P(_DPPoints))
 _sig("fem_cnn_create", C.c_int, VP, I32, I32, I32, _P(VP))
 _sig("fem_cnn_destroy", C.c_int, VP)
 _sig("fem_cnn_parameter_count", C.c_int, _P(I64), _P(I64))
@@ -252,7 +261,7 @@ EXPORTED = [
     "fem_stokes_set_pressure_source", "fem_cell_stokes_solve_adjoint", "fem_cell_stokes_material_gradient",
     "fem_stokes_postprocess", "fem_norms", "fem_cnn_create", "fem_cnn_destroy", "fem_cnn_parameter_count",
     "fem_cnn_set_weights", "fem_cnn_get_weights", "fem_cnn_get_grads", "fem_cnn_forward", "fem_cnn_train_step",
-    "fem_cnn_gemm_bf16", "fem_norms_noalloc",
+    "fem_cnn_gemm_bf16", "fem_norms_noalloc", "fem_dr_velocity_update", "fem_dp_return_map_batch",
 ]
 
 
@@ -995,12 +1004,15 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
 
 
 def solve_batch(problems, nthreads: int = 0, raise_errors: bool = True):
-    """Independent Stokes solves run concurrently (fem_stokes_solve_batch):
+    """Independent Stokes solves in one call (fem_stokes_solve_batch):
     ``problems`` is a list of (dr, mesh, bcs, kwargs) with the keywords of
-    solve_stokes; every dr must have its own Context. Returns the list of
-    stats dicts (with "status"); raises the first failure after all ran
-    unless raise_errors is False (then a failed solve shows as a nonzero
-    "status" and its message as "error")."""
+    solve_stokes; every dr shares one Context. Mixed-precision solves
+    without a yield model on structured meshes with cell counts multiples of
+    16 x 8 run their inner iterations as grouped launches; the rest are
+    solved one after the other. nthreads is accepted and ignored. Returns
+    the list of stats dicts (with "status"); raises the first failure after
+    all ran unless raise_errors is False (then a failed solve shows as a
+    nonzero "status" and its message as "error")."""
     n = len(problems)
     keep = []
     states = (VP * n)()
@@ -1121,6 +1133,30 @@ def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
 # ---------------------------------------------------------------------------
 # thermal diffusion (src/heat_diffusion/**)
 # ---------------------------------------------------------------------------
+def dr_velocity_update(ctx: Context, rate: DeviceArray, v: DeviceArray, R: DeviceArray, PC: DeviceArray, alpha,
+                       beta):
+    """update_stokes_velocity! for one component (helpers.jl:31-41) with the
+    solver's negated step: rate = beta rate + R / PC; v -= alpha rate."""
+    ctx.check(_lib.fem_dr_velocity_update(ctx.handle, len(v), rate.ptr, v.ptr, R.ptr, PC.ptr, float(alpha),
+                                          float(beta)))
+
+
+def dp_return_map_batch(ctx: Context, dt, *, exx, eyy, exy, tau_old, P, eta, G, C_, phi, psi, eta_reg, Kb):
+    """The plane-strain VEP stress with the Drucker-Prager return at
+    independent points (rheology.jl:226-245, 272-334); host arrays in,
+    (txx, tyy, txy, yields) out."""
+    n = len(exx)
+    ins = dict(exx=exx, eyy=eyy, exy=exy, txx_old=tau_old[0], tyy_old=tau_old[1], txy_old=tau_old[2], P=P, eta=eta,
+               G=G, C=C_, phi=phi, psi=psi, eta_reg=eta_reg, Kb=Kb)
+    dev = {k: device_array(ctx, np.broadcast_to(np.asarray(v, float), (n,))) for k, v in ins.items()}
+    outs = {k: device_array(ctx, shape=(n,)) for k in ("txx", "tyy", "txy")}
+    y = device_array(ctx, shape=(n,), dtype=np.int32)
+    pts = _DPPoints(**{k: a.ptr for k, a in {**dev, **outs}.items()}, yields=y.ptr)
+    ctx.check(_lib.fem_dp_return_map_batch(ctx.handle, n, float(dt), C.byref(pts)))
+    ctx.sync()
+    return outs["txx"].numpy(), outs["tyy"].numpy(), outs["txy"].numpy(), y.numpy()
+
+
 class _OwnedBuf:
     """A fem_buf_alloc allocation, freed with its owner."""
 

@@ -251,61 +251,144 @@ def hydrostatic_pressure(mesh, phases, rho, g):
 # ---------------------------------------------------------------------------
 # the training generator: viscous inclusions under pure shear
 # ---------------------------------------------------------------------------
-def random_inclusions(seed, max_inclusions=20):
-    """1..max_inclusions circles (centre U[0,1]^2, r U[0.01, 0.1], log10 eta
-    U[-2, 4]) in an eta = 1 matrix; later circles win."""
+def random_inclusions(seed, max_inclusions=10):
+    """1..max_inclusions circles (centre U[0.1, 0.9]^2, r U[0.02, 0.1], log10
+    eta U[-2, 3]) in an eta = 1 matrix; later circles win."""
     rng = np.random.default_rng(seed)
     k = int(rng.integers(1, max_inclusions + 1))
-    circles = list(zip(rng.uniform(0, 1, k), rng.uniform(0, 1, k), rng.uniform(0.01, 0.1, k)))
-    eta = (1.0,) + tuple(10.0 ** rng.uniform(-2, 4, k))
+    circles = list(zip(rng.uniform(0.1, 0.9, k), rng.uniform(0.1, 0.9, k), rng.uniform(0.02, 0.1, k)))
+    eta = (1.0,) + tuple(10.0 ** rng.uniform(-2, 3, k))
     return circles, eta
 
 
-def generate_dataset(seeds, n=GRID, tolerance=1e-6, max_iterations=60_000, nthreads=8, log=None):
+def _viscous_state(fa, mesh, circles, eta):
+    from . import problems as PB
+    nph = len(eta)
+    ph = PB.inclusion_phases(mesh.coords, mesh.el2n, circles)
+    inf = math.inf
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=eta, etab=(inf,) * nph, G=(inf,) * nph, rho0=(1.0,) * nph,
+                                             K=(inf,) * nph, g=(0.0, 0.0)))
+    dr.set_phases("v", ph)
+    dr.set_phases("P", ph)
+    return dr, ph
+
+
+_MESHES = {}   # n -> [(mesh, boundary conditions, background field, grid index)], one per batch slot
+
+
+def generate_dataset(seeds, n=GRID, tolerance=1e-6, max_iterations=100_000, batch=64, log=None):
     """DR solutions of the viscous inclusion generator on n x n T7 meshes, one
-    per seed (FP32-bulk solver, solved in parallel streams): returns features
-    [S][n][n][16], targets [S][n][n][3] and the per-solve stats."""
+    per seed (the FP32-bulk solver, `batch` problems per batched call):
+    returns features [S][n][n][16], targets [S][n][n][3] and the per-solve
+    stats."""
     import femtools_amd as fa
     from . import problems as PB
     X = np.zeros((len(seeds), n, n, IN_CH), np.float32)
     T = np.zeros((len(seeds), n, n, OUT_CH), np.float32)
     stats = []
-    chunk = max(1, nthreads)
-    meshes = []
-    for i0 in range(0, len(seeds), chunk):
-        batch = []
-        for j, s in enumerate(seeds[i0:i0 + chunk]):
+    meshes = _MESHES.setdefault(n, [])
+    for i0 in range(0, len(seeds), batch):
+        probs = []
+        for j, s in enumerate(seeds[i0:i0 + batch]):
             if len(meshes) <= j:
-                ctx = fa.Context(0)
-                mesh = fa.t7_box_mesh(n, n, ctx=ctx)
+                mesh = fa.t7_box_mesh(n, n, ctx=fa.default_context())
                 (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
-                meshes.append((ctx, mesh, (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)),
+                meshes.append((mesh, (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)),
                                PB.pure_shear_field(mesh.coords), element_grid_index(mesh, n)))
-            ctx, mesh, bcs, vbg, gidx = meshes[j]
+            mesh, bcs, vbg, gidx = meshes[j]
             circles, eta = random_inclusions(s)
-            nph = len(eta)
-            ph = PB.inclusion_phases(mesh.coords, mesh.el2n, circles)
-            inf = math.inf
-            dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=eta, etab=(inf,) * nph, G=(inf,) * nph,
-                                                     rho0=(1.0,) * nph, K=(inf,) * nph, g=(0.0, 0.0)))
-            dr.set_phases("v", ph)
-            dr.set_phases("P", ph)
+            dr, ph = _viscous_state(fa, mesh, circles, eta)
             dr.v.x.copyto(vbg[0])
             dr.v.y.copyto(vbg[1])
             kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
                       iterMax=max_iterations, throw_on_failure=False, precision="fp32")
-            batch.append((dr, mesh, bcs, kw, ph, eta, vbg, gidx))
-        res = fa.solve_batch([(dr, mesh, bcs, kw) for dr, mesh, bcs, kw, *_ in batch], nthreads=nthreads,
-                             raise_errors=False)
-        for j, ((dr, mesh, bcs, kw, ph, eta, vbg, gidx), st) in enumerate(zip(batch, res)):
+            probs.append((dr, mesh, bcs, kw, ph, eta, vbg, gidx))
+        res = fa.solve_batch([(dr, mesh, bcs, kw) for dr, mesh, bcs, kw, *_ in probs], raise_errors=False)
+        for j, ((dr, mesh, bcs, kw, ph, eta, vbg, gidx), st) in enumerate(zip(probs, res)):
             i = i0 + j
             X[i] = features(mesh, ph, eta, n, gidx)
             T[i] = targets(mesh, dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy(), vbg)
             stats.append({"iter": st["iter"], "converged": bool(st["converged"]), "status": st["status"],
                           "err_abs": st["err_abs"]})
         if log:
-            log(f"dataset {min(i0 + chunk, len(seeds))}/{len(seeds)}")
+            log(f"dataset {min(i0 + batch, len(seeds))}/{len(seeds)}")
     return X, T, stats
+
+
+def evaluate_gain(net, n, seeds, dt_advect=0.05, tolerance=1e-8, precision="fp32", max_iterations=400_000,
+                  log=None):
+    """What the learned guess gains on n x n problems of the generator (seeds
+    not used in training): each problem is the second step of a two-step
+    sequence whose inclusions move with the first step's flow at their
+    centres for dt_advect, so its previous-step solution exists. The step is
+    solved to `tolerance` from (i) the zero guess (boundary values only),
+    (ii) the previous step's solution and (iii) the network's prediction
+    (inference and resampling included in its time)."""
+    import femtools_amd as fa
+    from . import problems as PB
+    mesh = fa.t7_box_mesh(n, n, ctx=net.ctx)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
+    bcs = (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy))
+    vbg = PB.pure_shear_field(mesh.coords)
+    kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
+              iterMax=max_iterations, throw_on_failure=False, precision=precision)
+    out = []
+    for s in seeds:
+        circles, eta = random_inclusions(s)
+        dr, ph = _viscous_state(fa, mesh, circles, eta)
+        dr.v.x.copyto(vbg[0])
+        dr.v.y.copyto(vbg[1])
+        fa.solve(dr, mesh, *bcs, **kw)
+        prev = (dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy())
+        # advect the inclusion centres with the first step's velocity
+        moved = []
+        for cx, cy, r in circles:
+            k = int(np.argmin((mesh.coords[:, 0] - cx) ** 2 + (mesh.coords[:, 1] - cy) ** 2))
+            moved.append((cx + dt_advect * prev[0][k], cy + dt_advect * prev[1][k], r))
+        row = {"seed": s, "inclusions": len(circles)}
+        for name in ("zero", "previous_step", "learned"):
+            dr2, ph2 = _viscous_state(fa, mesh, moved, eta)
+            t0 = time.perf_counter()
+            if name == "zero":
+                z = np.zeros(mesh.nnodes)
+                dr2.v.x.copyto(z)
+                dr2.v.y.copyto(z)
+            elif name == "previous_step":
+                dr2.v.x.copyto(prev[0])
+                dr2.v.y.copyto(prev[1])
+                dr2.P.copyto(prev[2])
+            else:
+                gx, gy, gP = predict_on_mesh(net, mesh, ph2, eta, vbg)
+                dr2.v.x.copyto(gx)
+                dr2.v.y.copyto(gy)
+                dr2.P.copyto(gP)
+            try:
+                st = fa.solve(dr2, mesh, *bcs, **kw)
+                net.ctx.sync()
+                row[name] = {"iterations": st["iter"], "seconds": round(time.perf_counter() - t0, 4),
+                             "converged": bool(st["converged"])}
+            except fa.FemError as e:   # a diverging solve is recorded, not raised
+                row[name] = {"iterations": None, "seconds": round(time.perf_counter() - t0, 4),
+                             "converged": False, "error": str(e)}
+        out.append(row)
+        if log:
+            log(f"gain n={n} seed {s}: " + ", ".join(f"{k} {row[k]['iterations']} it {row[k]['seconds']:.2f}s"
+                                                    for k in ("zero", "previous_step", "learned")))
+    return out
+
+
+def summarize_gain(rows):
+    """Mean iteration and wall-time ratios of the learned guess to the zero
+    and previous-step guesses over the rows where all three converged."""
+    ok = [r for r in rows if all(r[k]["converged"] for k in ("zero", "previous_step", "learned"))]
+    if not ok:
+        return {"problems": len(rows), "all_converged": 0}
+    f = lambda a, b, key: float(np.mean([r[a][key] / r[b][key] for r in ok]))
+    return {"problems": len(rows), "all_converged": len(ok),
+            "iterations_learned_over_zero": round(f("learned", "zero", "iterations"), 3),
+            "iterations_learned_over_previous": round(f("learned", "previous_step", "iterations"), 3),
+            "time_learned_over_zero": round(f("learned", "zero", "seconds"), 3),
+            "time_learned_over_previous": round(f("learned", "previous_step", "seconds"), 3)}
 
 
 def train(net: Network, X, T, epochs=30, batch=16, lr=1e-3, seed=0, log=None, val=None):

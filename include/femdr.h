@@ -290,14 +290,20 @@ int fem_stokes_set_solve_inputs(fem_stokes* st, const fem_solve_inputs* inputs);
  * be finite and nonnegative; zero support with a nonzero Q2D is an error. 2-D. */
 int fem_stokes_set_pressure_source(fem_stokes* st, const double* Qq, double Q2D, int32_t has_Q2D);
 
-/* n independent solve! calls (DR.jl:302-337) run concurrently. No counterpart
- * in the reference, which solves one state per call; for callers with many
- * small problems (parameter sweeps, ensembles). Each state must own its
- * fem_ctx (one HIP stream per solve); nthreads host threads (<= 0: n) take
- * the states largest mesh first. bcs[i] holds the state's dim conditions,
- * plastic may be NULL or hold NULL entries, params/stats/status are arrays
- * of n. status[i] is the return code of solve i; the call returns the first
- * nonzero status in state order, FEM_OK if all succeed. */
+/* n independent solve! calls (DR.jl:302-337) on one context (stream). No
+ * counterpart in the reference, which solves one state per call; for callers
+ * with many problems (parameter sweeps, ensembles). Mixed-precision 2-D
+ * solves without a yield model on structured box meshes whose cell counts are
+ * multiples of 16 x 8 (and check_interval >= 2) run together: the inner
+ * iterations of all of them are grouped launches (one element and one
+ * grid-node update launch per iteration for every problem), a check interval
+ * captured as one HIP graph; each problem keeps the decisions and iteration
+ * counts of its single solve, and leaves the group when it finishes. Other
+ * problem kinds are solved one after the other. All states must share one
+ * fem_ctx; nthreads is ignored (no host threads). bcs[i] holds the state's dim
+ * conditions, plastic may be NULL or hold NULL entries, params/stats/status
+ * are arrays of n; status[i] is the return code of solve i; the call returns
+ * the first nonzero status in state order, FEM_OK if all succeed. */
 int fem_stokes_solve_batch(int32_t n, fem_stokes* const* states, const fem_bc* const* bcs,
                            const fem_drucker_prager* const* plastic, const fem_solve_params* params,
                            fem_solve_stats* stats, int32_t* status, int32_t nthreads);
@@ -575,6 +581,31 @@ int fem_cnn_train_step(fem_cnn* net, const float* x, const float* t, int32_t B, 
  * out: the matrix-core product every convolution of the network uses. K a
  * multiple of 16. Not synchronised. */
 int fem_cnn_gemm_bf16(fem_ctx* ctx, int64_t M, int64_t N, int64_t K, const uint16_t* A, const uint16_t* Bt, float* C);
+
+/* update_stokes_velocity_kernel! (src/stokes/helpers.jl:31-41) for one
+ * velocity component with the negated step of the solver (DR.jl:690-695):
+ * rate = beta rate + R / PC; v -= alpha rate. Device arrays of n; enqueued on
+ * the context's stream. */
+int fem_dr_velocity_update(fem_ctx* ctx, int64_t n, double* rate, double* v, const double* R, const double* PC,
+                           double alpha, double beta);
+
+/* Points of fem_dp_return_map_batch: device arrays of n, inputs then outputs. */
+typedef struct {
+    const double *exx, *eyy, *exy;              /* strain rate (ε̇xy the tensor shear)   */
+    const double *txx_old, *tyy_old, *txy_old;  /* stress of the previous time step      */
+    const double* P;                            /* pressure of the yield function        */
+    const double *eta, *G, *C, *phi, *psi, *eta_reg, *Kb;   /* the point's material (angles
+                                                   in radians; G = Inf: viscous)         */
+    double *txx, *tyy, *txy;                    /* out: the returned stress              */
+    int32_t* yields;                            /* out, nullable: 1 where the point yields */
+} fem_dp_points;
+
+/* The plane-strain visco-elasto-plastic stress with the Drucker-Prager
+ * return at n independent points (deviatoric_stress + the return of
+ * _deviatoric_stress_with_multiplier, src/stokes/assemblers/rheology.jl:35-42,
+ * 226-245, 272-334), the computation every quadrature point of the element
+ * kernel does. FP64; enqueued on the context's stream. */
+int fem_dp_return_map_batch(fem_ctx* ctx, int64_t n, double dt, const fem_dp_points* points);
 
 #ifdef __cplusplus
 }

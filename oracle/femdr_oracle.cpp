@@ -1243,7 +1243,7 @@ extern "C" {
 static void thermal_integrate(int32_t nn, int32_t nq, const double* N, const double* dNdx, const double* dOm,
                               const double* T, const double* T0, const double* s, const int32_t* ph,
                               const orc_thermal_material* m, const double* P, double dt, double Tref,
-                              const double* phi, double* Re, double* J) {
+                              const double* phi, double* Re, double* J, int dim = 2) {
     double beta[ORC_MAX_PHASES];
     for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];   // β = map(inv, K) (residual.jl:62-64)
     for (int i = 0; i < nn; ++i) Re[i] = 0;
@@ -1251,16 +1251,20 @@ static void thermal_integrate(int32_t nn, int32_t nq, const double* N, const dou
         for (int i = 0; i < nn * nn; ++i) J[i] = 0;
     for (int q = 0; q < nq; ++q) {
         const double* Nv = N + (int64_t)q * nn;
-        const double* G = dNdx + (int64_t)q * nn * 2;
-        double Tq = 0, Pq = 0, T0q = 0, sq = 0, gx = 0, gy = 0;
+        const double* G = dNdx + (int64_t)q * nn * dim;
+        double Tq = 0, Pq = 0, T0q = 0, sq = 0, gT[3] = {0, 0, 0};
         for (int i = 0; i < nn; ++i) {
             Tq += Nv[i] * T[i];
             Pq += Nv[i] * P[i];
             T0q += Nv[i] * T0[i];
             sq += Nv[i] * s[i];
-            gx += G[2 * i] * T[i];
-            gy += G[2 * i + 1] * T[i];
+            for (int d = 0; d < dim; ++d) gT[d] += G[dim * i + d] * T[i];
         }
+        auto gdot = [&](int i, const double* v) {
+            double r = 0;
+            for (int d = 0; d < dim; ++d) r += G[dim * i + d] * v[d];
+            return r;
+        };
         const double kq = interp_nodes(Nv, nn, m->k, ph);
         const double aq = interp_nodes(Nv, nn, m->alpha, ph);
         const double bq = interp_nodes(Nv, nn, beta, ph);
@@ -1270,16 +1274,16 @@ static void thermal_integrate(int32_t nn, int32_t nq, const double* N, const dou
         double rate = T0q - Tq + D * sq;
         if (phi) rate += D * phi[q];
         for (int i = 0; i < nn; ++i) {
-            const double KTi = kq * (G[2 * i] * gx + G[2 * i + 1] * gy);
+            const double KTi = kq * gdot(i, gT);
             Re[i] += rate * Nv[i] * dOm[q] - D * KTi * dOm[q];
         }
         if (J) {
             const double c = D * r0 * aq / rho;
             for (int i = 0; i < nn; ++i) {
-                const double KTi = kq * (G[2 * i] * gx + G[2 * i + 1] * gy);
+                const double KTi = kq * gdot(i, gT);
                 for (int j = 0; j < nn; ++j) {
                     const double dD = c * Nv[j];
-                    const double Kij = kq * (G[2 * i] * G[2 * j] + G[2 * i + 1] * G[2 * j + 1]);
+                    const double Kij = kq * gdot(i, G + dim * j);
                     double dr = -Nv[j] + dD * sq;
                     if (phi) dr += dD * phi[q];
                     J[i * nn + j] += (dr * Nv[i] - dD * KTi - D * Kij) * dOm[q];
@@ -1299,7 +1303,13 @@ void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double
 // assemble_diffusion_matrices_atomix! (heat residual.jl:90-121) via
 // assemble_dr_matrices_atomix! (dynamic_relaxation/assembly.jl:43-60); element
 // contributions are added in ascending element order.
+static void thermal_residual3(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian);
+
 void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian) {
+    if (t->dim == 3) {
+        thermal_residual3(t, dt, Tref, compute_jacobian);
+        return;
+    }
     const Tables& tb = tables();
     for (int64_t n = 0; n < t->nnodes; ++n) t->R[n] = 0;
     if (compute_jacobian)
@@ -2431,3 +2441,49 @@ int orc_cp3_solve(orc_stokes3* s, const orc_cp3_params* p, orc_cp3_stats* out, d
 }
 
 }  // extern "C"
+
+// The thermal residual on a Hex27 mesh (the 27-node QuadraticElement{3,27} with
+// the 3x3x3 Gauss rule, heat residual.jl:58-87 and the assembler of
+// residual.jl:90-121): t->invJ holds [nels][27][9] (∂ξ/∂x row-major per
+// point) and t->dOmega [nels][27], as orc_hex27_geometry writes them.
+static void thermal_residual3(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian) {
+    const Tables3& tb = tables3();
+    constexpr int NN = 27, NQ3 = 27;
+    for (int64_t n = 0; n < t->nnodes; ++n) t->R[n] = 0;
+    if (compute_jacobian)
+        for (int64_t n = 0; n < t->nnodes; ++n) t->dRdT[n] = t->PC[n] = 0;
+    static thread_local std::vector<double> J(NN * NN);
+    double N[NQ3][NN], dNdx[NQ3][NN][3], dOm[NQ3], Tl[NN], T0l[NN], sl[NN], Pl[NN], Re[NN];
+    int32_t ph[NN];
+    for (int q = 0; q < NQ3; ++q)
+        for (int a = 0; a < NN; ++a) N[q][a] = tb.Nq[q][a];
+    for (int64_t e = 0; e < t->nels; ++e) {
+        const int32_t* nd = t->el2n + e * NN;
+        for (int q = 0; q < NQ3; ++q) {
+            const double* iJ = t->invJ + (e * NQ3 + q) * 9;
+            for (int a = 0; a < NN; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double v = 0;
+                    for (int k = 0; k < 3; ++k) v += tb.dNdxi[q][a][k] * iJ[k * 3 + d];
+                    dNdx[q][a][d] = v;
+                }
+            dOm[q] = t->dOmega[e * NQ3 + q];
+        }
+        for (int a = 0; a < NN; ++a) {
+            Tl[a] = t->T[nd[a]]; T0l[a] = t->T0[nd[a]]; sl[a] = t->source[nd[a]]; Pl[a] = t->P[nd[a]];
+            ph[a] = t->phases[nd[a]];
+        }
+        thermal_integrate(NN, NQ3, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref,
+                          t->source_ip ? t->source_ip + e * NQ3 : nullptr, Re, compute_jacobian ? J.data() : nullptr,
+                          3);
+        for (int a = 0; a < NN; ++a) t->R[nd[a]] += Re[a];
+        if (compute_jacobian) {
+            for (int a = 0; a < NN; ++a) {
+                double rs = 0;
+                for (int b = 0; b < NN; ++b) rs += std::fabs(J[a * NN + b]);
+                t->dRdT[nd[a]] += rs;
+                t->PC[nd[a]] += std::fabs(J[a * NN + a]);
+            }
+        }
+    }
+}

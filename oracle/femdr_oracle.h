@@ -300,8 +300,10 @@ typedef struct {
     const int32_t* bc_dofs; /* 0-based */
     const double* bc_vals;
     double CFL, c_fact;     /* 0.98, 0.9 */
-    const double* source_ip; /* nullable [nels][7]: heat source at the quadrature points
+    const double* source_ip; /* nullable [nels][nq]: heat source at the quadrature points
                                 (source_ip of residual.jl:7-47, the shear heating of a coupled solve) */
+    int32_t dim;            /* 2 (T7; 0 reads as 2) or 3: Hex27 with el2n [nels][27],
+                               invJ [nels][27][9] and dOmega [nels][27] (orc_hex27_geometry) */
 } orc_thermal;
 
 typedef struct {

@@ -92,7 +92,7 @@ class ThermalC(C.Structure):
     _fields_ = [("nnodes", I64), ("nels", I64), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("phases", PI),
                 ("mat", ThermalMaterialC), ("R", PD), ("R0", PD), ("dRdT", PD), ("PC", PD), ("T", PD), ("T0", PD),
                 ("rate", PD), ("P", PD), ("source", PD), ("nbc", I64), ("bc_dofs", PI), ("bc_vals", PD),
-                ("CFL", D), ("c_fact", D), ("source_ip", PD)]
+                ("CFL", D), ("c_fact", D), ("source_ip", PD), ("dim", I32)]
 
 
 class CoupledParamsC(C.Structure):
@@ -601,7 +601,9 @@ class ThermalProblem:
         t.nbc = len(self.bc[0])
         t.bc_dofs, t.bc_vals = ptr(self.bc[0], PI), ptr(self.bc[1])
         t.CFL, t.c_fact = self.CFL, self.c_fact
-        t.source_ip = ptr(None)
+        t.source_ip = ptr(getattr(self, "source_ip", None)) if getattr(self, "source_ip", None) is not None \
+            else ptr(None)
+        t.dim = getattr(self, "dim", 2)
         self._struct = t
         return t
 
@@ -616,6 +618,26 @@ class ThermalProblem:
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
         return {"converged": bool(st.converged), "iterations": st.iterations, "residual": st.residual}
+
+
+class ThermalProblem3(ThermalProblem):
+    """A ThermalDiffusionDR on a Hex27 mesh (27 nodes, 3x3x3 Gauss points)."""
+
+    def __init__(self, coords, el2n, mat, phases=None, bc=(np.zeros(0, np.int32), np.zeros(0)), CFL=0.98,
+                 c_fact=0.9):
+        self.coords = f64(coords)
+        self.el2n = i32(el2n)
+        nn, ne = len(self.coords), len(self.el2n)
+        self.nnodes, self.nels = nn, ne
+        self.invJ, self.dOmega = hex27_geometry(self.coords, self.el2n)
+        self.phases = i32(np.ones(nn) if phases is None else phases)
+        self.mat = mat
+        for k in ("R", "R0", "dRdT", "PC", "T", "T0", "rate", "P", "source"):
+            setattr(self, k, np.zeros(nn))
+        self.bc = (i32(bc[0]), f64(bc[1]))
+        self.CFL, self.c_fact = CFL, c_fact
+        self.dim = 3
+        self.source_ip = None
 
 
 def rotate_elastic_stress(omega, tau, dt):

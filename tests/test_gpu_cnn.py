@@ -7,7 +7,9 @@
   emulated (weights, activations): tolerance 2e-2 of max |y|, a few bfloat16
   ulps, since a rounding boundary can fall either way after FP32 sums;
 * every layer's weight and bias gradient against the oracle's backward with
-  the same rounding, relative L2 error per layer 3e-2;
+  the same rounding: relative L2 error 1e-2 per layer on a network whose ReLU
+  masks cannot switch, and the bfloat16 sensitivity scale (0.15) on a
+  He-initialised one;
 * Adam steps on one fixed batch lower the loss.
 """
 import os
@@ -67,8 +69,7 @@ def test_forward_matches_oracle():
     assert err < 2e-2
 
 
-def test_gradients_match_oracle():
-    fa, WS, net, params, oparams, x, t = _setup()
+def _grad_errors(net, WS, oparams, x, t, fa):
     ctx = fa.default_context()
     xd = fa.device_array(ctx, x, dtype=np.float32)
     td = fa.device_array(ctx, t, dtype=np.float32)
@@ -76,14 +77,57 @@ def test_gradients_match_oracle():
     st = {}
     yo = CO.forward(oparams, x.astype(np.float64), store=st, round_bf16=True)
     lo, gy = CO.loss_and_grad(yo, t.astype(np.float64))
-    assert loss == pytest.approx(lo, rel=2e-2)
     go = CO.backward(oparams, st, gy, round_bf16=True)
-    gd = net.get_grads()
-    for li, ((gW, gb), (oW, ob)) in enumerate(zip(gd, go)):
+    errs = []
+    for li, ((gW, gb), (oW, ob)) in enumerate(zip(net.get_grads(), go)):
         eW = np.linalg.norm(gW - oW) / max(np.linalg.norm(oW), 1e-30)
         eb = np.linalg.norm(gb - ob) / max(np.linalg.norm(ob), 1e-30)
         print(WS.LAYERS[li][0], f"gW rel {eW:.2e} gb rel {eb:.2e}")
-        assert eW < 3e-2 and eb < 3e-2, WS.LAYERS[li][0]
+        errs.append((eW, eb))
+    return loss, lo, errs
+
+
+def test_gradients_match_oracle_without_relu_switching():
+    # all-positive weights, biases and inputs keep every pre-activation
+    # positive: the ReLU masks cannot differ between device and oracle, so the
+    # comparison pins the arithmetic of every forward and backward kernel
+    # (convolutions, dgrad, wgrad, pooling, upsampling, skips, biases)
+    fa, WS = _ws()
+    B, H = 2, 16
+    net = WS.Network(fa.default_context(), max_batch=B, H=H, W=H)
+    rng = np.random.default_rng(8)
+    params = []
+    for name, ci, co, _ in WS.LAYERS:
+        W = (np.abs(rng.standard_normal((co, 3, 3, ci))) / (9 * ci)).astype(np.float32)
+        b = rng.uniform(0.05, 0.2, co).astype(np.float32)
+        if name == "e0a":
+            W[..., 4:] = 0
+        if name == "head":
+            W[3:] = 0
+            b[3:] = 0
+        params.append((W, b))
+    net.set_params(params)
+    x = np.zeros((B, H, H, 16), np.float32)
+    x[..., :4] = rng.uniform(0.1, 1.0, (B, H, H, 4))
+    t = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    oparams = [(CO.bf16(W).astype(np.float64), b.astype(np.float64)) for W, b in params]
+    loss, lo, errs = _grad_errors(net, WS, oparams, x, t, fa)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    for (eW, eb), layer in zip(errs, WS.LAYERS):
+        assert eW < 1e-2 and eb < 1e-2, layer[0]
+
+
+def test_gradients_match_oracle_random_network():
+    # a He-initialised network: about a third of the pre-activations sit
+    # below zero and bfloat16 rounding switches some ReLU masks, which the
+    # backward pass amplifies -- the FP64 oracle with and without bfloat16
+    # rounding differs by 2-10 % per layer on this case, so the device is held
+    # to that scale
+    fa, WS, net, params, oparams, x, t = _setup()
+    loss, lo, errs = _grad_errors(net, WS, oparams, x, t, fa)
+    assert loss == pytest.approx(lo, rel=2e-2)
+    for (eW, eb), layer in zip(errs, WS.LAYERS):
+        assert eW < 0.15 and eb < 0.15, layer[0]
 
 
 def test_adam_steps_lower_the_loss():
