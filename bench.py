@@ -261,7 +261,7 @@ def batch_measure(count, seed=0, max_iterations=400_000):
         dr.v.y.copyto(v0[1])
         bcs = [fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy)]
         kw = dict(dt=w.dt, tolerance=1e-8, check_interval=50, max_iterations=max_iterations, iterMax=max_iterations,
-                  throw_on_failure=False, precision="fp32")
+                  max_ph_iterations=w.max_ph_iterations, throw_on_failure=False, precision="fp32")
         probs.append((w, dr, mesh, bcs, kw))
     ctx.sync()
     t0 = time.perf_counter()

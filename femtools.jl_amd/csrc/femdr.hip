@@ -433,6 +433,11 @@ int fem_stokes_destroy(fem_stokes* s) {
         for (size_t k = 0; k < sizeof(fem_stokes::W32) / sizeof(float*); ++k) hipFree(p[k]);
         delete s->w32;
     }
+    if (s->w3) {
+        float** p = (float**)s->w3;
+        for (size_t k = 0; k < sizeof(fem_stokes::W3) / sizeof(float*); ++k) hipFree(p[k]);
+        delete s->w3;
+    }
     delete s;
     return FEM_OK;
 }
@@ -1516,17 +1521,18 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // blocks with points near yield only) with τ_old an FP32 copy of the stress
 // history, and the Jacobian
 // diagnostics are re-assembled in FP64 at every check unless frozen, as the
-// reference does (DR.jl:726-745). The FP64 residual evaluations count the
-// elements with a yielding point; an interval whose last known count is zero
-// runs the cheaper visco-elastic operator, exact there (the count arrives one
-// check interval late; a point that starts to yield inside an interval is
-// corrected by the next fold). The solution x meets the Dirichlet values
-// before the first iteration.
+// reference does (DR.jl:726-745). Every FP64 residual evaluation refreshes the
+// per-element near-yield flags and counts the elements with a yielding point;
+// the count is read back at once, and iterations after an evaluation with a
+// count of zero run the cheaper visco-elastic operator, exact there. A point
+// that starts to yield inside such a stretch is corrected by the next fold,
+// at most FEM_FOLD_INTERVAL iterations later. The solution x meets the
+// Dirichlet values before the first iteration.
 #ifndef FEM_MIXED_RETURN_MAP
 #define FEM_MIXED_RETURN_MAP 1
 #endif
-// On intervals with yielding points δ is also folded every FEM_FOLD_INTERVAL
-// iterations between checks: the PH/DYREL trajectory under plasticity is
+// With a yield model δ is also folded every FEM_FOLD_INTERVAL iterations
+// between checks: the PH/DYREL trajectory under plasticity is
 // sensitive to perturbations, and smaller FP32 corrections keep the mixed
 // iteration closer to the FP64 one (512² VEP, plastic step: 203k iterations
 // folding at checks only, 151k every 25, 144k-165k every 10, 139k in FP64), for
@@ -1597,8 +1603,13 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     auto residual64 = [&](const LaunchOpts& oo) -> int {
         if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
         CHK(launch_element(s, d64, oo));
-        if (delta) HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 4, hipMemcpyDeviceToHost, c->stream));
-        return launch_gather(s, d64, oo);
+        CHK(launch_gather(s, d64, oo));
+        if (delta) {
+            HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            d32.delta = *h_yield > 0;
+        }
+        return FEM_OK;
     };
     for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
         itPH_done = itPH;
@@ -1609,7 +1620,6 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         CHK(residual64(oo));
         double r3[3];
         CHK(reduce_outer(s, d64, r3));
-        d32.delta = delta && *h_yield > 0;
         timer.collect();
         err_P = std::sqrt(r3[0]) / std::sqrt((double)np);
         err_v = std::max(std::sqrt(r3[1]), std::sqrt(r3[2])) / (2 * std::sqrt((double)nv));
@@ -1645,7 +1655,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
             int64_t nseg = 1;
             if (ncheck > 1 && !timed) {
                 nseg = ncheck - iter % ncheck;
-                if (d32.delta && FEM_FOLD_INTERVAL > 0)
+                if (delta && FEM_FOLD_INTERVAL > 0)
                     nseg = std::min<int64_t>(nseg, FEM_FOLD_INTERVAL - iter % FEM_FOLD_INTERVAL);
                 nseg = std::max<int64_t>(1, std::min(nseg, std::min(p->iter_max - itPT + 1, p->max_iterations - iter + 1)));
             }
@@ -1721,16 +1731,14 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 // FP64 residual there: FP32 rounding of L δ is relative to the
                 // size of δ, so the FP32 iterations only ever reduce the residual
                 // of the last ncheck steps, far above their rounding floor, and
-                // the operator is renewed at the current state
-                // the yield count of the previous fold has arrived with this
-                // check's synchronisation: it picks the next interval's operator
-                const int32_t ycount = *h_yield;
-                d32.delta = delta && ycount > 0;
+                // the operator is renewed at the current state; the yield count
+                // of this evaluation picks the next iterations' operator
                 CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
                 CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
                 HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
                 HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
                 CHK(residual64(oo));
+                const int32_t ycount = delta ? *h_yield : 0;
                 CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
                 CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
                 CHK(cast_to32(c, d64.RP, w.rhs_P, np));
@@ -1750,8 +1758,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 }
                 for (int k = 0; k < 2; ++k)
                     cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
-            } else if (FEM_FOLD_INTERVAL > 0 && d32.delta && iter % FEM_FOLD_INTERVAL == 0) {
-                // extra folds between checks on intervals with yielding points
+            } else if (FEM_FOLD_INTERVAL > 0 && delta && iter % FEM_FOLD_INTERVAL == 0) {
+                // extra folds between checks with a yield model
                 CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
                 CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
                 HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
@@ -2167,6 +2175,16 @@ int fem_stokes_time_element_kernel(fem_stokes* s, const fem_drucker_prager* dp, 
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = ms / n;
     return FEM_OK;
+}
+
+int fem_stokes_time_correction_kernel(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n,
+                                      double* avg_ms) {
+    if (!s || n <= 0 || !avg_ms) return FEM_EARG;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    if (s->dim != 3)
+        return set_err(s->ctx, FEM_EARG, "the correction-step timer is built for 3-D states (2-D solves time theirs "
+                                         "with time_kernels)");
+    return hex27_time_lin(s, dp, dt, n, avg_ms);
 }
 
 int fem_stokes_solve(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager* dp, const fem_solve_params* p,

@@ -89,6 +89,10 @@ struct Hex27Args {
     double *Rv_x, *Rv_y, *Rv_z;            // nullable
     double *Rv0_x, *Rv0_y, *Rv0_z;         // nullable
     const double *bf_x, *bf_y, *bf_z;      // nullable: nodal loads subtracted from the residual
+    // nullable (residual evaluations with plasticity): per element, a point at
+    // or near the yield surface; the number of elements with a yielding point
+    uint8_t* yield_flag;
+    int32_t* yield_count;
 };
 
 // global node of element e at tensor position t = i + 3 j + 9 k
@@ -136,8 +140,11 @@ using fem::val;
 
 // 3-D deviatoric stress with the Drucker-Prager return (rheology.jl:413-472);
 // in = (∂vx/∂x, ∂vy/∂y, ∂vz/∂z, ε̇xy, ε̇xz, ε̇yz); tau = (xx, yy, zz, xy, xz, yz)
+// Returns whether the return was applied; fy (nullable) receives the yield
+// function of the trial stress.
 template <class S, bool PLASTIC>
-__device__ inline void stress3d(const S in[6], const IpMat<double>& m, const double to[6], double Pq, S tau[6]) {
+__device__ inline bool stress3d(const S in[6], const IpMat<double>& m, const double to[6], double Pq, S tau[6],
+                                double* fy = nullptr) {
     const S tr = (in[0] + in[1] + in[2]) * (1.0 / 3.0);
     const S e[6] = {in[0] - tr, in[1] - tr, in[2] - tr, in[3], in[4], in[5]};
     const double two_eve = 2 * m.etave;
@@ -147,6 +154,7 @@ __device__ inline void stress3d(const S in[6], const IpMat<double>& m, const dou
         const S tII = dsqrt((tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]) * 0.5 + tau[3] * tau[3] +
                             tau[4] * tau[4] + tau[5] * tau[5] + eps2<double>());
         const S F_ = (tII - m.yield0) - m.sphi * Pq;
+        if (fy) *fy = val(F_);
         if (val(F_) > 0) {
             const S lam = F_ * m.invden;
             if (val(lam) > 0) {
@@ -158,9 +166,32 @@ __device__ inline void stress3d(const S in[6], const IpMat<double>& m, const dou
                 for (int k = 0; k < 3; ++k) tau[k] = tau[k] - h * (tau[k] * r2);
 #pragma unroll
                 for (int k = 3; k < 6; ++k) tau[k] = tau[k] - h * (tau[k] * r1);
+                return true;
             }
         }
     }
+    return false;
+}
+
+// the plastic correction c of stress3d's return for a visco-elastic trial
+// stress t (returned stress = t - c), FP64; false where the point does not yield
+__device__ inline bool dp_correction3(const double t[6], const IpMat<double>& m, double Pq, double c[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = 0;
+    const double tII = sqrt((t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) * 0.5 + t[3] * t[3] + t[4] * t[4] +
+                            t[5] * t[5] + eps2<double>());
+    const double F_ = (tII - m.yield0) - m.sphi * Pq;
+    if (!(F_ > 0)) return false;
+    const double lam = F_ * m.invden;
+    if (!(lam > 0)) return false;
+    const double tIIs = tII + eps2<double>();
+    const double h = (2 * m.etave) * lam;
+    const double r2 = 1.0 / (tIIs * 2.0), r1 = 1.0 / tIIs;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = h * (t[k] * r2);
+#pragma unroll
+    for (int k = 3; k < 6; ++k) c[k] = h * (t[k] * r1);
+    return true;
 }
 
 __device__ inline IpMat<double> hex_ipmat(const double* ptab, int p) {
@@ -174,19 +205,23 @@ __device__ inline IpMat<double> hex_ipmat(const double* ptab, int p) {
 }
 
 // sum over the 32 lanes of a lane group
-__device__ inline double group_sum(double v) {
+template <class F>
+__device__ inline F group_sum(F v) {
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
     return v;
 }
 
 // 1-D basis values at a point (lane = point) or of a node (lane = node)
-struct Lane1D {
-    double L0[3], L1[3], L2[3], d0[3], d1[3], d2[3];
+template <class F>
+struct Lane1DT {
+    F L0[3], L1[3], L2[3], d0[3], d1[3], d2[3];
 };
+typedef Lane1DT<double> Lane1D;
 // lane = point q = qi + 3 qj + 9 qk: L0[r] = L_r(x_qi) etc.
-__device__ inline Lane1D point_1d(int q) {
-    Lane1D b;
+template <class F = double>
+__device__ inline Lane1DT<F> point_1d(int q) {
+    Lane1DT<F> b;
     const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -196,8 +231,9 @@ __device__ inline Lane1D point_1d(int q) {
     return b;
 }
 // lane = node at tensor position t = i + 3 j + 9 k: L0[p] = L_i(x_p) etc.
-__device__ inline Lane1D node_1d(int t) {
-    Lane1D b;
+template <class F = double>
+__device__ inline Lane1DT<F> node_1d(int t) {
+    Lane1DT<F> b;
     const int i = t % 3, j = (t / 3) % 3, k = t / 9;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -213,20 +249,21 @@ __device__ inline Lane1D node_1d(int t) {
 // ∂/∂ξ = Σ_jk L_j L_k Σ_i L'_i v_ijk, ∂/∂η and ∂/∂ζ from Σ_i L_i v_ijk
 // (the ζ loop stays rolled, with its basis values recomputed, so that the
 // compiler does not hold all 81 nodal values in registers at once)
-__device__ inline void point_gradient(const HexGeo& g, const Lane1D& b, int q, const double* sv, double gv[3][3]) {
-    double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
-    const double x2 = gauss_x(q / 9);
+template <class F>
+__device__ inline void point_gradient(const HexGeo& g, const Lane1DT<F>& b, int q, const F* sv, F gv[3][3]) {
+    F a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+    const F x2 = (F)gauss_x(q / 9);
 #pragma unroll 1
     for (int k = 0; k < 3; ++k) {
-        const double L2k = lq(x2, k), d2k = dlq(x2, k);
+        const F L2k = (F)lq(x2, k), d2k = (F)dlq(x2, k);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double wLL = b.L1[j] * L2k, wdL = b.d1[j] * L2k, wLd = b.L1[j] * d2k;
-            const double* v = sv + (3 * j + 9 * k) * 3;
+            const F wLL = b.L1[j] * L2k, wdL = b.d1[j] * L2k, wLd = b.L1[j] * d2k;
+            const F* v = sv + (3 * j + 9 * k) * 3;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double s0 = b.d0[0] * v[c] + b.d0[1] * v[3 + c] + b.d0[2] * v[6 + c];
-                const double s1 = b.L0[0] * v[c] + b.L0[1] * v[3 + c] + b.L0[2] * v[6 + c];
+                const F s0 = b.d0[0] * v[c] + b.d0[1] * v[3 + c] + b.d0[2] * v[6 + c];
+                const F s1 = b.L0[0] * v[c] + b.L0[1] * v[3 + c] + b.L0[2] * v[6 + c];
                 a0[c] += wLL * s0;
                 a1[c] += wdL * s1;
                 a2[c] += wLd * s1;
@@ -235,43 +272,46 @@ __device__ inline void point_gradient(const HexGeo& g, const Lane1D& b, int q, c
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        gv[c][0] = a0[c] * g.jac[0];
-        gv[c][1] = a1[c] * g.jac[1];
-        gv[c][2] = a2[c] * g.jac[2];
+        gv[c][0] = a0[c] * (F)g.jac[0];
+        gv[c][1] = a1[c] * (F)g.jac[1];
+        gv[c][2] = a2[c] * (F)g.jac[2];
     }
 }
 
 // P1 modes at a point: (1 - ξ - η - ζ, ξ, η, ζ), and the point weight |det J| w
-__device__ inline void point_modes(int q, double NP[HNP], double& dOm, double det) {
+template <class F>
+__device__ inline void point_modes(int q, F NP[HNP], F& dOm, double det) {
     const int qi = q % 3, qj = (q / 3) % 3, qk = q / 9;
     const double xi = gauss_x(qi), eta = gauss_x(qj), zeta = gauss_x(qk);
-    NP[0] = 1 - xi - eta - zeta; NP[1] = xi; NP[2] = eta; NP[3] = zeta;
-    dOm = det * ((gauss_w(qi) * gauss_w(qj)) * gauss_w(qk));
+    NP[0] = (F)(1 - xi - eta - zeta); NP[1] = (F)xi; NP[2] = (F)eta; NP[3] = (F)zeta;
+    dOm = (F)(det * ((gauss_w(qi) * gauss_w(qj)) * gauss_w(qk)));
 }
 
 // nodal residual of the lane's node from the point fluxes in LDS
 // (accumulate_momentum, momentum_residuals.jl:28-33), factorised over the
 // axes like point_gradient. sf[q][0..6] = dΩ·(τxx - P, τyy - P, τzz - P,
 // τxy, τxz, τyz, ρ)
-__device__ inline void node_residual(const HexGeo& g, const Lane1D& b, int t, const double* sf, double gx, double gy,
-                                     double gz, double R[3]) {
-    double rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0}, rr = 0;   // ξ, η, ζ parts per row
+template <class F>
+__device__ inline void node_residual(const HexGeo& g, const Lane1DT<F>& b, int t, const F* sf, F gx, F gy, F gz,
+                                     F R[3]) {
+    F rx[3] = {0, 0, 0}, ry[3] = {0, 0, 0}, rz[3] = {0, 0, 0}, rr = 0;   // ξ, η, ζ parts per row
     const int kn = t / 9;
 #pragma unroll 1
     for (int r = 0; r < 3; ++r) {
-        const double xr = gauss_x(r), L2r = lq(xr, kn), d2r = dlq(xr, kn);
+        const double xr = gauss_x(r);
+        const F L2r = (F)lq(xr, kn), d2r = (F)dlq(xr, kn);
 #pragma unroll
         for (int qq = 0; qq < 3; ++qq) {
-            const double* f0 = sf + (3 * qq + 9 * r) * 8;
-            const double* f1 = f0 + 8;
-            const double* f2 = f0 + 16;
+            const F* f0 = sf + (3 * qq + 9 * r) * 8;
+            const F* f1 = f0 + 8;
+            const F* f2 = f0 + 16;
             auto dsum = [&](int k) { return b.d0[0] * f0[k] + b.d0[1] * f1[k] + b.d0[2] * f2[k]; };
             auto lsum = [&](int k) { return b.L0[0] * f0[k] + b.L0[1] * f1[k] + b.L0[2] * f2[k]; };
-            const double wLL = b.L1[qq] * L2r, wdL = b.d1[qq] * L2r, wLd = b.L1[qq] * d2r;
+            const F wLL = b.L1[qq] * L2r, wdL = b.d1[qq] * L2r, wLd = b.L1[qq] * d2r;
             rx[0] += wLL * dsum(0);
             ry[0] += wLL * dsum(3);
             rz[0] += wLL * dsum(4);
-            const double lxy = lsum(3), lyz = lsum(5);
+            const F lxy = lsum(3), lyz = lsum(5);
             rx[1] += wdL * lxy;
             ry[1] += wdL * lsum(1);
             rz[1] += wdL * lyz;
@@ -281,9 +321,10 @@ __device__ inline void node_residual(const HexGeo& g, const Lane1D& b, int t, co
             rr += wLL * lsum(6);
         }
     }
-    R[0] = (rx[0] * g.jac[0] + rx[1] * g.jac[1] + rx[2] * g.jac[2]) - gx * rr;
-    R[1] = (ry[0] * g.jac[0] + ry[1] * g.jac[1] + ry[2] * g.jac[2]) - gy * rr;
-    R[2] = (rz[0] * g.jac[0] + rz[1] * g.jac[1] + rz[2] * g.jac[2]) - gz * rr;
+    const F j0 = (F)g.jac[0], j1 = (F)g.jac[1], j2 = (F)g.jac[2];
+    R[0] = (rx[0] * j0 + rx[1] * j1 + rx[2] * j2) - gx * rr;
+    R[1] = (ry[0] * j0 + ry[1] * j1 + ry[2] * j2) - gy * rr;
+    R[2] = (rz[0] * j0 + rz[1] * j1 + rz[2] * j2) - gz * rr;
 }
 
 // element-group bookkeeping of a 256-thread block: 8 elements, 32 lanes each
@@ -371,8 +412,18 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
         for (int i = 0; i < HNP; ++i) { Pq += NP[i] * P[i]; Pnq += NP[i] * Pn[i]; }
         const double in[6] = {gv[0][0], gv[1][1], gv[2][2], (gv[0][1] + gv[1][0]) * 0.5, (gv[0][2] + gv[2][0]) * 0.5,
                               (gv[1][2] + gv[2][1]) * 0.5};
-        double tau[6];
-        stress3d<double, PLASTIC>(in, m, to, Pq, tau);
+        double tau[6], fy = -INFINITY;
+        const bool yel = stress3d<double, PLASTIC>(in, m, to, Pq, tau, &fy) && h.l < HNQ;
+        if (PLASTIC && !ITER && A.yield_flag) {
+            // lanes 0-31 and 32-63 of a wave serve two elements
+            const bool near = h.l < HNQ && fy > -0.2 * fabs(m.yield0 + m.sphi * Pq);
+            const int sh = 32 * (h.es & 1);
+            const uint64_t bn = __ballot(near), by = __ballot(yel);
+            const bool en = ((bn >> sh) & 0xffffffffull) != 0, ey = ((by >> sh) & 0xffffffffull) != 0;
+            if (h.l == 0 && h.valid) A.yield_flag[e] = en;
+            const uint64_t ew = __ballot(h.l == 0 && h.valid && ey);
+            if ((threadIdx.x & 63) == 0 && ew) atomicAdd(A.yield_count, (int32_t)__popcll(ew));
+        }
         if (A.tau_out[0] && h.active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) A.tau_out[k][e * HNQ + q] = tau[k];
@@ -465,6 +516,215 @@ __global__ __launch_bounds__(256) void k_gather3(Gather3Args A) {
         A.rate_x[node] = nrx; A.rate_y[node] = nry; A.rate_z[node] = nrz;
         A.vx[node] = nvx; A.vy[node] = nvy; A.vz[node] = nvz;
     }
+}
+
+// ---- FP32 correction operator of the mixed-precision solve. With x the FP64
+// state and δ = (δv, δP) the correction, the kernel evaluates the residual
+// R(x) + L δ, L the operator linearised at x: continuity with storage on δP,
+// Pnum, 2 ηve ε̇'(δ) − (δP + Pnum) in the momentum balance, the density term
+// ρ0 β δP g, Dirichlet nodes held at δ = 0, and the Chebyshev update of the
+// centre node. With DELTA, elements flagged hot (a point at or near yield in
+// the last FP64 evaluation) take the stress term τ(x + δ) − τ(x): x's strain
+// rate, trial stress and both Drucker-Prager returns in FP64 from the FP64
+// state; elsewhere neither state yields and the visco-elastic term is exact.
+struct Lin3Args {
+    HexGeo g;
+    float *vx, *vy, *vz;                     // δv (centre nodes updated here)
+    const float *P, *pscale, *rhs_P;         // δP, Pnum scaling, R_P(x): [nels][4]
+    const float *rhs_vx, *rhs_vy, *rhs_vz;   // R_v(x): centre nodes here, shared nodes in the gather
+    const int32_t* phase;
+    const double* ptab;
+    float gx, gy, gz, dt;
+    float* slot;                             // [nels][26][3]
+    float *rate_x, *rate_y, *rate_z;
+    const float *PCx, *PCy, *PCz;
+    const uint8_t* bcflag;
+    float ax, ay, az, bx, by, bz;
+    float *RP_out, *Pnum_out;                // nullable
+    float *Rv_x, *Rv_y, *Rv_z;               // nullable
+    float *Rv0_x, *Rv0_y, *Rv0_z;            // nullable
+    const double *xvx, *xvy, *xvz, *xP;      // DELTA: the FP64 state
+    const double* tau_old[6];                // DELTA and ELASTIC: [6][nels * 27]
+    const uint8_t* hot;                      // DELTA: per element
+};
+
+template <int FL>
+__global__ __launch_bounds__(HBLOCK) void k_hex27_lin(Lin3Args A) {
+    constexpr bool BULK = FL & 1, DELTA = FL & 2, ELASTIC = FL & 4;
+    __shared__ float s_v[HEPB][HNV * 3];
+    __shared__ float s_f[HEPB][HNQ * 8];
+    __shared__ double s_x[DELTA ? HEPB : 1][HNV * 3];
+    const HexGeo& g = A.g;
+    const HexLane h = hex_lane(g.nels);
+    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const bool hot_e = DELTA && A.hot[e] != 0;   // uniform over the element's 32 lanes
+    int64_t node = 0;
+    if (h.l < HNV) {
+        node = hex_node(g, e, h.l);
+        s_v[h.es][h.l * 3 + 0] = A.vx[node];
+        s_v[h.es][h.l * 3 + 1] = A.vy[node];
+        s_v[h.es][h.l * 3 + 2] = A.vz[node];
+        if (DELTA && hot_e) {
+            s_x[h.es][h.l * 3 + 0] = A.xvx[node];
+            s_x[h.es][h.l * 3 + 1] = A.xvy[node];
+            s_x[h.es][h.l * 3 + 2] = A.xvz[node];
+        }
+    }
+    __syncthreads();
+    const int pc = A.phase[e];
+    float P[HNP];
+#pragma unroll
+    for (int i = 0; i < HNP; ++i) P[i] = A.P[e * HNP + i];
+    {
+        const int q = h.l < HNQ ? h.l : 0;
+        const Lane1DT<float> b = point_1d<float>(q);
+        float gv[3][3];
+        point_gradient<float>(g, b, q, s_v[h.es], gv);
+        float NP[HNP], dOm;
+        point_modes<float>(q, NP, dOm, g.det);
+        const float div = gv[0][0] + gv[1][1] + gv[2][2];
+        float dPdt = 0;
+        if (BULK) {
+            float dP = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) dP += NP[i] * P[i];
+            const float bulk = (float)A.ptab[PC_BULK * MAXPH + pc];
+            float bq = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) bq += NP[i] * bulk;
+            dPdt = dP / (bq * A.dt);
+        }
+        float RP[HNP], Pn[HNP];
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) {
+            RP[i] = group_sum(h.l < HNQ ? NP[i] * ((-div) - dPdt) * dOm : 0.0f) + A.rhs_P[e * HNP + i];
+            Pn[i] = A.pscale[e * HNP + i] * RP[i];
+        }
+        if (h.valid && h.l < HNP) {
+            if (A.RP_out) A.RP_out[e * HNP + h.l] = RP[h.l];
+            if (A.Pnum_out) A.Pnum_out[e * HNP + h.l] = Pn[h.l];
+        }
+        const float eve2 = 2.0f * (float)A.ptab[PC_ETAVE * MAXPH + pc];
+        const float tr = div * (1.0f / 3.0f);
+        float tau[6] = {eve2 * (gv[0][0] - tr), eve2 * (gv[1][1] - tr), eve2 * (gv[2][2] - tr),
+                        eve2 * ((gv[0][1] + gv[1][0]) * 0.5f), eve2 * ((gv[0][2] + gv[2][0]) * 0.5f),
+                        eve2 * ((gv[1][2] + gv[2][1]) * 0.5f)};
+        float Pq = 0, Pnq = 0;
+#pragma unroll
+        for (int i = 0; i < HNP; ++i) { Pq += NP[i] * P[i]; Pnq += NP[i] * Pn[i]; }
+        if (DELTA && hot_e && h.l < HNQ) {
+            // x's strain rate and trial stress as the FP64 residual forms them,
+            // then the returns of x + δ and of x
+            const Lane1D bd = point_1d<double>(q);
+            double gx[3][3];
+            point_gradient<double>(g, bd, q, s_x[h.es], gx);
+            double NPd[HNP], dOmd;
+            point_modes<double>(q, NPd, dOmd, g.det);
+            double Pqx = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) Pqx += NPd[i] * A.xP[e * HNP + i];
+            const IpMat<double> m = hex_ipmat(A.ptab, pc);
+            double to[6] = {0, 0, 0, 0, 0, 0};
+            if (ELASTIC) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) to[k] = A.tau_old[k][e * HNQ + q];
+            }
+            const double in[6] = {gx[0][0], gx[1][1], gx[2][2], (gx[0][1] + gx[1][0]) * 0.5,
+                                  (gx[0][2] + gx[2][0]) * 0.5, (gx[1][2] + gx[2][1]) * 0.5};
+            const double trx = (in[0] + in[1] + in[2]) * (1.0 / 3.0);
+            const double ex[6] = {in[0] - trx, in[1] - trx, in[2] - trx, in[3], in[4], in[5]};
+            const double two_eve = 2 * m.etave;
+            double tB[6], tA[6], cA[6], cB[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                tB[k] = two_eve * (ex[k] + to[k] * m.inv2Gdt);
+                tA[k] = tB[k] + (double)tau[k];
+            }
+            const bool yA = dp_correction3(tA, m, Pqx + (double)Pq, cA);
+            const bool yB = dp_correction3(tB, m, Pqx, cB);
+            if (yA || yB) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) tau[k] = (float)((double)tau[k] + (cB[k] - cA[k]));
+            }
+        }
+        const float Ptot = Pq + Pnq;
+        const float rb = (float)(A.ptab[PC_RHO0 * MAXPH + pc] * A.ptab[PC_BETA * MAXPH + pc]);
+        if (h.l < HNQ) {
+            float* f = s_f[h.es] + q * 8;
+            f[0] = (tau[0] - Ptot) * dOm; f[1] = (tau[1] - Ptot) * dOm; f[2] = (tau[2] - Ptot) * dOm;
+            f[3] = tau[3] * dOm; f[4] = tau[4] * dOm; f[5] = tau[5] * dOm;
+            f[6] = (rb * Pq) * dOm;
+        }
+    }
+    __syncthreads();
+    if (!h.active) return;
+    const int t = h.l;
+    const Lane1DT<float> b = node_1d<float>(t);
+    float R[3];
+    node_residual<float>(g, b, t, s_f[h.es], A.gx, A.gy, A.gz, R);
+    if (t != HCENTRE_T) {
+        float* s = A.slot + ((e * HNSH + kHexA[t]) * 3);
+        s[0] = R[0]; s[1] = R[1]; s[2] = R[2];
+        return;
+    }
+    const uint8_t fl = A.bcflag[node];
+    R[0] += A.rhs_vx[node]; R[1] += A.rhs_vy[node]; R[2] += A.rhs_vz[node];
+    const float rx = (fl & 1) ? 0.0f : R[0], ry = (fl & 2) ? 0.0f : R[1], rz = (fl & 4) ? 0.0f : R[2];
+    if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
+    if (A.Rv0_x) { A.Rv0_x[node] = rx; A.Rv0_y[node] = ry; A.Rv0_z[node] = rz; }
+    float nrx = A.bx * A.rate_x[node] + rx / A.PCx[node];
+    float nry = A.by * A.rate_y[node] + ry / A.PCy[node];
+    float nrz = A.bz * A.rate_z[node] + rz / A.PCz[node];
+    float nvx = s_v[h.es][t * 3 + 0] + (-A.ax) * nrx;
+    float nvy = s_v[h.es][t * 3 + 1] + (-A.ay) * nry;
+    float nvz = s_v[h.es][t * 3 + 2] + (-A.az) * nrz;
+    if (fl & 1) { nrx = 0; nvx = 0; }
+    if (fl & 2) { nry = 0; nvy = 0; }
+    if (fl & 4) { nrz = 0; nvz = 0; }
+    A.rate_x[node] = nrx; A.rate_y[node] = nry; A.rate_z[node] = nrz;
+    A.vx[node] = nvx; A.vy[node] = nvy; A.vz[node] = nvz;
+}
+
+// the shared nodes of the correction: slot sums plus R_v(x), Dirichlet, update
+struct Gather3fArgs {
+    int64_t ns;
+    const int32_t* node_of; const int32_t* off; const int32_t* ent;
+    const float* slot;
+    float *vx, *vy, *vz, *rate_x, *rate_y, *rate_z;
+    const float *PCx, *PCy, *PCz;
+    const uint8_t* bcflag;
+    const float *rhs_vx, *rhs_vy, *rhs_vz;
+    float *Rv_x, *Rv_y, *Rv_z, *Rv0_x, *Rv0_y, *Rv0_z;
+    float ax, ay, az, bx, by, bz;
+};
+
+__global__ __launch_bounds__(256) void k_gather3f(Gather3fArgs A) {
+    const int64_t i = xcd_tile() * 256 + threadIdx.x;
+    if (i >= A.ns) return;
+    const int64_t node = A.node_of[i];
+    float rx = 0, ry = 0, rz = 0;
+    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
+        const float* s = A.slot + (int64_t)A.ent[k] * 3;
+        rx += s[0]; ry += s[1]; rz += s[2];
+    }
+    rx += A.rhs_vx[node]; ry += A.rhs_vy[node]; rz += A.rhs_vz[node];
+    const uint8_t fl = A.bcflag[node];
+    if (fl & 1) rx = 0;
+    if (fl & 2) ry = 0;
+    if (fl & 4) rz = 0;
+    if (A.Rv_x) { A.Rv_x[node] = rx; A.Rv_y[node] = ry; A.Rv_z[node] = rz; }
+    if (A.Rv0_x) { A.Rv0_x[node] = rx; A.Rv0_y[node] = ry; A.Rv0_z[node] = rz; }
+    float nrx = A.bx * A.rate_x[node] + rx / A.PCx[node];
+    float nry = A.by * A.rate_y[node] + ry / A.PCy[node];
+    float nrz = A.bz * A.rate_z[node] + rz / A.PCz[node];
+    float nvx = A.vx[node] + (-A.ax) * nrx;
+    float nvy = A.vy[node] + (-A.ay) * nry;
+    float nvz = A.vz[node] + (-A.az) * nrz;
+    if (fl & 1) { nrx = 0; nvx = 0; }
+    if (fl & 2) { nry = 0; nvy = 0; }
+    if (fl & 4) { nrz = 0; nvz = 0; }
+    A.rate_x[node] = nrx; A.rate_y[node] = nry; A.rate_z[node] = nrz;
+    A.vx[node] = nvx; A.vy[node] = nvy; A.vz[node] = nvz;
 }
 
 // ---- augmented Jacobian diagnostics (momentum_residuals.jl:919-1117): per
@@ -655,12 +915,16 @@ __global__ void k_pscale3(int64_t n, const double* g, const double* M, double* p
 }
 
 // ---- reductions for three components
-struct Red3 {
-    const double *R[3], *R0[3], *rate[3], *PC[3], *J[3];
+template <class T>
+struct Red3T {
+    const T *R[3], *R0[3], *rate[3], *PC[3], *J[3];
     double al[3];
 };
+typedef Red3T<double> Red3;
 
-__global__ __launch_bounds__(RED_BLOCK) void k_red3_check(int64_t n, Red3 r, double* partial) {
+// inner check sums (FP64 accumulation) over the FP64 state or the FP32 correction
+template <class T>
+__global__ __launch_bounds__(RED_BLOCK) void k_red3_check(int64_t n, Red3T<T> r, double* partial) {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
     for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride) {
@@ -668,9 +932,9 @@ __global__ __launch_bounds__(RED_BLOCK) void k_red3_check(int64_t n, Red3 r, dou
         for (int c = 0; c < 3; ++c) {
             const double Ri = r.R[c][i];
             acc[c] += Ri * Ri;
-            const double t = r.al[c] * r.rate[c][i];
+            const double t = r.al[c] * (double)r.rate[c][i];
             acc[3 + 2 * c] += t * t;
-            acc[4 + 2 * c] += t * ((Ri - r.R0[c][i]) / r.PC[c][i]);
+            acc[4 + 2 * c] += t * ((Ri - (double)r.R0[c][i]) / (double)r.PC[c][i]);
         }
     }
     block_sum_store<9>(acc, partial);
@@ -722,6 +986,8 @@ struct Opts3 {
     bool use_pnum = false, do_update = false, store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false,
          store_tau = false;
     double al[3] = {0, 0, 0}, be[3] = {0, 0, 0};
+    uint8_t* yield_flag = nullptr;   // residual evaluations with plasticity: near-yield elements
+    int32_t* yield_count = nullptr;  // and the count of yielding ones (added to)
 };
 
 int check3(fem_ctx* c, const char* what) { return check_launch(c, what); }
@@ -773,6 +1039,10 @@ int hex_element(fem_stokes* s, const Opts3& o, bool gather = true) {
     a.Pnum_out = o.store_Pnum ? f[FEM_FIELD_PNUM] : nullptr;
     if (o.store_Rv) { a.Rv_x = f[FEM_FIELD_RV_X]; a.Rv_y = f[FEM_FIELD_RV_Y]; a.Rv_z = f[FEM_FIELD_RV_Z]; }
     if (o.store_Rv0) { a.Rv0_x = f[FEM_FIELD_RV0_X]; a.Rv0_y = f[FEM_FIELD_RV0_Y]; a.Rv0_z = f[FEM_FIELD_RV0_Z]; }
+    if (o.plastic && !o.use_pnum && o.yield_flag && o.yield_count) {
+        a.yield_flag = o.yield_flag;
+        a.yield_count = o.yield_count;
+    }
     const int key = (o.elastic ? 1 : 0) | (o.plastic ? 2 : 0) | (o.bulk ? 4 : 0) | (o.use_pnum ? 8 : 0);
     dispatch_h27<15>(key, s, a);
     CHK(check3(s->ctx, "k_hex27_element"));
@@ -1772,11 +2042,319 @@ int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, dou
     return hex_element(s, o);
 }
 
+// ---- mixed precision in 3-D (precision = 1): the 2-D solve's scheme on the
+// Hex27 operator. The outer iterations evaluate R(x) in FP64 (no Pnum) and
+// apply the reference's errors, criterion and rel_drop rule to it; the inner
+// iterations run the reference's pseudo-transient loop in FP32 on a
+// correction δ of R(x) + L δ = 0 (k_hex27_lin), with the reference's inner
+// criterion, Chebyshev parameters and λmin estimates. δv is folded into the
+// FP64 state at every check (and, with plasticity, every kHexFold iterations
+// between checks), the correction then restarting from the FP64 residual
+// there; δP is folded after the pressure update that ends an outer iteration.
+// With a Drucker-Prager rheology every FP64 evaluation refreshes the per-
+// element near-yield flags and the count of yielding elements, which selects
+// the next iterations' operator (the return-map correction where any element
+// yields, the exact visco-elastic one otherwise), and the Jacobian diagnostics
+// are re-assembled at every check unless frozen (DR.jl:726-745).
+namespace {
+constexpr int64_t kHexFold = 25;
+
+int cast32(fem_ctx* c, const double* x, float* y, int64_t n) {
+    hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, y, 1.0);
+    return check_launch(c, "k_cast");
+}
+int add32(fem_ctx* c, double* x, const float* d, int64_t n) {
+    hipLaunchKernelGGL(k_add_correction, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n, x, d, 1.0);
+    return check_launch(c, "k_add_correction");
+}
+
+int ensure_w3(fem_stokes* s, bool plastic) {
+    fem_mesh* m = s->mesh;
+    fem_ctx* c = s->ctx;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ne = m->nels;
+    if (!s->w3) {
+        s->w3 = new fem_stokes::W3();
+        std::memset(s->w3, 0, sizeof(fem_stokes::W3));
+        fem_stokes::W3& w = *s->w3;
+        float** vl[] = {&w.vx,    &w.vy,    &w.vz,    &w.rate_x, &w.rate_y, &w.rate_z, &w.Rv_x,   &w.Rv_y,   &w.Rv_z,
+                        &w.Rv0_x, &w.Rv0_y, &w.Rv0_z, &w.PC_x,   &w.PC_y,   &w.PC_z,   &w.rhs_vx, &w.rhs_vy, &w.rhs_vz};
+        for (float** q : vl) CHK(dalloc_t(c, q, nv));
+        float** pl[] = {&w.P, &w.RP, &w.Pnum, &w.gammaP, &w.MP, &w.pscale, &w.rhs_P};
+        for (float** q : pl) CHK(dalloc_t(c, q, np));
+        CHK(dalloc_t(c, &w.slot, ne * HNSH * 3));
+    }
+    if (plastic && !s->d_yflag) {
+        CHK(dalloc_t(c, &s->d_yflag, ne));
+        HIPCHK(c, hipMemsetAsync(s->d_yflag, 1, ne, c->stream));
+    }
+    return FEM_OK;
+}
+
+// one FP32 correction iteration: element kernel, then the shared nodes
+int lin3_step(fem_stokes* s, const Opts3& o, bool delta, const double al[3], const double be[3], bool store_end,
+              bool store_Rv0) {
+    fem_mesh* m = s->mesh;
+    fem_ctx* c = s->ctx;
+    fem_stokes::W3& w = *s->w3;
+    double** f = s->f;
+    Lin3Args a;
+    std::memset(&a, 0, sizeof a);
+    a.g = hex_geo(m);
+    a.vx = w.vx; a.vy = w.vy; a.vz = w.vz;
+    a.P = w.P; a.pscale = w.pscale; a.rhs_P = w.rhs_P;
+    a.rhs_vx = w.rhs_vx; a.rhs_vy = w.rhs_vy; a.rhs_vz = w.rhs_vz;
+    a.phase = s->d_phase_v;
+    a.ptab = s->d_ptab;
+    a.gx = (float)s->mat.g[0]; a.gy = (float)s->mat.g[1]; a.gz = (float)s->mat.g[2];
+    a.dt = (float)o.dt;
+    a.slot = w.slot;
+    a.rate_x = w.rate_x; a.rate_y = w.rate_y; a.rate_z = w.rate_z;
+    a.PCx = w.PC_x; a.PCy = w.PC_y; a.PCz = w.PC_z;
+    a.bcflag = s->d_bcflag;
+    a.ax = (float)al[0]; a.ay = (float)al[1]; a.az = (float)al[2];
+    a.bx = (float)be[0]; a.by = (float)be[1]; a.bz = (float)be[2];
+    if (store_end) {
+        a.RP_out = w.RP; a.Pnum_out = w.Pnum;
+        a.Rv_x = w.Rv_x; a.Rv_y = w.Rv_y; a.Rv_z = w.Rv_z;
+    }
+    if (store_Rv0) { a.Rv0_x = w.Rv0_x; a.Rv0_y = w.Rv0_y; a.Rv0_z = w.Rv0_z; }
+    a.xvx = f[FEM_FIELD_V_X]; a.xvy = f[FEM_FIELD_V_Y]; a.xvz = f[FEM_FIELD_V_Z];
+    a.xP = f[FEM_FIELD_P];
+    const int tf[6] = {FEM_FIELD_TAU_OLD_XX, FEM_FIELD_TAU_OLD_YY, FEM_FIELD_TAU_OLD_ZZ,
+                       FEM_FIELD_TAU_OLD_XY, FEM_FIELD_TAU_OLD_XZ, FEM_FIELD_TAU_OLD_YZ};
+    for (int k = 0; k < 6; ++k) a.tau_old[k] = f[tf[k]];
+    a.hot = s->d_yflag;
+    const dim3 grid(grid_xcd(m->nels, HEPB)), block(HBLOCK);
+    const int key = (o.bulk ? 1 : 0) | (delta ? 2 : 0) | (delta && o.elastic ? 4 : 0);
+    switch (key) {
+        case 0: hipLaunchKernelGGL((k_hex27_lin<0>), grid, block, 0, c->stream, a); break;
+        case 1: hipLaunchKernelGGL((k_hex27_lin<1>), grid, block, 0, c->stream, a); break;
+        case 2: hipLaunchKernelGGL((k_hex27_lin<2>), grid, block, 0, c->stream, a); break;
+        case 3: hipLaunchKernelGGL((k_hex27_lin<3>), grid, block, 0, c->stream, a); break;
+        case 6: hipLaunchKernelGGL((k_hex27_lin<6>), grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL((k_hex27_lin<7>), grid, block, 0, c->stream, a); break;
+    }
+    CHK(check3(c, "k_hex27_lin"));
+    Gather3fArgs g;
+    std::memset(&g, 0, sizeof g);
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.slot = w.slot;
+    g.vx = w.vx; g.vy = w.vy; g.vz = w.vz;
+    g.rate_x = w.rate_x; g.rate_y = w.rate_y; g.rate_z = w.rate_z;
+    g.PCx = w.PC_x; g.PCy = w.PC_y; g.PCz = w.PC_z;
+    g.bcflag = s->d_bcflag;
+    g.rhs_vx = w.rhs_vx; g.rhs_vy = w.rhs_vy; g.rhs_vz = w.rhs_vz;
+    g.Rv_x = a.Rv_x; g.Rv_y = a.Rv_y; g.Rv_z = a.Rv_z;
+    g.Rv0_x = a.Rv0_x; g.Rv0_y = a.Rv0_y; g.Rv0_z = a.Rv0_z;
+    g.ax = a.ax; g.ay = a.ay; g.az = a.az; g.bx = a.bx; g.by = a.by; g.bz = a.bz;
+    hipLaunchKernelGGL(k_gather3f, dim3(grid_xcd(m->ns, 256)), dim3(256), 0, c->stream, g);
+    return check3(c, "k_gather3f");
+}
+
+Red3T<float> red3f(fem_stokes* s, const double al[3]) {
+    fem_stokes::W3& w = *s->w3;
+    Red3T<float> r;
+    std::memset(&r, 0, sizeof r);
+    r.R[0] = w.Rv_x; r.R[1] = w.Rv_y; r.R[2] = w.Rv_z;
+    r.R0[0] = w.Rv0_x; r.R0[1] = w.Rv0_y; r.R0[2] = w.Rv0_z;
+    r.rate[0] = w.rate_x; r.rate[1] = w.rate_y; r.rate[2] = w.rate_z;
+    r.PC[0] = w.PC_x; r.PC[1] = w.PC_y; r.PC[2] = w.PC_z;
+    for (int k = 0; k < 3; ++k) r.al[k] = al[k];
+    return r;
+}
+
+int hex27_mixed(fem_stokes* s, const fem_solve_params* p, const Opts3& o, bool freeze, fem_solve_stats* st) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
+    const bool delta = o.plastic;
+    CHK(ensure_w3(s, delta));
+    fem_stokes::W3& w = *s->w3;
+    double** f = s->f;
+    // yielding elements of the last FP64 evaluation (device count, pinned copy)
+    int32_t* d_yield = s->d_flag + 1;
+    int32_t* h_yield = reinterpret_cast<int32_t*>(c->h_red + (RED_MAXK - 1));
+    *h_yield = 1;
+    CHK(cast32(c, f[FEM_FIELD_GAMMA_P], w.gammaP, np));
+    CHK(cast32(c, f[FEM_FIELD_M_P], w.MP, np));
+    CHK(cast32(c, s->d_pscale, w.pscale, np));
+    hipLaunchKernelGGL(k_pin3, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, (const uint8_t*)s->d_bcflag,
+                       (const double*)s->d_bcval_x, (const double*)s->d_bcval_y, (const double*)s->d_bcval_z,
+                       f[FEM_FIELD_V_X], f[FEM_FIELD_V_Y], f[FEM_FIELD_V_Z]);
+    CHK(check_launch(c, "k_pin3"));
+    auto cast_pc = [&]() -> int {
+        CHK(cast32(c, f[FEM_FIELD_PC_V_X], w.PC_x, nv));
+        CHK(cast32(c, f[FEM_FIELD_PC_V_Y], w.PC_y, nv));
+        return cast32(c, f[FEM_FIELD_PC_V_Z], w.PC_z, nv);
+    };
+    CHK(hex_jacobian_impl(s, o));
+    double lmax[3];
+    CHK(checked_lmax3(s, lmax));
+    double lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    int64_t jac = 1;
+    bool jac_elastic = false;   // the last assembly was at a state without yielding
+    CHK(cast_pc());
+    for (float* z : {w.vx, w.vy, w.vz, w.rate_x, w.rate_y, w.rate_z, w.Rv0_x, w.Rv0_y, w.Rv0_z})
+        HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+    if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g, %g)\n", lmax[0], lmax[1], lmax[2]);
+    double al[3], be[3];
+    for (int k = 0; k < 3; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
+
+    Opts3 oo = o;
+    oo.store_RP = true;
+    oo.store_Rv = true;
+    oo.yield_flag = delta ? s->d_yflag : nullptr;
+    oo.yield_count = delta ? d_yield : nullptr;
+    // FP64 residual at x and the correction's right-hand sides; with
+    // plasticity the fresh yield count picks the operator
+    bool dlt = delta;
+    auto residual64 = [&]() -> int {
+        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
+        CHK(hex_element(s, oo));
+        CHK(cast32(c, f[FEM_FIELD_RV_X], w.rhs_vx, nv));
+        CHK(cast32(c, f[FEM_FIELD_RV_Y], w.rhs_vy, nv));
+        CHK(cast32(c, f[FEM_FIELD_RV_Z], w.rhs_vz, nv));
+        CHK(cast32(c, f[FEM_FIELD_RP], w.rhs_P, np));
+        if (delta) {
+            HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dlt = *h_yield > 0;
+        }
+        return FEM_OK;
+    };
+    auto fold_v = [&]() -> int {
+        CHK(add32(c, f[FEM_FIELD_V_X], w.vx, nv));
+        CHK(add32(c, f[FEM_FIELD_V_Y], w.vy, nv));
+        CHK(add32(c, f[FEM_FIELD_V_Z], w.vz, nv));
+        for (float* z : {w.vx, w.vy, w.vz}) HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
+        return FEM_OK;
+    };
+
+    const double eps_tol = p->tolerance;
+    double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
+    double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
+    int64_t iter = 0, itPH_done = 0, launches = 0;
+    bool converged = false;
+    double rel_drop = p->rel_drop0;
+    int rc = FEM_OK;
+    const Red3 rbase = red3(s);
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        CHK(residual64());
+        double r4[4];
+        CHK(reduce3(s, 4, 0, std::max(nv, np), [&](unsigned g) {
+            hipLaunchKernelGGL(k_red3_outer, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, np, rbase,
+                               (const double*)f[FEM_FIELD_RP], (const double*)f[FEM_FIELD_M_P], c->red_partial);
+        }, r4));
+        err_P = std::sqrt(r4[0]) / std::sqrt((double)np);
+        err_v = std::max(std::sqrt(r4[1]), std::max(std::sqrt(r4[2]), std::sqrt(r4[3]))) / (2 * std::sqrt((double)nv));
+        if (itPH == 1) { err_P0 = err_P + julia_eps(err_P); err_v0 = err_v + julia_eps(err_v); }
+        else if (itPH == 2) { err_P0 = err_P + julia_eps(err_P); }
+        err_abs = std::max(err_v, err_P);
+        err_rel = std::max(err_v / err_v0, err_P / err_P0);
+        err = std::min(err_abs, err_rel);
+        if (std::isnan(err_v) || std::isnan(err_P) || std::isnan(err)) {
+            rc = set_err(c, FEM_ENAN, "NaN detected in outer loop at PH=%lld", (long long)itPH);
+            break;
+        }
+        if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in outer loop at PH=%lld", (long long)itPH); break; }
+        if (p->verbose)
+            printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e\n", (long long)itPH, (long long)iter, err,
+                   err_abs, err_rel);
+        if (err < eps_tol) { converged = true; break; }
+        if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
+        err_min = std::min(err_min, err);
+        const double eps_vel = err * rel_drop;
+        int64_t itPT = 0;
+        bool stop_total = false;
+        while (err > eps_vel && itPT <= p->iter_max) {
+            ++itPT;
+            ++iter;
+            const bool do_check = iter % ncheck == 0;
+            const bool last_possible = do_check || itPT > p->iter_max || iter > p->max_iterations;
+            if (do_check && ncheck == 1) {
+                HIPCHK(c, hipMemcpyAsync(w.Rv0_x, w.Rv_x, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(w.Rv0_y, w.Rv_y, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(w.Rv0_z, w.Rv_z, nv * 4, hipMemcpyDeviceToDevice, c->stream));
+            }
+            CHK(lin3_step(s, o, dlt, al, be, last_possible, ncheck > 1 && (iter + 1) % ncheck == 0));
+            ++launches;
+            if (do_check) {
+                const Red3T<float> r = red3f(s, al);
+                double r9[9];
+                CHK(reduce3(s, 9, 0, nv, [&](unsigned g) {
+                    hipLaunchKernelGGL(k_red3_check<float>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, r,
+                                       c->red_partial);
+                }, r9));
+                const double evi = std::max(std::sqrt(r9[0]), std::max(std::sqrt(r9[1]), std::sqrt(r9[2]))) /
+                                   (2 * std::sqrt((double)nv));
+                if (iter == ncheck) err_v00 = evi + julia_eps(evi);
+                err = evi <= eps_tol ? evi : std::max(evi / err_v00, evi);
+                if (std::isnan(err)) { rc = set_err(c, FEM_ENAN, "NaN detected in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (err > 1.0e10) { rc = set_err(c, FEM_EKABOOM, "Kaboom! Error > 1e10 in inner loop PH=%lld PT=%lld", (long long)itPH, (long long)itPT); break; }
+                if (p->verbose > 1) printf("  it = %lld, iter = %lld, err = %.3e\n", (long long)itPT, (long long)iter, err);
+                double lmin[3];
+                for (int k = 0; k < 3; ++k) {
+                    const double den = r9[3 + 2 * k], num = r9[4 + 2 * k];
+                    lmin[k] = den == 0 ? 0.0 : std::fabs(num) / den;
+                }
+                CHK(fold_v());
+                CHK(residual64());
+                // without yielding the diagnostics are the visco-elastic ones:
+                // re-assemble only when yielding was seen since the last assembly
+                const bool skip_jac = !dlt && jac_elastic;
+                if (delta && !freeze && !skip_jac) {
+                    jac_elastic = !dlt;
+                    ++jac;
+                    CHK(hex_jacobian_impl(s, o));
+                    rc = checked_lmax3(s, lmax);
+                    if (rc != FEM_OK) break;
+                    lmax_g = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+                    CHK(cast_pc());
+                }
+                for (int k = 0; k < 3; ++k)
+                    cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
+            } else if (delta && iter % kHexFold == 0) {
+                CHK(fold_v());
+                CHK(residual64());
+            }
+            if (iter > p->max_iterations) { stop_total = true; break; }
+        }
+        if (rc != FEM_OK) break;
+        // P += γP RP / M_P on the correction, then fold δ into the FP64 state
+        hipLaunchKernelGGL(k_pressure_update<float>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, w.P,
+                           (const float*)w.gammaP, (const float*)w.RP, (const float*)w.MP);
+        CHK(check_launch(c, "k_pressure_update"));
+        CHK(fold_v());
+        CHK(add32(c, f[FEM_FIELD_P], w.P, np));
+        HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
+        if (stop_total || iter > p->max_iterations) break;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_abs = err_abs;
+    st->err_rel = err_rel;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged;
+    st->reached_total_iter = iter > p->max_iterations;
+    st->lambda_max = std::max(lmax[0], std::max(lmax[1], lmax[2]));
+    st->lambda_max_gershgorin = lmax_g;
+    st->lambda_max_iterations = 0;
+    st->jacobian_assemblies = jac;
+    st->element_launches = launches;
+    return rc;
+}
+}  // namespace
+
 // solve! + _solve_stokes_dyrel! (DR.jl:302-337, 348-802) with D = 3
 int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
                 fem_solve_stats* st) {
     fem_ctx* c = s->ctx;
-    if (p->precision != 0) return set_err(c, FEM_EARG, "the 3-D path runs in FP64 (precision 0)");
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
     const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
@@ -1789,6 +2367,15 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
     // the one the scaling kernel reads
     if (!p->skip_scaling) CHK(hex_scaling_launch(s, p->pressure_factor, dt));
     else CHK(hex_pscale(s));
+    if (p->precision == 1) {
+        const int rcm = hex27_mixed(s, p, o, freeze, st);
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rcm != FEM_OK) return rcm;
+        if (!st->converged && p->throw_on_failure)
+            return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                           (long long)st->iter, st->err_abs, p->tolerance);
+        return FEM_OK;
+    }
     double** f = s->f;
     for (int k : {FEM_FIELD_DVDTAU_X, FEM_FIELD_DVDTAU_Y, FEM_FIELD_DVDTAU_Z, FEM_FIELD_RV0_X, FEM_FIELD_RV0_Y,
                   FEM_FIELD_RV0_Z})
@@ -1866,7 +2453,7 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
                 for (int k = 0; k < 3; ++k) r.al[k] = al[k];
                 double r9[9];
                 CHK(reduce3(s, 9, 0, nv, [&](unsigned g) {
-                    hipLaunchKernelGGL(k_red3_check, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, r, c->red_partial);
+                    hipLaunchKernelGGL(k_red3_check<double>, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, r, c->red_partial);
                 }, r9));
                 const double evi = std::max(std::sqrt(r9[0]), std::max(std::sqrt(r9[1]), std::sqrt(r9[2]))) /
                                    (2 * std::sqrt((double)nv));
@@ -1932,6 +2519,34 @@ int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, i
     CHK(hex_element(s, o, false));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < n; ++i) CHK(hex_element(s, o, false));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / n;
+    return FEM_OK;
+}
+
+// the FP32 correction step alone, n sweeps, for the bench's device-event timing
+int hex27_time_lin(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms) {
+    fem_ctx* c = s->ctx;
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, 0, o, true));
+    CHK(ensure_w3(s, o.plastic));
+    fem_stokes::W3& w = *s->w3;
+    const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP, ne = s->mesh->nels;
+    for (float* z : {w.vx, w.vy, w.vz, w.rate_x, w.rate_y, w.rate_z, w.rhs_vx, w.rhs_vy, w.rhs_vz})
+        HIPCHK(c, hipMemsetAsync(z, 0, nv * 4, c->stream));
+    for (float* z : {w.P, w.rhs_P, w.pscale}) HIPCHK(c, hipMemsetAsync(z, 0, np * 4, c->stream));
+    for (float* z : {w.PC_x, w.PC_y, w.PC_z}) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, z, 1.0f);
+        CHK(check_launch(c, "k_fill"));
+    }
+    if (o.plastic) HIPCHK(c, hipMemsetAsync(s->d_yflag, 1, ne, c->stream));
+    const double al[3] = {0, 0, 0}, be[3] = {1, 1, 1};
+    CHK(lin3_step(s, o, o.plastic, al, be, false, false));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < n; ++i) CHK(lin3_step(s, o, o.plastic, al, be, false, false));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;

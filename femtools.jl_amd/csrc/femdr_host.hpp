@@ -152,6 +152,16 @@ struct fem_stokes {
         float *tau_old0, *tau_old1, *tau_old2;
     };
     W32* w32 = nullptr;
+    // FP32 work arrays of the 3-D mixed-precision solve: the correction, its
+    // rates and residuals, the right-hand sides R(x), FP32 copies of the
+    // scaling and preconditioner, and the element-slot residuals [nels][26][3]
+    struct W3 {
+        float *vx, *vy, *vz, *rate_x, *rate_y, *rate_z, *Rv_x, *Rv_y, *Rv_z, *Rv0_x, *Rv0_y, *Rv0_z;
+        float *PC_x, *PC_y, *PC_z, *rhs_vx, *rhs_vy, *rhs_vz;
+        float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_P;
+        float* slot;
+    };
+    W3* w3 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -226,6 +236,7 @@ int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, dou
 int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
                 fem_solve_stats* st);
 int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
+int hex27_time_lin(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
 int hex27_rotate_stress(fem_stokes* s, double dt);
 int hex27_postprocess(fem_stokes* s, double* out);
 

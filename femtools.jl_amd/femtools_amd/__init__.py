@@ -28,7 +28,7 @@ __all__ = [
     "DirichletBoundaryCondition", "StokesDR", "solve", "update_stokes_current_stress",
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
-    "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel",
+    "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel", "time_correction_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
     "solve_adjoint", "stokes_material_gradient_3d", "compute_strain_rate_stress_postprocess", "write_vtk",
     "write_stokes_vtk", "VtkMesh",
@@ -199,6 +199,7 @@ _sig("fem_stokes_update_current_stress", C.c_int, VP, _P(_DP), D)
 _sig("fem_stokes_residual", C.c_int, VP, _P(_DP), D, I32, I32)
 _sig("fem_stokes_jacobian_diagnostics", C.c_int, VP, _P(_DP), D, I32)
 _sig("fem_stokes_time_element_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
+_sig("fem_stokes_time_correction_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
 _sig("fem_thermal_create", C.c_int, VP, VP, _P(_ThermalMaterial), _P(_ThermalOpts), _P(VP))
 _sig("fem_thermal_destroy", C.c_int, VP)
 _sig("fem_thermal_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
@@ -252,7 +253,7 @@ EXPORTED = [
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
     "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
-    "fem_stokes_time_element_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
+    "fem_stokes_time_element_kernel", "fem_stokes_time_correction_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
     "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
     "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
@@ -1127,6 +1128,19 @@ def time_element_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
     d, dp = _dp_ptr(plastic)
     out = D()
     dr.ctx.check(_lib.fem_stokes_time_element_kernel(dr.handle, dp, float(dt), int(n), C.byref(out)))
+    return out.value
+
+
+def time_correction_kernel(dr: StokesDR, dt, n=20, plastic=None) -> float:
+    """Average device time (ms) of one FP32 correction step of the 3-D
+    mixed-precision solve (element kernel and shared-node gather) over n
+    sweeps on a zero correction; with plastic every element takes the FP64
+    returns. Overwrites the solve's FP32 work arrays only."""
+    if _is_cap(plastic):
+        raise ValueError("the correction-step timer takes DruckerPrager or no yield model")
+    d, dp = _dp_ptr(plastic)
+    out = D()
+    dr.ctx.check(_lib.fem_stokes_time_correction_kernel(dr.handle, dp, float(dt), int(n), C.byref(out)))
     return out.value
 
 

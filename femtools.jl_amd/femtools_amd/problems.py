@@ -82,11 +82,15 @@ class Workload:
     CFL_v: float = 0.98
     nsteps: int = 1
     seed_pure_shear: bool = False
+    # outer (pressure) iterations: warm-started and high-contrast solves take
+    # more than the solver's default cap of 1000 to reach 1e-8
+    max_ph_iterations: int = 20_000
     extra: dict = field(default_factory=dict)
 
     def solve_kwargs(self):
         return dict(tolerance=self.tolerance, check_interval=self.check_interval,
-                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0)
+                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0,
+                    max_ph_iterations=self.max_ph_iterations)
 
     @property
     def nphases(self):
@@ -169,11 +173,15 @@ class Workload3:
     check_interval: int = 100
     pressure_factor: float = 50.0
     rel_drop0: float = 1e-2
+    # outer (pressure) iterations: warm-started and high-contrast solves take
+    # more than the solver's default cap of 1000 to reach 1e-8
+    max_ph_iterations: int = 20_000
     extra: dict = field(default_factory=dict)
 
     def solve_kwargs(self):
         return dict(tolerance=self.tolerance, check_interval=self.check_interval,
-                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0)
+                    pressure_factor=self.pressure_factor, rel_drop0=self.rel_drop0,
+                    max_ph_iterations=self.max_ph_iterations)
 
     def sizes(self):
         s = 2 * self.n + 1

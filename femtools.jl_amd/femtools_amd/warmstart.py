@@ -301,7 +301,7 @@ def generate_dataset(seeds, n=GRID, tolerance=1e-6, max_iterations=100_000, batc
             dr.v.x.copyto(vbg[0])
             dr.v.y.copyto(vbg[1])
             kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
-                      iterMax=max_iterations, throw_on_failure=False, precision="fp32")
+                      iterMax=max_iterations, max_ph_iterations=20_000, throw_on_failure=False, precision="fp32")
             probs.append((dr, mesh, bcs, kw, ph, eta, vbg, gidx))
         res = fa.solve_batch([(dr, mesh, bcs, kw) for dr, mesh, bcs, kw, *_ in probs], raise_errors=False)
         for j, ((dr, mesh, bcs, kw, ph, eta, vbg, gidx), st) in enumerate(zip(probs, res)):
@@ -331,7 +331,7 @@ def evaluate_gain(net, n, seeds, dt_advect=0.05, tolerance=1e-8, precision="fp32
     bcs = (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy))
     vbg = PB.pure_shear_field(mesh.coords)
     kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
-              iterMax=max_iterations, throw_on_failure=False, precision=precision)
+              iterMax=max_iterations, max_ph_iterations=20_000, throw_on_failure=False, precision=precision)
     out = []
     for s in seeds:
         circles, eta = random_inclusions(s)

@@ -333,6 +333,14 @@ int fem_stokes_jacobian_diagnostics(fem_stokes* st, const fem_drucker_prager* pl
 int fem_stokes_time_element_kernel(fem_stokes* st, const fem_drucker_prager* plastic, double dt,
                                    int32_t n, double* avg_ms);
 
+/* 3-D states: the FP32 correction step of the mixed-precision solve
+ * (k_hex27_lin and the shared-node gather k_gather3f) swept n times on a zero
+ * correction, every element taking the FP64 returns when plastic is given;
+ * the average device time in milliseconds through avg_ms. Not a reference
+ * entry point: the bench's device-event timing of that solve's hot kernel. */
+int fem_stokes_time_correction_kernel(fem_stokes* st, const fem_drucker_prager* plastic, double dt, int32_t n,
+                                      double* avg_ms);
+
 /* ------------------------------------------------------- thermal diffusion
  * ThermalMaterial (src/heat_diffusion/types/heat_diffusion_types.jl:15-41) */
 typedef struct {
