@@ -203,40 +203,55 @@ def viscous64_gpu():
 HEX27_FLOPS_PER_ELEMENT = 27 * 1234
 
 
-def hex27_measure(n, iters):
-    """The 3-D path: n^3 Hex27 cells, 50 seeded spheres, viscous, pure shear;
-    iters DR iterations from the far-field guess (not to convergence) and the
-    element kernel alone timed with HIP events."""
+def hex27_measure(n, max_iterations=200_000):
+    """The 3-D path: n^3 Hex27 cells, 50 seeded spheres (r in U[0.02, 0.08],
+    log10 eta in U[-1, 3]), viscous, pure shear, from the far-field guess to
+    relative residual 1e-7 in FP64 and in FP32 bulk with FP64 norms (the same
+    criterion); the FP64 element kernel and the FP32 correction step timed
+    alone with HIP events."""
     import femtools_amd as fa
     from femtools_amd import problems as PB
     w = PB.hex27_inclusions(n)
     mesh = fa.hex27_box_mesh(n, n, n)
     ph = PB.hex27_sphere_phases(mesh.coords, mesh.el2n, w.spheres)
-    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
-    dr.set_phases("v", ph)
-    dr.set_phases("P", ph)
     bcs = [fa.DirichletBoundaryCondition(d, v) for d, v in PB.pure_shear_bc3(mesh.coords)]
     v0 = PB.pure_shear_field3(mesh.coords)
-    fa.assemble_viscosity_weighted_pressure_scaling(dr, w.pressure_factor, w.dt)
-    el_ms = fa.time_element_kernel(dr, w.dt, n=20)
-    for f, v in zip(dr.v, v0):
-        f.copyto(v)
-    dr.P.fill(0.0)
-    mesh.ctx.sync()
-    t0 = time.perf_counter()
-    st = fa.solve(dr, mesh, *bcs, dt=w.dt, max_iterations=iters, iterMax=iters, throw_on_failure=False,
-                  **w.solve_kwargs())
-    mesh.ctx.sync()
-    wall = time.perf_counter() - t0
+    out = {"workload": w.name + f", far-field guess, tol {w.tolerance:g}", "elements": mesh.nels, "dofs": w.dofs()}
+    fields = {}
+    for prec in ("fp32", "fp64"):
+        dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g))
+        dr.set_phases("v", ph)
+        dr.set_phases("P", ph)
+        fa.assemble_viscosity_weighted_pressure_scaling(dr, w.pressure_factor, w.dt)
+        if prec == "fp32":
+            out["fp32_correction_step_ms"] = round(fa.time_correction_kernel(dr, w.dt, n=20), 4)
+        else:
+            out["fp64_element_kernel_ms"] = round(fa.time_element_kernel(dr, w.dt, n=20), 4)
+        for f, v in zip(dr.v, v0):
+            f.copyto(v)
+        dr.P.fill(0.0)
+        mesh.ctx.sync()
+        t0 = time.perf_counter()
+        st = fa.solve(dr, mesh, *bcs, dt=w.dt, max_iterations=max_iterations, iterMax=max_iterations,
+                      throw_on_failure=False, precision=prec, **w.solve_kwargs())
+        mesh.ctx.sync()
+        wall = time.perf_counter() - t0
+        log(f"hex27 {n}^3 {prec}: iter {st['iter']} itPH {st['itPH']} converged {st['converged']} "
+            f"err_abs {st['err_abs']:.2e} {wall:.2f}s")
+        out[prec] = {"seconds": round(wall, 3), "iterations": st["iter"], "itPH": st["itPH"],
+                     "converged": bool(st["converged"]), "err_abs": st["err_abs"],
+                     "gdof_iter_per_s": round(w.dofs() * st["iter"] / wall / 1e9, 4)}
+        fields[prec] = [f.numpy() for f in dr.v] + [dr.P.numpy()]
+    out["speedup_fp32_vs_fp64"] = round(out["fp64"]["seconds"] / out["fp32"]["seconds"], 3)
+    out["fp32_vs_fp64_max_rel_diff"] = dict(zip(
+        ("vx", "vy", "vz", "P"),
+        (float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(fields["fp32"], fields["fp64"]))))
     flops = HEX27_FLOPS_PER_ELEMENT * mesh.nels
-    tf = flops / (el_ms * 1e-3) / 1e12
-    return {"workload": w.name + f", {iters} DR iterations from the far-field guess",
-            "elements": mesh.nels, "dofs": w.dofs(), "iterations": st["iter"], "seconds": round(wall, 4),
-            "ms_per_iter": round(wall / st["iter"] * 1e3, 4),
-            "gdof_iter_per_s": round(w.dofs() * st["iter"] / wall / 1e9, 4),
-            "roofline": {"bound": "fp64 alu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(tf / FP64_PEAK_TFLOPS, 4), "kernel": "k_hex27_element (viscous DR step)",
-                         "flops_per_launch": flops, "avg_launch_us": round(el_ms * 1e3, 2)}}
+    tf = flops / (out["fp64_element_kernel_ms"] * 1e-3) / 1e12
+    out["roofline"] = {"bound": "fp64 alu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(tf / FP64_PEAK_TFLOPS, 4), "kernel": "k_hex27_element (viscous DR step)",
+                       "flops_per_launch": flops, "avg_launch_us": round(out["fp64_element_kernel_ms"] * 1e3, 2)}
+    return out
 
 
 def batch_measure(count, seed=0, max_iterations=400_000):
@@ -362,9 +377,10 @@ def main():
     ap.add_argument("--tolerance", type=float, default=1e-8)
     ap.add_argument("--max-iterations", type=int, default=400_000)
     ap.add_argument("--eta-reg", type=float, default=None, help="Perzyna viscosity (default problems.VEP_ETA_REG)")
-    ap.add_argument("--deadline-s", type=float, default=400.0,
+    ap.add_argument("--deadline-s", type=float, default=330.0,
                     help="no new series step starts when it could end after this many seconds of the run")
-    ap.add_argument("--end-s", type=float, default=540.0, help="the measurements after the series fit in this")
+    ap.add_argument("--end-s", type=float, default=560.0, help="the measurements after the series fit in this")
+    ap.add_argument("--hex27", type=int, default=64, help="cells per side of the 3-D time-to-tolerance pair (0: skip)")
     ap.add_argument("--batch", type=int, default=16, help="problems of the batched measurement (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="fast form without the learned warm start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -529,9 +545,13 @@ def main():
     if args.extras and left() > 60:
         try:
             out["viscous64"] = viscous64_gpu()
-            out["hex27"] = hex27_measure(64, 2000)
         except Exception as e:   # reported, never required
             out["extras_error"] = repr(e)
+    if args.hex27 > 0 and left() > 120:
+        try:
+            out["hex27"] = hex27_measure(args.hex27)
+        except Exception as e:   # reported, never required
+            out["hex27"] = {"error": repr(e)}
     if model is not None and left() > 60:
         from femtools_amd import warmstart as WS
         try:

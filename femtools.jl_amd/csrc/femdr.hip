@@ -619,6 +619,7 @@ struct LaunchOpts {
     int32_t* yield_count = nullptr;   // nullable: count of elements with a yielding point
     uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
     double* phi_out = nullptr;        // nullable (FP64 residual launches): shear dissipation [nels][7]
+    float* tangent_out = nullptr;     // nullable (FP64 plastic residual launches): return linearisations
     bool cap = false;                 // the tensile cap is the yield model (FP64, EF_GEN launches)
     bool commit = false;              // a cap launch that commits the plastic history
 };
@@ -657,6 +658,7 @@ struct DRSet {
     const F *rhs_P = nullptr, *rhs_vx = nullptr, *rhs_vy = nullptr;
     const double *xvx = nullptr, *xvy = nullptr, *xP = nullptr, *xPf = nullptr;
     const uint8_t* hot = nullptr;   // near-yield flags of the elements (EF_DELTA)
+    const float* tangent = nullptr; // return linearisations about x (EF_DELTA|EF_TANGENT)
     typename Vec2<F>::T* bpart = nullptr;   // EF_TILE perimeter partial sums
     typename Vec2<F>::T* Rn = nullptr;      // EF_TILE interior-node sums
     bool tile = false;    // inner iterations run the EF_TILE element kernel
@@ -727,11 +729,11 @@ static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
 // without EF_ELASTIC; EF_TILE on structured meshes without EF_GEN
 template <int I>
 static void dispatch_lin(int key, fem_stokes* s, const ElemArgs<float>& a) {
-    constexpr bool el = I & 8, delta = I & 16, tile = I & 32;
+    constexpr bool el = I & 8, delta = I & 16, tile = I & 32, tan = I & 64;
     constexpr int FL = EF_LIN | EF_ITER | ((I & 1) ? EF_BULK : 0) | ((I & 2) ? EF_GEN : 0) |
                        ((I & 4) ? EF_STRUCT : 0) | (el ? EF_ELASTIC : 0) | (delta ? (EF_DELTA | EF_PLASTIC) : 0) |
-                       (tile ? EF_TILE : 0);
-    if constexpr ((!el || delta) && (!tile || ((I & 4) && !(I & 2)))) {
+                       (tile ? EF_TILE : 0) | (tan ? EF_TANGENT : 0);
+    if constexpr ((!el || delta) && (!tile || ((I & 4) && !(I & 2))) && (!tan || (delta && !el))) {
         if (key == FL) { launch_elem_t<float, FL>(s, a); return; }
     }
     if constexpr (I > 0) dispatch_lin<I - 1>(key, s, a);
@@ -792,6 +794,7 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
     a.yield_count = o.yield_count;
     a.yield_flag = o.yield_flag;
     a.hot = d.hot;
+    a.tangent = d.tangent;
     // phase arrays are always stored [nloc][nels]; row 0 is the cell phase
     const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
     // the correction operator has no temperature, source or fluid-pressure terms
@@ -820,6 +823,7 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
     if constexpr (std::is_same<F, double>::value) {
         if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
         a.phi_out = o.phi_out;
+        a.tangent_out = o.tangent_out;
         if (o.cap) {
             a.cap = 1;
             a.gamma = s->f[FEM_FIELD_GAMMA_PL];
@@ -844,8 +848,11 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
             a.bpart = d.bpart;
             a.Rn = d.Rn;
         }
+        // the linearised return needs no stress history (x's trial stress is in it)
         fl = key | EF_LIN | (tile ? EF_TILE : 0) |
-             (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic ? EF_ELASTIC : 0)) : 0);
+             (d.delta ? (EF_DELTA | EF_PLASTIC | (o.elastic && !d.tangent ? EF_ELASTIC : 0) |
+                         (d.tangent ? EF_TANGENT : 0))
+                      : 0);
     }
     return FEM_OK;
 }
@@ -856,7 +863,7 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     int fl = 0;
     CHK(build_elem_args(s, d, o, a, fl));
     if constexpr (std::is_same<F, double>::value) dispatch_elem<63>(fl, s, a);
-    else dispatch_lin<63>(fl, s, a);
+    else dispatch_lin<127>(fl, s, a);
     return check_launch(s->ctx, "k_t7_element");
 }
 
@@ -1548,12 +1555,15 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     // the return-map operator is used on the check intervals that start from a
     // state with yielding points; otherwise the visco-elastic operator is exact
     const bool delta = o.plastic && FEM_MIXED_RETURN_MAP;
+    const bool tangent = delta && p->precision == 2;
     CHK(ensure_w32(s, delta));
     fem_stokes::W32& w = *s->w32;
+    if (tangent && !w.tangent) CHK(dalloc_t(c, &w.tangent, (int64_t)NTAN * NQ * m->nels));
     const DRSet<double> d64 = set64(s);
     DRSet<float> d32 = set32(s);
     d32.delta = delta;
     d32.hot = s->d_yflag;
+    d32.tangent = tangent ? w.tangent : nullptr;
     // yielding elements of the last FP64 evaluation: counted on the device,
     // copied to pinned memory behind it, read at the next synchronisation
     int32_t* d_yield = s->d_flag + 1;
@@ -1617,6 +1627,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         oo.use_pnum = false; oo.do_update = false; oo.store_RP = true; oo.store_Rv = true;
         oo.yield_count = delta ? d_yield : nullptr;
         oo.yield_flag = delta ? s->d_yflag : nullptr;
+        oo.tangent_out = tangent ? w.tangent : nullptr;
         CHK(residual64(oo));
         double r3[3];
         CHK(reduce_outer(s, d64, r3));
@@ -1894,8 +1905,10 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     std::memset(st, 0, sizeof *st);
     HIPCHK(c, hipSetDevice(c->device));
     if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
-    if (p->precision != 0 && p->precision != 1)
-        return set_err(c, FEM_EARG, "precision must be 0 (FP64) or 1 (FP32 corrections of FP64 residuals), got %d",
+    if (p->precision < 0 || p->precision > 2)
+        return set_err(c, FEM_EARG,
+                       "precision must be 0 (FP64), 1 (FP32 corrections of FP64 residuals) or 2 (the same with "
+                       "the return map linearised about the FP64 state), got %d",
                        p->precision);
     if (cap && s->dim != 2) return set_err(c, FEM_EARG, "the tensile cap is built for 2-D states");
     if (cap && p->precision != 0)
@@ -1926,7 +1939,7 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
         hipLaunchKernelGGL(k_pscale, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np,
                            (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
     KernelTimer timer(c, p->time_kernels != 0);
-    if (p->precision == 1) {
+    if (p->precision >= 1) {
         const int rc = solve_mixed(s, p, o, freeze, timer, st);
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rc != FEM_OK) return rc;
@@ -2229,7 +2242,7 @@ static bool batch_groupable(int32_t n, fem_stokes* const* states, const fem_druc
     for (int32_t i = 0; i < n; ++i) {
         fem_stokes* s = states[i];
         const fem_solve_inputs& in = s->in;
-        if (s->ctx != states[0]->ctx || s->dim != 2 || !tile_ok(s->mesh) || params[i].precision != 1 ||
+        if (s->ctx != states[0]->ctx || s->dim != 2 || !tile_ok(s->mesh) || params[i].precision < 1 ||
             (plastic && plastic[i]) || params[i].check_interval < 2 || in.viscosity || in.scaling_viscosity ||
             in.P_old || in.history || in.body_force[0] || in.body_force[1] ||
             s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT)

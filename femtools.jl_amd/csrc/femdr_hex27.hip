@@ -2367,7 +2367,7 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
     // the one the scaling kernel reads
     if (!p->skip_scaling) CHK(hex_scaling_launch(s, p->pressure_factor, dt));
     else CHK(hex_pscale(s));
-    if (p->precision == 1) {
+    if (p->precision >= 1) {
         const int rcm = hex27_mixed(s, p, o, freeze, st);
         st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rcm != FEM_OK) return rcm;

@@ -150,6 +150,8 @@ struct fem_stokes {
         float2* Rn;      // summed residuals of the tiles' interior grid nodes
         // stress-history copy for the correction operator with the return map
         float *tau_old0, *tau_old1, *tau_old2;
+        // [6][7][nels] linearisations of the return about x (precision 2)
+        float* tangent;
     };
     W32* w32 = nullptr;
     // FP32 work arrays of the 3-D mixed-precision solve: the correction, its

@@ -145,7 +145,7 @@ struct IpMat {
 // whether the point yields.
 template <class S, class F, bool PLASTIC>
 __device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
-                                S& txx, S& tyy, S& txy, F* fyield = nullptr) {
+                                S& txx, S& tyy, S& txy, F* fyield = nullptr, float* tan = nullptr) {
     bool yields = false;
     S tr = (exx + eyy) * F(1.0 / 3.0);
     const F two_eve = F(2) * m.etave;
@@ -166,6 +166,17 @@ __device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F
                 S nxx = (F(2) * txx + tyy) * h;
                 S nyy = (txx + F(2) * tyy) * h;
                 S nxy = (F(2) * txy) * h;
+                if constexpr (std::is_same<S, F>::value) {
+                    if (tan) {
+                        const F tIIs = tII + eps2<F>();
+                        tan[0] = (float)h;
+                        tan[1] = (float)((m.etave / tIIs) * (m.invden - lam / tIIs) / (F(2) * tII));
+                        tan[2] = (float)(-(m.etave * m.invden * m.sphi) / tIIs);
+                        tan[3] = (float)(F(2) * txx + tyy);
+                        tan[4] = (float)(txx + F(2) * tyy);
+                        tan[5] = (float)(F(2) * txy);
+                    }
+                }
                 txx = txx - nxx;
                 tyy = tyy - nyy;
                 txy = txy - nxy;
@@ -409,7 +420,18 @@ enum ElemFlags {
                       // shared-node residuals are summed in LDS and stored per node for the
                       // tile's interior nodes, as per-tile partial sums for its perimeter
                       // nodes; k_tile_update finishes both
+    EF_TANGENT = 512, // with EF_DELTA: the return's part of τ(x + δ) - τ(x) is its linearisation
+                      // about x, read per point (tangent) instead of two FP64 returns
 };
+
+// Linearisation of the plane-strain Drucker-Prager return about a yielding
+// trial stress t at pressure P (stress2d's formulas). With q = M t, M =
+// [[2,1,0],[1,2,0],[0,0,2]], the correction c = t - τ is c = h q, h = ηve λ /
+// τII_s, λ = F invden, F = τII - y0 - sinϕ P, and
+//   ∂c/∂t u = h M u + kq q (q·u),   kq = (ηve/τII_s)(invden - λ/τII_s)/(2 τII),
+//   ∂c/∂P   = pq q,                 pq = -ηve invden sinϕ / τII_s,
+// from ∂τII/∂t = M t / (2 τII). Stored per point as (h, kq, pq, q0, q1, q2).
+constexpr int NTAN = 6;
 
 // Slot of the T6 grid node at offset pos = ox + 3 oy (ox, oy in 0..2) from a
 // cell's lower-left node in triangle tri of a cell of parity par, or -1
@@ -497,6 +519,11 @@ struct ElemArgs {
     // EF_DELTA: those flags; blocks without a flagged element skip the return
     // map (their operator is the exact visco-elastic one)
     const uint8_t* hot;
+    // [NTAN][NQ][nels] return linearisations: written by plastic residual
+    // launches (tangent_out, nullable; zeros at points that do not yield), read
+    // by EF_TANGENT launches at the points of hot elements (tangent)
+    float* tangent_out;
+    const float* tangent;
     // solve! array keywords (FP64 launches; nullable): nodal loads subtracted
     // from the momentum residual, with EF_GEN the viscosity at the points
     // [nels][NQ] and the previous-step pressure at the points [nels][NQ]
@@ -579,7 +606,8 @@ template <class F, int FL, int BLOCK>
 __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int64_t blk) {
     constexpr bool ELASTIC = FL & EF_ELASTIC, PLASTIC = FL & EF_PLASTIC, BULK = FL & EF_BULK;
     constexpr bool ITER = FL & EF_ITER, GEN = FL & EF_GEN, STRUCT = FL & EF_STRUCT, LIN = FL & EF_LIN;
-    constexpr bool DELTA = FL & EF_DELTA, TILE = FL & EF_TILE;
+    constexpr bool DELTA = FL & EF_DELTA, TILE = FL & EF_TILE, TANGENT = FL & EF_TANGENT;
+    static_assert(!TANGENT || DELTA, "EF_TANGENT linearises the EF_DELTA return map");
     static_assert(!(LIN && !DELTA && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
     static_assert(!DELTA || (LIN && PLASTIC), "EF_DELTA is the correction operator with the return map");
     static_assert(!TILE || (STRUCT && ITER && BLOCK % 64 == 0), "tiles are blocks of structured cells");
@@ -857,7 +885,18 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
             txx = two_eve * (gxx - trd);
             tyy = two_eve * (gyy - trd);
             txy = two_eve * ((gxy + gyx) * F(0.5));
-          if (hot_e) {
+          if (TANGENT && hot_e) {
+            // the return's linearisation about x: τ_δ = u - ∂c/∂t u - ∂c/∂P δP, u = 2 ηve ε̇'(δ)
+            const float* tg = A.tangent + (int64_t)q * ne + e;
+            const int64_t ks = (int64_t)NQ * ne;
+            const F h = tg[0], kq = tg[ks], pq = tg[2 * ks], q0 = tg[3 * ks], q1 = tg[4 * ks], q2 = tg[5 * ks];
+            const F sdot = q0 * txx + q1 * tyy + q2 * txy;
+            const F wq = kq * sdot + pq * Pq;
+            const F mxx = F(2) * txx + tyy, myy = txx + F(2) * tyy, mxy = F(2) * txy;
+            txx = txx - h * mxx - q0 * wq;
+            tyy = tyy - h * myy - q1 * wq;
+            txy = txy - h * mxy - q2 * wq;
+          } else if (!TANGENT && hot_e) {
             // x's strain rate and the two returns are evaluated in FP64 from the
             // FP64 state, read through the caches in blocks with points near
             // yield only: the stress of x then matches the FP64 residual's
@@ -939,11 +978,17 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
                 // the inner iteration only (no registers spent on them in EF_ITER)
                 constexpr bool STATS = PLASTIC && !ITER;
                 F fy = 0;
+                float tg[NTAN] = {0, 0, 0, 0, 0, 0};
                 const bool y = stress2d<F, F, PLASTIC>(gxx, gyy, (gxy + gyx) * F(0.5), m, txo, tyo, txyo, Pq - Pfq,
-                                                       txx, tyy, txy, STATS ? &fy : nullptr);
+                                                       txx, tyy, txy, STATS ? &fy : nullptr,
+                                                       STATS && A.tangent_out ? tg : nullptr);
                 if (STATS) {
                     yel |= y;
                     near |= fy > F(-0.2) * fabs(m.yield0 + m.sphi * (Pq - Pfq));
+                    if (A.tangent_out) {
+#pragma unroll
+                        for (int k = 0; k < NTAN; ++k) A.tangent_out[((int64_t)k * NQ + q) * ne + e] = tg[k];
+                    }
                 }
             }
         }

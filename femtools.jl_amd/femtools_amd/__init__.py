@@ -921,14 +921,20 @@ def solve(dr, mesh, *bcs, **kwargs):
     return solve_stokes(dr, mesh, *bcs, **kwargs)
 
 
+# fem_solve_params.precision: FP64; FP32 inner iterations on corrections of the
+# FP64 state (with a yield model, two FP64 returns per point near yield); the
+# same with the return linearised about the FP64 state (2-D Drucker-Prager)
+_PRECISIONS = {"fp64": 0, "fp32": 1, "fp32-tangent": 2}
+
+
 def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_factor=50.0, tolerance=1e-6,
                  max_iterations=50_000, check_interval=50, iterMax=50_000, max_ph_iterations=1000, rel_drop0=1e-2,
                  plastic=None, phases_v=None, phases_P=None, finite_K=None, freeze_jacobian=None,
                  throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False, time_kernels=False,
                  precision="fp64"):
     """Checks and C arguments of one solve! call: (bc array, plastic pointer, params)."""
-    if precision not in ("fp64", "fp32"):
-        raise ValueError(f"precision must be 'fp64' or 'fp32', got {precision!r}")
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
     if phases_v is not None:
@@ -950,7 +956,7 @@ def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_facto
                      freeze_jacobian=-1 if freeze_jacobian is None else int(bool(freeze_jacobian)),
                      throw_on_failure=int(bool(throw_on_failure)), use_tau_old=int(bool(tau_old)),
                      skip_scaling=int(bool(skip_scaling)), verbose=int(verbose),
-                     precision=0 if precision == "fp64" else 1, time_kernels=int(bool(time_kernels)))
+                     precision=_PRECISIONS[precision], time_kernels=int(bool(time_kernels)))
     return arr, dp, p
 
 
@@ -978,7 +984,10 @@ def solve_stokes(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryConditio
     keeps the state, the outer residuals, the norms and the convergence test
     in FP64 (same criterion and tolerance) and runs the inner pseudo-transient
     iterations in FP32 on a correction of the linearised visco-elastic
-    problem, folded into the FP64 state at every outer iteration."""
+    problem, folded into the FP64 state at every outer iteration.
+    precision="fp32-tangent" does the same, with the Drucker-Prager return of
+    2-D states linearised about the FP64 state at every fold instead of two
+    FP64 returns per point near yield."""
     keep = []
     inputs = _pop_inputs(kwargs)
     collect = bool(kwargs.pop("collect_history", False))

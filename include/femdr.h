@@ -223,7 +223,10 @@ typedef struct {
                                    residuals, norms and convergence test, inner
                                    DR iterations in FP32 on a correction δ of the
                                    linearised visco-elastic problem R(x) + L δ = 0,
-                                   folded into x at every outer iteration */
+                                   folded into x at every outer iteration (with a
+                                   Drucker-Prager model: two FP64 returns per point
+                                   near yield); 2 = as 1, with the 2-D return
+                                   linearised about x at every fold (3-D: as 1) */
     int32_t time_kernels;       /* 1: HIP events around every element / gather launch */
 } fem_solve_params;
 

@@ -73,13 +73,14 @@ def test_mixed_viscous_inclusion_reaches_tolerance(n):
     assert max(errs) < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp32-tangent"])
 @pytest.mark.parametrize("n", [12, 16])
-def test_mixed_visco_elastic_plastic_time_steps(n):
+def test_mixed_visco_elastic_plastic_time_steps(n, precision):
     # the reduced VEP series of the parity tests (4 inclusions, C=0.5, three
     # steps with the stress-history hand-off); 16 x 16 runs the FP32 inner
     # iterations in 16 x 8-cell tiles, 12 x 12 with the element-slot gather
     tol = 1e-8
-    say(f"[gpu] mixed precision VEP n={n}")
+    say(f"[gpu] mixed precision VEP n={n} {precision}")
     w = PB.vep_inclusions(n, ninc=4, seed=3, nsteps=3, C=0.5)
     coords, el2n = O.t7_mesh(n, n)
     ph = PB.inclusion_phases(coords, el2n, w.circles)
@@ -108,7 +109,7 @@ def test_mixed_visco_elastic_plastic_time_steps(n):
         dr.P0.copyto(P0)
         st_o = pr.solve(w.dt, CFL_v=w.CFL_v, **kw)
         ref = (pr.vx.copy(), pr.vy.copy(), pr.P.copy())
-        st = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, plastic=dp_g, precision="fp32", max_iterations=200_000,
+        st = fa.solve(dr, mesh, bcx, bcy, dt=w.dt, plastic=dp_g, precision=precision, max_iterations=200_000,
                       iterMax=200_000, **kw)
         got = (dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy())
         errs = [rel_err(a, b) for a, b in zip(got, ref)]
@@ -118,7 +119,10 @@ def test_mixed_visco_elastic_plastic_time_steps(n):
         ev, eP = oracle_errors(pr, *got, w.dt)
         assert max(ev, eP) == pytest.approx(st["err_abs"], rel=1e-6)
         assert min(st["err_abs"], st["err_rel"]) < tol
-        assert max(errs) < 1e-5
+        # two solutions meeting 1e-8 differ by a few times the distance of
+        # one of them to the exact discrete solution: the oracle's own step-0
+        # solution at 1e-8 lies 2.9e-6 (max rel) from its solution at 1e-11
+        assert max(errs) < 3e-5
         fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp_g)
         for c in ("xx", "yy", "xy"):
             getattr(dr.tau_old, c).copyto(getattr(dr.tau, c).numpy())
