@@ -278,6 +278,10 @@ def batch_measure(count, seed=0, max_iterations=400_000):
         kw = dict(dt=w.dt, tolerance=1e-8, check_interval=50, max_iterations=max_iterations, iterMax=max_iterations,
                   max_ph_iterations=w.max_ph_iterations, throw_on_failure=False, precision="fp32")
         probs.append((w, dr, mesh, bcs, kw))
+    from femtools_amd import autotune as AT
+    tuned = AT.tune_batch([(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw in probs], iterations=300,
+                          workload=f"{count} batched viscous T7 problems, n in 32..256")
+    log(f"batch tuning: {tuned['tuning']}")
     ctx.sync()
     t0 = time.perf_counter()
     res = fa.solve_batch([(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw in probs], raise_errors=False)
@@ -290,7 +294,7 @@ def batch_measure(count, seed=0, max_iterations=400_000):
                         f"inclusions, log10 eta in U[-2,4], FP32 bulk, tol 1e-8, one batched call",
             "seconds": round(wall, 3), "solves_per_s": round(count / wall, 3),
             "converged": sum(p["converged"] for p in per), "gdof_iter_per_s": round(dof_it / wall / 1e9, 3),
-            "problems": per}
+            "problems": per, "tuning": tuned["tuning"], "tuning_measurements": tuned["measurements"]}
 
 
 class Series:
@@ -375,7 +379,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="bounded warm-up runs (300 iterations per solver)")
     ap.add_argument("--n", type=int, default=512, help="cells per side")
     ap.add_argument("--tolerance", type=float, default=1e-8)
-    ap.add_argument("--max-iterations", type=int, default=400_000)
+    ap.add_argument("--max-iterations", type=int, default=600_000)
+    ap.add_argument("--fast-precision", default="fp32-tangent", choices=("fp32", "fp32-tangent"),
+                    help="the fast form's solve precision: FP32 corrections with two FP64 returns per point near "
+                         "yield, or with the return linearised about the FP64 state at every fold")
     ap.add_argument("--eta-reg", type=float, default=None, help="Perzyna viscosity (default problems.VEP_ETA_REG)")
     ap.add_argument("--deadline-s", type=float, default=330.0,
                     help="no new series step starts when it could end after this many seconds of the run")
@@ -401,6 +408,13 @@ def main():
     if steps != args.steps:
         log(f"--steps {args.steps}: the series has {SERIES_STEPS} physical steps; timing {steps}")
     S = Series(args.n, eta_reg)
+    # launch choices (tile shape, graph replay) measured on the series' start state
+    from femtools_amd import autotune as AT
+    kw_t = S.w.solve_kwargs()
+    kw_t.pop("tolerance", None)
+    tuned = AT.tune_solve(S.dr, S.mesh, S.bcs, iterations=1500, workload=f"{args.n}x{args.n} VEP step 0",
+                          dt=S.w.dt, plastic=S.dp, precision=args.fast_precision, **kw_t)
+    log(f"tuning: {tuned['tuning']}")
     w, mesh = S.w, S.mesh
     nv, ne, npp = mesh.nnodes, mesh.nels, mesh.nnodesP
     log(f"mesh {w.nx}x{w.ny}: {ne} elements, {w.dofs()} DoFs, eta_reg {eta_reg}")
@@ -417,7 +431,7 @@ def main():
     # state (kernels loaded, graphs instantiated), discarded
     s0 = S.snapshot()
     for i in range(args.warmup):
-        for prec in ("fp64", "fp32"):
+        for prec in ("fp64", args.fast_precision):
             S.restore(s0)
             st = S.solve(prec, False, args.tolerance, 300)
             log(f"warmup {prec}: {st['iter']} iterations {st['wall']:.2f}s")
@@ -445,7 +459,8 @@ def main():
                 guess = learned_guess(S, model)
                 tg = time.perf_counter() - t0
                 guess_s.append(tg)
-            st = S.solve(prec, True, args.tolerance, args.max_iterations, guess)
+            st = S.solve(args.fast_precision if prec == "fp32" else prec, True, args.tolerance, args.max_iterations,
+                         guess)
             st["wall"] += tg          # the warm start's inference counts in the time to tolerance
             stats[prec].append(st)
             sols[prec] = [f.numpy() for f in S.fields[:3]]
@@ -483,7 +498,14 @@ def main():
     tot_rm = ms_rm / max(n_rm, 1) * sum(s["iter"] for s in fast) * (n_rm / max(n_all, 1))
     tot_ve = ms_ve / max(n_ve, 1) * sum(s["iter"] for s in fast) * (n_ve / max(n_all, 1))
     nt6 = (2 * w.nx + 1) * (2 * w.ny + 1)
-    if n_rm > 0 and tot_rm >= tot_ve:
+    if n_rm > 0 and tot_rm >= tot_ve and args.fast_precision == "fp32-tangent":
+        kname = ("k_t7_element<float,EF_DELTA|EF_TANGENT tile> (FP32 correction operator with the linearised "
+                 "return, 16x8-cell tiles; bytes: the tile kernel's, the tangent reads of near-yield elements "
+                 "not counted)")
+        kbytes = algorithmic_bytes_fp32_tile(w, nv, ne, npp, nt6)
+        avg_s = ms_rm / n_rm * 1e-3
+        traffic = read_traffic("k_t7_element<float,tangent")
+    elif n_rm > 0 and tot_rm >= tot_ve:
         kname = "k_t7_element<float,EF_DELTA tile> (FP32 return-map correction operator, 16x8-cell tiles)"
         kbytes = algorithmic_bytes_fp32_rm(w, nv, ne, npp)
         avg_s = ms_rm / n_rm * 1e-3
@@ -524,6 +546,9 @@ def main():
         "speedup_vs_fp64": round(t64 / elapsed, 3) if elapsed > 0 else None,
         "gdof_iter_per_s_fp64": round(w.dofs() * sum(s["iter"] for s in ref) / t64 / 1e9, 4) if t64 > 0 else None,
         "fp32_vs_fp64_max_rel_diff": diffs,
+        "fast_precision": args.fast_precision,
+        "tuning": tuned["tuning"],
+        "tuning_measurements": tuned["measurements"],
         "warm_start_inference_s": [round(t, 4) for t in guess_s],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,

@@ -96,6 +96,23 @@ int fem_ctx_create(int device, fem_ctx** out) {
     return FEM_OK;
 }
 
+int fem_ctx_set_tuning(fem_ctx* c, const fem_tuning* t) {
+    if (!c || !t) return FEM_EARG;
+    if ((t->tiles != 0 && t->tiles != 1) || (t->graphs != 0 && t->graphs != 1))
+        return set_err(c, FEM_EARG, "tiles and graphs take 0 or 1");
+    if (t->tile_rows != 4 && t->tile_rows != 8 && t->tile_rows != 16)
+        return set_err(c, FEM_EARG, "tile_rows must be 4, 8 or 16, got %d", t->tile_rows);
+    if (t->fold_interval < 0) return set_err(c, FEM_EARG, "fold_interval must be >= 0, got %d", t->fold_interval);
+    c->tune = *t;
+    return FEM_OK;
+}
+
+int fem_ctx_get_tuning(fem_ctx* c, fem_tuning* t) {
+    if (!c || !t) return FEM_EARG;
+    *t = c->tune;
+    return FEM_OK;
+}
+
 int fem_ctx_destroy(fem_ctx* c) {
     if (!c) return FEM_OK;
     hipSetDevice(c->device);
@@ -626,18 +643,17 @@ struct LaunchOpts {
 
 constexpr int EBLOCK = 128;
 constexpr int GBLOCK = 256;
-constexpr int TBLOCK = 256;   // EF_TILE blocks: 16 x 8 cells
-#ifndef FEM_TILES
-#define FEM_TILES 1           // 0: FP32 inner iterations use the element-slot gather everywhere
-#endif
+// EF_TILE blocks: 16 x rows cells, 32 threads per cell row (fem_tuning.tile_rows)
 
 // FP32 inner iterations of structured 2-D meshes whose cell counts are
-// multiples of the 16 x 8 tile run the EF_TILE element kernel
-static bool tile_ok(const fem_mesh* m) {
-    return FEM_TILES && m->dim == 2 && m->structured && m->nx > 0 && m->nx % 16 == 0 && m->ny % (TBLOCK / 32) == 0;
+// multiples of the 16 x rows tile run the EF_TILE element kernel
+static int tile_rows(const fem_mesh* m) { return m->ctx->tune.tile_rows; }
+static bool tile_shape_ok(const fem_mesh* m, int rows) {
+    return m->dim == 2 && m->structured && m->nx > 0 && m->nx % 16 == 0 && m->ny % rows == 0;
 }
-static int64_t tile_count(const fem_mesh* m) { return (int64_t)(m->nx / 16) * (m->ny / (TBLOCK / 32)); }
-static int tile_perimeter_nodes() { return 2 * (32 + 1) + 2 * (2 * (TBLOCK / 32) - 1); }
+static bool tile_ok(const fem_mesh* m) { return m->ctx->tune.tiles && tile_shape_ok(m, tile_rows(m)); }
+static int64_t tile_count(const fem_mesh* m, int rows) { return (int64_t)(m->nx / 16) * (m->ny / rows); }
+static int tile_perimeter_nodes(int rows) { return 2 * (32 + 1) + 2 * (2 * rows - 1); }
 
 // One set of DR arrays in precision F. The FP64 solve runs on the state's own
 // fields; a correction solve of the mixed-precision path runs on FP32 work
@@ -701,7 +717,7 @@ static DRSet<float> set32(fem_stokes* s) {
     d.xvx = s->f[FEM_FIELD_V_X]; d.xvy = s->f[FEM_FIELD_V_Y]; d.xP = s->f[FEM_FIELD_P];
     d.xPf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
     d.tau_old[0] = w.tau_old0; d.tau_old[1] = w.tau_old1; d.tau_old[2] = w.tau_old2;
-    d.tile = w.bpart != nullptr;
+    d.tile = w.bpart != nullptr && tile_ok(s->mesh);
     d.bpart = w.bpart;
     d.Rn = w.Rn;
     d.lin = true;
@@ -710,10 +726,14 @@ static DRSet<float> set32(fem_stokes* s) {
 
 template <class F, int FL>
 static void launch_elem_t(fem_stokes* s, const ElemArgs<F>& a) {
-    if constexpr ((FL & EF_TILE) != 0)
-        hipLaunchKernelGGL((k_t7_element<F, FL, TBLOCK>), dim3(grid_xcd(a.ntiles, 1)), dim3(TBLOCK), 0,
-                           s->ctx->stream, a);
-    else
+    if constexpr ((FL & EF_TILE) != 0) {
+        const dim3 grid(grid_xcd(a.ntiles, 1));
+        switch (tile_rows(s->mesh)) {
+            case 4: hipLaunchKernelGGL((k_t7_element<F, FL, 128>), grid, dim3(128), 0, s->ctx->stream, a); break;
+            case 16: hipLaunchKernelGGL((k_t7_element<F, FL, 512>), grid, dim3(512), 0, s->ctx->stream, a); break;
+            default: hipLaunchKernelGGL((k_t7_element<F, FL, 256>), grid, dim3(256), 0, s->ctx->stream, a); break;
+        }
+    } else
         hipLaunchKernelGGL((k_t7_element<F, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
                            s->ctx->stream, a);
 }
@@ -844,7 +864,7 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
         const bool tile = d.tile && strct && !gen;
         if (tile) {
             a.ntx = m->nx / 16;
-            a.ntiles = tile_count(m);
+            a.ntiles = tile_count(m, tile_rows(m));
             a.bpart = d.bpart;
             a.Rn = d.Rn;
         }
@@ -905,8 +925,9 @@ template <class F>
 static void build_tile_update_args(fem_stokes* s, const DRSet<F>& d, const GatherArgs<F>& g, TileUpdArgs<F>& P) {
     fem_mesh* m = s->mesh;
     P.g = g;
-    P.ntx = m->nx / 16; P.nty = m->ny / (TBLOCK / 32);
-    P.TH = 2 * (TBLOCK / 32);
+    const int rows = tile_rows(m);
+    P.ntx = m->nx / 16; P.nty = m->ny / rows;
+    P.TH = 2 * rows;
     P.sx = 2 * m->nx + 1;
     P.nT6 = (int64_t)P.sx * (2 * m->ny + 1);
     P.bpart = d.bpart;
@@ -1205,19 +1226,9 @@ static int prepare(fem_stokes* s, const fem_drucker_prager* dp, double dt, int f
     return FEM_OK;
 }
 
-#ifndef FEM_GRAPHS
-#define FEM_GRAPHS 1   // 0: every inner iteration is launched from the host
-#endif
-
-// graph replay on (FEM_GRAPHS) unless the environment sets FEMDR_GRAPHS=0,
-// e.g. for tracing tools that do not follow graph launches
-static bool graphs_on() {
-    static const bool on = [] {
-        const char* e = std::getenv("FEMDR_GRAPHS");
-        return FEM_GRAPHS && !(e && e[0] == '0');
-    }();
-    return on;
-}
+// graph replay of check intervals (fem_tuning.graphs; off for tracing tools
+// that do not follow graph launches)
+static bool graphs_on(const fem_ctx* c) { return c->tune.graphs != 0; }
 
 
 // One check interval (ncheck inner iterations: element + gather launches with
@@ -1352,7 +1363,7 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
         while ((err > eps_vel || !thermal_ok()) && itPT <= ctl.iter_max) {
             // a whole check interval that cannot end early is replayed from the
             // graph (every 16th interval stays on the host when kernels are timed)
-            const bool whole = graphs_on() && !coupled && ncheck > 1 && iter % ncheck == 0 &&
+            const bool whole = graphs_on(s->ctx) && !coupled && ncheck > 1 && iter % ncheck == 0 &&
                                itPT + ncheck - 1 <= ctl.iter_max && iter + ncheck - 1 <= ctl.max_iterations &&
                                !(timer.on && (iter / ncheck) % 16 == 0);
             if (whole) {
@@ -1485,8 +1496,10 @@ static int ensure_w32(fem_stokes* s, bool delta) {
     for (float** q : pl) CHK(dalloc_t(c, q, np));
     CHK(dalloc_t(c, &w.geo, 5 * ne));
     CHK(dalloc_t(c, &w.Re, NSH * ne));
-    if (tile_ok(m)) {
-        CHK(dalloc_t(c, &w.bpart, tile_count(m) * tile_perimeter_nodes()));
+    // sized for the smallest tiles (4 rows: the most perimeter nodes), so
+    // that any tile_rows the mesh admits fits
+    if (tile_shape_ok(m, 4)) {
+        CHK(dalloc_t(c, &w.bpart, tile_count(m, 4) * tile_perimeter_nodes(4)));
         CHK(dalloc_t(c, &w.Rn, (int64_t)(2 * m->nx + 1) * (2 * m->ny + 1)));
     }
     HIPCHK(c, hipMemsetAsync(w.zero_v, 0, nv * 4, c->stream));
@@ -1533,20 +1546,14 @@ static int add_correction(fem_ctx* c, double* x, const float* dx, int64_t n, dou
 // the count is read back at once, and iterations after an evaluation with a
 // count of zero run the cheaper visco-elastic operator, exact there. A point
 // that starts to yield inside such a stretch is corrected by the next fold,
-// at most FEM_FOLD_INTERVAL iterations later. The solution x meets the
+// at most fem_tuning.fold_interval iterations later. The solution x meets the
 // Dirichlet values before the first iteration.
-#ifndef FEM_MIXED_RETURN_MAP
-#define FEM_MIXED_RETURN_MAP 1
-#endif
-// With a yield model δ is also folded every FEM_FOLD_INTERVAL iterations
-// between checks: the PH/DYREL trajectory under plasticity is
+// With a yield model δ is also folded every fem_tuning.fold_interval
+// iterations between checks: the PH/DYREL trajectory under plasticity is
 // sensitive to perturbations, and smaller FP32 corrections keep the mixed
 // iteration closer to the FP64 one (512² VEP, plastic step: 203k iterations
 // folding at checks only, 151k every 25, 144k-165k every 10, 139k in FP64), for
 // one FP64 residual per fold.
-#ifndef FEM_FOLD_INTERVAL
-#define FEM_FOLD_INTERVAL 25
-#endif
 static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpts& o, bool freeze, KernelTimer& timer,
                        fem_solve_stats* st) {
     fem_ctx* c = s->ctx;
@@ -1554,7 +1561,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     const int64_t nv = m->nnodes, np = m->nnodesP, nip = NQ * m->nels, ncheck = p->check_interval;
     // the return-map operator is used on the check intervals that start from a
     // state with yielding points; otherwise the visco-elastic operator is exact
-    const bool delta = o.plastic && FEM_MIXED_RETURN_MAP;
+    const bool delta = o.plastic;
+    const int64_t fold_every = c->tune.fold_interval;
     const bool tangent = delta && p->precision == 2;
     CHK(ensure_w32(s, delta));
     fem_stokes::W32& w = *s->w32;
@@ -1621,6 +1629,18 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         }
         return FEM_OK;
     };
+    // δv into the FP64 state, δv = 0
+    auto fold_v = [&]() -> int {
+        hipLaunchKernelGGL(k_fold2, dim3(grid_for(nv, 256)), dim3(256), 0, c->stream, nv, d64.vx, d64.vy, w.vx, w.vy);
+        return check_launch(c, "k_fold2");
+    };
+    // the right-hand sides of the correction problem from the FP64 residual
+    auto rhs32 = [&]() -> int {
+        hipLaunchKernelGGL(k_cast_rhs, dim3(grid_for(std::max(nv, np), 256)), dim3(256), 0, c->stream, nv, np,
+                           (const double*)d64.Rv_x, (const double*)d64.Rv_y, (const double*)d64.RP, w.rhs_vx,
+                           w.rhs_vy, w.rhs_P);
+        return check_launch(c, "k_cast_rhs");
+    };
     for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
         itPH_done = itPH;
         LaunchOpts oo = o;
@@ -1652,9 +1672,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         err_min = std::min(err_min, err);
         const double eps_vel = err * rel_drop;
         // right-hand side of the correction problem: the FP64 residual
-        CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
-        CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
-        CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+        CHK(rhs32());
         int64_t itPT = 0;
         bool stop_total = false;
         while (err > eps_vel && itPT <= p->iter_max) {
@@ -1666,8 +1684,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
             int64_t nseg = 1;
             if (ncheck > 1 && !timed) {
                 nseg = ncheck - iter % ncheck;
-                if (delta && FEM_FOLD_INTERVAL > 0)
-                    nseg = std::min<int64_t>(nseg, FEM_FOLD_INTERVAL - iter % FEM_FOLD_INTERVAL);
+                if (delta && fold_every > 0)
+                    nseg = std::min<int64_t>(nseg, fold_every - iter % fold_every);
                 nseg = std::max<int64_t>(1, std::min(nseg, std::min(p->iter_max - itPT + 1, p->max_iterations - iter + 1)));
             }
             const int64_t iter_end = iter + nseg;
@@ -1706,7 +1724,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 timer.mark(e0);
                 CHK(seg(nullptr));
                 timer.mark(e2);
-            } else if (graphs_on() && nseg > 1) {
+            } else if (graphs_on(c) && nseg > 1) {
                 const int64_t key = ((nseg * 2 + (lastp ? 1 : 0)) * (ncheck + 1) + mpos) * 2 + (d32.delta ? 1 : 0);
                 IntervalGraph* gph = nullptr;
                 for (auto& kv : graphs)
@@ -1744,15 +1762,10 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 // of the last ncheck steps, far above their rounding floor, and
                 // the operator is renewed at the current state; the yield count
                 // of this evaluation picks the next iterations' operator
-                CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
-                CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
-                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
-                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                CHK(fold_v());
                 CHK(residual64(oo));
                 const int32_t ycount = delta ? *h_yield : 0;
-                CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
-                CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
-                CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+                CHK(rhs32());
                 // without yielding points the diagnostics are the visco-elastic
                 // ones: re-assemble only when yielding was seen since the last
                 // assembly
@@ -1769,16 +1782,11 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 }
                 for (int k = 0; k < 2; ++k)
                     cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, lmin[k], s->opts.c_fact, &al[k], &be[k]);
-            } else if (FEM_FOLD_INTERVAL > 0 && delta && iter % FEM_FOLD_INTERVAL == 0) {
+            } else if (fold_every > 0 && delta && iter % fold_every == 0) {
                 // extra folds between checks with a yield model
-                CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
-                CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
-                HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
-                HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
+                CHK(fold_v());
                 CHK(residual64(oo));
-                CHK(cast_to32(c, d64.Rv_x, w.rhs_vx, nv));
-                CHK(cast_to32(c, d64.Rv_y, w.rhs_vy, nv));
-                CHK(cast_to32(c, d64.RP, w.rhs_P, np));
+                CHK(rhs32());
             }
             if (iter > p->max_iterations) { stop_total = true; break; }
         }
@@ -1787,11 +1795,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         hipLaunchKernelGGL(k_pressure_update<float>, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, w.P,
                            (const float*)w.gammaP, (const float*)w.RP, (const float*)w.MP);
         CHK(check_launch(c, "k_pressure_update"));
-        CHK(add_correction(c, d64.vx, w.vx, nv, 1.0));
-        CHK(add_correction(c, d64.vy, w.vy, nv, 1.0));
+        CHK(fold_v());
         CHK(add_correction(c, d64.P, w.P, np, 1.0));
-        HIPCHK(c, hipMemsetAsync(w.vx, 0, nv * 4, c->stream));
-        HIPCHK(c, hipMemsetAsync(w.vy, 0, nv * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(w.P, 0, np * 4, c->stream));
         if (stop_total || iter > p->max_iterations) break;
     }
@@ -2335,6 +2340,7 @@ static int batch_segment_single(BatchItem& b, int64_t nseg) {
 // per-problem Chebyshev scalars the launches read
 struct BatchGroup {
     int fl = 0;
+    int rows = 8;                          // tile rows of the group's launches
     std::vector<int32_t> members;          // indices into the item list
     ElemArgs<float>* d_ea = nullptr;       // [3][n]
     TileUpdArgs<float>* d_ua = nullptr;    // [3][n]
@@ -2350,14 +2356,16 @@ static bool group_key_ok(int key) {
     return key == (EF_LIN | EF_ITER | EF_STRUCT | EF_TILE) || key == (EF_LIN | EF_ITER | EF_STRUCT | EF_TILE | EF_BULK);
 }
 template <int I>
-static void launch_group_lin(int key, hipStream_t st, unsigned grid, const ElemArgs<float>* ea, const int32_t* off,
-                             int32_t n) {
+static void launch_group_lin(int key, int rows, hipStream_t st, unsigned grid, const ElemArgs<float>* ea,
+                             const int32_t* off, int32_t n) {
     constexpr int FL = EF_LIN | EF_ITER | EF_STRUCT | EF_TILE | ((I & 1) ? EF_BULK : 0);
     if (key == FL) {
-        hipLaunchKernelGGL((k_t7_element_group<float, FL, TBLOCK>), dim3(grid), dim3(TBLOCK), 0, st, ea, off, n);
+        if (rows == 4) hipLaunchKernelGGL((k_t7_element_group<float, FL, 128>), dim3(grid), dim3(128), 0, st, ea, off, n);
+        else if (rows == 16) hipLaunchKernelGGL((k_t7_element_group<float, FL, 512>), dim3(grid), dim3(512), 0, st, ea, off, n);
+        else hipLaunchKernelGGL((k_t7_element_group<float, FL, 256>), dim3(grid), dim3(256), 0, st, ea, off, n);
         return;
     }
-    if constexpr (I > 0) launch_group_lin<I - 1>(key, st, grid, ea, off, n);
+    if constexpr (I > 0) launch_group_lin<I - 1>(key, rows, st, grid, ea, off, n);
 }
 
 static void free_group(BatchGroup& g) {
@@ -2373,6 +2381,7 @@ static int build_group(fem_ctx* c, std::vector<BatchItem>& items, const std::vec
     free_group(g);
     if (!group_key_ok(fl)) return set_err(c, FEM_EARG, "no grouped element kernel for key %d", fl);
     g.fl = fl;
+    g.rows = c->tune.tile_rows;
     g.members = mem;
     g.ncheck = ncheck;
     const int32_t n = (int32_t)mem.size();
@@ -2414,13 +2423,13 @@ static int build_group(fem_ctx* c, std::vector<BatchItem>& items, const std::vec
     auto launches = [&]() -> int {
         for (int64_t j = 1; j <= ncheck; ++j) {
             const int v = j == ncheck ? 2 : (j == ncheck - 1 ? 1 : 0);
-            launch_group_lin<1>(fl, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
+            launch_group_lin<1>(fl, g.rows, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
             hipLaunchKernelGGL((k_tile_update_group<float, GBLOCK>), dim3(gu), dim3(GBLOCK), 0, c->stream,
                                (const TileUpdArgs<float>*)(g.d_ua + v * n), (const int32_t*)g.d_uoff, n);
         }
         return check_launch(c, "k_t7_element_group");
     };
-    if (!graphs_on()) return FEM_OK;
+    if (!graphs_on(c)) return FEM_OK;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const int rc = launches();
     hipGraph_t graph = nullptr;
@@ -2445,7 +2454,7 @@ static int run_group(fem_ctx* c, BatchGroup& g) {
     const unsigned ge = grid_xcd(g.eblocks, 1), gu = grid_xcd(g.ublocks, 1);
     for (int64_t j = 1; j <= g.ncheck; ++j) {
         const int v = j == g.ncheck ? 2 : (j == g.ncheck - 1 ? 1 : 0);
-        launch_group_lin<1>(g.fl, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
+        launch_group_lin<1>(g.fl, g.rows, c->stream, ge, g.d_ea + v * n, g.d_eoff, n);
         hipLaunchKernelGGL((k_tile_update_group<float, GBLOCK>), dim3(gu), dim3(GBLOCK), 0, c->stream,
                            (const TileUpdArgs<float>*)(g.d_ua + v * n), (const int32_t*)g.d_uoff, n);
     }
@@ -2641,7 +2650,7 @@ static int solve_batch_grouped(int32_t n, fem_stokes* const* states, const fem_b
                         break;
                 }
                 if (rc != FEM_OK) break;
-                if (graphs_on())
+                if (graphs_on(c))
                     HIPCHK(c, hipMemcpyAsync(d_cheb, h_cheb, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
             }
             for (auto& g : groups)

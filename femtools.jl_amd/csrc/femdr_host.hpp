@@ -36,6 +36,7 @@ struct fem_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;     // kernel-timing events (3 per timed iteration)
     size_t tev_used = 0;
+    fem_tuning tune{1, 8, 1, 25};    // launch choices (fem_ctx_set_tuning)
 };
 
 // Kernel timing: events e0 | element | e1 | gather | e2 around every
@@ -146,7 +147,7 @@ struct fem_stokes {
         float *vx, *vy, *rate_x, *rate_y, *Rv_x, *Rv_y, *Rv0_x, *Rv0_y, *PC_x, *PC_y;
         float *P, *RP, *Pnum, *gammaP, *MP, *pscale, *rhs_vx, *rhs_vy, *rhs_P, *zero_v, *geo;
         float2* Re;
-        float2* bpart;   // per-tile perimeter partial sums (structured meshes with 16 x 8-cell tiles)
+        float2* bpart;   // per-tile perimeter partial sums (structured meshes in 16-cell-wide tiles)
         float2* Rn;      // summed residuals of the tiles' interior grid nodes
         // stress-history copy for the correction operator with the return map
         float *tau_old0, *tau_old1, *tau_old2;

@@ -556,24 +556,9 @@ __device__ inline int64_t xcd_tile() {
     return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-// Occupancy target of the element kernel (waves per SIMD); 0 leaves the
-// register allocation to the compiler.
-#ifndef FEM_ELEM_WAVES
-#define FEM_ELEM_WAVES 3
-#endif
-// τ_old staged through LDS by direct global->LDS loads (1) or read per
-// quadrature point (0)
-#ifndef FEM_ALL_HOT
-#define FEM_ALL_HOT 0   // 1: the return-map correction operator in every block
-#endif
-#ifndef FEM_TAU_LDS
-#define FEM_TAU_LDS 1
-#endif
-#if FEM_ELEM_WAVES > 0
-#define FEM_ELEM_ATTR __attribute__((amdgpu_waves_per_eu(FEM_ELEM_WAVES)))
-#else
-#define FEM_ELEM_ATTR
-#endif
+// Occupancy target of the element kernel: 3 waves per SIMD (the FP64 kernel's
+// register budget at which it runs fastest on gfx950)
+#define FEM_ELEM_ATTR __attribute__((amdgpu_waves_per_eu(3)))
 
 // gathered residual (rx, ry) of a shared node: right-hand side, Dirichlet
 // pinning, optional stores and the Chebyshev update (helpers.jl:31-41,
@@ -611,7 +596,8 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
     static_assert(!(LIN && !DELTA && (ELASTIC || PLASTIC)), "the linearised operator has no history or return map");
     static_assert(!DELTA || (LIN && PLASTIC), "EF_DELTA is the correction operator with the return map");
     static_assert(!TILE || (STRUCT && ITER && BLOCK % 64 == 0), "tiles are blocks of structured cells");
-    constexpr bool TAU_LDS = ELASTIC && FEM_TAU_LDS;
+    // τ_old of the block's elements staged through LDS by direct global->LDS loads
+    constexpr bool TAU_LDS = ELASTIC;
     constexpr int TCY = BLOCK / 32;                 // EF_TILE: rows of 16 cells per tile
     const int64_t ne = A.nels;
     int64_t e0, e, tile = 0, ttx = 0, tty = 0;
@@ -642,7 +628,7 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
     // either state, so its correction operator is the visco-elastic one)
     bool hot = true, hot_e = true;
     if (DELTA) {
-        hot_e = FEM_ALL_HOT || (valid && A.hot[e]);
+        hot_e = valid && A.hot[e];
         hot = __syncthreads_or(hot_e);
     }
     if (TAU_LDS && TILE && hot) {
@@ -1723,6 +1709,28 @@ static __global__ void k_cast_sub(int64_t n, const double* a, const double* b, f
 static __global__ void k_add_correction(int64_t n, double* x, const float* d, double scale) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += scale * (double)d[i];
+}
+
+// folds a two-component correction into the FP64 state and clears it
+static __global__ void k_fold2(int64_t n, double* x0, double* x1, float* d0, float* d1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x0[i] += 1.0 * (double)d0[i];
+        x1[i] += 1.0 * (double)d1[i];
+        d0[i] = 0.0f;
+        d1[i] = 0.0f;
+    }
+}
+
+// the correction problem's right-hand sides: FP32 copies of R_v(x) (nv) and R_P(x) (np)
+static __global__ void k_cast_rhs(int64_t nv, int64_t np, const double* Rx, const double* Ry, const double* RP,
+                                  float* rx, float* ry, float* rp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nv) {
+        rx[i] = (float)(Rx[i] * 1.0);
+        ry[i] = (float)(Ry[i] * 1.0);
+    }
+    if (i < np) rp[i] = (float)(RP[i] * 1.0);
 }
 
 template <class T>

@@ -200,6 +200,8 @@ _sig("fem_stokes_residual", C.c_int, VP, _P(_DP), D, I32, I32)
 _sig("fem_stokes_jacobian_diagnostics", C.c_int, VP, _P(_DP), D, I32)
 _sig("fem_stokes_time_element_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
 _sig("fem_stokes_time_correction_kernel", C.c_int, VP, _P(_DP), D, I32, _P(D))
+_sig("fem_ctx_set_tuning", C.c_int, VP, VP)
+_sig("fem_ctx_get_tuning", C.c_int, VP, VP)
 _sig("fem_thermal_create", C.c_int, VP, VP, _P(_ThermalMaterial), _P(_ThermalOpts), _P(VP))
 _sig("fem_thermal_destroy", C.c_int, VP)
 _sig("fem_thermal_field", C.c_int, VP, I32, _P(VP), _P(I64), _P(I32))
@@ -253,7 +255,7 @@ EXPORTED = [
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
     "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
-    "fem_stokes_time_element_kernel", "fem_stokes_time_correction_kernel", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
+    "fem_stokes_time_element_kernel", "fem_stokes_time_correction_kernel", "fem_ctx_set_tuning", "fem_ctx_get_tuning", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
     "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
     "fem_stokes_jacobian_diagnostics_cap", "fem_stokes_update_current_stress_cap",
@@ -264,6 +266,10 @@ EXPORTED = [
     "fem_cnn_set_weights", "fem_cnn_get_weights", "fem_cnn_get_grads", "fem_cnn_forward", "fem_cnn_train_step",
     "fem_cnn_gemm_bf16", "fem_norms_noalloc", "fem_dr_velocity_update", "fem_dp_return_map_batch",
 ]
+
+
+class _Tuning(C.Structure):
+    _fields_ = [("tiles", I32), ("tile_rows", I32), ("graphs", I32), ("fold_interval", I32)]
 
 
 class Context:
@@ -295,14 +301,41 @@ class Context:
     def sync(self):
         self.check(_lib.fem_sync(self.handle))
 
+    @property
+    def tuning(self) -> dict:
+        """The context's launch choices (fem_tuning): tiles, tile_rows, graphs,
+        fold_interval."""
+        t = _Tuning()
+        self.check(_lib.fem_ctx_get_tuning(self.handle, C.byref(t)))
+        return {k: int(getattr(t, k)) for k, _ in _Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Changes some of the launch choices, e.g. set_tuning(tile_rows=16,
+        graphs=0); unknown names raise ValueError."""
+        cur = self.tuning
+        for k, v in kw.items():
+            if k not in cur:
+                raise ValueError(f"unknown tuning parameter {k!r} (known: {sorted(cur)})")
+            cur[k] = int(v)
+        self.check(_lib.fem_ctx_set_tuning(self.handle, C.byref(_Tuning(**cur))))
+
 
 _default_ctx = None
 
 
 def default_context() -> Context:
+    """The process's context on device FEMDR_DEVICE (default 0). If
+    FEMDR_TUNING names a JSON file written by femtools_amd.autotune, its
+    "tuning" object is applied."""
     global _default_ctx
     if _default_ctx is None:
-        _default_ctx = Context(int(os.environ.get("FEMDR_DEVICE", "0")))
+        ctx = Context(int(os.environ.get("FEMDR_DEVICE", "0")))
+        path = os.environ.get("FEMDR_TUNING")
+        if path:
+            import json
+            with open(path) as f:
+                ctx.set_tuning(**json.load(f)["tuning"])
+        _default_ctx = ctx
     return _default_ctx
 
 
@@ -606,6 +639,7 @@ class MixedMesh:
     def __init__(self, ctx: Context, handle, box=None):
         self.ctx = ctx
         self.handle = handle
+        self.nx = self.ny = self.nz = 0   # cells per side of generated box meshes (0: uploaded mesh)
         info = _MeshInfo()
         ctx.check(_lib.fem_mesh_get_info(handle, C.byref(info)))
         self.nnodes, self.nels, self.nnodesP = info.nnodes, info.nels, info.nnodesP
@@ -640,7 +674,9 @@ def t7_box_mesh(nx: int, ny: int, domain=((0.0, 1.0), (0.0, 1.0)), ctx: Context 
     (x0, x1), (y0, y1) = domain
     h = VP()
     ctx.check(_lib.fem_mesh_t7_create(ctx.handle, int(nx), int(ny), x0, x1, y0, y1, C.byref(h)))
-    return MixedMesh(ctx, h, box=(x0, x1, y0, y1))
+    m = MixedMesh(ctx, h, box=(x0, x1, y0, y1))
+    m.nx, m.ny = int(nx), int(ny)
+    return m
 
 
 def hex27_box_mesh(nx: int, ny: int, nz: int, domain=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)),
@@ -651,7 +687,9 @@ def hex27_box_mesh(nx: int, ny: int, nz: int, domain=((0.0, 1.0), (0.0, 1.0), (0
     h = VP()
     ctx.check(_lib.fem_mesh_hex27_create(ctx.handle, int(nx), int(ny), int(nz), x0, x1, y0, y1, z0, z1,
                                          C.byref(h)))
-    return MixedMesh(ctx, h, box=(x0, x1, y0, y1, z0, z1))
+    m = MixedMesh(ctx, h, box=(x0, x1, y0, y1, z0, z1))
+    m.nx, m.ny, m.nz = int(nx), int(ny), int(nz)
+    return m
 
 
 def t7_mesh_from_arrays(coords, el2n, ctx: Context | None = None) -> MixedMesh:

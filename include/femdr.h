@@ -46,6 +46,28 @@ typedef struct fem_stokes fem_stokes;
  * (src/FEMTools.jl:30-46, ext/FEMToolsCUDAExt.jl:5). */
 int fem_ctx_create(int device, fem_ctx** out);
 int fem_ctx_destroy(fem_ctx* ctx);
+
+/* Launch choices of a context's solves. No counterpart in the reference (its
+ * KernelAbstractions launches have no such parameters); the Python package's
+ * autotuner (femtools_amd.autotune) measures them on a problem and writes
+ * them as JSON. Changing them between solves is allowed; a batch or solve in
+ * progress keeps the values it started with. */
+typedef struct {
+    int32_t tiles;          /* 1 (default): FP32 inner iterations of structured 2-D meshes run the
+                               tile kernel where the cell counts allow; 0: element-slot gather */
+    int32_t tile_rows;      /* cell rows per tile of 16 cells: 4, 8 (default) or 16 (blocks of
+                               128, 256 or 512 threads); needs ny % tile_rows == 0 */
+    int32_t graphs;         /* 1 (default): check intervals replayed from HIP graphs; 0: every
+                               inner iteration launched from the host */
+    int32_t fold_interval;  /* mixed precision with a yield model: δv folded into the FP64 state
+                               every fold_interval iterations between checks (0: at checks only);
+                               default 25 */
+} fem_tuning;
+
+/* Sets / reads the context's launch choices; FEM_EARG for values outside the
+ * ranges above. */
+int fem_ctx_set_tuning(fem_ctx* ctx, const fem_tuning* t);
+int fem_ctx_get_tuning(fem_ctx* ctx, fem_tuning* t);
 int fem_last_error(const fem_ctx* ctx, char* buf, size_t len);
 int fem_sync(fem_ctx* ctx);
 
