@@ -36,6 +36,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "femtools.jl_amd"))
 
@@ -202,6 +204,9 @@ def viscous64_gpu():
 # lane's point 480, continuity / stress / fluxes ~106, nodal residual 648)
 HEX27_FLOPS_PER_ELEMENT = 27 * 1234
 
+# graph replay settings the autotuner may try (main sets (0,) under --no-graphs)
+GRAPHS = (1, 0)
+
 
 def hex27_measure(n, max_iterations=200_000):
     """The 3-D path: n^3 Hex27 cells, 50 seeded spheres (r in U[0.02, 0.08],
@@ -280,7 +285,7 @@ def batch_measure(count, seed=0, max_iterations=400_000):
         probs.append((w, dr, mesh, bcs, kw))
     from femtools_amd import autotune as AT
     tuned = AT.tune_batch([(dr, mesh, bcs, kw) for w, dr, mesh, bcs, kw in probs], iterations=300,
-                          workload=f"{count} batched viscous T7 problems, n in 32..256")
+                          workload=f"{count} batched viscous T7 problems, n in 32..256", graphs=GRAPHS)
     log(f"batch tuning: {tuned['tuning']}")
     ctx.sync()
     t0 = time.perf_counter()
@@ -388,6 +393,9 @@ def main():
                     help="no new series step starts when it could end after this many seconds of the run")
     ap.add_argument("--end-s", type=float, default=560.0, help="the measurements after the series fit in this")
     ap.add_argument("--hex27", type=int, default=64, help="cells per side of the 3-D time-to-tolerance pair (0: skip)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every inner iteration from the host (for tracing tools that do not follow "
+                         "graph launches); the autotuner then tries no graph replay")
     ap.add_argument("--batch", type=int, default=16, help="problems of the batched measurement (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="fast form without the learned warm start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -399,10 +407,13 @@ def main():
               "strips with an interface-node exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import femtools_amd as fa
     from femtools_amd import problems as PB
 
+    global GRAPHS
+    GRAPHS = (0,) if args.no_graphs else (1, 0)
+    if args.no_graphs:
+        fa.default_context().set_tuning(graphs=0)
     eta_reg = PB.VEP_ETA_REG if args.eta_reg is None else args.eta_reg
     steps = max(1, min(args.steps, SERIES_STEPS))
     if steps != args.steps:
@@ -413,7 +424,7 @@ def main():
     kw_t = S.w.solve_kwargs()
     kw_t.pop("tolerance", None)
     tuned = AT.tune_solve(S.dr, S.mesh, S.bcs, iterations=1500, workload=f"{args.n}x{args.n} VEP step 0",
-                          dt=S.w.dt, plastic=S.dp, precision=args.fast_precision, **kw_t)
+                          graphs=GRAPHS, dt=S.w.dt, plastic=S.dp, precision=args.fast_precision, **kw_t)
     log(f"tuning: {tuned['tuning']}")
     w, mesh = S.w, S.mesh
     nv, ne, npp = mesh.nnodes, mesh.nels, mesh.nnodesP
@@ -461,6 +472,18 @@ def main():
                 guess_s.append(tg)
             st = S.solve(args.fast_precision if prec == "fp32" else prec, True, args.tolerance, args.max_iterations,
                          guess)
+            if prec == "fp32" and not st["converged"] and args.fast_precision != "fp32":
+                # the fast form's fallback: the step again from its start in
+                # precision 1 (two FP64 returns per point near yield); both
+                # attempts count in its time to tolerance
+                log(f"step {i} {args.fast_precision}: {st.get('error', 'not converged')} after "
+                    f"{st['wall']:.2f}s; again in fp32")
+                S.restore(start)
+                st2 = S.solve("fp32", True, args.tolerance, args.max_iterations, guess)
+                st2["wall"] += st["wall"]
+                st2["iter"] += st["iter"]
+                st2["fallback"] = st.get("error", "not converged")
+                st = st2
             st["wall"] += tg          # the warm start's inference counts in the time to tolerance
             stats[prec].append(st)
             sols[prec] = [f.numpy() for f in S.fields[:3]]
@@ -486,6 +509,10 @@ def main():
     iters = sum(s["iter"] for s in fast)
     value = w.dofs() * iters / elapsed / 1e9 if elapsed > 0 else 0.0
     t64 = sum(s["wall"] for s in ref)
+    # the time-to-tolerance ratio over the steps both forms finished
+    both = [k for k in range(done) if fast[k]["converged"] and ref[k]["converged"]]
+    t_fast_both = sum(fast[k]["wall"] for k in both)
+    t64_both = sum(ref[k]["wall"] for k in both)
 
     # roofline on the kernel that takes the largest share of the fast form's
     # timed device time: the return-map correction operator (yielding steps)
@@ -543,7 +570,9 @@ def main():
         "iterations": {"fp32_learned": [s["iter"] for s in fast], "fp64": [s["iter"] for s in ref]},
         "converged": {"fp32_learned": [bool(s["converged"]) for s in fast], "fp64": [bool(s["converged"]) for s in ref]},
         "err_abs": {"fp32_learned": [s["err_abs"] for s in fast], "fp64": [s["err_abs"] for s in ref]},
-        "speedup_vs_fp64": round(t64 / elapsed, 3) if elapsed > 0 else None,
+        "speedup_vs_fp64": round(t64_both / t_fast_both, 3) if t_fast_both > 0 else None,
+        "speedup_steps": both,
+        "fallbacks": {k: fast[k]["fallback"] for k in range(done) if "fallback" in fast[k]},
         "gdof_iter_per_s_fp64": round(w.dofs() * sum(s["iter"] for s in ref) / t64 / 1e9, 4) if t64 > 0 else None,
         "fp32_vs_fp64_max_rel_diff": diffs,
         "fast_precision": args.fast_precision,
@@ -580,8 +609,10 @@ def main():
     if model is not None and left() > 60:
         from femtools_amd import warmstart as WS
         try:
-            rows = WS.evaluate_gain(model, 256, [20_000_000], log=log)
-            out["warm_start_gain"] = {"n256": rows, "summary": WS.summarize_gain(rows)}
+            rows = WS.evaluate_gain(model, 256, [20_000_000], log=log, tolerance=1e-6)
+            out["warm_start_gain"] = {"n256_tol1e-6": rows, "summary": WS.summarize_gain(rows)}
+            rows8 = WS.evaluate_gain(model, 256, [20_000_000], log=log, tolerance=1e-8, max_iterations=200_000)
+            out["warm_start_gain"]["n256_tol1e-8"] = rows8
         except Exception as e:
             out["warm_start_gain"] = {"error": repr(e)}
     if args.batch > 0 and left() > 60:

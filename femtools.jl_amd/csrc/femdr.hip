@@ -417,7 +417,7 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         (r = dalloc_t(c, &s->d_Jrow, NSH * m->nels)) || (r = dalloc_t(c, &s->d_Jdiag, NSH * m->nels)) ||
         (r = dalloc_t(c, &s->d_bcflag, nv)) || (r = dalloc_t(c, &s->d_bcval_x, nv)) ||
         (r = dalloc_t(c, &s->d_bcval_y, nv)) || (r = dalloc_t(c, &s->d_ptab, PC_NCOL * MAXPH)) ||
-        (r = dalloc_t(c, &s->d_flag, 2)) || (r = dalloc_t(c, &s->d_cheb, 8))) {
+        (r = dalloc_t(c, &s->d_flag, 4)) || (r = dalloc_t(c, &s->d_cheb, 8))) {
         fem_stokes_destroy(s);
         return r;
     }
@@ -637,6 +637,7 @@ struct LaunchOpts {
     uint8_t* yield_flag = nullptr;    // with yield_count: near-yield flag per element
     double* phi_out = nullptr;        // nullable (FP64 residual launches): shear dissipation [nels][7]
     float* tangent_out = nullptr;     // nullable (FP64 plastic residual launches): return linearisations
+    int32_t* tangent_bad = nullptr;   // with tangent_out: count of points whose linearisation is not positive
     bool cap = false;                 // the tensile cap is the yield model (FP64, EF_GEN launches)
     bool commit = false;              // a cap launch that commits the plastic history
 };
@@ -844,6 +845,7 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
         if (d.lin) return set_err(s->ctx, FEM_EARG, "no FP64 correction operator");
         a.phi_out = o.phi_out;
         a.tangent_out = o.tangent_out;
+        a.tangent_bad = o.tangent_out ? o.tangent_bad : nullptr;
         if (o.cap) {
             a.cap = 1;
             a.gamma = s->f[FEM_FIELD_GAMMA_PL];
@@ -1619,13 +1621,16 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
     hipEvent_t timer_e1 = nullptr;
     // FP64 residual at x; with plasticity it counts the yielding elements
     auto residual64 = [&](const LaunchOpts& oo) -> int {
-        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
+        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 8, c->stream));
         CHK(launch_element(s, d64, oo));
         CHK(launch_gather(s, d64, oo));
         if (delta) {
-            HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(h_yield, d_yield, 8, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
-            d32.delta = *h_yield > 0;
+            d32.delta = h_yield[0] > 0;
+            // the linearised return where every yielding point's linearisation
+            // is positive, the two FP64 returns otherwise
+            d32.tangent = tangent && h_yield[1] == 0 ? w.tangent : nullptr;
         }
         return FEM_OK;
     };
@@ -1648,6 +1653,7 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
         oo.yield_count = delta ? d_yield : nullptr;
         oo.yield_flag = delta ? s->d_yflag : nullptr;
         oo.tangent_out = tangent ? w.tangent : nullptr;
+        oo.tangent_bad = tangent ? d_yield + 1 : nullptr;
         CHK(residual64(oo));
         double r3[3];
         CHK(reduce_outer(s, d64, r3));
@@ -1725,7 +1731,8 @@ static int solve_mixed(fem_stokes* s, const fem_solve_params* p, const LaunchOpt
                 CHK(seg(nullptr));
                 timer.mark(e2);
             } else if (graphs_on(c) && nseg > 1) {
-                const int64_t key = ((nseg * 2 + (lastp ? 1 : 0)) * (ncheck + 1) + mpos) * 2 + (d32.delta ? 1 : 0);
+                const int64_t key = (((nseg * 2 + (lastp ? 1 : 0)) * (ncheck + 1) + mpos) * 2 + (d32.delta ? 1 : 0)) * 2 +
+                                    (d32.tangent ? 1 : 0);
                 IntervalGraph* gph = nullptr;
                 for (auto& kv : graphs)
                     if (kv.first == key) gph = kv.second.get();

@@ -2211,7 +2211,7 @@ int hex27_mixed(fem_stokes* s, const fem_solve_params* p, const Opts3& o, bool f
     // plasticity the fresh yield count picks the operator
     bool dlt = delta;
     auto residual64 = [&]() -> int {
-        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 4, c->stream));
+        if (delta) HIPCHK(c, hipMemsetAsync(d_yield, 0, 8, c->stream));
         CHK(hex_element(s, oo));
         CHK(cast32(c, f[FEM_FIELD_RV_X], w.rhs_vx, nv));
         CHK(cast32(c, f[FEM_FIELD_RV_Y], w.rhs_vy, nv));

@@ -524,6 +524,11 @@ struct ElemArgs {
     // by EF_TANGENT launches at the points of hot elements (tangent)
     float* tangent_out;
     const float* tangent;
+    // nullable, with tangent_out: receives the number of yielding points whose
+    // linearisation may not be positive (the lower bound 1 - 3h - max(kq, 0)|q|^2
+    // of its eigenvalues below 0.05: yield in tension, F > τII), for which the
+    // caller keeps the two FP64 returns
+    int32_t* tangent_bad;
     // solve! array keywords (FP64 launches; nullable): nodal loads subtracted
     // from the momentum residual, with EF_GEN the viscosity at the points
     // [nels][NQ] and the previous-step pressure at the points [nels][NQ]
@@ -793,6 +798,7 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
     }
     F Rx[NV], Ry[NV];
     bool yel = false, near = false;
+    int32_t nbad = 0;   // points of this element with a non-positive return linearisation
 #pragma unroll
     for (int a = 0; a < NV; ++a) { Rx[a] = 0; Ry[a] = 0; }
     const bool grav = (A.gx != F(0)) || (A.gy != F(0));
@@ -974,6 +980,8 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
                     if (A.tangent_out) {
 #pragma unroll
                         for (int k = 0; k < NTAN; ++k) A.tangent_out[((int64_t)k * NQ + q) * ne + e] = tg[k];
+                        const float q2 = tg[3] * tg[3] + tg[4] * tg[4] + tg[5] * tg[5];
+                        nbad += (y && 1.0f - 3.0f * tg[0] - fmaxf(tg[1], 0.0f) * q2 < 0.05f) ? 1 : 0;
                     }
                 }
             }
@@ -1060,6 +1068,7 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
         if (mask && (threadIdx.x & 63) == __ffsll((unsigned long long)__ballot(1)) - 1)
             atomicAdd(A.yield_count, (int32_t)__popcll(mask));
         A.yield_flag[e] = near;
+        if (A.tangent_bad && nbad > 0) atomicAdd(A.tangent_bad, nbad);
     }
 
     // ---- shared nodes: element-slot residuals for the gather

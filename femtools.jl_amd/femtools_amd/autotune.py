@@ -25,16 +25,15 @@ import time
 from . import solve, solve_batch
 
 
-def _candidates(meshes, with_tiles=True):
+def _candidates(meshes, graphs=(1, 0)):
     """(tiles, tile_rows, graphs) combinations the meshes admit."""
     out = []
-    if with_tiles:
-        for rows in (4, 8, 16):
-            if all(m.dim == 2 and getattr(m, "nx", 0) and m.nx % 16 == 0 and m.ny % rows == 0 for m in meshes):
-                for graphs in (1, 0):
-                    out.append(dict(tiles=1, tile_rows=rows, graphs=graphs))
-    for graphs in (1, 0):
-        out.append(dict(tiles=0, tile_rows=8, graphs=graphs))
+    for rows in (4, 8, 16):
+        if all(m.dim == 2 and getattr(m, "nx", 0) and m.nx % 16 == 0 and m.ny % rows == 0 for m in meshes):
+            for g in graphs:
+                out.append(dict(tiles=1, tile_rows=rows, graphs=g))
+    for g in graphs:
+        out.append(dict(tiles=0, tile_rows=8, graphs=g))
     return out
 
 
@@ -47,11 +46,13 @@ def _restore(dr, snap):
         f.copyto(a)
 
 
-def tune_solve(dr, mesh, bcs, iterations=1000, fold_intervals=None, out=None, workload="", **solve_kwargs):
+def tune_solve(dr, mesh, bcs, iterations=1000, fold_intervals=None, out=None, workload="", graphs=(1, 0),
+               **solve_kwargs):
     """Times fixed-length solves of (dr, mesh, bcs) under every admissible
-    (tiles, tile_rows, graphs) and keeps the fastest per iteration; with
-    fold_intervals (a list), then times the solve to its tolerance under each
-    fold interval. Returns the JSON dict (also written to out)."""
+    (tiles, tile_rows, graphs in `graphs`) and keeps the fastest per
+    iteration; with fold_intervals (a list), then times the solve to its
+    tolerance under each fold interval. Returns the JSON dict (also written to
+    out)."""
     ctx = dr.ctx
     saved = ctx.tuning
     snap = _snapshot(dr)
@@ -60,7 +61,7 @@ def tune_solve(dr, mesh, bcs, iterations=1000, fold_intervals=None, out=None, wo
     kw.setdefault("precision", "fp32")
     meas = []
     best = None
-    for cand in _candidates([mesh]):
+    for cand in _candidates([mesh], graphs):
         ctx.set_tuning(**dict(saved, **cand))
         walls = []
         for _ in range(2):   # the first run instantiates the graphs
@@ -103,15 +104,16 @@ def tune_solve(dr, mesh, bcs, iterations=1000, fold_intervals=None, out=None, wo
     return res
 
 
-def tune_batch(problems, iterations=500, out=None, workload=""):
+def tune_batch(problems, iterations=500, out=None, workload="", graphs=(1, 0)):
     """Times one fixed-length solve_batch of the problems ((dr, mesh, bcs, kw)
-    tuples on one context) under every admissible (tiles, tile_rows, graphs)
-    and keeps the fastest. Returns the JSON dict (also written to out)."""
+    tuples on one context) under every admissible (tiles, tile_rows, graphs in
+    `graphs`) and keeps the fastest. Returns the JSON dict (also written to
+    out)."""
     ctx = problems[0][0].ctx
     saved = ctx.tuning
     snaps = [_snapshot(dr) for dr, *_ in problems]
     meas, best = [], None
-    for cand in _candidates([m for _, m, *_ in problems]):
+    for cand in _candidates([m for _, m, *_ in problems], graphs):
         ctx.set_tuning(**dict(saved, **cand))
         walls = []
         for _ in range(2):

@@ -316,7 +316,7 @@ def generate_dataset(seeds, n=GRID, tolerance=1e-6, max_iterations=100_000, batc
 
 
 def evaluate_gain(net, n, seeds, dt_advect=0.05, tolerance=1e-8, precision="fp32", max_iterations=400_000,
-                  log=None):
+                  log=None, **solver):
     """What the learned guess gains on n x n problems of the generator (seeds
     not used in training): each problem is the second step of a two-step
     sequence whose inclusions move with the first step's flow at their
@@ -332,6 +332,7 @@ def evaluate_gain(net, n, seeds, dt_advect=0.05, tolerance=1e-8, precision="fp32
     vbg = PB.pure_shear_field(mesh.coords)
     kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
               iterMax=max_iterations, max_ph_iterations=20_000, throw_on_failure=False, precision=precision)
+    kw.update(solver)   # solver keywords (pressure_factor, rel_drop0, check_interval, ...)
     out = []
     for s in seeds:
         circles, eta = random_inclusions(s)

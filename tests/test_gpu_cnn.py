@@ -140,3 +140,25 @@ def test_adam_steps_lower_the_loss():
     assert losses[-1] < 0.5 * losses[0]
     # the padded outputs of the head stay zero
     assert np.all(net.get_params()[-1][0][3:] == 0)
+
+
+def test_trained_warm_start_gains_over_the_zero_guess():
+    # the committed weights on unseen problems of the training generator (two
+    # seeds outside the training and validation ranges), 128 x 128 cells,
+    # viscous, solved in FP32 bulk from (i) the boundary values only and (ii)
+    # the network's prediction: the prediction must converge and take fewer
+    # iterations in total; the previous-step guess is printed. Tolerance 1e-6:
+    # at 1e-8 the reference's outer loop stalls on these problems from every
+    # guess (pressure residual cycling about 1e-5 for thousands of outer
+    # iterations once the inner exit test reaches its floor)
+    from femtools_amd import warmstart as WS
+    net = WS.load_default()
+    rows = WS.evaluate_gain(net, 128, [20_000_101, 20_000_202], tolerance=1e-6)
+    for r in rows:
+        print({k: (v if not isinstance(v, dict) else (v["iterations"], v["converged"])) for k, v in r.items()},
+              flush=True)
+    assert all(r["learned"]["converged"] for r in rows)
+    zero = sum(r["zero"]["iterations"] for r in rows)
+    learned = sum(r["learned"]["iterations"] for r in rows)
+    print(f"iterations: zero {zero}, learned {learned} ({learned / zero:.3f})", flush=True)
+    assert learned < zero
