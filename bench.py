@@ -102,7 +102,7 @@ def read_traffic(kernel_prefix):
         if "traffic" not in fn or not fn.endswith(".txt"):
             continue
         for line in open(os.path.join(d, fn)):
-            m = re.match(r"\s*(\S+)\s+traffic_bytes\s+([0-9.eE+]+)", line)
+            m = re.match(r"\s*(.+?)\s+traffic_bytes\s+([0-9.eE+]+)", line)
             if m and m.group(1).startswith(kernel_prefix):
                 return float(m.group(2))
     return None
@@ -389,7 +389,7 @@ def main():
                     help="the fast form's solve precision: FP32 corrections with two FP64 returns per point near "
                          "yield, or with the return linearised about the FP64 state at every fold")
     ap.add_argument("--eta-reg", type=float, default=None, help="Perzyna viscosity (default problems.VEP_ETA_REG)")
-    ap.add_argument("--deadline-s", type=float, default=330.0,
+    ap.add_argument("--deadline-s", type=float, default=470.0,
                     help="no new series step starts when it could end after this many seconds of the run")
     ap.add_argument("--end-s", type=float, default=560.0, help="the measurements after the series fit in this")
     ap.add_argument("--hex27", type=int, default=64, help="cells per side of the 3-D time-to-tolerance pair (0: skip)")
@@ -399,7 +399,7 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="problems of the batched measurement (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="fast form without the learned warm start")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--extras", type=int, default=1, help="measure the 3-D and 64^2 rows when time is left")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -531,17 +531,17 @@ def main():
                  "not counted)")
         kbytes = algorithmic_bytes_fp32_tile(w, nv, ne, npp, nt6)
         avg_s = ms_rm / n_rm * 1e-3
-        traffic = read_traffic("k_t7_element<float,tangent")
+        traffic = read_traffic("k_t7_element<float, 1006, 256>")
     elif n_rm > 0 and tot_rm >= tot_ve:
         kname = "k_t7_element<float,EF_DELTA tile> (FP32 return-map correction operator, 16x8-cell tiles)"
         kbytes = algorithmic_bytes_fp32_rm(w, nv, ne, npp)
         avg_s = ms_rm / n_rm * 1e-3
-        traffic = read_traffic("k_t7_element<float,rm")
+        traffic = read_traffic("k_t7_element<float, 495, 256>")
     else:
         kname = "k_t7_element<float,EF_TILE> (FP32 visco-elastic correction operator, 16x8-cell tiles)"
         kbytes = algorithmic_bytes_fp32_tile(w, nv, ne, npp, nt6)
         avg_s = ms_ve / max(n_ve, 1) * 1e-3
-        traffic = read_traffic("k_t7_element<float,364")
+        traffic = read_traffic("k_t7_element<float, 364, 256>")
     achieved = kbytes / avg_s / 1e9 if avg_s > 0 else None
     e64, it64 = algorithmic_bytes(w, nv, ne, npp)
     n64 = max(1, sum(s["timed_launches"] for s in ref))
@@ -589,10 +589,17 @@ def main():
                                   "algorithmic_bytes_per_launch": e64, "avg_launch_us": round(avg64 * 1e6, 2),
                                   "achieved_GBs": round(e64 / avg64 / 1e9, 1) if avg64 > 0 else None,
                                   "frac": round(e64 / avg64 / 1e9 / HBM_PEAK_GBS, 4) if avg64 > 0 else None,
-                                  "traffic": read_traffic("k_t7_element<double>")},
+                                  "traffic": read_traffic("k_t7_element<double, 47, 128>")},
     }
     if errors:
         out["errors"] = errors
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    log("cpu baseline done")
+
     def left():
         return args.end_s - (time.perf_counter() - T_START)
 
@@ -606,13 +613,11 @@ def main():
             out["hex27"] = hex27_measure(args.hex27)
         except Exception as e:   # reported, never required
             out["hex27"] = {"error": repr(e)}
-    if model is not None and left() > 60:
+    if args.extras and model is not None and left() > 60:
         from femtools_amd import warmstart as WS
         try:
-            rows = WS.evaluate_gain(model, 256, [20_000_000], log=log, tolerance=1e-6)
-            out["warm_start_gain"] = {"n256_tol1e-6": rows, "summary": WS.summarize_gain(rows)}
-            rows8 = WS.evaluate_gain(model, 256, [20_000_000], log=log, tolerance=1e-8, max_iterations=200_000)
-            out["warm_start_gain"]["n256_tol1e-8"] = rows8
+            rows = WS.evaluate_gain(model, 256, [1, 2], log=log, tolerance=1e-8)
+            out["warm_start_gain"] = {"n256_steps01": rows, "summary": WS.summarize_gain(rows)}
         except Exception as e:
             out["warm_start_gain"] = {"error": repr(e)}
     if args.batch > 0 and left() > 60:
@@ -620,11 +625,6 @@ def main():
             out["batched"] = batch_measure(args.batch)
         except Exception as e:
             out["batched"] = {"error": repr(e)}
-    if not args.no_cpu_baseline:
-        try:
-            out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
-        except Exception as e:  # the baseline is reported, never required
-            out["cpu_baseline"] = {"value": None, "error": repr(e)}
     log("done")
     print(json.dumps(out), flush=True)
 

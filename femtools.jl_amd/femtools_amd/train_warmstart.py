@@ -11,8 +11,9 @@
 2. Training: the device network's own forward/backward kernels, Adam, cosine
    learning-rate decay, mean-squared error on the scaled targets.
 3. Output: BF16 weights and FP32 biases (.npz) and a JSON summary on stdout,
-   with the iterations and wall time to 1e-8 from the zero, previous-step and
-   learned guesses on unseen problems (warmstart.evaluate_gain).
+   with the iterations and wall time to 1e-8 of the first two steps of the
+   visco-elasto-plastic series generator from the zero, pure-shear,
+   previous-step and learned guesses (warmstart.evaluate_gain).
 """
 from __future__ import annotations
 

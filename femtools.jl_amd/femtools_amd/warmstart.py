@@ -315,81 +315,110 @@ def generate_dataset(seeds, n=GRID, tolerance=1e-6, max_iterations=100_000, batc
     return X, T, stats
 
 
-def evaluate_gain(net, n, seeds, dt_advect=0.05, tolerance=1e-8, precision="fp32", max_iterations=400_000,
-                  log=None, **solver):
-    """What the learned guess gains on n x n problems of the generator (seeds
-    not used in training): each problem is the second step of a two-step
-    sequence whose inclusions move with the first step's flow at their
-    centres for dt_advect, so its previous-step solution exists. The step is
-    solved to `tolerance` from (i) the zero guess (boundary values only),
-    (ii) the previous step's solution and (iii) the network's prediction
-    (inference and resampling included in its time)."""
+def _vep_series_state(fa, PB, mesh, n, seed, eta_reg=None):
+    """Step-0 state of the visco-elasto-plastic series generator
+    (problems.vep_inclusions) on the given mesh: the state, the Drucker-Prager
+    model, the workload, its phases and the boundary conditions."""
+    w = PB.vep_inclusions(n, seed=seed, **({} if eta_reg is None else {"eta_reg": eta_reg}))
+    ph = PB.inclusion_phases(mesh.coords, mesh.el2n, w.circles)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g),
+                     CFL_v=w.CFL_v)
+    dr.set_phases("v", ph)
+    dr.set_phases("P", ph)
+    dp = fa.DruckerPrager(w.plastic["phi"], w.plastic["psi"], w.plastic["C"], w.plastic["eta_reg"],
+                          w.plastic["Kb"])
+    return dr, dp, w, ph
+
+
+def vep_guess(net, mesh, w, ph, v_bg):
+    """The network's initial (vx, vy, P) for a step of the VEP series: the
+    prediction for the step's visco-elastic viscosity field 1/(1/eta + 1/(G dt)),
+    with the column's hydrostatic pressure."""
+    eta_ve = [1.0 / (1.0 / e + 1.0 / (g * w.dt)) for e, g in zip(w.eta, w.G)]
+    return predict_on_mesh(net, mesh, ph, eta_ve, v_bg, edot=1.0, eta_scale=eta_ve[0], rho=w.rho0, g=w.g)
+
+
+def evaluate_gain(net, n, seeds, tolerance=1e-8, precision="fp32-tangent", max_iterations=600_000, log=None):
+    """What the learned guess gains on n x n problems of the VEP series
+    generator (problems.vep_inclusions with the given seeds; the network was
+    trained on viscous problems only). Step 0 is solved to `tolerance` from
+    (i) zero velocity (boundary values only), (ii) the background pure shear
+    the reference's VEP examples start from and (iii) the network's
+    prediction; step 1 (after the stress-history hand-off of step 0's
+    solution) from (iv) the previous step's solution, as the reference's time
+    loops do, and (v) the prediction. Inference and resampling count in the
+    learned rows' time."""
     import femtools_amd as fa
     from . import problems as PB
     mesh = fa.t7_box_mesh(n, n, ctx=net.ctx)
     (bx, vx), (by, vy) = PB.pure_shear_bc(mesh.coords)
     bcs = (fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy))
     vbg = PB.pure_shear_field(mesh.coords)
-    kw = dict(dt=1.0, tolerance=tolerance, check_interval=50, max_iterations=max_iterations,
-              iterMax=max_iterations, max_ph_iterations=20_000, throw_on_failure=False, precision=precision)
-    kw.update(solver)   # solver keywords (pressure_factor, rel_drop0, check_interval, ...)
+    zero = np.zeros(mesh.nnodes)
     out = []
     for s in seeds:
-        circles, eta = random_inclusions(s)
-        dr, ph = _viscous_state(fa, mesh, circles, eta)
-        dr.v.x.copyto(vbg[0])
-        dr.v.y.copyto(vbg[1])
-        fa.solve(dr, mesh, *bcs, **kw)
-        prev = (dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy())
-        # advect the inclusion centres with the first step's velocity
-        moved = []
-        for cx, cy, r in circles:
-            k = int(np.argmin((mesh.coords[:, 0] - cx) ** 2 + (mesh.coords[:, 1] - cy) ** 2))
-            moved.append((cx + dt_advect * prev[0][k], cy + dt_advect * prev[1][k], r))
-        row = {"seed": s, "inclusions": len(circles)}
-        for name in ("zero", "previous_step", "learned"):
-            dr2, ph2 = _viscous_state(fa, mesh, moved, eta)
+        dr, dp, w, ph = _vep_series_state(fa, PB, mesh, n, s)
+        kw = dict(w.solve_kwargs(), dt=w.dt, plastic=dp, tolerance=tolerance, max_iterations=max_iterations,
+                  iterMax=max_iterations, throw_on_failure=False, precision=precision)
+        row = {"seed": s, "n": n}
+
+        def run(name, guess, P_old, extra_s=0.0):
+            dr.v.x.copyto(guess[0])
+            dr.v.y.copyto(guess[1])
+            dr.P.copyto(guess[2])
+            dr.P0.copyto(P_old)
+            net.ctx.sync()
             t0 = time.perf_counter()
-            if name == "zero":
-                z = np.zeros(mesh.nnodes)
-                dr2.v.x.copyto(z)
-                dr2.v.y.copyto(z)
-            elif name == "previous_step":
-                dr2.v.x.copyto(prev[0])
-                dr2.v.y.copyto(prev[1])
-                dr2.P.copyto(prev[2])
-            else:
-                gx, gy, gP = predict_on_mesh(net, mesh, ph2, eta, vbg)
-                dr2.v.x.copyto(gx)
-                dr2.v.y.copyto(gy)
-                dr2.P.copyto(gP)
             try:
-                st = fa.solve(dr2, mesh, *bcs, **kw)
+                st = fa.solve(dr, mesh, *bcs, **kw)
                 net.ctx.sync()
-                row[name] = {"iterations": st["iter"], "seconds": round(time.perf_counter() - t0, 4),
+                row[name] = {"iterations": st["iter"], "seconds": round(time.perf_counter() - t0 + extra_s, 4),
                              "converged": bool(st["converged"])}
             except fa.FemError as e:   # a diverging solve is recorded, not raised
-                row[name] = {"iterations": None, "seconds": round(time.perf_counter() - t0, 4),
+                row[name] = {"iterations": None, "seconds": round(time.perf_counter() - t0 + extra_s, 4),
                              "converged": False, "error": str(e)}
+
+        zP = np.zeros(mesh.nnodesP)
+        run("zero", (zero, zero, zP), zP)
+        t0 = time.perf_counter()
+        g0 = vep_guess(net, mesh, w, ph, vbg)
+        tg = time.perf_counter() - t0
+        run("learned", g0, zP, tg)
+        run("pure_shear", (vbg[0], vbg[1], zP), zP)
+        # step 1 from step 0's (pure-shear start) solution and stress
+        s1 = (dr.v.x.numpy(), dr.v.y.numpy(), dr.P.numpy())
+        fa.update_stokes_current_stress(dr, mesh, w.dt, plastic=dp)
+        for c in ("xx", "yy", "xy"):
+            fa.copyto(getattr(dr.tau_old, c), getattr(dr.tau, c))
+        run("step1_previous_step", s1, s1[2])
+        t0 = time.perf_counter()
+        g1 = vep_guess(net, mesh, w, ph, vbg)
+        tg = time.perf_counter() - t0
+        run("step1_learned", g1, s1[2], tg)
         out.append(row)
         if log:
             log(f"gain n={n} seed {s}: " + ", ".join(f"{k} {row[k]['iterations']} it {row[k]['seconds']:.2f}s"
-                                                    for k in ("zero", "previous_step", "learned")))
+                                                    for k in row if isinstance(row[k], dict)))
     return out
 
 
 def summarize_gain(rows):
-    """Mean iteration and wall-time ratios of the learned guess to the zero
-    and previous-step guesses over the rows where all three converged."""
-    ok = [r for r in rows if all(r[k]["converged"] for k in ("zero", "previous_step", "learned"))]
-    if not ok:
-        return {"problems": len(rows), "all_converged": 0}
-    f = lambda a, b, key: float(np.mean([r[a][key] / r[b][key] for r in ok]))
-    return {"problems": len(rows), "all_converged": len(ok),
-            "iterations_learned_over_zero": round(f("learned", "zero", "iterations"), 3),
-            "iterations_learned_over_previous": round(f("learned", "previous_step", "iterations"), 3),
-            "time_learned_over_zero": round(f("learned", "zero", "seconds"), 3),
-            "time_learned_over_previous": round(f("learned", "previous_step", "seconds"), 3)}
+    """Total iterations and wall time of each start over the rows, and the
+    learned guess's ratios to the zero, pure-shear and previous-step starts
+    over the rows where the compared starts all converged."""
+    keys = ("zero", "pure_shear", "learned", "step1_previous_step", "step1_learned")
+    out = {"problems": len(rows)}
+    for k in keys:
+        out[k] = {"iterations": sum(r[k]["iterations"] or 0 for r in rows),
+                  "seconds": round(sum(r[k]["seconds"] for r in rows), 3),
+                  "converged": sum(bool(r[k]["converged"]) for r in rows)}
+    for a, b in (("learned", "zero"), ("learned", "pure_shear"), ("step1_learned", "step1_previous_step")):
+        ok = [r for r in rows if r[a]["converged"] and r[b]["converged"]]
+        if ok:
+            out[f"{a}_over_{b}"] = {
+                "iterations": round(sum(r[a]["iterations"] for r in ok) / sum(r[b]["iterations"] for r in ok), 3),
+                "seconds": round(sum(r[a]["seconds"] for r in ok) / sum(r[b]["seconds"] for r in ok), 3)}
+    return out
 
 
 def train(net: Network, X, T, epochs=30, batch=16, lr=1e-3, seed=0, log=None, val=None):
