@@ -401,6 +401,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--extras", type=int, default=1, help="measure the 3-D and 64^2 rows when time is left")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch choices instead of the autotuner, e.g. tiles=1,tile_rows=8,graphs=0 (profiling "
+                         "runs: counter collection slows the kernels it samples and would mislead the autotuner)")
+    ap.add_argument("--advance", type=int, default=0,
+                    help="untimed FP64 series steps before the timed ones (to profile the yielding steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a single solve would shard by element "
@@ -415,7 +420,8 @@ def main():
     if args.no_graphs:
         fa.default_context().set_tuning(graphs=0)
     eta_reg = PB.VEP_ETA_REG if args.eta_reg is None else args.eta_reg
-    steps = max(1, min(args.steps, SERIES_STEPS))
+    args.advance = max(0, min(args.advance, SERIES_STEPS - 1))
+    steps = max(1, min(args.steps, SERIES_STEPS - args.advance))
     if steps != args.steps:
         log(f"--steps {args.steps}: the series has {SERIES_STEPS} physical steps; timing {steps}")
     S = Series(args.n, eta_reg)
@@ -423,8 +429,13 @@ def main():
     from femtools_amd import autotune as AT
     kw_t = S.w.solve_kwargs()
     kw_t.pop("tolerance", None)
-    tuned = AT.tune_solve(S.dr, S.mesh, S.bcs, iterations=1500, workload=f"{args.n}x{args.n} VEP step 0",
-                          graphs=GRAPHS, dt=S.w.dt, plastic=S.dp, precision=args.fast_precision, **kw_t)
+    if args.tuning:
+        fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        S.ctx.set_tuning(**fixed)
+        tuned = {"tuning": S.ctx.tuning, "measurements": "fixed by --tuning"}
+    else:
+        tuned = AT.tune_solve(S.dr, S.mesh, S.bcs, iterations=1500, workload=f"{args.n}x{args.n} VEP step 0",
+                              graphs=GRAPHS, dt=S.w.dt, plastic=S.dp, precision=args.fast_precision, **kw_t)
     log(f"tuning: {tuned['tuning']}")
     w, mesh = S.w, S.mesh
     nv, ne, npp = mesh.nnodes, mesh.nels, mesh.nnodesP
@@ -449,13 +460,17 @@ def main():
         if model is not None:
             learned_guess(S, model)
     S.restore(s0)
+    for i in range(args.advance):
+        st = S.solve("fp64", False, args.tolerance, 600_000)
+        log(f"advance: step {i} fp64 iter {st['iter']} converged {st['converged']} {st['wall']:.2f}s (untimed)")
+        S.advance()
 
     stats = {"fp64": [], "fp32": []}
     diffs, errors, guess_s = [], [], []
     # a step costs at most max_iterations of each solver (~160 us per FP64 +
     # FP32 iteration pair at 512^2); no step starts that could end past the deadline
     worst = args.max_iterations * 160e-6 * (w.nx * w.ny) / 512 ** 2
-    for i in range(steps):
+    for i in range(args.advance, args.advance + steps):
         if time.perf_counter() - T_START + worst > args.deadline_s:
             log(f"deadline {args.deadline_s:.0f}s: the series stops after {i} steps")
             break
