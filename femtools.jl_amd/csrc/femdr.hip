@@ -1870,13 +1870,7 @@ __global__ __launch_bounds__(128) void k_rotate_stress2(int64_t ne, const int32_
     }
 }
 
-// The thermal half of a coupled solve (solve_coupled!, DR.jl:81-131)
-struct CoupledSide {
-    ThermalCoupling tc;
-    const fem_bc* bc_T = nullptr;
-    bool shear = false;
-    double* d_phi = nullptr;   // [nels][7] shear dissipation, with shear heating
-};
+
 
 // _transfer_temperature_kernel! (DR.jl:44-47): T_P[DoFsP[i]] = T[el2nP[i]];
 // the pressure DoFs 3e + a of element e sit at its vertex nodes a = 0..2
@@ -1932,7 +1926,11 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     if (s->in.history && (p->precision != 0 || s->dim != 2))
         return set_err(c, FEM_EARG, "collect_history is recorded by 2-D FP64 solves (precision 0)");
     if (s->in.history_len) *s->in.history_len = 0;
-    if (s->dim == 3) return hex27_solve(s, bcs, dp, p, st);
+    if (s->dim == 3) {
+        if (cp && p->precision != 0)
+            return set_err(c, FEM_EARG, "coupled solves run in FP64 (precision 0), got precision %d", p->precision);
+        return hex27_solve(s, bcs, dp, p, st, cp);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
     const int64_t nv = m->nnodes, np = m->nnodesP;
@@ -2900,7 +2898,10 @@ int fem_stokes_shear_heating(fem_stokes* s, const fem_drucker_prager* dp, double
     if (!s || !phi) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    if (s->dim != 2) return set_err(c, FEM_EARG, "shear heating is assembled for 2-D states");
+    if (s->dim == 3) {
+        CHK(hex27_shear_heating(s, dp, dt, phi));
+        return fem_sync(c);
+    }
     LaunchOpts o;
     CHK(prepare(s, dp, dt, 0, o, true));
     CHK(shear_heating_launch(s, o, phi));
@@ -2908,7 +2909,7 @@ int fem_stokes_shear_heating(fem_stokes* s, const fem_drucker_prager* dp, double
 }
 
 // solve_coupled! (DR.jl:81-131) over the coupled loop of _solve_stokes_dyrel!
-int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const fem_bc bc_v[2],
+int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const fem_bc* bc_v,
                       const fem_drucker_prager* dp, const fem_solve_params* p, const fem_coupled_params* cp,
                       fem_solve_stats* st, fem_coupled_stats* cst) {
     if (!th || !s || !bc_v || !p || !st || !cst) return FEM_EARG;
@@ -2918,8 +2919,8 @@ int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const 
     cst->err_T = INFINITY;
     if (thermal_ctx(th) != c)
         return set_err(c, FEM_EARG, "the thermal and Stokes states must share one context (stream)");
-    if (s->dim != 2) return set_err(c, FEM_EARG, "coupled solves take 2-D states");
     fem_mesh* tm = thermal_mesh(th);
+    if (tm->dim != s->dim) return set_err(c, FEM_EDIM, "the thermal and Stokes meshes must have the same dimension");
     if (tm->nnodes != s->mesh->nnodes || tm->nels != s->mesh->nels)
         return set_err(c, FEM_EDIM, "thermal mesh nodes must match the Stokes velocity-node layout");
     if (p->precision != 0)
@@ -2933,7 +2934,9 @@ int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const 
     side.bc_T = bc_T;
     side.shear = cp && cp->shear_heating;
     if (side.shear) {
-        CHK(dalloc_t(c, &side.d_phi, NV * s->mesh->nels));
+        // Φ at the velocity quadrature points (7 per T7, 27 per Hex27), the
+        // thermal element's own points
+        CHK(dalloc_t(c, &side.d_phi, (int64_t)s->mesh->nloc * s->mesh->nels));
         side.tc.phi = side.d_phi;
     }
     const int rc = solve_2d(s, bc_v, dp, p, st, &side);

@@ -93,6 +93,12 @@ struct Hex27Args {
     // or near the yield surface; the number of elements with a yielding point
     uint8_t* yield_flag;
     int32_t* yield_count;
+    // nullable (a finite thermal expansivity): the temperature and its
+    // previous-step value at the pressure DoFs, [nels][4]
+    const double *T, *T0;
+    // nullable, residual evaluations only: the shear dissipation
+    // τ : (ε̇ − (τ − τ_old)/(2GΔt)) per point, [nels][27]
+    double* phi_out;
 };
 
 // global node of element e at tensor position t = i + 3 j + 9 k
@@ -389,9 +395,22 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
             for (int i = 0; i < HNP; ++i) bq += NP[i] * bulk;
             dPdt = dP / (bq * A.dt);
         }
+        // the thermal term α (T − T0) / Δt and the temperature of the density
+        // (pressure_residual.jl:132-158, momentum_residuals.jl:11-16), both
+        // interpolated with the pressure modes
+        double dTdt = 0, Tq = 0;
+        if (A.T) {
+            double dT = 0;
+#pragma unroll
+            for (int i = 0; i < HNP; ++i) {
+                dT += NP[i] * (A.T[e * HNP + i] - A.T0[e * HNP + i]);
+                Tq += NP[i] * A.T[e * HNP + i];
+            }
+            dTdt = A.ptab[PC_ALPHA * MAXPH + pc] * dT / A.dt;
+        }
         double RP[HNP];
 #pragma unroll
-        for (int i = 0; i < HNP; ++i) RP[i] = group_sum(h.l < HNQ ? NP[i] * ((-div) - dPdt) * dOm : 0.0);
+        for (int i = 0; i < HNP; ++i) RP[i] = group_sum(h.l < HNQ ? NP[i] * (((-div) - dPdt) + dTdt) * dOm : 0.0);
         if (ITER) {
 #pragma unroll
             for (int i = 0; i < HNP; ++i) Pn[i] = A.pscale[e * HNP + i] * RP[i];
@@ -431,8 +450,19 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
         const double Ptot = Pq + Pnq;
         const double rho0 = A.ptab[PC_RHO0 * MAXPH + pc], al = A.ptab[PC_ALPHA * MAXPH + pc],
                      be = A.ptab[PC_BETA * MAXPH + pc];
-        // eos_density (momentum_residuals.jl:11-16) with T = 0
-        const double rho = rho0 * ((1 - al * (0.0 - A.Tref)) + be * Pq);
+        // eos_density (momentum_residuals.jl:11-16); T = 0 without a temperature field
+        const double rho = rho0 * ((1 - al * (Tq - A.Tref)) + be * Pq);
+        if (A.phi_out && h.active) {
+            // shear_dissipation, 3-D (shear_heating.jl:91-103): the stress power
+            // less the elastic power, shear terms counted twice
+            const double power = tau[0] * gv[0][0] + tau[1] * gv[1][1] + tau[2] * gv[2][2] +
+                                 tau[3] * (gv[0][1] + gv[1][0]) + tau[4] * (gv[0][2] + gv[2][0]) +
+                                 tau[5] * (gv[1][2] + gv[2][1]);
+            const double elastic = (tau[0] * (tau[0] - to[0]) + tau[1] * (tau[1] - to[1]) + tau[2] * (tau[2] - to[2]) +
+                                    2 * (tau[3] * (tau[3] - to[3]) + tau[4] * (tau[4] - to[4]) +
+                                         tau[5] * (tau[5] - to[5]))) * m.inv2Gdt;
+            A.phi_out[e * HNQ + q] = power - elastic;
+        }
         if (h.l < HNQ) {
             double* f = s_f[h.es] + q * 8;
             f[0] = (tau[0] - Ptot) * dOm; f[1] = (tau[1] - Ptot) * dOm; f[2] = (tau[2] - Ptot) * dOm;
@@ -982,6 +1012,8 @@ __global__ __launch_bounds__(RED_BLOCK) void k_red3_lmax(int64_t n, Red3 r, doub
 // ---- host side
 struct Opts3 {
     bool elastic = false, plastic = false, bulk = false;
+    bool thermal = false;            // a finite expansivity: the T, T0 fields enter the residual
+    double* phi_out = nullptr;       // residual evaluations: the shear dissipation per point
     double dt = 1;
     bool use_pnum = false, do_update = false, store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false,
          store_tau = false;
@@ -1043,6 +1075,11 @@ int hex_element(fem_stokes* s, const Opts3& o, bool gather = true) {
         a.yield_flag = o.yield_flag;
         a.yield_count = o.yield_count;
     }
+    if (o.thermal) {
+        a.T = f[FEM_FIELD_T];
+        a.T0 = f[FEM_FIELD_T0];
+    }
+    a.phi_out = o.use_pnum ? nullptr : o.phi_out;
     const int key = (o.elastic ? 1 : 0) | (o.plastic ? 2 : 0) | (o.bulk ? 4 : 0) | (o.use_pnum ? 8 : 0);
     dispatch_h27<15>(key, s, a);
     CHK(check3(s->ctx, "k_hex27_element"));
@@ -1151,7 +1188,7 @@ int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_
         return set_err(s->ctx, FEM_EARG, "the 3-D path takes one phase per cell (a 1 x nels phase matrix)");
     CHK(check_solve_inputs(s, dp != nullptr));
     CHK(upload_ptab(s, dp, dt, finite_K, &bulk_inf, &thermal, &eos_p, &elastic));
-    (void)thermal;   // no temperature field in 3-D: ρ = ρ0 (1 + α Tref + β P)
+    o.thermal = thermal;
     o.plastic = dp != nullptr;
     o.elastic = elastic && use_tau_old;
     o.bulk = !bulk_inf;
@@ -2352,13 +2389,56 @@ int hex27_mixed(fem_stokes* s, const fem_solve_params* p, const Opts3& o, bool f
 }  // namespace
 
 // solve! + _solve_stokes_dyrel! (DR.jl:302-337, 348-802) with D = 3
+// _transfer_temperature_kernel! (DR.jl:44-47) on a Hex27 box: the four
+// pressure DoFs of element e sit at its centre and its +x, +y and +z face
+// centres (mixed_mesh.jl:399-417, local nodes 27, 23, 24, 26), tensor
+// positions 13, 14, 16 and 22
+__global__ void k_transfer_T3(HexGeo g, const double* T, double* TP) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * g.nels) return;
+    const int64_t e = i >> 2;
+    const int a = (int)(i & 3);
+    const int t = a == 0 ? 13 : a == 1 ? 14 : a == 2 ? 16 : 22;
+    TP[i] = T[hex_node(g, e, t)];
+}
+
+int hex27_transfer_T(fem_stokes* s, const double* T, double* TP) {
+    const HexGeo g = hex_geo(s->mesh);
+    hipLaunchKernelGGL(k_transfer_T3, dim3(grid_for(4 * g.nels, 256)), dim3(256), 0, s->ctx->stream, g, T, TP);
+    return check3(s->ctx, "k_transfer_T3");
+}
+
+// the shear dissipation at the points: one residual evaluation (v, P without
+// Pnum, τ_old) that stores Φ only
+int hex27_shear_heating_launch(fem_stokes* s, const Opts3& o, double* phi) {
+    Opts3 oo = o;
+    oo.use_pnum = false;
+    oo.do_update = false;
+    oo.store_RP = oo.store_Pnum = oo.store_Rv = oo.store_Rv0 = oo.store_tau = false;
+    oo.yield_flag = nullptr;
+    oo.yield_count = nullptr;
+    oo.phi_out = phi;
+    return hex_element(s, oo, false);
+}
+
+int hex27_shear_heating(fem_stokes* s, const fem_drucker_prager* dp, double dt, double* phi) {
+    Opts3 o;
+    CHK(prepare3(s, dp, dt, 0, o, true));
+    return hex27_shear_heating_launch(s, o, phi);
+}
+
 int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
-                fem_solve_stats* st) {
+                fem_solve_stats* st, CoupledSide* cp) {
     fem_ctx* c = s->ctx;
     const auto t0 = std::chrono::steady_clock::now();
     fem_mesh* m = s->mesh;
     const int64_t nv = m->nnodes, np = m->nnodesP, ncheck = p->check_interval;
     const double dt = p->dt;
+    if (cp) {
+        // the thermal field on the pressure DoFs before the Stokes set-up (DR.jl:123-124)
+        CHK(hex27_transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T), s->f[FEM_FIELD_T]));
+        CHK(hex27_transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T0), s->f[FEM_FIELD_T0]));
+    }
     Opts3 o;
     CHK(prepare3(s, dp, dt, p->finite_K, o, p->use_tau_old != 0));
     const bool freeze = p->freeze_jacobian < 0 ? (dp == nullptr) : (p->freeze_jacobian != 0);
@@ -2389,6 +2469,12 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
     for (int k = 0; k < 3; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
     if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g, %g)\n", lmax[0], lmax[1], lmax[2]);
 
+    if (cp) {
+        // ∂T∂τ = R0 = 0 and the dissipation of the initial velocity (DR.jl:524-530)
+        CHK(thermal_coupling_begin(cp->tc, cp->bc_T));
+        if (cp->shear) CHK(hex27_shear_heating_launch(s, o, cp->d_phi));
+    }
+    auto thermal_ok = [&]() { return !cp || cp->tc.converged; };
     const double eps_tol = p->tolerance;
     double err_min = INFINITY, err = 2 * eps_tol, err_abs = INFINITY, err_rel = INFINITY;
     double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0, err_v = INFINITY, err_P = INFINITY;
@@ -2422,13 +2508,13 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
         if (p->verbose)
             printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e\n", (long long)itPH, (long long)iter, err,
                    err_abs, err_rel);
-        if (err < eps_tol) { converged = true; break; }
+        if (err < eps_tol && thermal_ok()) { converged = true; break; }
         if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
         err_min = std::min(err_min, err);
         const double eps_vel = err * rel_drop;
         int64_t itPT = 0;
         bool stop_total = false;
-        while (err > eps_vel && itPT <= p->iter_max) {
+        while ((err > eps_vel || !thermal_ok()) && itPT <= p->iter_max) {
             ++itPT;
             ++iter;
             const bool do_check = iter % ncheck == 0;
@@ -2446,6 +2532,13 @@ int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, 
             oi.store_Rv = last_possible;
             oi.store_Rv0 = ncheck > 1 && ((iter + 1) % ncheck == 0);
             for (int k = 0; k < 3; ++k) { oi.al[k] = al[k]; oi.be[k] = be[k]; }
+            if (cp) {
+                // one thermal iteration ahead of the Stokes one (DR.jl:604-661)
+                CHK(thermal_coupling_step(cp->tc));
+                CHK(hex27_transfer_T(s, thermal_field_ptr(cp->tc.t, FEM_TFIELD_T), s->f[FEM_FIELD_T]));
+                // refreshed after the thermal spectral estimate (DR.jl:659-661)
+                if (cp->shear && cp->tc.checked) CHK(hex27_shear_heating_launch(s, o, cp->d_phi));
+            }
             CHK(hex_element(s, oi));
             ++launches;
             if (do_check) {

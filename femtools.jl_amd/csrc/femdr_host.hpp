@@ -236,11 +236,14 @@ int hex27_pressure_scaling(fem_stokes* s, double gamma_fact, double dt, int fini
 int hex27_residual(fem_stokes* s, const fem_drucker_prager* dp, double dt, bool with_pnum, int finite_K);
 int hex27_jacobian(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_K);
 int hex27_update_current_stress(fem_stokes* s, const fem_drucker_prager* dp, double dt);
+struct CoupledSide;
+// with cp, the coupled loop of solve_coupled! (DR.jl:514-661) on a Hex27 state
 int hex27_solve(fem_stokes* s, const fem_bc* bcs, const fem_drucker_prager* dp, const fem_solve_params* p,
-                fem_solve_stats* st);
+                fem_solve_stats* st, CoupledSide* cp = nullptr);
 int hex27_time_element(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
 int hex27_time_lin(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32_t n, double* avg_ms);
 int hex27_rotate_stress(fem_stokes* s, double dt);
+int hex27_shear_heating(fem_stokes* s, const fem_drucker_prager* dp, double dt, double* phi);   // [nels][27]
 int hex27_postprocess(fem_stokes* s, double* out);
 
 // eps(x) of Julia (spacing at |x|) and the Chebyshev step/momentum (DR.jl:39-42)
@@ -287,11 +290,18 @@ struct ThermalCoupling {
     fem_thermal* t = nullptr;
     double dt = 1, Tref = 273;
     int64_t nout = 1;                // check cadence: the Stokes check_interval
-    const double* phi = nullptr;     // nullable: heat source at the velocity quadrature points [nels][7]
+    const double* phi = nullptr;     // nullable: heat source at the velocity quadrature points [nels][nq]
     int64_t iter = 0;
     double alpha = 0, beta = 0, nr0 = 0, err_T = INFINITY;
     bool converged = false;
     bool checked = false;            // the last iteration was a check
+};
+// The thermal half of a coupled solve (solve_coupled!, DR.jl:81-131)
+struct CoupledSide {
+    ThermalCoupling tc;
+    const fem_bc* bc_T = nullptr;
+    bool shear = false;
+    double* d_phi = nullptr;   // [nels][nq] shear dissipation, with shear heating
 };
 int thermal_coupling_begin(ThermalCoupling& tc, const fem_bc* bc);   // tables, Dirichlet set, ∂T∂τ = R0 = 0
 int thermal_coupling_step(ThermalCoupling& tc);

@@ -1348,18 +1348,20 @@ def thermal_residual(dr: ThermalDiffusionDR, dt, Tref=273.0, compute_jacobian=Fa
 # coupled thermal--Stokes (src/stokes/solvers/DR.jl:44-165, 514-661)
 # ---------------------------------------------------------------------------
 def assemble_shear_heating(dr: StokesDR, mesh: MixedMesh, dt, plastic=None) -> np.ndarray:
-    """assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
+    """assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-103):
     Φ = τ : (ε̇ − (τ − τ_old)/(2GΔt)) at the velocity quadrature points for the
-    current dr.v, dr.P and dr.tau_old, as an (nels, 7) array."""
+    current dr.v, dr.P and dr.tau_old, as an (nels, nq) array (nq = 7 on T7,
+    27 on Hex27)."""
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
     if _is_cap(plastic):
         raise ValueError("shear heating is assembled for DruckerPrager or no yield model")
     d, dp = _dp_ptr(plastic)
     buf = VP()
-    dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, mesh.nels * 7 * 8, C.byref(buf)))
+    nq = mesh.nq
+    dr.ctx.check(_lib.fem_buf_alloc(dr.ctx.handle, mesh.nels * nq * 8, C.byref(buf)))
     try:
-        phi = DeviceArray(dr.ctx, buf.value, (mesh.nels, 7))
+        phi = DeviceArray(dr.ctx, buf.value, (mesh.nels, nq))
         dr.ctx.check(_lib.fem_stokes_shear_heating(dr.handle, dp, float(dt), phi.ptr))
         return phi.numpy()
     finally:
@@ -1368,12 +1370,15 @@ def assemble_shear_heating(dr: StokesDR, mesh: MixedMesh, dt, plastic=None) -> n
 
 def solve_coupled(thermal: ThermalDiffusionDR, stokes: StokesDR, thermal_mesh: MixedMesh, stokes_mesh: MixedMesh,
                   bc_T: DirichletBoundaryCondition, bc_vx: DirichletBoundaryCondition,
-                  bc_vy: DirichletBoundaryCondition, *, dt, Tref=273.0, shear_heating=False, **kwargs):
-    """solve_coupled!(thermal, stokes, thermal_mesh, stokes_mesh, bc_T, bc_vx,
-    bc_vy; dt, Tref=273, shear_heating=false, kwargs...) (DR.jl:81-165): one
-    coupled step of size dt. Every inner Stokes iteration first advances one
-    thermal DR iteration and gathers the thermal field at the pressure DoFs'
-    vertex nodes into stokes.T. The remaining keywords are solve_stokes's
+                  bc_vy: DirichletBoundaryCondition, bc_vz: DirichletBoundaryCondition | None = None, *, dt,
+                  Tref=273.0, shear_heating=False, **kwargs):
+    """solve_coupled!(thermal, stokes, thermal_mesh, stokes_mesh, bc_T, bc_v;
+    dt, Tref=273, shear_heating=false, kwargs...) (DR.jl:81-165): one coupled
+    step of size dt, with two velocity conditions on a T7 mesh and three on a
+    Hex27 mesh. Every inner Stokes iteration first advances one thermal DR
+    iteration and gathers the thermal field at the velocity nodes that carry
+    the pressure DoFs (the triangle vertices; the hexahedron's centre and +x,
+    +y, +z face centres) into stokes.T. The remaining keywords are solve_stokes's
     (check_interval sets the check cadence of both relaxations). Returns the
     Stokes stats dict plus err_T and thermal_iterations; converged requires
     the thermal test (err_T < thermal.eps) too. thermal.source is unchanged;
@@ -1392,7 +1397,7 @@ def solve_coupled(thermal: ThermalDiffusionDR, stokes: StokesDR, thermal_mesh: M
     if _is_cap(kwargs.get("plastic")):
         raise ValueError("coupled solves take DruckerPrager or no yield model, not the tensile cap")
     keep = []
-    bcs, dp, p = _stokes_call(stokes, stokes_mesh, (bc_vx, bc_vy, None), keep, dt=dt, **kwargs)
+    bcs, dp, p = _stokes_call(stokes, stokes_mesh, (bc_vx, bc_vy, bc_vz), keep, dt=dt, **kwargs)
     bct = C.pointer(bc_T._c(keep)) if bc_T is not None else C.cast(None, _P(_BC))
     cp = _CoupledParams(Tref=float(Tref), shear_heating=int(bool(shear_heating)))
     st, cst = _SolveStats(), _CoupledStats()

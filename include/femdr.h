@@ -524,10 +524,10 @@ int fem_cell_stokes_material_gradient(fem_stokes* st, const double* const v_fwd[
                                       double* density_gradient, double* viscosity_gradient);
 
 /* ------------------------------------------------- coupled thermal--Stokes */
-/* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-46):
- * phi (device, nq x nels = [nels][7] f64) receives τ : (ε̇ − (τ − τ_old)/(2GΔt))
- * at the velocity quadrature points for the state's current v, P, τ_old.
- * 2-D states. */
+/* assemble_shear_heating! (src/stokes/assemblers/shear_heating.jl:18-103):
+ * phi (device, nq x nels f64: [nels][7] on T7, [nels][27] on Hex27) receives
+ * τ : (ε̇ − (τ − τ_old)/(2GΔt)) at the velocity quadrature points for the
+ * state's current v, P, τ_old (plane strain with τzz = −(τxx + τyy), or 3-D). */
 int fem_stokes_shear_heating(fem_stokes* st, const fem_drucker_prager* plastic, double dt, double* phi);
 
 /* solve_coupled! keywords beyond those of solve! (DR.jl:81-91) */
@@ -550,9 +550,10 @@ typedef struct {
  * inner Stokes iteration first advances one thermal DR iteration (tolerance
  * th's ϵ, check cadence params->check_interval) and transfers T again. Both
  * loop tests also wait for the thermal test; stats->converged requires it.
- * Both states must share one fem_ctx and a node layout; 2-D, precision 0.
- * The thermal source field is left unchanged. */
-int fem_coupled_solve(fem_thermal* th, fem_stokes* st, const fem_bc* bc_T, const fem_bc bc_v[2],
+ * Both states must share one fem_ctx and a node layout (a T7 or a Hex27
+ * mesh); bc_v holds one condition per velocity component (2 or 3);
+ * precision 0. The thermal source field is left unchanged. */
+int fem_coupled_solve(fem_thermal* th, fem_stokes* st, const fem_bc* bc_T, const fem_bc* bc_v,
                       const fem_drucker_prager* plastic, const fem_solve_params* params,
                       const fem_coupled_params* coupled, fem_solve_stats* stats, fem_coupled_stats* cstats);
 

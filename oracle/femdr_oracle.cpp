@@ -636,6 +636,7 @@ void orc_stokes_cheb(double dtau, double lmin, double c_fact, double* a, double*
 struct ThermalHook {
     orc_thermal* t = nullptr;
     orc_stokes* s = nullptr;
+    orc_stokes3* s3 = nullptr;       // a 3-D coupled solve (s is null)
     double dt = 1, Tref = 273, eps = 1e-6;
     int64_t nout = 1;
     bool shear = false;
@@ -1574,12 +1575,30 @@ void transfer_T(const orc_stokes* s, const double* T, double* TP) {
         for (int a = 0; a < 3; ++a) TP[e * 3 + a] = T[s->el2n[e * 7 + a]];
 }
 
+// Hex27: the four pressure DoFs sit at the cell centre and the +x, +y, +z
+// face centres, local nodes 27, 23, 24, 26 (mixed_mesh.jl:413-417)
+void transfer_T3(const orc_stokes3* s, const double* T, double* TP) {
+    const int rows[4] = {26, 22, 23, 25};
+    for (int64_t e = 0; e < s->nels; ++e)
+        for (int a = 0; a < 4; ++a) TP[e * 4 + a] = T[s->el2n[e * 27 + rows[a]]];
+}
+
+void hook_transfer(ThermalHook& h) {
+    if (h.s3) transfer_T3(h.s3, h.t->T, const_cast<double*>(h.s3->T));
+    else transfer_T(h.s, h.t->T, h.s->T);
+}
+
+void hook_shear(ThermalHook& h) {
+    if (h.s3) orc_shear_heating3(h.s3, h.dt, h.phi.data());
+    else orc_shear_heating(h.s, h.dt, h.phi.data());
+}
+
 // DR.jl:524-530: zero ∂T∂τ and R0, initial dissipation
 void thermal_hook_begin(ThermalHook& h) {
     orc_thermal* t = h.t;
     std::fill(t->rate, t->rate + t->nnodes, 0.0);
     std::fill(t->R0, t->R0 + t->nnodes, 0.0);
-    if (h.shear) orc_shear_heating(h.s, h.dt, h.phi.data());
+    if (h.shear) hook_shear(h);
 }
 
 // One thermal DR iteration of the coupled inner loop (DR.jl:604-661)
@@ -1596,7 +1615,7 @@ int thermal_hook_step(ThermalHook& h) {
     for (int64_t i = 0; i < n; ++i) t->rate[i] = t->R[i] / t->PC[i] + h.beta * t->rate[i];
     for (int64_t i = 0; i < n; ++i) t->T[i] += h.alpha * t->rate[i];
     dirichlet(t->T, t->nbc, t->bc_dofs, t->bc_vals);
-    transfer_T(h.s, t->T, h.s->T);
+    hook_transfer(h);
     if (chk) {
         const double nr = nrm2(t->R, n);
         if (h.iter == 1) h.nr0 = std::max(nr, julia_eps(nr));
@@ -1617,7 +1636,7 @@ int thermal_hook_step(ThermalHook& h) {
         stokes_cheb(dtau, lmin, t->c_fact, &h.alpha, &h.beta);
         h.converged = h.err_T < h.eps;
         // the dissipation is refreshed after the spectral estimate (DR.jl:659-661)
-        if (h.shear) orc_shear_heating(h.s, h.dt, h.phi.data());
+        if (h.shear) hook_shear(h);
     }
     return 0;
 }
@@ -1764,11 +1783,13 @@ double stress_ip3(const S gv[3][3], const double* Nv, int NV, const int* ph, con
 
 // 3-D element momentum residual (momentum_residuals.jl:209-258,
 // accumulate_momentum :28-33, eos_density :11-16). dNdx [nq][NV][3],
-// tau_old [6][nq], T = 0 (no thermal field on this path).
+// tau_old [6][nq]; T (nullable): the element's temperature at its pressure
+// DoFs, interpolated with the pressure basis into the density (T = 0 without)
 template <class S>
 void momentum_element3(int NV, int NP, int nq, const double* dNdx, const double* dOm, const double* Nq,
                        const double* NqP, const S* const v[3], const double* P, const S* Pnum, const int* ph,
-                       const Rheo& r, const double g[3], const double* tau_old, S* const R[3], double* tau_out) {
+                       const Rheo& r, const double g[3], const double* tau_old, S* const R[3], double* tau_out,
+                       const double* T = nullptr) {
     const orc_material* m = r.mat;
     double beta[ORC_MAX_PHASES];
     for (int p = 0; p < m->nphases; ++p) beta[p] = 1.0 / m->K[p];
@@ -1800,8 +1821,11 @@ void momentum_element3(int NV, int NP, int nq, const double* dNdx, const double*
             for (int i = 0; i < NP; ++i) pn = pn + Np[i] * Pnum[i];
             Ptot = Ptot + pn;
         }
+        double Tq = 0.0;
+        if (T)
+            for (int i = 0; i < NP; ++i) Tq += Np[i] * T[i];
         const double rho = interp_phase(Nv, NV, m->rho0, ph) *
-                           (1 - interp_phase(Nv, NV, m->alpha, ph) * (0.0 - m->Tref) + interp_phase(Nv, NV, beta, ph) * Pq);
+                           (1 - interp_phase(Nv, NV, m->alpha, ph) * (Tq - m->Tref) + interp_phase(Nv, NV, beta, ph) * Pq);
         // τ as a symmetric matrix; column i holds τ_ji
         const S* tm[3][3] = {{&tau[0], &tau[3], &tau[4]}, {&tau[3], &tau[1], &tau[5]}, {&tau[4], &tau[5], &tau[2]}};
         for (int i = 0; i < 3; ++i)
@@ -1817,11 +1841,14 @@ void momentum_element3(int NV, int NP, int nq, const double* dNdx, const double*
     }
 }
 
-// 3-D element continuity residual (pressure_residual.jl:132-158), T = 0, no Q
+// 3-D element continuity residual (pressure_residual.jl:132-158), no Q; with
+// T (the element's temperature at its pressure DoFs, nullable) the thermal term
+// α_q N^P·(T − T0)/Δt, α interpolated like the bulk modulus
 template <class S>
 void pressure_element3(int NV, int NP, int nq, const double* dNdx, const double* dOmP, const double* NqP,
                        const S* const v[3], const double* P, const double* P0, const int* phP, const double* bulk,
-                       int nphases, double dt, S* RP) {
+                       int nphases, double dt, S* RP, const double* alpha = nullptr, const double* T = nullptr,
+                       const double* T0 = nullptr) {
     bool all_inf = true;
     for (int p = 0; p < nphases; ++p) all_inf = all_inf && std::isinf(bulk[p]);
     for (int i = 0; i < NP; ++i) RP[i] = S(0.0);
@@ -1832,10 +1859,16 @@ void pressure_element3(int NV, int NP, int nq, const double* dNdx, const double*
         double dP = 0;
         for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
         const double dPdt = dP / (etabq * dt);
+        double dTdt = 0.0;
+        if (T) {
+            double dT = 0;
+            for (int i = 0; i < NP; ++i) dT += Nv[i] * (T[i] - T0[i]);
+            dTdt = interp_phase(Nv, NP, alpha, phP) * dT / dt;
+        }
         S div = S(0.0);
         for (int c = 0; c < 3; ++c)
             for (int a = 0; a < NV; ++a) div = div + G[a * 3 + c] * v[c][a];
-        for (int i = 0; i < NP; ++i) RP[i] = RP[i] + Nv[i] * (((-div) - dPdt) + 0.0 + 0.0) * dOmP[q];
+        for (int i = 0; i < NP; ++i) RP[i] = RP[i] + Nv[i] * (((-div) - dPdt) + dTdt + 0.0) * dOmP[q];
     }
 }
 
@@ -1995,7 +2028,8 @@ void orc_pressure_residual3(orc_stokes3* s, double dt, int32_t finite_K) {
         for (int a = 0; a < 4; ++a) ph[a] = s->phases[e] - 1;
         double RPe[4];
         pressure_element3<double>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.NqP[0][0], v, s->P + e * 4,
-                                  s->P0 + e * 4, ph, bulk, s->mat.nphases, dt, RPe);
+                                  s->P0 + e * 4, ph, bulk, s->mat.nphases, dt, RPe, s->mat.alpha,
+                                  s->T ? s->T + e * 4 : nullptr, s->T ? s->T0 + e * 4 : nullptr);
         for (int a = 0; a < 4; ++a) s->RP[e * 4 + a] = RPe[a];
     }
 }
@@ -2023,7 +2057,7 @@ static void momentum_residual3_impl(orc_stokes3* s, double dt, const double* Pnu
         double* R[3] = {Rl[0], Rl[1], Rl[2]};
         momentum_element3<double>(27, 4, 27, &dNdx[0][0][0], s->dOmega + e * 27, &t.Nq[0][0], &t.NqP[0][0], v,
                                   s->P + e * 4, Pnum ? Pnum + e * 4 : nullptr, ph, r, s->g, has_old ? to : nullptr, R,
-                                  tau_store ? tout : nullptr);
+                                  tau_store ? tout : nullptr, s->T ? s->T + e * 4 : nullptr);
         for (int c = 0; c < 3; ++c)
             for (int a = 0; a < 27; ++a) Re[c][e * 27 + a] = Rl[c][a];
         if (tau_store)
@@ -2145,8 +2179,9 @@ void orc_augmented_jacobian3(orc_stokes3* s, double dt, int32_t finite_K) {
     }
 }
 
-// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802), D = 3
-int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
+// solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802), D = 3; with
+// th, the coupled loop of solve_coupled! (DR.jl:514-661)
+static int stokes_solve3_impl(orc_stokes3* s, const orc_params* p, orc_stats* out, ThermalHook* th) {
     std::memset(out, 0, sizeof *out);
     g_err.clear();
     const double dt = p->dt;
@@ -2171,6 +2206,8 @@ int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
         dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
         stokes_cheb(dtau[c], 0.0, p->c_fact, &al[c], &be[c]);
     }
+    if (th) thermal_hook_begin(*th);
+    auto thermal_ok = [&]() { return !th || th->converged; };
     double err_min = kInf, eps_tol = p->tolerance, err = 2 * eps_tol;
     double err_abs = kInf, err_rel = kInf, err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
     double err_v = kInf, err_P = kInf;
@@ -2212,15 +2249,19 @@ int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
         if (p->verbose)
             printf("itPH = %02lld iter = %06lld err = %.3e abs = %.3e rel = %.3e\n", (long long)itPH, (long long)iter,
                    err, err_abs, err_rel);
-        if (err < eps_tol) { converged = true; break; }
+        if (err < eps_tol && thermal_ok()) { converged = true; break; }
         if (err > err_min * 1.05) rel_drop = std::max(rel_drop * 0.1, 1.0e-3);
         err_min = std::min(err_min, err);
         const double eps_vel = err * rel_drop;
         int64_t itPT = 0;
-        while (err > eps_vel && itPT <= p->iter_max) {
+        while ((err > eps_vel || !thermal_ok()) && itPT <= p->iter_max) {
             ++itPT; ++iter;
             const bool do_check = iter % ncheck == 0;
             if (do_check) for (int c = 0; c < 3; ++c) std::copy(s->Rv[c], s->Rv[c] + nv, s->Rv0[c]);
+            if (th) {
+                const int rc = thermal_hook_step(*th);
+                if (rc != 0) { out->status = rc; return rc; }
+            }
             orc_pressure_residual3(s, dt, finite_K);
             for (int64_t i = 0; i < np; ++i) s->Pnum[i] = s->gammaP[i] * s->RP[i] / s->MP[i];
             orc_momentum_residual3(s, dt, s->Pnum);
@@ -2267,6 +2308,85 @@ int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) {
     out->jacobian_assemblies = jac_assemblies;
     out->status = 0;
     return 0;
+}
+
+int orc_stokes_solve3(orc_stokes3* s, const orc_params* p, orc_stats* out) { return stokes_solve3_impl(s, p, out, nullptr); }
+
+// assemble_shear_heating! (shear_heating.jl:18-103), 3-D: the stress power
+// τ : ∇v (shear terms twice) less the elastic power τ : (τ − τ_old)/(2GΔt)
+void orc_shear_heating3(const orc_stokes3* s, double dt, double* phi) {
+    const Tables3& t = tables3();
+    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
+    double invG[ORC_MAX_PHASES];
+    for (int p = 0; p < s->mat.nphases; ++p) invG[p] = 1.0 / s->mat.G[p];
+    const bool has_old = s->tau_old[0] != nullptr;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double dNdx[27][27][3];
+        elem_dNdx3(s, e, dNdx);
+        int ph[27];
+        for (int a = 0; a < 27; ++a) ph[a] = s->phases[e] - 1;
+        for (int q = 0; q < 27; ++q) {
+            double gv[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double acc = 0;
+                    for (int a = 0; a < 27; ++a) acc += dNdx[q][a][j] * s->v[i][s->el2n[e * 27 + a]];
+                    gv[i][j] = acc;
+                }
+            double to[6] = {0, 0, 0, 0, 0, 0};
+            if (has_old)
+                for (int k = 0; k < 6; ++k) to[k] = s->tau_old[k][e * 27 + q];
+            double Pq = 0;
+            for (int i = 0; i < 4; ++i) Pq += t.NqP[q][i] * s->P[e * 4 + i];
+            double tau[6];
+            stress_ip3<double>(gv, t.Nq[q], 27, ph, r, to, Pq, tau);
+            const double inv2Gdt = interp_phase(t.Nq[q], 27, invG, ph) / (2 * dt);
+            const double power = tau[0] * gv[0][0] + tau[1] * gv[1][1] + tau[2] * gv[2][2] +
+                                 tau[3] * (gv[0][1] + gv[1][0]) + tau[4] * (gv[0][2] + gv[2][0]) +
+                                 tau[5] * (gv[1][2] + gv[2][1]);
+            const double elastic = (tau[0] * (tau[0] - to[0]) + tau[1] * (tau[1] - to[1]) + tau[2] * (tau[2] - to[2]) +
+                                    2 * (tau[3] * (tau[3] - to[3]) + tau[4] * (tau[4] - to[4]) +
+                                         tau[5] * (tau[5] - to[5]))) * inv2Gdt;
+            phi[e * 27 + q] = power - elastic;
+        }
+    }
+}
+
+// solve_coupled! on a Hex27 state (thermal mesh: the same Hex27 nodes)
+int orc_coupled_solve3(orc_thermal* t, orc_stokes3* s, const orc_params* p, const orc_coupled_params* cp,
+                       orc_stats* out, orc_coupled_stats* cst) {
+    std::memset(out, 0, sizeof *out);
+    std::memset(cst, 0, sizeof *cst);
+    g_err.clear();
+    if (t->nnodes != s->nnodes || !s->T || !s->T0) {
+        g_err = t->nnodes != s->nnodes ? "thermal mesh nodes must match the Stokes velocity-node layout"
+                                       : "the 3-D state needs its T and T0 fields for a coupled solve";
+        out->status = -2;
+        return -2;
+    }
+    transfer_T3(s, t->T, const_cast<double*>(s->T));
+    transfer_T3(s, t->T0, const_cast<double*>(s->T0));
+    ThermalHook h;
+    h.t = t;
+    h.s3 = s;
+    h.dt = p->dt;
+    h.Tref = cp->Tref;
+    h.eps = cp->tolerance_T;
+    h.nout = p->check_interval;
+    h.shear = cp->shear_heating != 0;
+    const double* saved = t->source_ip;
+    if (h.shear) {
+        h.phi.assign((size_t)s->nels * 27, 0.0);
+        t->source_ip = h.phi.data();
+    }
+    const int rc = stokes_solve3_impl(s, p, out, &h);
+    t->source_ip = saved;
+    cst->err_T = h.err_T;
+    cst->thermal_iterations = h.iter;
+    cst->thermal_converged = h.converged;
+    if (rc == 0) out->converged = out->converged && h.converged;
+    return rc;
 }
 
 }  // extern "C"

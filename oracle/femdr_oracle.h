@@ -201,7 +201,8 @@ void orc_update_current_stress(orc_stokes* s, double dt);
 
 /* --- 3-D: Hex27 velocity, P1-disc pressure with four modes per cell --- */
 /* Whole 3-D Stokes problem. Cell phases (1-based, [nels]) serve velocity and
- * pressure; no temperature, source or fluid-pressure fields (T = 0). Pressure
+ * pressure; an optional temperature field (T = 0 without), no source or
+ * fluid-pressure fields. Pressure
  * DoF a of element e is 4 e + a; stress components (xx, yy, zz, xy, xz, yz). */
 typedef struct {
     int64_t nnodes, nels, nnodesP;
@@ -223,6 +224,10 @@ typedef struct {
     /* nullable: a momentum load subtracted in the cell-pressure relaxation
      * (_subtract_body_force!, DR.jl:1034; the adjoint's objective) */
     const double* body_force[3];
+    /* nullable (both or none): temperature and its previous-step value at the
+     * pressure DoFs [nels][4] (the thermal continuity term and the density) */
+    const double* T;
+    const double* T0;
 } orc_stokes3;
 
 /* reference Hex27 (shape_functions.jl:411-446, integration_points.jl:321-337) */
@@ -386,6 +391,11 @@ typedef struct {
  * out->converged requires the thermal test too. */
 int orc_coupled_solve(orc_thermal* t, orc_stokes* s, const orc_params* p, const orc_coupled_params* cp,
                       orc_stats* out, orc_coupled_stats* cst);
+/* the same on a Hex27 state (s->T and s->T0 set; thermal mesh: the same
+ * nodes), with the 3-D shear dissipation [nels][27] */
+void orc_shear_heating3(const orc_stokes3* s, double dt, double* phi);
+int orc_coupled_solve3(orc_thermal* t, orc_stokes3* s, const orc_params* p, const orc_coupled_params* cp,
+                       orc_stats* out, orc_coupled_stats* cst);
 
 #ifdef __cplusplus
 }

@@ -128,7 +128,7 @@ class Stokes3(C.Structure):
                 ("v", PD * 3), ("rate", PD * 3), ("Rv", PD * 3), ("Rv0", PD * 3), ("dRdv", PD * 3), ("PC", PD * 3),
                 ("P", PD), ("P0", PD), ("gammaP", PD), ("RP", PD), ("MP", PD), ("Pnum", PD),
                 ("tau_old", PD * 6), ("tau_store", PD * 6), ("nbc", I64 * 3), ("bc_dofs", PI * 3),
-                ("bc_vals", PD * 3), ("body_force", PD * 3)]
+                ("bc_vals", PD * 3), ("body_force", PD * 3), ("T", PD), ("T0", PD)]
 
 
 def _declare(L):
@@ -181,6 +181,11 @@ def _declare(L):
     L.orc_coupled_solve.argtypes = [C.POINTER(ThermalC), C.POINTER(Stokes), C.POINTER(Params),
                                     C.POINTER(CoupledParamsC), C.POINTER(Stats), C.POINTER(CoupledStatsC)]
     L.orc_coupled_solve.restype = C.c_int
+    L.orc_shear_heating3.argtypes = [C.POINTER(Stokes3), D, PD]
+    L.orc_shear_heating3.restype = None
+    L.orc_coupled_solve3.argtypes = [C.POINTER(ThermalC), C.POINTER(Stokes3), C.POINTER(Params),
+                                     C.POINTER(CoupledParamsC), C.POINTER(Stats), C.POINTER(CoupledStatsC)]
+    L.orc_coupled_solve3.restype = C.c_int
     L.orc_last_error.restype = C.c_char_p
     L.orc_t7_basis.argtypes = [D, D, PD, PD]
     L.orc_t7_rule.argtypes = [PD, PD, PD]
@@ -670,6 +675,25 @@ def coupled_solve(thermal: ThermalProblem, stokes: Problem, dt, Tref=273.0, tole
     return out
 
 
+def coupled_solve3(thermal: "ThermalProblem3", stokes: "Problem3", dt, Tref=273.0, tolerance_T=1e-6,
+                   shear_heating=False, **kw):
+    """solve_coupled! on a Hex27 state (DR.jl:81-131): as coupled_solve; the
+    Stokes problem's T and T0 arrays receive the transferred temperature."""
+    if stokes.T is None:
+        stokes.T, stokes.T0 = np.zeros(4 * stokes.nels), np.zeros(4 * stokes.nels)
+    p = Problem.params(dt, **kw)
+    cp = CoupledParamsC(Tref=Tref, tolerance_T=tolerance_T, shear_heating=int(bool(shear_heating)))
+    st, cst = Stats(), CoupledStatsC()
+    ts, ss = thermal.struct(), stokes.struct()
+    rc = lib().orc_coupled_solve3(C.byref(ts), C.byref(ss), C.byref(p), C.byref(cp), C.byref(st), C.byref(cst))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    out = st.as_dict()
+    out.update(err_T=cst.err_T, thermal_iterations=cst.thermal_iterations,
+               thermal_converged=bool(cst.thermal_converged))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # 3-D: Hex27 / P1-disc (4 modes)
 def hex27_basis(xi, eta, zeta):
@@ -735,6 +759,8 @@ class Problem3:
         e = (np.zeros(0, np.int32), np.zeros(0))
         self.bc = tuple((i32(d), f64(v)) for d, v in (bc if bc is not None else (e, e, e)))
         self.body_force = None   # nullable: 3 nodal loads of the cell-pressure relaxation
+        self.T = None            # nullable (with T0): temperature at the pressure DoFs [4 nels]
+        self.T0 = None
 
     def struct(self):
         s = Stokes3()
@@ -760,8 +786,16 @@ class Problem3:
         self._bf = None if self.body_force is None else [f64(b) for b in self.body_force]
         for c in range(3):
             s.body_force[c] = ptr(self._bf[c]) if self._bf is not None else ptr(None)
+        s.T = ptr(self.T) if self.T is not None else ptr(None)
+        s.T0 = ptr(self.T0) if self.T is not None else ptr(None)
         self._struct = s
         return s
+
+    def shear_heating(self, dt):
+        """assemble_shear_heating! (shear_heating.jl:18-103), 3-D: (nels, 27)."""
+        phi = np.zeros((self.nels, 27))
+        lib().orc_shear_heating3(C.byref(self.struct()), dt, ptr(phi))
+        return phi
 
     def cp3_solve_adjoint(self, objective, lam_v=None, lam_P=None, **kw):
         """solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP)
