@@ -350,7 +350,8 @@ int fem_mesh_t7_upload(fem_ctx* c, const double* host_coords, int64_t nnodes, co
 
 int fem_mesh_get_info(const fem_mesh* m, fem_mesh_info* o) {
     if (!m || !o) return FEM_EARG;
-    o->dim = m->dim; o->nv_local = m->nloc; o->np_local = m->dim == 3 ? 4 : 3; o->nq = m->nloc;
+    o->dim = m->dim; o->nv_local = m->nloc; o->np_local = m->dim == 3 ? 4 : (m->nloc == 9 ? 0 : 3);
+    o->nq = m->nloc;
     o->nnodes = m->nnodes; o->nels = m->nels; o->nnodesP = m->nnodesP; o->n_shared = m->ns;
     o->n_boundary = (int64_t)m->h_boundary.size();
     return FEM_OK;
@@ -388,6 +389,8 @@ int fem_stokes_create(fem_ctx* c, fem_mesh* m, const fem_material* mat, const fe
         return set_err(c, FEM_EARG, "nphases must be in 1..%d", FEM_MAX_PHASES);
     if (mat->ndim != m->dim)
         return set_err(c, FEM_EDIM, "material gravity must have %d components for a %d-D mesh", m->dim, m->dim);
+    if (m->nloc == 9)
+        return set_err(c, FEM_EARG, "Stokes states take T7 or Hex27 meshes; a Q9 mesh carries thermal states only");
     HIPCHK(c, hipSetDevice(c->device));
     fem_stokes* s = new fem_stokes();
     s->ctx = c;
@@ -2921,7 +2924,7 @@ int fem_coupled_solve(fem_thermal* th, fem_stokes* s, const fem_bc* bc_T, const 
         return set_err(c, FEM_EARG, "the thermal and Stokes states must share one context (stream)");
     fem_mesh* tm = thermal_mesh(th);
     if (tm->dim != s->dim) return set_err(c, FEM_EDIM, "the thermal and Stokes meshes must have the same dimension");
-    if (tm->nnodes != s->mesh->nnodes || tm->nels != s->mesh->nels)
+    if (tm->nnodes != s->mesh->nnodes || tm->nels != s->mesh->nels || tm->nloc != s->mesh->nloc)
         return set_err(c, FEM_EDIM, "thermal mesh nodes must match the Stokes velocity-node layout");
     if (p->precision != 0)
         return set_err(c, FEM_EARG, "coupled solves run in FP64 (precision 0), got precision %d", p->precision);

@@ -36,18 +36,6 @@ constexpr int8_t kHexA[HNV] = {0, 8, 1, 11, 20, 9, 3, 10, 2, 16, 21, 17, 24, 26,
                                19, 23, 18, 4, 12, 5, 15, 25, 13, 7, 14, 6};
 constexpr int HCENTRE_T = 13;
 
-// 3-point Gauss rule (integration_points.jl:321-337): ±√(3/5), 0; 5/9, 8/9
-__host__ __device__ inline double gauss_x(int p) {
-    return p == 0 ? -0x1.8c97ef43f7248p-1 : (p == 1 ? 0.0 : 0x1.8c97ef43f7248p-1);
-}
-__host__ __device__ inline double gauss_w(int p) { return p == 1 ? 8.0 / 9.0 : 5.0 / 9.0; }
-// 1-D quadratic Lagrange basis on -1, 0, +1 and its derivative
-// (shape_functions.jl:173-196)
-__host__ __device__ inline double lq(double x, int r) {
-    return r == 0 ? x * (x - 1) * 0.5 : (r == 1 ? 1 - x * x : x * (x + 1) * 0.5);
-}
-__host__ __device__ inline double dlq(double x, int r) { return r == 0 ? x - 0.5 : (r == 1 ? -2 * x : x + 0.5); }
-
 struct HexGeo {
     int64_t nels;
     int32_t nx, ny, nz;
@@ -1208,7 +1196,7 @@ int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_
 // finished (and with `update` relaxed) by its element.
 // ---------------------------------------------------------------------------
 template <bool JAC>
-__global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g, ThermHexArgs A) {
+__global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g, ThermBoxArgs A) {
     const int es = threadIdx.x >> 5, l = threadIdx.x & 31;
     const int64_t e = xcd_tile() * HEPB + es;
     const bool ev = e < g.nels;
@@ -1319,7 +1307,7 @@ __global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g, ThermHexArgs
 
 }  // namespace
 
-int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermHexArgs& a, bool jac) {
+int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermBoxArgs& a, bool jac) {
     const HexGeo g = hex_geo(m);
     const dim3 grid(grid_xcd(m->nels, HEPB));
     if (jac) hipLaunchKernelGGL(k_thermal_hex27<true>, grid, dim3(HBLOCK), 0, c->stream, g, a);

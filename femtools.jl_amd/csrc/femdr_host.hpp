@@ -93,7 +93,7 @@ struct KernelTimer {
 
 struct fem_mesh {
     fem_ctx* ctx = nullptr;
-    int32_t dim = 2, nloc = 7;         // T7 (2-D) or Hex27 (3-D)
+    int32_t dim = 2, nloc = 7;         // T7 (2-D), Hex27 (3-D) or, for thermal states only, Q9 (2-D, nloc 9)
     int64_t nnodes = 0, nels = 0, nnodesP = 0, ns = 0;
     bool structured = false;
     double box[6] = {0, 0, 0, 0, 0, 0};
@@ -223,6 +223,19 @@ static int check_launch(fem_ctx* c, const char* what) {
     return FEM_OK;
 }
 
+// Tensor-product quadratic elements (Hex27, Q9). The 3-point Gauss rule
+// (integration_points.jl:218-241, 321-337): ±√(3/5), 0; weights 5/9, 8/9
+__host__ __device__ inline double gauss_x(int p) {
+    return p == 0 ? -0x1.8c97ef43f7248p-1 : (p == 1 ? 0.0 : 0x1.8c97ef43f7248p-1);
+}
+__host__ __device__ inline double gauss_w(int p) { return p == 1 ? 8.0 / 9.0 : 5.0 / 9.0; }
+// 1-D quadratic Lagrange basis on the nodes r = 0, 1, 2 at -1, 0, +1 and its
+// derivative (shape_functions.jl:173-196)
+__host__ __device__ inline double lq(double x, int r) {
+    return r == 0 ? x * (x - 1) * 0.5 : (r == 1 ? 1 - x * x : x * (x + 1) * 0.5);
+}
+__host__ __device__ inline double dlq(double x, int r) { return r == 0 ? x - 0.5 : (r == 1 ? -2 * x : x + 0.5); }
+
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 static inline unsigned grid_xcd(int64_t n, int block) {
     unsigned g = grid_for(n, block);
@@ -267,20 +280,22 @@ int upload_ptab(fem_stokes* s, const fem_drucker_prager* dp, double dt, int fini
 // β = 1/K is interpolated rather than K (residual.jl:62-64)
 enum ThermCol { TC_K = 0, TC_CP, TC_RHO0, TC_ALPHA, TC_BETA, TC_NCOL };
 
-// arguments of the thermal element kernel on a structured Hex27 mesh
-struct ThermHexArgs {
+// arguments of the thermal element kernels on structured Hex27 and Q9 meshes
+struct ThermBoxArgs {
     const double *T, *T0, *src, *P;
     const int32_t* phases;                 // per node, 1-based
     const double* tab;
     double dt, Tref;
-    double* Rs; double2* Js;               // [nels][26] slots of the shared nodes
+    double* Rs; double2* Js;               // [nels][26] (Hex27) or [nels][8] (Q9) slots of the shared nodes
     double *R, *R0, *dRdT, *PC, *rate, *Tw;
     const uint8_t* bcflag; const double* bcval;
     double alpha, beta;
     int32_t update;
-    const double* phi;                     // nullable: source at the quadrature points [nels][27]
+    const double* phi;                     // nullable: source at the quadrature points [nels][nq]
 };
-int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermHexArgs& a, bool jac);
+int hex27_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermBoxArgs& a, bool jac);
+// the same arguments on a structured Q9 mesh (slots [nels][8])
+int q9_thermal_element(fem_ctx* c, const fem_mesh* m, const ThermBoxArgs& a, bool jac);
 
 // The thermal side of a coupled thermal-Stokes solve (DR.jl:514-529,
 // 604-661): one thermal DR iteration per Stokes inner iteration, on the

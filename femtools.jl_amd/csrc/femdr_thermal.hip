@@ -1,5 +1,5 @@
 // femdr_thermal.hip -- ThermalDiffusionDR on the device (T7 meshes here, Hex27
-// boxes through hex27_thermal_element): the backward-Euler
+// and Q9 boxes through hex27_thermal_element and q9_thermal_element): the backward-Euler
 // heat-diffusion residual of src/heat_diffusion/assembly/residual.jl:58-87,
 // its Jacobian diagnostics (residual.jl:30-47), and the Chebyshev-accelerated
 // pseudo-transient loop of src/dynamic_relaxation/solver.jl:61-122.
@@ -327,14 +327,14 @@ ThermArgs thermal_args(fem_thermal* t, double dt, double Tref) {
 int thermal_assemble(fem_thermal* t, ThermArgs& a, bool jac) {
     fem_ctx* c = t->ctx;
     fem_mesh* m = t->mesh;
-    if (m->dim == 3) {
-        ThermHexArgs h;
+    if (m->dim == 3 || m->nloc == 9) {
+        ThermBoxArgs h;
         h.T = a.T; h.T0 = a.T0; h.src = a.src; h.P = a.P; h.phases = a.phases; h.tab = a.tab;
         h.dt = a.dt; h.Tref = a.Tref; h.Rs = a.Rs; h.Js = a.Js;
         h.R = a.R; h.R0 = a.R0; h.dRdT = a.dRdT; h.PC = a.PC; h.rate = a.rate; h.Tw = a.Tw;
         h.bcflag = a.bcflag; h.bcval = a.bcval; h.alpha = a.alpha; h.beta = a.beta; h.update = a.update;
         h.phi = a.phi;
-        CHK(hex27_thermal_element(c, m, h, jac));
+        CHK(m->dim == 3 ? hex27_thermal_element(c, m, h, jac) : q9_thermal_element(c, m, h, jac));
     } else {
         const dim3 ge(grid_xcd(m->nels, 128));
         const bool strct = m->nx > 0;
@@ -473,8 +473,8 @@ int fem_thermal_create(fem_ctx* c, fem_mesh* m, const fem_thermal_material* mat,
     for (int k = 0; k < FEM_TFIELD_PHASES && r == FEM_OK; ++k) r = dalloc_t(c, &t->f[k], nn);
     if (r == FEM_OK) r = dalloc_t(c, &t->phases, nn);
     if (r == FEM_OK) r = dalloc_t(c, &t->d_tab, TC_NCOL * MAXPH);
-    // shared-node slots per element: six for T7, 26 for Hex27
-    const int64_t nsh = m->dim == 3 ? 26 : NSH;
+    // shared-node slots per element: six for T7, eight for Q9, 26 for Hex27
+    const int64_t nsh = m->dim == 3 ? 26 : (m->nloc == 9 ? 8 : NSH);
     if (r == FEM_OK) r = dalloc_t(c, &t->d_Rs, nsh * m->nels);
     if (r == FEM_OK) r = dalloc_t(c, &t->d_Js, nsh * m->nels);
     if (r == FEM_OK) r = dalloc_t(c, &t->d_bcflag, nn);

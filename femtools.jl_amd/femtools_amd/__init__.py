@@ -27,7 +27,7 @@ __all__ = [
     "StokesMaterial", "DruckerPrager", "MixedMesh", "t7_box_mesh", "t7_mesh_from_arrays",
     "DirichletBoundaryCondition", "StokesDR", "solve", "update_stokes_current_stress",
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
-    "hex27_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
+    "hex27_box_mesh", "q9_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel", "time_correction_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
     "solve_adjoint", "stokes_material_gradient_3d", "compute_strain_rate_stress_postprocess", "write_vtk",
@@ -182,6 +182,7 @@ _sig("fem_fill_i32", C.c_int, VP, VP, I32, I64)
 _sig("fem_mesh_t7_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
 _sig("fem_mesh_t7_upload", C.c_int, VP, _P(D), I64, _P(I32), I64, _P(VP))
 _sig("fem_mesh_hex27_create", C.c_int, VP, I32, I32, I32, D, D, D, D, D, D, _P(VP))
+_sig("fem_mesh_q9_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
 _sig("fem_mesh_get_info", C.c_int, VP, _P(_MeshInfo))
 _sig("fem_mesh_get_coords", C.c_int, VP, _P(D))
 _sig("fem_mesh_get_el2n", C.c_int, VP, _P(I32))
@@ -250,7 +251,7 @@ _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveP
 EXPORTED = [
     "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
     "fem_copy_h2d", "fem_copy_d2h", "fem_copy_d2d", "fem_fill_f64", "fem_fill_f32", "fem_fill_i32",
-    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
+    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_q9_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
     "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
     "fem_stokes_pressure_scaling",
@@ -634,7 +635,8 @@ class DruckerPragerCap:
 
 class MixedMesh:
     """MixedMesh of T7 (2-D) or Hex27 (3-D) velocity / P1-disc pressure
-    elements (src/mesh/mixed_mesh.jl:173-211, 358-417) resident on the device."""
+    elements (src/mesh/mixed_mesh.jl:173-211, 358-417) resident on the device;
+    also the plain Q9 Mesh of thermal states (no pressure space)."""
 
     def __init__(self, ctx: Context, handle, box=None):
         self.ctx = ctx
@@ -689,6 +691,20 @@ def hex27_box_mesh(nx: int, ny: int, nz: int, domain=((0.0, 1.0), (0.0, 1.0), (0
                                          C.byref(h)))
     m = MixedMesh(ctx, h, box=(x0, x1, y0, y1, z0, z1))
     m.nx, m.ny, m.nz = int(nx), int(ny), int(nz)
+    return m
+
+
+def q9_box_mesh(nx: int, ny: int, domain=((0.0, 1.0), (0.0, 1.0)), ctx: Context | None = None) -> MixedMesh:
+    """Mesh(backend, (x0..x1)×(y0..y1), ReferenceElement(QuadraticElement{2,9}), (nx, ny))
+    (mesh.jl:340-398, connectivity.jl:193-227): a structured Q9 mesh for
+    ThermalDiffusionDR states. It has no pressure space (nnodesP = 0) and
+    StokesDR rejects it."""
+    ctx = ctx or default_context()
+    (x0, x1), (y0, y1) = domain
+    h = VP()
+    ctx.check(_lib.fem_mesh_q9_create(ctx.handle, int(nx), int(ny), x0, x1, y0, y1, C.byref(h)))
+    m = MixedMesh(ctx, h, box=(x0, x1, y0, y1))
+    m.nx, m.ny = int(nx), int(ny)
     return m
 
 

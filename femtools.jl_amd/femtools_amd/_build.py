@@ -11,7 +11,7 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 LIB = os.path.join(PKG, "libfemdr.so")
 SOURCES = [os.path.join(CSRC, "femdr.hip"), os.path.join(CSRC, "femdr_thermal.hip"),
-           os.path.join(CSRC, "femdr_hex27.hip"), os.path.join(CSRC, "femcnn.hip")]
+           os.path.join(CSRC, "femdr_hex27.hip"), os.path.join(CSRC, "femdr_q9.hip"), os.path.join(CSRC, "femcnn.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, "femdr_kernels.hpp"), os.path.join(CSRC, "femdr_host.hpp"),
                   os.path.join(REPO, "include", "femdr.h")]
 

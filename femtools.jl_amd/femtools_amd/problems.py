@@ -265,3 +265,21 @@ def batch_workloads(count=32, seed=0, sizes=(32, 48, 64, 80, 96, 112, 128), log_
                             etab=(inf,) * nph, G=(inf,) * nph, rho0=(1.0,) * nph, K=(inf,) * nph, g=(0.0, 0.0),
                             dt=1.0, circles=circles, extra=dict(seed=seed, index=k)))
     return out
+
+
+
+def q9_thermal_step(coords, dt=1e-2):
+    """One implicit heat-diffusion step on a Q9 mesh of the unit square with
+    node coordinates coords (the Q9 thermal step of the reference's
+    performance notes is 48 x 48 cells): k = ρ0 = Cp = 1, no expansion, a
+    Gaussian temperature anomaly T0 = exp(-r²/0.01) about the centre with a
+    heat source of the same shape (amplitude 10), T = 0 held on the whole
+    boundary, the start T = T0, and the solver defaults (tolerance 1e-6,
+    10 000 iterations, checks every 100)."""
+    x, y = coords[:, 0], coords[:, 1]
+    g = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    on = (x == 0.0) | (x == 1.0) | (y == 0.0) | (y == 1.0)
+    bc = np.nonzero(on)[0].astype(np.int32)
+    return dict(material=dict(k=(1.0,), Cp=(1.0,), rho0=(1.0,), alpha=(0.0,), K=(math.inf,)),
+                T=g.copy(), T0=g, source=10.0 * g, bc_dofs=bc, bc_vals=np.zeros(len(bc)),
+                solve=dict(dt=dt, tolerance=1e-6, max_iterations=10_000, check_interval=100, Tref=273.0))

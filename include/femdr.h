@@ -86,9 +86,9 @@ int fem_fill_i32(fem_ctx* ctx, int32_t* dptr, int32_t value, int64_t n);
 /* ------------------------------------------------------------------- mesh */
 typedef struct {
     int32_t dim;        /* 2 or 3                                    */
-    int32_t nv_local;   /* 7 (T7) or 27 (Hex27)                      */
-    int32_t np_local;   /* 3 (P1-disc) or 4 (P1-disc modes on Hex27) */
-    int32_t nq;         /* 7 (Dunavant degree 5) or 27 (3x3x3 Gauss) */
+    int32_t nv_local;   /* 7 (T7), 9 (Q9) or 27 (Hex27)              */
+    int32_t np_local;   /* 3 (P1-disc), 4 (P1-disc modes on Hex27), 0 (Q9) */
+    int32_t nq;         /* 7 (Dunavant degree 5), 9 (3x3 Gauss) or 27 (3x3x3 Gauss) */
     int64_t nnodes;     /* velocity nodes                            */
     int64_t nels;       /* elements                                  */
     int64_t nnodesP;    /* pressure DoFs = 3 nels                    */
@@ -109,6 +109,13 @@ int fem_mesh_t7_create(fem_ctx* ctx, int32_t nx, int32_t ny, double x0, double x
  * discontinuous pressure with the four modes (1-ξ-η-ζ, ξ, η, ζ) per cell (DoF 4e+a). */
 int fem_mesh_hex27_create(fem_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double x0, double x1,
                           double y0, double y1, double z0, double z1, fem_mesh** out);
+/* Mesh(backend, (x0..x1)x(y0..y1), ReferenceElement(QuadraticElement{2,9}), (nx,ny))
+ * (mesh.jl:340-398; connectivity.jl:193-227): a structured Q9 mesh on the
+ * (2nx+1) x (2ny+1) grid for ThermalDiffusionDR states (heat_diffusion_types.jl:90-153
+ * takes any Mesh). It has no pressure space (nnodesP = 0, np_local = 0) and
+ * fem_stokes_create rejects it. */
+int fem_mesh_q9_create(fem_ctx* ctx, int32_t nx, int32_t ny, double x0, double x1, double y0, double y1,
+                       fem_mesh** out);
 /* Unstructured T7 mesh: Mesh(backend, coords, el2n, element) (mesh.jl:106-115)
  * + MixedMesh. host_el2n_1based is [nels][7]; local node 7 must be the
  * element's private centroid (bubble) node. */
@@ -378,7 +385,8 @@ typedef struct {
 } fem_thermal_material;
 
 /* ThermalDiffusionDR(mesh, material; CFL, c_fact, ϵ) (heat_diffusion_types.jl:
- * 90-153, mesh constructor :141-153): nodal fields, zero; phases all 1. */
+ * 90-153, mesh constructor :141-153): nodal fields, zero; phases all 1. The
+ * mesh may be T7 (generated or uploaded), a Q9 box or a Hex27 box. */
 typedef struct {
     double CFL;     /* 0.98 */
     double c_fact;  /* 0.9  */

@@ -1305,10 +1305,15 @@ void orc_thermal_integrate(int32_t nn, int32_t nq, const double* N, const double
 // assemble_dr_matrices_atomix! (dynamic_relaxation/assembly.jl:43-60); element
 // contributions are added in ascending element order.
 static void thermal_residual3(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian);
+static void thermal_residual_q9(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian);
 
 void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian) {
     if (t->dim == 3) {
         thermal_residual3(t, dt, Tref, compute_jacobian);
+        return;
+    }
+    if (t->nloc == 9) {
+        thermal_residual_q9(t, dt, Tref, compute_jacobian);
         return;
     }
     const Tables& tb = tables();
@@ -1651,7 +1656,7 @@ int orc_coupled_solve(orc_thermal* t, orc_stokes* s, const orc_params* p, const 
     std::memset(out, 0, sizeof *out);
     std::memset(cst, 0, sizeof *cst);
     g_err.clear();
-    if (t->nnodes != s->nnodes) {
+    if (t->nnodes != s->nnodes || (t->nloc != 0 && t->nloc != 7)) {
         g_err = "thermal mesh nodes must match the Stokes velocity-node layout";
         out->status = -2;
         return -2;
@@ -2596,6 +2601,146 @@ static void thermal_residual3(orc_thermal* t, double dt, double Tref, int32_t co
         thermal_integrate(NN, NQ3, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref,
                           t->source_ip ? t->source_ip + e * NQ3 : nullptr, Re, compute_jacobian ? J.data() : nullptr,
                           3);
+        for (int a = 0; a < NN; ++a) t->R[nd[a]] += Re[a];
+        if (compute_jacobian) {
+            for (int a = 0; a < NN; ++a) {
+                double rs = 0;
+                for (int b = 0; b < NN; ++b) rs += std::fabs(J[a * NN + b]);
+                t->dRdT[nd[a]] += rs;
+                t->PC[nd[a]] += std::fabs(J[a * NN + a]);
+            }
+        }
+    }
+}
+
+// ===========================================================================
+// 2-D: ThermalDiffusionDR on Q9 (QuadraticElement{2, 9}) meshes
+// ===========================================================================
+namespace {
+
+// QuadraticElement{2, 9} local node order (shape_functions.jl:259-283)
+const double kQ9Nodes[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}, {0, 0}};
+// grid offsets of the local nodes within a cell (connectivity.jl:193-227)
+const int kQ9Grid[9][2] = {{0, 0}, {2, 0}, {2, 2}, {0, 2}, {1, 0}, {2, 1}, {1, 2}, {0, 1}, {1, 1}};
+
+// tensor product of the 1-D quadratic basis (_tensor_product_shape_functions)
+void q9_basis(double xi, double eta, double N[9], double dN[9][2]) {
+    for (int a = 0; a < 9; ++a) {
+        const double fx = lq(xi, kQ9Nodes[a][0]), fy = lq(eta, kQ9Nodes[a][1]);
+        N[a] = fx * fy;
+        dN[a][0] = dlq(xi, kQ9Nodes[a][0]) * fy;
+        dN[a][1] = fx * dlq(eta, kQ9Nodes[a][1]);
+    }
+}
+
+// IntegrationPoints(::QuadraticElement{2, 9}) (integration_points.jl:218-241):
+// the weights are the literals 25/81, 40/81, 64/81, not products of 1-D weights
+void q9_rule(double xi[9], double eta[9], double w[9]) {
+    const double a = std::sqrt(3.0 / 5.0);
+    const double pts[3] = {-a, 0.0, a};
+    const double ws[9] = {25.0 / 81, 40.0 / 81, 25.0 / 81, 40.0 / 81, 64.0 / 81, 40.0 / 81, 25.0 / 81, 40.0 / 81,
+                          25.0 / 81};
+    for (int q = 0; q < 9; ++q) {
+        xi[q] = pts[q % 3];
+        eta[q] = pts[q / 3];
+        w[q] = ws[q];
+    }
+}
+
+struct TablesQ9 {
+    double Nq[9][9], dNdxi[9][9][2], w[9];
+    TablesQ9() {
+        double xi[9], eta[9];
+        q9_rule(xi, eta, w);
+        for (int q = 0; q < 9; ++q) q9_basis(xi[q], eta[q], Nq[q], dNdxi[q]);
+    }
+};
+const TablesQ9& tables_q9() { static TablesQ9 t; return t; }
+
+}  // namespace
+
+extern "C" {
+
+void orc_q9_basis(double xi, double eta, double* N, double* dN) {
+    q9_basis(xi, eta, N, reinterpret_cast<double(*)[2]>(dN));
+}
+void orc_q9_rule(double* xi, double* eta, double* w) { q9_rule(xi, eta, w); }
+
+void orc_q9_mesh_sizes(int32_t nx, int32_t ny, int64_t* nnodes, int64_t* nels) {
+    *nnodes = (int64_t)(2 * nx + 1) * (2 * ny + 1);
+    *nels = (int64_t)nx * ny;
+}
+
+// _tensor_grid_coordinates with LinRange(x0, x1, 2nx + 1): (1 - t) x0 + t x1
+// with t = i/(2nx); the element loop has ex fastest
+void orc_q9_mesh(int32_t nx, int32_t ny, double x0, double x1, double y0, double y1, double* coords,
+                 int32_t* el2n) {
+    const int64_t sx = 2 * (int64_t)nx + 1, sy = 2 * (int64_t)ny + 1;
+    for (int64_t iy = 0; iy < sy; ++iy)
+        for (int64_t ix = 0; ix < sx; ++ix) {
+            const int64_t n = iy * sx + ix;
+            const double tx = (double)ix / (double)(sx - 1), ty = (double)iy / (double)(sy - 1);
+            coords[2 * n + 0] = (1 - tx) * x0 + tx * x1;
+            coords[2 * n + 1] = (1 - ty) * y0 + ty * y1;
+        }
+    int64_t e = 0;
+    for (int64_t ey = 0; ey < ny; ++ey)
+        for (int64_t ex = 0; ex < nx; ++ex, ++e)
+            for (int a = 0; a < 9; ++a)
+                el2n[e * 9 + a] = (int32_t)((2 * ey + kQ9Grid[a][1]) * sx + 2 * ex + kQ9Grid[a][0]);
+}
+
+void orc_q9_geometry(int64_t nels, const double* coords, const int32_t* el2n, double* invJ, double* dOmega) {
+    const TablesQ9& t = tables_q9();
+    for (int64_t e = 0; e < nels; ++e)
+        for (int q = 0; q < 9; ++q) {
+            double J[2][2] = {{0, 0}, {0, 0}};   // J[i][j] = ∂x_i/∂ξ_j
+            for (int a = 0; a < 9; ++a) {
+                const double* x = coords + 2 * (int64_t)el2n[e * 9 + a];
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) J[i][j] += x[i] * t.dNdxi[q][a][j];
+            }
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            double* iJ = invJ + (e * 9 + q) * 4;   // (J^-1)[j][i] = ∂ξ_j/∂x_i
+            iJ[0] = J[1][1] / det;
+            iJ[1] = -J[0][1] / det;
+            iJ[2] = -J[1][0] / det;
+            iJ[3] = J[0][0] / det;
+            dOmega[e * 9 + q] = std::fabs(det) * t.w[q];
+        }
+}
+
+}  // extern "C"
+
+// assemble_diffusion_matrices_atomix! on a Q9 mesh: integrate_residual
+// (residual.jl:58-87) at the nine Gauss points with the element's precomputed
+// J^-1 and |det J| w, nodal contributions added in ascending element order
+static void thermal_residual_q9(orc_thermal* t, double dt, double Tref, int32_t compute_jacobian) {
+    const TablesQ9& tb = tables_q9();
+    constexpr int NN = 9, NQ9 = 9;
+    for (int64_t n = 0; n < t->nnodes; ++n) t->R[n] = 0;
+    if (compute_jacobian)
+        for (int64_t n = 0; n < t->nnodes; ++n) t->dRdT[n] = t->PC[n] = 0;
+    double N[NQ9][NN], dNdx[NQ9][NN][2], dOm[NQ9], Tl[NN], T0l[NN], sl[NN], Pl[NN], Re[NN], J[NN * NN];
+    int32_t ph[NN];
+    for (int q = 0; q < NQ9; ++q)
+        for (int a = 0; a < NN; ++a) N[q][a] = tb.Nq[q][a];
+    for (int64_t e = 0; e < t->nels; ++e) {
+        const int32_t* nd = t->el2n + e * NN;
+        for (int q = 0; q < NQ9; ++q) {
+            const double* iJ = t->invJ + (e * NQ9 + q) * 4;
+            for (int a = 0; a < NN; ++a) {
+                dNdx[q][a][0] = tb.dNdxi[q][a][0] * iJ[0] + tb.dNdxi[q][a][1] * iJ[2];
+                dNdx[q][a][1] = tb.dNdxi[q][a][0] * iJ[1] + tb.dNdxi[q][a][1] * iJ[3];
+            }
+            dOm[q] = t->dOmega[e * NQ9 + q];
+        }
+        for (int a = 0; a < NN; ++a) {
+            Tl[a] = t->T[nd[a]]; T0l[a] = t->T0[nd[a]]; sl[a] = t->source[nd[a]]; Pl[a] = t->P[nd[a]];
+            ph[a] = t->phases[nd[a]];
+        }
+        thermal_integrate(NN, NQ9, &N[0][0], &dNdx[0][0][0], dOm, Tl, T0l, sl, ph, &t->mat, Pl, dt, Tref,
+                          t->source_ip ? t->source_ip + e * NQ9 : nullptr, Re, compute_jacobian ? J : nullptr);
         for (int a = 0; a < NN; ++a) t->R[nd[a]] += Re[a];
         if (compute_jacobian) {
             for (int a = 0; a < NN; ++a) {

@@ -309,6 +309,9 @@ typedef struct {
                                 (source_ip of residual.jl:7-47, the shear heating of a coupled solve) */
     int32_t dim;            /* 2 (T7; 0 reads as 2) or 3: Hex27 with el2n [nels][27],
                                invJ [nels][27][9] and dOmega [nels][27] (orc_hex27_geometry) */
+    int32_t nloc;           /* nodes per element: 0 reads as 7 (dim 2) or 27 (dim 3); 9 with dim 2
+                               is Q9 with el2n [nels][9], invJ [nels][9][4] and dOmega [nels][9]
+                               (orc_q9_geometry) */
 } orc_thermal;
 
 typedef struct {
@@ -338,6 +341,20 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
 /* solve!(::ThermalDiffusionDR, ...) -> solve_dynamic_relaxation!
  * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
 int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
+
+/* --- Q9: QuadraticElement{2, 9} (shape_functions.jl:259-283, integration_points.jl:218-241) --- */
+/* N [9], dN [9][2] at (ξ, η); local nodes (-1,-1),(1,-1),(1,1),(-1,1),(0,-1),(1,0),(0,1),(-1,0),(0,0) */
+void orc_q9_basis(double xi, double eta, double* N, double* dN);
+/* the 3x3 Gauss rule, ξ fastest, weights 25/81, 40/81, 64/81 */
+void orc_q9_rule(double* xi, double* eta, double* w);
+void orc_q9_mesh_sizes(int32_t nx, int32_t ny, int64_t* nnodes, int64_t* nels);
+/* generate_coordinates (mesh.jl:340-398) and generate_element2node (connectivity.jl:193-227):
+ * coords [(2nx+1)(2ny+1)][2], x fastest; el2n [nx ny][9], 0-based */
+void orc_q9_mesh(int32_t nx, int32_t ny, double x0, double x1, double y0, double y1, double* coords,
+                 int32_t* el2n);
+/* precompute_geometry (mesh.jl:592-628): the isoparametric J = Σ_a x_a ⊗ ∂N_a/∂ξ at each point;
+ * invJ [nels][9][4] row-major [ξ-index][x-index], dOmega [nels][9] = |det J| w */
+void orc_q9_geometry(int64_t nels, const double* coords, const int32_t* el2n, double* invJ, double* dOmega);
 
 /* --- Drucker-Prager tensile cap (src/stokes/assemblers/rheology.jl:510-787) --- */
 /* cap_geometry: out = (a, b, p_y, R_y, p_d, τ_d, p_q) (rheology.jl:546-557) */

@@ -92,7 +92,7 @@ class ThermalC(C.Structure):
     _fields_ = [("nnodes", I64), ("nels", I64), ("el2n", PI), ("invJ", PD), ("dOmega", PD), ("phases", PI),
                 ("mat", ThermalMaterialC), ("R", PD), ("R0", PD), ("dRdT", PD), ("PC", PD), ("T", PD), ("T0", PD),
                 ("rate", PD), ("P", PD), ("source", PD), ("nbc", I64), ("bc_dofs", PI), ("bc_vals", PD),
-                ("CFL", D), ("c_fact", D), ("source_ip", PD), ("dim", I32)]
+                ("CFL", D), ("c_fact", D), ("source_ip", PD), ("dim", I32), ("nloc", I32)]
 
 
 class CoupledParamsC(C.Structure):
@@ -141,6 +141,11 @@ def _declare(L):
     L.orc_hex27_mesh_sizes.argtypes = [I32, I32, I32, C.POINTER(I64), C.POINTER(I64)]
     L.orc_hex27_mesh.argtypes = [I32, I32, I32, D, D, D, D, D, D, PD, PI]
     L.orc_hex27_geometry.argtypes = [I64, PD, PI, PD, PD]
+    L.orc_q9_basis.argtypes = [D, D, PD, PD]
+    L.orc_q9_rule.argtypes = [PD, PD, PD]
+    L.orc_q9_mesh_sizes.argtypes = [I32, I32, C.POINTER(I64), C.POINTER(I64)]
+    L.orc_q9_mesh.argtypes = [I32, I32, D, D, D, D, PD, PI]
+    L.orc_q9_geometry.argtypes = [I64, PD, PI, PD, PD]
     L.orc_momentum_element3.argtypes = [I32, I32, I32, PD, PD, PD, PD, PD, PD, PD, PD, PD, PI, C.POINTER(Material),
                                         PD, C.POINTER(DP), D, PD, PD, PD, PD, PD]
     for f in ("orc_pressure_scaling3", "orc_augmented_jacobian3"):
@@ -609,6 +614,7 @@ class ThermalProblem:
         t.source_ip = ptr(getattr(self, "source_ip", None)) if getattr(self, "source_ip", None) is not None \
             else ptr(None)
         t.dim = getattr(self, "dim", 2)
+        t.nloc = getattr(self, "nloc", 0)
         self._struct = t
         return t
 
@@ -643,6 +649,55 @@ class ThermalProblem3(ThermalProblem):
         self.CFL, self.c_fact = CFL, c_fact
         self.dim = 3
         self.source_ip = None
+
+
+class ThermalProblemQ9(ThermalProblem):
+    """A ThermalDiffusionDR on a Q9 mesh (9 nodes, 3x3 Gauss points)."""
+
+    def __init__(self, coords, el2n, mat, phases=None, bc=(np.zeros(0, np.int32), np.zeros(0)), CFL=0.98,
+                 c_fact=0.9):
+        self.coords = f64(coords)
+        self.el2n = i32(el2n)
+        nn, ne = len(self.coords), len(self.el2n)
+        self.nnodes, self.nels = nn, ne
+        self.invJ, self.dOmega = q9_geometry(self.coords, self.el2n)
+        self.phases = i32(np.ones(nn) if phases is None else phases)
+        self.mat = mat
+        for k in ("R", "R0", "dRdT", "PC", "T", "T0", "rate", "P", "source"):
+            setattr(self, k, np.zeros(nn))
+        self.bc = (i32(bc[0]), f64(bc[1]))
+        self.CFL, self.c_fact = CFL, c_fact
+        self.dim, self.nloc = 2, 9
+        self.source_ip = None
+
+
+def q9_basis(xi, eta):
+    N, dN = np.zeros(9), np.zeros((9, 2))
+    lib().orc_q9_basis(float(xi), float(eta), ptr(N), ptr(dN))
+    return N, dN
+
+
+def q9_rule():
+    xi, eta, w = np.zeros(9), np.zeros(9), np.zeros(9)
+    lib().orc_q9_rule(ptr(xi), ptr(eta), ptr(w))
+    return xi, eta, w
+
+
+def q9_mesh(nx, ny, box=(0.0, 1.0, 0.0, 1.0)):
+    nn, ne = I64(), I64()
+    lib().orc_q9_mesh_sizes(nx, ny, C.byref(nn), C.byref(ne))
+    coords = np.zeros((nn.value, 2))
+    el2n = np.zeros((ne.value, 9), np.int32)
+    lib().orc_q9_mesh(nx, ny, *box, ptr(coords), ptr(el2n, PI))
+    return coords, el2n
+
+
+def q9_geometry(coords, el2n):
+    ne = len(el2n)
+    invJ = np.zeros((ne, 9, 4))
+    dOm = np.zeros((ne, 9))
+    lib().orc_q9_geometry(ne, ptr(f64(coords)), ptr(i32(el2n), PI), ptr(invJ), ptr(dOm))
+    return invJ, dOm
 
 
 def rotate_elastic_stress(omega, tau, dt):
