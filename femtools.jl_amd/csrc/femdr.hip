@@ -631,6 +631,7 @@ int upload_bcs(fem_stokes* s, const fem_bc* bcs) {
 
 struct LaunchOpts {
     bool plastic = false, elastic = false, bulk_inf = true, thermal = false, eos_p = false;
+    double* Ablk = nullptr;   // the Jacobian launch also stores the velocity blocks
     bool use_pnum = false, do_update = false;
     bool store_RP = false, store_Pnum = false, store_Rv = false, store_Rv0 = false, store_tau = false;
     double dt = 1.0;
@@ -994,6 +995,7 @@ static int launch_jacobian(fem_stokes* s, const LaunchOpts& o) {
     a.cap = o.cap;
     a.gamma = o.cap ? s->f[FEM_FIELD_GAMMA_PL] : nullptr;
     a.eta_ip = s->in.viscosity;
+    a.Ablk = o.Ablk;
     const bool np_ = s->layout_v == FEM_PHASE_ELEMENT;
     const int key = (o.elastic ? 4 : 0) | (o.plastic ? 2 : 0) | (np_ ? 1 : 0);
     switch (key) {
@@ -2022,6 +2024,582 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     if (!r.converged && p->throw_on_failure)
         return set_err(c, FEM_ENOCONV, "Stokes solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
                        (long long)r.iter, r.err_abs, p->tolerance);
+    return FEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The frozen-operator adjoint of a 2-D StokesDR state (solve_adjoint! with
+// operator = :blocks, DR_adjoint.jl:1-268, 396-427; FrozenAdjointOperator,
+// adjoint_operator.jl:1-660) for models without a yield model. At the
+// converged forward state the adjoint residual is affine in λ, so the element
+// blocks are assembled once: A (14×14, the augmented velocity Jacobian) by
+// the Jacobian-diagnostics kernel, B = ∂Rv/∂P, C = ∂RP/∂v, D = ∂RP/∂P (the ηb
+// storage) and W = ∂Rv/∂Pnum·Γ here. All blocks are stored
+// structure-of-arrays, [entry][nels], so a wave's loads of one entry are
+// contiguous. An adjoint iteration is the rate/λ update, the element apply
+// (gather λ, dense element products, shared-node slots) and the slot gather.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int ANU = 2 * NV, ANP = NP;
+
+struct AdjBlkArgs {
+    int64_t nels;
+    const double* geo;              // [5][nels]
+    const int32_t* phase_v; int32_t nodeph_v;   // cell [nels] or element-node [7][nels], 0-based
+    const int32_t* phase_P; int32_t nodeph_P;   // cell [nels] or [3][nels]
+    const double* ptab; int32_t nphases;
+    const double* pscale;           // Γ = γP/M_P, [nels][3]
+    double gx, gy, dt;
+    int32_t bulk_inf;               // every ηb infinite: D is zero
+    double *B, *C, *D, *W;          // [14·3][nels], [3·14][nels], [3·3][nels], [14·3][nels]
+};
+
+// B, C, D, W of one element (element_adjoint_operator_blocks,
+// adjoint_operator.jl:318-373): with no yield model P enters the momentum
+// residual through the total pressure and the density ρ0(1 − α(T − Tref) +
+// βP), and the continuity residual through the storage term.
+__global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
+    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (e >= A.nels) return;
+    const int64_t ne = A.nels;
+    const double i00 = A.geo[0 * ne + e], i01 = A.geo[1 * ne + e], i10 = A.geo[2 * ne + e],
+                 i11 = A.geo[3 * ne + e], det = A.geo[4 * ne + e];
+    int pv[NV], pp[NP];
+    for (int a = 0; a < NV; ++a) pv[a] = A.nodeph_v ? A.phase_v[a * ne + e] : A.phase_v[e];
+    for (int i = 0; i < NP; ++i) pp[i] = A.nodeph_P ? A.phase_P[i * ne + e] : A.phase_P[e];
+    double B[ANU][ANP], Bn[ANU][ANP], C[ANP][ANU], D[ANP][ANP];
+    for (int r = 0; r < ANU; ++r)
+        for (int i = 0; i < ANP; ++i) { B[r][i] = 0; Bn[r][i] = 0; C[i][r] = 0; }
+    for (int i = 0; i < ANP; ++i)
+        for (int j = 0; j < ANP; ++j) D[i][j] = 0;
+    for (int q = 0; q < NQ; ++q) {
+        const double dOm = kT.w[q] * det;
+        double r0 = 0, be = 0;
+        for (int a = 0; a < NV; ++a) {
+            r0 += kT.N[q][a] * A.ptab[PC_RHO0 * MAXPH + pv[a]];
+            be += kT.N[q][a] * A.ptab[PC_BETA * MAXPH + pv[a]];
+        }
+        double eb = INFINITY;
+        if (!A.bulk_inf) {
+            eb = 0;
+            for (int i = 0; i < NP; ++i) eb += kT.NP1[q][i] * A.ptab[PC_BULK * MAXPH + pp[i]];
+        }
+        for (int a = 0; a < NV; ++a) {
+            const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+            const double Gx = d0 * i00 + d1 * i10, Gy = d0 * i01 + d1 * i11, Na = kT.N[q][a];
+            for (int i = 0; i < NP; ++i) {
+                const double Np = kT.NP1[q][i];
+                const double drho = r0 * (be * Np);
+                B[a][i] += (Gx * (-Np) - Na * (drho * A.gx)) * dOm;
+                B[NV + a][i] += (Gy * (-Np) - Na * (drho * A.gy)) * dOm;
+                Bn[a][i] += (Gx * (-Np)) * dOm;
+                Bn[NV + a][i] += (Gy * (-Np)) * dOm;
+                C[i][a] += (Np * (-Gx)) * dOm;
+                C[i][NV + a] += (Np * (-Gy)) * dOm;
+            }
+        }
+        for (int i = 0; i < NP; ++i)
+            for (int j = 0; j < NP; ++j) D[i][j] += (kT.NP1[q][i] * (-(kT.NP1[q][j] / (eb * A.dt)))) * dOm;
+    }
+    for (int r = 0; r < ANU; ++r)
+        for (int i = 0; i < ANP; ++i) {
+            A.B[(r * ANP + i) * ne + e] = B[r][i];
+            A.W[(r * ANP + i) * ne + e] = Bn[r][i] * A.pscale[e * NP + i];
+            A.C[(i * ANU + r) * ne + e] = C[i][r];
+        }
+    for (int i = 0; i < ANP; ++i)
+        for (int j = 0; j < ANP; ++j) A.D[(i * ANP + j) * ne + e] = D[i][j];
+}
+
+struct AdjApplyArgs {
+    int64_t nels;
+    const int32_t* el2n;                        // [7][nels]
+    const double *A, *B, *C, *D, *W;
+    int32_t sym, stores;
+    const double *lx, *ly, *lP;
+    double2* slot;                              // [nels][6]
+    double *outx, *outy;                        // bubble rows written here
+    double* dP;                                 // [nels][3]
+    const double *objx, *objy;                  // nullable: added to the velocity rows
+    const double *PCx, *PCy;                    // nullable: rows divided by √PC (power iteration)
+    const uint8_t* bcflag;
+};
+
+__device__ inline double adj_row_out(double v, const double* obj, const double* PC, int64_t n, bool fixed) {
+    if (fixed) return 0.0;
+    if (PC) return v / sqrt(PC[n]);
+    return obj ? obj[n] + v : v;
+}
+
+// apply_adjoint_operator! (adjoint_operator.jl:512-535, 631-660) for one
+// element: resv = Aᵀλv + CᵀλP (the upper triangle of A and B λP for a
+// symmetric tangent), resp = Bᵀλv + Dᵀ(Wᵀλv + λP)
+__global__ __launch_bounds__(128) void k_adj_apply(AdjApplyArgs A) {
+    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (e >= A.nels) return;
+    const int64_t ne = A.nels;
+    int32_t n[NV];
+    double lv[ANU], lp[ANP];
+    for (int a = 0; a < NV; ++a) {
+        n[a] = A.el2n[a * ne + e];
+        lv[a] = A.lx[n[a]];
+        lv[NV + a] = A.ly[n[a]];
+    }
+    for (int i = 0; i < ANP; ++i) lp[i] = A.lP[e * NP + i];
+    for (int i = 0; i < ANP; ++i) {
+        double b = 0;
+        for (int k = 0; k < ANU; ++k) b += A.B[(k * ANP + i) * ne + e] * lv[k];
+        double st = 0;
+        if (A.stores) {
+            double w[ANP];
+            for (int j = 0; j < ANP; ++j) {
+                double acc = 0;
+                for (int k = 0; k < ANU; ++k) acc += A.W[(k * ANP + j) * ne + e] * lv[k];
+                w[j] = acc + lp[j];
+            }
+            for (int j = 0; j < ANP; ++j) st += A.D[(j * ANP + i) * ne + e] * w[j];
+        }
+        A.dP[e * NP + i] = b + st;   // discontinuous pressure: unshared
+    }
+    double rv[ANU];
+    for (int r = 0; r < ANU; ++r) {
+        double a = 0, b = 0;
+        for (int k = 0; k < ANU; ++k) {
+            const int64_t idx = A.sym ? (int64_t)(r < k ? r : k) * ANU + (r < k ? k : r) : (int64_t)k * ANU + r;
+            a += A.A[idx * ne + e] * lv[k];
+        }
+        for (int i = 0; i < ANP; ++i)
+            b += (A.sym ? A.B[(r * ANP + i) * ne + e] : A.C[(i * ANU + r) * ne + e]) * lp[i];
+        rv[r] = a + b;
+    }
+    for (int a = 0; a < NSH; ++a) A.slot[e * NSH + a] = make_double2(rv[a], rv[NV + a]);
+    const int32_t bn = n[NSH];
+    const uint8_t f = A.bcflag[bn];
+    A.outx[bn] = adj_row_out(rv[NSH], A.objx, A.PCx, bn, f & 1);
+    A.outy[bn] = adj_row_out(rv[NV + NSH], A.objy, A.PCy, bn, f & 2);
+}
+
+struct AdjGatherArgs {
+    int64_t ns;
+    const int32_t *node_of, *off, *ent;
+    const double2* slot;
+    double *outx, *outy;
+    const double *objx, *objy, *PCx, *PCy;
+    const uint8_t* bcflag;
+};
+
+// shared-node rows in ascending element order
+__global__ __launch_bounds__(256) void k_adj_gather(AdjGatherArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.ns) return;
+    const int64_t node = A.node_of ? A.node_of[i] : i;
+    double x = 0, y = 0;
+    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
+        const double2 v = A.slot[A.ent[k]];
+        x += v.x;
+        y += v.y;
+    }
+    const uint8_t f = A.bcflag[node];
+    A.outx[node] = adj_row_out(x, A.objx, A.PCx, node, f & 1);
+    A.outy[node] = adj_row_out(y, A.objy, A.PCy, node, f & 2);
+}
+
+// R0 ← R, then the rate and λ update of every node (stokes_update_rate!,
+// stokes_update_variable! with −α, homogeneous Dirichlet rows; DR_adjoint.jl:234-249)
+__global__ void k_adj_update(int64_t n, const double* Rx, const double* Ry, double* R0x, double* R0y, double* rx,
+                             double* ry, double* lx, double* ly, const double* PCx, const double* PCy,
+                             const uint8_t* bcflag, double ax, double ay, double bx, double by) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t f = bcflag[i];
+    const double Rxi = Rx[i], Ryi = Ry[i];
+    R0x[i] = Rxi;
+    R0y[i] = Ryi;
+    if (f & 1) { rx[i] = 0; lx[i] = 0; }
+    else { const double nr = bx * rx[i] + Rxi / PCx[i]; rx[i] = nr; lx[i] += -ax * nr; }
+    if (f & 2) { ry[i] = 0; ly[i] = 0; }
+    else { const double nr = by * ry[i] + Ryi / PCy[i]; ry[i] = nr; ly[i] += -ay * nr; }
+}
+
+// power-iteration step: x ← x/‖x‖, z = x/√PC (estimate_adjoint_λmax, adjoint_operator.jl:589-596)
+__global__ void k_adj_power(int64_t n, double* xx, double* xy, double* zx, double* zy, const double* PCx,
+                            const double* PCy, double nrm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = xx[i] / nrm, b = xy[i] / nrm;
+    xx[i] = a;
+    xy[i] = b;
+    zx[i] = a / sqrt(PCx[i]);
+    zy[i] = b / sqrt(PCy[i]);
+}
+
+// Σ a², Σ b², Σ c² (c over its own length)
+__global__ __launch_bounds__(RED_BLOCK) void k_adj_norms(int64_t n, int64_t nc, const double* a, const double* b,
+                                                         const double* c, double* partial) {
+    double acc[3] = {0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride) {
+        acc[0] += a[i] * a[i];
+        acc[1] += b[i] * b[i];
+    }
+    if (c)
+        for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < nc; i += stride) acc[2] += c[i] * c[i];
+    block_sum_store<3>(acc, partial);
+}
+
+// Arrow-Hurwicz update of the pressure adjoint, λP += γP ResλP / M_P
+__global__ void k_adj_lamP(int64_t n, double* lP, const double* gP, const double* RP, const double* MP) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lP[i] += gP[i] * RP[i] / MP[i];
+}
+
+struct AdjWork {
+    fem_ctx* c = nullptr;
+    std::vector<void*> bufs;
+    template <class T>
+    int get(T** p, size_t n) {
+        int r = dalloc_t(c, p, n);
+        if (r == FEM_OK) bufs.push_back(*p);
+        return r;
+    }
+    ~AdjWork() {
+        if (c) hipStreamSynchronize(c->stream);
+        for (void* b : bufs) hipFree(b);
+    }
+};
+
+struct AdjointOp {
+    double *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr, *W = nullptr;
+    double2* slot = nullptr;
+    bool sym = false, stores = false;
+};
+
+// the blocks at the state in s; also fills ∂Rv∂v and PC_v from A
+static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    const int64_t ne = m->nels;
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, 0, o, true));   // PC_BULK = ηb: the operator's pressure bulk
+    const fem_material& mt = s->mat;
+    op.sym = true;
+    op.stores = false;
+    for (int p = 0; p < mt.nphases; ++p) {
+        if (!std::isinf(mt.K[p])) op.sym = false;
+        if (std::isfinite(mt.etab[p])) op.stores = true;
+    }
+    CHK(w.get(&op.A, (size_t)ANU * ANU * ne));
+    CHK(w.get(&op.B, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.C, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.D, (size_t)ANP * ANP * ne));
+    CHK(w.get(&op.W, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.slot, (size_t)NSH * ne));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(m->nnodesP, 256)), dim3(256), 0, c->stream, m->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    CHK(check_launch(c, "k_pscale"));
+    o.Ablk = op.A;
+    CHK(launch_jacobian(s, o));
+    AdjBlkArgs b;
+    b.nels = ne;
+    b.geo = m->d_geo;
+    b.phase_v = s->d_phase_v;
+    b.nodeph_v = s->layout_v == FEM_PHASE_ELEMENT;
+    b.phase_P = s->d_phase_P;
+    b.nodeph_P = s->layout_P == FEM_PHASE_ELEMENT;
+    b.ptab = s->d_ptab;
+    b.nphases = mt.nphases;
+    b.pscale = s->d_pscale;
+    b.gx = mt.g[0];
+    b.gy = mt.g[1];
+    b.dt = dt;
+    b.bulk_inf = o.bulk_inf;
+    b.B = op.B; b.C = op.C; b.D = op.D; b.W = op.W;
+    hipLaunchKernelGGL(k_adj_blocks, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
+    CHK(check_launch(c, "k_adj_blocks"));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+// out = op applied to (lx, ly, lP): velocity rows into (ox, oy) with obj added
+// (or divided by √PC when PCx is set), homogeneous Dirichlet rows; dP pressure rows
+static int adjoint_apply(fem_stokes* s, const AdjointOp& op, const double* lx, const double* ly, const double* lP,
+                         double* ox, double* oy, double* dP, const double* objx, const double* objy,
+                         const double* PCx, const double* PCy) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    AdjApplyArgs a;
+    a.nels = m->nels;
+    a.el2n = m->d_el2n;
+    a.A = op.A; a.B = op.B; a.C = op.C; a.D = op.D; a.W = op.W;
+    a.sym = op.sym; a.stores = op.stores;
+    a.lx = lx; a.ly = ly; a.lP = lP;
+    a.slot = op.slot;
+    a.outx = ox; a.outy = oy; a.dP = dP;
+    a.objx = objx; a.objy = objy; a.PCx = PCx; a.PCy = PCy;
+    a.bcflag = s->d_bcflag;
+    hipLaunchKernelGGL(k_adj_apply, dim3(grid_for(m->nels, 128)), dim3(128), 0, c->stream, a);
+    CHK(check_launch(c, "k_adj_apply"));
+    AdjGatherArgs g;
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.slot = op.slot;
+    g.outx = ox; g.outy = oy;
+    g.objx = objx; g.objy = objy; g.PCx = PCx; g.PCy = PCy;
+    g.bcflag = s->d_bcflag;
+    hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m->ns, 256)), dim3(256), 0, c->stream, g);
+    return check_launch(c, "k_adj_gather");
+}
+
+// Σa², Σb², Σc² to the host
+static int adjoint_norms(fem_stokes* s, const double* a, const double* b, const double* cc, int64_t nc,
+                         double out[3]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, nc), RED_BLOCK));
+    hipLaunchKernelGGL(k_adj_norms, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, nc, a, b, cc, c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_adj_norms"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_red, 3 * 8);
+    return FEM_OK;
+}
+
+// _assert_frozen_symmetry (adjoint_operator.jl:149-160) on the stored blocks
+static int adjoint_check_symmetry(fem_stokes* s, const AdjointOp& op) {
+    fem_ctx* c = s->ctx;
+    const int64_t ne = s->mesh->nels;
+    std::vector<double> A((size_t)ANU * ANU * ne), B((size_t)ANU * ANP * ne), C((size_t)ANU * ANP * ne);
+    HIPCHK(c, hipMemcpy(A.data(), op.A, A.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(B.data(), op.B, B.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(C.data(), op.C, C.size() * 8, hipMemcpyDeviceToHost));
+    double wA = 0, wC = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        double nA = 0, dA = 0, nC = 0, dC = 0;
+        for (int i = 0; i < ANU; ++i)
+            for (int j = 0; j < ANU; ++j) {
+                const double x = A[(i * ANU + j) * ne + e], y = A[(j * ANU + i) * ne + e];
+                nA += x * x;
+                dA += (x - y) * (x - y);
+            }
+        for (int i = 0; i < ANP; ++i)
+            for (int k = 0; k < ANU; ++k) {
+                const double x = C[(i * ANU + k) * ne + e], y = B[(k * ANP + i) * ne + e];
+                nC += x * x;
+                dC += (x - y) * (x - y);
+            }
+        wA = std::max(wA, nA == 0 ? 0.0 : std::sqrt(dA) / std::sqrt(nA));
+        wC = std::max(wC, nC == 0 ? 0.0 : std::sqrt(dC) / std::sqrt(nC));
+    }
+    const double tol = std::sqrt(std::numeric_limits<double>::epsilon());
+    if (wA > tol || wC > tol)
+        return set_err(c, FEM_EARG,
+                       "the frozen adjoint operator assumes the element tangent is symmetric, but the assembled "
+                       "blocks disagree: max ‖A - Aᵀ‖/‖A‖ = %g, max ‖C - Bᵀ‖/‖C‖ = %g over %lld elements",
+                       wA, wC, (long long)ne);
+    return FEM_OK;
+}
+
+}  // namespace
+
+static int adjoint_checks(fem_stokes* s) {
+    if (s->dim != 2) return set_err(s->ctx, FEM_EARG, "the frozen-operator adjoint runs on 2-D (T7/P1-disc) states");
+    const int64_t np = s->mesh->nnodesP;
+    std::vector<double> g(np);
+    HIPCHK(s->ctx, hipMemcpy(g.data(), s->f[FEM_FIELD_GAMMA_P], np * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < np; ++i)
+        if (g[i] != 0) return FEM_OK;
+    return set_err(s->ctx, FEM_EARG, "dr.γP is unassembled; run the forward solve! on this state before solve_adjoint!");
+}
+
+extern "C" int fem_stokes_adjoint_blocks(fem_stokes* s, double dt, double* host_A, double* host_B, double* host_C,
+                                         double* host_D, double* host_W) {
+    if (!s || !host_A || !host_B || !host_C || !host_D || !host_W) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (s->dim != 2) return set_err(c, FEM_EARG, "the frozen-operator adjoint runs on 2-D (T7/P1-disc) states");
+    AdjWork w;
+    w.c = c;
+    AdjointOp op;
+    CHK(adjoint_assemble(s, dt, w, op));
+    const int64_t ne = s->mesh->nels;
+    struct { const double* d; double* h; int rows; } out[5] = {
+        {op.A, host_A, ANU * ANU}, {op.B, host_B, ANU * ANP}, {op.C, host_C, ANU * ANP}, {op.D, host_D, ANP * ANP},
+        {op.W, host_W, ANU * ANP}};
+    std::vector<double> t;
+    for (auto& o : out) {   // [entry][nels] on the device -> [nels][entry]
+        t.resize((size_t)o.rows * ne);
+        HIPCHK(c, hipMemcpy(t.data(), o.d, t.size() * 8, hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < o.rows; ++k) o.h[e * o.rows + k] = t[(size_t)k * ne + e];
+    }
+    return FEM_OK;
+}
+
+// solve_adjoint!(dr, mesh, bc_v; dt, objective_v, λv, λP, ...) with operator =
+// :blocks (DR_adjoint.jl:396-427 -> _solve_stokes_adjoint_dyrel!, :1-268)
+extern "C" int fem_stokes_solve_adjoint(fem_stokes* s, const fem_bc* bc_v, const double* objective_x,
+                                        const double* objective_y, double* lam_x, double* lam_y, double* lam_P,
+                                        const fem_adjoint_params* p, fem_adjoint_stats* st) {
+    if (!s || !bc_v || !objective_x || !objective_y || !lam_x || !lam_y || !lam_P || !p || !st) return FEM_EARG;
+    fem_ctx* c = s->ctx;
+    std::memset(st, 0, sizeof *st);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (p->check_interval < 1) return set_err(c, FEM_EARG, "check_interval must be positive");
+    const auto t0 = std::chrono::steady_clock::now();
+    CHK(adjoint_checks(s));
+    CHK(upload_bcs(s, bc_v));
+    fem_mesh* m = s->mesh;
+    const int64_t nv = m->nnodes, np = m->nnodesP;
+    AdjWork w;
+    w.c = c;
+    AdjointOp op;
+    CHK(adjoint_assemble(s, p->dt, w, op));
+    if (op.sym) CHK(adjoint_check_symmetry(s, op));
+    double* PCx = s->f[FEM_FIELD_PC_V_X];
+    double* PCy = s->f[FEM_FIELD_PC_V_Y];
+    double lmc[2];
+    CHK(reduce_lmax(s, lmc));
+    const char* labels[2] = {"adjoint vx", "adjoint vy"};
+    for (int k = 0; k < 2; ++k)
+        if (!(std::isfinite(lmc[k]) && lmc[k] > 0))
+            return set_err(c, FEM_EPRECOND,
+                           "%s preconditioner produced invalid λmax = %g; check for zero or non-finite preconditioner entries",
+                           labels[k], lmc[k]);
+    st->lmax_gershgorin = std::max(lmc[0], lmc[1]);
+    double lmax_v[2] = {lmc[0], lmc[1]};
+    double *Rx, *Ry, *R0x, *R0y, *rx, *ry, *RP;
+    CHK(w.get(&Rx, nv)); CHK(w.get(&Ry, nv)); CHK(w.get(&R0x, nv)); CHK(w.get(&R0y, nv));
+    CHK(w.get(&rx, nv)); CHK(w.get(&ry, nv)); CHK(w.get(&RP, np));
+    const unsigned gv = grid_for(nv, 256);
+    if (p->measure_lmax) {
+        // estimate_adjoint_λmax (adjoint_operator.jl:564-607): symmetric Jacobi
+        // scaling, λP = 0, Dirichlet rows projected out, 100 steps, rtol 1e-3
+        double *xx, *xy, *zx, *zy, *yx, *yy, *zP, *sP;
+        CHK(w.get(&xx, nv)); CHK(w.get(&xy, nv)); CHK(w.get(&zx, nv)); CHK(w.get(&zy, nv));
+        CHK(w.get(&yx, nv)); CHK(w.get(&yy, nv)); CHK(w.get(&zP, np)); CHK(w.get(&sP, np));
+        HIPCHK(c, hipMemsetAsync(zP, 0, np * 8, c->stream));
+        std::vector<double> sx(nv), sy(nv);
+        for (int64_t i = 0; i < nv; ++i) {   // _power_seed(i, d), 1-based i
+            sx[i] = std::sin(0.7 * (double)(i + 1));
+            sy[i] = std::cos(1.3 * (double)(i + 1));
+        }
+        std::vector<uint8_t> flag(nv);
+        HIPCHK(c, hipMemcpy(flag.data(), s->d_bcflag, nv, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nv; ++i) {
+            if (flag[i] & 1) sx[i] = 0;
+            if (flag[i] & 2) sy[i] = 0;
+        }
+        HIPCHK(c, hipMemcpy(xx, sx.data(), nv * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(xy, sy.data(), nv * 8, hipMemcpyHostToDevice));
+        double lam = 0, nrm[3];
+        int64_t its = 0;
+        CHK(adjoint_norms(s, xx, xy, nullptr, 0, nrm));
+        for (int64_t it = 1; it <= 100; ++it) {
+            its = it;
+            const double nx = std::sqrt(nrm[0] + nrm[1]);
+            if (!(nx > 0)) return set_err(c, FEM_EARG, "power iteration collapsed to the zero vector");
+            hipLaunchKernelGGL(k_adj_power, dim3(gv), dim3(256), 0, c->stream, nv, xx, xy, zx, zy,
+                               (const double*)PCx, (const double*)PCy, nx);
+            CHK(check_launch(c, "k_adj_power"));
+            CHK(adjoint_apply(s, op, zx, zy, zP, yx, yy, sP, nullptr, nullptr, PCx, PCy));
+            CHK(adjoint_norms(s, yx, yy, nullptr, 0, nrm));
+            const double ln = std::sqrt(nrm[0] + nrm[1]);
+            std::swap(xx, yx);
+            std::swap(xy, yy);
+            const bool conv = it > 1 && std::fabs(ln - lam) <= 1e-3 * ln;
+            lam = ln;
+            if (conv) break;
+        }
+        if (!(std::isfinite(lam) && lam > 0))
+            return set_err(c, FEM_EARG, "power iteration produced a non-positive λmax: %g", lam);
+        lmax_v[0] = lmax_v[1] = lam;
+        st->lmax_iterations = its;
+    }
+    st->lmax = lmax_v[0];
+    double dtau[2], al[2], be[2];
+    for (int k = 0; k < 2; ++k) {
+        dtau[k] = 2 / std::sqrt(lmax_v[k]) * s->opts.CFL_v;
+        cheb(dtau[k], 0.0, s->opts.c_fact, &al[k], &be[k]);
+    }
+    HIPCHK(c, hipMemsetAsync(rx, 0, nv * 8, c->stream));   // the rates start from rest
+    HIPCHK(c, hipMemsetAsync(ry, 0, nv * 8, c->stream));
+    DRSet<double> d;
+    d.Rv_x = Rx; d.Rv_y = Ry; d.Rv0_x = R0x; d.Rv0_y = R0y; d.rate_x = rx; d.rate_y = ry;
+    d.PC_x = PCx; d.PC_y = PCy;
+    auto assemble = [&]() {
+        return adjoint_apply(s, op, lam_x, lam_y, lam_P, Rx, Ry, RP, objective_x, objective_y, nullptr, nullptr);
+    };
+    const double snv = std::sqrt((double)nv), snp = std::sqrt((double)np);
+    int64_t iter = 0, itPH_done = 0;
+    double err = INFINITY, err_v0 = 1, err_P0 = 1, err_v = INFINITY, err_P = INFINITY;
+    bool converged = false;
+    double nrm[3], red[6];
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        CHK(assemble());
+        CHK(adjoint_norms(s, Rx, Ry, RP, np, nrm));
+        err_v = std::max(std::sqrt(nrm[0]), std::sqrt(nrm[1])) / snv;
+        err_P = std::sqrt(nrm[2]) / snp;
+        if (itPH == 1) {
+            err_v0 = err_v + julia_eps(err_v);
+            err_P0 = err_P + julia_eps(err_P);
+        }
+        err = std::max(std::min(err_v, err_v / err_v0), std::min(err_P, err_P / err_P0));
+        if (p->verbose)
+            printf("adj PH=%03lld iter=%06lld err=%.3e Rv=%.3e RP=%.3e\n", (long long)itPH, (long long)iter, err, err_v,
+                   err_P);
+        if (err < p->tolerance) {
+            converged = true;
+            break;
+        }
+        const double target = std::max(err_v * p->rel_drop, p->tolerance);
+        int64_t inner = 0;
+        while (err_v > target && inner < p->iter_max && iter < p->max_iterations) {
+            ++inner;
+            ++iter;
+            hipLaunchKernelGGL(k_adj_update, dim3(gv), dim3(256), 0, c->stream, nv, (const double*)Rx,
+                               (const double*)Ry, R0x, R0y, rx, ry, lam_x, lam_y, (const double*)PCx,
+                               (const double*)PCy, (const uint8_t*)s->d_bcflag, al[0], al[1], be[0], be[1]);
+            CHK(check_launch(c, "k_adj_update"));
+            CHK(assemble());
+            if (iter % p->check_interval == 0) {
+                CHK(reduce_check(s, d, al, red));
+                err_v = std::max(std::sqrt(red[0]), std::sqrt(red[1])) / snv;
+                for (int k = 0; k < 2; ++k) {
+                    const double den = red[2 + 2 * k], num = red[3 + 2 * k];
+                    const double lmin = den == 0 ? 0.0 : std::fabs(num) / den;
+                    cheb(dtau[k], lmin, s->opts.c_fact, &al[k], &be[k]);
+                }
+            }
+        }
+        hipLaunchKernelGGL(k_adj_lamP, dim3(grid_for(np, 256)), dim3(256), 0, c->stream, np, lam_P,
+                           (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)RP,
+                           (const double*)s->f[FEM_FIELD_M_P]);
+        CHK(check_launch(c, "k_adj_lamP"));
+        if (iter >= p->max_iterations) break;
+    }
+    if (!converged) {
+        CHK(assemble());
+        CHK(adjoint_norms(s, Rx, Ry, RP, np, nrm));
+        err_v = std::max(std::sqrt(nrm[0]), std::sqrt(nrm[1])) / snv;
+        err_P = std::sqrt(nrm[2]) / snp;
+        err = std::max(std::min(err_v, err_v / err_v0), std::min(err_P, err_P / err_P0));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged || err < p->tolerance;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!st->converged && p->throw_on_failure)
+        return set_err(c, FEM_ENOCONV,
+                       "Stokes adjoint solve did not converge after %lld iterations (residual = %g, tolerance = %g)",
+                       (long long)iter, err, p->tolerance);
     return FEM_OK;
 }
 

@@ -1302,6 +1302,7 @@ struct JacArgs {
     const F* gamma;              // nullable: accepted plastic strain (cohesion softening)
     const F* eta_ip;             // nullable: viscosity at the points [nels][NQ]; the element
                                  // maximum replaces the phase maximum (momentum_residuals.jl:43)
+    F* Ablk;                     // nullable: the whole velocity block, [14][14][nels] (row = c·7 + a)
 };
 
 template <class F, bool ELASTIC, bool PLASTIC, bool NODEPH, int BLOCK>
@@ -1478,6 +1479,15 @@ __global__ __launch_bounds__(BLOCK) void k_t7_jacobian(JacArgs<F> A) {
                 Jx[0][b] += fx * bx; Jx[1][b] += fx * by;
                 Jy[0][b] += fy * bx; Jy[1][b] += fy * by;
             }
+        }
+        if (A.Ablk) {   // stored for the frozen adjoint operator (adjoint_operator.jl:412-436)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < NV; ++b) {
+                    A.Ablk[((int64_t)a * 2 * NV + c * NV + b) * ne + e] = Jx[c][b];
+                    A.Ablk[((int64_t)(NV + a) * 2 * NV + c * NV + b) * ne + e] = Jy[c][b];
+                }
         }
         F sx = 0, sy = 0;
 #pragma unroll

@@ -30,7 +30,7 @@ __all__ = [
     "hex27_box_mesh", "q9_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
     "solve_thermal", "thermal_residual", "residual", "jacobian_diagnostics", "time_element_kernel", "time_correction_kernel",
     "solve_coupled", "assemble_shear_heating", "rotate_stress", "copyto", "DruckerPragerCap", "set_pressure_source",
-    "solve_adjoint", "stokes_material_gradient_3d", "compute_strain_rate_stress_postprocess", "write_vtk",
+    "solve_adjoint", "solve_stokes_adjoint", "adjoint_blocks", "stokes_material_gradient_3d", "compute_strain_rate_stress_postprocess", "write_vtk",
     "write_stokes_vtk", "VtkMesh",
     "commit_stokes_plastic_history", "PlasticHistory", "CellPressureStokesDR",
     "assemble_stokes_momentum_residual_3d", "assemble_stokes_pressure_residual_3d", "stokes_preconditioner_3d",
@@ -146,6 +146,17 @@ class _CellParams(C.Structure):
                 ("pressure_step", D), ("verbose", I32), ("throw_on_failure", I32)]
 
 
+class _AdjointParams(C.Structure):
+    _fields_ = [("dt", D), ("tolerance", D), ("max_iterations", I64), ("check_interval", I64), ("iter_max", I64),
+                ("max_ph_iterations", I64), ("rel_drop", D), ("measure_lmax", I32), ("throw_on_failure", I32),
+                ("verbose", I32)]
+
+
+class _AdjointStats(C.Structure):
+    _fields_ = [("converged", I32), ("itPH", I64), ("iter", I64), ("lmax_iterations", I64), ("err", D),
+                ("err_v", D), ("err_P", D), ("lmax", D), ("lmax_gershgorin", D), ("seconds", D)]
+
+
 class _CellStats(C.Structure):
     _fields_ = [("converged", I32), ("reached_total_iter", I32), ("iterations", I64), ("residual", D),
                 ("err_v", D), ("err_P", D), ("seconds", D)]
@@ -244,6 +255,8 @@ _sig("fem_cnn_gemm_bf16", C.c_int, VP, I64, I64, I64, VP, VP, VP)
 _sig("fem_cell_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(VP), _P(VP), VP, _P(_CellParams), _P(_CellStats),
      _P(D), I64, _P(I64))
 _sig("fem_cell_stokes_material_gradient", C.c_int, VP, _P(VP), _P(VP), _P(D), _P(D))
+_sig("fem_stokes_solve_adjoint", C.c_int, VP, _P(_BC), VP, VP, VP, VP, VP, _P(_AdjointParams), _P(_AdjointStats))
+_sig("fem_stokes_adjoint_blocks", C.c_int, VP, D, _P(D), _P(D), _P(D), _P(D), _P(D))
 _sig("fem_stokes_set_pressure_source", C.c_int, VP, VP, D, I32)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
@@ -256,6 +269,7 @@ EXPORTED = [
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
     "fem_stokes_pressure_scaling",
     "fem_stokes_update_current_stress", "fem_stokes_residual", "fem_stokes_jacobian_diagnostics",
+    "fem_stokes_solve_adjoint", "fem_stokes_adjoint_blocks",
     "fem_stokes_time_element_kernel", "fem_stokes_time_correction_kernel", "fem_ctx_set_tuning", "fem_ctx_get_tuning", "fem_thermal_create", "fem_thermal_destroy", "fem_thermal_field",
     "fem_thermal_solve", "fem_thermal_residual", "fem_stokes_shear_heating", "fem_coupled_solve",
     "fem_stokes_rotate_stress", "fem_stokes_solve_cap", "fem_stokes_residual_cap",
@@ -1526,7 +1540,82 @@ def _cell_result(st, hist, nh):
     return out
 
 
-def solve_adjoint(dr: CellPressureStokesDR, mesh: MixedMesh, bc_vx, bc_vy, bc_vz, *, objective_v, lam_v, lam_P,
+def solve_adjoint(dr, mesh, *bcs, **kwargs):
+    """solve_adjoint!(dr, mesh, bc...; kwargs...), dispatched on the state type
+    like the reference's methods: StokesDR -> solve_stokes_adjoint
+    (DR_adjoint.jl:396-427), CellPressureStokesDR -> solve_cell_adjoint
+    (DR_adjoint.jl:446-480)."""
+    if isinstance(dr, CellPressureStokesDR):
+        return solve_cell_adjoint(dr, mesh, *bcs, **kwargs)
+    return solve_stokes_adjoint(dr, mesh, *bcs, **kwargs)
+
+
+def solve_stokes_adjoint(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundaryCondition,
+                         bc_vy: DirichletBoundaryCondition, *, dt, objective_vx, objective_vy, lam_vx, lam_vy, lam_P,
+                         tolerance=1e-6, max_iterations=50_000, check_interval=50, throw_on_failure=True,
+                         iterMax=50_000, max_ph_iterations=100, rel_drop=0.1, measure_lmax=True, plastic=None,
+                         phases_v=None, phases_P=None, operator="blocks", verbose=False):
+    """solve_adjoint!(dr::StokesDR, mesh, bc_vx, bc_vy; dt, objective_vx,
+    objective_vy, λvx, λvy, λP, ...) (DR_adjoint.jl:396-427) with the stored
+    element blocks (operator = :blocks) on a converged 2-D forward state without
+    a yield model. lam_* are DeviceArrays updated in place (their values are the
+    initial iterate) or host arrays (then the results are in the returned
+    dict as lam_vx, lam_vy, lam_P). Returns (; itPH, iter, err, err_v, err_P,
+    converged, iterations, residual, lmax, lmax_gershgorin, lmax_iterations)."""
+    if operator != "blocks":
+        raise ValueError(f"operator must be 'blocks' in this build, got {operator!r}")
+    if plastic is not None:
+        raise ValueError("the frozen-operator adjoint covers models without a yield model")
+    if mesh is not dr.mesh:
+        raise ValueError("dr was built on a different mesh")
+    if mesh.dim != 2:
+        raise ValueError("the frozen-operator adjoint runs on 2-D (T7/P1-disc) states")
+    if phases_v is not None:
+        dr.set_phases("v", phases_v)
+    if phases_P is not None:
+        dr.set_phases("P", phases_P)
+    nv, npp = mesh.nnodes, mesh.nnodesP
+    for o in (objective_vx, objective_vy, lam_vx, lam_vy):
+        if _size(o) != nv:
+            raise DimensionMismatch("objective and velocity adjoint arrays must have mesh.nnodes entries")
+    if _size(lam_P) != npp:
+        raise DimensionMismatch("the pressure adjoint must have mesh.nnodesP entries")
+    keep = []
+    arr = (_BC * 2)(*(b._c(keep) for b in (bc_vx, bc_vy)))
+    p = _AdjointParams(dt=float(dt), tolerance=float(tolerance), max_iterations=int(max_iterations),
+                       check_interval=int(check_interval), iter_max=int(iterMax),
+                       max_ph_iterations=int(max_ph_iterations), rel_drop=float(rel_drop),
+                       measure_lmax=int(bool(measure_lmax)), throw_on_failure=int(bool(throw_on_failure)),
+                       verbose=int(bool(verbose)))
+    st = _AdjointStats()
+    with _Inputs(dr) as io:
+        ox, oy = io._dev(objective_vx, (nv,)), io._dev(objective_vy, (nv,))
+        lams = [io._dev(a, (n,)) for a, n in ((lam_vx, nv), (lam_vy, nv), (lam_P, npp))]
+        rc = _lib.fem_stokes_solve_adjoint(dr.handle, arr, ox, oy, *lams, C.byref(p), C.byref(st))
+        out = {k: getattr(st, k) for k, _ in _AdjointStats._fields_}
+        out["converged"] = bool(out["converged"])
+        out["iterations"], out["residual"] = out["iter"], out["err"]
+        if rc == FEM_OK or rc == FEM_ENOCONV:
+            for name, a, ptr_, n in (("lam_vx", lam_vx, lams[0], nv), ("lam_vy", lam_vy, lams[1], nv),
+                                     ("lam_P", lam_P, lams[2], npp)):
+                out[name] = a if isinstance(a, DeviceArray) else DeviceArray(dr.ctx, ptr_, (n,)).numpy()
+    dr.ctx.check(rc)
+    return out
+
+
+def adjoint_blocks(dr: StokesDR, dt):
+    """assemble_adjoint_operator (adjoint_operator.jl:412-475) at the state in
+    dr: dict of host arrays A (nels, 14, 14), B (nels, 14, 3), C (nels, 3, 14),
+    D (nels, 3, 3), W (nels, 14, 3); velocity DoFs stacked by component."""
+    ne = dr.mesh.nels
+    out = dict(A=np.zeros((ne, 14, 14)), B=np.zeros((ne, 14, 3)), C=np.zeros((ne, 3, 14)), D=np.zeros((ne, 3, 3)),
+               W=np.zeros((ne, 14, 3)))
+    dr.ctx.check(_lib.fem_stokes_adjoint_blocks(dr.handle, float(dt), *(out[k].ctypes.data_as(_P(D))
+                                                                        for k in "ABCDW")))
+    return out
+
+
+def solve_cell_adjoint(dr: CellPressureStokesDR, mesh: MixedMesh, bc_vx, bc_vy, bc_vz, *, objective_v, lam_v, lam_P,
                   tolerance=1e-5, max_iterations=3000, check_interval=100, velocity_step=0.6, pressure_step=0.2,
                   verbose=False, collect_history=False, throw_on_failure=True):
     """solve_adjoint!(dr::CellPressureStokesDR, mesh, bc_v; objective_v, λv, λP, ...)

@@ -430,11 +430,13 @@ void cap_stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, con
 
 // Element momentum residual, src/stokes/assemblers/momentum_residuals.jl:136-189
 // (2-D with Pnum, T and gravity); accumulate_momentum :28-33; eos_density :11-16.
-// Generic in NV/NP/nq; dNdx [nq][NV][2], Nq [nq][NV], NqP [nq][NP].
-template <class S>
+// Generic in NV/NP/nq; dNdx [nq][NV][2], Nq [nq][NV], NqP [nq][NP]. SP is
+// double, or S to differentiate with respect to P (models without a yield
+// model only: the return map is evaluated at the value of P).
+template <class S, class SP = double>
 void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* dOm,
                       const double* Nq, const double* NqP, const S* vx, const S* vy,
-                      const double* P, const S* Pnum /*nullable*/, const double* T,
+                      const SP* P, const S* Pnum /*nullable*/, const double* T,
                       const double* Pf /*nullable*/, const int* ph, const Rheo& r,
                       const double* tau_old /*nullable [3][nq]*/, S* Rx, S* Ry,
                       double* tau_out /*nullable [3][nq]*/, double* Pq_out /*nullable [nq]*/) {
@@ -456,17 +458,17 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
         }
         double to[3] = {0, 0, 0};
         if (tau_old) { to[0] = tau_old[0 * nq + q]; to[1] = tau_old[1 * nq + q]; to[2] = tau_old[2 * nq + q]; }
-        double Pq = 0;
-        for (int i = 0; i < NP; ++i) Pq += Np[i] * P[i];
+        SP Pq = SP(0.0);
+        for (int i = 0; i < NP; ++i) Pq = Pq + Np[i] * P[i];
         double Pfq = 0;
         if (Pf) for (int i = 0; i < NP; ++i) Pfq += Np[i] * Pf[i];
-        const double Peq = Pq - Pfq;
+        const SP Peq = Pq - Pfq;
         S tau[3];
         S Pe = S(Peq);   // the cap corrects the pressure the momentum balance uses
         Rheo rq = r;
         if (r.eta_ip) rq.eta_pt = r.eta_ip[q];
-        if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, Peq, r.gamma ? r.gamma + q : nullptr, tau, Pe);
-        else stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, Peq, tau);
+        if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, val(Peq), r.gamma ? r.gamma + q : nullptr, tau, Pe);
+        else stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, val(Peq), tau);
         const S Pq_local = Pe + Pfq;
         if (tau_out) { tau_out[0 * nq + q] = val(tau[0]); tau_out[1 * nq + q] = val(tau[1]); tau_out[2 * nq + q] = val(tau[2]); }
         if (Pq_out) Pq_out[q] = val(Pq_local);
@@ -474,9 +476,9 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
         if (Pnum) { S pn = S(0.0); for (int i = 0; i < NP; ++i) pn = pn + Np[i] * Pnum[i]; Ptot = Ptot + pn; }
         double Tq = 0;
         for (int i = 0; i < NP; ++i) Tq += Np[i] * T[i];
-        const double rho = interp_phase(Nv, NV, m->rho0, ph) *
-                           (1 - interp_phase(Nv, NV, m->alpha, ph) * (Tq - m->Tref) +
-                            interp_phase(Nv, NV, beta, ph) * Pq);
+        const SP rho = interp_phase(Nv, NV, m->rho0, ph) *
+                       (1 - interp_phase(Nv, NV, m->alpha, ph) * (Tq - m->Tref) +
+                        interp_phase(Nv, NV, beta, ph) * Pq);
         for (int a = 0; a < NV; ++a) {
             Rx[a] = Rx[a] + ((G[a * 2 + 0] * (tau[0] - Ptot) + G[a * 2 + 1] * tau[2]) - Nv[a] * (rho * m->g[0])) * dOm[q];
             Ry[a] = Ry[a] + ((G[a * 2 + 0] * tau[2] + G[a * 2 + 1] * (tau[1] - Ptot)) - Nv[a] * (rho * m->g[1])) * dOm[q];
@@ -486,9 +488,10 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
 
 // Element continuity residual, src/stokes/assemblers/pressure_residual.jl:132-158;
 // divergence :18-30.  bulk = etab or K per phase (pressure_residual.jl:1-5).
-template <class S>
+// SP is double, or S to differentiate with respect to P.
+template <class S, class SP = double>
 void pressure_element(int NV, int NP, int nq, const double* dNdx, const double* dOmP,
-                      const double* NqP, const S* vx, const S* vy, const double* P,
+                      const double* NqP, const S* vx, const S* vy, const SP* P,
                       const double* P0, const double* T, const double* T0, const double* Q,
                       const int* phP, const double* alpha, const double* bulk, int nphases,
                       double dt, S* RP, const double* P0q = nullptr /* [nq]: P_old at the points */) {
@@ -502,15 +505,16 @@ void pressure_element(int NV, int NP, int nq, const double* dNdx, const double* 
         if (Q) for (int i = 0; i < NP; ++i) Qq += Nv[i] * Q[i];
         const double etabq = all_inf ? kInf : interp_phase(Nv, NP, bulk, phP);
         const double aq = interp_phase(Nv, NP, alpha, phP);
-        double dP = 0, dT = 0;
+        SP dP = SP(0.0);
+        double dT = 0;
         if (P0q) {   // pressure_increment_at_ip with IntegrationPointPressure (pressure_residual.jl:90-92)
-            for (int i = 0; i < NP; ++i) dP += Nv[i] * P[i];
-            dP -= P0q[q];
+            for (int i = 0; i < NP; ++i) dP = dP + Nv[i] * P[i];
+            dP = dP - P0q[q];
         } else {
-            for (int i = 0; i < NP; ++i) dP += Nv[i] * (P[i] - P0[i]);
+            for (int i = 0; i < NP; ++i) dP = dP + Nv[i] * (P[i] - P0[i]);
         }
         for (int i = 0; i < NP; ++i) dT += Nv[i] * (T[i] - T0[i]);
-        const double dPdt = dP / (etabq * dt);
+        const SP dPdt = dP / (etabq * dt);
         const double dTdt = aq * dT / dt;
         S div = S(0.0);
         for (int a = 0; a < NV; ++a) div = div + G[a * 2 + 0] * vx[a];
@@ -958,63 +962,75 @@ void orc_update_current_stress(orc_stokes* s, double dt) {
     momentum_residual_impl(s, dt, nullptr, s->tau_store);
 }
 
+}  // extern "C"
+
+// One element of the augmented Jacobian: block[i][j][row][col] = ∂R_i[row]/∂v_j[col]
+// with Pnum(v) = γP RP(v)/M_P inline (momentum_residuals.jl:929-970) and the
+// element-maximum viscosity for every phase (momentum_residuals.jl:36-44).
+static void augmented_element_block(const orc_stokes* s, int64_t e, double dt, const double* bulk,
+                                    double block[2][2][7][7]) {
+    const Tables& t = tables();
+    typedef Dual<7> D7;
+    double dNdx[7][7][2];
+    elem_dNdx(s, e, dNdx);
+    double vx[7], vy[7];
+    for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
+    int ph[7], phP[3];
+    elem_phases_v(s, e, ph);
+    elem_phases_P(s, e, phP);
+    // _element_max_phase_property: every phase gets the element max.
+    double emax = s->mat.eta[ph[0]];
+    for (int a = 1; a < 7; ++a) emax = std::max(emax, s->mat.eta[ph[a]]);
+    if (s->eta_ip) {   // a sampled field: maximum over the element's points (momentum_residuals.jl:43)
+        emax = s->eta_ip[e * 7];
+        for (int q = 1; q < 7; ++q) emax = std::max(emax, s->eta_ip[e * 7 + q]);
+    }
+    double eta_pc[ORC_MAX_PHASES];
+    for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = emax;
+    Rheo r{&s->mat, eta_pc, s->cap ? nullptr : s->dp, dt};
+    r.cap = s->cap;
+    if (s->cap && s->gamma_hist) r.gamma = s->gamma_hist + e * 7;
+    double to[21];
+    bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
+    for (int c = 0; c < 3; ++c)
+        for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
+    for (int j = 0; j < 2; ++j) {
+        D7 ux[7], uy[7];
+        for (int a = 0; a < 7; ++a) {
+            ux[a] = D7(vx[a]); uy[a] = D7(vy[a]);
+            if (j == 0) ux[a].d[a] = 1.0; else uy[a].d[a] = 1.0;
+        }
+        D7 RPe[3];
+        pressure_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], ux, uy,
+                             s->P + e * 3, s->P0 + e * 3, s->T + e * 3, s->T0 + e * 3,
+                             s->Q ? s->Q + e * 3 : nullptr, phP, s->mat.alpha, bulk, s->mat.nphases, dt, RPe);
+        D7 Pn[3];
+        for (int a = 0; a < 3; ++a) Pn[a] = s->gammaP[e * 3 + a] * RPe[a] / s->MP[e * 3 + a];
+        D7 Rx[7], Ry[7];
+        momentum_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], ux, uy,
+                             s->P + e * 3, Pn, s->T + e * 3, s->Pf ? s->Pf + e * 3 : nullptr, ph, r,
+                             has_old ? to : nullptr, Rx, Ry, nullptr, nullptr);
+        for (int row = 0; row < 7; ++row)
+            for (int col = 0; col < 7; ++col) {
+                block[0][j][row][col] = Rx[row].d[col];
+                block[1][j][row][col] = Ry[row].d[col];
+            }
+    }
+}
+
+extern "C" {
+
 // assemble_augmented_momentum_jacobian_matrices_atomix!: momentum_residuals.jl
 // :998-1033, kernel :1044-1066, problem :929-970, folding :1073-1117; the
 // element viscosity is the element maximum (:36-44). Each (i, j) block is the
 // forward-mode derivative of R_i with respect to v_j with Pnum(v) inline.
 void orc_augmented_jacobian(orc_stokes* s, double dt, int32_t finite_K) {
-    const Tables& t = tables();
     const double* bulk = pressure_bulk(s, finite_K);
     std::vector<double> Jx((size_t)s->nels * 7), Jy((size_t)s->nels * 7), Dx((size_t)s->nels * 7), Dy((size_t)s->nels * 7);
-    typedef Dual<7> D7;
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < s->nels; ++e) {
-        double dNdx[7][7][2];
-        elem_dNdx(s, e, dNdx);
-        double vx[7], vy[7];
-        for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
-        int ph[7], phP[3];
-        elem_phases_v(s, e, ph);
-        elem_phases_P(s, e, phP);
-        // _element_max_phase_property: every phase gets the element max.
-        double emax = s->mat.eta[ph[0]];
-        for (int a = 1; a < 7; ++a) emax = std::max(emax, s->mat.eta[ph[a]]);
-        if (s->eta_ip) {   // a sampled field: maximum over the element's points (momentum_residuals.jl:43)
-            emax = s->eta_ip[e * 7];
-            for (int q = 1; q < 7; ++q) emax = std::max(emax, s->eta_ip[e * 7 + q]);
-        }
-        double eta_pc[ORC_MAX_PHASES];
-        for (int p = 0; p < s->mat.nphases; ++p) eta_pc[p] = emax;
-        Rheo r{&s->mat, eta_pc, s->cap ? nullptr : s->dp, dt};
-        r.cap = s->cap;
-        if (s->cap && s->gamma_hist) r.gamma = s->gamma_hist + e * 7;
-        double to[21];
-        bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
-        for (int c = 0; c < 3; ++c)
-            for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
         double block[2][2][7][7];  // [i][j][row][col] = dR_i[row]/dv_j[col]
-        for (int j = 0; j < 2; ++j) {
-            D7 ux[7], uy[7];
-            for (int a = 0; a < 7; ++a) {
-                ux[a] = D7(vx[a]); uy[a] = D7(vy[a]);
-                if (j == 0) ux[a].d[a] = 1.0; else uy[a].d[a] = 1.0;
-            }
-            D7 RPe[3];
-            pressure_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], ux, uy,
-                                 s->P + e * 3, s->P0 + e * 3, s->T + e * 3, s->T0 + e * 3,
-                                 s->Q ? s->Q + e * 3 : nullptr, phP, s->mat.alpha, bulk, s->mat.nphases, dt, RPe);
-            D7 Pn[3];
-            for (int a = 0; a < 3; ++a) Pn[a] = s->gammaP[e * 3 + a] * RPe[a] / s->MP[e * 3 + a];
-            D7 Rx[7], Ry[7];
-            momentum_element<D7>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], ux, uy,
-                                 s->P + e * 3, Pn, s->T + e * 3, s->Pf ? s->Pf + e * 3 : nullptr, ph, r,
-                                 has_old ? to : nullptr, Rx, Ry, nullptr, nullptr);
-            for (int row = 0; row < 7; ++row)
-                for (int col = 0; col < 7; ++col) {
-                    block[0][j][row][col] = Rx[row].d[col];
-                    block[1][j][row][col] = Ry[row].d[col];
-                }
-        }
+        augmented_element_block(s, e, dt, bulk, block);
         for (int row = 0; row < 7; ++row) {
             double sx = 0, sy = 0;
             for (int j = 0; j < 2; ++j) {
@@ -2752,3 +2768,397 @@ static void thermal_residual_q9(orc_thermal* t, double dt, double Tref, int32_t 
         }
     }
 }
+
+// ===========================================================================
+// 2-D: the frozen-operator adjoint of StokesDR (operator = :blocks)
+// ===========================================================================
+namespace {
+
+constexpr int ANU = 14, ANP = 3;
+
+// _symmetric_tangent (adjoint_operator.jl:172-182): no yield model and K = Inf
+bool adjoint_symmetric(const orc_stokes* s) {
+    if (s->dp || s->cap) return false;
+    for (int p = 0; p < s->mat.nphases; ++p)
+        if (!std::isinf(s->mat.K[p])) return false;
+    return true;
+}
+// D and W are kept only where some phase has a finite bulk viscosity (:455-458)
+bool adjoint_stores(const orc_stokes* s) {
+    for (int p = 0; p < s->mat.nphases; ++p)
+        if (std::isfinite(s->mat.etab[p])) return true;
+    return false;
+}
+
+// element_adjoint_operator_blocks (adjoint_operator.jl:318-373): A is the
+// augmented Jacobian (the element-maximum viscosity, Pnum inline with ηb);
+// B, C, D and ∂Rv/∂Pnum are forward-mode derivatives of the element residuals
+void adjoint_element(const orc_stokes* s, int64_t e, double dt, double A[ANU][ANU], double B[ANU][ANP],
+                     double C[ANP][ANU], double D[ANP][ANP], double W[ANU][ANP]) {
+    const Tables& t = tables();
+    double block[2][2][7][7];
+    augmented_element_block(s, e, dt, s->mat.etab, block);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 7; ++r)
+                for (int c = 0; c < 7; ++c) A[i * 7 + r][j * 7 + c] = block[i][j][r][c];
+    double dNdx[7][7][2];
+    elem_dNdx(s, e, dNdx);
+    int ph[7], phP[3];
+    elem_phases_v(s, e, ph);
+    elem_phases_P(s, e, phP);
+    double vx[7], vy[7];
+    for (int a = 0; a < 7; ++a) { vx[a] = s->vx[s->el2n[e * 7 + a]]; vy[a] = s->vy[s->el2n[e * 7 + a]]; }
+    double to[21];
+    const bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
+    for (int c = 0; c < 3; ++c)
+        for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
+    Rheo r{&s->mat, s->mat.eta, nullptr, dt};
+    r.eta_ip = s->eta_ip ? s->eta_ip + e * 7 : nullptr;
+    const double* Pf = s->Pf ? s->Pf + e * 3 : nullptr;
+    typedef Dual<3> D3;
+    D3 ux[7], uy[7];
+    for (int a = 0; a < 7; ++a) { ux[a] = D3(vx[a]); uy[a] = D3(vy[a]); }
+    {   // B = ∂Rv/∂P with Pnum = 0 held as an independent variable
+        D3 Pd[3], Pn[3];
+        for (int i = 0; i < 3; ++i) { Pd[i] = D3(s->P[e * 3 + i]); Pd[i].d[i] = 1.0; }
+        D3 Rx[7], Ry[7];
+        momentum_element<D3, D3>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], ux, uy, Pd,
+                                 Pn, s->T + e * 3, Pf, ph, r, has_old ? to : nullptr, Rx, Ry, nullptr, nullptr);
+        for (int a = 0; a < 7; ++a)
+            for (int i = 0; i < 3; ++i) { B[a][i] = Rx[a].d[i]; B[7 + a][i] = Ry[a].d[i]; }
+    }
+    {   // W = ∂Rv/∂Pnum column-scaled by Γ = γP/M_P
+        D3 Pn[3];
+        for (int i = 0; i < 3; ++i) Pn[i].d[i] = 1.0;
+        D3 Rx[7], Ry[7];
+        momentum_element<D3>(7, 3, 7, &dNdx[0][0][0], s->dOmega + e * 7, &t.Nq[0][0], &t.NqP[0][0], ux, uy,
+                             s->P + e * 3, Pn, s->T + e * 3, Pf, ph, r, has_old ? to : nullptr, Rx, Ry, nullptr,
+                             nullptr);
+        for (int i = 0; i < 3; ++i) {
+            const double G = s->gammaP[e * 3 + i] / s->MP[e * 3 + i];
+            for (int a = 0; a < 7; ++a) { W[a][i] = Rx[a].d[i] * G; W[7 + a][i] = Ry[a].d[i] * G; }
+        }
+    }
+    {   // C = ∂RP/∂v
+        typedef Dual<ANU> DU;
+        DU wx[7], wy[7];
+        for (int a = 0; a < 7; ++a) {
+            wx[a] = DU(vx[a]); wx[a].d[a] = 1.0;
+            wy[a] = DU(vy[a]); wy[a].d[7 + a] = 1.0;
+        }
+        DU RP[3];
+        pressure_element<DU>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], wx, wy, s->P + e * 3,
+                             s->P0 + e * 3, s->T + e * 3, s->T0 + e * 3, nullptr, phP, s->mat.alpha, s->mat.etab,
+                             s->mat.nphases, dt, RP);
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < ANU; ++k) C[i][k] = RP[i].d[k];
+    }
+    {   // D = ∂RP/∂P: the storage term −∫ N_i N_j /(ηb Δt)
+        D3 Pd[3];
+        for (int i = 0; i < 3; ++i) { Pd[i] = D3(s->P[e * 3 + i]); Pd[i].d[i] = 1.0; }
+        D3 RP[3];
+        pressure_element<D3, D3>(7, 3, 7, &dNdx[0][0][0], s->dOmegaP + e * 7, &t.NqP[0][0], ux, uy, Pd,
+                                 s->P0 + e * 3, s->T + e * 3, s->T0 + e * 3, nullptr, phP, s->mat.alpha,
+                                 s->mat.etab, s->mat.nphases, dt, RP);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) D[i][j] = RP[i].d[j];
+    }
+}
+
+// the velocity part of one element's apply: Aᵀλ (the packed upper triangle when
+// symmetric, _symmetric_matvec) plus the pressure coupling (B when symmetric,
+// else Cᵀ), and its pressure part Bᵀλv + Dᵀ(Wᵀλv + λP)
+void adjoint_element_apply(const double* A, const double* B, const double* C, const double* D, const double* W,
+                           bool sym, bool stores, const double lv[ANU], const double lp[ANP], double resv[ANU],
+                           double resp[ANP]) {
+    for (int r = 0; r < ANU; ++r) {
+        double a = 0, b = 0;
+        for (int k = 0; k < ANU; ++k)
+            a += (sym ? A[std::min(r, k) * ANU + std::max(r, k)] : A[k * ANU + r]) * lv[k];
+        for (int i = 0; i < ANP; ++i) b += (sym ? B[r * ANP + i] : C[i * ANU + r]) * lp[i];
+        resv[r] = a + b;
+    }
+    for (int i = 0; i < ANP; ++i) {
+        double b = 0;
+        for (int k = 0; k < ANU; ++k) b += B[k * ANP + i] * lv[k];
+        double st = 0;
+        if (stores) {
+            double w[ANP];
+            for (int j = 0; j < ANP; ++j) {
+                double acc = 0;
+                for (int k = 0; k < ANU; ++k) acc += W[k * ANP + j] * lv[k];
+                w[j] = acc + lp[j];
+            }
+            for (int j = 0; j < ANP; ++j) st += D[j * ANP + i] * w[j];
+        }
+        resp[i] = b + st;
+    }
+}
+
+void adjoint_apply_impl(const orc_stokes* s, const double* A, const double* B, const double* C, const double* D,
+                        const double* W, bool sym, bool stores, const double* lx, const double* ly, const double* lP,
+                        double* dvx, double* dvy, double* dP) {
+    std::vector<double> Rx((size_t)s->nels * 7), Ry((size_t)s->nels * 7);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double lv[ANU], lp[ANP], rv[ANU], rp[ANP];
+        for (int a = 0; a < 7; ++a) { lv[a] = lx[s->el2n[e * 7 + a]]; lv[7 + a] = ly[s->el2n[e * 7 + a]]; }
+        for (int i = 0; i < ANP; ++i) lp[i] = lP[e * 3 + i];
+        adjoint_element_apply(A + e * ANU * ANU, B + e * ANU * ANP, C ? C + e * ANP * ANU : nullptr,
+                              D ? D + e * ANP * ANP : nullptr, W ? W + e * ANU * ANP : nullptr, sym, stores, lv, lp,
+                              rv, rp);
+        for (int a = 0; a < 7; ++a) { Rx[e * 7 + a] = rv[a]; Ry[e * 7 + a] = rv[7 + a]; }
+        for (int i = 0; i < ANP; ++i) dP[e * 3 + i] = rp[i];   // discontinuous pressure: unshared
+    }
+    gather(s, Rx, dvx);
+    gather(s, Ry, dvy);
+}
+
+double power_seed(int64_t i, int d) { return d == 1 ? std::sin(0.7 * i) : d == 2 ? std::cos(1.3 * i) : std::sin(1.9 * i); }
+
+}  // namespace
+
+extern "C" {
+
+int orc_adjoint_blocks(orc_stokes* s, double dt, double* A, double* B, double* C, double* D, double* W,
+                       double* defects) {
+    g_err.clear();
+    if (s->dp || s->cap) {
+        g_err = "the frozen-operator adjoint covers models without a yield model";
+        return -1;
+    }
+    std::vector<double> Jx((size_t)s->nels * 7), Jy((size_t)s->nels * 7), Dx((size_t)s->nels * 7),
+        Dy((size_t)s->nels * 7), dA(s->nels), dC(s->nels);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double Ae[ANU][ANU], Be[ANU][ANP], Ce[ANP][ANU], De[ANP][ANP], We[ANU][ANP];
+        adjoint_element(s, e, dt, Ae, Be, Ce, De, We);
+        std::memcpy(A + e * ANU * ANU, Ae, sizeof Ae);
+        std::memcpy(B + e * ANU * ANP, Be, sizeof Be);
+        std::memcpy(C + e * ANP * ANU, Ce, sizeof Ce);
+        std::memcpy(D + e * ANP * ANP, De, sizeof De);
+        std::memcpy(W + e * ANU * ANP, We, sizeof We);
+        double nA = 0, nAs = 0, nC = 0, nCs = 0;
+        for (int i = 0; i < ANU; ++i)
+            for (int j = 0; j < ANU; ++j) { nA += Ae[i][j] * Ae[i][j]; nAs += sq(Ae[i][j] - Ae[j][i]); }
+        for (int i = 0; i < ANP; ++i)
+            for (int k = 0; k < ANU; ++k) { nC += Ce[i][k] * Ce[i][k]; nCs += sq(Ce[i][k] - Be[k][i]); }
+        dA[e] = nA == 0 ? 0.0 : std::sqrt(nAs) / std::sqrt(nA);
+        dC[e] = nC == 0 ? 0.0 : std::sqrt(nCs) / std::sqrt(nC);
+        // _scatter_velocity_diagnostics! (adjoint_operator.jl:390-403)
+        for (int a = 0; a < 7; ++a) {
+            double sx = 0, sy = 0;
+            for (int k = 0; k < ANU; ++k) { sx += std::fabs(Ae[a][k]); sy += std::fabs(Ae[7 + a][k]); }
+            Jx[e * 7 + a] = sx; Jy[e * 7 + a] = sy;
+            Dx[e * 7 + a] = std::fabs(Ae[a][a]);
+            Dy[e * 7 + a] = std::fabs(Ae[7 + a][7 + a]);
+        }
+    }
+    gather(s, Jx, s->dRdvx);
+    gather(s, Jy, s->dRdvy);
+    gather(s, Dx, s->PCx);
+    gather(s, Dy, s->PCy);
+    if (defects) {
+        defects[0] = defects[1] = 0;
+        for (int64_t e = 0; e < s->nels; ++e) {
+            defects[0] = std::max(defects[0], dA[e]);
+            defects[1] = std::max(defects[1], dC[e]);
+        }
+    }
+    return 0;
+}
+
+void orc_adjoint_apply(const orc_stokes* s, const double* A, const double* B, const double* C, const double* D,
+                       const double* W, int32_t symmetric, int32_t stores, const double* lx, const double* ly,
+                       const double* lP, double* dvx, double* dvy, double* dP) {
+    adjoint_apply_impl(s, A, B, C, D, W, symmetric != 0, stores != 0, lx, ly, lP, dvx, dvy, dP);
+}
+
+int orc_adjoint_solve(orc_stokes* s, const orc_adjoint_params* p, const double* obj_x, const double* obj_y,
+                      double* lam_x, double* lam_y, double* lam_P, orc_adjoint_stats* st) {
+    std::memset(st, 0, sizeof *st);
+    g_err.clear();
+    const int64_t nv = s->nnodes, np = s->nnodesP, ne = s->nels;
+    bool assembled = false;
+    for (int64_t i = 0; i < np && !assembled; ++i) assembled = s->gammaP[i] != 0;
+    if (!assembled) {
+        g_err = "dr.γP is unassembled; run the forward solve! on this state before solve_adjoint!";
+        st->status = -1;
+        return -1;
+    }
+    const bool sym = adjoint_symmetric(s), stores = adjoint_stores(s);
+    std::vector<double> A((size_t)ne * ANU * ANU), B((size_t)ne * ANU * ANP), C((size_t)ne * ANP * ANU),
+        D((size_t)ne * ANP * ANP), W((size_t)ne * ANU * ANP);
+    double defects[2];
+    if (orc_adjoint_blocks(s, p->dt, A.data(), B.data(), C.data(), D.data(), W.data(), defects) != 0) {
+        st->status = -1;
+        return -1;
+    }
+    // _assert_frozen_symmetry (adjoint_operator.jl:149-160)
+    const double tol_sym = std::sqrt(std::numeric_limits<double>::epsilon());
+    if (sym && (defects[0] > tol_sym || defects[1] > tol_sym)) {
+        char buf[320];
+        std::snprintf(buf, sizeof buf,
+                      "the frozen adjoint operator assumes the element tangent is symmetric, but the assembled blocks "
+                      "disagree: max ‖A - Aᵀ‖/‖A‖ = %g, max ‖C - Bᵀ‖/‖C‖ = %g over %lld elements",
+                      defects[0], defects[1], (long long)ne);
+        g_err = buf;
+        st->status = -1;
+        return -1;
+    }
+    auto apply = [&](const double* lx, const double* ly, const double* lP, double* dvx, double* dvy, double* dP) {
+        adjoint_apply_impl(s, A.data(), B.data(), sym ? nullptr : C.data(), stores ? D.data() : nullptr,
+                           stores ? W.data() : nullptr, sym, stores, lx, ly, lP, dvx, dvy, dP);
+    };
+    double* PC[2] = {s->PCx, s->PCy};
+    double lmax_c[2];
+    const char* labels[2] = {"adjoint vx", "adjoint vy"};
+    for (int c = 0; c < 2; ++c)
+        if (!checked_lmax(c == 0 ? s->dRdvx : s->dRdvy, PC[c], nv, labels[c], &lmax_c[c])) {
+            st->status = -6;
+            return -6;
+        }
+    st->lmax_gershgorin = std::max(lmax_c[0], lmax_c[1]);
+    double lmax_v[2] = {lmax_c[0], lmax_c[1]};
+    std::vector<double> x[2], z[2], y[2], zP(np, 0.0), dPs(np);
+    auto project = [&](std::vector<double>* u) {
+        for (int c = 0; c < 2; ++c) dirichlet(u[c].data(), s->nbc[c], s->bc_dofs[c], nullptr);
+    };
+    auto stacked_norm = [&](std::vector<double>* u) {
+        double a = 0, b = 0;
+        for (int64_t i = 0; i < nv; ++i) a += u[0][i] * u[0][i];
+        for (int64_t i = 0; i < nv; ++i) b += u[1][i] * u[1][i];
+        return std::sqrt(a + b);
+    };
+    if (p->measure_lmax) {
+        // estimate_adjoint_λmax (adjoint_operator.jl:564-607), 100 iterations, rtol 1e-3
+        for (int c = 0; c < 2; ++c) {
+            x[c].resize(nv); z[c].resize(nv); y[c].resize(nv);
+            for (int64_t i = 0; i < nv; ++i) x[c][i] = power_seed(i + 1, c + 1);
+        }
+        project(x);
+        double lam = 0;
+        int64_t its = 0;
+        for (int64_t it = 1; it <= 100; ++it) {
+            its = it;
+            const double nx = stacked_norm(x);
+            if (!(nx > 0)) {
+                g_err = "power iteration collapsed to the zero vector";
+                st->status = -1;
+                return -1;
+            }
+            for (int c = 0; c < 2; ++c)
+                for (int64_t i = 0; i < nv; ++i) x[c][i] /= nx;
+            for (int c = 0; c < 2; ++c)
+                for (int64_t i = 0; i < nv; ++i) z[c][i] = x[c][i] / std::sqrt(PC[c][i]);
+            apply(z[0].data(), z[1].data(), zP.data(), y[0].data(), y[1].data(), dPs.data());
+            for (int c = 0; c < 2; ++c)
+                for (int64_t i = 0; i < nv; ++i) y[c][i] /= std::sqrt(PC[c][i]);
+            project(y);
+            const double ln = stacked_norm(y);
+            x[0] = y[0];
+            x[1] = y[1];
+            const bool conv = it > 1 && std::fabs(ln - lam) <= 1e-3 * ln;
+            lam = ln;
+            if (conv) break;
+        }
+        if (!(std::isfinite(lam) && lam > 0)) {
+            char buf[128];
+            std::snprintf(buf, sizeof buf, "power iteration produced a non-positive λmax: %g", lam);
+            g_err = buf;
+            st->status = -1;
+            return -1;
+        }
+        lmax_v[0] = lmax_v[1] = lam;
+        st->lmax_iterations = its;
+    }
+    st->lmax = lmax_v[0];
+    double dtau[2], al[2], be[2];
+    for (int c = 0; c < 2; ++c) {
+        dtau[c] = 2 / std::sqrt(lmax_v[c]) * p->CFL_v;
+        stokes_cheb(dtau[c], 0.0, p->c_fact, &al[c], &be[c]);
+    }
+    double* lam[2] = {lam_x, lam_y};
+    const double* obj[2] = {obj_x, obj_y};
+    std::vector<double> R[2], R0[2], rate[2], dv[2], RP(np);
+    for (int c = 0; c < 2; ++c) {
+        R[c].assign(nv, 0.0); R0[c].assign(nv, 0.0); rate[c].assign(nv, 0.0); dv[c].assign(nv, 0.0);
+    }
+    auto assemble = [&]() {
+        apply(lam_x, lam_y, lam_P, dv[0].data(), dv[1].data(), RP.data());
+        for (int c = 0; c < 2; ++c) {
+            for (int64_t i = 0; i < nv; ++i) R[c][i] = obj[c][i] + dv[c][i];
+            dirichlet(R[c].data(), s->nbc[c], s->bc_dofs[c], nullptr);
+        }
+    };
+    auto velocity_error = [&]() {
+        return std::max(nrm2(R[0].data(), nv), nrm2(R[1].data(), nv)) / std::sqrt((double)nv);
+    };
+    int64_t iter = 0, itPH_done = 0;
+    double err = kInf, err_v0 = 1, err_P0 = 1, err_v = kInf, err_P = kInf;
+    bool converged = false;
+    for (int64_t itPH = 1; itPH <= p->max_ph_iterations; ++itPH) {
+        itPH_done = itPH;
+        assemble();
+        err_v = velocity_error();
+        err_P = nrm2(RP.data(), np) / std::sqrt((double)np);
+        if (itPH == 1) {
+            err_v0 = err_v + julia_eps(err_v);
+            err_P0 = err_P + julia_eps(err_P);
+        }
+        err = std::max(std::min(err_v, err_v / err_v0), std::min(err_P, err_P / err_P0));
+        if (p->verbose)
+            printf("adj PH=%03lld iter=%06lld err=%.3e Rv=%.3e RP=%.3e\n", (long long)itPH, (long long)iter, err,
+                   err_v, err_P);
+        if (err < p->tolerance) {
+            converged = true;
+            break;
+        }
+        const double target = std::max(err_v * p->rel_drop, p->tolerance);
+        int64_t inner = 0;
+        while (err_v > target && inner < p->iter_max && iter < p->max_iterations) {
+            ++inner;
+            ++iter;
+            for (int c = 0; c < 2; ++c) R0[c] = R[c];
+            for (int c = 0; c < 2; ++c) {
+                const double a = -al[c], b = be[c];
+                for (int64_t i = 0; i < nv; ++i) {
+                    const double nr = b * rate[c][i] + R[c][i] / PC[c][i];
+                    rate[c][i] = nr;
+                    lam[c][i] += a * nr;
+                }
+            }
+            for (int c = 0; c < 2; ++c) {
+                dirichlet(lam[c], s->nbc[c], s->bc_dofs[c], nullptr);
+                dirichlet(rate[c].data(), s->nbc[c], s->bc_dofs[c], nullptr);
+            }
+            assemble();
+            if (iter % p->check_interval == 0) {
+                err_v = velocity_error();
+                for (int c = 0; c < 2; ++c) {
+                    const double lmin = stokes_lmin(al[c], rate[c].data(), R[c].data(), R0[c].data(), PC[c], nv);
+                    stokes_cheb(dtau[c], lmin, p->c_fact, &al[c], &be[c]);
+                }
+            }
+        }
+        for (int64_t i = 0; i < np; ++i) lam_P[i] += s->gammaP[i] * RP[i] / s->MP[i];   // Arrow-Hurwicz
+        if (iter >= p->max_iterations) break;
+    }
+    if (!converged) {
+        assemble();
+        err_v = velocity_error();
+        err_P = nrm2(RP.data(), np) / std::sqrt((double)np);
+        err = std::max(std::min(err_v, err_v / err_v0), std::min(err_P, err_P / err_P0));
+    }
+    st->itPH = itPH_done;
+    st->iter = iter;
+    st->err = err;
+    st->err_v = err_v;
+    st->err_P = err_P;
+    st->converged = converged || err < p->tolerance;
+    st->status = 0;
+    return 0;
+}
+
+}  // extern "C"

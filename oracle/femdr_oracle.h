@@ -342,6 +342,43 @@ void orc_thermal_residual(orc_thermal* t, double dt, double Tref, int32_t comput
  * (heat_diffusion/solvers/DR.jl:50-54, dynamic_relaxation/solver.jl:61-122) */
 int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_stats* out);
 
+/* --- frozen-operator adjoint of StokesDR (src/stokes/solvers/DR_adjoint.jl:1-531,
+ * src/stokes/assemblers/adjoint_operator.jl:1-700), operator = :blocks, 2-D T7/P1-disc,
+ * models without a yield model --- */
+typedef struct {
+    double dt, tolerance, rel_drop, CFL_v, c_fact;  /* tolerance 1e-6, rel_drop 0.1 */
+    int64_t max_iterations;     /* total_iterMax, 50_000 */
+    int64_t iter_max;           /* iterMax (inner), 50_000 */
+    int64_t check_interval;     /* ncheck, 50 */
+    int64_t max_ph_iterations;  /* 100 */
+    int32_t measure_lmax;       /* 1: power iteration (estimate_adjoint_λmax), 0: Gershgorin */
+    int32_t verbose;
+} orc_adjoint_params;
+typedef struct {
+    int64_t itPH, iter, lmax_iterations;
+    double err, err_v, err_P, lmax, lmax_gershgorin;
+    int32_t converged, status;
+} orc_adjoint_stats;
+/* Element blocks (element_adjoint_operator_blocks, adjoint_operator.jl:318-373) at
+ * the state in s: A [nels][14][14] = ∂Rv/∂v with the augmentation, B [nels][14][3]
+ * = ∂Rv/∂P (Pnum held), C [nels][3][14] = ∂RP/∂v, D [nels][3][3] = ∂RP/∂P (ηb
+ * storage), W [nels][14][3] = ∂Rv/∂Pnum diag(γP/M_P); velocity DoFs stacked by
+ * component (vx 0..6, vy 7..13). defects (nullable [2]) = max over elements of
+ * ‖A−Aᵀ‖/‖A‖ and ‖C−Bᵀ‖/‖C‖. Also fills s->dRdvx/y and s->PCx/y from A.
+ * Returns -1 (orc_last_error) for a state with a yield model. */
+int orc_adjoint_blocks(orc_stokes* s, double dt, double* A, double* B, double* C, double* D, double* W,
+                       double* defects);
+/* apply_adjoint_operator! (adjoint_operator.jl:631-660): dv = Aᵀλv + CᵀλP (B λP and the
+ * packed upper triangle of A when symmetric), dP = Bᵀλv + Dᵀ(Wᵀλv + λP) (Bᵀλv alone
+ * unless stores) */
+void orc_adjoint_apply(const orc_stokes* s, const double* A, const double* B, const double* C, const double* D,
+                       const double* W, int32_t symmetric, int32_t stores, const double* lx, const double* ly,
+                       const double* lP, double* dvx, double* dvy, double* dP);
+/* _solve_stokes_adjoint_dyrel! (DR_adjoint.jl:1-268) with the stored blocks; objective
+ * and λ are nodal [nnodes] (x, y) and [nnodesP]; λ holds the initial iterate. */
+int orc_adjoint_solve(orc_stokes* s, const orc_adjoint_params* p, const double* obj_x, const double* obj_y,
+                      double* lam_x, double* lam_y, double* lam_P, orc_adjoint_stats* st);
+
 /* --- Q9: QuadraticElement{2, 9} (shape_functions.jl:259-283, integration_points.jl:218-241) --- */
 /* N [9], dN [9][2] at (ξ, η); local nodes (-1,-1),(1,-1),(1,1),(-1,1),(0,-1),(1,0),(0,1),(-1,0),(0,0) */
 void orc_q9_basis(double xi, double eta, double* N, double* dN);

@@ -112,6 +112,17 @@ class ThermalStatsC(C.Structure):
     _fields_ = [("converged", I32), ("iterations", I64), ("residual", D), ("status", I32)]
 
 
+class AdjointParamsC(C.Structure):
+    _fields_ = [("dt", D), ("tolerance", D), ("rel_drop", D), ("CFL_v", D), ("c_fact", D), ("max_iterations", I64),
+                ("iter_max", I64), ("check_interval", I64), ("max_ph_iterations", I64), ("measure_lmax", I32),
+                ("verbose", I32)]
+
+
+class AdjointStatsC(C.Structure):
+    _fields_ = [("itPH", I64), ("iter", I64), ("lmax_iterations", I64), ("err", D), ("err_v", D), ("err_P", D),
+                ("lmax", D), ("lmax_gershgorin", D), ("converged", I32), ("status", I32)]
+
+
 class CP3Params(C.Structure):
     _fields_ = [("tolerance", D), ("max_iterations", I64), ("check_interval", I64), ("velocity_step", D),
                 ("pressure_step", D), ("verbose", I32)]
@@ -141,6 +152,10 @@ def _declare(L):
     L.orc_hex27_mesh_sizes.argtypes = [I32, I32, I32, C.POINTER(I64), C.POINTER(I64)]
     L.orc_hex27_mesh.argtypes = [I32, I32, I32, D, D, D, D, D, D, PD, PI]
     L.orc_hex27_geometry.argtypes = [I64, PD, PI, PD, PD]
+    L.orc_adjoint_blocks.argtypes = [C.POINTER(Stokes), D, PD, PD, PD, PD, PD, PD]
+    L.orc_adjoint_apply.argtypes = [C.POINTER(Stokes), PD, PD, PD, PD, PD, I32, I32, PD, PD, PD, PD, PD, PD]
+    L.orc_adjoint_solve.argtypes = [C.POINTER(Stokes), C.POINTER(AdjointParamsC), PD, PD, PD, PD, PD,
+                                    C.POINTER(AdjointStatsC)]
     L.orc_q9_basis.argtypes = [D, D, PD, PD]
     L.orc_q9_rule.argtypes = [PD, PD, PD]
     L.orc_q9_mesh_sizes.argtypes = [I32, I32, C.POINTER(I64), C.POINTER(I64)]
@@ -472,6 +487,61 @@ class Problem:
         phi = np.zeros((self.nels, 7))
         lib().orc_shear_heating(C.byref(self.struct()), dt, ptr(phi))
         return phi
+
+    # the frozen-operator adjoint (DR_adjoint.jl, adjoint_operator.jl) ------------
+    def adjoint_flags(self):
+        """(symmetric, stores) of the block layout (adjoint_operator.jl:172-182, 455-458)."""
+        m = self.mat
+        n = m.nphases
+        sym = self.dp is None and self.cap is None and all(math.isinf(m.K[i]) for i in range(n))
+        stores = any(math.isfinite(m.etab[i]) for i in range(n))
+        return sym, stores
+
+    def adjoint_blocks(self, dt):
+        """Element blocks A, B, C, D, W ([nels][14][14], [14][3], [3][14], [3][3],
+        [14][3]) and the symmetry defects; fills dRdvx/y and PCx/y."""
+        ne = self.nels
+        A, B, Cm = np.zeros((ne, 14, 14)), np.zeros((ne, 14, 3)), np.zeros((ne, 3, 14))
+        Dm, W, df = np.zeros((ne, 3, 3)), np.zeros((ne, 14, 3)), np.zeros(2)
+        if lib().orc_adjoint_blocks(C.byref(self.struct()), dt, ptr(A), ptr(B), ptr(Cm), ptr(Dm), ptr(W), ptr(df)):
+            raise ValueError(lib().orc_last_error().decode())
+        return dict(A=A, B=B, C=Cm, D=Dm, W=W, defects=df)
+
+    def adjoint_apply(self, blocks, lx, ly, lP):
+        sym, stores = self.adjoint_flags()
+        dvx, dvy, dP = np.zeros(self.nnodes), np.zeros(self.nnodes), np.zeros(self.nnodesP)
+        b = {k: f64(v) for k, v in blocks.items()}
+        lib().orc_adjoint_apply(C.byref(self.struct()), ptr(b["A"]), ptr(b["B"]), ptr(b["C"]), ptr(b["D"]),
+                                ptr(b["W"]), int(sym), int(stores), ptr(f64(lx)), ptr(f64(ly)), ptr(f64(lP)),
+                                ptr(dvx), ptr(dvy), ptr(dP))
+        return dvx, dvy, dP
+
+    def solve_adjoint(self, dt, objective, lam, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
+                      check_interval=50, max_ph_iterations=100, rel_drop=0.1, CFL_v=0.98, c_fact=0.9,
+                      measure_lmax=True, verbose=False, throw_on_failure=True):
+        """solve_adjoint! (DR_adjoint.jl:396-427) with operator = :blocks. objective =
+        (∂J/∂vx, ∂J/∂vy); lam = [λvx, λvy, λP] (float64, updated in place from
+        their values as the initial iterate). Returns the statistics."""
+        p = AdjointParamsC(dt=dt, tolerance=tolerance, rel_drop=rel_drop, CFL_v=CFL_v, c_fact=c_fact,
+                           max_iterations=max_iterations, iter_max=iter_max, check_interval=check_interval,
+                           max_ph_iterations=max_ph_iterations, measure_lmax=int(bool(measure_lmax)),
+                           verbose=int(bool(verbose)))
+        st = AdjointStatsC()
+        ox, oy = f64(objective[0]), f64(objective[1])
+        for a in lam:
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        rc = lib().orc_adjoint_solve(C.byref(self.struct()), C.byref(p), ptr(ox), ptr(oy), ptr(lam[0]), ptr(lam[1]),
+                                     ptr(lam[2]), C.byref(st))
+        if rc != 0:
+            msg = lib().orc_last_error().decode()
+            raise (ValueError(msg) if rc == -1 else RuntimeError(msg))
+        out = {k: getattr(st, k) for k, _ in AdjointStatsC._fields_}
+        out["converged"] = bool(out["converged"])
+        out["iterations"], out["residual"] = out["iter"], out["err"]
+        if throw_on_failure and not out["converged"]:
+            raise RuntimeError(f"Stokes adjoint solve did not converge after {out['iter']} iterations "
+                               f"(residual = {out['err']}, tolerance = {tolerance})")
+        return out
 
     @staticmethod
     def params(dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,

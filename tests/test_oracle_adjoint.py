@@ -1,0 +1,190 @@
+"""The oracle's frozen-operator adjoint of StokesDR (solve_adjoint! with
+operator = :blocks, src/stokes/solvers/DR_adjoint.jl:1-531; the element blocks
+and their apply, src/stokes/assemblers/adjoint_operator.jl:1-660), pinned by:
+
+* the reference's own gradient test (test/test_stokes_adjoint_api.jl:9-198):
+  on its 3x3 T7/P1 buoyancy case the adjoint gradient λᵀ∂R/∂ρ₂ equals the
+  central finite difference of the forward solves' objective (rtol 1e-4), the
+  measured λmax is below the Gershgorin bound, a converged λ is a fixed point
+  of a warm-started solve, and an unreachable budget fails;
+* a dot-product test of the operator against central differences of the
+  augmented residual (Rv with Pnum = γP RP/M_P inline, RP): for any δu and λ,
+  λ·(R(u + hδu) − R(u − hδu))/2h = δu·(op λ); the residual is affine in u for
+  models without a yield model, so the match is to rounding (finite ηb and
+  finite K included: the storage blocks D, W and the density term of B);
+* the velocity block's row sums and diagonal equal the augmented Jacobian
+  diagnostics of the forward solver, and the packed symmetric layout applies
+  the same operator as the full blocks where K = Inf.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+INF = math.inf
+
+
+def buoyancy_case():
+    """The reference's _adjoint_gradient_case (test_stokes_adjoint_api.jl:13-46)."""
+    coords, el2n = O.t7_mesh(3, 3)
+    cen = coords[el2n[:, :3]].mean(axis=1)
+    ph = np.where((cen[:, 1] < 0.5) & (np.abs(cen[:, 0] - 0.5) < 0.34), 2, 1).astype(np.int32)
+    gam = O.box_boundary_nodes(coords)
+    tol = 1e-9
+    xw = gam[(np.abs(coords[gam, 0]) <= tol) | (np.abs(coords[gam, 0] - 1) <= tol)]
+    yw = gam[(np.abs(coords[gam, 1]) <= tol) | (np.abs(coords[gam, 1] - 1) <= tol)]
+    obs = np.nonzero(coords[:, 1] > 0.6)[0]
+    return coords, el2n, ph, xw.astype(np.int32), yw.astype(np.int32), obs
+
+
+FWD = dict(pressure_factor=20.0, tolerance=1e-10, max_iterations=200_000, iter_max=200_000, check_interval=100,
+           rel_drop0=0.1, CFL_v=0.9, c_fact=0.7)
+ADJ = dict(check_interval=100, tolerance=1e-10, rel_drop=0.1, iter_max=200_000, max_iterations=200_000,
+           max_ph_iterations=200, CFL_v=0.9, c_fact=0.7)
+
+
+def forward(rho2, case):
+    coords, el2n, ph, xw, yw, obs = case
+    mat = O.material(eta=(1.0, 1.0), etab=(INF, INF), alpha=(0.0, 0.0), rho0=(1.0, rho2), K=(INF, INF),
+                     g=(0.0, -1.0), Tref=0.0)
+    pr = O.Problem(coords, el2n, mat, phases_v=ph, phases_P=ph,
+                   bc=((xw, np.zeros(len(xw))), (yw, np.zeros(len(yw)))))
+    st = pr.solve(1.0, **FWD)
+    assert st["converged"]
+    return pr
+
+
+def test_adjoint_gradient_matches_finite_differences():
+    case = buoyancy_case()
+    obs = case[5]
+    pr = forward(2.0, case)
+    nn, npp = pr.nnodes, pr.nnodesP
+    obj = (np.zeros(nn), np.zeros(nn))
+    obj[1][obs] = -1.0
+    lam = [np.zeros(nn), np.zeros(nn), np.zeros(npp)]
+    st = pr.solve_adjoint(1.0, obj, lam, **ADJ)
+    assert st["converged"] and np.any(lam[1] != 0)
+    assert st["lmax_iterations"] > 0 and st["lmax"] < st["lmax_gershgorin"]
+
+    # a converged λ is the warm start of a solve that needs no iteration
+    before = [a.copy() for a in lam]
+    warm = pr.solve_adjoint(1.0, obj, lam, **{**ADJ, "tolerance": 1e-9}, measure_lmax=False)
+    assert warm["converged"] and warm["iter"] == 0 and warm["lmax_iterations"] == 0
+    assert all(np.array_equal(a, b) for a, b in zip(lam, before))
+
+    # ∂R/∂ρ₂ is a residual difference: the momentum residual is linear in ρ0
+    def residual(rho):
+        pr.mat.rho0[0], pr.mat.rho0[1] = rho
+        pr.momentum_residual(1.0)
+        return pr.Rvx.copy(), pr.Rvy.copy()
+    rx1, ry1 = residual((0.0, 1.0))
+    rx0, ry0 = residual((0.0, 0.0))
+    pr.mat.rho0[0], pr.mat.rho0[1] = 1.0, 2.0
+    grad_adj = lam[0] @ (rx1 - rx0) + lam[1] @ (ry1 - ry0)
+    d = 1e-4
+    jp = -forward(2.0 + d, case).vy[obs].sum()
+    jm = -forward(2.0 - d, case).vy[obs].sum()
+    grad_fd = (jp - jm) / (2 * d)
+    assert grad_adj == pytest.approx(grad_fd, rel=1e-4)
+    assert abs(grad_fd) > 1e-3
+
+
+def test_adjoint_budget_and_unassembled_scale():
+    case = buoyancy_case()
+    coords, el2n, ph, xw, yw, obs = case
+    mat = O.material(eta=(1.0, 1.0), etab=(INF, INF), rho0=(1.0, 2.0), g=(0.0, -1.0))
+    pr = O.Problem(coords, el2n, mat, phases_v=ph, phases_P=ph,
+                   bc=((xw, np.zeros(len(xw))), (yw, np.zeros(len(yw)))))
+    nn, npp = pr.nnodes, pr.nnodesP
+    obj = (np.zeros(nn), np.zeros(nn))
+    obj[1][obs] = -1.0
+    lam = [np.zeros(nn), np.zeros(nn), np.zeros(npp)]
+    with pytest.raises(ValueError, match="unassembled"):
+        pr.solve_adjoint(1.0, obj, lam, **ADJ)
+    pr.solve(1.0, **FWD)
+    budget = dict(ADJ, max_iterations=2, check_interval=1, iter_max=2, max_ph_iterations=1, tolerance=1e-14)
+    with pytest.raises(RuntimeError, match="did not converge after 2 iterations"):
+        pr.solve_adjoint(1.0, obj, lam, **budget)
+    lam = [np.zeros(nn), np.zeros(nn), np.zeros(npp)]
+    st = pr.solve_adjoint(1.0, obj, lam, throw_on_failure=False, **budget)
+    assert not st["converged"] and st["iter"] == 2
+
+
+def _state(K, etab, G=(INF, INF), seed=0, n=3):
+    rng = np.random.default_rng(seed)
+    coords, el2n = O.t7_mesh(n, n)
+    ne = len(el2n)
+    ph = rng.integers(1, 3, ne).astype(np.int32)
+    tau = [0.3 * rng.normal(size=(ne, 7)) for _ in range(3)]
+    mat = O.material(eta=(1.0, 4.0), etab=etab, G=G, alpha=(0.01, 0.02), rho0=(1.0, 1.5), K=K, g=(0.3, -1.0),
+                     Tref=0.2)
+    pr = O.Problem(coords, el2n, mat, phases_v=ph, phases_P=ph, tau_old=tau)
+    pr.vx[:] = rng.normal(size=pr.nnodes)
+    pr.vy[:] = rng.normal(size=pr.nnodes)
+    pr.P[:] = rng.normal(size=pr.nnodesP)
+    pr.P0[:] = rng.normal(size=pr.nnodesP)
+    pr.T[:] = 1 + 0.1 * rng.normal(size=pr.nnodesP)
+    pr.T0[:] = 1 + 0.1 * rng.normal(size=pr.nnodesP)
+    pr.pressure_scaling(20.0, 0.5)
+    return pr, rng
+
+
+def _augmented_residual(pr, dt, vx, vy, P):
+    pr.vx[:], pr.vy[:], pr.P[:] = vx, vy, P
+    pr.pressure_residual(dt)
+    RP = pr.RP.copy()
+    pr.momentum_residual(dt, pr.gammaP * RP / pr.MP)
+    return pr.Rvx.copy(), pr.Rvy.copy(), RP
+
+
+@pytest.mark.parametrize("K,etab,G", [((INF, INF), (INF, INF), (INF, INF)),
+                                      ((INF, INF), (3.0, 8.0), (0.7, 2.0)),
+                                      ((40.0, 90.0), (3.0, INF), (INF, INF))])
+def test_operator_is_the_transposed_augmented_jacobian(K, etab, G):
+    dt = 0.5
+    pr, rng = _state(K, etab, G)
+    blocks = pr.adjoint_blocks(dt)
+    sym, stores = pr.adjoint_flags()
+    assert stores == any(math.isfinite(x) for x in etab)
+    lx, ly, lP = rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodesP)
+    dvx, dvy, dP = pr.adjoint_apply(blocks, lx, ly, lP)
+    u0 = (pr.vx.copy(), pr.vy.copy(), pr.P.copy())
+    for trial in range(3):
+        du = (rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodesP))
+        h = 1e-3
+        Rp = _augmented_residual(pr, dt, *(a + h * b for a, b in zip(u0, du)))
+        Rm = _augmented_residual(pr, dt, *(a - h * b for a, b in zip(u0, du)))
+        lhs = sum(l @ ((p - m) / (2 * h)) for l, p, m in zip((lx, ly, lP), Rp, Rm))
+        rhs = du[0] @ dvx + du[1] @ dvy + du[2] @ dP
+        assert lhs == pytest.approx(rhs, rel=1e-9, abs=1e-9 * np.abs(rhs))
+    pr.vx[:], pr.vy[:], pr.P[:] = u0
+
+
+def test_velocity_block_matches_the_jacobian_diagnostics_and_layouts_agree():
+    dt = 0.5
+    pr, rng = _state((INF, INF), (2.0, 5.0), (0.8, 3.0), seed=4)
+    blocks = pr.adjoint_blocks(dt)
+    rows = (pr.dRdvx.copy(), pr.dRdvy.copy(), pr.PCx.copy(), pr.PCy.copy())
+    pr.augmented_jacobian(dt)
+    for a, b in zip(rows, (pr.dRdvx, pr.dRdvy, pr.PCx, pr.PCy)):
+        assert np.allclose(a, b, rtol=1e-14, atol=0)
+    assert blocks["defects"][0] < 1e-13 and blocks["defects"][1] < 1e-13
+    sym, stores = pr.adjoint_flags()
+    assert sym and stores
+    lx, ly, lP = rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodesP)
+    packed = pr.adjoint_apply(blocks, lx, ly, lP)
+    # the full-matrix layout (Aᵀ, Cᵀ) gives the same operator
+    pr.mat.K[0] = 1e300   # finite: the flags switch to the full layout
+    full = pr.adjoint_apply(blocks, lx, ly, lP)
+    pr.mat.K[0] = INF
+    for a, b in zip(packed, full):
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
+
+
+def test_plastic_state_is_rejected():
+    pr, _ = _state((INF, INF), (INF, INF))
+    pr.dp = O.drucker_prager(phi=(0.5, 0.5), psi=(0.0, 0.0), C_=(1.0, 1.0), eta_reg=(0.01, 0.01), Kb=(1.0, 1.0))
+    with pytest.raises(ValueError, match="yield model"):
+        pr.adjoint_blocks(0.5)
