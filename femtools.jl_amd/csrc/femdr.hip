@@ -1281,6 +1281,478 @@ struct IntervalGraph {
 };
 
 // Control and result of one PH/DYREL run (_solve_stokes_dyrel!, DR.jl:348-802).
+// ---------------------------------------------------------------------------
+// The frozen-operator adjoint of a 2-D StokesDR state (solve_adjoint! with
+// operator = :blocks, DR_adjoint.jl:1-268, 396-427; FrozenAdjointOperator,
+// adjoint_operator.jl:1-660) for models without a yield model. At the
+// converged forward state the adjoint residual is affine in λ, so the element
+// blocks are assembled once: A (14×14, the augmented velocity Jacobian) by
+// the Jacobian-diagnostics kernel, B = ∂Rv/∂P, C = ∂RP/∂v, D = ∂RP/∂P (the ηb
+// storage) and W = ∂Rv/∂Pnum·Γ here. All blocks are stored
+// structure-of-arrays, [entry][nels], so a wave's loads of one entry are
+// contiguous. An adjoint iteration is the rate/λ update, the element apply
+// (gather λ, dense element products, shared-node slots) and the slot gather.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int ANU = 2 * NV, ANP = NP;
+
+struct AdjBlkArgs {
+    int64_t nels;
+    const double* geo;              // [5][nels]
+    const int32_t* phase_v; int32_t nodeph_v;   // cell [nels] or element-node [7][nels], 0-based
+    const int32_t* phase_P; int32_t nodeph_P;   // cell [nels] or [3][nels]
+    const double* ptab; int32_t nphases;
+    const double* pscale;           // Γ = γP/M_P, [nels][3]
+    double gx, gy, dt;
+    int32_t bulk_inf;               // every ηb infinite: D is zero
+    double *B, *C, *D, *W;          // [14·3][nels], [3·14][nels], [3·3][nels], [14·3][nels]
+};
+
+// B, C, D, W of one element (element_adjoint_operator_blocks,
+// adjoint_operator.jl:318-373): with no yield model P enters the momentum
+// residual through the total pressure and the density ρ0(1 − α(T − Tref) +
+// βP), and the continuity residual through the storage term.
+__global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
+    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (e >= A.nels) return;
+    const int64_t ne = A.nels;
+    const double i00 = A.geo[0 * ne + e], i01 = A.geo[1 * ne + e], i10 = A.geo[2 * ne + e],
+                 i11 = A.geo[3 * ne + e], det = A.geo[4 * ne + e];
+    int pv[NV], pp[NP];
+    for (int a = 0; a < NV; ++a) pv[a] = A.nodeph_v ? A.phase_v[a * ne + e] : A.phase_v[e];
+    for (int i = 0; i < NP; ++i) pp[i] = A.nodeph_P ? A.phase_P[i * ne + e] : A.phase_P[e];
+    double B[ANU][ANP], Bn[ANU][ANP], C[ANP][ANU], D[ANP][ANP];
+    for (int r = 0; r < ANU; ++r)
+        for (int i = 0; i < ANP; ++i) { B[r][i] = 0; Bn[r][i] = 0; C[i][r] = 0; }
+    for (int i = 0; i < ANP; ++i)
+        for (int j = 0; j < ANP; ++j) D[i][j] = 0;
+    for (int q = 0; q < NQ; ++q) {
+        const double dOm = kT.w[q] * det;
+        double r0 = 0, be = 0;
+        for (int a = 0; a < NV; ++a) {
+            r0 += kT.N[q][a] * A.ptab[PC_RHO0 * MAXPH + pv[a]];
+            be += kT.N[q][a] * A.ptab[PC_BETA * MAXPH + pv[a]];
+        }
+        double eb = INFINITY;
+        if (!A.bulk_inf) {
+            eb = 0;
+            for (int i = 0; i < NP; ++i) eb += kT.NP1[q][i] * A.ptab[PC_BULK * MAXPH + pp[i]];
+        }
+        for (int a = 0; a < NV; ++a) {
+            const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+            const double Gx = d0 * i00 + d1 * i10, Gy = d0 * i01 + d1 * i11, Na = kT.N[q][a];
+            for (int i = 0; i < NP; ++i) {
+                const double Np = kT.NP1[q][i];
+                const double drho = r0 * (be * Np);
+                B[a][i] += (Gx * (-Np) - Na * (drho * A.gx)) * dOm;
+                B[NV + a][i] += (Gy * (-Np) - Na * (drho * A.gy)) * dOm;
+                Bn[a][i] += (Gx * (-Np)) * dOm;
+                Bn[NV + a][i] += (Gy * (-Np)) * dOm;
+                C[i][a] += (Np * (-Gx)) * dOm;
+                C[i][NV + a] += (Np * (-Gy)) * dOm;
+            }
+        }
+        for (int i = 0; i < NP; ++i)
+            for (int j = 0; j < NP; ++j) D[i][j] += (kT.NP1[q][i] * (-(kT.NP1[q][j] / (eb * A.dt)))) * dOm;
+    }
+    for (int r = 0; r < ANU; ++r)
+        for (int i = 0; i < ANP; ++i) {
+            A.B[(r * ANP + i) * ne + e] = B[r][i];
+            A.W[(r * ANP + i) * ne + e] = Bn[r][i] * A.pscale[e * NP + i];
+            A.C[(i * ANU + r) * ne + e] = C[i][r];
+        }
+    for (int i = 0; i < ANP; ++i)
+        for (int j = 0; j < ANP; ++j) A.D[(i * ANP + j) * ne + e] = D[i][j];
+}
+
+struct AdjApplyArgs {
+    int64_t nels;
+    const int32_t* el2n;                        // [7][nels]
+    const double *A, *B, *C, *D, *W;     // B null: velocity rows only (no pressure coupling)
+    int32_t sym, stores;
+    int32_t forward;                            // 1: A x (the forward velocity operator), not Aᵀ x
+    const double *lx, *ly, *lP;
+    double2* slot;                              // [nels][6]
+    double *outx, *outy;                        // bubble rows written here
+    double* dP;                                 // [nels][3]
+    const double *objx, *objy;                  // nullable: added to the velocity rows
+    const double *PCx, *PCy;                    // nullable: rows divided by √PC (power iteration)
+    const uint8_t* bcflag;
+};
+
+__device__ inline double adj_row_out(double v, const double* obj, const double* PC, int64_t n, bool fixed) {
+    if (fixed) return 0.0;
+    if (PC) return v / sqrt(PC[n]);
+    return obj ? obj[n] + v : v;
+}
+
+// apply_adjoint_operator! (adjoint_operator.jl:512-535, 631-660) for one
+// element: resv = Aᵀλv + CᵀλP (the upper triangle of A and B λP for a
+// symmetric tangent), resp = Bᵀλv + Dᵀ(Wᵀλv + λP)
+__global__ __launch_bounds__(128) void k_adj_apply(AdjApplyArgs A) {
+    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (e >= A.nels) return;
+    const int64_t ne = A.nels;
+    int32_t n[NV];
+    double lv[ANU], lp[ANP];
+    for (int a = 0; a < NV; ++a) {
+        n[a] = A.el2n[a * ne + e];
+        lv[a] = A.lx[n[a]];
+        lv[NV + a] = A.ly[n[a]];
+    }
+    for (int i = 0; i < ANP; ++i) lp[i] = A.B ? A.lP[e * NP + i] : 0.0;
+    for (int i = 0; i < ANP && A.B; ++i) {
+        double b = 0;
+        for (int k = 0; k < ANU; ++k) b += A.B[(k * ANP + i) * ne + e] * lv[k];
+        double st = 0;
+        if (A.stores) {
+            double w[ANP];
+            for (int j = 0; j < ANP; ++j) {
+                double acc = 0;
+                for (int k = 0; k < ANU; ++k) acc += A.W[(k * ANP + j) * ne + e] * lv[k];
+                w[j] = acc + lp[j];
+            }
+            for (int j = 0; j < ANP; ++j) st += A.D[(j * ANP + i) * ne + e] * w[j];
+        }
+        A.dP[e * NP + i] = b + st;   // discontinuous pressure: unshared
+    }
+    double rv[ANU];
+    for (int r = 0; r < ANU; ++r) {
+        double a = 0, b = 0;
+        for (int k = 0; k < ANU; ++k) {
+            const int64_t idx = A.forward ? (int64_t)r * ANU + k
+                                          : (A.sym ? (int64_t)(r < k ? r : k) * ANU + (r < k ? k : r)
+                                                   : (int64_t)k * ANU + r);
+            a += A.A[idx * ne + e] * lv[k];
+        }
+        for (int i = 0; i < ANP && A.B; ++i)
+            b += (A.sym ? A.B[(r * ANP + i) * ne + e] : A.C[(i * ANU + r) * ne + e]) * lp[i];
+        rv[r] = a + b;
+    }
+    for (int a = 0; a < NSH; ++a) A.slot[e * NSH + a] = make_double2(rv[a], rv[NV + a]);
+    const int32_t bn = n[NSH];
+    const uint8_t f = A.bcflag[bn];
+    A.outx[bn] = adj_row_out(rv[NSH], A.objx, A.PCx, bn, f & 1);
+    A.outy[bn] = adj_row_out(rv[NV + NSH], A.objy, A.PCy, bn, f & 2);
+}
+
+struct AdjGatherArgs {
+    int64_t ns;
+    const int32_t *node_of, *off, *ent;
+    const double2* slot;
+    double *outx, *outy;
+    const double *objx, *objy, *PCx, *PCy;
+    const uint8_t* bcflag;
+};
+
+// shared-node rows in ascending element order
+__global__ __launch_bounds__(256) void k_adj_gather(AdjGatherArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.ns) return;
+    const int64_t node = A.node_of ? A.node_of[i] : i;
+    double x = 0, y = 0;
+    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
+        const double2 v = A.slot[A.ent[k]];
+        x += v.x;
+        y += v.y;
+    }
+    const uint8_t f = A.bcflag[node];
+    A.outx[node] = adj_row_out(x, A.objx, A.PCx, node, f & 1);
+    A.outy[node] = adj_row_out(y, A.objy, A.PCy, node, f & 2);
+}
+
+// R0 ← R, then the rate and λ update of every node (stokes_update_rate!,
+// stokes_update_variable! with −α, homogeneous Dirichlet rows; DR_adjoint.jl:234-249)
+__global__ void k_adj_update(int64_t n, const double* Rx, const double* Ry, double* R0x, double* R0y, double* rx,
+                             double* ry, double* lx, double* ly, const double* PCx, const double* PCy,
+                             const uint8_t* bcflag, double ax, double ay, double bx, double by) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t f = bcflag[i];
+    const double Rxi = Rx[i], Ryi = Ry[i];
+    R0x[i] = Rxi;
+    R0y[i] = Ryi;
+    if (f & 1) { rx[i] = 0; lx[i] = 0; }
+    else { const double nr = bx * rx[i] + Rxi / PCx[i]; rx[i] = nr; lx[i] += -ax * nr; }
+    if (f & 2) { ry[i] = 0; ly[i] = 0; }
+    else { const double nr = by * ry[i] + Ryi / PCy[i]; ry[i] = nr; ly[i] += -ay * nr; }
+}
+
+// power-iteration step: x ← x/‖x‖, z = x/√PC (estimate_adjoint_λmax, adjoint_operator.jl:589-596)
+__global__ void k_adj_power(int64_t n, double* xx, double* xy, double* zx, double* zy, const double* PCx,
+                            const double* PCy, double nrm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = xx[i] / nrm, b = xy[i] / nrm;
+    xx[i] = a;
+    xy[i] = b;
+    zx[i] = a / sqrt(PCx[i]);
+    zy[i] = b / sqrt(PCy[i]);
+}
+
+// Σ a², Σ b², Σ c² (c over its own length)
+__global__ __launch_bounds__(RED_BLOCK) void k_adj_norms(int64_t n, int64_t nc, const double* a, const double* b,
+                                                         const double* c, double* partial) {
+    double acc[3] = {0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride) {
+        acc[0] += a[i] * a[i];
+        acc[1] += b[i] * b[i];
+    }
+    if (c)
+        for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < nc; i += stride) acc[2] += c[i] * c[i];
+    block_sum_store<3>(acc, partial);
+}
+
+// Arrow-Hurwicz update of the pressure adjoint, λP += γP ResλP / M_P
+__global__ void k_adj_lamP(int64_t n, double* lP, const double* gP, const double* RP, const double* MP) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lP[i] += gP[i] * RP[i] / MP[i];
+}
+
+struct AdjWork {
+    fem_ctx* c = nullptr;
+    std::vector<void*> bufs;
+    template <class T>
+    int get(T** p, size_t n) {
+        int r = dalloc_t(c, p, n);
+        if (r == FEM_OK) bufs.push_back(*p);
+        return r;
+    }
+    ~AdjWork() {
+        if (c) hipStreamSynchronize(c->stream);
+        for (void* b : bufs) hipFree(b);
+    }
+};
+
+struct AdjointOp {
+    double *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr, *W = nullptr;
+    double2* slot = nullptr;
+    bool sym = false, stores = false;
+};
+
+// the blocks at the state in s; also fills ∂Rv∂v and PC_v from A
+static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    const int64_t ne = m->nels;
+    LaunchOpts o;
+    CHK(prepare(s, nullptr, dt, 0, o, true));   // PC_BULK = ηb: the operator's pressure bulk
+    const fem_material& mt = s->mat;
+    op.sym = true;
+    op.stores = false;
+    for (int p = 0; p < mt.nphases; ++p) {
+        if (!std::isinf(mt.K[p])) op.sym = false;
+        if (std::isfinite(mt.etab[p])) op.stores = true;
+    }
+    CHK(w.get(&op.A, (size_t)ANU * ANU * ne));
+    CHK(w.get(&op.B, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.C, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.D, (size_t)ANP * ANP * ne));
+    CHK(w.get(&op.W, (size_t)ANU * ANP * ne));
+    CHK(w.get(&op.slot, (size_t)NSH * ne));
+    hipLaunchKernelGGL(k_pscale, dim3(grid_for(m->nnodesP, 256)), dim3(256), 0, c->stream, m->nnodesP,
+                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
+    CHK(check_launch(c, "k_pscale"));
+    o.Ablk = op.A;
+    CHK(launch_jacobian(s, o));
+    AdjBlkArgs b;
+    b.nels = ne;
+    b.geo = m->d_geo;
+    b.phase_v = s->d_phase_v;
+    b.nodeph_v = s->layout_v == FEM_PHASE_ELEMENT;
+    b.phase_P = s->d_phase_P;
+    b.nodeph_P = s->layout_P == FEM_PHASE_ELEMENT;
+    b.ptab = s->d_ptab;
+    b.nphases = mt.nphases;
+    b.pscale = s->d_pscale;
+    b.gx = mt.g[0];
+    b.gy = mt.g[1];
+    b.dt = dt;
+    b.bulk_inf = o.bulk_inf;
+    b.B = op.B; b.C = op.C; b.D = op.D; b.W = op.W;
+    hipLaunchKernelGGL(k_adj_blocks, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
+    CHK(check_launch(c, "k_adj_blocks"));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FEM_OK;
+}
+
+// out = op applied to (lx, ly, lP): velocity rows into (ox, oy) with obj added
+// (or divided by √PC when PCx is set), homogeneous Dirichlet rows; dP pressure rows
+static int adjoint_apply(fem_stokes* s, const AdjointOp& op, const double* lx, const double* ly, const double* lP,
+                         double* ox, double* oy, double* dP, const double* objx, const double* objy,
+                         const double* PCx, const double* PCy, bool forward = false) {
+    fem_ctx* c = s->ctx;
+    fem_mesh* m = s->mesh;
+    AdjApplyArgs a;
+    a.nels = m->nels;
+    a.el2n = m->d_el2n;
+    a.A = op.A; a.B = op.B; a.C = op.C; a.D = op.D; a.W = op.W;
+    a.sym = op.sym; a.stores = op.stores;
+    a.forward = forward;
+    a.lx = lx; a.ly = ly; a.lP = lP;
+    a.slot = op.slot;
+    a.outx = ox; a.outy = oy; a.dP = dP;
+    a.objx = objx; a.objy = objy; a.PCx = PCx; a.PCy = PCy;
+    a.bcflag = s->d_bcflag;
+    hipLaunchKernelGGL(k_adj_apply, dim3(grid_for(m->nels, 128)), dim3(128), 0, c->stream, a);
+    CHK(check_launch(c, "k_adj_apply"));
+    AdjGatherArgs g;
+    g.ns = m->ns;
+    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
+    g.slot = op.slot;
+    g.outx = ox; g.outy = oy;
+    g.objx = objx; g.objy = objy; g.PCx = PCx; g.PCy = PCy;
+    g.bcflag = s->d_bcflag;
+    hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m->ns, 256)), dim3(256), 0, c->stream, g);
+    return check_launch(c, "k_adj_gather");
+}
+
+// Σa², Σb², Σc² to the host
+static int adjoint_norms(fem_stokes* s, const double* a, const double* b, const double* cc, int64_t nc,
+                         double out[3]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, nc), RED_BLOCK));
+    hipLaunchKernelGGL(k_adj_norms, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, nc, a, b, cc, c->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
+                       (const double*)c->red_partial, c->red_out);
+    CHK(check_launch(c, "k_adj_norms"));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_red, 3 * 8);
+    return FEM_OK;
+}
+
+// _assert_frozen_symmetry (adjoint_operator.jl:149-160) on the stored blocks
+static int adjoint_check_symmetry(fem_stokes* s, const AdjointOp& op) {
+    fem_ctx* c = s->ctx;
+    const int64_t ne = s->mesh->nels;
+    std::vector<double> A((size_t)ANU * ANU * ne), B((size_t)ANU * ANP * ne), C((size_t)ANU * ANP * ne);
+    HIPCHK(c, hipMemcpy(A.data(), op.A, A.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(B.data(), op.B, B.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(C.data(), op.C, C.size() * 8, hipMemcpyDeviceToHost));
+    double wA = 0, wC = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        double nA = 0, dA = 0, nC = 0, dC = 0;
+        for (int i = 0; i < ANU; ++i)
+            for (int j = 0; j < ANU; ++j) {
+                const double x = A[(i * ANU + j) * ne + e], y = A[(j * ANU + i) * ne + e];
+                nA += x * x;
+                dA += (x - y) * (x - y);
+            }
+        for (int i = 0; i < ANP; ++i)
+            for (int k = 0; k < ANU; ++k) {
+                const double x = C[(i * ANU + k) * ne + e], y = B[(k * ANP + i) * ne + e];
+                nC += x * x;
+                dC += (x - y) * (x - y);
+            }
+        wA = std::max(wA, nA == 0 ? 0.0 : std::sqrt(dA) / std::sqrt(nA));
+        wC = std::max(wC, nC == 0 ? 0.0 : std::sqrt(dC) / std::sqrt(nC));
+    }
+    const double tol = std::sqrt(std::numeric_limits<double>::epsilon());
+    if (wA > tol || wC > tol)
+        return set_err(c, FEM_EARG,
+                       "the frozen adjoint operator assumes the element tangent is symmetric, but the assembled "
+                       "blocks disagree: max ‖A - Aᵀ‖/‖A‖ = %g, max ‖C - Bᵀ‖/‖C‖ = %g over %lld elements",
+                       wA, wC, (long long)ne);
+    return FEM_OK;
+}
+
+// measure_λmax of the forward solve (DR.jl:375-378, 459-490, 728-760): the
+// Jacobian launch also stores the velocity blocks A, and a power iteration on
+// the symmetrically Jacobi-scaled A (estimate_velocity_λmax,
+// adjoint_operator.jl:240-281; A x, not Aᵀ x) measures λmax, warm-started from
+// the previous estimate's vector at every later assembly. λmax_v = min(the
+// Gershgorin bound, safety · measured) for both components.
+struct PowerState {
+    AdjWork w;
+    AdjointOp op;
+    double *xx = nullptr, *xy = nullptr, *zx = nullptr, *zy = nullptr, *yx = nullptr, *yy = nullptr;
+    bool seeded = false;
+    int64_t its = 0;        // λmax_iterations: power steps over the whole solve
+    int max_it = 12;
+    double rtol = 1e-2, safety = 1.1;
+};
+
+static int power_init(fem_stokes* s, PowerState& ps, const fem_solve_params* p, bool plastic) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes, ne = s->mesh->nels;
+    ps.w.c = c;
+    ps.max_it = p->lmax_power_iterations > 0 ? p->lmax_power_iterations : 12;
+    ps.rtol = p->lmax_power_rtol > 0 ? p->lmax_power_rtol : 1e-2;
+    const double safety = p->lmax_safety > 0 ? p->lmax_safety : 1.1;
+    ps.safety = plastic ? std::max(safety, 1.5) : safety;   // _stokes_spectral_safety (DR.jl:931-933)
+    CHK(ps.w.get(&ps.op.A, (size_t)ANU * ANU * ne));
+    CHK(ps.w.get(&ps.op.slot, (size_t)NSH * ne));
+    for (double** v : {&ps.xx, &ps.xy, &ps.zx, &ps.zy, &ps.yx, &ps.yy}) CHK(ps.w.get(v, nv));
+    return FEM_OK;
+}
+
+// after a Jacobian assembly with PowerState.op.A set and lmax holding the
+// Gershgorin bounds of the two components
+static int power_lmax_fwd(fem_stokes* s, PowerState& ps, double lmax[2]) {
+    fem_ctx* c = s->ctx;
+    const int64_t nv = s->mesh->nnodes;
+    double* PCx = s->f[FEM_FIELD_PC_V_X];
+    double* PCy = s->f[FEM_FIELD_PC_V_Y];
+    std::vector<uint8_t> flag(nv);
+    HIPCHK(c, hipMemcpy(flag.data(), s->d_bcflag, nv, hipMemcpyDeviceToHost));
+    std::vector<double> hx(nv), hy(nv);
+    if (!ps.seeded) {
+        for (int64_t i = 0; i < nv; ++i) {   // x .= sin.(i .* 0.7), y .= cos.(i .* 1.3), 1-based i
+            hx[i] = std::sin((double)(i + 1) * 0.7);
+            hy[i] = std::cos((double)(i + 1) * 1.3);
+        }
+        ps.seeded = true;
+    } else {
+        HIPCHK(c, hipMemcpy(hx.data(), ps.xx, nv * 8, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(hy.data(), ps.xy, nv * 8, hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < nv; ++i) {
+        if (flag[i] & 1) hx[i] = 0;
+        if (flag[i] & 2) hy[i] = 0;
+    }
+    HIPCHK(c, hipMemcpy(ps.xx, hx.data(), nv * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(ps.xy, hy.data(), nv * 8, hipMemcpyHostToDevice));
+    const unsigned gv = grid_for(nv, 256);
+    double lam = 0, nrm[3], meas = 0;
+    int64_t used = ps.max_it;
+    bool done = false;
+    CHK(adjoint_norms(s, ps.xx, ps.xy, nullptr, 0, nrm));
+    for (int it = 1; it <= ps.max_it; ++it) {
+        const double n = std::sqrt(nrm[0] + nrm[1]);
+        if (!(n > 0)) return set_err(c, FEM_EARG, "power iteration collapsed to the zero vector");
+        hipLaunchKernelGGL(k_adj_power, dim3(gv), dim3(256), 0, c->stream, nv, ps.xx, ps.xy, ps.zx, ps.zy,
+                           (const double*)PCx, (const double*)PCy, n);
+        CHK(check_launch(c, "k_adj_power"));
+        CHK(adjoint_apply(s, ps.op, ps.zx, ps.zy, nullptr, ps.yx, ps.yy, nullptr, nullptr, nullptr, PCx, PCy, true));
+        CHK(adjoint_norms(s, ps.yx, ps.yy, nullptr, 0, nrm));
+        const double ln = std::sqrt(nrm[0] + nrm[1]);
+        std::swap(ps.xx, ps.yx);
+        std::swap(ps.xy, ps.yy);
+        if (it > 1 && std::fabs(ln - lam) <= ps.rtol * ln) {
+            meas = ln;
+            used = it;
+            done = true;
+            break;
+        }
+        lam = ln;
+    }
+    if (!done) {
+        if (!(std::isfinite(lam) && lam > 0))
+            return set_err(c, FEM_EARG, "power iteration produced a non-positive λmax: %g", lam);
+        meas = lam;
+    }
+    ps.its += used;
+    const double g = std::max(lmax[0], lmax[1]);
+    lmax[0] = lmax[1] = std::min(g, ps.safety * meas);
+    return FEM_OK;
+}
+
+}  // namespace
+
 struct DRControl {
     double tol = 0;            // converged when min(err_abs, err_rel) < tol
     int64_t max_iterations = 0, iter_max = 0, max_ph = 0, ncheck = 1;
@@ -1292,6 +1764,8 @@ struct DRControl {
     // inner iteration; both loop tests also wait for *thermal_converged
     std::function<int()> thermal_step;
     const bool* thermal_converged = nullptr;
+    PowerState* power = nullptr;   // measure_λmax: re-measured at every Jacobian assembly
+    double lmax_g0 = 0;            // the initial Gershgorin bound when lmax holds a measured value
 };
 struct DRResult {
     int64_t itPH = 0, iter = 0, launches = 0, jac = 0;
@@ -1322,7 +1796,7 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
     const int64_t nv = s->mesh->nnodes, np = s->mesh->nnodesP, ncheck = ctl.ncheck;
     double al[2], be[2];
     for (int k = 0; k < 2; ++k) cheb(2 / std::sqrt(lmax[k]) * s->opts.CFL_v, 0.0, s->opts.c_fact, &al[k], &be[k]);
-    r.lmax_g = std::max(lmax[0], lmax[1]);
+    r.lmax_g = ctl.lmax_g0 > 0 ? ctl.lmax_g0 : std::max(lmax[0], lmax[1]);
     const double eps_tol = ctl.tol;
     double err_min = INFINITY, err = 2 * eps_tol;
     double err_v0 = 1.0, err_P0 = 1.0, err_v00 = 1.0;
@@ -1450,6 +1924,10 @@ static int ph_dyrel(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o, const
                         rc = checked_lmax(s, lmax);
                         if (rc != FEM_OK) break;
                         r.lmax_g = std::max(lmax[0], lmax[1]);
+                        if (ctl.power) {
+                            rc = power_lmax_fwd(s, *ctl.power, lmax);
+                            if (rc != FEM_OK) break;
+                        }
                     }
                 }
                 for (int k = 0; k < 2; ++k)
@@ -1931,6 +2409,13 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     if (s->in.history && (p->precision != 0 || s->dim != 2))
         return set_err(c, FEM_EARG, "collect_history is recorded by 2-D FP64 solves (precision 0)");
     if (s->in.history_len) *s->in.history_len = 0;
+    if (p->measure_lmax) {   // DR.jl:394, 425-426
+        if (s->dim != 2) return set_err(c, FEM_EARG, "measure_λmax is currently supported only in plane strain");
+        if (s->in.viscosity) return set_err(c, FEM_EARG, "quadrature viscosity requires measure_λmax=false");
+        if (p->precision != 0)
+            return set_err(c, FEM_EARG, "measure_λmax runs with the FP64 solve (precision 0), got precision %d",
+                           p->precision);
+    }
     if (s->dim == 3) {
         if (cp && p->precision != 0)
             return set_err(c, FEM_EARG, "coupled solves run in FP64 (precision 0), got precision %d", p->precision);
@@ -1968,12 +2453,22 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_X], 0, nv * 8, c->stream));
     HIPCHK(c, hipMemsetAsync(s->f[FEM_FIELD_RV0_Y], 0, nv * 8, c->stream));
 
+    std::unique_ptr<PowerState> power;
+    if (p->measure_lmax) {
+        power.reset(new PowerState());
+        CHK(power_init(s, *power, p, dp != nullptr || cap != nullptr));
+        o.Ablk = power->op.A;
+    }
     CHK(launch_jacobian(s, o));
     double lmax[2];
     CHK(checked_lmax(s, lmax));
+    const double lmax_g0 = std::max(lmax[0], lmax[1]);
+    if (power) CHK(power_lmax_fwd(s, *power, lmax));
     if (p->verbose) printf("Initial momentum preconditioner λmax_v = (%g, %g)\n", lmax[0], lmax[1]);
 
     DRControl ctl;
+    ctl.power = power.get();
+    ctl.lmax_g0 = lmax_g0;
     ctl.tol = p->tolerance;
     ctl.max_iterations = p->max_iterations;
     ctl.iter_max = p->iter_max;
@@ -2016,7 +2511,7 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
     st->reached_total_iter = r.iter > p->max_iterations;
     st->lambda_max = std::max(lmax[0], lmax[1]);
     st->lambda_max_gershgorin = r.lmax_g;
-    st->lambda_max_iterations = 0;
+    st->lambda_max_iterations = power ? power->its : 0;
     st->jacobian_assemblies = 1 + r.jac;
     st->element_launches = r.launches;
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2026,383 +2521,6 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
                        (long long)r.iter, r.err_abs, p->tolerance);
     return FEM_OK;
 }
-
-// ---------------------------------------------------------------------------
-// The frozen-operator adjoint of a 2-D StokesDR state (solve_adjoint! with
-// operator = :blocks, DR_adjoint.jl:1-268, 396-427; FrozenAdjointOperator,
-// adjoint_operator.jl:1-660) for models without a yield model. At the
-// converged forward state the adjoint residual is affine in λ, so the element
-// blocks are assembled once: A (14×14, the augmented velocity Jacobian) by
-// the Jacobian-diagnostics kernel, B = ∂Rv/∂P, C = ∂RP/∂v, D = ∂RP/∂P (the ηb
-// storage) and W = ∂Rv/∂Pnum·Γ here. All blocks are stored
-// structure-of-arrays, [entry][nels], so a wave's loads of one entry are
-// contiguous. An adjoint iteration is the rate/λ update, the element apply
-// (gather λ, dense element products, shared-node slots) and the slot gather.
-// ---------------------------------------------------------------------------
-namespace {
-
-constexpr int ANU = 2 * NV, ANP = NP;
-
-struct AdjBlkArgs {
-    int64_t nels;
-    const double* geo;              // [5][nels]
-    const int32_t* phase_v; int32_t nodeph_v;   // cell [nels] or element-node [7][nels], 0-based
-    const int32_t* phase_P; int32_t nodeph_P;   // cell [nels] or [3][nels]
-    const double* ptab; int32_t nphases;
-    const double* pscale;           // Γ = γP/M_P, [nels][3]
-    double gx, gy, dt;
-    int32_t bulk_inf;               // every ηb infinite: D is zero
-    double *B, *C, *D, *W;          // [14·3][nels], [3·14][nels], [3·3][nels], [14·3][nels]
-};
-
-// B, C, D, W of one element (element_adjoint_operator_blocks,
-// adjoint_operator.jl:318-373): with no yield model P enters the momentum
-// residual through the total pressure and the density ρ0(1 − α(T − Tref) +
-// βP), and the continuity residual through the storage term.
-__global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
-    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (e >= A.nels) return;
-    const int64_t ne = A.nels;
-    const double i00 = A.geo[0 * ne + e], i01 = A.geo[1 * ne + e], i10 = A.geo[2 * ne + e],
-                 i11 = A.geo[3 * ne + e], det = A.geo[4 * ne + e];
-    int pv[NV], pp[NP];
-    for (int a = 0; a < NV; ++a) pv[a] = A.nodeph_v ? A.phase_v[a * ne + e] : A.phase_v[e];
-    for (int i = 0; i < NP; ++i) pp[i] = A.nodeph_P ? A.phase_P[i * ne + e] : A.phase_P[e];
-    double B[ANU][ANP], Bn[ANU][ANP], C[ANP][ANU], D[ANP][ANP];
-    for (int r = 0; r < ANU; ++r)
-        for (int i = 0; i < ANP; ++i) { B[r][i] = 0; Bn[r][i] = 0; C[i][r] = 0; }
-    for (int i = 0; i < ANP; ++i)
-        for (int j = 0; j < ANP; ++j) D[i][j] = 0;
-    for (int q = 0; q < NQ; ++q) {
-        const double dOm = kT.w[q] * det;
-        double r0 = 0, be = 0;
-        for (int a = 0; a < NV; ++a) {
-            r0 += kT.N[q][a] * A.ptab[PC_RHO0 * MAXPH + pv[a]];
-            be += kT.N[q][a] * A.ptab[PC_BETA * MAXPH + pv[a]];
-        }
-        double eb = INFINITY;
-        if (!A.bulk_inf) {
-            eb = 0;
-            for (int i = 0; i < NP; ++i) eb += kT.NP1[q][i] * A.ptab[PC_BULK * MAXPH + pp[i]];
-        }
-        for (int a = 0; a < NV; ++a) {
-            const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
-            const double Gx = d0 * i00 + d1 * i10, Gy = d0 * i01 + d1 * i11, Na = kT.N[q][a];
-            for (int i = 0; i < NP; ++i) {
-                const double Np = kT.NP1[q][i];
-                const double drho = r0 * (be * Np);
-                B[a][i] += (Gx * (-Np) - Na * (drho * A.gx)) * dOm;
-                B[NV + a][i] += (Gy * (-Np) - Na * (drho * A.gy)) * dOm;
-                Bn[a][i] += (Gx * (-Np)) * dOm;
-                Bn[NV + a][i] += (Gy * (-Np)) * dOm;
-                C[i][a] += (Np * (-Gx)) * dOm;
-                C[i][NV + a] += (Np * (-Gy)) * dOm;
-            }
-        }
-        for (int i = 0; i < NP; ++i)
-            for (int j = 0; j < NP; ++j) D[i][j] += (kT.NP1[q][i] * (-(kT.NP1[q][j] / (eb * A.dt)))) * dOm;
-    }
-    for (int r = 0; r < ANU; ++r)
-        for (int i = 0; i < ANP; ++i) {
-            A.B[(r * ANP + i) * ne + e] = B[r][i];
-            A.W[(r * ANP + i) * ne + e] = Bn[r][i] * A.pscale[e * NP + i];
-            A.C[(i * ANU + r) * ne + e] = C[i][r];
-        }
-    for (int i = 0; i < ANP; ++i)
-        for (int j = 0; j < ANP; ++j) A.D[(i * ANP + j) * ne + e] = D[i][j];
-}
-
-struct AdjApplyArgs {
-    int64_t nels;
-    const int32_t* el2n;                        // [7][nels]
-    const double *A, *B, *C, *D, *W;
-    int32_t sym, stores;
-    const double *lx, *ly, *lP;
-    double2* slot;                              // [nels][6]
-    double *outx, *outy;                        // bubble rows written here
-    double* dP;                                 // [nels][3]
-    const double *objx, *objy;                  // nullable: added to the velocity rows
-    const double *PCx, *PCy;                    // nullable: rows divided by √PC (power iteration)
-    const uint8_t* bcflag;
-};
-
-__device__ inline double adj_row_out(double v, const double* obj, const double* PC, int64_t n, bool fixed) {
-    if (fixed) return 0.0;
-    if (PC) return v / sqrt(PC[n]);
-    return obj ? obj[n] + v : v;
-}
-
-// apply_adjoint_operator! (adjoint_operator.jl:512-535, 631-660) for one
-// element: resv = Aᵀλv + CᵀλP (the upper triangle of A and B λP for a
-// symmetric tangent), resp = Bᵀλv + Dᵀ(Wᵀλv + λP)
-__global__ __launch_bounds__(128) void k_adj_apply(AdjApplyArgs A) {
-    const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (e >= A.nels) return;
-    const int64_t ne = A.nels;
-    int32_t n[NV];
-    double lv[ANU], lp[ANP];
-    for (int a = 0; a < NV; ++a) {
-        n[a] = A.el2n[a * ne + e];
-        lv[a] = A.lx[n[a]];
-        lv[NV + a] = A.ly[n[a]];
-    }
-    for (int i = 0; i < ANP; ++i) lp[i] = A.lP[e * NP + i];
-    for (int i = 0; i < ANP; ++i) {
-        double b = 0;
-        for (int k = 0; k < ANU; ++k) b += A.B[(k * ANP + i) * ne + e] * lv[k];
-        double st = 0;
-        if (A.stores) {
-            double w[ANP];
-            for (int j = 0; j < ANP; ++j) {
-                double acc = 0;
-                for (int k = 0; k < ANU; ++k) acc += A.W[(k * ANP + j) * ne + e] * lv[k];
-                w[j] = acc + lp[j];
-            }
-            for (int j = 0; j < ANP; ++j) st += A.D[(j * ANP + i) * ne + e] * w[j];
-        }
-        A.dP[e * NP + i] = b + st;   // discontinuous pressure: unshared
-    }
-    double rv[ANU];
-    for (int r = 0; r < ANU; ++r) {
-        double a = 0, b = 0;
-        for (int k = 0; k < ANU; ++k) {
-            const int64_t idx = A.sym ? (int64_t)(r < k ? r : k) * ANU + (r < k ? k : r) : (int64_t)k * ANU + r;
-            a += A.A[idx * ne + e] * lv[k];
-        }
-        for (int i = 0; i < ANP; ++i)
-            b += (A.sym ? A.B[(r * ANP + i) * ne + e] : A.C[(i * ANU + r) * ne + e]) * lp[i];
-        rv[r] = a + b;
-    }
-    for (int a = 0; a < NSH; ++a) A.slot[e * NSH + a] = make_double2(rv[a], rv[NV + a]);
-    const int32_t bn = n[NSH];
-    const uint8_t f = A.bcflag[bn];
-    A.outx[bn] = adj_row_out(rv[NSH], A.objx, A.PCx, bn, f & 1);
-    A.outy[bn] = adj_row_out(rv[NV + NSH], A.objy, A.PCy, bn, f & 2);
-}
-
-struct AdjGatherArgs {
-    int64_t ns;
-    const int32_t *node_of, *off, *ent;
-    const double2* slot;
-    double *outx, *outy;
-    const double *objx, *objy, *PCx, *PCy;
-    const uint8_t* bcflag;
-};
-
-// shared-node rows in ascending element order
-__global__ __launch_bounds__(256) void k_adj_gather(AdjGatherArgs A) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.ns) return;
-    const int64_t node = A.node_of ? A.node_of[i] : i;
-    double x = 0, y = 0;
-    for (int32_t k = A.off[i]; k < A.off[i + 1]; ++k) {
-        const double2 v = A.slot[A.ent[k]];
-        x += v.x;
-        y += v.y;
-    }
-    const uint8_t f = A.bcflag[node];
-    A.outx[node] = adj_row_out(x, A.objx, A.PCx, node, f & 1);
-    A.outy[node] = adj_row_out(y, A.objy, A.PCy, node, f & 2);
-}
-
-// R0 ← R, then the rate and λ update of every node (stokes_update_rate!,
-// stokes_update_variable! with −α, homogeneous Dirichlet rows; DR_adjoint.jl:234-249)
-__global__ void k_adj_update(int64_t n, const double* Rx, const double* Ry, double* R0x, double* R0y, double* rx,
-                             double* ry, double* lx, double* ly, const double* PCx, const double* PCy,
-                             const uint8_t* bcflag, double ax, double ay, double bx, double by) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint8_t f = bcflag[i];
-    const double Rxi = Rx[i], Ryi = Ry[i];
-    R0x[i] = Rxi;
-    R0y[i] = Ryi;
-    if (f & 1) { rx[i] = 0; lx[i] = 0; }
-    else { const double nr = bx * rx[i] + Rxi / PCx[i]; rx[i] = nr; lx[i] += -ax * nr; }
-    if (f & 2) { ry[i] = 0; ly[i] = 0; }
-    else { const double nr = by * ry[i] + Ryi / PCy[i]; ry[i] = nr; ly[i] += -ay * nr; }
-}
-
-// power-iteration step: x ← x/‖x‖, z = x/√PC (estimate_adjoint_λmax, adjoint_operator.jl:589-596)
-__global__ void k_adj_power(int64_t n, double* xx, double* xy, double* zx, double* zy, const double* PCx,
-                            const double* PCy, double nrm) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double a = xx[i] / nrm, b = xy[i] / nrm;
-    xx[i] = a;
-    xy[i] = b;
-    zx[i] = a / sqrt(PCx[i]);
-    zy[i] = b / sqrt(PCy[i]);
-}
-
-// Σ a², Σ b², Σ c² (c over its own length)
-__global__ __launch_bounds__(RED_BLOCK) void k_adj_norms(int64_t n, int64_t nc, const double* a, const double* b,
-                                                         const double* c, double* partial) {
-    double acc[3] = {0, 0, 0};
-    const int64_t stride = (int64_t)gridDim.x * RED_BLOCK;
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n; i += stride) {
-        acc[0] += a[i] * a[i];
-        acc[1] += b[i] * b[i];
-    }
-    if (c)
-        for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < nc; i += stride) acc[2] += c[i] * c[i];
-    block_sum_store<3>(acc, partial);
-}
-
-// Arrow-Hurwicz update of the pressure adjoint, λP += γP ResλP / M_P
-__global__ void k_adj_lamP(int64_t n, double* lP, const double* gP, const double* RP, const double* MP) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) lP[i] += gP[i] * RP[i] / MP[i];
-}
-
-struct AdjWork {
-    fem_ctx* c = nullptr;
-    std::vector<void*> bufs;
-    template <class T>
-    int get(T** p, size_t n) {
-        int r = dalloc_t(c, p, n);
-        if (r == FEM_OK) bufs.push_back(*p);
-        return r;
-    }
-    ~AdjWork() {
-        if (c) hipStreamSynchronize(c->stream);
-        for (void* b : bufs) hipFree(b);
-    }
-};
-
-struct AdjointOp {
-    double *A = nullptr, *B = nullptr, *C = nullptr, *D = nullptr, *W = nullptr;
-    double2* slot = nullptr;
-    bool sym = false, stores = false;
-};
-
-// the blocks at the state in s; also fills ∂Rv∂v and PC_v from A
-static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op) {
-    fem_ctx* c = s->ctx;
-    fem_mesh* m = s->mesh;
-    const int64_t ne = m->nels;
-    LaunchOpts o;
-    CHK(prepare(s, nullptr, dt, 0, o, true));   // PC_BULK = ηb: the operator's pressure bulk
-    const fem_material& mt = s->mat;
-    op.sym = true;
-    op.stores = false;
-    for (int p = 0; p < mt.nphases; ++p) {
-        if (!std::isinf(mt.K[p])) op.sym = false;
-        if (std::isfinite(mt.etab[p])) op.stores = true;
-    }
-    CHK(w.get(&op.A, (size_t)ANU * ANU * ne));
-    CHK(w.get(&op.B, (size_t)ANU * ANP * ne));
-    CHK(w.get(&op.C, (size_t)ANU * ANP * ne));
-    CHK(w.get(&op.D, (size_t)ANP * ANP * ne));
-    CHK(w.get(&op.W, (size_t)ANU * ANP * ne));
-    CHK(w.get(&op.slot, (size_t)NSH * ne));
-    hipLaunchKernelGGL(k_pscale, dim3(grid_for(m->nnodesP, 256)), dim3(256), 0, c->stream, m->nnodesP,
-                       (const double*)s->f[FEM_FIELD_GAMMA_P], (const double*)s->f[FEM_FIELD_M_P], s->d_pscale);
-    CHK(check_launch(c, "k_pscale"));
-    o.Ablk = op.A;
-    CHK(launch_jacobian(s, o));
-    AdjBlkArgs b;
-    b.nels = ne;
-    b.geo = m->d_geo;
-    b.phase_v = s->d_phase_v;
-    b.nodeph_v = s->layout_v == FEM_PHASE_ELEMENT;
-    b.phase_P = s->d_phase_P;
-    b.nodeph_P = s->layout_P == FEM_PHASE_ELEMENT;
-    b.ptab = s->d_ptab;
-    b.nphases = mt.nphases;
-    b.pscale = s->d_pscale;
-    b.gx = mt.g[0];
-    b.gy = mt.g[1];
-    b.dt = dt;
-    b.bulk_inf = o.bulk_inf;
-    b.B = op.B; b.C = op.C; b.D = op.D; b.W = op.W;
-    hipLaunchKernelGGL(k_adj_blocks, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
-    CHK(check_launch(c, "k_adj_blocks"));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return FEM_OK;
-}
-
-// out = op applied to (lx, ly, lP): velocity rows into (ox, oy) with obj added
-// (or divided by √PC when PCx is set), homogeneous Dirichlet rows; dP pressure rows
-static int adjoint_apply(fem_stokes* s, const AdjointOp& op, const double* lx, const double* ly, const double* lP,
-                         double* ox, double* oy, double* dP, const double* objx, const double* objy,
-                         const double* PCx, const double* PCy) {
-    fem_ctx* c = s->ctx;
-    fem_mesh* m = s->mesh;
-    AdjApplyArgs a;
-    a.nels = m->nels;
-    a.el2n = m->d_el2n;
-    a.A = op.A; a.B = op.B; a.C = op.C; a.D = op.D; a.W = op.W;
-    a.sym = op.sym; a.stores = op.stores;
-    a.lx = lx; a.ly = ly; a.lP = lP;
-    a.slot = op.slot;
-    a.outx = ox; a.outy = oy; a.dP = dP;
-    a.objx = objx; a.objy = objy; a.PCx = PCx; a.PCy = PCy;
-    a.bcflag = s->d_bcflag;
-    hipLaunchKernelGGL(k_adj_apply, dim3(grid_for(m->nels, 128)), dim3(128), 0, c->stream, a);
-    CHK(check_launch(c, "k_adj_apply"));
-    AdjGatherArgs g;
-    g.ns = m->ns;
-    g.node_of = m->d_node_of; g.off = m->d_off; g.ent = m->d_ent;
-    g.slot = op.slot;
-    g.outx = ox; g.outy = oy;
-    g.objx = objx; g.objy = objy; g.PCx = PCx; g.PCy = PCy;
-    g.bcflag = s->d_bcflag;
-    hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m->ns, 256)), dim3(256), 0, c->stream, g);
-    return check_launch(c, "k_adj_gather");
-}
-
-// Σa², Σb², Σc² to the host
-static int adjoint_norms(fem_stokes* s, const double* a, const double* b, const double* cc, int64_t nc,
-                         double out[3]) {
-    fem_ctx* c = s->ctx;
-    const int64_t nv = s->mesh->nnodes;
-    const unsigned g = std::min<unsigned>(RED_GRID, grid_for(std::max(nv, nc), RED_BLOCK));
-    hipLaunchKernelGGL(k_adj_norms, dim3(g), dim3(RED_BLOCK), 0, c->stream, nv, nc, a, b, cc, c->red_partial);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, (int)g, 3, 0,
-                       (const double*)c->red_partial, c->red_out);
-    CHK(check_launch(c, "k_adj_norms"));
-    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::memcpy(out, c->h_red, 3 * 8);
-    return FEM_OK;
-}
-
-// _assert_frozen_symmetry (adjoint_operator.jl:149-160) on the stored blocks
-static int adjoint_check_symmetry(fem_stokes* s, const AdjointOp& op) {
-    fem_ctx* c = s->ctx;
-    const int64_t ne = s->mesh->nels;
-    std::vector<double> A((size_t)ANU * ANU * ne), B((size_t)ANU * ANP * ne), C((size_t)ANU * ANP * ne);
-    HIPCHK(c, hipMemcpy(A.data(), op.A, A.size() * 8, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(B.data(), op.B, B.size() * 8, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(C.data(), op.C, C.size() * 8, hipMemcpyDeviceToHost));
-    double wA = 0, wC = 0;
-    for (int64_t e = 0; e < ne; ++e) {
-        double nA = 0, dA = 0, nC = 0, dC = 0;
-        for (int i = 0; i < ANU; ++i)
-            for (int j = 0; j < ANU; ++j) {
-                const double x = A[(i * ANU + j) * ne + e], y = A[(j * ANU + i) * ne + e];
-                nA += x * x;
-                dA += (x - y) * (x - y);
-            }
-        for (int i = 0; i < ANP; ++i)
-            for (int k = 0; k < ANU; ++k) {
-                const double x = C[(i * ANU + k) * ne + e], y = B[(k * ANP + i) * ne + e];
-                nC += x * x;
-                dC += (x - y) * (x - y);
-            }
-        wA = std::max(wA, nA == 0 ? 0.0 : std::sqrt(dA) / std::sqrt(nA));
-        wC = std::max(wC, nC == 0 ? 0.0 : std::sqrt(dC) / std::sqrt(nC));
-    }
-    const double tol = std::sqrt(std::numeric_limits<double>::epsilon());
-    if (wA > tol || wC > tol)
-        return set_err(c, FEM_EARG,
-                       "the frozen adjoint operator assumes the element tangent is symmetric, but the assembled "
-                       "blocks disagree: max ‖A - Aᵀ‖/‖A‖ = %g, max ‖C - Bᵀ‖/‖C‖ = %g over %lld elements",
-                       wA, wC, (long long)ne);
-    return FEM_OK;
-}
-
-}  // namespace
 
 static int adjoint_checks(fem_stokes* s) {
     if (s->dim != 2) return set_err(s->ctx, FEM_EARG, "the frozen-operator adjoint runs on 2-D (T7/P1-disc) states");

@@ -105,7 +105,8 @@ class _SolveParams(C.Structure):
     _fields_ = [("dt", D), ("pressure_factor", D), ("tolerance", D), ("max_iterations", I64), ("iter_max", I64),
                 ("check_interval", I64), ("max_ph_iterations", I64), ("rel_drop0", D), ("finite_K", I32),
                 ("freeze_jacobian", I32), ("throw_on_failure", I32), ("use_tau_old", I32), ("skip_scaling", I32),
-                ("verbose", I32), ("precision", I32), ("time_kernels", I32)]
+                ("verbose", I32), ("precision", I32), ("time_kernels", I32), ("measure_lmax", I32),
+                ("lmax_power_iterations", I32), ("lmax_power_rtol", D), ("lmax_safety", D)]
 
 
 class _SolveInputs(C.Structure):
@@ -999,7 +1000,8 @@ def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_facto
                  max_iterations=50_000, check_interval=50, iterMax=50_000, max_ph_iterations=1000, rel_drop0=1e-2,
                  plastic=None, phases_v=None, phases_P=None, finite_K=None, freeze_jacobian=None,
                  throw_on_failure=True, tau_old=True, skip_scaling=False, verbose=False, time_kernels=False,
-                 precision="fp64"):
+                 precision="fp64", measure_lmax=False, lmax_power_iterations=12, lmax_power_rtol=1e-2,
+                 lmax_safety=1.1):
     """Checks and C arguments of one solve! call: (bc array, plastic pointer, params)."""
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
@@ -1024,7 +1026,9 @@ def _stokes_call(dr: StokesDR, mesh: MixedMesh, bcs, keep, *, dt, pressure_facto
                      freeze_jacobian=-1 if freeze_jacobian is None else int(bool(freeze_jacobian)),
                      throw_on_failure=int(bool(throw_on_failure)), use_tau_old=int(bool(tau_old)),
                      skip_scaling=int(bool(skip_scaling)), verbose=int(verbose),
-                     precision=_PRECISIONS[precision], time_kernels=int(bool(time_kernels)))
+                     precision=_PRECISIONS[precision], time_kernels=int(bool(time_kernels)),
+                     measure_lmax=int(bool(measure_lmax)), lmax_power_iterations=int(lmax_power_iterations),
+                     lmax_power_rtol=float(lmax_power_rtol), lmax_safety=float(lmax_safety))
     return arr, dp, p
 
 

@@ -257,6 +257,12 @@ typedef struct {
                                    near yield); 2 = as 1, with the 2-D return
                                    linearised about x at every fold (3-D: as 1) */
     int32_t time_kernels;       /* 1: HIP events around every element / gather launch */
+    int32_t measure_lmax;       /* measure_λmax (DR.jl:210-211, 375-378): 1 = power iteration on the
+                                   Jacobi-scaled velocity operator at every Jacobian assembly,
+                                   λmax = min(Gershgorin, safety · measured); 2-D FP64 solves only */
+    int32_t lmax_power_iterations;  /* λmax_power_iterations, 12 (0 reads as 12)  */
+    double lmax_power_rtol;     /* λmax_power_rtol, 1e-2 (0 reads as 1e-2)           */
+    double lmax_safety;         /* λmax_safety, 1.1 (0 reads as 1.1); ≥ 1.5 with a yield model */
 } fem_solve_params;
 
 /* stats NamedTuple (DR.jl:783-801) + device timing */

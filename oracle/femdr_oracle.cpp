@@ -1086,6 +1086,86 @@ void orc_dr_iterations(orc_stokes* s, double dt, int32_t finite_K, const double*
     }
 }
 
+}  // extern "C"
+
+// assemble_velocity_operator (adjoint_operator.jl:205-224): the 14×14 element
+// blocks [nels][14][14], velocity DoFs stacked by component
+static void velocity_blocks(const orc_stokes* s, double dt, const double* bulk, std::vector<double>& A) {
+    A.resize((size_t)s->nels * 196);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < s->nels; ++e) {
+        double block[2][2][7][7];
+        augmented_element_block(s, e, dt, bulk, block);
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int r = 0; r < 7; ++r)
+                    for (int c = 0; c < 7; ++c) A[e * 196 + (i * 7 + r) * 14 + j * 7 + c] = block[i][j][r][c];
+    }
+}
+
+// estimate_velocity_λmax (adjoint_operator.jl:240-281): x, y hold the start
+// (the seed or the previous estimate's vector) and return the last iterate
+static bool estimate_velocity_lmax(const orc_stokes* s, const std::vector<double>& A, std::vector<double> x[2],
+                                   int max_it, double rtol, double* lam_out, int64_t* its) {
+    const int64_t nv = s->nnodes, ne = s->nels;
+    const double* PC[2] = {s->PCx, s->PCy};
+    for (int c = 0; c < 2; ++c) dirichlet(x[c].data(), s->nbc[c], s->bc_dofs[c], nullptr);
+    std::vector<double> z[2], a[2], Re[2];
+    for (int c = 0; c < 2; ++c) { z[c].resize(nv); a[c].resize(nv); Re[c].resize((size_t)ne * 7); }
+    double lam = 0;
+    for (int it = 1; it <= max_it; ++it) {
+        double sx = 0, sy = 0;
+        for (int64_t i = 0; i < nv; ++i) sx += x[0][i] * x[0][i];
+        for (int64_t i = 0; i < nv; ++i) sy += x[1][i] * x[1][i];
+        const double n = std::sqrt(sx + sy);
+        if (!(n > 0)) { g_err = "power iteration collapsed to the zero vector"; return false; }
+        for (int c = 0; c < 2; ++c)
+            for (int64_t i = 0; i < nv; ++i) {
+                x[c][i] /= n;
+                z[c][i] = x[c][i] / std::sqrt(PC[c][i]);
+            }
+        // velocity_operator_apply_kernel!: A · (x_loc; y_loc) per element, summed per node
+        for (int64_t e = 0; e < ne; ++e) {
+            double u[14];
+            for (int k = 0; k < 7; ++k) { u[k] = z[0][s->el2n[e * 7 + k]]; u[7 + k] = z[1][s->el2n[e * 7 + k]]; }
+            for (int r = 0; r < 14; ++r) {
+                double acc = 0;
+                for (int k = 0; k < 14; ++k) acc += A[e * 196 + r * 14 + k] * u[k];
+                Re[r / 7][e * 7 + r % 7] = acc;
+            }
+        }
+        gather(s, Re[0], a[0].data());
+        gather(s, Re[1], a[1].data());
+        for (int c = 0; c < 2; ++c) {
+            for (int64_t i = 0; i < nv; ++i) a[c][i] /= std::sqrt(PC[c][i]);
+            dirichlet(a[c].data(), s->nbc[c], s->bc_dofs[c], nullptr);
+        }
+        double ax = 0, ay = 0;
+        for (int64_t i = 0; i < nv; ++i) ax += a[0][i] * a[0][i];
+        for (int64_t i = 0; i < nv; ++i) ay += a[1][i] * a[1][i];
+        const double ln = std::sqrt(ax + ay);
+        x[0] = a[0];
+        x[1] = a[1];
+        if (it > 1 && std::fabs(ln - lam) <= rtol * ln) {
+            *lam_out = ln;
+            *its = it;
+            return true;
+        }
+        lam = ln;
+    }
+    if (!(std::isfinite(lam) && lam > 0)) {
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "power iteration produced a non-positive λmax: %g", lam);
+        g_err = buf;
+        return false;
+    }
+    *lam_out = lam;
+    *its = max_it;
+    return true;
+}
+
+extern "C" {
+
 // solve! (DR.jl:302-337) + _solve_stokes_dyrel! (DR.jl:348-802); th is the
 // thermal side of solve_coupled! (null for solve!).
 static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out, ThermalHook* th) {
@@ -1107,11 +1187,44 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
     for (int c = 0; c < 2; ++c) { std::fill(rate[c], rate[c] + nv, 0.0); std::fill(Rv0[c], Rv0[c] + nv, 0.0); }
     const char* labels[2] = {"stokes vx", "stokes vy"};
 
+    // measure_λmax (DR.jl:375-378, 394, 425-426, 459-490): the velocity blocks
+    // A of every Jacobian assembly and a power iteration on P^-1/2 A P^-1/2
+    // (estimate_velocity_λmax, adjoint_operator.jl:240-281), warm-started from
+    // the previous estimate's vector; λmax_v = min(Gershgorin, safety · measured)
+    if (p->measure_lmax && s->eta_ip) {
+        g_err = "quadrature viscosity requires measure_λmax=false";
+        out->status = -1;
+        return -1;
+    }
+    const int power_max = p->lmax_power_iterations > 0 ? p->lmax_power_iterations : 12;
+    const double power_rtol = p->lmax_power_rtol > 0 ? p->lmax_power_rtol : 1e-2;
+    double safety = p->lmax_safety > 0 ? p->lmax_safety : 1.1;
+    if (s->dp || s->cap) safety = std::max(safety, 1.5);   // _stokes_spectral_safety (DR.jl:931-933)
+    std::vector<double> px[2], Ablk;
+    int64_t power_its = 0;
+    auto measure = [&](double lmax[2]) -> bool {
+        if (!p->measure_lmax) return true;
+        velocity_blocks(s, dt, pressure_bulk(s, finite_K), Ablk);
+        if (px[0].empty())
+            for (int c = 0; c < 2; ++c) {
+                px[c].resize(nv);
+                for (int64_t i = 0; i < nv; ++i) px[c][i] = c == 0 ? std::sin((double)(i + 1) * 0.7) : std::cos((double)(i + 1) * 1.3);
+            }
+        double meas = 0;
+        int64_t used = 0;
+        if (!estimate_velocity_lmax(s, Ablk, px, power_max, power_rtol, &meas, &used)) return false;
+        power_its += used;
+        const double g = std::max(lmax[0], lmax[1]);
+        lmax[0] = lmax[1] = std::min(g, safety * meas);
+        return true;
+    };
+
     orc_augmented_jacobian(s, dt, finite_K);
     double lmax[2];
     for (int c = 0; c < 2; ++c)
         if (!checked_lmax(dRdv[c], PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
     double lmax_g = std::max(lmax[0], lmax[1]);
+    if (!measure(lmax)) { out->status = -1; return -1; }
     int64_t jac_assemblies = 1;
     double dtau[2], al[2], be[2];
     for (int c = 0; c < 2; ++c) {
@@ -1207,6 +1320,7 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
                     for (int c = 0; c < 2; ++c)
                         if (!checked_lmax(dRdv[c], PC[c], nv, labels[c], &lmax[c])) { out->status = -6; return -6; }
                     lmax_g = std::max(lmax[0], lmax[1]);
+                    if (!measure(lmax)) { out->status = -1; return -1; }
                 }
                 for (int c = 0; c < 2; ++c) {
                     dtau[c] = 2 / std::sqrt(lmax[c]) * p->CFL_v;
@@ -1230,6 +1344,7 @@ static int stokes_solve_impl(orc_stokes* s, const orc_params* p, orc_stats* out,
     out->lambda_max = std::max(lmax[0], lmax[1]);
     out->lambda_max_gershgorin = lmax_g;
     out->jacobian_assemblies = jac_assemblies;
+    out->lambda_max_iterations = power_its;
     out->status = 0;
     return 0;
 }

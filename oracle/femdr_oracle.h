@@ -125,6 +125,9 @@ typedef struct {
     int32_t freeze_jacobian;   /* -1: default (plastic == nothing)  */
     int32_t skip_scaling;      /* 1: gammaP/MP already assembled    */
     int32_t verbose;
+    int32_t measure_lmax;      /* measure_λmax (DR.jl:375-378): power iteration on A */
+    int32_t lmax_power_iterations;  /* 12 (0 reads as 12)         */
+    double lmax_power_rtol, lmax_safety;  /* 1e-2, 1.1 (0 reads as the default) */
 } orc_params;
 
 typedef struct {
@@ -134,6 +137,7 @@ typedef struct {
     double lambda_max, lambda_max_gershgorin;
     int64_t jacobian_assemblies;
     int32_t status;            /* 0 ok, <0 error (see orc_last_error) */
+    int64_t lambda_max_iterations;
 } orc_stats;
 
 const char* orc_last_error(void);

@@ -71,13 +71,15 @@ class Params(C.Structure):
     _fields_ = [("dt", D), ("pressure_factor", D), ("tolerance", D), ("max_iterations", I64),
                 ("iter_max", I64), ("check_interval", I64), ("max_ph_iterations", I64), ("rel_drop0", D),
                 ("CFL_v", D), ("c_fact", D), ("finite_K", I32), ("freeze_jacobian", I32),
-                ("skip_scaling", I32), ("verbose", I32)]
+                ("skip_scaling", I32), ("verbose", I32), ("measure_lmax", I32), ("lmax_power_iterations", I32),
+                ("lmax_power_rtol", D), ("lmax_safety", D)]
 
 
 class Stats(C.Structure):
     _fields_ = [("itPH", I64), ("iter", I64), ("err", D), ("err_abs", D), ("err_rel", D), ("err_v", D),
                 ("err_P", D), ("converged", I32), ("reached_total_iter", I32), ("lambda_max", D),
-                ("lambda_max_gershgorin", D), ("jacobian_assemblies", I64), ("status", I32)]
+                ("lambda_max_gershgorin", D), ("jacobian_assemblies", I64), ("status", I32),
+                ("lambda_max_iterations", I64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -546,11 +548,14 @@ class Problem:
     @staticmethod
     def params(dt, pressure_factor=50.0, tolerance=1e-6, max_iterations=50_000, iter_max=50_000,
                check_interval=50, max_ph_iterations=1000, rel_drop0=1e-2, CFL_v=0.98, c_fact=0.9,
-               finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False):
+               finite_K=False, freeze_jacobian=-1, skip_scaling=False, verbose=False, measure_lmax=False,
+               lmax_power_iterations=12, lmax_power_rtol=1e-2, lmax_safety=1.1):
         return Params(dt=dt, pressure_factor=pressure_factor, tolerance=tolerance, max_iterations=max_iterations,
                       iter_max=iter_max, check_interval=check_interval, max_ph_iterations=max_ph_iterations,
                       rel_drop0=rel_drop0, CFL_v=CFL_v, c_fact=c_fact, finite_K=int(finite_K),
-                      freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose))
+                      freeze_jacobian=int(freeze_jacobian), skip_scaling=int(skip_scaling), verbose=int(verbose),
+                      measure_lmax=int(bool(measure_lmax)), lmax_power_iterations=int(lmax_power_iterations),
+                      lmax_power_rtol=float(lmax_power_rtol), lmax_safety=float(lmax_safety))
 
     def solve(self, dt, collect_history=False, **kw):
         p = self.params(dt, **kw)

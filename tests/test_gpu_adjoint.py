@@ -225,3 +225,69 @@ def test_gpu_adjoint_errors():
         fa.solve_adjoint(dr, mesh, *bcs, dt=1.0, **budget, **z)
     st = fa.solve_adjoint(dr, mesh, *bcs, dt=1.0, throw_on_failure=False, **budget, **z)
     assert not st["converged"] and st["iter"] == 2
+
+
+def test_gpu_forward_measure_lmax_parity():
+    # measure_λmax of the forward solve (DR.jl:375-378, 459-490, 728-760): the
+    # same velocity blocks drive a power iteration at every Jacobian assembly
+    case = buoyancy_case()
+    coords, el2n, ph, xw, yw, obs = case
+    say("[gpu] forward solve with measure_lmax")
+    mat_kw = dict(eta=(1.0, 1.0), etab=(INF, INF), alpha=(0.0, 0.0), rho0=(1.0, 2.0), K=(INF, INF), g=(0.0, -1.0),
+                  Tref=0.0)
+    pr = O.Problem(coords, el2n, O.material(**mat_kw), phases_v=ph, phases_P=ph,
+                   bc=((xw, np.zeros(len(xw))), (yw, np.zeros(len(yw)))))
+    kw = dict(pressure_factor=20.0, tolerance=1e-9, check_interval=100, rel_drop0=0.1)
+    so = pr.solve(1.0, CFL_v=0.9, c_fact=0.7, measure_lmax=True, **kw)
+    mesh = fa.t7_box_mesh(3, 3)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(**mat_kw), CFL_v=0.9, c_fact=0.7)
+    bcs = (fa.DirichletBoundaryCondition(xw, np.zeros(len(xw))), fa.DirichletBoundaryCondition(yw, np.zeros(len(yw))))
+    sg = fa.solve(dr, mesh, *bcs, dt=1.0, phases_v=ph, phases_P=ph, measure_lmax=True, **kw)
+    say("  oracle", {k: so[k] for k in ("iter", "itPH", "lambda_max", "lambda_max_iterations")},
+        "gpu", {k: sg[k] for k in ("iter", "itPH", "lambda_max", "lambda_max_iterations")})
+    assert so["converged"] and sg["converged"]
+    assert sg["iter"] == so["iter"] and sg["itPH"] == so["itPH"]
+    assert sg["lambda_max_iterations"] == so["lambda_max_iterations"] > 0
+    assert sg["lambda_max"] == pytest.approx(so["lambda_max"], rel=1e-12)
+    assert sg["lambda_max"] < sg["lambda_max_gershgorin"]
+    for a, b in ((dr.v.x, pr.vx), (dr.v.y, pr.vy), (dr.P, pr.P)):
+        assert rel(a.numpy(), b) < 1e-10
+    with pytest.raises(ValueError, match="precision 0"):
+        fa.solve(dr, mesh, *bcs, dt=1.0, measure_lmax=True, precision="fp32", **kw)
+
+
+def test_gpu_forward_measure_lmax_plastic_parity():
+    # with a yield model the Jacobian and the estimate are renewed at every
+    # check (warm-started power vectors, safety ≥ 1.5)
+    from femtools_amd import problems as PB
+    say("[gpu] forward VEP solve with measure_lmax")
+    n = 10
+    w = PB.vep_inclusions(n, ninc=3, seed=5, nsteps=1, C=0.5)
+    coords, el2n = O.t7_mesh(n, n)
+    ph = PB.inclusion_phases(coords, el2n, w.circles)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(coords)
+    ne = len(el2n)
+    dp = dict(phi=w.plastic["phi"], psi=w.plastic["psi"], C_=w.plastic["C"], eta_reg=w.plastic["eta_reg"],
+              Kb=w.plastic["Kb"])
+    mat_kw = dict(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0, K=w.K, g=w.g)
+    pr = O.Problem(coords, el2n, O.material(**mat_kw), dp=O.drucker_prager(**dp), phases_v=ph, phases_P=ph,
+                   tau_old=tuple(np.zeros((ne, 7)) for _ in range(3)), bc=((bx, vx), (by, vy)))
+    v0x, v0y = PB.pure_shear_field(coords)
+    pr.vx[:], pr.vy[:] = v0x, v0y
+    kw = dict(pressure_factor=w.pressure_factor, tolerance=1e-8, check_interval=w.check_interval,
+              rel_drop0=w.rel_drop0)
+    so = pr.solve(w.dt, CFL_v=w.CFL_v, measure_lmax=True, **kw)
+    mesh = fa.t7_box_mesh(n, n)
+    dr = fa.StokesDR(mesh, fa.StokesMaterial(**mat_kw), CFL_v=w.CFL_v)
+    dr.v.x.copyto(v0x)
+    dr.v.y.copyto(v0y)
+    sg = fa.solve(dr, mesh, fa.DirichletBoundaryCondition(bx, vx), fa.DirichletBoundaryCondition(by, vy), dt=w.dt,
+                  phases_v=ph, phases_P=ph, measure_lmax=True,
+                  plastic=fa.DruckerPrager(dp["phi"], dp["psi"], dp["C_"], dp["eta_reg"], dp["Kb"]), **kw)
+    say("  oracle", {k: so[k] for k in ("iter", "itPH", "jacobian_assemblies", "lambda_max_iterations")},
+        "gpu", {k: sg[k] for k in ("iter", "itPH", "jacobian_assemblies", "lambda_max_iterations")})
+    assert so["converged"] and sg["converged"]
+    assert sg["iter"] == so["iter"] and sg["jacobian_assemblies"] == so["jacobian_assemblies"] > 1
+    assert sg["lambda_max_iterations"] == so["lambda_max_iterations"]
+    for a, b in ((dr.v.x, pr.vx), (dr.v.y, pr.vy), (dr.P, pr.P)):
+        assert rel(a.numpy(), b) < 1e-9
