@@ -442,6 +442,7 @@ int fem_stokes_destroy(fem_stokes* s) {
     hipStreamSynchronize(s->ctx->stream);
     for (int k = 0; k < FEM_FIELD_COUNT; ++k) hipFree(s->f[k]);
     hipFree(s->d_phase_v); hipFree(s->d_phase_P); hipFree(s->d_pscale); hipFree(s->d_Re);
+    hipFree(s->d_bpart64); hipFree(s->d_Rn64);
     hipFree(s->d_Jrow); hipFree(s->d_Jdiag); hipFree(s->d_bcflag); hipFree(s->d_bcval_x);
     hipFree(s->d_bcval_y); hipFree(s->d_bc_stage); hipFree(s->d_bc_stage_v); hipFree(s->d_ptab);
     hipFree(s->d_flag);
@@ -650,8 +651,10 @@ constexpr int EBLOCK = 128;
 constexpr int GBLOCK = 256;
 // EF_TILE blocks: 16 x rows cells, 32 threads per cell row (fem_tuning.tile_rows)
 
-// FP32 inner iterations of structured 2-D meshes whose cell counts are
-// multiples of the 16 x rows tile run the EF_TILE element kernel
+// Inner iterations of structured 2-D meshes whose cell counts are multiples
+// of the 16 x rows tile run the EF_TILE element kernel (FP32 corrections at
+// 4, 8 or 16 rows; FP64 at 4 or 8: the FP64 τ_old and slot stages of a
+// 16-row tile would leave one block per CU)
 static int tile_rows(const fem_mesh* m) { return m->ctx->tune.tile_rows; }
 static bool tile_shape_ok(const fem_mesh* m, int rows) {
     return m->dim == 2 && m->structured && m->nx > 0 && m->nx % 16 == 0 && m->ny % rows == 0;
@@ -659,6 +662,7 @@ static bool tile_shape_ok(const fem_mesh* m, int rows) {
 static bool tile_ok(const fem_mesh* m) { return m->ctx->tune.tiles && tile_shape_ok(m, tile_rows(m)); }
 static int64_t tile_count(const fem_mesh* m, int rows) { return (int64_t)(m->nx / 16) * (m->ny / rows); }
 static int tile_perimeter_nodes(int rows) { return 2 * (32 + 1) + 2 * (2 * rows - 1); }
+static bool tile64_ok(const fem_mesh* m) { return tile_ok(m) && tile_rows(m) <= 8; }
 
 // One set of DR arrays in precision F. The FP64 solve runs on the state's own
 // fields; a correction solve of the mixed-precision path runs on FP32 work
@@ -733,11 +737,13 @@ template <class F, int FL>
 static void launch_elem_t(fem_stokes* s, const ElemArgs<F>& a) {
     if constexpr ((FL & EF_TILE) != 0) {
         const dim3 grid(grid_xcd(a.ntiles, 1));
-        switch (tile_rows(s->mesh)) {
-            case 4: hipLaunchKernelGGL((k_t7_element<F, FL, 128>), grid, dim3(128), 0, s->ctx->stream, a); break;
-            case 16: hipLaunchKernelGGL((k_t7_element<F, FL, 512>), grid, dim3(512), 0, s->ctx->stream, a); break;
-            default: hipLaunchKernelGGL((k_t7_element<F, FL, 256>), grid, dim3(256), 0, s->ctx->stream, a); break;
-        }
+        const int rows = tile_rows(s->mesh);
+        if (rows == 4) hipLaunchKernelGGL((k_t7_element<F, FL, 128>), grid, dim3(128), 0, s->ctx->stream, a);
+        else if constexpr (std::is_same<F, float>::value) {
+            if (rows == 16) hipLaunchKernelGGL((k_t7_element<F, FL, 512>), grid, dim3(512), 0, s->ctx->stream, a);
+            else hipLaunchKernelGGL((k_t7_element<F, FL, 256>), grid, dim3(256), 0, s->ctx->stream, a);
+        } else
+            hipLaunchKernelGGL((k_t7_element<F, FL, 256>), grid, dim3(256), 0, s->ctx->stream, a);
     } else
         hipLaunchKernelGGL((k_t7_element<F, FL, EBLOCK>), dim3(grid_xcd(s->mesh->nels, EBLOCK)), dim3(EBLOCK), 0,
                            s->ctx->stream, a);
@@ -747,6 +753,32 @@ template <int FL>
 static void dispatch_elem(int key, fem_stokes* s, const ElemArgs<double>& a) {
     if (key == FL) launch_elem_t<double, FL>(s, a);
     else if constexpr (FL > 0) dispatch_elem<FL - 1>(key, s, a);
+}
+
+// FP64 inner iterations in tiles: EF_ITER|EF_STRUCT|EF_TILE with any of
+// EF_ELASTIC, EF_PLASTIC, EF_BULK
+template <int I>
+static void dispatch_elem_tile(int key, fem_stokes* s, const ElemArgs<double>& a) {
+    constexpr int FL = EF_ITER | EF_STRUCT | EF_TILE | ((I & 1) ? EF_ELASTIC : 0) | ((I & 2) ? EF_PLASTIC : 0) |
+                       ((I & 4) ? EF_BULK : 0);
+    if (key == FL) launch_elem_t<double, FL>(s, a);
+    else if constexpr (I > 0) dispatch_elem_tile<I - 1>(key, s, a);
+}
+
+// Whether an inner-iteration launch runs in tiles: the element kernel and the
+// node update decide alike. FP64 tiles run the plain inner iteration only (no
+// per-point outputs, no temperature, sources, fluid pressure, cap or sampled
+// viscosity).
+template <class F>
+static bool tile_launch(const fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) {
+    const fem_mesh* m = s->mesh;
+    if (!d.tile || !o.use_pnum || m->nx <= 0) return false;
+    const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
+    if (nodeph) return false;
+    if (d.lin) return true;
+    const bool gen = o.thermal || s->pf_nonzero || s->q_nonzero || o.cap || s->in.viscosity ||
+                     (s->in.P_old && s->in.P_old_at_points);
+    return !gen && !o.store_tau && !o.phi_out && !o.tangent_out && !o.yield_count && !s->in.body_force[0];
 }
 
 // inner iterations on a correction: EF_LIN|EF_ITER with any of EF_BULK,
@@ -863,11 +895,18 @@ static int build_elem_args(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o
         a.bf_y = s->in.body_force[1];
         a.eta_ip = s->in.viscosity;
         a.Pold_ip = s->in.P_old_at_points ? s->in.P_old : nullptr;
-        fl = key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0);
+        const bool tile = tile_launch(s, d, o);
+        if (tile) {
+            a.ntx = m->nx / 16;
+            a.ntiles = tile_count(m, tile_rows(m));
+            a.bpart = d.bpart;
+            a.Rn = d.Rn;
+        }
+        fl = key | (o.elastic ? EF_ELASTIC : 0) | (o.plastic ? EF_PLASTIC : 0) | (tile ? EF_TILE : 0);
     } else {
         if (!d.lin || !o.use_pnum)
             return set_err(s->ctx, FEM_EARG, "FP32 element residuals exist for inner iterations on a correction only");
-        const bool tile = d.tile && strct && !gen;
+        const bool tile = tile_launch(s, d, o) && strct && !gen;
         if (tile) {
             a.ntx = m->nx / 16;
             a.ntiles = tile_count(m, tile_rows(m));
@@ -888,8 +927,10 @@ static int launch_element(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o)
     ElemArgs<F> a;
     int fl = 0;
     CHK(build_elem_args(s, d, o, a, fl));
-    if constexpr (std::is_same<F, double>::value) dispatch_elem<63>(fl, s, a);
-    else dispatch_lin<127>(fl, s, a);
+    if constexpr (std::is_same<F, double>::value) {
+        if (fl & EF_TILE) dispatch_elem_tile<7>(fl, s, a);
+        else dispatch_elem<63>(fl, s, a);
+    } else dispatch_lin<127>(fl, s, a);
     return check_launch(s->ctx, "k_t7_element");
 }
 
@@ -945,8 +986,7 @@ static int launch_gather(fem_stokes* s, const DRSet<F>& d, const LaunchOpts& o) 
     fem_mesh* m = s->mesh;
     GatherArgs<F> g;
     build_gather_args(s, d, o, g);
-    const bool nodeph = s->layout_v == FEM_PHASE_ELEMENT || s->layout_P == FEM_PHASE_ELEMENT;
-    if (d.tile && o.use_pnum && m->nx > 0 && !nodeph) {
+    if (tile_launch(s, d, o)) {
         TileUpdArgs<F> P;
         build_tile_update_args(s, d, g, P);
         hipLaunchKernelGGL((k_tile_update<F, GBLOCK>), dim3(grid_xcd(P.nT6, GBLOCK)), dim3(GBLOCK), 0,
@@ -2493,8 +2533,17 @@ static int solve_2d(fem_stokes* s, const fem_bc bcs[2], const fem_drucker_prager
         };
         ctl.thermal_converged = &cp->tc.converged;
     }
+    DRSet<double> d64 = set64(s);
+    if (tile64_ok(m)) {
+        // sized for 4-row tiles (the most perimeter nodes): any admitted row count fits
+        if (!s->d_bpart64) CHK(dalloc_t(c, &s->d_bpart64, tile_count(m, 4) * tile_perimeter_nodes(4)));
+        if (!s->d_Rn64) CHK(dalloc_t(c, &s->d_Rn64, (int64_t)(2 * m->nx + 1) * (2 * m->ny + 1)));
+        d64.tile = true;
+        d64.bpart = s->d_bpart64;
+        d64.Rn = s->d_Rn64;
+    }
     DRResult r;
-    const int rc = ph_dyrel<double>(s, set64(s), o, ctl, lmax, timer, r);
+    const int rc = ph_dyrel<double>(s, d64, o, ctl, lmax, timer, r);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     timer.collect();
     st->kernel_ms_element = timer.ms_elem;

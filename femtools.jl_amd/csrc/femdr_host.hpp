@@ -121,6 +121,8 @@ struct fem_stokes {
     int32_t layout_v = FEM_PHASE_CELL, layout_P = FEM_PHASE_CELL;
     bool phase_same = true;         // cell phases of v and P identical
     double* d_pscale = nullptr;
+    double2* d_bpart64 = nullptr;   // FP64 EF_TILE perimeter partial sums (structured 2-D meshes)
+    double2* d_Rn64 = nullptr;      // FP64 EF_TILE interior-node sums
     double2* d_Re = nullptr;
     double2* d_Jrow = nullptr;
     double2* d_Jdiag = nullptr;

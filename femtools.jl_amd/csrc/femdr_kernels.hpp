@@ -627,7 +627,13 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
     // τ_old of the block's elements is one contiguous chunk per component:
     // stream it straight into LDS (no registers held) while the velocity
     // gathers and the continuity residual proceed.
-    __shared__ F s_tau[TAU_LDS ? 3 * BLOCK * NQ : 1];
+    // with EF_TILE the slot sums of the tile stage reuse this stage once every
+    // τ_old read of the block is done (FP64: 168 B per thread instead of 264,
+    // so three blocks of 128 fit a CU's LDS next to the register budget)
+    constexpr bool SHARE_TAU = TAU_LDS && TILE;
+    constexpr int TAU_N = TAU_LDS ? 3 * BLOCK * NQ : 1;
+    constexpr int RE_N = (int)((BLOCK * NSH * sizeof(typename Vec2<F>::T) + sizeof(F) - 1) / sizeof(F));
+    __shared__ __align__(16) F s_tau[SHARE_TAU && RE_N > TAU_N ? RE_N : TAU_N];
     // hot: the block stages τ_old (some element is near yield); hot_e: this
     // element takes the FP64 returns (an element away from yield has none in
     // either state, so its correction operator is the visco-elastic one)
@@ -1077,7 +1083,9 @@ __device__ __forceinline__ void t7_element_body(const ElemArgs<F>& A, const int6
         // sum each tile node's slots in LDS in ascending element order (cell
         // rows, cells, triangles); k_tile_update adds the perimeter nodes'
         // partial sums of the neighbouring tiles and updates every node
-        __shared__ V2 s_re[TILE ? BLOCK * NSH : 1];
+        __shared__ V2 s_re_own[TILE && !SHARE_TAU ? BLOCK * NSH : 1];
+        V2* s_re = SHARE_TAU ? reinterpret_cast<V2*>(s_tau) : s_re_own;
+        if (SHARE_TAU) __syncthreads();   // the block's τ_old stage is read out
 #pragma unroll
         for (int a = 0; a < NSH; ++a) {
             V2 r; r.x = Rx[a]; r.y = Ry[a];
