@@ -92,6 +92,20 @@ def algorithmic_bytes_fp32_tile(w, nv, ne, npp, nt6):
     return 4 * (D * nv + 3 * npp + ne + D * nt6 + D * ne * 5)
 
 
+def algorithmic_bytes_fp64_tile(w, nv, ne, npp, nt6):
+    """Compulsory traffic of one launch of the FP64 inner-iteration kernel in
+    16 x 8-cell tiles (structured mesh): velocity (every node), pressure, the
+    previous-step pressure (finite bulk modulus), the scale gamma_P/M_P, the
+    stress history and the cell phase read; the summed residual of every grid
+    node written; the element-private bubble nodes updated (rate read and
+    written, preconditioner read, velocity written)."""
+    D, nq = 2, 7
+    elastic = any(math.isfinite(g) for g in w.G)
+    p_old = any(math.isfinite(b) for b in w.etab)
+    tau = 3 * nq * ne if elastic else 0
+    return 8 * (D * nv + npp + (npp if p_old else 0) + npp + tau + D * nt6 + D * ne * 4) + 4 * ne
+
+
 def read_traffic(kernel_prefix):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
     as committed in profiles/*traffic*.txt (line: '<kernel> traffic_bytes <N>')."""
@@ -558,7 +572,16 @@ def main():
         avg_s = ms_ve / max(n_ve, 1) * 1e-3
         traffic = read_traffic("k_t7_element<float, 364, 256>")
     achieved = kbytes / avg_s / 1e9 if avg_s > 0 else None
-    e64, it64 = algorithmic_bytes(w, nv, ne, npp)
+    tun = S.ctx.tuning
+    tile64 = bool(tun["tiles"]) and tun["tile_rows"] <= 8 and w.nx % 16 == 0 and w.ny % tun["tile_rows"] == 0
+    if tile64:
+        e64 = algorithmic_bytes_fp64_tile(w, nv, ne, npp, nt6)
+        name64 = "k_t7_element<double,EF_TILE> (fused FP64 element residual + bubble update, 16-cell-wide tiles)"
+        traffic64 = read_traffic("k_t7_element<double, 303")
+    else:
+        e64, _ = algorithmic_bytes(w, nv, ne, npp)
+        name64 = "k_t7_element<double> (fused FP64 element residual + bubble update, element-slot gather)"
+        traffic64 = None
     n64 = max(1, sum(s["timed_launches"] for s in ref))
     avg64 = sum(s["kernel_ms_element"] for s in ref) / n64 * 1e-3
     out = {
@@ -600,11 +623,11 @@ def main():
                      "avg_launch_us": round(avg_s * 1e6, 2),
                      "time_share": {"return_map": round(tot_rm / max(tot_rm + tot_ve, 1e-30), 3),
                                     "visco_elastic": round(tot_ve / max(tot_rm + tot_ve, 1e-30), 3)}},
-        "fp64_element_roofline": {"kernel": "k_t7_element<double> (fused FP64 element residual + bubble update)",
+        "fp64_element_roofline": {"kernel": name64,
                                   "algorithmic_bytes_per_launch": e64, "avg_launch_us": round(avg64 * 1e6, 2),
                                   "achieved_GBs": round(e64 / avg64 / 1e9, 1) if avg64 > 0 else None,
                                   "frac": round(e64 / avg64 / 1e9 / HBM_PEAK_GBS, 4) if avg64 > 0 else None,
-                                  "traffic": read_traffic("k_t7_element<double, 47, 128>")},
+                                  "traffic": traffic64},
     }
     if errors:
         out["errors"] = errors
