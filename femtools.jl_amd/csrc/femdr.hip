@@ -1339,7 +1339,10 @@ constexpr int ANU = 2 * NV, ANP = NP;
 
 struct AdjBlkArgs {
     int64_t nels;
+    const int32_t* el2n;            // [7][nels]
     const double* geo;              // [5][nels]
+    const double *vx, *vy, *P, *Pf;                  // the state (PLASTIC: the return's pressure derivative)
+    const double *tau_old0, *tau_old1, *tau_old2;    // nullable: no stress history
     const int32_t* phase_v; int32_t nodeph_v;   // cell [nels] or element-node [7][nels], 0-based
     const int32_t* phase_P; int32_t nodeph_P;   // cell [nels] or [3][nels]
     const double* ptab; int32_t nphases;
@@ -1350,9 +1353,13 @@ struct AdjBlkArgs {
 };
 
 // B, C, D, W of one element (element_adjoint_operator_blocks,
-// adjoint_operator.jl:318-373): with no yield model P enters the momentum
-// residual through the total pressure and the density ρ0(1 − α(T − Tref) +
-// βP), and the continuity residual through the storage term.
+// adjoint_operator.jl:318-373): P enters the momentum residual through the
+// total pressure, the density ρ0(1 − α(T − Tref) + βP) and, with the
+// Drucker-Prager model, the return (∂τ/∂P at each point from the return's
+// forward-mode derivative at the trial stress of the current state, with the
+// per-phase viscosity as the residual uses it), and the continuity residual
+// through the storage term.
+template <bool PLASTIC>
 __global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
     const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
     if (e >= A.nels) return;
@@ -1367,12 +1374,61 @@ __global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
         for (int i = 0; i < ANP; ++i) { B[r][i] = 0; Bn[r][i] = 0; C[i][r] = 0; }
     for (int i = 0; i < ANP; ++i)
         for (int j = 0; j < ANP; ++j) D[i][j] = 0;
+    double vx[NV], vy[NV], Pl[NP], Pfl[NP];
+    if (PLASTIC) {
+        for (int a = 0; a < NV; ++a) {
+            const int32_t n = A.el2n[a * ne + e];
+            vx[a] = A.vx[n];
+            vy[a] = A.vy[n];
+        }
+        for (int i = 0; i < NP; ++i) {
+            Pl[i] = A.P[e * NP + i];
+            Pfl[i] = A.Pf ? A.Pf[e * NP + i] : 0.0;
+        }
+    }
     for (int q = 0; q < NQ; ++q) {
         const double dOm = kT.w[q] * det;
         double r0 = 0, be = 0;
         for (int a = 0; a < NV; ++a) {
             r0 += kT.N[q][a] * A.ptab[PC_RHO0 * MAXPH + pv[a]];
             be += kT.N[q][a] * A.ptab[PC_BETA * MAXPH + pv[a]];
+        }
+        double dtxx = 0, dtyy = 0, dtxy = 0;   // ∂τ/∂P at the point
+        if (PLASTIC) {
+            double ax = 0, ay = 0, bx = 0, by = 0;
+            for (int a = 0; a < NV; ++a) {
+                const double d0 = kT.dN[q][a][0], d1 = kT.dN[q][a][1];
+                ax += d0 * vx[a]; ay += d1 * vx[a]; bx += d0 * vy[a]; by += d1 * vy[a];
+            }
+            const double gxx = ax * i00 + ay * i10, gxy = ax * i01 + ay * i11;
+            const double gyx = bx * i00 + by * i10, gyy = bx * i01 + by * i11;
+            auto at = [&](int col) {
+                double acc = 0;
+                for (int a = 0; a < NV; ++a) acc += kT.N[q][a] * A.ptab[col * MAXPH + pv[a]];
+                return acc;
+            };
+            IpMat<double> m;
+            const double etaq = at(PC_ETA), invG = at(PC_INVG);
+            m.etave = 1.0 / (1.0 / etaq + invG / A.dt);
+            m.inv2Gdt = invG / (2 * A.dt);
+            const double sphi = at(PC_SINPHI), spsi = at(PC_SINPSI);
+            m.yield0 = at(PC_COSPHI) * at(PC_C);
+            m.sphi = sphi;
+            m.invden = 1.0 / ((m.etave + at(PC_EREG)) + ((at(PC_KB) * A.dt) * (-spsi)) * (-sphi));
+            double txo = 0, tyo = 0, txyo = 0;
+            if (A.tau_old0) {
+                txo = A.tau_old0[e * NQ + q]; tyo = A.tau_old1[e * NQ + q]; txyo = A.tau_old2[e * NQ + q];
+            }
+            double Pq = 0;
+            for (int i = 0; i < NP; ++i) Pq += kT.NP1[q][i] * Pl[i];
+            for (int i = 0; i < NP; ++i) Pq -= kT.NP1[q][i] * Pfl[i];
+            D3<double> txx, tyy, txy;
+            stress2d<D3<double>, double, true, D3<double>>(D3<double>(gxx), D3<double>(gyy),
+                                                           D3<double>((gxy + gyx) * 0.5), m, txo, tyo, txyo,
+                                                           D3<double>(Pq, 1.0, 0.0, 0.0), txx, tyy, txy);
+            dtxx = txx.d0;
+            dtyy = tyy.d0;
+            dtxy = txy.d0;
         }
         double eb = INFINITY;
         if (!A.bulk_inf) {
@@ -1385,8 +1441,13 @@ __global__ __launch_bounds__(128) void k_adj_blocks(AdjBlkArgs A) {
             for (int i = 0; i < NP; ++i) {
                 const double Np = kT.NP1[q][i];
                 const double drho = r0 * (be * Np);
-                B[a][i] += (Gx * (-Np) - Na * (drho * A.gx)) * dOm;
-                B[NV + a][i] += (Gy * (-Np) - Na * (drho * A.gy)) * dOm;
+                if (PLASTIC) {
+                    B[a][i] += ((Gx * (dtxx * Np - Np) + Gy * (dtxy * Np)) - Na * (drho * A.gx)) * dOm;
+                    B[NV + a][i] += ((Gx * (dtxy * Np) + Gy * (dtyy * Np - Np)) - Na * (drho * A.gy)) * dOm;
+                } else {
+                    B[a][i] += (Gx * (-Np) - Na * (drho * A.gx)) * dOm;
+                    B[NV + a][i] += (Gy * (-Np) - Na * (drho * A.gy)) * dOm;
+                }
                 Bn[a][i] += (Gx * (-Np)) * dOm;
                 Bn[NV + a][i] += (Gy * (-Np)) * dOm;
                 C[i][a] += (Np * (-Gx)) * dOm;
@@ -1573,14 +1634,14 @@ struct AdjointOp {
 };
 
 // the blocks at the state in s; also fills ∂Rv∂v and PC_v from A
-static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op) {
+static int adjoint_assemble(fem_stokes* s, const fem_drucker_prager* dp, double dt, AdjWork& w, AdjointOp& op) {
     fem_ctx* c = s->ctx;
     fem_mesh* m = s->mesh;
     const int64_t ne = m->nels;
     LaunchOpts o;
-    CHK(prepare(s, nullptr, dt, 0, o, true));   // PC_BULK = ηb: the operator's pressure bulk
+    CHK(prepare(s, dp, dt, 0, o, true));   // PC_BULK = ηb: the operator's pressure bulk
     const fem_material& mt = s->mat;
-    op.sym = true;
+    op.sym = dp == nullptr;                // a yield model makes the tangent non-normal
     op.stores = false;
     for (int p = 0; p < mt.nphases; ++p) {
         if (!std::isinf(mt.K[p])) op.sym = false;
@@ -1599,7 +1660,16 @@ static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op)
     CHK(launch_jacobian(s, o));
     AdjBlkArgs b;
     b.nels = ne;
+    b.el2n = m->d_el2n;
     b.geo = m->d_geo;
+    b.vx = s->f[FEM_FIELD_V_X];
+    b.vy = s->f[FEM_FIELD_V_Y];
+    b.P = s->f[FEM_FIELD_P];
+    b.Pf = s->pf_nonzero ? s->f[FEM_FIELD_PF] : nullptr;
+    const bool old = o.elastic;
+    b.tau_old0 = old ? s->f[FEM_FIELD_TAU_OLD_XX] : nullptr;
+    b.tau_old1 = old ? s->f[FEM_FIELD_TAU_OLD_YY] : nullptr;
+    b.tau_old2 = old ? s->f[FEM_FIELD_TAU_OLD_XY] : nullptr;
     b.phase_v = s->d_phase_v;
     b.nodeph_v = s->layout_v == FEM_PHASE_ELEMENT;
     b.phase_P = s->d_phase_P;
@@ -1612,7 +1682,8 @@ static int adjoint_assemble(fem_stokes* s, double dt, AdjWork& w, AdjointOp& op)
     b.dt = dt;
     b.bulk_inf = o.bulk_inf;
     b.B = op.B; b.C = op.C; b.D = op.D; b.W = op.W;
-    hipLaunchKernelGGL(k_adj_blocks, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
+    if (dp) hipLaunchKernelGGL(k_adj_blocks<true>, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
+    else hipLaunchKernelGGL(k_adj_blocks<false>, dim3(grid_for(ne, 128)), dim3(128), 0, c->stream, b);
     CHK(check_launch(c, "k_adj_blocks"));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FEM_OK;
@@ -2581,8 +2652,8 @@ static int adjoint_checks(fem_stokes* s) {
     return set_err(s->ctx, FEM_EARG, "dr.γP is unassembled; run the forward solve! on this state before solve_adjoint!");
 }
 
-extern "C" int fem_stokes_adjoint_blocks(fem_stokes* s, double dt, double* host_A, double* host_B, double* host_C,
-                                         double* host_D, double* host_W) {
+extern "C" int fem_stokes_adjoint_blocks(fem_stokes* s, const fem_drucker_prager* plastic, double dt, double* host_A,
+                                         double* host_B, double* host_C, double* host_D, double* host_W) {
     if (!s || !host_A || !host_B || !host_C || !host_D || !host_W) return FEM_EARG;
     fem_ctx* c = s->ctx;
     HIPCHK(c, hipSetDevice(c->device));
@@ -2590,7 +2661,7 @@ extern "C" int fem_stokes_adjoint_blocks(fem_stokes* s, double dt, double* host_
     AdjWork w;
     w.c = c;
     AdjointOp op;
-    CHK(adjoint_assemble(s, dt, w, op));
+    CHK(adjoint_assemble(s, plastic, dt, w, op));
     const int64_t ne = s->mesh->nels;
     struct { const double* d; double* h; int rows; } out[5] = {
         {op.A, host_A, ANU * ANU}, {op.B, host_B, ANU * ANP}, {op.C, host_C, ANU * ANP}, {op.D, host_D, ANP * ANP},
@@ -2607,9 +2678,10 @@ extern "C" int fem_stokes_adjoint_blocks(fem_stokes* s, double dt, double* host_
 
 // solve_adjoint!(dr, mesh, bc_v; dt, objective_v, λv, λP, ...) with operator =
 // :blocks (DR_adjoint.jl:396-427 -> _solve_stokes_adjoint_dyrel!, :1-268)
-extern "C" int fem_stokes_solve_adjoint(fem_stokes* s, const fem_bc* bc_v, const double* objective_x,
-                                        const double* objective_y, double* lam_x, double* lam_y, double* lam_P,
-                                        const fem_adjoint_params* p, fem_adjoint_stats* st) {
+extern "C" int fem_stokes_solve_adjoint(fem_stokes* s, const fem_bc* bc_v, const fem_drucker_prager* plastic,
+                                        const double* objective_x, const double* objective_y, double* lam_x,
+                                        double* lam_y, double* lam_P, const fem_adjoint_params* p,
+                                        fem_adjoint_stats* st) {
     if (!s || !bc_v || !objective_x || !objective_y || !lam_x || !lam_y || !lam_P || !p || !st) return FEM_EARG;
     fem_ctx* c = s->ctx;
     std::memset(st, 0, sizeof *st);
@@ -2623,7 +2695,7 @@ extern "C" int fem_stokes_solve_adjoint(fem_stokes* s, const fem_bc* bc_v, const
     AdjWork w;
     w.c = c;
     AdjointOp op;
-    CHK(adjoint_assemble(s, p->dt, w, op));
+    CHK(adjoint_assemble(s, plastic, p->dt, w, op));
     if (op.sym) CHK(adjoint_check_symmetry(s, op));
     double* PCx = s->f[FEM_FIELD_PC_V_X];
     double* PCy = s->f[FEM_FIELD_PC_V_Y];

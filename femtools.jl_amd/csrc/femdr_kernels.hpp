@@ -143,8 +143,9 @@ struct IpMat {
 // Deviatoric stress with the plane-strain Drucker-Prager return
 // (rheology.jl:226-245, 272-334). S is F, or D3<F> for the tangent. Returns
 // whether the point yields.
-template <class S, class F, bool PLASTIC>
-__device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, F Pq,
+// PQ is F, or S to carry the derivative with respect to the point pressure.
+template <class S, class F, bool PLASTIC, class PQ = F>
+__device__ inline bool stress2d(S exx, S eyy, S exy, const IpMat<F>& m, F txo, F tyo, F txyo, PQ Pq,
                                 S& txx, S& tyy, S& txy, F* fyield = nullptr, float* tan = nullptr) {
     bool yields = false;
     S tr = (exx + eyy) * F(1.0 / 3.0);

@@ -256,8 +256,9 @@ _sig("fem_cnn_gemm_bf16", C.c_int, VP, I64, I64, I64, VP, VP, VP)
 _sig("fem_cell_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(VP), _P(VP), VP, _P(_CellParams), _P(_CellStats),
      _P(D), I64, _P(I64))
 _sig("fem_cell_stokes_material_gradient", C.c_int, VP, _P(VP), _P(VP), _P(D), _P(D))
-_sig("fem_stokes_solve_adjoint", C.c_int, VP, _P(_BC), VP, VP, VP, VP, VP, _P(_AdjointParams), _P(_AdjointStats))
-_sig("fem_stokes_adjoint_blocks", C.c_int, VP, D, _P(D), _P(D), _P(D), _P(D), _P(D))
+_sig("fem_stokes_solve_adjoint", C.c_int, VP, _P(_BC), _P(_DP), VP, VP, VP, VP, VP, _P(_AdjointParams),
+     _P(_AdjointStats))
+_sig("fem_stokes_adjoint_blocks", C.c_int, VP, _P(_DP), D, _P(D), _P(D), _P(D), _P(D), _P(D))
 _sig("fem_stokes_set_pressure_source", C.c_int, VP, VP, D, I32)
 _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveParams), _P(_CoupledParams),
      _P(_SolveStats), _P(_CoupledStats))
@@ -1562,14 +1563,16 @@ def solve_stokes_adjoint(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundary
     """solve_adjoint!(dr::StokesDR, mesh, bc_vx, bc_vy; dt, objective_vx,
     objective_vy, λvx, λvy, λP, ...) (DR_adjoint.jl:396-427) with the stored
     element blocks (operator = :blocks) on a converged 2-D forward state without
-    a yield model. lam_* are DeviceArrays updated in place (their values are the
+    a yield model or with the Drucker-Prager model (plastic; not the tensile
+    cap). lam_* are DeviceArrays updated in place (their values are the
     initial iterate) or host arrays (then the results are in the returned
     dict as lam_vx, lam_vy, lam_P). Returns (; itPH, iter, err, err_v, err_P,
     converged, iterations, residual, lmax, lmax_gershgorin, lmax_iterations)."""
     if operator != "blocks":
         raise ValueError(f"operator must be 'blocks' in this build, got {operator!r}")
-    if plastic is not None:
-        raise ValueError("the frozen-operator adjoint covers models without a yield model")
+    if _is_cap(plastic):
+        raise ValueError("the frozen-operator adjoint covers models without a yield model or with the "
+                         "Drucker-Prager model")
     if mesh is not dr.mesh:
         raise ValueError("dr was built on a different mesh")
     if mesh.dim != 2:
@@ -1592,10 +1595,12 @@ def solve_stokes_adjoint(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundary
                        measure_lmax=int(bool(measure_lmax)), throw_on_failure=int(bool(throw_on_failure)),
                        verbose=int(bool(verbose)))
     st = _AdjointStats()
+    d, dp = _dp_ptr(plastic)
+    keep.append(d)
     with _Inputs(dr) as io:
         ox, oy = io._dev(objective_vx, (nv,)), io._dev(objective_vy, (nv,))
         lams = [io._dev(a, (n,)) for a, n in ((lam_vx, nv), (lam_vy, nv), (lam_P, npp))]
-        rc = _lib.fem_stokes_solve_adjoint(dr.handle, arr, ox, oy, *lams, C.byref(p), C.byref(st))
+        rc = _lib.fem_stokes_solve_adjoint(dr.handle, arr, dp, ox, oy, *lams, C.byref(p), C.byref(st))
         out = {k: getattr(st, k) for k, _ in _AdjointStats._fields_}
         out["converged"] = bool(out["converged"])
         out["iterations"], out["residual"] = out["iter"], out["err"]
@@ -1607,15 +1612,19 @@ def solve_stokes_adjoint(dr: StokesDR, mesh: MixedMesh, bc_vx: DirichletBoundary
     return out
 
 
-def adjoint_blocks(dr: StokesDR, dt):
+def adjoint_blocks(dr: StokesDR, dt, plastic=None):
     """assemble_adjoint_operator (adjoint_operator.jl:412-475) at the state in
     dr: dict of host arrays A (nels, 14, 14), B (nels, 14, 3), C (nels, 3, 14),
     D (nels, 3, 3), W (nels, 14, 3); velocity DoFs stacked by component."""
+    if _is_cap(plastic):
+        raise ValueError("the frozen-operator adjoint covers models without a yield model or with the "
+                         "Drucker-Prager model")
+    d, dp = _dp_ptr(plastic)
     ne = dr.mesh.nels
     out = dict(A=np.zeros((ne, 14, 14)), B=np.zeros((ne, 14, 3)), C=np.zeros((ne, 3, 14)), D=np.zeros((ne, 3, 3)),
                W=np.zeros((ne, 14, 3)))
-    dr.ctx.check(_lib.fem_stokes_adjoint_blocks(dr.handle, float(dt), *(out[k].ctypes.data_as(_P(D))
-                                                                        for k in "ABCDW")))
+    dr.ctx.check(_lib.fem_stokes_adjoint_blocks(dr.handle, dp, float(dt), *(out[k].ctypes.data_as(_P(D))
+                                                                            for k in "ABCDW")))
     return out
 
 

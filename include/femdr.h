@@ -371,10 +371,11 @@ int fem_stokes_jacobian_diagnostics(fem_stokes* st, const fem_drucker_prager* pl
  * throw_on_failure=true, iterMax, max_ph_iterations, rel_drop, measure_λmax)
  * (src/stokes/solvers/DR_adjoint.jl:396-427 -> _solve_stokes_adjoint_dyrel!
  * :1-268) with operator = :blocks (adjoint_operator.jl:1-660), on a 2-D state
- * without a yield model: solves (∂R/∂u)ᵀλ = -∂J/∂u at the converged forward
- * state in the state (v, P, τ_old, dr.γP and dr.M_P of the forward solve, the
- * phases and material), with homogeneous conditions on the bc nodes (values
- * unused). CFL_v and c_fact are the state's. */
+ * without a yield model or with the Drucker-Prager model `plastic` (nullable;
+ * the tensile cap is not covered): solves (∂R/∂u)ᵀλ = -∂J/∂u at the converged
+ * forward state in the state (v, P, τ_old, dr.γP and dr.M_P of the forward
+ * solve, the phases and material), with homogeneous conditions on the bc nodes
+ * (values unused). CFL_v and c_fact are the state's. */
 typedef struct {
     double dt;
     double tolerance;           /* 1e-6                                         */
@@ -400,16 +401,16 @@ typedef struct {
  * device arrays holding the initial iterate, overwritten with the adjoint.
  * FEM_EARG for a 3-D state, an unassembled dr.γP (the forward solve assembles
  * it) or a tangent that breaks the symmetric layout; FEM_ENOCONV as solve!. */
-int fem_stokes_solve_adjoint(fem_stokes* st, const fem_bc* bc_v /* [2] */, const double* objective_x,
-                             const double* objective_y, double* lam_x, double* lam_y, double* lam_P,
-                             const fem_adjoint_params* params, fem_adjoint_stats* stats);
+int fem_stokes_solve_adjoint(fem_stokes* st, const fem_bc* bc_v /* [2] */, const fem_drucker_prager* plastic,
+                             const double* objective_x, const double* objective_y, double* lam_x, double* lam_y,
+                             double* lam_P, const fem_adjoint_params* params, fem_adjoint_stats* stats);
 /* assemble_adjoint_operator (adjoint_operator.jl:412-475): the element blocks at
  * the current state, to host arrays A [nels][14][14] (∂Rv/∂v with the
  * augmentation), B [nels][14][3] (∂Rv/∂P), C [nels][3][14] (∂RP/∂v), D [nels][3][3]
  * (∂RP/∂P, the ηb storage), W [nels][14][3] (∂Rv/∂Pnum · γP/M_P); velocity DoFs
  * stacked by component (vx 0..6, vy 7..13). Also fills ∂Rv∂v and PC_v. */
-int fem_stokes_adjoint_blocks(fem_stokes* st, double dt, double* host_A, double* host_B, double* host_C,
-                              double* host_D, double* host_W);
+int fem_stokes_adjoint_blocks(fem_stokes* st, const fem_drucker_prager* plastic, double dt, double* host_A,
+                              double* host_B, double* host_C, double* host_D, double* host_W);
 
 /* Sweeps the element kernel alone (no control flow) n times; used by the
  * bench to time the dominant kernel with device events. Returns the average

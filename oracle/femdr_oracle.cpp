@@ -149,9 +149,9 @@ struct Rheo {
     double eta_pt = 0;                // ... at the current point (set per point from eta_ip)
 };
 
-template <class S>
+template <class S, class SP = double>
 double stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, const int* ph,
-                 const Rheo& r, const double tau_o[3], double Pq, S tau[3]) {
+                 const Rheo& r, const double tau_o[3], SP Pq, S tau[3]) {
     S exx = gvx[0];
     S eyy = gvy[1];
     S exy = (gvx[1] + gvy[0]) / 2.0;
@@ -431,8 +431,9 @@ void cap_stress_ip(const S gvx[2], const S gvy[2], const double* Nv, int NV, con
 // Element momentum residual, src/stokes/assemblers/momentum_residuals.jl:136-189
 // (2-D with Pnum, T and gravity); accumulate_momentum :28-33; eos_density :11-16.
 // Generic in NV/NP/nq; dNdx [nq][NV][2], Nq [nq][NV], NqP [nq][NP]. SP is
-// double, or S to differentiate with respect to P (models without a yield
-// model only: the return map is evaluated at the value of P).
+// double, or S to differentiate with respect to P (through the total
+// pressure, the density and the Drucker-Prager return; the cap's return is
+// evaluated at the value of P).
 template <class S, class SP = double>
 void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* dOm,
                       const double* Nq, const double* NqP, const S* vx, const S* vy,
@@ -468,7 +469,7 @@ void momentum_element(int NV, int NP, int nq, const double* dNdx, const double* 
         Rheo rq = r;
         if (r.eta_ip) rq.eta_pt = r.eta_ip[q];
         if (r.cap) cap_stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, val(Peq), r.gamma ? r.gamma + q : nullptr, tau, Pe);
-        else stress_ip<S>(gvx, gvy, Nv, NV, ph, rq, to, val(Peq), tau);
+        else stress_ip<S, SP>(gvx, gvy, Nv, NV, ph, rq, to, Peq, tau);
         const S Pq_local = Pe + Pfq;
         if (tau_out) { tau_out[0 * nq + q] = val(tau[0]); tau_out[1 * nq + q] = val(tau[1]); tau_out[2 * nq + q] = val(tau[2]); }
         if (Pq_out) Pq_out[q] = val(Pq_local);
@@ -2928,7 +2929,9 @@ void adjoint_element(const orc_stokes* s, int64_t e, double dt, double A[ANU][AN
     const bool has_old = s->tau_old[0] || s->tau_old[1] || s->tau_old[2];
     for (int c = 0; c < 3; ++c)
         for (int q = 0; q < 7; ++q) to[c * 7 + q] = s->tau_old[c] ? s->tau_old[c][e * 7 + q] : 0.0;
-    Rheo r{&s->mat, s->mat.eta, nullptr, dt};
+    // B and W differentiate the residual with the state's per-phase viscosity
+    // and yield model (the momentum closure of adjoint_operator.jl:338-345)
+    Rheo r{&s->mat, s->mat.eta, s->dp, dt};
     r.eta_ip = s->eta_ip ? s->eta_ip + e * 7 : nullptr;
     const double* Pf = s->Pf ? s->Pf + e * 3 : nullptr;
     typedef Dual<3> D3;
@@ -3039,8 +3042,8 @@ extern "C" {
 int orc_adjoint_blocks(orc_stokes* s, double dt, double* A, double* B, double* C, double* D, double* W,
                        double* defects) {
     g_err.clear();
-    if (s->dp || s->cap) {
-        g_err = "the frozen-operator adjoint covers models without a yield model";
+    if (s->cap) {
+        g_err = "the frozen-operator adjoint covers models without a yield model or with the Drucker-Prager model";
         return -1;
     }
     std::vector<double> Jx((size_t)s->nels * 7), Jy((size_t)s->nels * 7), Dx((size_t)s->nels * 7),

@@ -348,7 +348,7 @@ int orc_thermal_solve(orc_thermal* t, const orc_thermal_params* p, orc_thermal_s
 
 /* --- frozen-operator adjoint of StokesDR (src/stokes/solvers/DR_adjoint.jl:1-531,
  * src/stokes/assemblers/adjoint_operator.jl:1-700), operator = :blocks, 2-D T7/P1-disc,
- * models without a yield model --- */
+ * models without a yield model or with the Drucker-Prager model --- */
 typedef struct {
     double dt, tolerance, rel_drop, CFL_v, c_fact;  /* tolerance 1e-6, rel_drop 0.1 */
     int64_t max_iterations;     /* total_iterMax, 50_000 */
@@ -369,7 +369,7 @@ typedef struct {
  * storage), W [nels][14][3] = ∂Rv/∂Pnum diag(γP/M_P); velocity DoFs stacked by
  * component (vx 0..6, vy 7..13). defects (nullable [2]) = max over elements of
  * ‖A−Aᵀ‖/‖A‖ and ‖C−Bᵀ‖/‖C‖. Also fills s->dRdvx/y and s->PCx/y from A.
- * Returns -1 (orc_last_error) for a state with a yield model. */
+ * Returns -1 (orc_last_error) for a state with the tensile cap. */
 int orc_adjoint_blocks(orc_stokes* s, double dt, double* A, double* B, double* C, double* D, double* W,
                        double* defects);
 /* apply_adjoint_operator! (adjoint_operator.jl:631-660): dv = Aᵀλv + CᵀλP (B λP and the

@@ -14,7 +14,11 @@ and their apply, src/stokes/assemblers/adjoint_operator.jl:1-660), pinned by:
   finite K included: the storage blocks D, W and the density term of B);
 * the velocity block's row sums and diagonal equal the augmented Jacobian
   diagnostics of the forward solver, and the packed symmetric layout applies
-  the same operator as the full blocks where K = Inf.
+  the same operator as the full blocks where K = Inf;
+* with the Drucker-Prager model (non-normal tangent, full blocks) the
+  dot-product test to 1e-6 on a yielding state, and the gradient of a
+  yielding visco-elasto-plastic step against central differences of its
+  forward solves to 1e-6.
 """
 import math
 
@@ -183,8 +187,90 @@ def test_velocity_block_matches_the_jacobian_diagnostics_and_layouts_agree():
         assert np.allclose(a, b, rtol=1e-13, atol=1e-13 * np.abs(b).max())
 
 
-def test_plastic_state_is_rejected():
+def test_tensile_cap_state_is_rejected():
     pr, _ = _state((INF, INF), (INF, INF))
-    pr.dp = O.drucker_prager(phi=(0.5, 0.5), psi=(0.0, 0.0), C_=(1.0, 1.0), eta_reg=(0.01, 0.01), Kb=(1.0, 1.0))
-    with pytest.raises(ValueError, match="yield model"):
+    pr.cap = O.drucker_prager_cap(phi=(0.5, 0.5), psi=(0.0, 0.0), C_=(1.0, 1.0), pT=(-0.1, -0.1),
+                                  eta_reg=(0.01, 0.01), Kb=(1.0, 1.0))
+    with pytest.raises(ValueError, match="Drucker-Prager model"):
         pr.adjoint_blocks(0.5)
+
+
+def test_operator_with_drucker_prager_is_the_transposed_jacobian():
+    # with the yield model the residual is piecewise smooth: central
+    # differences with a small step, away from the yield surface, to 1e-6
+    dt = 0.5
+    pr, rng = _state((INF, INF), (3.0, 8.0), (0.7, 2.0), seed=5)
+    dp = O.drucker_prager(phi=(0.5, 0.6), psi=(0.0, 0.1), C_=(0.05, 0.08), eta_reg=(0.01, 0.02), Kb=(5.0, 5.0))
+    blocks_ve = pr.adjoint_blocks(dt)
+    pr.dp = dp
+    blocks = pr.adjoint_blocks(dt)
+    assert np.abs(blocks["B"] - blocks_ve["B"]).max() > 1e-3 * np.abs(blocks["B"]).max()   # points yield
+    assert pr.adjoint_flags() == (False, True)
+    lx, ly, lP = rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodesP)
+    dvx, dvy, dP = pr.adjoint_apply(blocks, lx, ly, lP)
+    u0 = (pr.vx.copy(), pr.vy.copy(), pr.P.copy())
+    for trial in range(3):
+        du = (rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodes), rng.normal(size=pr.nnodesP))
+        h = 1e-6
+        Rp = _augmented_residual(pr, dt, *(a + h * b for a, b in zip(u0, du)))
+        Rm = _augmented_residual(pr, dt, *(a - h * b for a, b in zip(u0, du)))
+        lhs = sum(l @ ((p - m) / (2 * h)) for l, p, m in zip((lx, ly, lP), Rp, Rm))
+        rhs = du[0] @ dvx + du[1] @ dvy + du[2] @ dP
+        assert lhs == pytest.approx(rhs, rel=1e-6)
+    pr.vx[:], pr.vy[:], pr.P[:] = u0
+
+
+def vep_adjoint_case(rho0=None, C=0.05, n=10):
+    """A yielding visco-elasto-plastic step (the VEP generator at 10², 3
+    inclusions, cohesion 0.05) converged to 1e-12, and the objective
+    J = −Σ vy over y > 0.6."""
+    from femtools_amd import problems as PB
+    w = PB.vep_inclusions(n, ninc=3, seed=5, nsteps=1, C=C)
+    coords, el2n = O.t7_mesh(n, n)
+    ph = PB.inclusion_phases(coords, el2n, w.circles)
+    (bx, vx), (by, vy) = PB.pure_shear_bc(coords)
+    ne = len(el2n)
+    dp = O.drucker_prager(phi=w.plastic["phi"], psi=w.plastic["psi"], C_=w.plastic["C"],
+                          eta_reg=w.plastic["eta_reg"], Kb=w.plastic["Kb"])
+    mat = O.material(eta=w.eta, etab=w.etab, G=w.G, rho0=w.rho0 if rho0 is None else rho0, K=w.K, g=w.g)
+    pr = O.Problem(coords, el2n, mat, dp=dp, phases_v=ph, phases_P=ph,
+                   tau_old=tuple(np.zeros((ne, 7)) for _ in range(3)), bc=((bx, vx), (by, vy)))
+    pr.vx[:], pr.vy[:] = PB.pure_shear_field(coords)
+    st = pr.solve(w.dt, CFL_v=w.CFL_v, pressure_factor=w.pressure_factor, tolerance=1e-12,
+                  check_interval=w.check_interval, rel_drop0=w.rel_drop0, max_iterations=400_000, iter_max=400_000)
+    assert st["converged"]
+    obs = np.nonzero(coords[:, 1] > 0.6)[0]
+    adj = dict(check_interval=100, tolerance=1e-11, rel_drop=0.1, iter_max=400_000, max_iterations=400_000,
+               max_ph_iterations=400, CFL_v=w.CFL_v, c_fact=0.9)
+    return pr, w, obs, adj
+
+
+def test_plastic_adjoint_gradient_matches_finite_differences():
+    # the gradient of J with respect to the matrix density through a
+    # yielding step: the adjoint of the non-normal Drucker-Prager tangent
+    # (full A and C blocks) against central differences of forward solves
+    pr, w, obs, adj = vep_adjoint_case()
+    nn, npp = pr.nnodes, pr.nnodesP
+    b = pr.adjoint_blocks(w.dt)
+    assert b["defects"][0] > 1e-6 and b["defects"][1] > 1e-3   # A ≠ Aᵀ, C ≠ Bᵀ: the yield model is active
+    obj = (np.zeros(nn), np.zeros(nn))
+    obj[1][obs] = -1.0
+    lam = [np.zeros(nn), np.zeros(nn), np.zeros(npp)]
+    st = pr.solve_adjoint(w.dt, obj, lam, **adj)
+    assert st["converged"] and st["lmax_iterations"] > 0
+    base = tuple(pr.mat.rho0[i] for i in range(pr.mat.nphases))
+
+    def residual(rho):
+        for i, r in enumerate(rho):
+            pr.mat.rho0[i] = r
+        pr.momentum_residual(w.dt)
+        return pr.Rvx.copy(), pr.Rvy.copy()
+    r1 = residual((1.0,) + (0.0,) * (len(base) - 1))
+    r0 = residual((0.0,) * len(base))
+    residual(base)
+    grad = lam[0] @ (r1[0] - r0[0]) + lam[1] @ (r1[1] - r0[1])
+    d = 1e-4
+    jp = -vep_adjoint_case(rho0=(base[0] + d,) + base[1:])[0].vy[obs].sum()
+    jm = -vep_adjoint_case(rho0=(base[0] - d,) + base[1:])[0].vy[obs].sum()
+    assert grad == pytest.approx((jp - jm) / (2 * d), rel=1e-6)
+    assert abs(grad) > 0.1
