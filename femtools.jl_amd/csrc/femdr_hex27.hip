@@ -16,6 +16,9 @@
 // in ascending element order; the centre node is private and is updated by
 // its element.
 #include "femdr_host.hpp"
+#include <algorithm>
+#include <array>
+#include <map>
 
 namespace {
 
@@ -41,6 +44,11 @@ struct HexGeo {
     int32_t nx, ny, nz;
     int64_t sx, sxy;                 // refined-grid strides
     double jac[3], det;              // ∂ξ/∂x per axis, |det J|
+    // uploaded meshes (null on generated boxes): the connectivity [nels][27]
+    // in local node order and the per-element metric [nels][4] = (∂ξ/∂x,
+    // ∂η/∂y, ∂ζ/∂z, |det J|) of axis-aligned box elements of any size
+    const int32_t* el2n;
+    const double* metric;
 };
 
 HexGeo hex_geo(const fem_mesh* m) {
@@ -53,6 +61,15 @@ HexGeo hex_geo(const fem_mesh* m) {
                  hz = (m->box[5] - m->box[4]) / m->nz;
     g.jac[0] = 2 / hx; g.jac[1] = 2 / hy; g.jac[2] = 2 / hz;
     g.det = (hx / 2) * (hy / 2) * (hz / 2);
+    g.el2n = nullptr;
+    g.metric = nullptr;
+    if (!m->structured) {
+        g.nx = g.ny = g.nz = 1;
+        g.sx = g.sxy = 0;
+        g.jac[0] = g.jac[1] = g.jac[2] = g.det = 0;
+        g.el2n = m->d_el2n;
+        g.metric = m->d_geo;
+    }
     return g;
 }
 
@@ -91,8 +108,20 @@ struct Hex27Args {
 
 // global node of element e at tensor position t = i + 3 j + 9 k
 __device__ inline int64_t hex_node(const HexGeo& g, int64_t e, int t) {
+    if (g.el2n) return g.el2n[e * HNV + kHexA[t]];
     const int64_t ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / ((int64_t)g.nx * g.ny);
     return (2 * ez + t / 9) * g.sxy + (2 * ey + (t / 3) % 3) * g.sx + (2 * ex + t % 3);
+}
+
+// element e's geometry: the box's own on generated meshes, the element's
+// metric on uploaded ones
+__device__ inline HexGeo hex_at(const HexGeo& g, int64_t e) {
+    HexGeo r = g;
+    if (g.metric && e >= 0 && e < g.nels) {
+        const double* m = g.metric + e * 4;
+        r.jac[0] = m[0]; r.jac[1] = m[1]; r.jac[2] = m[2]; r.det = m[3];
+    }
+    return r;
 }
 
 // forward-mode dual with six partials (the strain-rate inputs of stress3d)
@@ -346,9 +375,9 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
     constexpr bool ELASTIC = FL & 1, PLASTIC = FL & 2, BULK = FL & 4, ITER = FL & 8;
     __shared__ double s_v[HEPB][HNV * 3];
     __shared__ double s_f[HEPB][HNQ * 8];
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
-    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const HexLane h = hex_lane(A.g.nels);
+    const int64_t e = h.valid ? h.e : A.g.nels - 1;
+    const HexGeo g = hex_at(A.g, e);
     // phase 0: nodal velocities (lane = node at tensor position h.l)
     int64_t node = 0;
     if (h.l < HNV) {
@@ -572,9 +601,9 @@ __global__ __launch_bounds__(HBLOCK) void k_hex27_lin(Lin3Args A) {
     __shared__ float s_v[HEPB][HNV * 3];
     __shared__ float s_f[HEPB][HNQ * 8];
     __shared__ double s_x[DELTA ? HEPB : 1][HNV * 3];
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
-    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const HexLane h = hex_lane(A.g.nels);
+    const int64_t e = h.valid ? h.e : A.g.nels - 1;
+    const HexGeo g = hex_at(A.g, e);
     const bool hot_e = DELTA && A.hot[e] != 0;   // uniform over the element's 32 lanes
     int64_t node = 0;
     if (h.l < HNV) {
@@ -763,9 +792,9 @@ template <bool ELASTIC, bool PLASTIC>
 __global__ __launch_bounds__(HBLOCK) void k_hex27_jacobian(Jac3Args A) {
     __shared__ double s_v[HEPB][HNV * 3];
     __shared__ double s_f[HEPB][HNQ * 8];
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
-    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const HexLane h = hex_lane(A.g.nels);
+    const int64_t e = h.valid ? h.e : A.g.nels - 1;
+    const HexGeo g = hex_at(A.g, e);
     int64_t node = 0;
     if (h.l < HNV) {
         node = hex_node(g, e, h.l);
@@ -887,10 +916,11 @@ __global__ __launch_bounds__(256) void k_gather_jac3(GatherJ3Args A) {
 
 // ---- assemble_viscosity_weighted_pressure_scaling! with the Hex27 Jacobi
 // weight N_a² dΩ (pressure_scaling.jl:166-216); one thread per element
-__global__ void k_hex27_scaling(HexGeo g, const int32_t* phase, const double* ptab, double gamma_fact, double dt,
+__global__ void k_hex27_scaling(HexGeo g0, const int32_t* phase, const double* ptab, double gamma_fact, double dt,
                                 double* MP, double* gammaP, double* pscale) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= g.nels) return;
+    if (e >= g0.nels) return;
+    const HexGeo g = hex_at(g0, e);
     const int pc = phase[e];
     const double eta = ptab[PC_ETA * MAXPH + pc], bulk = ptab[PC_BULK * MAXPH + pc];
     const double invK = 1.0 / bulk;
@@ -1196,9 +1226,10 @@ int prepare3(fem_stokes* s, const fem_drucker_prager* dp, double dt, int finite_
 // finished (and with `update` relaxed) by its element.
 // ---------------------------------------------------------------------------
 template <bool JAC>
-__global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g, ThermBoxArgs A) {
+__global__ __launch_bounds__(HBLOCK) void k_thermal_hex27(HexGeo g0, ThermBoxArgs A) {
     const int es = threadIdx.x >> 5, l = threadIdx.x & 31;
     const int64_t e = xcd_tile() * HEPB + es;
+    const HexGeo g = hex_at(g0, e);
     const bool ev = e < g.nels;
     __shared__ double sT[HEPB][HNV], sT0[HEPB][HNV], sS[HEPB][HNV], sP[HEPB][HNV];
     __shared__ double sK[HEPB][HNV], sCp[HEPB][HNV], sR0[HEPB][HNV], sAl[HEPB][HNV], sBe[HEPB][HNV];
@@ -1381,14 +1412,15 @@ __global__ __launch_bounds__(256) void k_hex27_rotate(RotArgs3 A) {
     if (i >= A.g.nels * HNQ) return;
     const int64_t e = i / HNQ;
     const int q = (int)(i - e * HNQ);
+    const HexGeo g = hex_at(A.g, e);
     const double x0 = gauss_x(q % 3), x1 = gauss_x((q / 3) % 3), x2 = gauss_x(q / 9);
     double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // G[c][k] = ∂v_c/∂x_k
     for (int t = 0; t < HNV; ++t) {
         const int r0 = t % 3, r1 = (t / 3) % 3, r2 = t / 9;
         const double L0 = lq(x0, r0), L1 = lq(x1, r1), L2 = lq(x2, r2);
-        const double d[3] = {dlq(x0, r0) * L1 * L2 * A.g.jac[0], L0 * dlq(x1, r1) * L2 * A.g.jac[1],
-                             L0 * L1 * dlq(x2, r2) * A.g.jac[2]};
-        const int64_t n = hex_node(A.g, e, t);
+        const double d[3] = {dlq(x0, r0) * L1 * L2 * g.jac[0], L0 * dlq(x1, r1) * L2 * g.jac[1],
+                             L0 * L1 * dlq(x2, r2) * g.jac[2]};
+        const int64_t n = hex_node(g, e, t);
         const double v[3] = {A.vx[n], A.vy[n], A.vz[n]};
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -1460,9 +1492,9 @@ struct Post3Args {
 };
 
 __global__ __launch_bounds__(128) void k_postprocess3(Post3Args A) {
-    const HexGeo& g = A.g;
     const int64_t e = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (e >= g.nels) return;
+    if (e >= A.g.nels) return;
+    const HexGeo g = hex_at(A.g, e);
     double sv[HNV * 3];
     for (int t = 0; t < HNV; ++t) {
         const int64_t n = hex_node(g, e, t);
@@ -1555,10 +1587,10 @@ __device__ inline void point_modes_cp(int q, double NP[HNP], double& dOm, double
 // points into the element slot (the centre node directly); lanes 0-3 the
 // mode mass Σ_q mode² dΩ
 __global__ __launch_bounds__(HBLOCK) void k_cp3_precond(CP3Args A) {
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
+    const HexLane h = hex_lane(A.g.nels);
     if (!h.active) return;
     const int64_t e = h.e;
+    const HexGeo g = hex_at(A.g, e);
     const double eta = A.ptab[PC_ETA * MAXPH + A.phase[e]];
     const int t = h.l;
     const Lane1D b = node_1d(t);
@@ -1600,9 +1632,9 @@ __global__ __launch_bounds__(HBLOCK) void k_cp3_precond(CP3Args A) {
 __global__ __launch_bounds__(HBLOCK) void k_cp3_pressure(CP3Args A) {
     __shared__ double s_v[HEPB][HNV * 3];
     __shared__ double s_part[HEPB];
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
-    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const HexLane h = hex_lane(A.g.nels);
+    const int64_t e = h.valid ? h.e : A.g.nels - 1;
+    const HexGeo g = hex_at(A.g, e);
     if (h.l < HNV) {
         const int64_t node = hex_node(g, e, h.l);
         s_v[h.es][h.l * 3 + 0] = A.vx[node];
@@ -1647,9 +1679,9 @@ __global__ __launch_bounds__(HBLOCK) void k_cp3_pressure(CP3Args A) {
 __global__ __launch_bounds__(HBLOCK) void k_cp3_momentum(CP3Args A) {
     __shared__ double s_v[HEPB][HNV * 3];
     __shared__ double s_f[HEPB][HNQ * 8];
-    const HexGeo& g = A.g;
-    const HexLane h = hex_lane(g.nels);
-    const int64_t e = h.valid ? h.e : g.nels - 1;
+    const HexLane h = hex_lane(A.g.nels);
+    const int64_t e = h.valid ? h.e : A.g.nels - 1;
+    const HexGeo g = hex_at(A.g, e);
     int64_t node = 0;
     if (h.l < HNV) {
         node = hex_node(g, e, h.l);
@@ -2636,6 +2668,39 @@ int hex27_time_lin(fem_stokes* s, const fem_drucker_prager* dp, double dt, int32
     return FEM_OK;
 }
 
+// the 26 shared local nodes of every element: node -> element-slot lists in
+// ascending element order (generate_node2element, connectivity.jl:341-349),
+// over the nodes that some element uses and that are no cell centre (cell
+// centres are private to their element and updated by the element kernel)
+static int hex27_lists(fem_ctx* c, fem_mesh* m, const std::vector<uint8_t>& centre) {
+    const int64_t nn = m->nnodes, ne = m->nels;
+    std::vector<uint8_t> used(nn, 0);
+    for (int64_t i = 0; i < ne * HNV; ++i) used[m->h_el2n[i]] = 1;
+    std::vector<int64_t> sid(nn, -1);
+    std::vector<int32_t> node_of;
+    for (int64_t k = 0; k < nn; ++k) {
+        if (centre[k] || !used[k]) continue;
+        sid[k] = (int64_t)node_of.size();
+        node_of.push_back((int32_t)k);
+    }
+    m->ns = (int64_t)node_of.size();
+    std::vector<int32_t> off(m->ns + 1, 0);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < HNSH; ++a) off[sid[m->h_el2n[e * HNV + a]] + 1]++;
+    for (int64_t i = 0; i < m->ns; ++i) off[i + 1] += off[i];
+    std::vector<int32_t> ent(off[m->ns]);
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < HNSH; ++a) ent[fill[sid[m->h_el2n[e * HNV + a]]]++] = (int32_t)(e * HNSH + a);
+    CHK(dalloc_t(c, &m->d_off, off.size()));
+    CHK(dalloc_t(c, &m->d_ent, ent.size()));
+    CHK(dalloc_t(c, &m->d_node_of, node_of.size()));
+    HIPCHK(c, hipMemcpy(m->d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_node_of, node_of.data(), node_of.size() * 4, hipMemcpyHostToDevice));
+    return FEM_OK;
+}
+
 extern "C" int fem_mesh_hex27_create(fem_ctx* c, int32_t nx, int32_t ny, int32_t nz, double x0, double x1, double y0,
                                      double y1, double z0, double z1, fem_mesh** out) {
     if (!c || !out) return FEM_EARG;
@@ -2679,38 +2744,118 @@ extern "C" int fem_mesh_hex27_create(fem_ctx* c, int32_t nx, int32_t ny, int32_t
             m->h_el2n[e * HNV + a] =
                 (int32_t)((2 * ez + kHexR[a][2]) * sx * sy + (2 * ey + kHexR[a][1]) * sx + (2 * ex + kHexR[a][0]));
     }
-    // the 26 shared local nodes: node -> element-slot lists in ascending
-    // element order (generate_node2element, connectivity.jl:341-349); cell
-    // centres (odd, odd, odd grid indices) are private to their element
-    std::vector<int64_t> sid(nn, -1);
-    std::vector<int32_t> node_of;
-    for (int64_t k = 0; k < nn; ++k) {
-        const int64_t ix = k % sx, iy = (k / sx) % sy, iz = k / (sx * sy);
-        if ((ix & 1) && (iy & 1) && (iz & 1)) continue;
-        sid[k] = (int64_t)node_of.size();
-        node_of.push_back((int32_t)k);
-    }
-    m->ns = (int64_t)node_of.size();
-    std::vector<int32_t> off(m->ns + 1, 0);
-    for (int64_t e = 0; e < ne; ++e)
-        for (int a = 0; a < HNSH; ++a) off[sid[m->h_el2n[e * HNV + a]] + 1]++;
-    for (int64_t i = 0; i < m->ns; ++i) off[i + 1] += off[i];
-    std::vector<int32_t> ent(off[m->ns]);
-    std::vector<int32_t> fill(off.begin(), off.end() - 1);
-    for (int64_t e = 0; e < ne; ++e)
-        for (int a = 0; a < HNSH; ++a) ent[fill[sid[m->h_el2n[e * HNV + a]]]++] = (int32_t)(e * HNSH + a);
-    int r = FEM_OK;
-    if ((r = dalloc_t(c, &m->d_off, off.size())) || (r = dalloc_t(c, &m->d_ent, ent.size())) ||
-        (r = dalloc_t(c, &m->d_node_of, node_of.size()))) {
+    std::vector<uint8_t> centre(nn, 0);
+    for (int64_t e = 0; e < ne; ++e) centre[m->h_el2n[e * HNV + HNSH]] = 1;
+    const int r = hex27_lists(c, m, centre);
+    if (r != FEM_OK) {
         fem_mesh_destroy(m);
         return r;
     }
-    hipError_t he = hipMemcpy(m->d_off, off.data(), off.size() * 4, hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(m->d_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMemcpy(m->d_node_of, node_of.data(), node_of.size() * 4, hipMemcpyHostToDevice);
-    if (he != hipSuccess) {
+    *out = m;
+    return FEM_OK;
+}
+
+// Mesh(backend, coords, el2n, Hex27) (mesh.jl:147-188) from host arrays:
+// coords [nnodes][3], el2n [nels][27] 1-based in the local order of
+// generate_element2node (connectivity.jl:273-322). The element kernels take
+// one metric per element, so every element must be an axis-aligned box whose
+// local ξ, η, ζ run along +x, +y, +z with its 27 nodes at the box's corners,
+// edge, face and centre midpoints (to 1e-10 of its size); nodes and elements
+// may come in any order and elements may differ in size (graded meshes).
+// Γnodes are the nodes of the faces that belong to one element only.
+extern "C" int fem_mesh_hex27_upload(fem_ctx* c, const double* host_coords, int64_t nnodes,
+                                     const int32_t* host_el2n_1based, int64_t nels, fem_mesh** out) {
+    if (!c || !out || !host_coords || !host_el2n_1based) return FEM_EARG;
+    *out = nullptr;
+    if (nnodes <= 0 || nels <= 0) return set_err(c, FEM_EARG, "empty mesh");
+    if (nnodes > std::numeric_limits<int32_t>::max() || nels * HNSH > std::numeric_limits<int32_t>::max())
+        return set_err(c, FEM_EDIM, "mesh too large for 32-bit node and element-slot indices");
+    std::vector<int32_t> el((size_t)nels * HNV);
+    std::vector<int32_t> val(nnodes, 0);
+    for (int64_t i = 0; i < nels * HNV; ++i) {
+        const int32_t n = host_el2n_1based[i] - 1;
+        if (n < 0 || n >= nnodes)
+            return set_err(c, FEM_EARG, "el2n entry %d out of range at element %lld", n + 1, (long long)(i / HNV) + 1);
+        el[i] = n;
+        val[n]++;
+    }
+    std::vector<double> metric((size_t)nels * 4);
+    std::vector<uint8_t> centre(nnodes, 0);
+    for (int64_t e = 0; e < nels; ++e) {
+        const int32_t* en = &el[e * HNV];
+        const double* lo = host_coords + 3 * (int64_t)en[kHexA[0]];
+        const double* hi = host_coords + 3 * (int64_t)en[kHexA[HNV - 1]];
+        double h[3];
+        for (int d = 0; d < 3; ++d) h[d] = hi[d] - lo[d];
+        if (!(h[0] > 0 && h[1] > 0 && h[2] > 0) || !std::isfinite(h[0] * h[1] * h[2]))
+            return set_err(c, FEM_EARG, "Hex27 element %lld is inverted or degenerate: its local axes must run "
+                           "along +x, +y, +z", (long long)e + 1);
+        const double tol = 1e-10 * std::max(h[0], std::max(h[1], h[2]));
+        for (int a = 0; a < HNV; ++a)
+            for (int d = 0; d < 3; ++d) {
+                const double want = lo[d] + 0.5 * kHexR[a][d] * h[d];
+                if (!(std::fabs(host_coords[3 * (int64_t)en[a] + d] - want) <= tol))
+                    return set_err(c, FEM_EARG, "Hex27 element %lld is not an axis-aligned box with its nodes at "
+                                   "the corner, edge, face and centre points (local node %d); uploaded Hex27 "
+                                   "meshes take box elements of any size", (long long)e + 1, a + 1);
+            }
+        const int32_t cn = en[HNSH];
+        if (val[cn] != 1)
+            return set_err(c, FEM_EARG, "Hex27 centre node %d of element %lld is shared", cn + 1, (long long)e + 1);
+        centre[cn] = 1;
+        metric[e * 4 + 0] = 2 / h[0];
+        metric[e * 4 + 1] = 2 / h[1];
+        metric[e * 4 + 2] = 2 / h[2];
+        metric[e * 4 + 3] = (h[0] / 2) * (h[1] / 2) * (h[2] / 2);
+    }
+    // faces that one element owns: keyed by their sorted corner nodes
+    std::map<std::array<int32_t, 4>, int> faces;
+    auto face_key = [&](int64_t e, int d, int side) {
+        std::array<int32_t, 4> k;
+        int n = 0;
+        for (int t = 0; t < HNV; ++t) {
+            const int r[3] = {t % 3, (t / 3) % 3, t / 9};
+            if (r[d] == side && r[(d + 1) % 3] != 1 && r[(d + 2) % 3] != 1) k[n++] = el[e * HNV + kHexA[t]];
+        }
+        std::sort(k.begin(), k.end());
+        return k;
+    };
+    for (int64_t e = 0; e < nels; ++e)
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side <= 2; side += 2) faces[face_key(e, d, side)]++;
+    std::vector<uint8_t> bnd(nnodes, 0);
+    for (int64_t e = 0; e < nels; ++e)
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side <= 2; side += 2)
+                if (faces[face_key(e, d, side)] == 1)
+                    for (int t = 0; t < HNV; ++t) {
+                        const int r[3] = {t % 3, (t / 3) % 3, t / 9};
+                        if (r[d] == side) bnd[el[e * HNV + kHexA[t]]] = 1;
+                    }
+    HIPCHK(c, hipSetDevice(c->device));
+    fem_mesh* m = new fem_mesh();
+    m->ctx = c;
+    m->dim = 3;
+    m->nloc = HNV;
+    m->structured = false;
+    m->nnodes = nnodes;
+    m->nels = nels;
+    m->nnodesP = 4 * nels;
+    m->h_coords.assign(host_coords, host_coords + 3 * nnodes);
+    m->h_el2n = std::move(el);
+    for (int64_t k = 0; k < nnodes; ++k)
+        if (bnd[k]) m->h_boundary.push_back((int32_t)k);
+    int r = hex27_lists(c, m, centre);
+    if (r == FEM_OK) r = dalloc_t(c, &m->d_el2n, m->h_el2n.size());
+    if (r == FEM_OK) r = dalloc_t(c, &m->d_geo, metric.size());
+    if (r == FEM_OK) {
+        hipError_t he = hipMemcpy(m->d_el2n, m->h_el2n.data(), m->h_el2n.size() * 4, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(m->d_geo, metric.data(), metric.size() * 8, hipMemcpyHostToDevice);
+        if (he != hipSuccess) r = set_err(c, FEM_EHIP, "hipMemcpy of the Hex27 mesh failed: %s", hipGetErrorString(he));
+    }
+    if (r != FEM_OK) {
         fem_mesh_destroy(m);
-        return set_err(c, FEM_EHIP, "hipMemcpy of the Hex27 node lists failed: %s", hipGetErrorString(he));
+        return r;
     }
     *out = m;
     return FEM_OK;

@@ -24,7 +24,7 @@ import numpy as np
 
 __all__ = [
     "FemError", "DimensionMismatch", "Context", "DeviceArray", "VectorField2D", "SymmetricTensor2D",
-    "StokesMaterial", "DruckerPrager", "MixedMesh", "t7_box_mesh", "t7_mesh_from_arrays",
+    "StokesMaterial", "DruckerPrager", "MixedMesh", "t7_box_mesh", "t7_mesh_from_arrays", "hex27_mesh_from_arrays",
     "DirichletBoundaryCondition", "StokesDR", "solve", "update_stokes_current_stress",
     "assemble_viscosity_weighted_pressure_scaling", "lib_path", "default_context", "solve_batch",
     "hex27_box_mesh", "q9_box_mesh", "VectorField3D", "SymmetricTensor3D", "ThermalMaterial", "ThermalDiffusionDR",
@@ -194,6 +194,7 @@ _sig("fem_fill_i32", C.c_int, VP, VP, I32, I64)
 _sig("fem_mesh_t7_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
 _sig("fem_mesh_t7_upload", C.c_int, VP, _P(D), I64, _P(I32), I64, _P(VP))
 _sig("fem_mesh_hex27_create", C.c_int, VP, I32, I32, I32, D, D, D, D, D, D, _P(VP))
+_sig("fem_mesh_hex27_upload", C.c_int, VP, _P(D), I64, _P(I32), I64, _P(VP))
 _sig("fem_mesh_q9_create", C.c_int, VP, I32, I32, D, D, D, D, _P(VP))
 _sig("fem_mesh_get_info", C.c_int, VP, _P(_MeshInfo))
 _sig("fem_mesh_get_coords", C.c_int, VP, _P(D))
@@ -266,7 +267,7 @@ _sig("fem_coupled_solve", C.c_int, VP, VP, _P(_BC), _P(_BC), _P(_DP), _P(_SolveP
 EXPORTED = [
     "fem_ctx_create", "fem_ctx_destroy", "fem_last_error", "fem_sync", "fem_buf_alloc", "fem_buf_free",
     "fem_copy_h2d", "fem_copy_d2h", "fem_copy_d2d", "fem_fill_f64", "fem_fill_f32", "fem_fill_i32",
-    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_q9_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
+    "fem_mesh_t7_create", "fem_mesh_t7_upload", "fem_mesh_hex27_create", "fem_mesh_hex27_upload", "fem_mesh_q9_create", "fem_mesh_get_info", "fem_mesh_get_coords", "fem_mesh_get_el2n",
     "fem_mesh_get_boundary_nodes", "fem_mesh_destroy", "fem_stokes_create", "fem_stokes_destroy",
     "fem_stokes_field", "fem_stokes_set_phases", "fem_stokes_solve", "fem_stokes_solve_batch",
     "fem_stokes_pressure_scaling",
@@ -732,6 +733,23 @@ def t7_mesh_from_arrays(coords, el2n, ctx: Context | None = None) -> MixedMesh:
     h = VP()
     ctx.check(_lib.fem_mesh_t7_upload(ctx.handle, coords.ctypes.data_as(_P(D)), len(coords),
                                       e1.ctypes.data_as(_P(I32)), len(e1), C.byref(h)))
+    return MixedMesh(ctx, h)
+
+
+def hex27_mesh_from_arrays(coords, el2n, ctx: Context | None = None) -> MixedMesh:
+    """MixedMesh(Mesh(backend, coords, el2n, Hex27), P1) (mesh.jl:147-188) from
+    host arrays (0-based el2n in the local order of connectivity.jl:273-322).
+    Elements must be axis-aligned boxes with their nodes at the corner, edge,
+    face and centre points; their sizes and the node and element order are free.
+    Raises ValueError naming the first element that is not."""
+    ctx = ctx or default_context()
+    coords = np.ascontiguousarray(coords, np.float64)
+    e1 = np.ascontiguousarray(np.asarray(el2n, np.int32) + 1)
+    if coords.ndim != 2 or coords.shape[1] != 3 or e1.ndim != 2 or e1.shape[1] != 27:
+        raise ValueError("hex27_mesh_from_arrays takes coords [nnodes, 3] and el2n [nels, 27]")
+    h = VP()
+    ctx.check(_lib.fem_mesh_hex27_upload(ctx.handle, coords.ctypes.data_as(_P(D)), len(coords),
+                                         e1.ctypes.data_as(_P(I32)), len(e1), C.byref(h)))
     return MixedMesh(ctx, h)
 
 

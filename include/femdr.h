@@ -109,6 +109,16 @@ int fem_mesh_t7_create(fem_ctx* ctx, int32_t nx, int32_t ny, double x0, double x
  * discontinuous pressure with the four modes (1-ξ-η-ζ, ξ, η, ζ) per cell (DoF 4e+a). */
 int fem_mesh_hex27_create(fem_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, double x0, double x1,
                           double y0, double y1, double z0, double z1, fem_mesh** out);
+/* Mesh(backend, coords, el2n, ReferenceElement(QuadraticElement{3,27})) then
+ * MixedMesh(mesh, ReferenceElement(LinearElement{3,4})) (mesh.jl:147-188,
+ * mixed_mesh.jl:358-417) from host arrays: coords [nnodes][3], el2n [nels][27]
+ * 1-based in the local node order of generate_element2node (connectivity.jl:273-322).
+ * Every element must be an axis-aligned box (local ξ, η, ζ along +x, +y, +z) with
+ * its nodes at the corner, edge, face and centre points; sizes may differ per
+ * element and nodes and elements may come in any order. Γnodes are the nodes of
+ * faces owned by one element. FEM_EARG names the first element that breaks this. */
+int fem_mesh_hex27_upload(fem_ctx* ctx, const double* coords, int64_t nnodes,
+                          const int32_t* el2n_1based, int64_t nels, fem_mesh** out);
 /* Mesh(backend, (x0..x1)x(y0..y1), ReferenceElement(QuadraticElement{2,9}), (nx,ny))
  * (mesh.jl:340-398; connectivity.jl:193-227): a structured Q9 mesh on the
  * (2nx+1) x (2ny+1) grid for ThermalDiffusionDR states (heat_diffusion_types.jl:90-153
