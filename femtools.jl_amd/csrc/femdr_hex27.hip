@@ -114,8 +114,11 @@ __device__ inline int64_t hex_node(const HexGeo& g, int64_t e, int t) {
 }
 
 // element e's geometry: the box's own on generated meshes, the element's
-// metric on uploaded ones
+// metric on uploaded ones. UP = false (kernels instantiated for generated
+// meshes only) keeps the box's metric uniform, out of per-lane registers
+template <bool UP = true>
 __device__ inline HexGeo hex_at(const HexGeo& g, int64_t e) {
+    if constexpr (!UP) return g;
     HexGeo r = g;
     if (g.metric && e >= 0 && e < g.nels) {
         const double* m = g.metric + e * 4;
@@ -372,12 +375,12 @@ __device__ inline HexLane hex_lane(int64_t nels) {
 // centre node
 template <int FL>
 __global__ __launch_bounds__(HBLOCK) void k_hex27_element(Hex27Args A) {
-    constexpr bool ELASTIC = FL & 1, PLASTIC = FL & 2, BULK = FL & 4, ITER = FL & 8;
+    constexpr bool ELASTIC = FL & 1, PLASTIC = FL & 2, BULK = FL & 4, ITER = FL & 8, UP = FL & 64;
     __shared__ double s_v[HEPB][HNV * 3];
     __shared__ double s_f[HEPB][HNQ * 8];
     const HexLane h = hex_lane(A.g.nels);
     const int64_t e = h.valid ? h.e : A.g.nels - 1;
-    const HexGeo g = hex_at(A.g, e);
+    const HexGeo g = hex_at<UP>(A.g, e);
     // phase 0: nodal velocities (lane = node at tensor position h.l)
     int64_t node = 0;
     if (h.l < HNV) {
@@ -597,13 +600,13 @@ struct Lin3Args {
 
 template <int FL>
 __global__ __launch_bounds__(HBLOCK) void k_hex27_lin(Lin3Args A) {
-    constexpr bool BULK = FL & 1, DELTA = FL & 2, ELASTIC = FL & 4;
+    constexpr bool BULK = FL & 1, DELTA = FL & 2, ELASTIC = FL & 4, UP = FL & 64;
     __shared__ float s_v[HEPB][HNV * 3];
     __shared__ float s_f[HEPB][HNQ * 8];
     __shared__ double s_x[DELTA ? HEPB : 1][HNV * 3];
     const HexLane h = hex_lane(A.g.nels);
     const int64_t e = h.valid ? h.e : A.g.nels - 1;
-    const HexGeo g = hex_at(A.g, e);
+    const HexGeo g = hex_at<UP>(A.g, e);
     const bool hot_e = DELTA && A.hot[e] != 0;   // uniform over the element's 32 lanes
     int64_t node = 0;
     if (h.l < HNV) {
@@ -1044,8 +1047,12 @@ int check3(fem_ctx* c, const char* what) { return check_launch(c, what); }
 
 template <int FL>
 void launch_h27(fem_stokes* s, const Hex27Args& a) {
-    hipLaunchKernelGGL((k_hex27_element<FL>), dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0, s->ctx->stream,
-                       a);
+    if (s->mesh->structured)
+        hipLaunchKernelGGL((k_hex27_element<FL>), dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0,
+                           s->ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_hex27_element<FL | 64>), dim3(grid_xcd(s->mesh->nels, HEPB)), dim3(HBLOCK), 0,
+                           s->ctx->stream, a);
 }
 template <int FL>
 void dispatch_h27(int key, fem_stokes* s, const Hex27Args& a) {
@@ -2182,14 +2189,20 @@ int lin3_step(fem_stokes* s, const Opts3& o, bool delta, const double al[3], con
     for (int k = 0; k < 6; ++k) a.tau_old[k] = f[tf[k]];
     a.hot = s->d_yflag;
     const dim3 grid(grid_xcd(m->nels, HEPB)), block(HBLOCK);
-    const int key = (o.bulk ? 1 : 0) | (delta ? 2 : 0) | (delta && o.elastic ? 4 : 0);
+    const int key = (o.bulk ? 1 : 0) | (delta ? 2 : 0) | (delta && o.elastic ? 4 : 0) | (m->structured ? 0 : 64);
     switch (key) {
         case 0: hipLaunchKernelGGL((k_hex27_lin<0>), grid, block, 0, c->stream, a); break;
         case 1: hipLaunchKernelGGL((k_hex27_lin<1>), grid, block, 0, c->stream, a); break;
         case 2: hipLaunchKernelGGL((k_hex27_lin<2>), grid, block, 0, c->stream, a); break;
         case 3: hipLaunchKernelGGL((k_hex27_lin<3>), grid, block, 0, c->stream, a); break;
         case 6: hipLaunchKernelGGL((k_hex27_lin<6>), grid, block, 0, c->stream, a); break;
-        default: hipLaunchKernelGGL((k_hex27_lin<7>), grid, block, 0, c->stream, a); break;
+        case 7: hipLaunchKernelGGL((k_hex27_lin<7>), grid, block, 0, c->stream, a); break;
+        case 64: hipLaunchKernelGGL((k_hex27_lin<64>), grid, block, 0, c->stream, a); break;
+        case 65: hipLaunchKernelGGL((k_hex27_lin<65>), grid, block, 0, c->stream, a); break;
+        case 66: hipLaunchKernelGGL((k_hex27_lin<66>), grid, block, 0, c->stream, a); break;
+        case 67: hipLaunchKernelGGL((k_hex27_lin<67>), grid, block, 0, c->stream, a); break;
+        case 70: hipLaunchKernelGGL((k_hex27_lin<70>), grid, block, 0, c->stream, a); break;
+        default: hipLaunchKernelGGL((k_hex27_lin<71>), grid, block, 0, c->stream, a); break;
     }
     CHK(check3(c, "k_hex27_lin"));
     Gather3fArgs g;
